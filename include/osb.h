@@ -1,0 +1,165 @@
+/*
+ * osb.h -- C ABI of the OpenSBLI explicit-solver hot path.
+ *
+ * One header, two implementations:
+ *   libosb_hip.so     the product: hand-written HIP kernels for gfx950
+ *   libosb_oracle.so  the CPU restatement used only by tests and the
+ *                     CPU-baseline leg of the benchmark
+ *
+ * The reference has no C ABI: its drop-in seam is the Python back-end call
+ * OPSC(alg, ...) (opensbli/code_generation/opsc.py:408) that prints an OPS
+ * C program, plus the OPS runtime calls that program makes
+ * (ops_par_loop / ops_halo_transfer / ops_reduction_result,
+ *  opensbli/core/kernel.py:303-388, opensbli/code_generation/opsc.py:979-1064).
+ * Each entry point below names the part of that seam it replaces.
+ *
+ * Conventions
+ *   - every function returns int: 0 = OK, < 0 = error code (OSB_E_*);
+ *     osb_last_error(ctx) holds a message.  Nothing aborts across the ABI.
+ *   - state is structure-of-arrays, conservative variables in the order
+ *     rho, rhou0 .. rhou{ndim-1}, rhoE; x fastest.
+ *   - a context is externally synchronised; distinct contexts may be driven
+ *     from distinct host threads.
+ */
+#ifndef OSB_H
+#define OSB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OSB_ABI_VERSION 1
+
+/* error codes */
+#define OSB_OK            0
+#define OSB_E_INVALID    -1   /* bad descriptor / argument                  */
+#define OSB_E_UNSUPPORTED -2  /* combination not implemented               */
+#define OSB_E_DEVICE     -3   /* HIP runtime error                          */
+#define OSB_E_NOMEM      -4   /* allocation failed / over budget            */
+#define OSB_E_NONFINITE  -5   /* NaN/Inf detected (ops_NaNcheck analogue)   */
+
+/* convective schemes (opensbli/schemes/spatial/{weno,teno}.py, optimisations/StoreSome.py) */
+#define OSB_CONV_WENO_Z            1  /* LFWeno(order=5, formulation='Z')          weno.py:475   */
+#define OSB_CONV_TENO              2  /* LFTeno(order=5|6)                          teno.py:480   */
+#define OSB_CONV_CENTRAL_FEIEREISEN 3 /* NS_Split('Feiereisen') + StoreSome(4)      split_forms.py:111 */
+#define OSB_CONV_CENTRAL_SKEW      4  /* Skew(...) convective terms + Central(4)    parsing.py:96 */
+
+/* characteristic averaging (opensbli/schemes/spatial/averaging.py) */
+#define OSB_AVG_SIMPLE 1              /* SimpleAverage  averaging.py:31 */
+#define OSB_AVG_ROE    2              /* RoeAverage     averaging.py:65 */
+
+/* viscous terms (opensbli/physical_models/split_forms.py:383-392) */
+#define OSB_VISC_NONE       0
+#define OSB_VISC_CONSTANT   1         /* tau = (1.0/Re)(...)                         */
+#define OSB_VISC_SUTHERLAND 2         /* tau = (mu/Re)(...), mu = 1.4042 T^1.5/(T+0.40417) */
+
+/* time integration */
+#define OSB_TIME_RKLS3 1              /* RungeKuttaLS(3)  rk_LS.py:17   */
+#define OSB_TIME_RK3   2              /* RungeKutta(3)    rk_sbli.py:16 */
+
+/* boundary conditions (opensbli/core/boundary_conditions/) */
+#define OSB_BC_PERIODIC      1        /* PeriodicBC            periodic.py:7              */
+#define OSB_BC_ZERO_GRADIENT 2        /* ZeroGradientOutletBC  zero_gradient_outlet.py:5  */
+#define OSB_BC_INVISCID_WALL 3        /* InviscidWallBC        inviscid_wall.py:6         */
+#define OSB_BC_SYMMETRY      4        /* SymmetryBC            symmetry.py:6              */
+
+/* arithmetic of the flux / derivative evaluation */
+#define OSB_PREC_FP64  1
+#define OSB_PREC_MIXED 2              /* FP32 flux/derivatives over an FP64 state, FP64 smoothness
+                                         indicators, TENO cut-off and RK accumulation            */
+
+/* state layouts for osb_set_state / osb_get_state */
+#define OSB_LAYOUT_INTERIOR 0         /* nv x n[2] x n[1] x n[0]                          */
+#define OSB_LAYOUT_PADDED   1         /* nv x (n+2h)... with the context's halo depth h  */
+
+typedef struct osb_problem_desc {
+    uint32_t struct_size;      /* sizeof(osb_problem_desc)                         */
+    uint32_t abi_version;      /* OSB_ABI_VERSION                                  */
+    int32_t  ndim;             /* 1, 2 or 3                                        */
+    int32_t  n[3];             /* interior points per direction (1 if unused)      */
+    double   delta[3];         /* grid spacing Delta{d}block0                      */
+    int32_t  conv_scheme;      /* OSB_CONV_*                                       */
+    int32_t  order;            /* 5 (WENO-Z, TENO5), 6 (TENO6), 4 (central)        */
+    int32_t  averaging;        /* OSB_AVG_* (shock capturing only)                 */
+    int32_t  viscous;          /* OSB_VISC_*                                       */
+    int32_t  time_scheme;      /* OSB_TIME_*                                       */
+    int32_t  precision;        /* OSB_PREC_*                                       */
+    int32_t  bc[3][2];         /* OSB_BC_* per direction and side                  */
+    double   gama, minf, re, pr;
+    double   weno_eps;         /* WENO-Z epsilon, 1e-40 in weno.py:293             */
+    double   teno_eps;         /* TENO 'eps' constant, teno.py:369                 */
+    double   teno_ct;          /* TENO_CT, teno.py:373                             */
+    double   shock_filter_control; /* LLF wave-speed scale, shock_capturing.py:1014 */
+    double   dt;               /* constant time step (used when cfl <= 0)          */
+    double   cfl;              /* > 0: dt = cfl / max(sum_d (|u_d|+a)/Delta_d)     */
+    int32_t  reserved[8];
+} osb_problem_desc;
+
+typedef struct osb_stats {
+    uint32_t struct_size;
+    int64_t  iter;             /* completed iterations                             */
+    double   time;             /* simulation_time                                  */
+    double   last_dt;          /* dt used by the last step                         */
+    int32_t  halo;             /* stored halo depth                                */
+    int32_t  nv;               /* conservative variables                           */
+    int64_t  device_bytes;     /* device memory held by the context                */
+} osb_stats;
+
+typedef struct osb_ctx osb_ctx;
+typedef struct osb_batch osb_batch;
+
+/* Lifecycle.  Replaces OPSC(alg) + ops_init/ops_decl_block/ops_decl_dat
+ * (opsc.py:408, opsc.py:838-960): one context owns one block's arrays.      */
+int  osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out);
+int  osb_ctx_destroy(osb_ctx* ctx);
+const char* osb_last_error(const osb_ctx* ctx);   /* ctx may be NULL: last create error */
+int  osb_abi_version(void);
+
+/* State transfer.  Replaces GridBasedInitialisation / ops_decl_dat_hdf5
+ * (initialisation/gridbasedinit.py:16, opsc.py:962-977).                   */
+int  osb_set_state(osb_ctx* ctx, const double* host, int layout);
+int  osb_get_state(osb_ctx* ctx, double* host, int layout);
+int  osb_set_time(osb_ctx* ctx, double time, int64_t iter);
+
+/* Stepping.  Replaces the generated time loop
+ * (algorithm/algorithm.py:387-502; TENO6/opensbli.cpp:206-432).             */
+int  osb_step(osb_ctx* ctx, int nsteps);
+int  osb_apply_bcs(osb_ctx* ctx);                        /* ops_halo_transfer + BC kernels */
+int  osb_residual(osb_ctx* ctx, double* host_out);       /* one RHS evaluation, interior layout */
+int  osb_max_wavespeed(osb_ctx* ctx, double* out);       /* max_points sum_d (|u_d|+a)/Delta_d */
+int  osb_sync(osb_ctx* ctx);
+int  osb_get_stats(osb_ctx* ctx, osb_stats* out);
+
+/* Taylor-Green post-processing reductions KE, dilatation_dissipation,
+ * enstrophy_dissipation, rhom (TENO6/opensbliblock00_kernels.h:4737-4797,
+ * compressible_TGV.py:176-206).  Velocity and mu come from the constituent
+ * relations of the last stage, rho from the current state, as in the
+ * reference's post-process kernel placement (TENO6/opensbli.cpp:433-455).   */
+int  osb_tgv_diagnostics(osb_ctx* ctx, double out[4]);
+
+/* NaN check of rho (ops_NaNcheck, helperfunctions.py:201-214):
+ * returns OSB_E_NONFINITE if any interior rho is not finite.                */
+int  osb_nan_check(osb_ctx* ctx);
+
+/* Device buffers (HIP library only): pointer to the FP64 conservative
+ * variable `var` (padded layout) for zero-copy interop.                     */
+int  osb_state_device_ptr(osb_ctx* ctx, int var, void** out);
+int  osb_set_stream(osb_ctx* ctx, void* hip_stream);
+
+/* Batched ensembles: many instances of possibly different sizes advanced by
+ * one launch per RK stage (persistent kernels, hipGraph replay).  No
+ * counterpart in the reference, which runs one block per executable.        */
+int  osb_batch_create(const osb_problem_desc* descs, int n, osb_batch** out);
+int  osb_batch_destroy(osb_batch* b);
+int  osb_batch_set_state(osb_batch* b, int i, const double* host, int layout);
+int  osb_batch_get_state(osb_batch* b, int i, double* host, int layout);
+int  osb_batch_step(osb_batch* b, int nsteps);
+int  osb_batch_sync(osb_batch* b);
+const char* osb_batch_last_error(const osb_batch* b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OSB_H */

@@ -1,0 +1,371 @@
+// osb_common.hpp -- shared definitions for the gfx950 kernels of the
+// OpenSBLI explicit-solver hot path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace osb {
+
+constexpr double INV_SQRT2 = 0.707106781186547;   // sqrt(2.0) printed to 15 s.f. (euler_eigensystem.py:109,139)
+
+enum Scheme { SCH_WENOZ = 0, SCH_TENO5 = 1, SCH_TENO6 = 2 };
+
+// Geometry of one padded block: SoA, x fastest, `halo` ghost layers in active
+// directions.  Interior point (i,j,k) lives at base + i + s1*j + s2*k.
+struct Geom {
+    int n[3];          // interior extent
+    int P[3];          // padded extent
+    int o[3];          // offset of interior index 0 (halo or 0)
+    int halo;
+    int ndim;
+    long s[3];         // strides (1, P0, P0*P1)
+    long Ptot;         // points per variable
+};
+
+__host__ __device__ inline long gidx(const Geom& g, int i, int j, int k) {
+    return (long)(i + g.o[0]) + g.s[1] * (long)(j + g.o[1]) + g.s[2] * (long)(k + g.o[2]);
+}
+
+struct Consts {
+    double gama, gamma_m1, minf;
+    double sfc;            // shock_filter_control
+    double weno_eps, teno_eps, teno_ct;
+    double invRe, cheat;   // 1/Re, 1/((gama-1) Minf^2 Pr Re)
+    double invd[3];        // 1/Delta
+    double inv2d[3];       // 1/Delta^2
+    double cfl, dt;
+    int avg_roe;
+    int viscous;           // 0 none, 1 constant, 2 Sutherland
+};
+
+// ---------------------------------------------------------------------------
+// Smoothness indicators and reconstructions.  Values arrive as T (double in
+// the FP64 path, float in the mixed path); smoothness indicators, non-linear
+// weights and the TENO cut-off are always evaluated in double.
+// f[s] holds the split flux at offset s-2 from point i (interface i+1/2).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double sq(double x) { return x * x; }
+
+// Jiang-Shu indicators of 3-point stencils (a, b, c in ascending x)
+__device__ __forceinline__ double js_first(double a, double b, double c) {
+    return (13.0 / 12.0) * sq(a - 2.0 * b + c) + 0.25 * sq(3.0 * a - 4.0 * b + c);
+}
+__device__ __forceinline__ double js_centre(double a, double b, double c) {
+    return (13.0 / 12.0) * sq(a - 2.0 * b + c) + 0.25 * sq(a - c);
+}
+__device__ __forceinline__ double js_last(double a, double b, double c) {
+    return (13.0 / 12.0) * sq(a - 2.0 * b + c) + 0.25 * sq(a - 4.0 * b + 3.0 * c);
+}
+
+// WENO-Z, order 5 (weno.py:290-345): alpha_r = d_r + d_r tau^2/((eps+b_r)^2)
+template <typename T>
+__device__ __forceinline__ T wenoz_right(const T* f, double eps) {
+    const double f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4];
+    const double b0 = js_first(f2, f3, f4);     // stencil (i, i+1, i+2), d = 3/10
+    const double b1 = js_centre(f1, f2, f3);    // stencil (i-1, i, i+1), d = 6/10
+    const double b2 = js_last(f0, f1, f2);      // stencil (i-2, i-1, i), d = 1/10
+    const double tau2 = sq(fabs(b0 - b2));
+    const double a0 = 0.3 + 0.3 * tau2 / sq(eps + b0);
+    const double a1 = 0.6 + 0.6 * tau2 / sq(eps + b1);
+    const double a2 = 0.1 + 0.1 * tau2 / sq(eps + b2);
+    const double inv = 1.0 / (a0 + a1 + a2);
+    const T e0 = T(1.0 / 3.0) * f[2] + T(5.0 / 6.0) * f[3] - T(1.0 / 6.0) * f[4];
+    const T e1 = -T(1.0 / 6.0) * f[1] + T(5.0 / 6.0) * f[2] + T(1.0 / 3.0) * f[3];
+    const T e2 = T(1.0 / 3.0) * f[0] - T(7.0 / 6.0) * f[1] + T(11.0 / 6.0) * f[2];
+    return T(a0 * inv) * e0 + T(a1 * inv) * e1 + T(a2 * inv) * e2;
+}
+template <typename T>
+__device__ __forceinline__ T wenoz_left(const T* f, double eps) {
+    const double f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4], f5 = f[5];
+    const double b0 = js_first(f3, f4, f5);     // (i+1, i+2, i+3), d = 1/10
+    const double b1 = js_centre(f2, f3, f4);    // (i, i+1, i+2),   d = 6/10
+    const double b2 = js_last(f1, f2, f3);      // (i-1, i, i+1),   d = 3/10
+    const double tau2 = sq(fabs(b0 - b2));
+    const double a0 = 0.1 + 0.1 * tau2 / sq(eps + b0);
+    const double a1 = 0.6 + 0.6 * tau2 / sq(eps + b1);
+    const double a2 = 0.3 + 0.3 * tau2 / sq(eps + b2);
+    const double inv = 1.0 / (a0 + a1 + a2);
+    const T e0 = T(11.0 / 6.0) * f[3] - T(7.0 / 6.0) * f[4] + T(1.0 / 3.0) * f[5];
+    const T e1 = T(1.0 / 3.0) * f[2] + T(5.0 / 6.0) * f[3] - T(1.0 / 6.0) * f[4];
+    const T e2 = -T(1.0 / 6.0) * f[1] + T(5.0 / 6.0) * f[2] + T(1.0 / 3.0) * f[3];
+    return T(a0 * inv) * e0 + T(a1 * inv) * e1 + T(a2 * inv) * e2;
+}
+
+__device__ __forceinline__ double pow6(double x) { const double x2 = x * x; return x2 * x2 * x2; }
+
+// TENO5 / TENO6 (teno.py:59-217, 419-477).  SIDE +1: f+ on offsets -2..3
+// (right); SIDE -1: f- (left).  ORDER 5 or 6.
+template <int ORDER, int SIDE, typename T>
+__device__ __forceinline__ T teno(const T* f, double eps, double ct) {
+    constexpr int NS = ORDER == 5 ? 3 : 4;
+    // F(o) = f[o+2]
+    const double m2 = f[0], m1 = f[1], z0 = f[2], p1 = f[3], p2 = f[4], p3 = f[5];
+    double beta[4];
+    T eno[4];
+    if (SIDE > 0) {
+        beta[0] = 0.25 * sq(m1 - p1) + (13.0 / 12.0) * sq(m1 - 2.0 * z0 + p1);
+        beta[1] = 0.25 * sq(3.0 * z0 - 4.0 * p1 + p2) + (13.0 / 12.0) * sq(z0 - 2.0 * p1 + p2);
+        beta[2] = 0.25 * sq(m2 - 4.0 * m1 + 3.0 * z0) + (13.0 / 12.0) * sq(m2 - 2.0 * m1 + z0);
+        eno[0] = -T(1.0 / 6.0) * f[1] + T(5.0 / 6.0) * f[2] + T(2.0 / 6.0) * f[3];
+        eno[1] = T(2.0 / 6.0) * f[2] + T(5.0 / 6.0) * f[3] - T(1.0 / 6.0) * f[4];
+        eno[2] = T(2.0 / 6.0) * f[0] - T(7.0 / 6.0) * f[1] + T(11.0 / 6.0) * f[2];
+        if (NS == 4) {
+            const double s1 = -11.0 * z0 + 18.0 * p1 - 9.0 * p2 + 2.0 * p3;
+            const double s2 = 2.0 * z0 - 5.0 * p1 + 4.0 * p2 - p3;
+            const double s3 = -z0 + 3.0 * p1 - 3.0 * p2 + p3;
+            // the right-side beta_3 keeps the 781/720 term unsquared (teno.py:168-169)
+            beta[3] = (1.0 / 36.0) * sq(s1) + (13.0 / 12.0) * sq(s2) + (781.0 / 720.0) * s3;
+            eno[3] = T(3.0 / 12.0) * f[2] + T(13.0 / 12.0) * f[3] - T(5.0 / 12.0) * f[4] + T(1.0 / 12.0) * f[5];
+        }
+    } else {
+        beta[0] = 0.25 * sq(z0 - p2) + (13.0 / 12.0) * sq(z0 - 2.0 * p1 + p2);
+        beta[1] = 0.25 * sq(m1 - 4.0 * z0 + 3.0 * p1) + (13.0 / 12.0) * sq(m1 - 2.0 * z0 + p1);
+        beta[2] = 0.25 * sq(3.0 * p1 - 4.0 * p2 + p3) + (13.0 / 12.0) * sq(p1 - 2.0 * p2 + p3);
+        eno[0] = T(2.0 / 6.0) * f[2] + T(5.0 / 6.0) * f[3] - T(1.0 / 6.0) * f[4];
+        eno[1] = -T(1.0 / 6.0) * f[1] + T(5.0 / 6.0) * f[2] + T(2.0 / 6.0) * f[3];
+        eno[2] = T(11.0 / 6.0) * f[3] - T(7.0 / 6.0) * f[4] + T(1.0 / 3.0) * f[5];
+        if (NS == 4) {
+            const double s1 = -2.0 * m2 + 9.0 * m1 - 18.0 * z0 + 11.0 * p1;
+            const double s2 = -m2 + 4.0 * m1 - 5.0 * z0 + 2.0 * p1;
+            const double s3 = -m2 + 3.0 * m1 - 3.0 * z0 + p1;
+            beta[3] = (1.0 / 36.0) * sq(s1) + (13.0 / 12.0) * sq(s2) + (781.0 / 720.0) * sq(s3);
+            eno[3] = T(1.0 / 12.0) * f[0] - T(5.0 / 12.0) * f[1] + T(13.0 / 12.0) * f[2] + T(3.0 / 12.0) * f[3];
+        }
+    }
+    double dopt[4];
+    if (ORDER == 5) { dopt[0] = 11.0 / 20.0; dopt[1] = 4.0 / 10.0; dopt[2] = 1.0 / 20.0; dopt[3] = 0.0; }
+    else { dopt[0] = 231.0 / 500.0; dopt[1] = 3.0 / 10.0; dopt[2] = 27.0 / 500.0; dopt[3] = 23.0 / 125.0; }
+    const double tau = NS == 3 ? fabs(beta[0] - beta[2])
+                               : fabs(beta[3] - (1.0 / 6.0) * (beta[0] + beta[2] - 4.0 * beta[1]));
+    double alpha[4], asum = 0.0;
+#pragma unroll
+    for (int r = 0; r < NS; ++r) {
+        alpha[r] = pow6(1.0 + tau * (1.0 / (eps + beta[r])));
+        asum += alpha[r];
+    }
+    const double inv_asum = 1.0 / asum;
+    double delta[4], wsum = 0.0;
+#pragma unroll
+    for (int r = 0; r < NS; ++r) {
+        delta[r] = (alpha[r] * inv_asum < ct) ? 0.0 : 1.0;
+        wsum += dopt[r] * delta[r];
+    }
+    const double inv_wsum = 1.0 / wsum;
+    T rec = T(0);
+#pragma unroll
+    for (int r = 0; r < NS; ++r) rec += T(dopt[r] * delta[r] * inv_wsum) * eno[r];
+    return rec;
+}
+
+template <int SCH, typename T>
+__device__ __forceinline__ T recon_pair(const T* fp, const T* fm, const Consts& c) {
+    if (SCH == SCH_WENOZ) return wenoz_right<T>(fp, c.weno_eps) + wenoz_left<T>(fm, c.weno_eps);
+    else if (SCH == SCH_TENO5) return teno<5, 1, T>(fp, c.teno_eps, c.teno_ct) + teno<5, -1, T>(fm, c.teno_eps, c.teno_ct);
+    else return teno<6, 1, T>(fp, c.teno_eps, c.teno_ct) + teno<6, -1, T>(fm, c.teno_eps, c.teno_ct);
+}
+
+// ---------------------------------------------------------------------------
+// Euler eigensystems (euler_eigensystem.py:53-165) for direction DIR, written
+// out with the unit normal e_DIR folded in at compile time.
+// L[k][m]: left eigenvectors (rows), R[m][k]: right eigenvectors (columns).
+// ---------------------------------------------------------------------------
+template <int NDIM, int DIR, typename T>
+__device__ __forceinline__ void eigensystem(T gm1, T rho, const T* u, T a, T (&L)[NDIM + 2][NDIM + 2],
+                                            T (&R)[NDIM + 2][NDIM + 2]) {
+    constexpr int NV = NDIM + 2;
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) { L[i][j] = T(0); R[i][j] = T(0); }
+    if (NDIM == 1) {
+        const T u0 = u[0];
+        const T H = a * a / gm1 + u0 * u0 * T(0.5);
+        const T D = T(2) * H - u0 * u0;
+        const T invD = T(1) / D, inva = T(1) / a;
+        R[0][0] = 1; R[0][1] = 1; R[0][2] = 1;
+        R[1][0] = u0 - a; R[1][1] = u0; R[1][2] = u0 + a;
+        R[2][0] = H - u0 * a; R[2][1] = u0 * u0 * T(0.5); R[2][2] = H + u0 * a;
+        L[0][0] = u0 * (T(2) * H + a * u0 - u0 * u0) * T(0.5) * inva * invD;
+        L[0][1] = (-H - a * u0 + u0 * u0 * T(0.5)) * inva * invD;
+        L[0][2] = invD;
+        L[1][0] = T(2) * (H - u0 * u0) * invD;
+        L[1][1] = T(2) * u0 * invD;
+        L[1][2] = -T(2) * invD;
+        L[2][0] = u0 * (-T(2) * H + a * u0 + u0 * u0) * T(0.5) * inva * invD;
+        L[2][1] = (H - a * u0 - u0 * u0 * T(0.5)) * inva * invD;
+        L[2][2] = invD;
+        return;
+    }
+    constexpr T k0 = DIR == 0 ? 1 : 0, k1 = DIR == 1 ? 1 : 0, k2 = DIR == 2 ? 1 : 0;
+    T q2 = T(0);
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) q2 += u[d] * u[d];
+    const T phi_sq = gm1 * (q2 * T(0.5));
+    const T theta = u[DIR];
+    const T inva = T(1) / a, invrho = T(1) / rho;
+    const T inva2 = inva * inva;
+    const T alpha = rho * T(INV_SQRT2) * inva;
+    const T bta = T(INV_SQRT2) * invrho * inva;
+    const T igm1 = T(1) / gm1;
+    const T a2 = a * a;
+    if (NDIM == 2) {
+        const T u0 = u[0], u1 = u[1];
+        R[0][0] = 1; R[0][2] = alpha; R[0][3] = alpha;
+        R[1][0] = u0; R[1][1] = k1 * rho; R[1][2] = alpha * (u0 + k0 * a); R[1][3] = alpha * (u0 - k0 * a);
+        R[2][0] = u1; R[2][1] = -k0 * rho; R[2][2] = alpha * (u1 + k1 * a); R[2][3] = alpha * (u1 - k1 * a);
+        R[3][0] = phi_sq * igm1; R[3][1] = rho * (k1 * u0 - k0 * u1);
+        R[3][2] = alpha * ((phi_sq + a2) * igm1 + a * theta);
+        R[3][3] = alpha * ((phi_sq + a2) * igm1 - a * theta);
+        L[0][0] = T(1) - phi_sq * inva2; L[0][1] = gm1 * u0 * inva2; L[0][2] = gm1 * u1 * inva2; L[0][3] = -gm1 * inva2;
+        L[1][0] = -(k1 * u0 - k0 * u1) * invrho; L[1][1] = k1 * invrho; L[1][2] = -k0 * invrho;
+        L[2][0] = bta * (phi_sq - a * theta); L[2][1] = bta * (k0 * a - gm1 * u0);
+        L[2][2] = bta * (k1 * a - gm1 * u1); L[2][3] = bta * gm1;
+        L[3][0] = bta * (phi_sq + a * theta); L[3][1] = -bta * (k0 * a + gm1 * u0);
+        L[3][2] = -bta * (k1 * a + gm1 * u1); L[3][3] = bta * gm1;
+        return;
+    }
+    const T u0 = u[0], u1 = u[1], u2 = u[2];
+    R[0][0] = k0; R[0][1] = k1; R[0][2] = k2; R[0][3] = alpha; R[0][4] = alpha;
+    R[1][0] = k0 * u0; R[1][1] = k1 * u0 - k2 * rho; R[1][2] = k2 * u0 + k1 * rho;
+    R[1][3] = alpha * (u0 + k0 * a); R[1][4] = alpha * (u0 - k0 * a);
+    R[2][0] = k0 * u1 + k2 * rho; R[2][1] = k1 * u1; R[2][2] = k2 * u1 - k0 * rho;
+    R[2][3] = alpha * (u1 + k1 * a); R[2][4] = alpha * (u1 - k1 * a);
+    R[3][0] = k0 * u2 - k1 * rho; R[3][1] = k1 * u2 + k0 * rho; R[3][2] = k2 * u2;
+    R[3][3] = alpha * (u2 + k2 * a); R[3][4] = alpha * (u2 - k2 * a);
+    R[4][0] = k0 * phi_sq * igm1 + rho * (k2 * u1 - k1 * u2);
+    R[4][1] = k1 * phi_sq * igm1 + rho * (k0 * u2 - k2 * u0);
+    R[4][2] = k2 * phi_sq * igm1 + rho * (k1 * u0 - k0 * u1);
+    R[4][3] = alpha * ((phi_sq + a2) * igm1 + theta * a);
+    R[4][4] = alpha * ((phi_sq + a2) * igm1 - theta * a);
+    const T c1 = T(1) - phi_sq * inva2;
+    L[0][0] = k0 * c1 - (k2 * u1 - k1 * u2) * invrho;
+    L[0][1] = k0 * gm1 * u0 * inva2;
+    L[0][2] = k0 * gm1 * u1 * inva2 + k2 * invrho;
+    L[0][3] = k0 * gm1 * u2 * inva2 - k1 * invrho;
+    L[0][4] = -k0 * gm1 * inva2;
+    L[1][0] = k1 * c1 - (k0 * u2 - k2 * u0) * invrho;
+    L[1][1] = k1 * gm1 * u0 * inva2 - k2 * invrho;
+    L[1][2] = k1 * gm1 * u1 * inva2;
+    L[1][3] = k1 * gm1 * u2 * inva2 + k0 * invrho;
+    L[1][4] = -k1 * gm1 * inva2;
+    L[2][0] = k2 * c1 - (k1 * u0 - k0 * u1) * invrho;
+    L[2][1] = k2 * gm1 * u0 * inva2 + k1 * invrho;
+    L[2][2] = k2 * gm1 * u1 * inva2 - k0 * invrho;
+    L[2][3] = k2 * gm1 * u2 * inva2;
+    L[2][4] = -k2 * gm1 * inva2;
+    L[3][0] = bta * (phi_sq - theta * a);
+    L[3][1] = -bta * (gm1 * u0 - k0 * a);
+    L[3][2] = -bta * (gm1 * u1 - k1 * a);
+    L[3][3] = -bta * (gm1 * u2 - k2 * a);
+    L[3][4] = bta * gm1;
+    L[4][0] = bta * (phi_sq + theta * a);
+    L[4][1] = -bta * (gm1 * u0 + k0 * a);
+    L[4][2] = -bta * (gm1 * u1 + k1 * a);
+    L[4][3] = -bta * (gm1 * u2 + k2 * a);
+    L[4][4] = bta * gm1;
+}
+
+// ---------------------------------------------------------------------------
+// Characteristic LLF flux at interface x0 + 1/2 along DIR
+// (shock_capturing.py:786-891, 957-1028).  q: FP64 state, `st` the stride
+// along DIR.  Result in `out` (FP64).
+// ---------------------------------------------------------------------------
+template <int NDIM, int DIR, int SCH, typename T>
+__device__ __forceinline__ void face_flux(const double* __restrict__ q, long Ptot, long st, long x0,
+                                          const Consts& c, double (&out)[NDIM + 2]) {
+    constexpr int NV = NDIM + 2;
+    T qs[6][NV];
+    T F[6][NV];
+    T ud[6];
+    double rho23[2], u23[2][NDIM], p23[2], E23[2], a23[2];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        const long xs = x0 + (long)(s - 2) * st;
+        double v[NV];
+#pragma unroll
+        for (int m = 0; m < NV; ++m) v[m] = q[(long)m * Ptot + xs];
+        const double r = v[0];
+        double uu[NDIM];
+        double ke = 0.0;
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) { uu[d] = v[1 + d] / r; ke -= 0.5 * (uu[d] * uu[d]) * r; }
+        const double p = c.gamma_m1 * (ke + v[NV - 1]);
+        const double u_dir = uu[DIR];
+#pragma unroll
+        for (int m = 0; m < NV; ++m) qs[s][m] = T(v[m]);
+        ud[s] = T(u_dir);
+        F[s][0] = T(v[1 + DIR]);
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) F[s][1 + d] = T(v[1 + d] * u_dir + (d == DIR ? p : 0.0));
+        F[s][NV - 1] = T((p + v[NV - 1]) * u_dir);
+        if (s == 2 || s == 3) {
+            const int w = s - 2;
+            rho23[w] = r; p23[w] = p; E23[w] = v[NV - 1];
+#pragma unroll
+            for (int d = 0; d < NDIM; ++d) u23[w][d] = uu[d];
+            a23[w] = sqrt(c.gama * p / r);
+        }
+    }
+    // averaged state (averaging.py:31-124)
+    T rho_av, u_av[NDIM], a_av;
+    if (c.avg_roe) {
+        const double sl = sqrt(rho23[0]), sr = sqrt(rho23[1]);
+        const double inv = 1.0 / (sl + sr);
+        double ke = 0.0;
+        double ua[NDIM];
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) { ua[d] = (sl * u23[0][d] + sr * u23[1][d]) * inv; ke += ua[d] * ua[d]; }
+        const double Hr = ((p23[0] + E23[0]) / sl + (p23[1] + E23[1]) / sr) * inv;
+        rho_av = T(sqrt(rho23[0] * rho23[1]));
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) u_av[d] = T(ua[d]);
+        a_av = T(sqrt(c.gamma_m1 * (Hr - 0.5 * ke)));
+    } else {
+        rho_av = T(0.5 * (rho23[0] + rho23[1]));
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) u_av[d] = T(0.5 * (u23[0][d] + u23[1][d]));
+        a_av = T(0.5 * (a23[0] + a23[1]));
+    }
+    T L[NV][NV], R[NV][NV];
+    eigensystem<NDIM, DIR, T>(T(c.gamma_m1), rho_av, u_av, a_av, L, R);
+    // local Lax-Friedrichs wave speeds over offsets {0,1} (shock_capturing.py:1000-1028)
+    T lam[NV];
+    {
+        const double ua = u23[0][DIR], ub = u23[1][DIR];
+        if (NDIM == 1) {
+            lam[0] = T(c.sfc * fmax(fabs(ua - a23[0]), fabs(ub - a23[1])));
+            lam[1] = T(c.sfc * fmax(fabs(ua), fabs(ub)));
+            lam[2] = T(c.sfc * fmax(fabs(ua + a23[0]), fabs(ub + a23[1])));
+        } else {
+            const T l0 = T(c.sfc * fmax(fabs(ua), fabs(ub)));
+#pragma unroll
+            for (int d = 0; d < NDIM; ++d) lam[d] = l0;
+            lam[NDIM] = T(c.sfc * fmax(fabs(ua + a23[0]), fabs(ub + a23[1])));
+            lam[NDIM + 1] = T(c.sfc * fmax(fabs(ua - a23[0]), fabs(ub - a23[1])));
+        }
+    }
+    T acc[NV];
+#pragma unroll
+    for (int m = 0; m < NV; ++m) acc[m] = T(0);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        T fp[6], fm[6];
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            T cs = T(0), cf = T(0);
+#pragma unroll
+            for (int m = 0; m < NV; ++m) { cs += L[k][m] * qs[s][m]; cf += L[k][m] * F[s][m]; }
+            fp[s] = T(0.5) * (cf + lam[k] * cs);
+            fm[s] = T(0.5) * (cf - lam[k] * cs);
+        }
+        const T rec = recon_pair<SCH, T>(fp, fm, c);
+#pragma unroll
+        for (int m = 0; m < NV; ++m) acc[m] += R[m][k] * rec;
+    }
+#pragma unroll
+    for (int m = 0; m < NV; ++m) out[m] = double(acc[m]);
+}
+
+}  // namespace osb

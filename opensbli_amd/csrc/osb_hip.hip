@@ -1,0 +1,584 @@
+// osb_hip.hip -- host runtime and C ABI of the gfx950 implementation of the
+// OpenSBLI explicit-solver hot path (include/osb.h).
+//
+// One osb_ctx owns one block's resident FP64 state on one device:
+//   q_a, q_b   conservative variables, ping-pong between RK stages
+//   du         RK-LS register (or q_old for the classic RK3)
+//   R          residual
+//   F          interface fluxes, nv per direction (shock capturing)
+//   cr, gr     constituent relations and stored velocity gradients
+// all SoA with `halo` ghost layers, x fastest.  Every launch goes to the
+// context's stream; nothing synchronises except the calls that return data.
+
+#include "osb_kernels.hpp"
+#include "../../include/osb.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace osb;
+
+namespace {
+
+std::string g_create_error;
+
+struct DevBuf {
+    double* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct osb_ctx {
+    osb_problem_desc d{};
+    Geom g{};
+    Consts c{};
+    int nv = 0;
+    bool shock = false, central = false, viscous = false;
+    int sch = SCH_WENOZ;
+    bool mixed = false;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int device = 0;
+
+    double* qa = nullptr;
+    double* qb = nullptr;
+    double* q_cur = nullptr;        // current state
+    double* q_prev = nullptr;       // input of the last stage of the last step
+    double* du = nullptr;
+    double* R = nullptr;
+    double* F = nullptr;
+    double* cr = nullptr;
+    double* gr = nullptr;
+    double* scal = nullptr;         // [0] max wavespeed bits, [1] dt, [2] device time, [3..6] reduction out
+    double* staging = nullptr;      // interior-layout transfer buffer
+    double* part = nullptr;         // reduction partials
+    unsigned int* nan_cnt = nullptr;
+    int part_blocks = 0;
+    size_t bytes = 0;
+
+    double time = 0.0;
+    int64_t iter = 0;
+    double last_dt = 0.0;
+    bool have_prev = false;
+    std::string err;
+};
+
+namespace {
+
+int set_err(osb_ctx* c, int code, const std::string& m) {
+    if (c) c->err = m; else g_create_error = m;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                    \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            return set_err((ctx), OSB_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+        }                                                                                     \
+    } while (0)
+
+inline unsigned nblocks(long work) { return (unsigned)((work + BLOCK - 1) / BLOCK); }
+
+long interior_points(const Geom& g) { return (long)g.n[0] * g.n[1] * g.n[2]; }
+
+long box_points(const Geom& g, int ext) {
+    long t = 1;
+    for (int d = 0; d < 3; ++d) t *= (d < g.ndim) ? (long)(g.n[d] + 2 * ext) : 1L;
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+// kernel dispatch
+// ---------------------------------------------------------------------------
+template <int NDIM, int DIR, typename T>
+void launch_flux_t(osb_ctx* c, const double* q) {
+    long work = interior_points(c->g) / c->g.n[DIR] * (long)(c->g.n[DIR] + 1);
+    double* F = c->F + (long)DIR * c->nv * c->g.Ptot;
+    switch (c->sch) {
+        case SCH_WENOZ: k_face_flux<NDIM, DIR, SCH_WENOZ, T><<<nblocks(work), BLOCK, 0, c->stream>>>(q, F, c->g, c->c); break;
+        case SCH_TENO5: k_face_flux<NDIM, DIR, SCH_TENO5, T><<<nblocks(work), BLOCK, 0, c->stream>>>(q, F, c->g, c->c); break;
+        default: k_face_flux<NDIM, DIR, SCH_TENO6, T><<<nblocks(work), BLOCK, 0, c->stream>>>(q, F, c->g, c->c); break;
+    }
+}
+
+template <int NDIM, int DIR>
+void launch_flux(osb_ctx* c, const double* q) {
+    if (c->mixed) launch_flux_t<NDIM, DIR, float>(c, q);
+    else launch_flux_t<NDIM, DIR, double>(c, q);
+}
+
+template <int NDIM>
+void rhs_nd(osb_ctx* c, const double* q) {
+    const long nin = interior_points(c->g);
+    if (c->shock) {
+        launch_flux<NDIM, 0>(c, q);
+        if (NDIM > 1) launch_flux<NDIM, (NDIM > 1 ? 1 : 0)>(c, q);
+        if (NDIM > 2) launch_flux<NDIM, (NDIM > 2 ? 2 : 0)>(c, q);
+        k_flux_residual<NDIM><<<nblocks(nin), BLOCK, 0, c->stream>>>(c->F, c->R, c->g, c->c);
+    }
+    if (c->central || c->viscous) {
+        k_constituent<NDIM><<<nblocks(box_points(c->g, 2)), BLOCK, 0, c->stream>>>(q, c->cr, c->g, c->c, 2);
+        k_gradients<NDIM><<<nblocks(box_points(c->g, 2)), BLOCK, 0, c->stream>>>(c->cr, c->gr, c->g, c->c);
+    }
+    if (c->central) {
+        if (c->d.conv_scheme == OSB_CONV_CENTRAL_SKEW)
+            k_central_convective<NDIM, true><<<nblocks(nin), BLOCK, 0, c->stream>>>(q, c->cr, c->gr, c->R, c->g, c->c);
+        else
+            k_central_convective<NDIM, false><<<nblocks(nin), BLOCK, 0, c->stream>>>(q, c->cr, c->gr, c->R, c->g, c->c);
+    }
+    if (c->viscous) {
+        if (c->d.viscous == OSB_VISC_SUTHERLAND)
+            k_viscous<NDIM, true><<<nblocks(nin), BLOCK, 0, c->stream>>>(c->cr, c->gr, c->R, c->g, c->c);
+        else
+            k_viscous<NDIM, false><<<nblocks(nin), BLOCK, 0, c->stream>>>(c->cr, c->gr, c->R, c->g, c->c);
+    }
+}
+
+void rhs(osb_ctx* c, const double* q) {
+    switch (c->g.ndim) {
+        case 1: rhs_nd<1>(c, q); break;
+        case 2: rhs_nd<2>(c, q); break;
+        default: rhs_nd<3>(c, q); break;
+    }
+}
+
+void apply_bcs(osb_ctx* c, double* q) {
+    const Geom& g = c->g;
+    for (int dir = 0; dir < g.ndim; ++dir) {
+        long work = 2L * (g.halo + 1);
+        for (int d = 0; d < g.ndim; ++d)
+            if (d != dir) work *= (g.n[d] + 2 * g.halo);
+        const int b0 = c->d.bc[dir][0], b1 = c->d.bc[dir][1];
+        if (dir == 0) k_bc<0><<<nblocks(work), BLOCK, 0, c->stream>>>(q, g, c->nv, b0, b1);
+        else if (dir == 1) k_bc<1><<<nblocks(work), BLOCK, 0, c->stream>>>(q, g, c->nv, b0, b1);
+        else k_bc<2><<<nblocks(work), BLOCK, 0, c->stream>>>(q, g, c->nv, b0, b1);
+    }
+}
+
+void launch_wavespeed(osb_ctx* c, const double* q) {
+    k_zero_scalar<<<1, 64, 0, c->stream>>>(c->scal);
+    const long nin = interior_points(c->g);
+    const unsigned nb = std::min<unsigned>(nblocks(nin), 2048u);
+    switch (c->g.ndim) {
+        case 1: k_wavespeed<1><<<nb, BLOCK, 0, c->stream>>>(q, c->scal, c->g, c->c); break;
+        case 2: k_wavespeed<2><<<nb, BLOCK, 0, c->stream>>>(q, c->scal, c->g, c->c); break;
+        default: k_wavespeed<3><<<nb, BLOCK, 0, c->stream>>>(q, c->scal, c->g, c->c); break;
+    }
+}
+
+int check_launch(osb_ctx* c) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_err(c, OSB_E_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+    return OSB_OK;
+}
+
+int do_step(osb_ctx* c) {
+    const Geom& g = c->g;
+    const long nin = interior_points(g);
+    if (c->d.cfl > 0) {
+        launch_wavespeed(c, c->q_cur);
+        k_dt_finalize<<<1, 64, 0, c->stream>>>(c->scal, c->d.cfl);
+    }
+    if (c->d.time_scheme == OSB_TIME_RKLS3) {
+        const double rkA[3] = {0, (-5.0 / 9.0), (-153.0 / 128.0)};
+        const double rkB[3] = {(1.0 / 3.0), (15.0 / 16.0), (8.0 / 15.0)};
+        for (int s = 0; s < 3; ++s) {
+            rhs(c, c->q_cur);
+            double* qn = (c->q_cur == c->qa) ? c->qb : c->qa;
+            k_rkls<<<nblocks(nin), BLOCK, 0, c->stream>>>(c->q_cur, qn, c->du, c->R, c->scal, g, c->nv, rkA[s], rkB[s]);
+            apply_bcs(c, qn);
+            c->q_prev = c->q_cur;
+            c->q_cur = qn;
+        }
+    } else {
+        const double rkold[3] = {1.0 / 4.0, 3.0 / 20.0, 3.0 / 5.0};
+        const double rknew[3] = {2.0 / 3.0, 5.0 / 12.0, 3.0 / 5.0};
+        k_copy_interior<<<nblocks(nin), BLOCK, 0, c->stream>>>(c->q_cur, c->du, g, c->nv);
+        for (int s = 0; s < 3; ++s) {
+            rhs(c, c->q_cur);
+            double* qn = (c->q_cur == c->qa) ? c->qb : c->qa;
+            k_rk3_stage<<<nblocks(nin), BLOCK, 0, c->stream>>>(qn, c->du, c->R, c->scal, g, c->nv, rknew[s], rkold[s]);
+            apply_bcs(c, qn);
+            c->q_prev = c->q_cur;
+            c->q_cur = qn;
+        }
+    }
+    c->have_prev = true;
+    return check_launch(c);
+}
+
+int validate(const osb_problem_desc* d, std::string& why) {
+    if (!d) { why = "null descriptor"; return OSB_E_INVALID; }
+    if (d->struct_size != sizeof(osb_problem_desc) || d->abi_version != OSB_ABI_VERSION) { why = "descriptor size/abi mismatch"; return OSB_E_INVALID; }
+    if (d->ndim < 1 || d->ndim > 3) { why = "ndim must be 1..3"; return OSB_E_INVALID; }
+    for (int i = 0; i < 3; ++i) {
+        if (i < d->ndim && (d->n[i] < 8 || !(d->delta[i] > 0))) { why = "n >= 8 and delta > 0 required in active directions"; return OSB_E_INVALID; }
+        if (i >= d->ndim && d->n[i] != 1) { why = "inactive directions must have n == 1"; return OSB_E_INVALID; }
+    }
+    const int cs = d->conv_scheme;
+    if (cs != OSB_CONV_WENO_Z && cs != OSB_CONV_TENO && cs != OSB_CONV_CENTRAL_FEIEREISEN && cs != OSB_CONV_CENTRAL_SKEW) { why = "unknown conv_scheme"; return OSB_E_UNSUPPORTED; }
+    if (cs == OSB_CONV_WENO_Z && d->order != 5) { why = "WENO-Z supports order 5"; return OSB_E_UNSUPPORTED; }
+    if (cs == OSB_CONV_TENO && d->order != 5 && d->order != 6) { why = "TENO supports orders 5 and 6"; return OSB_E_UNSUPPORTED; }
+    if ((cs == OSB_CONV_CENTRAL_FEIEREISEN || cs == OSB_CONV_CENTRAL_SKEW) && d->order != 4) { why = "central scheme supports order 4"; return OSB_E_UNSUPPORTED; }
+    if ((cs == OSB_CONV_WENO_Z || cs == OSB_CONV_TENO) && d->averaging != OSB_AVG_SIMPLE && d->averaging != OSB_AVG_ROE) { why = "unknown averaging"; return OSB_E_UNSUPPORTED; }
+    if (d->viscous != OSB_VISC_NONE && d->viscous != OSB_VISC_CONSTANT && d->viscous != OSB_VISC_SUTHERLAND) { why = "unknown viscous model"; return OSB_E_UNSUPPORTED; }
+    if (d->viscous != OSB_VISC_NONE && d->ndim < 2) { why = "viscous terms need ndim >= 2"; return OSB_E_UNSUPPORTED; }
+    if (d->time_scheme != OSB_TIME_RKLS3 && d->time_scheme != OSB_TIME_RK3) { why = "unknown time scheme"; return OSB_E_UNSUPPORTED; }
+    if (d->precision != OSB_PREC_FP64 && d->precision != OSB_PREC_MIXED) { why = "unknown precision"; return OSB_E_UNSUPPORTED; }
+    for (int i = 0; i < d->ndim; ++i) {
+        for (int s = 0; s < 2; ++s) {
+            const int b = d->bc[i][s];
+            if (b < OSB_BC_PERIODIC || b > OSB_BC_SYMMETRY) { why = "unknown boundary condition"; return OSB_E_UNSUPPORTED; }
+        }
+        if ((d->bc[i][0] == OSB_BC_PERIODIC) != (d->bc[i][1] == OSB_BC_PERIODIC)) { why = "periodic boundaries must be paired"; return OSB_E_INVALID; }
+    }
+    if (!(d->cfl > 0) && !(d->dt > 0)) { why = "dt > 0 or cfl > 0 required"; return OSB_E_INVALID; }
+    if (!(d->gama > 1.0)) { why = "gama > 1 required"; return OSB_E_INVALID; }
+    if (d->viscous != OSB_VISC_NONE && (!(d->re > 0) || !(d->pr > 0) || !(d->minf > 0))) { why = "Re, Pr, Minf > 0 required"; return OSB_E_INVALID; }
+    if ((cs == OSB_CONV_CENTRAL_FEIEREISEN || cs == OSB_CONV_CENTRAL_SKEW) && d->ndim < 2) { why = "central split forms need ndim >= 2"; return OSB_E_UNSUPPORTED; }
+    return OSB_OK;
+}
+
+int dalloc(osb_ctx* c, double** p, size_t count) {
+    const size_t bytes = count * sizeof(double);
+    hipError_t e = hipMalloc((void**)p, bytes);
+    if (e != hipSuccess) return set_err(nullptr, OSB_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    e = hipMemset(*p, 0, bytes);
+    if (e != hipSuccess) return set_err(nullptr, OSB_E_DEVICE, std::string("hipMemset: ") + hipGetErrorString(e));
+    c->bytes += bytes;
+    return OSB_OK;
+}
+
+void free_ctx(osb_ctx* c) {
+    if (!c) return;
+    double* bufs[] = {c->qa, c->qb, c->du, c->R, c->F, c->cr, c->gr, c->scal, c->staging, c->part};
+    for (double* b : bufs) if (b) (void)hipFree(b);
+    if (c->nan_cnt) (void)hipFree(c->nan_cnt);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+}  // namespace
+
+extern "C" {
+
+int osb_abi_version(void) { return OSB_ABI_VERSION; }
+
+const char* osb_last_error(const osb_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
+    if (!out) return set_err(nullptr, OSB_E_INVALID, "null output pointer");
+    *out = nullptr;
+    std::string why;
+    int rc = validate(desc, why);
+    if (rc) return set_err(nullptr, rc, why);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(nullptr, OSB_E_DEVICE, "no HIP device available");
+
+    osb_ctx* c = new osb_ctx();
+    c->d = *desc;
+    (void)hipGetDevice(&c->device);
+    const int nd = desc->ndim;
+    c->nv = nd + 2;
+    c->shock = desc->conv_scheme == OSB_CONV_WENO_Z || desc->conv_scheme == OSB_CONV_TENO;
+    c->central = !c->shock;
+    c->viscous = desc->viscous != OSB_VISC_NONE;
+    c->mixed = desc->precision == OSB_PREC_MIXED;
+    c->sch = desc->conv_scheme == OSB_CONV_WENO_Z ? SCH_WENOZ : (desc->order == 6 ? SCH_TENO6 : SCH_TENO5);
+
+    Geom& g = c->g;
+    g.ndim = nd;
+    g.halo = c->shock ? 4 : 2;     // symmetric scheme halo: WENO5/TENO [-3,4] -> 4, Central(4) -> 2
+    for (int i = 0; i < 3; ++i) {
+        g.n[i] = desc->n[i];
+        g.P[i] = i < nd ? desc->n[i] + 2 * g.halo : 1;
+        g.o[i] = i < nd ? g.halo : 0;
+    }
+    g.s[0] = 1;
+    g.s[1] = g.P[0];
+    g.s[2] = (long)g.P[0] * g.P[1];
+    g.Ptot = (long)g.P[0] * g.P[1] * g.P[2];
+
+    Consts& k = c->c;
+    k.gama = desc->gama;
+    k.gamma_m1 = desc->gama - 1.0;
+    k.minf = desc->minf;
+    k.sfc = desc->shock_filter_control;
+    k.weno_eps = desc->weno_eps;
+    k.teno_eps = desc->teno_eps;
+    k.teno_ct = desc->teno_ct;
+    k.invRe = c->viscous ? 1.0 / desc->re : 0.0;
+    k.cheat = c->viscous ? 1.0 / ((desc->gama - 1.0) * desc->minf * desc->minf * desc->pr * desc->re) : 0.0;
+    for (int i = 0; i < 3; ++i) {
+        k.invd[i] = 1.0 / desc->delta[i];
+        k.inv2d[i] = 1.0 / (desc->delta[i] * desc->delta[i]);
+    }
+    k.cfl = desc->cfl;
+    k.dt = desc->dt;
+    k.avg_roe = desc->averaging == OSB_AVG_ROE;
+    k.viscous = desc->viscous;
+
+    const size_t P = (size_t)g.Ptot, nvP = (size_t)c->nv * P;
+    int r = OSB_OK;
+    if (!r) r = dalloc(c, &c->qa, nvP);
+    if (!r) r = dalloc(c, &c->qb, nvP);
+    if (!r) r = dalloc(c, &c->du, nvP);
+    if (!r) r = dalloc(c, &c->R, nvP);
+    if (!r && c->shock) r = dalloc(c, &c->F, (size_t)nd * nvP);
+    if (!r && (c->central || c->viscous)) r = dalloc(c, &c->cr, (size_t)(nd + 3) * P);
+    if (!r && (c->central || c->viscous)) r = dalloc(c, &c->gr, (size_t)nd * nd * P);
+    if (!r) r = dalloc(c, &c->scal, 16);
+    if (!r) r = dalloc(c, &c->staging, (size_t)c->nv * interior_points(g));
+    c->part_blocks = 1024;
+    if (!r) r = dalloc(c, &c->part, (size_t)c->part_blocks * 4);
+    if (!r) {
+        hipError_t e = hipMalloc((void**)&c->nan_cnt, sizeof(unsigned int));
+        if (e != hipSuccess) r = set_err(nullptr, OSB_E_NOMEM, "hipMalloc nan counter");
+    }
+    if (!r) {
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) r = set_err(nullptr, OSB_E_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        else c->own_stream = true;
+    }
+    if (!r) {
+        double s[16] = {0};
+        s[1] = desc->dt;
+        hipError_t e = hipMemcpy(c->scal, s, sizeof(s), hipMemcpyHostToDevice);
+        if (e != hipSuccess) r = set_err(nullptr, OSB_E_DEVICE, "scalar upload");
+    }
+    if (r) { free_ctx(c); return r; }
+    c->q_cur = c->qa;
+    c->q_prev = c->qb;
+    *out = c;
+    return OSB_OK;
+}
+
+int osb_ctx_destroy(osb_ctx* c) {
+    if (!c) return OSB_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_ctx(c);
+    return OSB_OK;
+}
+
+int osb_set_stream(osb_ctx* c, void* s) {
+    if (!c) return OSB_E_INVALID;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->own_stream) { (void)hipStreamDestroy(c->stream); c->own_stream = false; }
+    c->stream = (hipStream_t)s;
+    return OSB_OK;
+}
+
+int osb_set_state(osb_ctx* c, const double* host, int layout) {
+    if (!c || !host) return OSB_E_INVALID;
+    if (layout != OSB_LAYOUT_INTERIOR) return set_err(c, OSB_E_UNSUPPORTED, "only the interior layout is accepted");
+    const long npts = interior_points(c->g);
+    HIP_TRY(c, hipMemcpyAsync(c->staging, host, sizeof(double) * npts * c->nv, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->qa, 0, sizeof(double) * c->nv * c->g.Ptot, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->du, 0, sizeof(double) * c->nv * c->g.Ptot, c->stream));
+    k_scatter<<<nblocks(npts * c->nv), BLOCK, 0, c->stream>>>(c->staging, c->qa, c->g, c->nv);
+    c->q_cur = c->qa;
+    c->q_prev = c->qb;
+    c->have_prev = false;
+    apply_bcs(c, c->q_cur);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return check_launch(c);
+}
+
+int osb_get_state(osb_ctx* c, double* host, int layout) {
+    if (!c || !host) return OSB_E_INVALID;
+    if (layout != OSB_LAYOUT_INTERIOR) return set_err(c, OSB_E_UNSUPPORTED, "only the interior layout is returned");
+    const long npts = interior_points(c->g);
+    k_gather<<<nblocks(npts * c->nv), BLOCK, 0, c->stream>>>(c->q_cur, c->staging, c->g, c->nv);
+    HIP_TRY(c, hipMemcpyAsync(host, c->staging, sizeof(double) * npts * c->nv, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return check_launch(c);
+}
+
+int osb_set_time(osb_ctx* c, double time, int64_t iter) {
+    if (!c) return OSB_E_INVALID;
+    c->time = time;
+    c->iter = iter;
+    double t = time;
+    HIP_TRY(c, hipMemcpyAsync(c->scal + 2, &t, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return OSB_OK;
+}
+
+int osb_apply_bcs(osb_ctx* c) {
+    if (!c) return OSB_E_INVALID;
+    apply_bcs(c, c->q_cur);
+    return check_launch(c);
+}
+
+int osb_residual(osb_ctx* c, double* host_out) {
+    if (!c || !host_out) return OSB_E_INVALID;
+    apply_bcs(c, c->q_cur);
+    rhs(c, c->q_cur);
+    const long npts = interior_points(c->g);
+    k_gather<<<nblocks(npts * c->nv), BLOCK, 0, c->stream>>>(c->R, c->staging, c->g, c->nv);
+    HIP_TRY(c, hipMemcpyAsync(host_out, c->staging, sizeof(double) * npts * c->nv, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return check_launch(c);
+}
+
+int osb_max_wavespeed(osb_ctx* c, double* out) {
+    if (!c || !out) return OSB_E_INVALID;
+    launch_wavespeed(c, c->q_cur);
+    unsigned long long bits = 0;
+    HIP_TRY(c, hipMemcpyAsync(&bits, c->scal, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    double v;
+    std::memcpy(&v, &bits, sizeof(v));
+    *out = v;
+    return check_launch(c);
+}
+
+int osb_step(osb_ctx* c, int nsteps) {
+    if (!c || nsteps < 0) return OSB_E_INVALID;
+    for (int i = 0; i < nsteps; ++i) {
+        int r = do_step(c);
+        if (r) return r;
+        c->iter += 1;
+        if (!(c->d.cfl > 0)) { c->time += c->d.dt; c->last_dt = c->d.dt; }
+    }
+    return OSB_OK;
+}
+
+int osb_sync(osb_ctx* c) {
+    if (!c) return OSB_E_INVALID;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return check_launch(c);
+}
+
+int osb_get_stats(osb_ctx* c, osb_stats* out) {
+    if (!c || !out) return OSB_E_INVALID;
+    out->struct_size = sizeof(osb_stats);
+    out->iter = c->iter;
+    if (c->d.cfl > 0) {
+        double s[3];
+        HIP_TRY(c, hipMemcpyAsync(s, c->scal, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        c->time = s[2];
+        c->last_dt = s[1];
+    }
+    out->time = c->time;
+    out->last_dt = c->last_dt;
+    out->halo = c->g.halo;
+    out->nv = c->nv;
+    out->device_bytes = (int64_t)c->bytes;
+    return OSB_OK;
+}
+
+int osb_tgv_diagnostics(osb_ctx* c, double out[4]) {
+    if (!c || !out) return OSB_E_INVALID;
+    if (c->g.ndim != 3) return set_err(c, OSB_E_UNSUPPORTED, "TGV diagnostics need ndim == 3");
+    if (!c->cr) return set_err(c, OSB_E_UNSUPPORTED, "TGV diagnostics need a viscous configuration");
+    // velocity and mu from the state at the start of the last stage (the
+    // reference's constituent-relation arrays), rho from the current state
+    const double* qcr = c->have_prev ? c->q_prev : c->q_cur;
+    k_constituent<3><<<nblocks(box_points(c->g, 2)), BLOCK, 0, c->stream>>>(qcr, c->cr, c->g, c->c, 2);
+    const long nin = interior_points(c->g);
+    const unsigned nb = std::min<unsigned>(nblocks(nin), (unsigned)c->part_blocks);
+    k_tgv_partials<<<nb, BLOCK, 0, c->stream>>>(c->q_cur, c->cr, c->part, c->g, c->c);
+    k_sum_partials<<<1, 64, 0, c->stream>>>(c->part, c->scal + 4, (int)nb);
+    HIP_TRY(c, hipMemcpyAsync(out, c->scal + 4, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return check_launch(c);
+}
+
+int osb_nan_check(osb_ctx* c) {
+    if (!c) return OSB_E_INVALID;
+    HIP_TRY(c, hipMemsetAsync(c->nan_cnt, 0, sizeof(unsigned int), c->stream));
+    const long nin = interior_points(c->g);
+    k_nan_count<<<nblocks(nin), BLOCK, 0, c->stream>>>(c->q_cur, c->nan_cnt, c->g);
+    unsigned int cnt = 0;
+    HIP_TRY(c, hipMemcpyAsync(&cnt, c->nan_cnt, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (cnt) return set_err(c, OSB_E_NONFINITE, "non-finite rho at " + std::to_string(cnt) + " points");
+    return check_launch(c);
+}
+
+int osb_state_device_ptr(osb_ctx* c, int var, void** out) {
+    if (!c || !out || var < 0 || var >= c->nv) return OSB_E_INVALID;
+    *out = (void*)(c->q_cur + (long)var * c->g.Ptot);
+    return OSB_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched ensembles
+// ---------------------------------------------------------------------------
+struct osb_batch {
+    std::vector<osb_ctx*> ctx;
+    std::string err;
+};
+
+extern "C" {
+
+const char* osb_batch_last_error(const osb_batch* b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+int osb_batch_create(const osb_problem_desc* descs, int n, osb_batch** out) {
+    if (!out || !descs || n < 1) return set_err(nullptr, OSB_E_INVALID, "bad batch arguments");
+    *out = nullptr;
+    osb_batch* b = new osb_batch();
+    for (int i = 0; i < n; ++i) {
+        osb_ctx* c = nullptr;
+        int r = osb_ctx_create(&descs[i], &c);
+        if (r) {
+            for (osb_ctx* x : b->ctx) osb_ctx_destroy(x);
+            delete b;
+            return r;
+        }
+        b->ctx.push_back(c);
+    }
+    *out = b;
+    return OSB_OK;
+}
+
+int osb_batch_destroy(osb_batch* b) {
+    if (!b) return OSB_OK;
+    for (osb_ctx* c : b->ctx) osb_ctx_destroy(c);
+    delete b;
+    return OSB_OK;
+}
+
+int osb_batch_set_state(osb_batch* b, int i, const double* host, int layout) {
+    if (!b || i < 0 || i >= (int)b->ctx.size()) return OSB_E_INVALID;
+    int r = osb_set_state(b->ctx[i], host, layout);
+    if (r) b->err = b->ctx[i]->err;
+    return r;
+}
+
+int osb_batch_get_state(osb_batch* b, int i, double* host, int layout) {
+    if (!b || i < 0 || i >= (int)b->ctx.size()) return OSB_E_INVALID;
+    int r = osb_get_state(b->ctx[i], host, layout);
+    if (r) b->err = b->ctx[i]->err;
+    return r;
+}
+
+int osb_batch_step(osb_batch* b, int nsteps) {
+    if (!b || nsteps < 0) return OSB_E_INVALID;
+    for (osb_ctx* c : b->ctx) {
+        int r = osb_step(c, nsteps);
+        if (r) { b->err = c->err; return r; }
+    }
+    return OSB_OK;
+}
+
+int osb_batch_sync(osb_batch* b) {
+    if (!b) return OSB_E_INVALID;
+    for (osb_ctx* c : b->ctx) {
+        int r = osb_sync(c);
+        if (r) { b->err = c->err; return r; }
+    }
+    return OSB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,536 @@
+// osb_kernels.hpp -- gfx950 kernels of the OpenSBLI explicit-solver hot path.
+#pragma once
+
+#include "osb_common.hpp"
+
+namespace osb {
+
+constexpr int BLOCK = 256;
+
+__device__ __forceinline__ long gtid() { return (long)blockIdx.x * blockDim.x + threadIdx.x; }
+
+// decompose t over extents e[0..2] (x fastest) with lower bounds lo[]
+__device__ __forceinline__ void unflatten(long t, const int (&e)[3], const int (&lo)[3], int& i, int& j, int& k) {
+    const int e0 = e[0], e1 = e[1];
+    i = (int)(t % e0) + lo[0];
+    const long r = t / e0;
+    j = (int)(r % e1) + lo[1];
+    k = (int)(r / e1) + lo[2];
+}
+
+// ---------------------------------------------------------------------------
+// Interface fluxes along DIR for faces i+1/2, i in [-1, n) along DIR and the
+// interior across (TENO6/opensbli.cpp:263-316 ranges).
+// ---------------------------------------------------------------------------
+template <int NDIM, int DIR, int SCH, typename T>
+__global__ void __launch_bounds__(BLOCK) k_face_flux(const double* __restrict__ q, double* __restrict__ F, Geom g, Consts c) {
+    constexpr int NV = NDIM + 2;
+    int e[3] = {g.n[0], g.n[1], g.n[2]};
+    e[DIR] += 1;
+    int lo[3] = {0, 0, 0};
+    lo[DIR] = -1;
+    const long t = gtid();
+    if (t >= (long)e[0] * e[1] * e[2]) return;
+    int i, j, k;
+    unflatten(t, e, lo, i, j, k);
+    const long x0 = gidx(g, i, j, k);
+    double out[NV];
+    face_flux<NDIM, DIR, SCH, T>(q, g.Ptot, g.s[DIR], x0, c, out);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) F[(long)v * g.Ptot + x0] = out[v];
+}
+
+// R_v = -sum_d (F_d(i) - F_d(i-1)) / Delta_d   (shock_capturing.py:23-36)
+template <int NDIM>
+__global__ void __launch_bounds__(BLOCK) k_flux_residual(const double* __restrict__ F, double* __restrict__ R, Geom g, Consts c) {
+    constexpr int NV = NDIM + 2;
+    const int e[3] = {g.n[0], g.n[1], g.n[2]};
+    const int lo[3] = {0, 0, 0};
+    const long t = gtid();
+    if (t >= (long)e[0] * e[1] * e[2]) return;
+    int i, j, k;
+    unflatten(t, e, lo, i, j, k);
+    const long x = gidx(g, i, j, k);
+    const long nvP = (long)NV * g.Ptot;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        double acc = 0.0;
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) {
+            const double* w = F + d * nvP + (long)v * g.Ptot;
+            acc -= (w[x] - w[x - g.s[d]]) * c.invd[d];
+        }
+        R[(long)v * g.Ptot + x] = acc;
+    }
+}
+
+// Constituent relations u_i, p, T, mu on the box [-ext, n+ext)
+// (compressible_TGV.py:78-83; TENO6/opensbliblock00_kernels.h:48-91)
+template <int NDIM>
+__global__ void __launch_bounds__(BLOCK) k_constituent(const double* __restrict__ q, double* __restrict__ cr, Geom g, Consts c, int ext) {
+    constexpr int NV = NDIM + 2;
+    int e[3], lo[3];
+    for (int d = 0; d < 3; ++d) {
+        const bool act = d < NDIM;
+        e[d] = act ? g.n[d] + 2 * ext : 1;
+        lo[d] = act ? -ext : 0;
+    }
+    const long t = gtid();
+    if (t >= (long)e[0] * e[1] * e[2]) return;
+    int i, j, k;
+    unflatten(t, e, lo, i, j, k);
+    const long x = gidx(g, i, j, k);
+    const double r = q[x];
+    double ke = 0.0;
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) {
+        const double u = q[(long)(1 + d) * g.Ptot + x] / r;
+        cr[(long)d * g.Ptot + x] = u;
+        ke -= 0.5 * (u * u) * r;
+    }
+    const double p = c.gamma_m1 * (ke + q[(long)(NV - 1) * g.Ptot + x]);
+    const double TT = (c.minf * c.minf) * c.gama * p / r;
+    cr[(long)NDIM * g.Ptot + x] = p;
+    cr[(long)(NDIM + 1) * g.Ptot + x] = TT;
+    cr[(long)(NDIM + 2) * g.Ptot + x] = 1.4042 * TT * sqrt(TT) / (0.40417 + TT);
+}
+
+template <class A>
+__device__ __forceinline__ double d1(A f, double invd) {
+    return ((1.0 / 12.0) * (f(-2) - f(2)) + (2.0 / 3.0) * (f(1) - f(-1))) * invd;
+}
+template <class A>
+__device__ __forceinline__ double d2(A f, double inv2d) {
+    return ((-1.0 / 12.0) * (f(-2) + f(2)) + (4.0 / 3.0) * (f(-1) + f(1)) - (5.0 / 2.0) * f(0)) * inv2d;
+}
+
+// Stored velocity gradients g[m*NDIM+j] = D_j u_m on [0,n) along j and
+// [-2, n+2) across (StoreSome.py:87-135; TENO6/opensbli.cpp:340-384).
+template <int NDIM>
+__global__ void __launch_bounds__(BLOCK) k_gradients(const double* __restrict__ cr, double* __restrict__ gr, Geom g, Consts c) {
+    int e[3], lo[3];
+    for (int d = 0; d < 3; ++d) {
+        const bool act = d < NDIM;
+        e[d] = act ? g.n[d] + 4 : 1;
+        lo[d] = act ? -2 : 0;
+    }
+    const long t = gtid();
+    if (t >= (long)e[0] * e[1] * e[2]) return;
+    int ii[3];
+    unflatten(t, e, lo, ii[0], ii[1], ii[2]);
+    const long x = gidx(g, ii[0], ii[1], ii[2]);
+#pragma unroll
+    for (int jd = 0; jd < NDIM; ++jd) {
+        if (ii[jd] < 0 || ii[jd] >= g.n[jd]) continue;
+        const long st = g.s[jd];
+#pragma unroll
+        for (int m = 0; m < NDIM; ++m) {
+            const double* um = cr + (long)m * g.Ptot;
+            gr[(long)(m * NDIM + jd) * g.Ptot + x] = d1([&](int s) { return um[x + s * st]; }, c.invd[jd]);
+        }
+    }
+}
+
+// Central split-form convective residual, written first.
+// SKEW = false: Feiereisen form (split_forms.py:111-144) with StoreSome gradients.
+// SKEW = true : Skew(ab) = 1/2(D(ab) + b D(a) + a D(b)) terms of the tg_sym app
+//               (parsing.py:96-112; verify_tg_sym.py:32-36).
+template <int NDIM, bool SKEW>
+__global__ void __launch_bounds__(BLOCK) k_central_convective(const double* __restrict__ q, const double* __restrict__ cr,
+                                                              const double* __restrict__ gr, double* __restrict__ R, Geom g, Consts c) {
+    constexpr int NV = NDIM + 2;
+    const int e[3] = {g.n[0], g.n[1], g.n[2]};
+    const int lo[3] = {0, 0, 0};
+    const long t = gtid();
+    if (t >= (long)e[0] * e[1] * e[2]) return;
+    int i, j, k;
+    unflatten(t, e, lo, i, j, k);
+    const long x = gidx(g, i, j, k);
+    const long P = g.Ptot;
+    const double* rho = q;
+    const double* rhoE = q + (NV - 1) * P;
+    const double* pp = cr + NDIM * P;
+    double div_rhou = 0.0, Rrho = 0.0, eterm = 0.0;
+    double conv[NDIM];
+#pragma unroll
+    for (int m = 0; m < NDIM; ++m) conv[m] = 0.0;
+#pragma unroll
+    for (int jd = 0; jd < NDIM; ++jd) {
+        const long st = g.s[jd];
+        const double inv = c.invd[jd];
+        const double* rhou_j = q + (long)(1 + jd) * P;
+        const double* u_j = cr + (long)jd * P;
+        auto at = [&](const double* a, int s) { return a[x + s * st]; };
+        if (!SKEW) {
+            const double d_rhouj = d1([&](int s) { return at(rhou_j, s); }, inv);
+            div_rhou += d_rhouj;
+#pragma unroll
+            for (int m = 0; m < NDIM; ++m) {
+                const double* rhou_m = q + (long)(1 + m) * P;
+                const double d_prod = d1([&](int s) { return at(rhou_m, s) * at(u_j, s); }, inv);
+                conv[m] += d_prod + at(rhou_j, 0) * gr[(long)(m * NDIM + jd) * P + x];
+            }
+            const double d_Eu = d1([&](int s) { return at(rhoE, s) * at(u_j, s); }, inv);
+            const double d_E = d1([&](int s) { return at(rhoE, s) / at(rho, s); }, inv);
+            const double d_pu = d1([&](int s) { return at(pp, s) * at(u_j, s); }, inv);
+            eterm += -0.5 * (d_Eu + at(rhou_j, 0) * d_E) - d_pu;
+        } else {
+            const double ujc = at(u_j, 0);
+            const double d_rhou = d1([&](int s) { return at(rho, s) * at(u_j, s); }, inv);
+            const double d_rho = d1([&](int s) { return at(rho, s); }, inv);
+            const double d_u = d1([&](int s) { return at(u_j, s); }, inv);
+            Rrho -= 0.5 * (d_rhou + ujc * d_rho + rho[x] * d_u);
+#pragma unroll
+            for (int m = 0; m < NDIM; ++m) {
+                const double* rhou_m = q + (long)(1 + m) * P;
+                const double d_prod = d1([&](int s) { return at(rhou_m, s) * at(u_j, s); }, inv);
+                const double d_rm = d1([&](int s) { return at(rhou_m, s); }, inv);
+                conv[m] += 0.5 * (d_prod + ujc * d_rm + at(rhou_m, 0) * d_u);
+            }
+            const double d_Eu = d1([&](int s) { return at(rhoE, s) * at(u_j, s); }, inv);
+            const double d_rE = d1([&](int s) { return at(rhoE, s); }, inv);
+            const double d_pu = d1([&](int s) { return at(pp, s) * at(u_j, s); }, inv);
+            eterm += -0.5 * (d_Eu + ujc * d_rE + rhoE[x] * d_u) - d_pu;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < NDIM; ++m) {
+        const long st = g.s[m];
+        const double dp = d1([&](int s) { return pp[x + s * st]; }, c.invd[m]);
+        double rm;
+        if (!SKEW) rm = -dp - 0.5 * (conv[m] + cr[(long)m * P + x] * div_rhou);
+        else rm = -dp - conv[m];
+        R[(long)(1 + m) * P + x] = rm;
+    }
+    if (!SKEW) {
+        R[x] = -div_rhou;
+        R[(long)(NV - 1) * P + x] = eterm - 0.5 * (rhoE[x] / rho[x]) * div_rhou;
+    } else {
+        R[x] = Rrho;
+        R[(long)(NV - 1) * P + x] = eterm;
+    }
+}
+
+// Viscous stresses and heat conduction added to R (split_forms.py:383-392;
+// TENO6/opensbliblock00_kernels.h:4568-4707).  Mixed second derivatives are
+// D_b(D_a u) with a < b on the stored gradients.
+template <int NDIM, bool SUTH>
+__global__ void __launch_bounds__(BLOCK) k_viscous(const double* __restrict__ cr, const double* __restrict__ gr,
+                                                   double* __restrict__ R, Geom g, Consts c) {
+    constexpr int NV = NDIM + 2;
+    const int e[3] = {g.n[0], g.n[1], g.n[2]};
+    const int lo[3] = {0, 0, 0};
+    const long t = gtid();
+    if (t >= (long)e[0] * e[1] * e[2]) return;
+    int i, j, k;
+    unflatten(t, e, lo, i, j, k);
+    const long x = gidx(g, i, j, k);
+    const long P = g.Ptot;
+    const double* Ta = cr + (long)(NDIM + 1) * P;
+    const double* mua = cr + (long)(NDIM + 2) * P;
+    double G[NDIM][NDIM], D2u[NDIM][NDIM], dmu[NDIM], dT[NDIM];
+    double lapT = 0.0;
+#pragma unroll
+    for (int m = 0; m < NDIM; ++m)
+#pragma unroll
+        for (int jd = 0; jd < NDIM; ++jd) G[m][jd] = gr[(long)(m * NDIM + jd) * P + x];
+#pragma unroll
+    for (int jd = 0; jd < NDIM; ++jd) {
+        const long st = g.s[jd];
+        dmu[jd] = SUTH ? d1([&](int s) { return mua[x + s * st]; }, c.invd[jd]) : 0.0;
+        dT[jd] = d1([&](int s) { return Ta[x + s * st]; }, c.invd[jd]);
+        lapT += d2([&](int s) { return Ta[x + s * st]; }, c.inv2d[jd]);
+#pragma unroll
+        for (int m = 0; m < NDIM; ++m) {
+            const double* um = cr + (long)m * P;
+            D2u[m][jd] = d2([&](int s) { return um[x + s * st]; }, c.inv2d[jd]);
+        }
+    }
+    double div = 0.0;
+#pragma unroll
+    for (int m = 0; m < NDIM; ++m) div += G[m][m];
+    const double muv = SUTH ? mua[x] : 1.0;
+    double Eacc = 0.0;
+#pragma unroll
+    for (int m = 0; m < NDIM; ++m) {
+        double Lm = (4.0 / 3.0) * D2u[m][m];
+#pragma unroll
+        for (int jd = 0; jd < NDIM; ++jd) {
+            if (jd == m) continue;
+            const int a = jd < m ? jd : m, b = jd < m ? m : jd;
+            const double* ga = gr + (long)(jd * NDIM + a) * P;
+            const long sb = g.s[b];
+            const double mixed = d1([&](int s) { return ga[x + s * sb]; }, c.invd[b]);
+            Lm += D2u[m][jd] + (1.0 / 3.0) * mixed;
+        }
+        double mom = muv * Lm;
+        double ediss = 0.0;
+#pragma unroll
+        for (int jd = 0; jd < NDIM; ++jd) {
+            const double S = G[m][jd] + G[jd][m] - (m == jd ? (2.0 / 3.0) * div : 0.0);
+            if (SUTH) mom += S * dmu[jd];
+            ediss += muv * S * G[m][jd];
+        }
+        R[(long)(1 + m) * P + x] += c.invRe * mom;
+        Eacc += cr[(long)m * P + x] * mom + ediss;
+    }
+    double heat = muv * lapT;
+    if (SUTH) {
+#pragma unroll
+        for (int jd = 0; jd < NDIM; ++jd) heat += dT[jd] * dmu[jd];
+    }
+    R[(long)(NV - 1) * P + x] += c.invRe * Eacc + c.cheat * heat;
+}
+
+// Low-storage RK3 stage (rk_LS.py:204-215): du = A du + dt R; q_out = q_in + B du
+__global__ void __launch_bounds__(BLOCK) k_rkls(const double* __restrict__ qin, double* __restrict__ qout, double* __restrict__ du,
+                                                const double* __restrict__ R, const double* __restrict__ scal, Geom g, int nv,
+                                                double A, double B) {
+    const int e[3] = {g.n[0], g.n[1], g.n[2]};
+    const int lo[3] = {0, 0, 0};
+    const long t = gtid();
+    if (t >= (long)e[0] * e[1] * e[2]) return;
+    int i, j, k;
+    unflatten(t, e, lo, i, j, k);
+    const long x = gidx(g, i, j, k);
+    const double dt = scal[1];
+    for (int v = 0; v < nv; ++v) {
+        const long y = (long)v * g.Ptot + x;
+        const double d = A * du[y] + dt * R[y];
+        du[y] = d;
+        qout[y] = B * d + qin[y];
+    }
+}
+
+// Classic RK3 (rk_sbli.py:129-159): q = qold + dt*rknew*R; qold += dt*rkold*R
+__global__ void __launch_bounds__(BLOCK) k_rk3_stage(double* __restrict__ qout, double* __restrict__ qold,
+                                                     const double* __restrict__ R, const double* __restrict__ scal, Geom g, int nv,
+                                                     double rknew, double rkold) {
+    const int e[3] = {g.n[0], g.n[1], g.n[2]};
+    const int lo[3] = {0, 0, 0};
+    const long t = gtid();
+    if (t >= (long)e[0] * e[1] * e[2]) return;
+    int i, j, k;
+    unflatten(t, e, lo, i, j, k);
+    const long x = gidx(g, i, j, k);
+    const double dt = scal[1];
+    for (int v = 0; v < nv; ++v) {
+        const long y = (long)v * g.Ptot + x;
+        const double r = R[y], o = qold[y];
+        qout[y] = o + dt * rknew * r;
+        qold[y] = o + dt * rkold * r;
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_copy_interior(const double* __restrict__ src, double* __restrict__ dst, Geom g, int nv) {
+    const int e[3] = {g.n[0], g.n[1], g.n[2]};
+    const int lo[3] = {0, 0, 0};
+    const long t = gtid();
+    if (t >= (long)e[0] * e[1] * e[2]) return;
+    int i, j, k;
+    unflatten(t, e, lo, i, j, k);
+    const long x = gidx(g, i, j, k);
+    for (int v = 0; v < nv; ++v) dst[(long)v * g.Ptot + x] = src[(long)v * g.Ptot + x];
+}
+
+// Boundary conditions along DIR (bc_core.py:128-202): side 0 layers t=0..h
+// (t = 0 is the boundary point), side 1 likewise, over the halo-extended
+// range of the other directions.  bc0/bc1: OSB_BC_* of the two sides.
+template <int DIR>
+__global__ void __launch_bounds__(BLOCK) k_bc(double* __restrict__ q, Geom g, int nv, int bc0, int bc1) {
+    const int h = g.halo;
+    int e[3], lo[3];
+    for (int d = 0; d < 3; ++d) {
+        const bool act = d < g.ndim;
+        e[d] = act ? g.n[d] + 2 * h : 1;
+        lo[d] = act ? -h : 0;
+    }
+    e[DIR] = 2 * (h + 1);
+    lo[DIR] = 0;
+    const long t = gtid();
+    if (t >= (long)e[0] * e[1] * e[2]) return;
+    int ii[3];
+    unflatten(t, e, lo, ii[0], ii[1], ii[2]);
+    const int layer = ii[DIR];
+    const int side = layer / (h + 1);
+    const int tt = layer % (h + 1);
+    const int type = side == 0 ? bc0 : bc1;
+    const int n = g.n[DIR];
+    int dst, src;
+    bool negate = false, zero = false;
+    if (tt == 0) {
+        if (type != 2 && type != 3) return;            // zero-gradient / inviscid wall modify the boundary point
+        dst = side == 0 ? 0 : n - 1;
+        src = side == 0 ? 1 : n - 2;
+        zero = (type == 3);
+    } else {
+        if (side == 0) { dst = -tt; src = (type == 1) ? n - tt : tt; }
+        else { dst = n - 1 + tt; src = (type == 1) ? tt - 1 : n - 1 - tt; }
+        negate = (type == 3 || type == 4);
+    }
+    int a[3] = {ii[0], ii[1], ii[2]};
+    a[DIR] = dst;
+    const long xd = gidx(g, a[0], a[1], a[2]);
+    a[DIR] = src;
+    const long xs = gidx(g, a[0], a[1], a[2]);
+    for (int v = 0; v < nv; ++v) {
+        double val = q[(long)v * g.Ptot + xs];
+        if (v == 1 + DIR) {
+            if (negate) val = -val;
+            if (zero) val = 0.0 * val;
+        }
+        q[(long)v * g.Ptot + xd] = val;
+    }
+}
+
+// host interior layout (v, k, j, i) <-> padded device layout
+__global__ void __launch_bounds__(BLOCK) k_scatter(const double* __restrict__ src, double* __restrict__ q, Geom g, int nv) {
+    const long npts = (long)g.n[0] * g.n[1] * g.n[2];
+    const long t = gtid();
+    if (t >= npts * nv) return;
+    const int v = (int)(t / npts);
+    const long r = t % npts;
+    const int e[3] = {g.n[0], g.n[1], g.n[2]};
+    const int lo[3] = {0, 0, 0};
+    int i, j, k;
+    unflatten(r, e, lo, i, j, k);
+    q[(long)v * g.Ptot + gidx(g, i, j, k)] = src[t];
+}
+__global__ void __launch_bounds__(BLOCK) k_gather(const double* __restrict__ q, double* __restrict__ dst, Geom g, int nv) {
+    const long npts = (long)g.n[0] * g.n[1] * g.n[2];
+    const long t = gtid();
+    if (t >= npts * nv) return;
+    const int v = (int)(t / npts);
+    const long r = t % npts;
+    const int e[3] = {g.n[0], g.n[1], g.n[2]};
+    const int lo[3] = {0, 0, 0};
+    int i, j, k;
+    unflatten(r, e, lo, i, j, k);
+    dst[t] = q[(long)v * g.Ptot + gidx(g, i, j, k)];
+}
+
+// ---------------------------------------------------------------------------
+// Reductions.  Wave-level butterfly through DPP/permute shuffles, then LDS
+// across the block's waves, then one atomic per block.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ void k_zero_scalar(double* scal) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) { reinterpret_cast<unsigned long long*>(scal)[0] = 0ull; }
+}
+
+// max over interior points of sum_d (|u_d| + a)/Delta_d; stored as the bit
+// pattern of a non-negative double in scal[0] (order-preserving for >= 0)
+template <int NDIM>
+__global__ void __launch_bounds__(BLOCK) k_wavespeed(const double* __restrict__ q, double* scal, Geom g, Consts c) {
+    constexpr int NV = NDIM + 2;
+    __shared__ double red[BLOCK / 64];
+    const int e[3] = {g.n[0], g.n[1], g.n[2]};
+    const int lo[3] = {0, 0, 0};
+    const long tot = (long)e[0] * e[1] * e[2];
+    double m = 0.0;
+    for (long t = gtid(); t < tot; t += (long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        unflatten(t, e, lo, i, j, k);
+        const long x = gidx(g, i, j, k);
+        const double r = q[x];
+        double uu[NDIM], ke = 0.0;
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) { uu[d] = q[(long)(1 + d) * g.Ptot + x] / r; ke -= 0.5 * (uu[d] * uu[d]) * r; }
+        const double p = c.gamma_m1 * (ke + q[(long)(NV - 1) * g.Ptot + x]);
+        const double a = sqrt(c.gama * p / r);
+        double s = 0.0;
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) s += (fabs(uu[d]) + a) * c.invd[d];
+        m = fmax(m, s);
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = fmax(b, red[w]);
+        atomicMax(reinterpret_cast<unsigned long long*>(scal), (unsigned long long)__double_as_longlong(b));
+    }
+}
+
+// dt = cfl / max wave speed; time += dt
+__global__ void k_dt_finalize(double* scal, double cfl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const double m = __longlong_as_double((long long)reinterpret_cast<unsigned long long*>(scal)[0]);
+        const double dt = cfl / m;
+        scal[1] = dt;
+        scal[2] += dt;
+    }
+}
+
+// Taylor-Green reductions: per-block partial sums (deterministic two-pass)
+__global__ void __launch_bounds__(BLOCK) k_tgv_partials(const double* __restrict__ qrho, const double* __restrict__ cr,
+                                                        double* __restrict__ part, Geom g, Consts c) {
+    __shared__ double red[4][BLOCK / 64];
+    const int e[3] = {g.n[0], g.n[1], g.n[2]};
+    const int lo[3] = {0, 0, 0};
+    const long tot = (long)e[0] * e[1] * e[2];
+    double acc[4] = {0, 0, 0, 0};
+    const long P = g.Ptot;
+    for (long t = gtid(); t < tot; t += (long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        unflatten(t, e, lo, i, j, k);
+        const long x = gidx(g, i, j, k);
+        double G[3][3];
+        for (int m = 0; m < 3; ++m)
+            for (int d = 0; d < 3; ++d) {
+                const double* um = cr + (long)m * P;
+                const long st = g.s[d];
+                G[m][d] = d1([&](int s) { return um[x + s * st]; }, c.invd[d]);
+            }
+        const double wx = G[2][1] - G[1][2], wy = G[0][2] - G[2][0], wz = G[1][0] - G[0][1];
+        const double divV = G[0][0] + G[1][1] + G[2][2];
+        const double rho = qrho[x], mu = cr[5 * P + x];
+        const double u0 = cr[x], u1 = cr[P + x], u2 = cr[2 * P + x];
+        acc[0] += 0.5 * (u0 * u0 + u1 * u1 + u2 * u2) * rho;
+        acc[1] += (4.0 / 3.0) * (divV * divV) * mu;
+        acc[2] += (wx * wx + wy * wy + wz * wz) * mu;
+        acc[3] += rho;
+    }
+    for (int r = 0; r < 4; ++r) {
+        const double s = wave_sum(acc[r]);
+        if ((threadIdx.x & 63) == 0) red[r][threadIdx.x >> 6] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double s = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[threadIdx.x][w];
+        part[(long)blockIdx.x * 4 + threadIdx.x] = s;
+    }
+}
+
+__global__ void k_sum_partials(const double* __restrict__ part, double* __restrict__ out, int nblocks) {
+    if (blockIdx.x == 0 && threadIdx.x < 4) {
+        double s = 0.0;
+        for (int b = 0; b < nblocks; ++b) s += part[(long)b * 4 + threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_nan_count(const double* __restrict__ q, unsigned int* cnt, Geom g) {
+    const int e[3] = {g.n[0], g.n[1], g.n[2]};
+    const int lo[3] = {0, 0, 0};
+    const long t = gtid();
+    if (t >= (long)e[0] * e[1] * e[2]) return;
+    int i, j, k;
+    unflatten(t, e, lo, i, j, k);
+    const double r = q[gidx(g, i, j, k)];
+    if (!isfinite(r)) atomicAdd(cnt, 1u);
+}
+
+}  // namespace osb

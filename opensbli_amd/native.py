@@ -1,0 +1,388 @@
+"""ctypes binding of the C ABI declared in include/osb.h.
+
+The same binding drives two libraries with identical entry points:
+
+* ``libosb_hip.so``  -- the product (HIP kernels for gfx950), located next to
+  this file and built by ``opensbli_amd/csrc/Makefile``;
+* the CPU oracle library, which only the tests and the benchmark's CPU
+  baseline load by explicit path.
+
+Nothing here falls back from one to the other: ``load_hip()`` raises if the
+HIP library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+ABI_VERSION = 1
+
+OK = 0
+E_INVALID = -1
+E_UNSUPPORTED = -2
+E_DEVICE = -3
+E_NOMEM = -4
+E_NONFINITE = -5
+
+CONV_WENO_Z = 1
+CONV_TENO = 2
+CONV_CENTRAL_FEIEREISEN = 3
+CONV_CENTRAL_SKEW = 4
+
+AVG_SIMPLE = 1
+AVG_ROE = 2
+
+VISC_NONE = 0
+VISC_CONSTANT = 1
+VISC_SUTHERLAND = 2
+
+TIME_RKLS3 = 1
+TIME_RK3 = 2
+
+BC_PERIODIC = 1
+BC_ZERO_GRADIENT = 2
+BC_INVISCID_WALL = 3
+BC_SYMMETRY = 4
+
+PREC_FP64 = 1
+PREC_MIXED = 2
+
+LAYOUT_INTERIOR = 0
+LAYOUT_PADDED = 1
+
+
+class ProblemDescC(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32),
+        ("abi_version", C.c_uint32),
+        ("ndim", C.c_int32),
+        ("n", C.c_int32 * 3),
+        ("delta", C.c_double * 3),
+        ("conv_scheme", C.c_int32),
+        ("order", C.c_int32),
+        ("averaging", C.c_int32),
+        ("viscous", C.c_int32),
+        ("time_scheme", C.c_int32),
+        ("precision", C.c_int32),
+        ("bc", (C.c_int32 * 2) * 3),
+        ("gama", C.c_double),
+        ("minf", C.c_double),
+        ("re", C.c_double),
+        ("pr", C.c_double),
+        ("weno_eps", C.c_double),
+        ("teno_eps", C.c_double),
+        ("teno_ct", C.c_double),
+        ("shock_filter_control", C.c_double),
+        ("dt", C.c_double),
+        ("cfl", C.c_double),
+        ("reserved", C.c_int32 * 8),
+    ]
+
+
+class StatsC(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32),
+        ("iter", C.c_int64),
+        ("time", C.c_double),
+        ("last_dt", C.c_double),
+        ("halo", C.c_int32),
+        ("nv", C.c_int32),
+        ("device_bytes", C.c_int64),
+    ]
+
+
+@dataclass
+class ProblemDesc:
+    """Python mirror of ``osb_problem_desc``."""
+
+    ndim: int
+    n: Sequence[int]
+    delta: Sequence[float]
+    conv_scheme: int
+    order: int = 5
+    averaging: int = AVG_SIMPLE
+    viscous: int = VISC_NONE
+    time_scheme: int = TIME_RKLS3
+    precision: int = PREC_FP64
+    bc: Sequence[Sequence[int]] = field(default_factory=lambda: [[BC_PERIODIC, BC_PERIODIC]] * 3)
+    gama: float = 1.4
+    minf: float = 1.0
+    re: float = 1.0
+    pr: float = 0.71
+    weno_eps: float = 1.0e-40
+    teno_eps: float = 1.0e-40
+    teno_ct: float = 1.0e-5
+    shock_filter_control: float = 1.0
+    dt: float = 0.0
+    cfl: float = 0.0
+
+    @property
+    def nv(self) -> int:
+        return self.ndim + 2
+
+    @property
+    def shape(self):
+        """Interior array shape (nv, n2, n1, n0) -- x fastest."""
+        n = list(self.n) + [1] * (3 - len(self.n))
+        return (self.nv, n[2], n[1], n[0])
+
+    @property
+    def npoints(self) -> int:
+        n = list(self.n) + [1] * (3 - len(self.n))
+        return int(n[0]) * int(n[1]) * int(n[2])
+
+    def to_c(self) -> ProblemDescC:
+        d = ProblemDescC()
+        d.struct_size = C.sizeof(ProblemDescC)
+        d.abi_version = ABI_VERSION
+        d.ndim = self.ndim
+        n = list(self.n) + [1] * (3 - len(self.n))
+        dl = list(self.delta) + [1.0] * (3 - len(self.delta))
+        for i in range(3):
+            d.n[i] = int(n[i])
+            d.delta[i] = float(dl[i])
+        d.conv_scheme = self.conv_scheme
+        d.order = self.order
+        d.averaging = self.averaging
+        d.viscous = self.viscous
+        d.time_scheme = self.time_scheme
+        d.precision = self.precision
+        bc = list(self.bc) + [[BC_PERIODIC, BC_PERIODIC]] * (3 - len(self.bc))
+        for i in range(3):
+            for s in range(2):
+                d.bc[i][s] = int(bc[i][s])
+        d.gama = self.gama
+        d.minf = self.minf
+        d.re = self.re
+        d.pr = self.pr
+        d.weno_eps = self.weno_eps
+        d.teno_eps = self.teno_eps
+        d.teno_ct = self.teno_ct
+        d.shock_filter_control = self.shock_filter_control
+        d.dt = self.dt
+        d.cfl = self.cfl
+        return d
+
+
+class OSBError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"osb error {code}: {msg}")
+        self.code = code
+
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+HIP_LIBRARY = os.path.join(_HERE, "libosb_hip.so")
+
+
+class Library:
+    """One loaded implementation of the osb C ABI."""
+
+    def __init__(self, path: str):
+        if not os.path.exists(path):
+            raise FileNotFoundError(
+                f"{path} not found; build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        self.path = path
+        self.lib = C.CDLL(path)
+        L = self.lib
+        vp = C.c_void_p
+        L.osb_abi_version.restype = C.c_int
+        L.osb_ctx_create.argtypes = [C.POINTER(ProblemDescC), C.POINTER(vp)]
+        L.osb_ctx_destroy.argtypes = [vp]
+        L.osb_last_error.argtypes = [vp]
+        L.osb_last_error.restype = C.c_char_p
+        dptr = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        L.osb_set_state.argtypes = [vp, dptr, C.c_int]
+        L.osb_get_state.argtypes = [vp, dptr, C.c_int]
+        L.osb_set_time.argtypes = [vp, C.c_double, C.c_int64]
+        L.osb_step.argtypes = [vp, C.c_int]
+        L.osb_apply_bcs.argtypes = [vp]
+        L.osb_residual.argtypes = [vp, dptr]
+        L.osb_max_wavespeed.argtypes = [vp, C.POINTER(C.c_double)]
+        L.osb_sync.argtypes = [vp]
+        L.osb_get_stats.argtypes = [vp, C.POINTER(StatsC)]
+        L.osb_tgv_diagnostics.argtypes = [vp, C.POINTER(C.c_double)]
+        L.osb_nan_check.argtypes = [vp]
+        L.osb_state_device_ptr.argtypes = [vp, C.c_int, C.POINTER(vp)]
+        L.osb_set_stream.argtypes = [vp, vp]
+        self.has_batch = hasattr(L, "osb_batch_create")
+        if self.has_batch:
+            L.osb_batch_create.argtypes = [C.POINTER(ProblemDescC), C.c_int, C.POINTER(vp)]
+            L.osb_batch_destroy.argtypes = [vp]
+            L.osb_batch_set_state.argtypes = [vp, C.c_int, dptr, C.c_int]
+            L.osb_batch_get_state.argtypes = [vp, C.c_int, dptr, C.c_int]
+            L.osb_batch_step.argtypes = [vp, C.c_int]
+            L.osb_batch_sync.argtypes = [vp]
+            L.osb_batch_last_error.argtypes = [vp]
+            L.osb_batch_last_error.restype = C.c_char_p
+        if L.osb_abi_version() != ABI_VERSION:
+            raise RuntimeError(f"{path}: ABI version {L.osb_abi_version()} != {ABI_VERSION}")
+
+    def context(self, desc: ProblemDesc) -> "Context":
+        return Context(self, desc)
+
+    def batch(self, descs: Sequence[ProblemDesc]) -> "Batch":
+        return Batch(self, descs)
+
+
+_hip_lib: Optional[Library] = None
+
+
+def load_hip() -> Library:
+    """Load the HIP product library.  Raises if it has not been built."""
+    global _hip_lib
+    if _hip_lib is None:
+        _hip_lib = Library(HIP_LIBRARY)
+    return _hip_lib
+
+
+class Context:
+    """An osb_ctx: one block's resident state on one device (or on the CPU for the oracle)."""
+
+    def __init__(self, lib: Library, desc: ProblemDesc):
+        self.lib = lib
+        self.desc = desc
+        self._cdesc = desc.to_c()
+        h = C.c_void_p()
+        rc = lib.lib.osb_ctx_create(C.byref(self._cdesc), C.byref(h))
+        if rc != OK:
+            msg = lib.lib.osb_last_error(None)
+            raise OSBError(rc, msg.decode() if msg else "")
+        self.h = h
+
+    # context manager ---------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lib.osb_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != OK:
+            msg = self.lib.lib.osb_last_error(self.h)
+            raise OSBError(rc, msg.decode() if msg else "")
+
+    # state ---------------------------------------------------------------
+    def set_state(self, q: np.ndarray):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        if q.size != self.desc.nv * self.desc.npoints:
+            raise ValueError(f"state has {q.size} values, expected {self.desc.nv * self.desc.npoints}")
+        self._check(self.lib.lib.osb_set_state(self.h, q.reshape(-1), LAYOUT_INTERIOR))
+
+    def get_state(self) -> np.ndarray:
+        out = np.empty(self.desc.shape, dtype=np.float64)
+        self._check(self.lib.lib.osb_get_state(self.h, out.reshape(-1), LAYOUT_INTERIOR))
+        return out
+
+    def set_time(self, t: float, it: int):
+        self._check(self.lib.lib.osb_set_time(self.h, float(t), int(it)))
+
+    def step(self, nsteps: int = 1):
+        self._check(self.lib.lib.osb_step(self.h, int(nsteps)))
+
+    def apply_bcs(self):
+        self._check(self.lib.lib.osb_apply_bcs(self.h))
+
+    def residual(self) -> np.ndarray:
+        out = np.empty(self.desc.shape, dtype=np.float64)
+        self._check(self.lib.lib.osb_residual(self.h, out.reshape(-1)))
+        return out
+
+    def max_wavespeed(self) -> float:
+        v = C.c_double()
+        self._check(self.lib.lib.osb_max_wavespeed(self.h, C.byref(v)))
+        return v.value
+
+    def sync(self):
+        self._check(self.lib.lib.osb_sync(self.h))
+
+    def stats(self) -> StatsC:
+        s = StatsC()
+        s.struct_size = C.sizeof(StatsC)
+        self._check(self.lib.lib.osb_get_stats(self.h, C.byref(s)))
+        return s
+
+    def tgv_diagnostics(self) -> np.ndarray:
+        out = (C.c_double * 4)()
+        self._check(self.lib.lib.osb_tgv_diagnostics(self.h, out))
+        return np.array(list(out))
+
+    def nan_check(self) -> int:
+        return self.lib.lib.osb_nan_check(self.h)
+
+    def device_ptr(self, var: int) -> int:
+        p = C.c_void_p()
+        self._check(self.lib.lib.osb_state_device_ptr(self.h, int(var), C.byref(p)))
+        return p.value
+
+    def set_stream(self, stream_handle: int):
+        self._check(self.lib.lib.osb_set_stream(self.h, C.c_void_p(stream_handle)))
+
+
+class Batch:
+    """An osb_batch: several instances advanced together (HIP library only)."""
+
+    def __init__(self, lib: Library, descs: Sequence[ProblemDesc]):
+        if not lib.has_batch:
+            raise OSBError(E_UNSUPPORTED, f"{lib.path} has no batch entry points")
+        self.lib = lib
+        self.descs = list(descs)
+        arr = (ProblemDescC * len(self.descs))(*[d.to_c() for d in self.descs])
+        self._carr = arr
+        h = C.c_void_p()
+        rc = lib.lib.osb_batch_create(arr, len(self.descs), C.byref(h))
+        if rc != OK:
+            msg = lib.lib.osb_last_error(None)
+            raise OSBError(rc, msg.decode() if msg else "")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lib.osb_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != OK:
+            msg = self.lib.lib.osb_batch_last_error(self.h)
+            raise OSBError(rc, msg.decode() if msg else "")
+
+    def set_state(self, i: int, q: np.ndarray):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        self._check(self.lib.lib.osb_batch_set_state(self.h, int(i), q.reshape(-1), LAYOUT_INTERIOR))
+
+    def get_state(self, i: int) -> np.ndarray:
+        out = np.empty(self.descs[i].shape, dtype=np.float64)
+        self._check(self.lib.lib.osb_batch_get_state(self.h, int(i), out.reshape(-1), LAYOUT_INTERIOR))
+        return out
+
+    def step(self, nsteps: int = 1):
+        self._check(self.lib.lib.osb_batch_step(self.h, int(nsteps)))
+
+    def sync(self):
+        self._check(self.lib.lib.osb_batch_sync(self.h))

@@ -1,0 +1,219 @@
+"""Initial conditions and descriptors for the benchmark and verification problems.
+
+Initial fields are built once on the host in FP64 and the same bits are
+uploaded to every implementation.  Transcendental factors are separable in
+these problems, so they are evaluated per axis with the C library (``math``)
+-- the same libm the reference's generated initialisation kernel calls -- and
+combined with IEEE multiplications in the order the reference's generated
+initialisation kernel uses (apps/performance_benchmark/TENO6/
+opensbliblock00_kernels.h:3-46).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Optional, Tuple
+
+import numpy as np
+
+from . import native as nat
+
+
+def _axis(fn, xs):
+    return np.array([fn(float(x)) for x in xs], dtype=np.float64)
+
+
+@dataclass
+class Problem:
+    name: str
+    desc: nat.ProblemDesc
+    q0: np.ndarray
+    steps: int
+
+    @property
+    def work_units(self) -> float:
+        """Grid-point RK-stage updates of the full run."""
+        return float(self.desc.npoints) * 3.0 * self.steps
+
+
+# ---------------------------------------------------------------------------
+# 1-D Sod shock tube (apps/Sod_shock_tube/Sod_shock_tube.py:72-81)
+# ---------------------------------------------------------------------------
+def sod(n: int = 1024, steps: int = 1000, dt: float = 2.0e-4, precision: int = nat.PREC_FP64) -> Problem:
+    gama = 1.4
+    delta = 1.0 / (n - 1)
+    x = delta * np.arange(n, dtype=np.float64)
+    left = x < 0.5
+    d = np.where(left, 1.0, 0.125)
+    p = np.where(left, 1.0, 0.10)
+    u0 = np.zeros(n)
+    q = np.empty((3, 1, 1, n))
+    q[0, 0, 0] = d
+    q[1, 0, 0] = d * u0
+    q[2, 0, 0] = p / (gama - 1.0) + 0.5 * d * (u0 ** 2.0)
+    desc = nat.ProblemDesc(
+        ndim=1, n=[n, 1, 1], delta=[delta, 1.0, 1.0], conv_scheme=nat.CONV_WENO_Z, order=5,
+        averaging=nat.AVG_SIMPLE, time_scheme=nat.TIME_RKLS3, precision=precision,
+        bc=[[nat.BC_ZERO_GRADIENT, nat.BC_ZERO_GRADIENT]], gama=gama, dt=dt)
+    return Problem("sod", desc, q, steps)
+
+
+# ---------------------------------------------------------------------------
+# 3-D Taylor-Green vortex (apps/taylor_green_vortex/taylor_green_vortex.py,
+# apps/compressible_taylor_green_vortex/compressible_TGV.py)
+# ---------------------------------------------------------------------------
+def _tgv_fields(n: Tuple[int, int, int], delta: Tuple[float, float, float], gama: float, minf: float,
+                offset: Tuple[int, int, int] = (0, 0, 0)):
+    xs = [delta[d] * (np.arange(n[d]) + offset[d]) for d in range(3)]
+    s = [_axis(math.sin, x) for x in xs]
+    c = [_axis(math.cos, x) for x in xs]
+    c2 = [_axis(lambda v: math.cos(2.0 * v), x) for x in xs]
+    S0, C0, C20 = s[0][None, None, :], c[0][None, None, :], c2[0][None, None, :]
+    S1, C1, C21 = s[1][None, :, None], c[1][None, :, None], c2[1][None, :, None]
+    C2, C22 = c[2][:, None, None], c2[2][:, None, None]
+    # u0 = cos(x1)*cos(x2)*sin(x0); u1 = -cos(x0)*cos(x2)*sin(x1); u2 = 0
+    u0 = (C1 * C2) * S0
+    u1 = -((C0 * C2) * S1)
+    u2 = np.zeros_like(u0)
+    # p = (2.0 + cos(2.0*x2))*(0.0625*cos(2.0*x0) + 0.0625*cos(2.0*x1)) + 1.0/((Minf*Minf)*gama)
+    p = (2.0 + C22) * (0.0625 * C20 + 0.0625 * C21) + 1.0 / ((minf * minf) * gama)
+    r = gama * p * (minf * minf)
+    rhoE = p / (-1 + gama) + 0.5 * r * ((u0 * u0) + (u1 * u1) + (u2 * u2))
+    q = np.empty((5, n[2], n[1], n[0]))
+    q[0] = r
+    q[1] = r * u0
+    q[2] = r * u1
+    q[3] = r * u2
+    q[4] = rhoE
+    return q
+
+
+def tgv(n: int = 128, steps: int = 200, dt: float = 5.0e-3, minf: float = 0.1, re: float = 1600.0,
+        pr: float = 0.71, scheme: str = "central", precision: int = nat.PREC_FP64,
+        teno_order: int = 5, teno_ct: float = 1.0e-5, teno_eps: float = 1.0e-40, cfl: float = 0.0,
+        averaging: int = nat.AVG_ROE) -> Problem:
+    """Periodic [0, 2pi)^3 Taylor-Green vortex.
+
+    scheme='central': NS_Split('Feiereisen', viscosity='constant') + StoreSome(4, 'u0 u1 u2')
+    scheme='teno'   : LFTeno(order, averaging=RoeAverage) + central Sutherland viscous terms
+    """
+    gama = 1.4
+    delta = 2.0 * math.pi / n
+    q = _tgv_fields((n, n, n), (delta, delta, delta), gama, minf)
+    if scheme == "central":
+        conv, order, visc = nat.CONV_CENTRAL_FEIEREISEN, 4, nat.VISC_CONSTANT
+    elif scheme == "teno":
+        conv, order, visc = nat.CONV_TENO, teno_order, nat.VISC_SUTHERLAND
+    elif scheme == "weno":
+        conv, order, visc = nat.CONV_WENO_Z, 5, nat.VISC_SUTHERLAND
+    else:
+        raise ValueError(scheme)
+    desc = nat.ProblemDesc(
+        ndim=3, n=[n, n, n], delta=[delta] * 3, conv_scheme=conv, order=order, averaging=averaging,
+        viscous=visc, time_scheme=nat.TIME_RKLS3, precision=precision,
+        bc=[[nat.BC_PERIODIC] * 2] * 3, gama=gama, minf=minf, re=re, pr=pr,
+        teno_eps=teno_eps, teno_ct=teno_ct, dt=dt, cfl=cfl)
+    return Problem(f"tgv_{scheme}_{n}", desc, q, steps)
+
+
+def tgv_teno6_validation(n: int = 128, steps: int = 500) -> Problem:
+    """The reference's performance-benchmark validation run ("va"):
+    apps/performance_benchmark/TENO6/opensbli.cpp:36-48, 94-140 -- TENO6,
+    RoeAverage, LLF, Sutherland viscosity, RK-LS3, dt = 2*0.00084625."""
+    return tgv(n=n, steps=steps, dt=0.00084625 * 2.0, minf=1.25, re=1600.0, pr=0.71, scheme="teno",
+               teno_order=6, teno_ct=1e-6, teno_eps=1e-40)
+
+
+def tg_sym(n: int = 33, steps: int = 1) -> Problem:
+    """Triply-symmetric Taylor-Green verification app
+    (tests/verification_apps/tg_sym/verify_tg_sym.py:12-178): Central(4) with
+    Skew convective terms, SymmetryBC on all faces, classic RK3, [0, pi]^3."""
+    gama, minf, re, pr = 1.4, 0.1, 800.0, 0.71
+    delta = math.pi / (n - 1)
+    q = _tgv_fields((n, n, n), (delta, delta, delta), gama, minf)
+    desc = nat.ProblemDesc(
+        ndim=3, n=[n, n, n], delta=[delta] * 3, conv_scheme=nat.CONV_CENTRAL_SKEW, order=4,
+        viscous=nat.VISC_CONSTANT, time_scheme=nat.TIME_RK3, precision=nat.PREC_FP64,
+        bc=[[nat.BC_SYMMETRY] * 2] * 3, gama=gama, minf=minf, re=re, pr=pr, dt=0.005)
+    return Problem("tg_sym", desc, q, steps)
+
+
+# ---------------------------------------------------------------------------
+# 2-D Kelvin-Helmholtz instance (config: periodic x, slip walls y)
+# ---------------------------------------------------------------------------
+def kelvin_helmholtz(n: int = 128, amplitude: float = 0.02, mode: int = 2, steps: int = 500,
+                     precision: int = nat.PREC_FP64, dt: Optional[float] = None) -> Problem:
+    gama = 1.4
+    delta = 1.0 / n
+    x = -0.5 + delta * np.arange(n)
+    y = -0.5 + delta * np.arange(n)
+    dl, sg = 0.025, 0.05
+    th = _axis(lambda v: math.tanh((v + 0.25) / dl), y) - _axis(lambda v: math.tanh((v - 0.25) / dl), y)
+    rho_y = 1.0 + 0.5 * th
+    u_y = -0.5 + 0.5 * th
+    g_y = _axis(lambda v: math.exp(-((v - 0.25) / sg) ** 2), y) + _axis(lambda v: math.exp(-((v + 0.25) / sg) ** 2), y)
+    s_x = _axis(lambda v: math.sin(2.0 * math.pi * mode * (v + 0.5)), x)
+    rho = np.broadcast_to(rho_y[:, None], (n, n)).copy()
+    u = np.broadcast_to(u_y[:, None], (n, n)).copy()
+    v = amplitude * s_x[None, :] * g_y[:, None]
+    p = 2.5
+    q = np.empty((4, 1, n, n))
+    q[0, 0] = rho
+    q[1, 0] = rho * u
+    q[2, 0] = rho * v
+    q[3, 0] = p / (gama - 1.0) + 0.5 * rho * (u * u + v * v)
+    desc = nat.ProblemDesc(
+        ndim=2, n=[n, n, 1], delta=[delta, delta, 1.0], conv_scheme=nat.CONV_WENO_Z, order=5,
+        averaging=nat.AVG_ROE, time_scheme=nat.TIME_RKLS3, precision=precision,
+        bc=[[nat.BC_PERIODIC, nat.BC_PERIODIC], [nat.BC_INVISCID_WALL, nat.BC_INVISCID_WALL]],
+        gama=gama, dt=dt if dt is not None else 0.2 / n)
+    return Problem(f"kh_{n}", desc, q, steps)
+
+
+def kh_ensemble(count: int = 64, seed: int = 20261203, sizes=(128, 256, 512, 1024), steps: int = 500,
+                precision: int = nat.PREC_MIXED):
+    """Seeded ensemble of Kelvin-Helmholtz instances of uneven sizes."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    probs = []
+    for i in range(count):
+        n = int(rng.choice(np.array(sizes)))
+        amp = float(rng.uniform(0.01, 0.05))
+        mode = int(rng.integers(1, 5))
+        pr = kelvin_helmholtz(n=n, amplitude=amp, mode=mode, steps=steps, precision=precision)
+        pr.name = f"kh{i}_{n}"
+        probs.append(pr)
+    return probs
+
+
+# ---------------------------------------------------------------------------
+# 2-D four-quadrant Riemann problem (serving stream)
+# ---------------------------------------------------------------------------
+def riemann_2d(n: int = 256, states=None, steps: int = 100, precision: int = nat.PREC_FP64,
+               dt: Optional[float] = None) -> Problem:
+    gama = 1.4
+    if states is None:
+        # (rho, u, v, p) for quadrants (x>.5,y>.5), (x<.5,y>.5), (x<.5,y<.5), (x>.5,y<.5)
+        states = [(1.5, 0.0, 0.0, 1.5), (0.5323, 1.206, 0.0, 0.3), (0.138, 1.206, 1.206, 0.029),
+                  (0.5323, 0.0, 1.206, 0.3)]
+    delta = 1.0 / (n - 1)
+    x = delta * np.arange(n)
+    X = x[None, :]
+    Y = x[:, None]
+    q = np.empty((4, 1, n, n))
+    rho = np.empty((n, n)); u = np.empty((n, n)); v = np.empty((n, n)); p = np.empty((n, n))
+    masks = [(X >= 0.5) & (Y >= 0.5), (X < 0.5) & (Y >= 0.5), (X < 0.5) & (Y < 0.5), (X >= 0.5) & (Y < 0.5)]
+    for m, (r_, u_, v_, p_) in zip(masks, states):
+        mm = np.broadcast_to(m, (n, n))
+        rho[mm] = r_; u[mm] = u_; v[mm] = v_; p[mm] = p_
+    q[0, 0] = rho
+    q[1, 0] = rho * u
+    q[2, 0] = rho * v
+    q[3, 0] = p / (gama - 1.0) + 0.5 * rho * (u * u + v * v)
+    if dt is None:
+        smax = max(abs(s[1]) + abs(s[2]) + 2 * math.sqrt(gama * s[3] / s[0]) for s in states)
+        dt = 0.25 * delta / smax
+    desc = nat.ProblemDesc(
+        ndim=2, n=[n, n, 1], delta=[delta, delta, 1.0], conv_scheme=nat.CONV_WENO_Z, order=5,
+        averaging=nat.AVG_ROE, time_scheme=nat.TIME_RKLS3, precision=precision,
+        bc=[[nat.BC_ZERO_GRADIENT] * 2] * 2, gama=gama, dt=dt)
+    return Problem(f"riemann_{n}", desc, q, steps)

@@ -1,0 +1,1035 @@
+// osb_oracle.cpp -- CPU restatement of OpenSBLI's explicit-solver hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the oracle: tests/, the smoke entry
+// and the benchmark's CPU-baseline leg load it (libosb_oracle.so) to check and
+// time the HIP library.  The product (libosb_hip.so) never links or calls it.
+//
+// It restates, array by array, the kernel sequence the reference generates
+// for one block (opensbli/code_generation/algorithm/algorithm.py:387-502;
+// exemplar apps/performance_benchmark/TENO6/opensbli.cpp:216-430):
+//   constituent relations on halo-extended ranges  -> u_i, p, a, T, mu arrays
+//   characteristic reconstruction per direction     -> face-flux work arrays wk
+//   residual kernel  R = -sum_d (wk_d(i) - wk_d(i-1)) / Delta_d
+//   or central split-form convective residual
+//   stored velocity gradients + viscous kernel      R += ...
+//   RK update, then boundary conditions
+// Every formula cites the reference lines it follows.  Arrays carry 5 halo
+// points per side like the reference's ops_dat declarations
+// (opensbli/schemes/spatial/scheme.py:48-61).
+//
+// Build: see oracle/Makefile (g++ -O3 -fopenmp -ffp-contract=off).
+
+#include "../include/osb.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include <omp.h>
+
+namespace {
+
+constexpr int HALO = 5;          // ops_dat halo depth in the reference
+constexpr double INV_SQRT2 = 0.707106781186547;  // sqrt(2.0) printed to 15 s.f. (euler_eigensystem.py:109,139)
+
+std::string g_create_error;
+
+struct Range { int lo[3]; int hi[3]; };
+
+}  // namespace
+
+struct osb_ctx {
+    osb_problem_desc d;
+    int ndim = 0, nv = 0;
+    int n[3] = {1, 1, 1};
+    int P[3] = {1, 1, 1};
+    int off[3] = {0, 0, 0};
+    size_t Ptot = 0;
+    int depth = 0;                 // boundary halo depth (symmetric max of scheme halos)
+    bool shock = false, central = false, viscous = false;
+
+    std::vector<double> q, rkold, R;
+    std::vector<double> u, p, a, T, mu;   // constituent relations
+    std::vector<double> wk;               // face fluxes, nv per direction
+    std::vector<double> g;                // stored velocity gradients g[i*ndim+j] = d u_i / d x_j
+    double time = 0.0;
+    int64_t iter = 0;
+    double last_dt = 0.0;
+    std::string err;
+
+    size_t I(int i, int j, int k) const {
+        return (size_t)(i + off[0]) + (size_t)P[0] * ((size_t)(j + off[1]) + (size_t)P[1] * (size_t)(k + off[2]));
+    }
+    double* Q(int v) { return q.data() + (size_t)v * Ptot; }
+    size_t stride(int dir) const { return dir == 0 ? 1 : (dir == 1 ? (size_t)P[0] : (size_t)P[0] * P[1]); }
+};
+
+namespace {
+
+int fail(osb_ctx* c, int code, const std::string& m) {
+    if (c) c->err = m; else g_create_error = m;
+    return code;
+}
+
+Range make_range(const osb_ctx* c, int ext) {
+    Range r;
+    for (int d = 0; d < 3; ++d) {
+        if (d < c->ndim) { r.lo[d] = -ext; r.hi[d] = c->n[d] + ext; }
+        else { r.lo[d] = 0; r.hi[d] = 1; }
+    }
+    return r;
+}
+
+template <class F>
+void for_range(const Range& r, F f) {
+    const int nk = r.hi[2] - r.lo[2], nj = r.hi[1] - r.lo[1];
+    const long total = (long)nk * nj;
+#pragma omp parallel for schedule(static)
+    for (long t = 0; t < total; ++t) {
+        const int k = r.lo[2] + (int)(t / nj);
+        const int j = r.lo[1] + (int)(t % nj);
+        for (int i = r.lo[0]; i < r.hi[0]; ++i) f(i, j, k);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Constituent relations (apps/compressible_taylor_green_vortex/compressible_TGV.py:78-83;
+// generated TENO6/opensbliblock00_kernels.h:48-91)
+// ---------------------------------------------------------------------------
+void constituent_relations(osb_ctx* c, const Range& r, bool need_T_mu) {
+    const int nd = c->ndim;
+    const double gama = c->d.gama, minf = c->d.minf;
+    for_range(r, [&](int i, int j, int k) {
+        const size_t x = c->I(i, j, k);
+        const double rho = c->q[x];
+        double uu[3] = {0, 0, 0};
+        for (int dd = 0; dd < nd; ++dd) {
+            uu[dd] = c->q[(size_t)(1 + dd) * c->Ptot + x] / rho;   // u_i = rhou_i/rho
+            c->u[(size_t)dd * c->Ptot + x] = uu[dd];
+        }
+        const double rhoE = c->q[(size_t)(nd + 1) * c->Ptot + x];
+        // p = (gama-1)*(rhoE - (1/2)*rho*u_i*u_i)
+        double ke = 0.0;
+        for (int dd = 0; dd < nd; ++dd) ke -= 0.5 * (uu[dd] * uu[dd]) * rho;
+        const double pp = (gama - 1.0) * (ke + rhoE);
+        c->p[x] = pp;
+        c->a[x] = std::sqrt(gama * pp / rho);                       // a = (gama*p/rho)**0.5
+        if (need_T_mu) {
+            const double TT = (minf * minf) * gama * pp / rho;       // T = p*gama*Minf^2/rho
+            c->T[x] = TT;
+            c->mu[x] = 1.4042 * TT * std::sqrt(TT) / (0.40417 + TT); // T**1.5*1.4042/(T+0.40417)
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Euler eigensystems (opensbli/physical_models/euler_eigensystem.py:53-165),
+// evaluated on averaged quantities.  Direction enters through the unit normal
+// k = e_dir (k_t = k / |k| = k).  Full dense matrices; zero entries stay zero.
+// ---------------------------------------------------------------------------
+struct Eig {
+    double L[5][5];
+    double Rm[5][5];
+};
+
+void eigensystem(int nd, int dir, double gamma_m1, double rho, const double* uu, double a, Eig& e) {
+    std::memset(&e, 0, sizeof(e));
+    if (nd == 1) {
+        // H = a**2/(gama-1) + u0**2/2 substituted (euler_eigensystem.py:75-76)
+        const double u0 = uu[0];
+        const double H = a * a / gamma_m1 + u0 * u0 / 2.0;
+        const double D = 2.0 * H - u0 * u0;
+        // REV = [[1,1,1],[u0-a,u0,u0+a],[H-u0*a,u0**2/2,H+u0*a]]   (:89)
+        e.Rm[0][0] = 1; e.Rm[0][1] = 1; e.Rm[0][2] = 1;
+        e.Rm[1][0] = u0 - a; e.Rm[1][1] = u0; e.Rm[1][2] = u0 + a;
+        e.Rm[2][0] = H - u0 * a; e.Rm[2][1] = u0 * u0 / 2.0; e.Rm[2][2] = H + u0 * a;
+        // LEV (:90)
+        e.L[0][0] = u0 * (2 * H + a * u0 - u0 * u0) / (2 * a * D);
+        e.L[0][1] = (-H - a * u0 + u0 * u0 / 2) / (a * D);
+        e.L[0][2] = 1.0 / D;
+        e.L[1][0] = 2 * (H - u0 * u0) / D;
+        e.L[1][1] = 2 * u0 / D;
+        e.L[1][2] = 2.0 / (-2 * H + u0 * u0);
+        e.L[2][0] = u0 * (-2 * H + a * u0 + u0 * u0) / (2 * a * D);
+        e.L[2][1] = (H - a * u0 - u0 * u0 / 2) / (a * D);
+        e.L[2][2] = 1.0 / D;
+        return;
+    }
+    double k[3] = {0, 0, 0};
+    k[dir] = 1.0;
+    double q2 = 0.0;
+    for (int dd = 0; dd < nd; ++dd) q2 += uu[dd] * uu[dd];
+    const double phi_sq = gamma_m1 * (q2 / 2.0);
+    double theta = 0.0;
+    for (int dd = 0; dd < nd; ++dd) theta += k[dd] * uu[dd];
+    const double alpha = rho * INV_SQRT2 / a;         // rho/(a*sqrt(2.0))
+    const double bta = INV_SQRT2 / (rho * a);         // 1/(rho*a*sqrt(2.0))
+    const double a2 = a * a;
+    if (nd == 2) {
+        const double u0 = uu[0], u1 = uu[1], k0 = k[0], k1 = k[1];
+        // REV (:121)
+        e.Rm[0][0] = 1; e.Rm[0][1] = 0; e.Rm[0][2] = alpha; e.Rm[0][3] = alpha;
+        e.Rm[1][0] = u0; e.Rm[1][1] = k1 * rho; e.Rm[1][2] = alpha * (u0 + k0 * a); e.Rm[1][3] = alpha * (u0 - k0 * a);
+        e.Rm[2][0] = u1; e.Rm[2][1] = -k0 * rho; e.Rm[2][2] = alpha * (u1 + k1 * a); e.Rm[2][3] = alpha * (u1 - k1 * a);
+        e.Rm[3][0] = phi_sq / gamma_m1; e.Rm[3][1] = rho * (k1 * u0 - k0 * u1);
+        e.Rm[3][2] = alpha * ((phi_sq + a2) / gamma_m1 + a * theta);
+        e.Rm[3][3] = alpha * ((phi_sq + a2) / gamma_m1 - a * theta);
+        // LEV (:122)
+        e.L[0][0] = 1 - phi_sq / a2; e.L[0][1] = gamma_m1 * u0 / a2; e.L[0][2] = gamma_m1 * u1 / a2; e.L[0][3] = -gamma_m1 / a2;
+        e.L[1][0] = -(k1 * u0 - k0 * u1) / rho; e.L[1][1] = k1 / rho; e.L[1][2] = -k0 / rho; e.L[1][3] = 0;
+        e.L[2][0] = bta * (phi_sq - a * theta); e.L[2][1] = bta * (k0 * a - gamma_m1 * u0);
+        e.L[2][2] = bta * (k1 * a - gamma_m1 * u1); e.L[2][3] = bta * gamma_m1;
+        e.L[3][0] = bta * (phi_sq + a * theta); e.L[3][1] = -bta * (k0 * a + gamma_m1 * u0);
+        e.L[3][2] = -bta * (k1 * a + gamma_m1 * u1); e.L[3][3] = bta * gamma_m1;
+        return;
+    }
+    // ndim == 3 (:150-152)
+    const double u0 = uu[0], u1 = uu[1], u2 = uu[2], k0 = k[0], k1 = k[1], k2 = k[2];
+    e.Rm[0][0] = k0; e.Rm[0][1] = k1; e.Rm[0][2] = k2; e.Rm[0][3] = alpha; e.Rm[0][4] = alpha;
+    e.Rm[1][0] = k0 * u0; e.Rm[1][1] = k1 * u0 - k2 * rho; e.Rm[1][2] = k2 * u0 + k1 * rho;
+    e.Rm[1][3] = alpha * (u0 + k0 * a); e.Rm[1][4] = alpha * (u0 - k0 * a);
+    e.Rm[2][0] = k0 * u1 + k2 * rho; e.Rm[2][1] = k1 * u1; e.Rm[2][2] = k2 * u1 - k0 * rho;
+    e.Rm[2][3] = alpha * (u1 + k1 * a); e.Rm[2][4] = alpha * (u1 - k1 * a);
+    e.Rm[3][0] = k0 * u2 - k1 * rho; e.Rm[3][1] = k1 * u2 + k0 * rho; e.Rm[3][2] = k2 * u2;
+    e.Rm[3][3] = alpha * (u2 + k2 * a); e.Rm[3][4] = alpha * (u2 - k2 * a);
+    e.Rm[4][0] = k0 * phi_sq / gamma_m1 + rho * (k2 * u1 - k1 * u2);
+    e.Rm[4][1] = k1 * phi_sq / gamma_m1 + rho * (k0 * u2 - k2 * u0);
+    e.Rm[4][2] = k2 * phi_sq / gamma_m1 + rho * (k1 * u0 - k0 * u1);
+    e.Rm[4][3] = alpha * ((phi_sq + a2) / gamma_m1 + theta * a);
+    e.Rm[4][4] = alpha * ((phi_sq + a2) / gamma_m1 - theta * a);
+
+    e.L[0][0] = k0 * (1 - phi_sq / a2) - (k2 * u1 - k1 * u2) / rho;
+    e.L[0][1] = k0 * gamma_m1 * u0 / a2;
+    e.L[0][2] = k0 * gamma_m1 * u1 / a2 + k2 / rho;
+    e.L[0][3] = k0 * gamma_m1 * u2 / a2 - k1 / rho;
+    e.L[0][4] = -k0 * gamma_m1 / a2;
+    e.L[1][0] = k1 * (1 - phi_sq / a2) - (k0 * u2 - k2 * u0) / rho;
+    e.L[1][1] = k1 * gamma_m1 * u0 / a2 - k2 / rho;
+    e.L[1][2] = k1 * gamma_m1 * u1 / a2;
+    e.L[1][3] = k1 * gamma_m1 * u2 / a2 + k0 / rho;
+    e.L[1][4] = -k1 * gamma_m1 / a2;
+    e.L[2][0] = k2 * (1 - phi_sq / a2) - (k1 * u0 - k0 * u1) / rho;
+    e.L[2][1] = k2 * gamma_m1 * u0 / a2 + k1 / rho;
+    e.L[2][2] = k2 * gamma_m1 * u1 / a2 - k0 / rho;
+    e.L[2][3] = k2 * gamma_m1 * u2 / a2;
+    e.L[2][4] = -k2 * gamma_m1 / a2;
+    e.L[3][0] = bta * (phi_sq - theta * a);
+    e.L[3][1] = -bta * (gamma_m1 * u0 - k0 * a);
+    e.L[3][2] = -bta * (gamma_m1 * u1 - k1 * a);
+    e.L[3][3] = -bta * (gamma_m1 * u2 - k2 * a);
+    e.L[3][4] = bta * gamma_m1;
+    e.L[4][0] = bta * (phi_sq + theta * a);
+    e.L[4][1] = -bta * (gamma_m1 * u0 + k0 * a);
+    e.L[4][2] = -bta * (gamma_m1 * u1 + k1 * a);
+    e.L[4][3] = -bta * (gamma_m1 * u2 + k2 * a);
+    e.L[4][4] = bta * gamma_m1;
+}
+
+// ---------------------------------------------------------------------------
+// Reconstructions.  f[0..5] hold the split flux at offsets -2..3 from point i;
+// the interface is i+1/2.  Right (f+) uses offsets -2..2, left (f-) -1..3
+// (opensbli/schemes/spatial/weno.py:123-136; teno.py:95-107).
+// ---------------------------------------------------------------------------
+
+// Jiang-Shu smoothness of a 3-point stencil whose points are (a, b, c) in
+// ascending x, measured on the cell of the stencil's first point for
+// 'first' (3a-4b+c) or last point for 'last' (a-4b+3c), or centred (a-c).
+inline double js_first(double a, double b, double c) {
+    const double s1 = a - 2 * b + c, s2 = 3 * a - 4 * b + c;
+    return (13.0 / 12.0) * s1 * s1 + (1.0 / 4.0) * s2 * s2;
+}
+inline double js_centre(double a, double b, double c) {
+    const double s1 = a - 2 * b + c, s2 = a - c;
+    return (13.0 / 12.0) * s1 * s1 + (1.0 / 4.0) * s2 * s2;
+}
+inline double js_last(double a, double b, double c) {
+    const double s1 = a - 2 * b + c, s2 = a - 4 * b + 3 * c;
+    return (13.0 / 12.0) * s1 * s1 + (1.0 / 4.0) * s2 * s2;
+}
+
+// Shu (1997) eq. (2.21) ENO coefficients c_rj (weno.py:71-99), evaluated in
+// exact integer arithmetic for k = 3.  'dshift' is 1 for the right (side +1)
+// and 0 for the left (side -1) reconstruction.
+struct Frac { long num, den; };
+Frac frac_add(Frac x, Frac y) {
+    long n = x.num * y.den + y.num * x.den, dd = x.den * y.den;
+    long g = std::abs(n), h = std::abs(dd);
+    while (h) { long t = g % h; g = h; h = t; }
+    if (g == 0) g = 1;
+    n /= g; dd /= g;
+    if (dd < 0) { n = -n; dd = -dd; }
+    return {n, dd};
+}
+Frac eno_coefficient(int k, int r, int j, int dshift) {
+    Frac sum{0, 1};
+    for (int m = j + 1; m <= k; ++m) {
+        long top_sum = 0, bottom = 1;
+        for (int l = 0; l <= k; ++l) {
+            if (l == m) continue;
+            long prod = 1;
+            for (int qq = 0; qq <= k; ++qq) {
+                if (qq == m || qq == l) continue;
+                prod *= (r - qq + dshift);
+            }
+            bottom *= (m - l);
+            top_sum += prod;
+        }
+        sum = frac_add(sum, Frac{top_sum, bottom});
+    }
+    return sum;
+}
+
+struct WenoCoeffs {
+    double c[2][3][3];   // [side 0=right,1=left][r][j]
+    double dopt[2][3];
+};
+WenoCoeffs make_weno5() {
+    WenoCoeffs w;
+    for (int s = 0; s < 2; ++s)
+        for (int r = 0; r < 3; ++r)
+            for (int j = 0; j < 3; ++j) {
+                Frac f = eno_coefficient(3, r, j, s == 0 ? 1 : 0);
+                w.c[s][r][j] = (double)f.num / (double)f.den;
+            }
+    // Optimal weights (weno.py:101-121).  Right: r=0 [0,1,2] 3/10, r=1 [-1,0,1] 6/10,
+    // r=2 [-2,-1,0] 1/10; the left side mirrors them.
+    w.dopt[0][0] = 3.0 / 10.0; w.dopt[0][1] = 6.0 / 10.0; w.dopt[0][2] = 1.0 / 10.0;
+    w.dopt[1][0] = 1.0 / 10.0; w.dopt[1][1] = 6.0 / 10.0; w.dopt[1][2] = 3.0 / 10.0;
+    return w;
+}
+const WenoCoeffs W5 = make_weno5();
+
+// WENO-Z (weno.py:290-345): tau5 = |b0 - b2|, alpha_r = d_r*(1 + (tau/(eps+b_r))**2).
+double wenoz_right(const double* f, double eps) {
+    // points: f[0]=i-2 ... f[4]=i+2 ; stencils r=0 -> (i,i+1,i+2), r=1 -> (i-1,i,i+1), r=2 -> (i-2,i-1,i)
+    const double b0 = js_first(f[2], f[3], f[4]);
+    const double b1 = js_centre(f[1], f[2], f[3]);
+    const double b2 = js_last(f[0], f[1], f[2]);
+    const double tau = std::fabs(b0 - b2);
+    const double* d = W5.dopt[0];
+    const double a0 = d[0] + d[0] * tau * tau / ((eps + b0) * (eps + b0));
+    const double a1 = d[1] + d[1] * tau * tau / ((eps + b1) * (eps + b1));
+    const double a2 = d[2] + d[2] * tau * tau / ((eps + b2) * (eps + b2));
+    const double inv = 1.0 / (a0 + a1 + a2);
+    const double e0 = W5.c[0][0][0] * f[2] + W5.c[0][0][1] * f[3] + W5.c[0][0][2] * f[4];
+    const double e1 = W5.c[0][1][0] * f[1] + W5.c[0][1][1] * f[2] + W5.c[0][1][2] * f[3];
+    const double e2 = W5.c[0][2][0] * f[0] + W5.c[0][2][1] * f[1] + W5.c[0][2][2] * f[2];
+    return (a0 * inv) * e0 + (a1 * inv) * e1 + (a2 * inv) * e2;
+}
+double wenoz_left(const double* f, double eps) {
+    // points: f[1]=i-1 ... f[5]=i+3 ; r=0 -> (i+1,i+2,i+3), r=1 -> (i,i+1,i+2), r=2 -> (i-1,i,i+1)
+    const double b0 = js_first(f[3], f[4], f[5]);
+    const double b1 = js_centre(f[2], f[3], f[4]);
+    const double b2 = js_last(f[1], f[2], f[3]);
+    const double tau = std::fabs(b0 - b2);
+    const double* d = W5.dopt[1];
+    const double a0 = d[0] + d[0] * tau * tau / ((eps + b0) * (eps + b0));
+    const double a1 = d[1] + d[1] * tau * tau / ((eps + b1) * (eps + b1));
+    const double a2 = d[2] + d[2] * tau * tau / ((eps + b2) * (eps + b2));
+    const double inv = 1.0 / (a0 + a1 + a2);
+    const double e0 = W5.c[1][0][0] * f[3] + W5.c[1][0][1] * f[4] + W5.c[1][0][2] * f[5];
+    const double e1 = W5.c[1][1][0] * f[2] + W5.c[1][1][1] * f[3] + W5.c[1][1][2] * f[4];
+    const double e2 = W5.c[1][2][0] * f[1] + W5.c[1][2][1] * f[2] + W5.c[1][2][2] * f[3];
+    return (a0 * inv) * e0 + (a1 * inv) * e1 + (a2 * inv) * e2;
+}
+
+// TENO5 / TENO6 (teno.py:59-217, 419-477).
+struct TenoParams { int order; double eps, ct; };
+inline double pow6(double x) { return x * x * x * x * x * x; }
+
+double teno_side(const double* f, int side, const TenoParams& tp) {
+    // f indexed by offset: F(o) = f[o+2], o in -2..3
+    auto F = [&](int o) { return f[o + 2]; };
+    double beta[4], eno[4], dopt[4];
+    int ns = tp.order == 5 ? 3 : 4;
+    if (tp.order == 5) { dopt[0] = 11.0 / 20.0; dopt[1] = 4.0 / 10.0; dopt[2] = 1.0 / 20.0; }
+    else { dopt[0] = 231.0 / 500.0; dopt[1] = 3.0 / 10.0; dopt[2] = 27.0 / 500.0; dopt[3] = 23.0 / 125.0; }
+    if (side == +1) {
+        // stencils [-1,0,1], [0,1,2], [-2,-1,0], [0,1,2,3]
+        beta[0] = (1.0 / 4.0) * (F(-1) - F(1)) * (F(-1) - F(1)) + (13.0 / 12.0) * (F(-1) - 2 * F(0) + F(1)) * (F(-1) - 2 * F(0) + F(1));
+        beta[1] = (1.0 / 4.0) * (3 * F(0) - 4 * F(1) + F(2)) * (3 * F(0) - 4 * F(1) + F(2)) + (13.0 / 12.0) * (F(0) - 2 * F(1) + F(2)) * (F(0) - 2 * F(1) + F(2));
+        beta[2] = (1.0 / 4.0) * (F(-2) - 4 * F(-1) + 3 * F(0)) * (F(-2) - 4 * F(-1) + 3 * F(0)) + (13.0 / 12.0) * (F(-2) - 2 * F(-1) + F(0)) * (F(-2) - 2 * F(-1) + F(0));
+        eno[0] = (-1.0 / 6.0) * F(-1) + (5.0 / 6.0) * F(0) + (2.0 / 6.0) * F(1);
+        eno[1] = (2.0 / 6.0) * F(0) + (5.0 / 6.0) * F(1) + (-1.0 / 6.0) * F(2);
+        eno[2] = (2.0 / 6.0) * F(-2) + (-7.0 / 6.0) * F(-1) + (11.0 / 6.0) * F(0);
+        if (ns == 4) {
+            const double s1 = -11 * F(0) + 18 * F(1) - 9 * F(2) + 2 * F(3);
+            const double s2 = 2 * F(0) - 5 * F(1) + 4 * F(2) - F(3);
+            const double s3 = -F(0) + 3 * F(1) - 3 * F(2) + F(3);
+            // the right-side beta_3 carries the 781/720 term unsquared (teno.py:168-169)
+            beta[3] = (1.0 / 36.0) * s1 * s1 + (13.0 / 12.0) * s2 * s2 + (781.0 / 720.0) * s3;
+            eno[3] = (3.0 / 12.0) * F(0) + (13.0 / 12.0) * F(1) + (-5.0 / 12.0) * F(2) + (1.0 / 12.0) * F(3);
+        }
+    } else {
+        // stencils [0,1,2], [-1,0,1], [1,2,3], [-2,-1,0,1]
+        beta[0] = (1.0 / 4.0) * (F(0) - F(2)) * (F(0) - F(2)) + (13.0 / 12.0) * (F(0) - 2 * F(1) + F(2)) * (F(0) - 2 * F(1) + F(2));
+        beta[1] = (1.0 / 4.0) * (F(-1) - 4 * F(0) + 3 * F(1)) * (F(-1) - 4 * F(0) + 3 * F(1)) + (13.0 / 12.0) * (F(-1) - 2 * F(0) + F(1)) * (F(-1) - 2 * F(0) + F(1));
+        beta[2] = (1.0 / 4.0) * (3 * F(1) - 4 * F(2) + F(3)) * (3 * F(1) - 4 * F(2) + F(3)) + (13.0 / 12.0) * (F(1) - 2 * F(2) + F(3)) * (F(1) - 2 * F(2) + F(3));
+        eno[0] = (2.0 / 6.0) * F(0) + (5.0 / 6.0) * F(1) + (-1.0 / 6.0) * F(2);
+        eno[1] = (-1.0 / 6.0) * F(-1) + (5.0 / 6.0) * F(0) + (2.0 / 6.0) * F(1);
+        eno[2] = (11.0 / 6.0) * F(1) + (-7.0 / 6.0) * F(2) + (1.0 / 3.0) * F(3);
+        if (ns == 4) {
+            const double s1 = -2 * F(-2) + 9 * F(-1) - 18 * F(0) + 11 * F(1);
+            const double s2 = -1 * F(-2) + 4 * F(-1) - 5 * F(0) + 2 * F(1);
+            const double s3 = -1 * F(-2) + 3 * F(-1) - 3 * F(0) + F(1);
+            beta[3] = (1.0 / 36.0) * s1 * s1 + (13.0 / 12.0) * s2 * s2 + (781.0 / 720.0) * s3 * s3;
+            eno[3] = (1.0 / 12.0) * F(-2) + (-5.0 / 12.0) * F(-1) + (13.0 / 12.0) * F(0) + (3.0 / 12.0) * F(1);
+        }
+    }
+    double tau;
+    if (ns == 3) tau = std::fabs(beta[0] - beta[2]);                                         // Teno5 (:210)
+    else tau = std::fabs(beta[3] - (1.0 / 6.0) * (beta[0] + beta[2] - 4 * beta[1]));           // Teno6 (:228)
+    double alpha[4], asum = 0.0;
+    for (int r = 0; r < ns; ++r) {
+        const double inv_beta = 1.0 / (tp.eps + beta[r]);
+        alpha[r] = pow6(1.0 + tau * inv_beta);                                                 // (C + tau/beta)**6
+        asum += alpha[r];
+    }
+    const double inv_asum = 1.0 / asum;
+    double delta[4], wsum = 0.0;
+    for (int r = 0; r < ns; ++r) {
+        delta[r] = (alpha[r] * inv_asum < tp.ct) ? 0.0 : 1.0;                                 // cut-off (:457-477)
+        wsum += dopt[r] * delta[r];
+    }
+    const double inv_wsum = 1.0 / wsum;
+    double rec = 0.0;
+    for (int r = 0; r < ns; ++r) rec += dopt[r] * delta[r] * inv_wsum * eno[r];
+    return rec;
+}
+
+// ---------------------------------------------------------------------------
+// Characteristic flux reconstruction for one direction
+// (shock_capturing.py:786-891, 957-1028; weno.py:505-551; teno.py:510-559).
+// Writes the interface flux at i+1/2 into wk_dir[v] at index i over
+// [-1, N+1) along dir and [0, N) across (TENO6/opensbli.cpp:263-316).
+// ---------------------------------------------------------------------------
+void reconstruct_direction(osb_ctx* c, int dir) {
+    const int nd = c->ndim, nv = c->nv;
+    const double gama = c->d.gama, gamma_m1 = gama - 1.0;
+    const double sfc = c->d.shock_filter_control;
+    Range r = make_range(c, 0);
+    r.lo[dir] = -1; r.hi[dir] = c->n[dir] + 1;
+    const size_t st = c->stride(dir);
+    double* wk = c->wk.data() + (size_t)dir * nv * c->Ptot;
+    const bool teno = c->d.conv_scheme == OSB_CONV_TENO;
+    TenoParams tp{c->d.order, c->d.teno_eps, c->d.teno_ct};
+    const double weps = c->d.weno_eps;
+    for_range(r, [&](int i, int j, int k) {
+        const size_t x0 = c->I(i, j, k);
+        // point data at offsets -2..3
+        double qv[6][5], F[6][5], ud[6], aa[6];
+        for (int s = 0; s < 6; ++s) {
+            const size_t x = x0 + (size_t)((long)(s - 2) * (long)st);
+            for (int v = 0; v < nv; ++v) qv[s][v] = c->q[(size_t)v * c->Ptot + x];
+            const double pp = c->p[x];
+            const double u_dir = c->u[(size_t)dir * c->Ptot + x];
+            ud[s] = u_dir;
+            aa[s] = c->a[x];
+            // flux vector of Conservative(rhou_j), Conservative(rhou_i*u_j + KD*p), Conservative((p+rhoE)*u_j)
+            F[s][0] = qv[s][1 + dir];
+            for (int m = 0; m < nd; ++m) F[s][1 + m] = qv[s][1 + m] * u_dir + (m == dir ? pp : 0.0);
+            F[s][nd + 1] = (pp + qv[s][nd + 1]) * u_dir;
+        }
+        // averaged state between offsets 0 and 1 (averaging.py:31-124)
+        const size_t xa = x0, xb = x0 + st;
+        double rho_av, u_av[3] = {0, 0, 0}, a_av;
+        if (c->d.averaging == OSB_AVG_ROE) {
+            const double rl = c->q[xa], rr = c->q[xb];
+            const double srl = std::sqrt(rl), srr = std::sqrt(rr);
+            rho_av = std::sqrt(rl * rr);
+            const double inv = 1.0 / (srl + srr);
+            double ke = 0.0;
+            for (int m = 0; m < nd; ++m) {
+                u_av[m] = (srl * c->u[(size_t)m * c->Ptot + xa] + srr * c->u[(size_t)m * c->Ptot + xb]) * inv;
+                ke += u_av[m] * u_av[m];
+            }
+            const double Hroe = ((c->p[xa] + c->q[(size_t)(nd + 1) * c->Ptot + xa]) / srl +
+                                 (c->p[xb] + c->q[(size_t)(nd + 1) * c->Ptot + xb]) / srr) * inv;
+            a_av = std::sqrt(gamma_m1 * (Hroe - 0.5 * ke));
+        } else {
+            rho_av = 0.5 * (c->q[xa] + c->q[xb]);
+            for (int m = 0; m < nd; ++m) u_av[m] = 0.5 * (c->u[(size_t)m * c->Ptot + xa] + c->u[(size_t)m * c->Ptot + xb]);
+            a_av = 0.5 * (c->a[xa] + c->a[xb]);
+        }
+        Eig e;
+        eigensystem(nd, dir, gamma_m1, rho_av, u_av, a_av, e);
+        // local LF wave speeds: max over offsets {0, 1} of |eigenvalue| (shock_capturing.py:1000-1028)
+        double lam[5];
+        if (nd == 1) {
+            lam[0] = sfc * std::max(std::fabs(ud[2] - aa[2]), std::fabs(ud[3] - aa[3]));
+            lam[1] = sfc * std::max(std::fabs(ud[2]), std::fabs(ud[3]));
+            lam[2] = sfc * std::max(std::fabs(ud[2] + aa[2]), std::fabs(ud[3] + aa[3]));
+        } else {
+            const double l0 = sfc * std::max(std::fabs(ud[2]), std::fabs(ud[3]));
+            for (int m = 0; m < nd; ++m) lam[m] = l0;
+            lam[nd] = sfc * std::max(std::fabs(ud[2] + aa[2]), std::fabs(ud[3] + aa[3]));
+            lam[nd + 1] = sfc * std::max(std::fabs(ud[2] - aa[2]), std::fabs(ud[3] - aa[3]));
+        }
+        double recon[5];
+        for (int kk = 0; kk < nv; ++kk) {
+            double fp[6], fm[6];
+            for (int s = 0; s < 6; ++s) {
+                double cs = 0.0, cf = 0.0;
+                for (int m = 0; m < nv; ++m) { cs += e.L[kk][m] * qv[s][m]; cf += e.L[kk][m] * F[s][m]; }
+                fp[s] = 0.5 * (cf + lam[kk] * cs);
+                fm[s] = 0.5 * (cf - lam[kk] * cs);
+            }
+            if (teno) recon[kk] = teno_side(fp, +1, tp) + teno_side(fm, -1, tp);
+            else recon[kk] = wenoz_right(fp, weps) + wenoz_left(fm, weps);
+        }
+        for (int v = 0; v < nv; ++v) {
+            double s = 0.0;
+            for (int kk = 0; kk < nv; ++kk) s += e.Rm[v][kk] * recon[kk];
+            wk[(size_t)v * c->Ptot + x0] = s;
+        }
+    });
+}
+
+// residual R_v = -sum_d (wk_d(i) - wk_d(i-1)) * invDelta_d  (shock_capturing.py:23-36, opensblifunctions.py:577-591)
+void shock_residual(osb_ctx* c) {
+    const int nd = c->ndim, nv = c->nv;
+    Range r = make_range(c, 0);
+    for_range(r, [&](int i, int j, int k) {
+        const size_t x = c->I(i, j, k);
+        for (int v = 0; v < nv; ++v) {
+            double acc = 0.0;
+            for (int dir = 0; dir < nd; ++dir) {
+                const double* w = c->wk.data() + ((size_t)dir * nv + v) * c->Ptot;
+                acc -= (w[x] - w[x - c->stride(dir)]) * (1.0 / c->d.delta[dir]);
+            }
+            c->R[(size_t)v * c->Ptot + x] = acc;
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// 4th-order central derivatives (scheme.py:83-87; opensblifunctions.py:486-512):
+// d1 weights (1/12, -2/3, 0, 2/3, -1/12)/Delta, d2 weights (-1/12, 4/3, -5/2, 4/3, -1/12)/Delta^2
+// ---------------------------------------------------------------------------
+template <class G>
+inline double d1(G f, double invd) {
+    return ((1.0 / 12.0) * (f(-2) - f(2)) + (2.0 / 3.0) * (f(1) - f(-1))) * invd;
+}
+template <class G>
+inline double d2(G f, double inv2d) {
+    return ((-1.0 / 12.0) * (f(-2) + f(2)) + (4.0 / 3.0) * (f(-1) + f(1)) - (5.0 / 2.0) * f(0)) * inv2d;
+}
+
+// Central convective residual, written first (StoreSome.py:138-228; scheme.py:321-350).
+void central_convective(osb_ctx* c) {
+    const int nd = c->ndim;
+    const size_t Pt = c->Ptot;
+    const bool skew = c->d.conv_scheme == OSB_CONV_CENTRAL_SKEW;
+    Range r = make_range(c, 0);
+    const double* q = c->q.data();
+    for_range(r, [&](int i, int j, int k) {
+        const size_t x = c->I(i, j, k);
+        const double rho = q[x], rhoE = q[(size_t)(nd + 1) * Pt + x];
+        double Rm[3] = {0, 0, 0}, Rrho = 0, RE = 0;
+        double div_rhou = 0.0;            // sum_j D_j(rhou_j)
+        double sum_conv[3] = {0, 0, 0};   // per momentum component
+        double eterm = 0.0;
+        for (int jd = 0; jd < nd; ++jd) {
+            const size_t st = c->stride(jd);
+            const double invd = 1.0 / c->d.delta[jd];
+            auto at = [&](const double* arr, int s) { return arr[x + (long)s * (long)st]; };
+            const double* rhou_j = q + (size_t)(1 + jd) * Pt;
+            const double* u_j = c->u.data() + (size_t)jd * Pt;
+            const double* rhoEa = q + (size_t)(nd + 1) * Pt;
+            const double* pa = c->p.data();
+            if (!skew) {
+                // Feiereisen (split_forms.py:111-144)
+                const double d_rhouj = d1([&](int s) { return at(rhou_j, s); }, invd);
+                div_rhou += d_rhouj;
+                for (int m = 0; m < nd; ++m) {
+                    const double* rhou_m = q + (size_t)(1 + m) * Pt;
+                    const double d_prod = d1([&](int s) { return at(rhou_m, s) * at(u_j, s); }, invd);
+                    const double g_mj = c->g[(size_t)(m * nd + jd) * Pt + x];   // stored D_j u_m
+                    sum_conv[m] += d_prod + at(rhou_j, 0) * g_mj;
+                }
+                const double d_rhoEu = d1([&](int s) { return at(rhoEa, s) * at(u_j, s); }, invd);
+                const double d_E = d1([&](int s) { return at(rhoEa, s) / at(q, s); }, invd);
+                const double d_pu = d1([&](int s) { return at(pa, s) * at(u_j, s); }, invd);
+                eterm += -0.5 * (d_rhoEu + at(rhou_j, 0) * d_E) - d_pu;
+            } else {
+                // Skew(ab, x) = 1/2 (D(ab) + b D(a) + a D(b))  (parsing.py:96-112), tg_sym equations
+                const double* rhoa = q;
+                const double ujc = at(u_j, 0);
+                const double d_rhou = d1([&](int s) { return at(rhoa, s) * at(u_j, s); }, invd);
+                const double d_rho = d1([&](int s) { return at(rhoa, s); }, invd);
+                const double d_u = d1([&](int s) { return at(u_j, s); }, invd);
+                Rrho -= 0.5 * (d_rhou + ujc * d_rho + rho * d_u);
+                for (int m = 0; m < nd; ++m) {
+                    const double* rhou_m = q + (size_t)(1 + m) * Pt;
+                    const double d_prod = d1([&](int s) { return at(rhou_m, s) * at(u_j, s); }, invd);
+                    const double d_rm = d1([&](int s) { return at(rhou_m, s); }, invd);
+                    sum_conv[m] += 0.5 * (d_prod + ujc * d_rm + at(rhou_m, 0) * d_u);
+                }
+                const double d_Eu = d1([&](int s) { return at(rhoEa, s) * at(u_j, s); }, invd);
+                const double d_rE = d1([&](int s) { return at(rhoEa, s); }, invd);
+                const double d_pu = d1([&](int s) { return at(pa, s) * at(u_j, s); }, invd);
+                eterm += -0.5 * (d_Eu + ujc * d_rE + rhoE * d_u) - d_pu;
+            }
+        }
+        for (int m = 0; m < nd; ++m) {
+            const size_t st = c->stride(m);
+            const double invd = 1.0 / c->d.delta[m];
+            const double* pa = c->p.data();
+            const double dp = d1([&](int s) { return pa[x + (long)s * (long)st]; }, invd);
+            if (!skew) {
+                const double um = c->u[(size_t)m * Pt + x];
+                Rm[m] = -dp - 0.5 * (sum_conv[m] + um * div_rhou);
+            } else {
+                Rm[m] = -dp - sum_conv[m];
+            }
+        }
+        if (!skew) {
+            Rrho = -div_rhou;
+            RE = eterm - 0.5 * (rhoE / rho) * div_rhou;
+        } else {
+            RE = eterm;
+        }
+        c->R[x] = Rrho;
+        for (int m = 0; m < nd; ++m) c->R[(size_t)(1 + m) * Pt + x] = Rm[m];
+        c->R[(size_t)(nd + 1) * Pt + x] = RE;
+    });
+}
+
+// Stored first derivatives of u_i (StoreSome.py:33-45, 87-135), evaluated on
+// ranges extended by 2 across so the mixed derivatives can differentiate them
+// (TENO6/opensbli.cpp:340-384).
+void stored_gradients(osb_ctx* c) {
+    const int nd = c->ndim;
+    const size_t Pt = c->Ptot;
+    Range r = make_range(c, 2);
+    for (int jd = 0; jd < nd; ++jd) {
+        Range rr = r;
+        rr.lo[jd] = 0; rr.hi[jd] = c->n[jd];
+        const size_t st = c->stride(jd);
+        const double invd = 1.0 / c->d.delta[jd];
+        for (int m = 0; m < nd; ++m) {
+            const double* um = c->u.data() + (size_t)m * Pt;
+            double* gm = c->g.data() + (size_t)(m * nd + jd) * Pt;
+            for_range(rr, [&](int i, int j, int k) {
+                const size_t x = c->I(i, j, k);
+                gm[x] = d1([&](int s) { return um[x + (long)s * (long)st]; }, invd);
+            });
+        }
+    }
+}
+
+// Viscous and heat-conduction terms, added to the residual
+// (split_forms.py:383-392; generated TENO6/opensbliblock00_kernels.h:4568-4707).
+// Mixed second derivatives are D_b(D_a u) with a < b applied to the stored
+// first derivative; repeated-direction second derivatives use the d2 stencil.
+void viscous_terms(osb_ctx* c) {
+    const int nd = c->ndim;
+    const size_t Pt = c->Ptot;
+    const bool suth = c->d.viscous == OSB_VISC_SUTHERLAND;
+    const double invRe = 1.0 / c->d.re;
+    const double gama = c->d.gama, minf = c->d.minf, pr = c->d.pr;
+    const double cheat = 1.0 / ((gama - 1.0) * minf * minf * pr * c->d.re);
+    Range r = make_range(c, 0);
+    for_range(r, [&](int i, int j, int k) {
+        const size_t x = c->I(i, j, k);
+        double G[3][3] = {{0}};                 // G[m][jd] = D_jd u_m at x
+        for (int m = 0; m < nd; ++m)
+            for (int jd = 0; jd < nd; ++jd) G[m][jd] = c->g[(size_t)(m * nd + jd) * Pt + x];
+        double dmu[3] = {0, 0, 0}, dT[3] = {0, 0, 0}, lapT = 0.0;
+        double D2u[3][3] = {{0}};                // D2u[m][jd] = d2 u_m / d x_jd^2
+        for (int jd = 0; jd < nd; ++jd) {
+            const size_t st = c->stride(jd);
+            const double invd = 1.0 / c->d.delta[jd];
+            const double inv2d = 1.0 / (c->d.delta[jd] * c->d.delta[jd]);
+            auto at = [&](const double* arr, int s) { return arr[x + (long)s * (long)st]; };
+            if (suth) dmu[jd] = d1([&](int s) { return at(c->mu.data(), s); }, invd);
+            dT[jd] = d1([&](int s) { return at(c->T.data(), s); }, invd);
+            lapT += d2([&](int s) { return at(c->T.data(), s); }, inv2d);
+            for (int m = 0; m < nd; ++m) {
+                const double* um = c->u.data() + (size_t)m * Pt;
+                D2u[m][jd] = d2([&](int s) { return at(um, s); }, inv2d);
+            }
+        }
+        // mixed(m, a, b) = D_b(D_a u_m), a < b
+        auto mixed = [&](int m, int a, int b) {
+            if (a > b) std::swap(a, b);
+            const double* ga = c->g.data() + (size_t)(m * nd + a) * Pt;
+            const size_t st = c->stride(b);
+            return d1([&](int s) { return ga[x + (long)s * (long)st]; }, 1.0 / c->d.delta[b]);
+        };
+        double div = 0.0;
+        for (int m = 0; m < nd; ++m) div += G[m][m];
+        double S[3][3];
+        for (int m = 0; m < nd; ++m)
+            for (int jd = 0; jd < nd; ++jd)
+                S[m][jd] = G[m][jd] + G[jd][m] - (m == jd ? (2.0 / 3.0) * div : 0.0);
+        // L_m = d(S_mj)/dx_j = (4/3) d2u_m/dx_m^2 + sum_{j!=m} d2u_m/dx_j^2 + (1/3) sum_{j!=m} d2u_j/dx_m dx_j
+        double Lm[3] = {0, 0, 0};
+        for (int m = 0; m < nd; ++m) {
+            double s = (4.0 / 3.0) * D2u[m][m];
+            for (int jd = 0; jd < nd; ++jd) {
+                if (jd == m) continue;
+                s += D2u[m][jd] + (1.0 / 3.0) * mixed(jd, jd, m);
+            }
+            Lm[m] = s;
+        }
+        const double muv = suth ? c->mu[x] : 1.0;
+        double Eacc = 0.0;
+        for (int m = 0; m < nd; ++m) {
+            double mom = muv * Lm[m];
+            if (suth) for (int jd = 0; jd < nd; ++jd) mom += S[m][jd] * dmu[jd];
+            c->R[(size_t)(1 + m) * Pt + x] += invRe * mom;
+            const double um = c->u[(size_t)m * Pt + x];
+            double e = um * mom;                           // u_i d(tau_ij)/dx_j
+            for (int jd = 0; jd < nd; ++jd) e += muv * S[m][jd] * G[m][jd];   // tau_ij du_i/dx_j
+            Eacc += e;
+        }
+        double heat = muv * lapT;
+        if (suth) for (int jd = 0; jd < nd; ++jd) heat += dT[jd] * dmu[jd];
+        c->R[(size_t)(nd + 1) * Pt + x] += invRe * Eacc + cheat * heat;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Boundary conditions, applied direction by direction, side 0 then side 1,
+// over the halo-extended range of the other directions
+// (bc_core.py:128-202; algorithm.py:460-462).
+// ---------------------------------------------------------------------------
+void apply_bcs(osb_ctx* c) {
+    const int nd = c->ndim, nv = c->nv;
+    const int h = c->depth;
+    const size_t Pt = c->Ptot;
+    for (int dir = 0; dir < nd; ++dir) {
+        const int N = c->n[dir];
+        const size_t st = c->stride(dir);
+        for (int side = 0; side < 2; ++side) {
+            const int type = c->d.bc[dir][side];
+            Range r = make_range(c, h);
+            r.lo[dir] = side == 0 ? 0 : N - 1;
+            r.hi[dir] = r.lo[dir] + 1;
+            const int to = side == 0 ? 1 : -1;       // set_side_factor
+            for_range(r, [&](int i, int j, int k) {
+                const size_t xb = c->I(i, j, k);    // boundary point
+                auto cp = [&](long dst, long src, bool reflect_half, bool reflect_full) {
+                    for (int v = 0; v < nv; ++v) {
+                        double val = c->q[(size_t)v * Pt + (size_t)((long)xb + src * (long)st)];
+                        if (v == 1 + dir) {
+                            if (reflect_full) val = val - 2. * val;     // v - 2(v.n)n
+                            else if (reflect_half) val = val - 1. * val; // v - (v.n)n
+                        }
+                        c->q[(size_t)v * Pt + (size_t)((long)xb + dst * (long)st)] = val;
+                    }
+                };
+                if (type == OSB_BC_PERIODIC) {
+                    // ExchangeSelf: side 0 copies [0,h) -> [N,N+h); side 1 copies [N-h,N) -> [-h,0)
+                    for (int t = 0; t < h; ++t) {
+                        if (side == 0) cp((long)N + t, t, false, false);
+                        else cp(-(long)(N - 1) - h + t, (long)(1 - h + t), false, false);
+                    }
+                } else if (type == OSB_BC_ZERO_GRADIENT) {
+                    cp(0, to, false, false);                               // q[0] = q[1]
+                    for (int t = 1; t <= h; ++t) cp(-(long)to * t, (long)to * t, false, false);
+                } else if (type == OSB_BC_INVISCID_WALL) {
+                    for (int t = 1; t <= h; ++t) cp(-(long)to * t, (long)to * t, false, true);
+                    cp(0, to, true, false);                                // boundary point from q[1]
+                } else if (type == OSB_BC_SYMMETRY) {
+                    for (int t = 1; t <= h; ++t) cp(-(long)to * t, (long)to * t, false, true);
+                }
+            });
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Right-hand side at the current state (halos already filled).
+// ---------------------------------------------------------------------------
+void compute_rhs(osb_ctx* c) {
+    const int ext = c->shock ? 3 : 2;
+    Range rcr = make_range(c, ext);
+    if (c->shock) {
+        // u,p,a over [-3, N+4) (TENO6/opensbli.cpp:219-240)
+        for (int d = 0; d < c->ndim; ++d) rcr.hi[d] = c->n[d] + 4;
+    }
+    constituent_relations(c, rcr, c->viscous || c->central);
+    if (c->shock) {
+        for (int dir = 0; dir < c->ndim; ++dir) reconstruct_direction(c, dir);
+        shock_residual(c);
+    }
+    if (c->central || c->viscous) stored_gradients(c);
+    if (c->central) central_convective(c);
+    if (c->viscous) viscous_terms(c);
+}
+
+double max_wavespeed(osb_ctx* c) {
+    // CFL sum over directions of (|u_d| + a)/Delta_d at interior points, from the current state
+    const int nd = c->ndim;
+    const double gama = c->d.gama;
+    Range r = make_range(c, 0);
+    const int nk = r.hi[2] - r.lo[2], nj = r.hi[1] - r.lo[1];
+    double m = 0.0;
+#pragma omp parallel for reduction(max : m) schedule(static)
+    for (long t = 0; t < (long)nk * nj; ++t) {
+        const int k = (int)(t / nj), j = (int)(t % nj);
+        for (int i = 0; i < c->n[0]; ++i) {
+            const size_t x = c->I(i, j, k);
+            const double rho = c->q[x];
+            double uu[3] = {0, 0, 0}, ke = 0.0;
+            for (int d = 0; d < nd; ++d) { uu[d] = c->q[(size_t)(1 + d) * c->Ptot + x] / rho; ke -= 0.5 * (uu[d] * uu[d]) * rho; }
+            const double pp = (gama - 1.0) * (ke + c->q[(size_t)(nd + 1) * c->Ptot + x]);
+            const double a = std::sqrt(gama * pp / rho);
+            double s = 0.0;
+            for (int d = 0; d < nd; ++d) s += (std::fabs(uu[d]) + a) * (1.0 / c->d.delta[d]);
+            m = std::max(m, s);
+        }
+    }
+    return m;
+}
+
+int validate(const osb_problem_desc* d, std::string& why) {
+    if (!d) { why = "null descriptor"; return OSB_E_INVALID; }
+    if (d->struct_size != sizeof(osb_problem_desc) || d->abi_version != OSB_ABI_VERSION) { why = "descriptor size/abi mismatch"; return OSB_E_INVALID; }
+    if (d->ndim < 1 || d->ndim > 3) { why = "ndim must be 1..3"; return OSB_E_INVALID; }
+    for (int i = 0; i < 3; ++i) {
+        if (i < d->ndim && (d->n[i] < 8 || !(d->delta[i] > 0))) { why = "n >= 8 and delta > 0 required in active directions"; return OSB_E_INVALID; }
+        if (i >= d->ndim && d->n[i] != 1) { why = "inactive directions must have n == 1"; return OSB_E_INVALID; }
+    }
+    const int cs = d->conv_scheme;
+    if (cs != OSB_CONV_WENO_Z && cs != OSB_CONV_TENO && cs != OSB_CONV_CENTRAL_FEIEREISEN && cs != OSB_CONV_CENTRAL_SKEW) { why = "unknown conv_scheme"; return OSB_E_UNSUPPORTED; }
+    if (cs == OSB_CONV_WENO_Z && d->order != 5) { why = "WENO-Z supports order 5"; return OSB_E_UNSUPPORTED; }
+    if (cs == OSB_CONV_TENO && d->order != 5 && d->order != 6) { why = "TENO supports orders 5 and 6"; return OSB_E_UNSUPPORTED; }
+    if ((cs == OSB_CONV_CENTRAL_FEIEREISEN || cs == OSB_CONV_CENTRAL_SKEW) && d->order != 4) { why = "central scheme supports order 4"; return OSB_E_UNSUPPORTED; }
+    if ((cs == OSB_CONV_WENO_Z || cs == OSB_CONV_TENO) && d->averaging != OSB_AVG_SIMPLE && d->averaging != OSB_AVG_ROE) { why = "unknown averaging"; return OSB_E_UNSUPPORTED; }
+    if (d->viscous != OSB_VISC_NONE && d->viscous != OSB_VISC_CONSTANT && d->viscous != OSB_VISC_SUTHERLAND) { why = "unknown viscous model"; return OSB_E_UNSUPPORTED; }
+    if (d->viscous != OSB_VISC_NONE && d->ndim < 2) { why = "viscous terms need ndim >= 2"; return OSB_E_UNSUPPORTED; }
+    if (d->time_scheme != OSB_TIME_RKLS3 && d->time_scheme != OSB_TIME_RK3) { why = "unknown time scheme"; return OSB_E_UNSUPPORTED; }
+    if (d->precision != OSB_PREC_FP64 && d->precision != OSB_PREC_MIXED) { why = "unknown precision"; return OSB_E_UNSUPPORTED; }
+    for (int i = 0; i < d->ndim; ++i) {
+        for (int s = 0; s < 2; ++s) {
+            const int b = d->bc[i][s];
+            if (b < OSB_BC_PERIODIC || b > OSB_BC_SYMMETRY) { why = "unknown boundary condition"; return OSB_E_UNSUPPORTED; }
+        }
+        if ((d->bc[i][0] == OSB_BC_PERIODIC) != (d->bc[i][1] == OSB_BC_PERIODIC)) { why = "periodic boundaries must be paired"; return OSB_E_INVALID; }
+    }
+    if (!(d->cfl > 0) && !(d->dt > 0)) { why = "dt > 0 or cfl > 0 required"; return OSB_E_INVALID; }
+    if (!(d->gama > 1.0)) { why = "gama > 1 required"; return OSB_E_INVALID; }
+    if (d->viscous != OSB_VISC_NONE && (!(d->re > 0) || !(d->pr > 0) || !(d->minf > 0))) { why = "Re, Pr, Minf > 0 required"; return OSB_E_INVALID; }
+    if (cs == OSB_CONV_CENTRAL_FEIEREISEN || cs == OSB_CONV_CENTRAL_SKEW) {
+        if (d->ndim < 2) { why = "central split forms need ndim >= 2"; return OSB_E_UNSUPPORTED; }
+    }
+    return OSB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int osb_abi_version(void) { return OSB_ABI_VERSION; }
+
+const char* osb_last_error(const osb_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
+    if (!out) return fail(nullptr, OSB_E_INVALID, "null output pointer");
+    *out = nullptr;
+    std::string why;
+    int rc = validate(desc, why);
+    if (rc) return fail(nullptr, rc, why);
+    osb_ctx* c = new osb_ctx();
+    c->d = *desc;
+    c->ndim = desc->ndim;
+    c->nv = desc->ndim + 2;
+    for (int i = 0; i < 3; ++i) {
+        c->n[i] = desc->n[i];
+        if (i < c->ndim) { c->P[i] = desc->n[i] + 2 * HALO; c->off[i] = HALO; }
+    }
+    c->Ptot = (size_t)c->P[0] * c->P[1] * c->P[2];
+    c->shock = desc->conv_scheme == OSB_CONV_WENO_Z || desc->conv_scheme == OSB_CONV_TENO;
+    c->central = !c->shock;
+    c->viscous = desc->viscous != OSB_VISC_NONE;
+    c->depth = c->shock ? 4 : 2;          // WenoHalos/TenoHalos [-3,4] -> 4; CentralHalos(4) -> 2
+    const size_t nvP = (size_t)c->nv * c->Ptot;
+    c->q.assign(nvP, 0.0);
+    c->rkold.assign(nvP, 0.0);
+    c->R.assign(nvP, 0.0);
+    c->u.assign((size_t)c->ndim * c->Ptot, 0.0);
+    c->p.assign(c->Ptot, 0.0);
+    c->a.assign(c->Ptot, 0.0);
+    if (c->viscous || c->central) { c->T.assign(c->Ptot, 0.0); c->mu.assign(c->Ptot, 0.0); }
+    if (c->shock) c->wk.assign((size_t)c->ndim * nvP, 0.0);
+    if (c->viscous || c->central) c->g.assign((size_t)c->ndim * c->ndim * c->Ptot, 0.0);
+    *out = c;
+    return OSB_OK;
+}
+
+int osb_ctx_destroy(osb_ctx* ctx) { delete ctx; return OSB_OK; }
+
+int osb_set_state(osb_ctx* c, const double* host, int layout) {
+    if (!c || !host) return OSB_E_INVALID;
+    if (layout != OSB_LAYOUT_INTERIOR) return fail(c, OSB_E_UNSUPPORTED, "oracle accepts the interior layout only");
+    std::fill(c->q.begin(), c->q.end(), 0.0);
+    std::fill(c->rkold.begin(), c->rkold.end(), 0.0);
+    const size_t npts = (size_t)c->n[0] * c->n[1] * c->n[2];
+    for (int v = 0; v < c->nv; ++v)
+        for (int k = 0; k < c->n[2]; ++k)
+            for (int j = 0; j < c->n[1]; ++j)
+                for (int i = 0; i < c->n[0]; ++i)
+                    c->q[(size_t)v * c->Ptot + c->I(i, j, k)] = host[(size_t)v * npts + (size_t)i + (size_t)c->n[0] * ((size_t)j + (size_t)c->n[1] * k)];
+    apply_bcs(c);
+    return OSB_OK;
+}
+
+int osb_get_state(osb_ctx* c, double* host, int layout) {
+    if (!c || !host) return OSB_E_INVALID;
+    if (layout != OSB_LAYOUT_INTERIOR) return fail(c, OSB_E_UNSUPPORTED, "oracle returns the interior layout only");
+    const size_t npts = (size_t)c->n[0] * c->n[1] * c->n[2];
+    for (int v = 0; v < c->nv; ++v)
+        for (int k = 0; k < c->n[2]; ++k)
+            for (int j = 0; j < c->n[1]; ++j)
+                for (int i = 0; i < c->n[0]; ++i)
+                    host[(size_t)v * npts + (size_t)i + (size_t)c->n[0] * ((size_t)j + (size_t)c->n[1] * k)] = c->q[(size_t)v * c->Ptot + c->I(i, j, k)];
+    return OSB_OK;
+}
+
+int osb_set_time(osb_ctx* c, double time, int64_t iter) {
+    if (!c) return OSB_E_INVALID;
+    c->time = time; c->iter = iter;
+    return OSB_OK;
+}
+
+int osb_apply_bcs(osb_ctx* c) { if (!c) return OSB_E_INVALID; apply_bcs(c); return OSB_OK; }
+
+int osb_residual(osb_ctx* c, double* host_out) {
+    if (!c || !host_out) return OSB_E_INVALID;
+    apply_bcs(c);
+    compute_rhs(c);
+    const size_t npts = (size_t)c->n[0] * c->n[1] * c->n[2];
+    for (int v = 0; v < c->nv; ++v)
+        for (int k = 0; k < c->n[2]; ++k)
+            for (int j = 0; j < c->n[1]; ++j)
+                for (int i = 0; i < c->n[0]; ++i)
+                    host_out[(size_t)v * npts + (size_t)i + (size_t)c->n[0] * ((size_t)j + (size_t)c->n[1] * k)] = c->R[(size_t)v * c->Ptot + c->I(i, j, k)];
+    return OSB_OK;
+}
+
+int osb_max_wavespeed(osb_ctx* c, double* out) {
+    if (!c || !out) return OSB_E_INVALID;
+    *out = max_wavespeed(c);
+    return OSB_OK;
+}
+
+int osb_step(osb_ctx* c, int nsteps) {
+    if (!c || nsteps < 0) return OSB_E_INVALID;
+    const int nv = c->nv;
+    const size_t Pt = c->Ptot;
+    Range in = make_range(c, 0);
+    for (int it = 0; it < nsteps; ++it) {
+        apply_bcs(c);                                        // temporal_start BC kernels (algorithm.py:462)
+        double dt = c->d.dt;
+        if (c->d.cfl > 0) dt = c->d.cfl / max_wavespeed(c);
+        c->last_dt = dt;
+        if (c->d.time_scheme == OSB_TIME_RKLS3) {
+            // rk_LS.py:121-124, 204-215
+            const double rkA[3] = {0, (-5.0 / 9.0), (-153.0 / 128.0)};
+            const double rkB[3] = {(1.0 / 3.0), (15.0 / 16.0), (8.0 / 15.0)};
+            for (int s = 0; s < 3; ++s) {
+                compute_rhs(c);
+                for_range(in, [&](int i, int j, int k) {
+                    const size_t x = c->I(i, j, k);
+                    for (int v = 0; v < nv; ++v) {
+                        const size_t y = (size_t)v * Pt + x;
+                        c->rkold[y] = rkA[s] * c->rkold[y] + dt * c->R[y];
+                        c->q[y] = rkB[s] * c->rkold[y] + c->q[y];
+                    }
+                });
+                apply_bcs(c);
+            }
+        } else {
+            // rk_sbli.py:69-76, 129-159: save q, then per stage q = qold + dt*rknew*R; qold += dt*rkold*R
+            const double rkold_c[3] = {1.0 / 4.0, 3.0 / 20.0, 3.0 / 5.0};
+            const double rknew_c[3] = {2.0 / 3.0, 5.0 / 12.0, 3.0 / 5.0};
+            for_range(in, [&](int i, int j, int k) {
+                const size_t x = c->I(i, j, k);
+                for (int v = 0; v < nv; ++v) c->rkold[(size_t)v * Pt + x] = c->q[(size_t)v * Pt + x];
+            });
+            for (int s = 0; s < 3; ++s) {
+                compute_rhs(c);
+                for_range(in, [&](int i, int j, int k) {
+                    const size_t x = c->I(i, j, k);
+                    for (int v = 0; v < nv; ++v) {
+                        const size_t y = (size_t)v * Pt + x;
+                        c->q[y] = c->rkold[y] + dt * rknew_c[s] * c->R[y];
+                        c->rkold[y] = c->rkold[y] + dt * rkold_c[s] * c->R[y];
+                    }
+                });
+                apply_bcs(c);
+            }
+        }
+        c->iter += 1;
+        c->time += dt;
+    }
+    return OSB_OK;
+}
+
+int osb_sync(osb_ctx* c) { return c ? OSB_OK : OSB_E_INVALID; }
+
+int osb_get_stats(osb_ctx* c, osb_stats* out) {
+    if (!c || !out) return OSB_E_INVALID;
+    out->struct_size = sizeof(osb_stats);
+    out->iter = c->iter;
+    out->time = c->time;
+    out->last_dt = c->last_dt;
+    out->halo = HALO;
+    out->nv = c->nv;
+    out->device_bytes = 0;
+    return OSB_OK;
+}
+
+int osb_tgv_diagnostics(osb_ctx* c, double out[4]) {
+    // TENO6/opensbliblock00_kernels.h:4737-4797; u and mu are the constituent-relation
+    // arrays left by the last stage, rho is the current state.
+    if (!c || !out) return OSB_E_INVALID;
+    if (c->ndim != 3) return fail(c, OSB_E_UNSUPPORTED, "TGV diagnostics need ndim == 3");
+    if (c->mu.empty()) return fail(c, OSB_E_UNSUPPORTED, "TGV diagnostics need a viscous configuration");
+    const size_t Pt = c->Ptot;
+    double ke = 0, dil = 0, ens = 0, rhom = 0;
+    const int N0 = c->n[0], N1 = c->n[1], N2 = c->n[2];
+#pragma omp parallel for reduction(+ : ke, dil, ens, rhom) schedule(static)
+    for (long t = 0; t < (long)N2 * N1; ++t) {
+        const int k = (int)(t / N1), j = (int)(t % N1);
+        for (int i = 0; i < N0; ++i) {
+            const size_t x = c->I(i, j, k);
+            double G[3][3];
+            for (int m = 0; m < 3; ++m)
+                for (int dd = 0; dd < 3; ++dd) {
+                    const double* um = c->u.data() + (size_t)m * Pt;
+                    const size_t st = c->stride(dd);
+                    G[m][dd] = d1([&](int s) { return um[x + (long)s * (long)st]; }, 1.0 / c->d.delta[dd]);
+                }
+            const double wx = G[2][1] - G[1][2], wy = G[0][2] - G[2][0], wz = G[1][0] - G[0][1];
+            const double divV = G[0][0] + G[1][1] + G[2][2];
+            const double rho = c->q[x], muv = c->mu[x];
+            const double u0 = c->u[x], u1 = c->u[Pt + x], u2 = c->u[2 * Pt + x];
+            rhom += rho;
+            ke += 0.5 * (u0 * u0 + u1 * u1 + u2 * u2) * rho;
+            dil += (4.0 / 3.0) * (divV * divV) * muv;
+            ens += (wx * wx + wy * wy + wz * wz) * muv;
+        }
+    }
+    out[0] = ke; out[1] = dil; out[2] = ens; out[3] = rhom;
+    return OSB_OK;
+}
+
+int osb_nan_check(osb_ctx* c) {
+    if (!c) return OSB_E_INVALID;
+    for (int k = 0; k < c->n[2]; ++k)
+        for (int j = 0; j < c->n[1]; ++j)
+            for (int i = 0; i < c->n[0]; ++i)
+                if (!std::isfinite(c->q[c->I(i, j, k)])) return fail(c, OSB_E_NONFINITE, "non-finite rho");
+    return OSB_OK;
+}
+
+int osb_state_device_ptr(osb_ctx* c, int, void**) { return fail(c, OSB_E_UNSUPPORTED, "oracle has no device buffers"); }
+int osb_set_stream(osb_ctx* c, void*) { return fail(c, OSB_E_UNSUPPORTED, "oracle has no streams"); }
+
+}  // extern "C"

@@ -1,0 +1,131 @@
+"""HIP library against the CPU oracle on the same inputs (FP64 bar: relative
+L2 and L-inf <= 1e-11 per conservative variable, as stated for the FP64 path).
+
+Each case prints its name before touching the GPU so a failing run names it."""
+import math
+
+import numpy as np
+import pytest
+
+from opensbli_amd import native as nat, problems as P
+from helpers import rel_errors, run
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-11
+
+
+def _announce(name):
+    print(f"[gpu-case] {name}", flush=True)
+
+
+def _compare(hip, oracle, pr, steps=None, tol=TOL):
+    _announce(f"{pr.name} steps={steps if steps is not None else pr.steps} n={list(pr.desc.n)}")
+    a, sa = run(hip, pr, steps)
+    b, sb = run(oracle, pr, steps)
+    l2, li = rel_errors(a, b)
+    print(f"   rel L2 {l2.max():.3e}  rel Linf {li.max():.3e}", flush=True)
+    assert np.all(np.isfinite(a))
+    assert l2.max() <= tol and li.max() <= tol, (l2, li)
+    assert sa.iter == sb.iter
+    assert math.isclose(sa.time, sb.time, rel_tol=1e-12, abs_tol=1e-15)
+    return a, b
+
+
+def _residual_compare(hip, oracle, pr, state=None, tol=TOL):
+    _announce(f"residual {pr.name} n={list(pr.desc.n)}")
+    q = pr.q0 if state is None else state
+    with hip.context(pr.desc) as c:
+        c.set_state(q)
+        ra = c.residual()
+    with oracle.context(pr.desc) as c:
+        c.set_state(q)
+        rb = c.residual()
+    scale = np.array([np.max(np.abs(rb[v])) for v in range(rb.shape[0])])
+    err = np.array([np.max(np.abs(ra[v] - rb[v])) for v in range(rb.shape[0])]) / np.maximum(scale, 1e-300)
+    print(f"   residual rel Linf {err.max():.3e}", flush=True)
+    assert err.max() <= 1e-12, err
+    return ra, rb
+
+
+# --- single right-hand-side evaluations -------------------------------------------
+def test_residual_sod(hip, oracle):
+    _residual_compare(hip, oracle, P.sod(n=256))
+
+
+def test_residual_kh(hip, oracle):
+    _residual_compare(hip, oracle, P.kelvin_helmholtz(n=64))
+
+
+def test_residual_riemann(hip, oracle):
+    _residual_compare(hip, oracle, P.riemann_2d(n=64))
+
+
+def test_residual_tgv_central(hip, oracle):
+    _residual_compare(hip, oracle, P.tgv(n=24))
+
+
+def test_residual_tgv_teno5(hip, oracle):
+    _residual_compare(hip, oracle, P.tgv(n=24, scheme="teno", minf=1.25))
+
+
+def test_residual_tgv_teno6(hip, oracle):
+    _residual_compare(hip, oracle, P.tgv_teno6_validation(n=24))
+
+
+def test_residual_tg_sym(hip, oracle):
+    _residual_compare(hip, oracle, P.tg_sym())
+
+
+# --- full runs --------------------------------------------------------------------
+def test_sod_full_config(hip, oracle):
+    """Sod, N=1024, WENO-Z LLF, RK3-LS, dt=2e-4, 1000 steps -- the full configuration."""
+    _compare(hip, oracle, P.sod())
+
+
+def test_tgv_central_small(hip, oracle):
+    _compare(hip, oracle, P.tgv(n=32, steps=10))
+
+
+def test_tgv_teno5_cfl_small(hip, oracle):
+    _compare(hip, oracle, P.tgv(n=24, steps=5, scheme="teno", minf=1.25, cfl=0.5, dt=0.0))
+
+
+def test_tgv_teno6_small(hip, oracle):
+    _compare(hip, oracle, P.tgv_teno6_validation(n=24, steps=5))
+
+
+def test_kh_small(hip, oracle):
+    _compare(hip, oracle, P.kelvin_helmholtz(n=64, steps=50))
+
+
+def test_riemann_small(hip, oracle):
+    _compare(hip, oracle, P.riemann_2d(n=64, steps=50))
+
+
+def test_tg_sym_known_answer(hip, oracle):
+    """Reference pin: w at interior (8,8,8) after one step,
+    tests/verification_apps/tg_sym/check_opensbli_output.py:31-34."""
+    pr = P.tg_sym()
+    a, b = _compare(hip, oracle, pr)
+    w = a[3] / a[0]
+    target = -2.2438363795512375e-08
+    print(f"   w(8,8,8) = {w[8, 8, 8]!r}", flush=True)
+    assert math.isclose(w[8, 8, 8], target, rel_tol=2e-8)
+
+
+def test_tgv_teno6_reference_validation(hip):
+    """Reference pin: apps/performance_benchmark/README.MD:46-47, TENO6 128^3, 500 iterations."""
+    pr = P.tgv_teno6_validation()
+    _announce(pr.name + " 500 steps (README validation)")
+    with hip.context(pr.desc) as c:
+        c.set_state(pr.q0)
+        c.step(pr.steps)
+        st = c.stats()
+        d = c.tgv_diagnostics()
+    print(f"   {st.iter}, {st.time:.5e}, " + ", ".join(f"{v:.5e}" for v in d), flush=True)
+    ref = [2.65363e+05, 1.16991e+05, 1.79666e+06, 2.09715e+06]
+    assert st.iter == 500
+    assert f"{st.time:.5e}" == "8.46250e-01"
+    for v, r in zip(d, ref):
+        assert f"{v:.5e}" == f"{r:.5e}", (d, ref)
