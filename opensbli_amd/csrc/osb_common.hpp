@@ -45,93 +45,97 @@ struct Consts {
 // the FP64 path, float in the mixed path); smoothness indicators, non-linear
 // weights and the TENO cut-off are always evaluated in double.
 // f[s] holds the split flux at offset s-2 from point i (interface i+1/2).
+// The expression sequence (including every fused multiply-add, written out
+// as fma) is fixed so that the FP64 path rounds exactly like the CPU oracle;
+// the library is compiled with -ffp-contract=off.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double sq(double x) { return x * x; }
+// 13/12 s1^2 + 1/4 s2^2 (JS_smoothness, weno.py:139-206; teno.py:147-182)
+__device__ __forceinline__ double js_sum(double s1, double s2) { return fma((13.0 / 12.0) * s1, s1, (0.25 * s2) * s2); }
+__device__ __forceinline__ double js_first(double a, double b, double c) { return js_sum(a - 2.0 * b + c, 3.0 * a - 4.0 * b + c); }
+__device__ __forceinline__ double js_centre(double a, double b, double c) { return js_sum(a - 2.0 * b + c, a - c); }
+__device__ __forceinline__ double js_last(double a, double b, double c) { return js_sum(a - 2.0 * b + c, a - 4.0 * b + 3.0 * c); }
 
-// Jiang-Shu indicators of 3-point stencils (a, b, c in ascending x)
-__device__ __forceinline__ double js_first(double a, double b, double c) {
-    return (13.0 / 12.0) * sq(a - 2.0 * b + c) + 0.25 * sq(3.0 * a - 4.0 * b + c);
-}
-__device__ __forceinline__ double js_centre(double a, double b, double c) {
-    return (13.0 / 12.0) * sq(a - 2.0 * b + c) + 0.25 * sq(a - c);
-}
-__device__ __forceinline__ double js_last(double a, double b, double c) {
-    return (13.0 / 12.0) * sq(a - 2.0 * b + c) + 0.25 * sq(a - 4.0 * b + 3.0 * c);
+template <typename T>
+__device__ __forceinline__ T eno3(T c0, T c1, T c2, T x0, T x1, T x2) { return fma(c2, x2, fma(c1, x1, c0 * x0)); }
+
+__device__ __forceinline__ double zalpha(double d, double t2, double b, double eps) {
+    const double eb = eps + b;
+    return d + (d * t2) / (eb * eb);
 }
 
-// WENO-Z, order 5 (weno.py:290-345): alpha_r = d_r + d_r tau^2/((eps+b_r)^2)
+// WENO-Z, order 5 (weno.py:290-345): tau5 = |b0 - b2|,
+// alpha_r = d_r + d_r tau^2/((eps+b_r)(eps+b_r)), omega_r = alpha_r / sum(alpha)
 template <typename T>
 __device__ __forceinline__ T wenoz_right(const T* f, double eps) {
-    const double f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4];
-    const double b0 = js_first(f2, f3, f4);     // stencil (i, i+1, i+2), d = 3/10
-    const double b1 = js_centre(f1, f2, f3);    // stencil (i-1, i, i+1), d = 6/10
-    const double b2 = js_last(f0, f1, f2);      // stencil (i-2, i-1, i), d = 1/10
-    const double tau2 = sq(fabs(b0 - b2));
-    const double a0 = 0.3 + 0.3 * tau2 / sq(eps + b0);
-    const double a1 = 0.6 + 0.6 * tau2 / sq(eps + b1);
-    const double a2 = 0.1 + 0.1 * tau2 / sq(eps + b2);
+    const double b0 = js_first(f[2], f[3], f[4]);     // stencil (i, i+1, i+2), d = 3/10
+    const double b1 = js_centre(f[1], f[2], f[3]);    // stencil (i-1, i, i+1), d = 6/10
+    const double b2 = js_last(f[0], f[1], f[2]);      // stencil (i-2, i-1, i), d = 1/10
+    const double tau = fabs(b0 - b2), t2 = tau * tau;
+    const double a0 = zalpha(3.0 / 10.0, t2, b0, eps);
+    const double a1 = zalpha(6.0 / 10.0, t2, b1, eps);
+    const double a2 = zalpha(1.0 / 10.0, t2, b2, eps);
     const double inv = 1.0 / (a0 + a1 + a2);
-    const T e0 = T(1.0 / 3.0) * f[2] + T(5.0 / 6.0) * f[3] - T(1.0 / 6.0) * f[4];
-    const T e1 = -T(1.0 / 6.0) * f[1] + T(5.0 / 6.0) * f[2] + T(1.0 / 3.0) * f[3];
-    const T e2 = T(1.0 / 3.0) * f[0] - T(7.0 / 6.0) * f[1] + T(11.0 / 6.0) * f[2];
-    return T(a0 * inv) * e0 + T(a1 * inv) * e1 + T(a2 * inv) * e2;
+    const T e0 = eno3<T>(T(1.0 / 3.0), T(5.0 / 6.0), T(-1.0 / 6.0), f[2], f[3], f[4]);
+    const T e1 = eno3<T>(T(-1.0 / 6.0), T(5.0 / 6.0), T(1.0 / 3.0), f[1], f[2], f[3]);
+    const T e2 = eno3<T>(T(1.0 / 3.0), T(-7.0 / 6.0), T(11.0 / 6.0), f[0], f[1], f[2]);
+    return fma(T(a2 * inv), e2, fma(T(a1 * inv), e1, T(a0 * inv) * e0));
 }
 template <typename T>
 __device__ __forceinline__ T wenoz_left(const T* f, double eps) {
-    const double f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4], f5 = f[5];
-    const double b0 = js_first(f3, f4, f5);     // (i+1, i+2, i+3), d = 1/10
-    const double b1 = js_centre(f2, f3, f4);    // (i, i+1, i+2),   d = 6/10
-    const double b2 = js_last(f1, f2, f3);      // (i-1, i, i+1),   d = 3/10
-    const double tau2 = sq(fabs(b0 - b2));
-    const double a0 = 0.1 + 0.1 * tau2 / sq(eps + b0);
-    const double a1 = 0.6 + 0.6 * tau2 / sq(eps + b1);
-    const double a2 = 0.3 + 0.3 * tau2 / sq(eps + b2);
+    const double b0 = js_first(f[3], f[4], f[5]);     // (i+1, i+2, i+3), d = 1/10
+    const double b1 = js_centre(f[2], f[3], f[4]);    // (i, i+1, i+2),   d = 6/10
+    const double b2 = js_last(f[1], f[2], f[3]);      // (i-1, i, i+1),   d = 3/10
+    const double tau = fabs(b0 - b2), t2 = tau * tau;
+    const double a0 = zalpha(1.0 / 10.0, t2, b0, eps);
+    const double a1 = zalpha(6.0 / 10.0, t2, b1, eps);
+    const double a2 = zalpha(3.0 / 10.0, t2, b2, eps);
     const double inv = 1.0 / (a0 + a1 + a2);
-    const T e0 = T(11.0 / 6.0) * f[3] - T(7.0 / 6.0) * f[4] + T(1.0 / 3.0) * f[5];
-    const T e1 = T(1.0 / 3.0) * f[2] + T(5.0 / 6.0) * f[3] - T(1.0 / 6.0) * f[4];
-    const T e2 = -T(1.0 / 6.0) * f[1] + T(5.0 / 6.0) * f[2] + T(1.0 / 3.0) * f[3];
-    return T(a0 * inv) * e0 + T(a1 * inv) * e1 + T(a2 * inv) * e2;
+    const T e0 = eno3<T>(T(11.0 / 6.0), T(-7.0 / 6.0), T(1.0 / 3.0), f[3], f[4], f[5]);
+    const T e1 = eno3<T>(T(1.0 / 3.0), T(5.0 / 6.0), T(-1.0 / 6.0), f[2], f[3], f[4]);
+    const T e2 = eno3<T>(T(-1.0 / 6.0), T(5.0 / 6.0), T(1.0 / 3.0), f[1], f[2], f[3]);
+    return fma(T(a2 * inv), e2, fma(T(a1 * inv), e1, T(a0 * inv) * e0));
 }
 
-__device__ __forceinline__ double pow6(double x) { const double x2 = x * x; return x2 * x2 * x2; }
+// (C + tau/beta)**6 printed as six repeated multiplies (opsc.py:222-241)
+__device__ __forceinline__ double pow6(double x) { return x * x * x * x * x * x; }
 
-// TENO5 / TENO6 (teno.py:59-217, 419-477).  SIDE +1: f+ on offsets -2..3
-// (right); SIDE -1: f- (left).  ORDER 5 or 6.
+// TENO5 / TENO6 (teno.py:59-217, 419-477).  SIDE +1: f+ (right), -1: f- (left).
 template <int ORDER, int SIDE, typename T>
 __device__ __forceinline__ T teno(const T* f, double eps, double ct) {
     constexpr int NS = ORDER == 5 ? 3 : 4;
-    // F(o) = f[o+2]
     const double m2 = f[0], m1 = f[1], z0 = f[2], p1 = f[3], p2 = f[4], p3 = f[5];
     double beta[4];
     T eno[4];
     if (SIDE > 0) {
-        beta[0] = 0.25 * sq(m1 - p1) + (13.0 / 12.0) * sq(m1 - 2.0 * z0 + p1);
-        beta[1] = 0.25 * sq(3.0 * z0 - 4.0 * p1 + p2) + (13.0 / 12.0) * sq(z0 - 2.0 * p1 + p2);
-        beta[2] = 0.25 * sq(m2 - 4.0 * m1 + 3.0 * z0) + (13.0 / 12.0) * sq(m2 - 2.0 * m1 + z0);
-        eno[0] = -T(1.0 / 6.0) * f[1] + T(5.0 / 6.0) * f[2] + T(2.0 / 6.0) * f[3];
-        eno[1] = T(2.0 / 6.0) * f[2] + T(5.0 / 6.0) * f[3] - T(1.0 / 6.0) * f[4];
-        eno[2] = T(2.0 / 6.0) * f[0] - T(7.0 / 6.0) * f[1] + T(11.0 / 6.0) * f[2];
+        // stencils [-1,0,1], [0,1,2], [-2,-1,0], [0,1,2,3]
+        beta[0] = js_sum(m1 - 2.0 * z0 + p1, m1 - p1);
+        beta[1] = js_sum(z0 - 2.0 * p1 + p2, 3.0 * z0 - 4.0 * p1 + p2);
+        beta[2] = js_sum(m2 - 2.0 * m1 + z0, m2 - 4.0 * m1 + 3.0 * z0);
+        eno[0] = fma(T(2.0 / 6.0), f[3], fma(T(5.0 / 6.0), f[2], T(-1.0 / 6.0) * f[1]));
+        eno[1] = fma(T(-1.0 / 6.0), f[4], fma(T(5.0 / 6.0), f[3], T(2.0 / 6.0) * f[2]));
+        eno[2] = fma(T(11.0 / 6.0), f[2], fma(T(-7.0 / 6.0), f[1], T(2.0 / 6.0) * f[0]));
         if (NS == 4) {
             const double s1 = -11.0 * z0 + 18.0 * p1 - 9.0 * p2 + 2.0 * p3;
             const double s2 = 2.0 * z0 - 5.0 * p1 + 4.0 * p2 - p3;
             const double s3 = -z0 + 3.0 * p1 - 3.0 * p2 + p3;
             // the right-side beta_3 keeps the 781/720 term unsquared (teno.py:168-169)
-            beta[3] = (1.0 / 36.0) * sq(s1) + (13.0 / 12.0) * sq(s2) + (781.0 / 720.0) * s3;
-            eno[3] = T(3.0 / 12.0) * f[2] + T(13.0 / 12.0) * f[3] - T(5.0 / 12.0) * f[4] + T(1.0 / 12.0) * f[5];
+            beta[3] = fma((1.0 / 36.0) * s1, s1, fma((13.0 / 12.0) * s2, s2, (781.0 / 720.0) * s3));
+            eno[3] = fma(T(1.0 / 12.0), f[5], fma(T(-5.0 / 12.0), f[4], fma(T(13.0 / 12.0), f[3], T(3.0 / 12.0) * f[2])));
         }
     } else {
-        beta[0] = 0.25 * sq(z0 - p2) + (13.0 / 12.0) * sq(z0 - 2.0 * p1 + p2);
-        beta[1] = 0.25 * sq(m1 - 4.0 * z0 + 3.0 * p1) + (13.0 / 12.0) * sq(m1 - 2.0 * z0 + p1);
-        beta[2] = 0.25 * sq(3.0 * p1 - 4.0 * p2 + p3) + (13.0 / 12.0) * sq(p1 - 2.0 * p2 + p3);
-        eno[0] = T(2.0 / 6.0) * f[2] + T(5.0 / 6.0) * f[3] - T(1.0 / 6.0) * f[4];
-        eno[1] = -T(1.0 / 6.0) * f[1] + T(5.0 / 6.0) * f[2] + T(2.0 / 6.0) * f[3];
-        eno[2] = T(11.0 / 6.0) * f[3] - T(7.0 / 6.0) * f[4] + T(1.0 / 3.0) * f[5];
+        // stencils [0,1,2], [-1,0,1], [1,2,3], [-2,-1,0,1]
+        beta[0] = js_sum(z0 - 2.0 * p1 + p2, z0 - p2);
+        beta[1] = js_sum(m1 - 2.0 * z0 + p1, m1 - 4.0 * z0 + 3.0 * p1);
+        beta[2] = js_sum(p1 - 2.0 * p2 + p3, 3.0 * p1 - 4.0 * p2 + p3);
+        eno[0] = fma(T(-1.0 / 6.0), f[4], fma(T(5.0 / 6.0), f[3], T(2.0 / 6.0) * f[2]));
+        eno[1] = fma(T(2.0 / 6.0), f[3], fma(T(5.0 / 6.0), f[2], T(-1.0 / 6.0) * f[1]));
+        eno[2] = fma(T(1.0 / 3.0), f[5], fma(T(-7.0 / 6.0), f[4], T(11.0 / 6.0) * f[3]));
         if (NS == 4) {
             const double s1 = -2.0 * m2 + 9.0 * m1 - 18.0 * z0 + 11.0 * p1;
             const double s2 = -m2 + 4.0 * m1 - 5.0 * z0 + 2.0 * p1;
             const double s3 = -m2 + 3.0 * m1 - 3.0 * z0 + p1;
-            beta[3] = (1.0 / 36.0) * sq(s1) + (13.0 / 12.0) * sq(s2) + (781.0 / 720.0) * sq(s3);
-            eno[3] = T(1.0 / 12.0) * f[0] - T(5.0 / 12.0) * f[1] + T(13.0 / 12.0) * f[2] + T(3.0 / 12.0) * f[3];
+            beta[3] = fma((1.0 / 36.0) * s1, s1, fma((13.0 / 12.0) * s2, s2, ((781.0 / 720.0) * s3) * s3));
+            eno[3] = fma(T(3.0 / 12.0), f[3], fma(T(13.0 / 12.0), f[2], fma(T(-5.0 / 12.0), f[1], T(1.0 / 12.0) * f[0])));
         }
     }
     double dopt[4];
@@ -142,7 +146,8 @@ __device__ __forceinline__ T teno(const T* f, double eps, double ct) {
     double alpha[4], asum = 0.0;
 #pragma unroll
     for (int r = 0; r < NS; ++r) {
-        alpha[r] = pow6(1.0 + tau * (1.0 / (eps + beta[r])));
+        const double inv_beta = 1.0 / (eps + beta[r]);
+        alpha[r] = pow6(1.0 + tau * inv_beta);
         asum += alpha[r];
     }
     const double inv_asum = 1.0 / asum;
@@ -155,7 +160,7 @@ __device__ __forceinline__ T teno(const T* f, double eps, double ct) {
     const double inv_wsum = 1.0 / wsum;
     T rec = T(0);
 #pragma unroll
-    for (int r = 0; r < NS; ++r) rec += T(dopt[r] * delta[r] * inv_wsum) * eno[r];
+    for (int r = 0; r < NS; ++r) rec = fma(T(dopt[r] * delta[r] * inv_wsum), eno[r], rec);
     return rec;
 }
 
@@ -181,20 +186,21 @@ __device__ __forceinline__ void eigensystem(T gm1, T rho, const T* u, T a, T (&L
         for (int j = 0; j < NV; ++j) { L[i][j] = T(0); R[i][j] = T(0); }
     if (NDIM == 1) {
         const T u0 = u[0];
-        const T H = a * a / gm1 + u0 * u0 * T(0.5);
+        const T inva = T(1) / a;
+        const T H = (a * a) * (T(1) / gm1) + (u0 * u0) * T(0.5);
         const T D = T(2) * H - u0 * u0;
-        const T invD = T(1) / D, inva = T(1) / a;
+        const T invD = T(1) / D;
         R[0][0] = 1; R[0][1] = 1; R[0][2] = 1;
         R[1][0] = u0 - a; R[1][1] = u0; R[1][2] = u0 + a;
-        R[2][0] = H - u0 * a; R[2][1] = u0 * u0 * T(0.5); R[2][2] = H + u0 * a;
+        R[2][0] = H - u0 * a; R[2][1] = (u0 * u0) * T(0.5); R[2][2] = H + u0 * a;
         L[0][0] = u0 * (T(2) * H + a * u0 - u0 * u0) * T(0.5) * inva * invD;
-        L[0][1] = (-H - a * u0 + u0 * u0 * T(0.5)) * inva * invD;
+        L[0][1] = (-H - a * u0 + (u0 * u0) * T(0.5)) * inva * invD;
         L[0][2] = invD;
         L[1][0] = T(2) * (H - u0 * u0) * invD;
         L[1][1] = T(2) * u0 * invD;
-        L[1][2] = -T(2) * invD;
+        L[1][2] = T(-2) * invD;
         L[2][0] = u0 * (-T(2) * H + a * u0 + u0 * u0) * T(0.5) * inva * invD;
-        L[2][1] = (H - a * u0 - u0 * u0 * T(0.5)) * inva * invD;
+        L[2][1] = (H - a * u0 - (u0 * u0) * T(0.5)) * inva * invD;
         L[2][2] = invD;
         return;
     }
@@ -356,13 +362,13 @@ __device__ __forceinline__ void face_flux(const double* __restrict__ q, long Pto
         for (int s = 0; s < 6; ++s) {
             T cs = T(0), cf = T(0);
 #pragma unroll
-            for (int m = 0; m < NV; ++m) { cs += L[k][m] * qs[s][m]; cf += L[k][m] * F[s][m]; }
-            fp[s] = T(0.5) * (cf + lam[k] * cs);
-            fm[s] = T(0.5) * (cf - lam[k] * cs);
+            for (int m = 0; m < NV; ++m) { cs = fma(L[k][m], qs[s][m], cs); cf = fma(L[k][m], F[s][m], cf); }
+            fp[s] = T(0.5) * fma(lam[k], cs, cf);
+            fm[s] = T(0.5) * fma(-lam[k], cs, cf);
         }
         const T rec = recon_pair<SCH, T>(fp, fm, c);
 #pragma unroll
-        for (int m = 0; m < NV; ++m) acc[m] += R[m][k] * rec;
+        for (int m = 0; m < NV; ++m) acc[m] = fma(R[m][k], rec, acc[m]);
     }
 #pragma unroll
     for (int m = 0; m < NV; ++m) out[m] = double(acc[m]);

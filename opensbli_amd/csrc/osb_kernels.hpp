@@ -97,11 +97,11 @@ __global__ void __launch_bounds__(BLOCK) k_constituent(const double* __restrict_
 
 template <class A>
 __device__ __forceinline__ double d1(A f, double invd) {
-    return ((1.0 / 12.0) * (f(-2) - f(2)) + (2.0 / 3.0) * (f(1) - f(-1))) * invd;
+    return fma(1.0 / 12.0, f(-2) - f(2), (2.0 / 3.0) * (f(1) - f(-1))) * invd;
 }
 template <class A>
 __device__ __forceinline__ double d2(A f, double inv2d) {
-    return ((-1.0 / 12.0) * (f(-2) + f(2)) + (4.0 / 3.0) * (f(-1) + f(1)) - (5.0 / 2.0) * f(0)) * inv2d;
+    return fma(-1.0 / 12.0, f(-2) + f(2), fma(4.0 / 3.0, f(-1) + f(1), (-5.0 / 2.0) * f(0))) * inv2d;
 }
 
 // Stored velocity gradients g[m*NDIM+j] = D_j u_m on [0,n) along j and
@@ -263,16 +263,19 @@ __global__ void __launch_bounds__(BLOCK) k_viscous(const double* __restrict__ cr
             const double mixed = d1([&](int s) { return ga[x + s * sb]; }, c.invd[b]);
             Lm += D2u[m][jd] + (1.0 / 3.0) * mixed;
         }
-        double mom = muv * Lm;
-        double ediss = 0.0;
+        double S[NDIM];
 #pragma unroll
-        for (int jd = 0; jd < NDIM; ++jd) {
-            const double S = G[m][jd] + G[jd][m] - (m == jd ? (2.0 / 3.0) * div : 0.0);
-            if (SUTH) mom += S * dmu[jd];
-            ediss += muv * S * G[m][jd];
+        for (int jd = 0; jd < NDIM; ++jd) S[jd] = G[m][jd] + G[jd][m] - (m == jd ? (2.0 / 3.0) * div : 0.0);
+        double mom = muv * Lm;
+        if (SUTH) {
+#pragma unroll
+            for (int jd = 0; jd < NDIM; ++jd) mom += S[jd] * dmu[jd];
         }
         R[(long)(1 + m) * P + x] += c.invRe * mom;
-        Eacc += cr[(long)m * P + x] * mom + ediss;
+        double e = cr[(long)m * P + x] * mom;                 // u_i d(tau_ij)/dx_j
+#pragma unroll
+        for (int jd = 0; jd < NDIM; ++jd) e += muv * S[jd] * G[m][jd];   // tau_ij du_i/dx_j
+        Eacc += e;
     }
     double heat = muv * lapT;
     if (SUTH) {
@@ -296,9 +299,9 @@ __global__ void __launch_bounds__(BLOCK) k_rkls(const double* __restrict__ qin, 
     const double dt = scal[1];
     for (int v = 0; v < nv; ++v) {
         const long y = (long)v * g.Ptot + x;
-        const double d = A * du[y] + dt * R[y];
+        const double d = fma(A, du[y], dt * R[y]);
         du[y] = d;
-        qout[y] = B * d + qin[y];
+        qout[y] = fma(B, d, qin[y]);
     }
 }
 
@@ -317,8 +320,8 @@ __global__ void __launch_bounds__(BLOCK) k_rk3_stage(double* __restrict__ qout, 
     for (int v = 0; v < nv; ++v) {
         const long y = (long)v * g.Ptot + x;
         const double r = R[y], o = qold[y];
-        qout[y] = o + dt * rknew * r;
-        qold[y] = o + dt * rkold * r;
+        qout[y] = fma(dt * rknew, r, o);
+        qold[y] = fma(dt * rkold, r, o);
     }
 }
 
@@ -376,8 +379,8 @@ __global__ void __launch_bounds__(BLOCK) k_bc(double* __restrict__ q, Geom g, in
     for (int v = 0; v < nv; ++v) {
         double val = q[(long)v * g.Ptot + xs];
         if (v == 1 + DIR) {
-            if (negate) val = -val;
-            if (zero) val = 0.0 * val;
+            if (negate) val = val - 2.0 * val;     // v - 2 (v.n) n   (inviscid_wall.py:37)
+            if (zero) val = val - 1.0 * val;       // v - (v.n) n     (inviscid_wall.py:40)
         }
         q[(long)v * g.Ptot + xd] = val;
     }

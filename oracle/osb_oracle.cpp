@@ -128,65 +128,71 @@ void constituent_relations(osb_ctx* c, const Range& r, bool need_T_mu) {
 // Euler eigensystems (opensbli/physical_models/euler_eigensystem.py:53-165),
 // evaluated on averaged quantities.  Direction enters through the unit normal
 // k = e_dir (k_t = k / |k| = k).  Full dense matrices; zero entries stay zero.
+// Repeated divisions are replaced by reciprocals of a and rho as the
+// reference does (create_LEV_inverses / create_REV_inverses,
+// shock_capturing.py:475-530) and of gamma-1 (invgamma_m1, TENO6/opensbli.cpp:140).
 // ---------------------------------------------------------------------------
 struct Eig {
     double L[5][5];
     double Rm[5][5];
 };
 
-void eigensystem(int nd, int dir, double gamma_m1, double rho, const double* uu, double a, Eig& e) {
+void eigensystem(int nd, int dir, double gm1, double rho, const double* uu, double a, Eig& e) {
     std::memset(&e, 0, sizeof(e));
+    const double igm1 = 1.0 / gm1;
+    const double ia = 1.0 / a;
     if (nd == 1) {
         // H = a**2/(gama-1) + u0**2/2 substituted (euler_eigensystem.py:75-76)
         const double u0 = uu[0];
-        const double H = a * a / gamma_m1 + u0 * u0 / 2.0;
+        const double H = (a * a) * igm1 + (u0 * u0) * 0.5;
         const double D = 2.0 * H - u0 * u0;
+        const double iD = 1.0 / D;
         // REV = [[1,1,1],[u0-a,u0,u0+a],[H-u0*a,u0**2/2,H+u0*a]]   (:89)
-        e.Rm[0][0] = 1; e.Rm[0][1] = 1; e.Rm[0][2] = 1;
+        e.Rm[0][0] = 1.0; e.Rm[0][1] = 1.0; e.Rm[0][2] = 1.0;
         e.Rm[1][0] = u0 - a; e.Rm[1][1] = u0; e.Rm[1][2] = u0 + a;
-        e.Rm[2][0] = H - u0 * a; e.Rm[2][1] = u0 * u0 / 2.0; e.Rm[2][2] = H + u0 * a;
-        // LEV (:90)
-        e.L[0][0] = u0 * (2 * H + a * u0 - u0 * u0) / (2 * a * D);
-        e.L[0][1] = (-H - a * u0 + u0 * u0 / 2) / (a * D);
-        e.L[0][2] = 1.0 / D;
-        e.L[1][0] = 2 * (H - u0 * u0) / D;
-        e.L[1][1] = 2 * u0 / D;
-        e.L[1][2] = 2.0 / (-2 * H + u0 * u0);
-        e.L[2][0] = u0 * (-2 * H + a * u0 + u0 * u0) / (2 * a * D);
-        e.L[2][1] = (H - a * u0 - u0 * u0 / 2) / (a * D);
-        e.L[2][2] = 1.0 / D;
+        e.Rm[2][0] = H - u0 * a; e.Rm[2][1] = (u0 * u0) * 0.5; e.Rm[2][2] = H + u0 * a;
+        // LEV (:90), 1/(2H - u0**2) and 1/a as reciprocals
+        e.L[0][0] = u0 * (2.0 * H + a * u0 - u0 * u0) * 0.5 * ia * iD;
+        e.L[0][1] = (-H - a * u0 + (u0 * u0) * 0.5) * ia * iD;
+        e.L[0][2] = iD;
+        e.L[1][0] = 2.0 * (H - u0 * u0) * iD;
+        e.L[1][1] = 2.0 * u0 * iD;
+        e.L[1][2] = -2.0 * iD;
+        e.L[2][0] = u0 * (-2.0 * H + a * u0 + u0 * u0) * 0.5 * ia * iD;
+        e.L[2][1] = (H - a * u0 - (u0 * u0) * 0.5) * ia * iD;
+        e.L[2][2] = iD;
         return;
     }
-    double k[3] = {0, 0, 0};
-    k[dir] = 1.0;
+    const double k0 = dir == 0 ? 1.0 : 0.0, k1 = dir == 1 ? 1.0 : 0.0, k2 = dir == 2 ? 1.0 : 0.0;
     double q2 = 0.0;
     for (int dd = 0; dd < nd; ++dd) q2 += uu[dd] * uu[dd];
-    const double phi_sq = gamma_m1 * (q2 / 2.0);
-    double theta = 0.0;
-    for (int dd = 0; dd < nd; ++dd) theta += k[dd] * uu[dd];
-    const double alpha = rho * INV_SQRT2 / a;         // rho/(a*sqrt(2.0))
-    const double bta = INV_SQRT2 / (rho * a);         // 1/(rho*a*sqrt(2.0))
+    const double phi_sq = gm1 * (q2 * 0.5);              // (gama-1)*((u0**2+u1**2+u2**2)/2)
+    const double theta = uu[dir];                          // k0_t*u0 + k1_t*u1 + k2_t*u2
+    const double irho = 1.0 / rho;
+    const double ia2 = ia * ia;
+    const double alpha = rho * INV_SQRT2 * ia;             // rho/(a*sqrt(2.0))
+    const double bta = INV_SQRT2 * irho * ia;              // 1/(rho*a*sqrt(2.0))
     const double a2 = a * a;
     if (nd == 2) {
-        const double u0 = uu[0], u1 = uu[1], k0 = k[0], k1 = k[1];
+        const double u0 = uu[0], u1 = uu[1];
         // REV (:121)
-        e.Rm[0][0] = 1; e.Rm[0][1] = 0; e.Rm[0][2] = alpha; e.Rm[0][3] = alpha;
+        e.Rm[0][0] = 1.0; e.Rm[0][1] = 0.0; e.Rm[0][2] = alpha; e.Rm[0][3] = alpha;
         e.Rm[1][0] = u0; e.Rm[1][1] = k1 * rho; e.Rm[1][2] = alpha * (u0 + k0 * a); e.Rm[1][3] = alpha * (u0 - k0 * a);
         e.Rm[2][0] = u1; e.Rm[2][1] = -k0 * rho; e.Rm[2][2] = alpha * (u1 + k1 * a); e.Rm[2][3] = alpha * (u1 - k1 * a);
-        e.Rm[3][0] = phi_sq / gamma_m1; e.Rm[3][1] = rho * (k1 * u0 - k0 * u1);
-        e.Rm[3][2] = alpha * ((phi_sq + a2) / gamma_m1 + a * theta);
-        e.Rm[3][3] = alpha * ((phi_sq + a2) / gamma_m1 - a * theta);
+        e.Rm[3][0] = phi_sq * igm1; e.Rm[3][1] = rho * (k1 * u0 - k0 * u1);
+        e.Rm[3][2] = alpha * ((phi_sq + a2) * igm1 + a * theta);
+        e.Rm[3][3] = alpha * ((phi_sq + a2) * igm1 - a * theta);
         // LEV (:122)
-        e.L[0][0] = 1 - phi_sq / a2; e.L[0][1] = gamma_m1 * u0 / a2; e.L[0][2] = gamma_m1 * u1 / a2; e.L[0][3] = -gamma_m1 / a2;
-        e.L[1][0] = -(k1 * u0 - k0 * u1) / rho; e.L[1][1] = k1 / rho; e.L[1][2] = -k0 / rho; e.L[1][3] = 0;
-        e.L[2][0] = bta * (phi_sq - a * theta); e.L[2][1] = bta * (k0 * a - gamma_m1 * u0);
-        e.L[2][2] = bta * (k1 * a - gamma_m1 * u1); e.L[2][3] = bta * gamma_m1;
-        e.L[3][0] = bta * (phi_sq + a * theta); e.L[3][1] = -bta * (k0 * a + gamma_m1 * u0);
-        e.L[3][2] = -bta * (k1 * a + gamma_m1 * u1); e.L[3][3] = bta * gamma_m1;
+        e.L[0][0] = 1.0 - phi_sq * ia2; e.L[0][1] = gm1 * u0 * ia2; e.L[0][2] = gm1 * u1 * ia2; e.L[0][3] = -gm1 * ia2;
+        e.L[1][0] = -(k1 * u0 - k0 * u1) * irho; e.L[1][1] = k1 * irho; e.L[1][2] = -k0 * irho; e.L[1][3] = 0.0;
+        e.L[2][0] = bta * (phi_sq - a * theta); e.L[2][1] = bta * (k0 * a - gm1 * u0);
+        e.L[2][2] = bta * (k1 * a - gm1 * u1); e.L[2][3] = bta * gm1;
+        e.L[3][0] = bta * (phi_sq + a * theta); e.L[3][1] = -bta * (k0 * a + gm1 * u0);
+        e.L[3][2] = -bta * (k1 * a + gm1 * u1); e.L[3][3] = bta * gm1;
         return;
     }
     // ndim == 3 (:150-152)
-    const double u0 = uu[0], u1 = uu[1], u2 = uu[2], k0 = k[0], k1 = k[1], k2 = k[2];
+    const double u0 = uu[0], u1 = uu[1], u2 = uu[2];
     e.Rm[0][0] = k0; e.Rm[0][1] = k1; e.Rm[0][2] = k2; e.Rm[0][3] = alpha; e.Rm[0][4] = alpha;
     e.Rm[1][0] = k0 * u0; e.Rm[1][1] = k1 * u0 - k2 * rho; e.Rm[1][2] = k2 * u0 + k1 * rho;
     e.Rm[1][3] = alpha * (u0 + k0 * a); e.Rm[1][4] = alpha * (u0 - k0 * a);
@@ -194,60 +200,53 @@ void eigensystem(int nd, int dir, double gamma_m1, double rho, const double* uu,
     e.Rm[2][3] = alpha * (u1 + k1 * a); e.Rm[2][4] = alpha * (u1 - k1 * a);
     e.Rm[3][0] = k0 * u2 - k1 * rho; e.Rm[3][1] = k1 * u2 + k0 * rho; e.Rm[3][2] = k2 * u2;
     e.Rm[3][3] = alpha * (u2 + k2 * a); e.Rm[3][4] = alpha * (u2 - k2 * a);
-    e.Rm[4][0] = k0 * phi_sq / gamma_m1 + rho * (k2 * u1 - k1 * u2);
-    e.Rm[4][1] = k1 * phi_sq / gamma_m1 + rho * (k0 * u2 - k2 * u0);
-    e.Rm[4][2] = k2 * phi_sq / gamma_m1 + rho * (k1 * u0 - k0 * u1);
-    e.Rm[4][3] = alpha * ((phi_sq + a2) / gamma_m1 + theta * a);
-    e.Rm[4][4] = alpha * ((phi_sq + a2) / gamma_m1 - theta * a);
-
-    e.L[0][0] = k0 * (1 - phi_sq / a2) - (k2 * u1 - k1 * u2) / rho;
-    e.L[0][1] = k0 * gamma_m1 * u0 / a2;
-    e.L[0][2] = k0 * gamma_m1 * u1 / a2 + k2 / rho;
-    e.L[0][3] = k0 * gamma_m1 * u2 / a2 - k1 / rho;
-    e.L[0][4] = -k0 * gamma_m1 / a2;
-    e.L[1][0] = k1 * (1 - phi_sq / a2) - (k0 * u2 - k2 * u0) / rho;
-    e.L[1][1] = k1 * gamma_m1 * u0 / a2 - k2 / rho;
-    e.L[1][2] = k1 * gamma_m1 * u1 / a2;
-    e.L[1][3] = k1 * gamma_m1 * u2 / a2 + k0 / rho;
-    e.L[1][4] = -k1 * gamma_m1 / a2;
-    e.L[2][0] = k2 * (1 - phi_sq / a2) - (k1 * u0 - k0 * u1) / rho;
-    e.L[2][1] = k2 * gamma_m1 * u0 / a2 + k1 / rho;
-    e.L[2][2] = k2 * gamma_m1 * u1 / a2 - k0 / rho;
-    e.L[2][3] = k2 * gamma_m1 * u2 / a2;
-    e.L[2][4] = -k2 * gamma_m1 / a2;
+    e.Rm[4][0] = k0 * phi_sq * igm1 + rho * (k2 * u1 - k1 * u2);
+    e.Rm[4][1] = k1 * phi_sq * igm1 + rho * (k0 * u2 - k2 * u0);
+    e.Rm[4][2] = k2 * phi_sq * igm1 + rho * (k1 * u0 - k0 * u1);
+    e.Rm[4][3] = alpha * ((phi_sq + a2) * igm1 + theta * a);
+    e.Rm[4][4] = alpha * ((phi_sq + a2) * igm1 - theta * a);
+    const double c1 = 1.0 - phi_sq * ia2;
+    e.L[0][0] = k0 * c1 - (k2 * u1 - k1 * u2) * irho;
+    e.L[0][1] = k0 * gm1 * u0 * ia2;
+    e.L[0][2] = k0 * gm1 * u1 * ia2 + k2 * irho;
+    e.L[0][3] = k0 * gm1 * u2 * ia2 - k1 * irho;
+    e.L[0][4] = -k0 * gm1 * ia2;
+    e.L[1][0] = k1 * c1 - (k0 * u2 - k2 * u0) * irho;
+    e.L[1][1] = k1 * gm1 * u0 * ia2 - k2 * irho;
+    e.L[1][2] = k1 * gm1 * u1 * ia2;
+    e.L[1][3] = k1 * gm1 * u2 * ia2 + k0 * irho;
+    e.L[1][4] = -k1 * gm1 * ia2;
+    e.L[2][0] = k2 * c1 - (k1 * u0 - k0 * u1) * irho;
+    e.L[2][1] = k2 * gm1 * u0 * ia2 + k1 * irho;
+    e.L[2][2] = k2 * gm1 * u1 * ia2 - k0 * irho;
+    e.L[2][3] = k2 * gm1 * u2 * ia2;
+    e.L[2][4] = -k2 * gm1 * ia2;
     e.L[3][0] = bta * (phi_sq - theta * a);
-    e.L[3][1] = -bta * (gamma_m1 * u0 - k0 * a);
-    e.L[3][2] = -bta * (gamma_m1 * u1 - k1 * a);
-    e.L[3][3] = -bta * (gamma_m1 * u2 - k2 * a);
-    e.L[3][4] = bta * gamma_m1;
+    e.L[3][1] = -bta * (gm1 * u0 - k0 * a);
+    e.L[3][2] = -bta * (gm1 * u1 - k1 * a);
+    e.L[3][3] = -bta * (gm1 * u2 - k2 * a);
+    e.L[3][4] = bta * gm1;
     e.L[4][0] = bta * (phi_sq + theta * a);
-    e.L[4][1] = -bta * (gamma_m1 * u0 + k0 * a);
-    e.L[4][2] = -bta * (gamma_m1 * u1 + k1 * a);
-    e.L[4][3] = -bta * (gamma_m1 * u2 + k2 * a);
-    e.L[4][4] = bta * gamma_m1;
+    e.L[4][1] = -bta * (gm1 * u0 + k0 * a);
+    e.L[4][2] = -bta * (gm1 * u1 + k1 * a);
+    e.L[4][3] = -bta * (gm1 * u2 + k2 * a);
+    e.L[4][4] = bta * gm1;
 }
 
 // ---------------------------------------------------------------------------
 // Reconstructions.  f[0..5] hold the split flux at offsets -2..3 from point i;
 // the interface is i+1/2.  Right (f+) uses offsets -2..2, left (f-) -1..3
 // (opensbli/schemes/spatial/weno.py:123-136; teno.py:95-107).
+// Fused multiply-adds appear only where written as std::fma.
 // ---------------------------------------------------------------------------
 
-// Jiang-Shu smoothness of a 3-point stencil whose points are (a, b, c) in
-// ascending x, measured on the cell of the stencil's first point for
-// 'first' (3a-4b+c) or last point for 'last' (a-4b+3c), or centred (a-c).
-inline double js_first(double a, double b, double c) {
-    const double s1 = a - 2 * b + c, s2 = 3 * a - 4 * b + c;
-    return (13.0 / 12.0) * s1 * s1 + (1.0 / 4.0) * s2 * s2;
-}
-inline double js_centre(double a, double b, double c) {
-    const double s1 = a - 2 * b + c, s2 = a - c;
-    return (13.0 / 12.0) * s1 * s1 + (1.0 / 4.0) * s2 * s2;
-}
-inline double js_last(double a, double b, double c) {
-    const double s1 = a - 2 * b + c, s2 = a - 4 * b + 3 * c;
-    return (13.0 / 12.0) * s1 * s1 + (1.0 / 4.0) * s2 * s2;
-}
+// Jiang-Shu smoothness of a 3-point stencil (a, b, c in ascending x):
+// 13/12 (a-2b+c)^2 + 1/4 s2^2 with s2 = 3a-4b+c ('first'), a-c ('centre'),
+// a-4b+3c ('last') -- the k=3 coefficients of JS_smoothness (weno.py:139-206).
+inline double js_sum(double s1, double s2) { return std::fma((13.0 / 12.0) * s1, s1, (0.25 * s2) * s2); }
+inline double js_first(double a, double b, double c) { return js_sum(a - 2.0 * b + c, 3.0 * a - 4.0 * b + c); }
+inline double js_centre(double a, double b, double c) { return js_sum(a - 2.0 * b + c, a - c); }
+inline double js_last(double a, double b, double c) { return js_sum(a - 2.0 * b + c, a - 4.0 * b + 3.0 * c); }
 
 // Shu (1997) eq. (2.21) ENO coefficients c_rj (weno.py:71-99), evaluated in
 // exact integer arithmetic for k = 3.  'dshift' is 1 for the right (side +1)
@@ -301,101 +300,107 @@ WenoCoeffs make_weno5() {
 }
 const WenoCoeffs W5 = make_weno5();
 
-// WENO-Z (weno.py:290-345): tau5 = |b0 - b2|, alpha_r = d_r*(1 + (tau/(eps+b_r))**2).
+// three-point ENO candidate, points in ascending x
+inline double eno3(const double* c, double x0, double x1, double x2) {
+    return std::fma(c[2], x2, std::fma(c[1], x1, c[0] * x0));
+}
+// alpha_r = d_r*(1 + (tau/(eps+b_r))**2), printed expanded as d_r + d_r*tau^2/((eps+b_r)(eps+b_r))
+inline double zalpha(double d, double t2, double b, double eps) {
+    const double eb = eps + b;
+    return d + (d * t2) / (eb * eb);
+}
+
+// WENO-Z (weno.py:290-345): tau5 = |b0 - b2|.
 double wenoz_right(const double* f, double eps) {
-    // points: f[0]=i-2 ... f[4]=i+2 ; stencils r=0 -> (i,i+1,i+2), r=1 -> (i-1,i,i+1), r=2 -> (i-2,i-1,i)
+    // points f[0]=i-2 ... f[4]=i+2 ; r=0 -> (i,i+1,i+2), r=1 -> (i-1,i,i+1), r=2 -> (i-2,i-1,i)
     const double b0 = js_first(f[2], f[3], f[4]);
     const double b1 = js_centre(f[1], f[2], f[3]);
     const double b2 = js_last(f[0], f[1], f[2]);
-    const double tau = std::fabs(b0 - b2);
+    const double tau = std::fabs(b0 - b2), t2 = tau * tau;
     const double* d = W5.dopt[0];
-    const double a0 = d[0] + d[0] * tau * tau / ((eps + b0) * (eps + b0));
-    const double a1 = d[1] + d[1] * tau * tau / ((eps + b1) * (eps + b1));
-    const double a2 = d[2] + d[2] * tau * tau / ((eps + b2) * (eps + b2));
+    const double a0 = zalpha(d[0], t2, b0, eps), a1 = zalpha(d[1], t2, b1, eps), a2 = zalpha(d[2], t2, b2, eps);
     const double inv = 1.0 / (a0 + a1 + a2);
-    const double e0 = W5.c[0][0][0] * f[2] + W5.c[0][0][1] * f[3] + W5.c[0][0][2] * f[4];
-    const double e1 = W5.c[0][1][0] * f[1] + W5.c[0][1][1] * f[2] + W5.c[0][1][2] * f[3];
-    const double e2 = W5.c[0][2][0] * f[0] + W5.c[0][2][1] * f[1] + W5.c[0][2][2] * f[2];
-    return (a0 * inv) * e0 + (a1 * inv) * e1 + (a2 * inv) * e2;
+    const double e0 = eno3(W5.c[0][0], f[2], f[3], f[4]);
+    const double e1 = eno3(W5.c[0][1], f[1], f[2], f[3]);
+    const double e2 = eno3(W5.c[0][2], f[0], f[1], f[2]);
+    return std::fma(a2 * inv, e2, std::fma(a1 * inv, e1, (a0 * inv) * e0));
 }
 double wenoz_left(const double* f, double eps) {
-    // points: f[1]=i-1 ... f[5]=i+3 ; r=0 -> (i+1,i+2,i+3), r=1 -> (i,i+1,i+2), r=2 -> (i-1,i,i+1)
+    // points f[1]=i-1 ... f[5]=i+3 ; r=0 -> (i+1,i+2,i+3), r=1 -> (i,i+1,i+2), r=2 -> (i-1,i,i+1)
     const double b0 = js_first(f[3], f[4], f[5]);
     const double b1 = js_centre(f[2], f[3], f[4]);
     const double b2 = js_last(f[1], f[2], f[3]);
-    const double tau = std::fabs(b0 - b2);
+    const double tau = std::fabs(b0 - b2), t2 = tau * tau;
     const double* d = W5.dopt[1];
-    const double a0 = d[0] + d[0] * tau * tau / ((eps + b0) * (eps + b0));
-    const double a1 = d[1] + d[1] * tau * tau / ((eps + b1) * (eps + b1));
-    const double a2 = d[2] + d[2] * tau * tau / ((eps + b2) * (eps + b2));
+    const double a0 = zalpha(d[0], t2, b0, eps), a1 = zalpha(d[1], t2, b1, eps), a2 = zalpha(d[2], t2, b2, eps);
     const double inv = 1.0 / (a0 + a1 + a2);
-    const double e0 = W5.c[1][0][0] * f[3] + W5.c[1][0][1] * f[4] + W5.c[1][0][2] * f[5];
-    const double e1 = W5.c[1][1][0] * f[2] + W5.c[1][1][1] * f[3] + W5.c[1][1][2] * f[4];
-    const double e2 = W5.c[1][2][0] * f[1] + W5.c[1][2][1] * f[2] + W5.c[1][2][2] * f[3];
-    return (a0 * inv) * e0 + (a1 * inv) * e1 + (a2 * inv) * e2;
+    const double e0 = eno3(W5.c[1][0], f[3], f[4], f[5]);
+    const double e1 = eno3(W5.c[1][1], f[2], f[3], f[4]);
+    const double e2 = eno3(W5.c[1][2], f[1], f[2], f[3]);
+    return std::fma(a2 * inv, e2, std::fma(a1 * inv, e1, (a0 * inv) * e0));
 }
 
 // TENO5 / TENO6 (teno.py:59-217, 419-477).
 struct TenoParams { int order; double eps, ct; };
+// (C + tau/beta)**6 printed as six repeated multiplies (opsc.py:222-241)
 inline double pow6(double x) { return x * x * x * x * x * x; }
 
 double teno_side(const double* f, int side, const TenoParams& tp) {
-    // f indexed by offset: F(o) = f[o+2], o in -2..3
     auto F = [&](int o) { return f[o + 2]; };
     double beta[4], eno[4], dopt[4];
-    int ns = tp.order == 5 ? 3 : 4;
-    if (tp.order == 5) { dopt[0] = 11.0 / 20.0; dopt[1] = 4.0 / 10.0; dopt[2] = 1.0 / 20.0; }
+    const int ns = tp.order == 5 ? 3 : 4;
+    if (tp.order == 5) { dopt[0] = 11.0 / 20.0; dopt[1] = 4.0 / 10.0; dopt[2] = 1.0 / 20.0; dopt[3] = 0.0; }
     else { dopt[0] = 231.0 / 500.0; dopt[1] = 3.0 / 10.0; dopt[2] = 27.0 / 500.0; dopt[3] = 23.0 / 125.0; }
     if (side == +1) {
-        // stencils [-1,0,1], [0,1,2], [-2,-1,0], [0,1,2,3]
-        beta[0] = (1.0 / 4.0) * (F(-1) - F(1)) * (F(-1) - F(1)) + (13.0 / 12.0) * (F(-1) - 2 * F(0) + F(1)) * (F(-1) - 2 * F(0) + F(1));
-        beta[1] = (1.0 / 4.0) * (3 * F(0) - 4 * F(1) + F(2)) * (3 * F(0) - 4 * F(1) + F(2)) + (13.0 / 12.0) * (F(0) - 2 * F(1) + F(2)) * (F(0) - 2 * F(1) + F(2));
-        beta[2] = (1.0 / 4.0) * (F(-2) - 4 * F(-1) + 3 * F(0)) * (F(-2) - 4 * F(-1) + 3 * F(0)) + (13.0 / 12.0) * (F(-2) - 2 * F(-1) + F(0)) * (F(-2) - 2 * F(-1) + F(0));
-        eno[0] = (-1.0 / 6.0) * F(-1) + (5.0 / 6.0) * F(0) + (2.0 / 6.0) * F(1);
-        eno[1] = (2.0 / 6.0) * F(0) + (5.0 / 6.0) * F(1) + (-1.0 / 6.0) * F(2);
-        eno[2] = (2.0 / 6.0) * F(-2) + (-7.0 / 6.0) * F(-1) + (11.0 / 6.0) * F(0);
+        // stencils [-1,0,1], [0,1,2], [-2,-1,0], [0,1,2,3]  (teno.py:95-99, 110-116, 149-169)
+        beta[0] = js_sum(F(-1) - 2.0 * F(0) + F(1), F(-1) - F(1));
+        beta[1] = js_sum(F(0) - 2.0 * F(1) + F(2), 3.0 * F(0) - 4.0 * F(1) + F(2));
+        beta[2] = js_sum(F(-2) - 2.0 * F(-1) + F(0), F(-2) - 4.0 * F(-1) + 3.0 * F(0));
+        eno[0] = std::fma(2.0 / 6.0, F(1), std::fma(5.0 / 6.0, F(0), (-1.0 / 6.0) * F(-1)));
+        eno[1] = std::fma(-1.0 / 6.0, F(2), std::fma(5.0 / 6.0, F(1), (2.0 / 6.0) * F(0)));
+        eno[2] = std::fma(11.0 / 6.0, F(0), std::fma(-7.0 / 6.0, F(-1), (2.0 / 6.0) * F(-2)));
         if (ns == 4) {
-            const double s1 = -11 * F(0) + 18 * F(1) - 9 * F(2) + 2 * F(3);
-            const double s2 = 2 * F(0) - 5 * F(1) + 4 * F(2) - F(3);
-            const double s3 = -F(0) + 3 * F(1) - 3 * F(2) + F(3);
+            const double s1 = -11.0 * F(0) + 18.0 * F(1) - 9.0 * F(2) + 2.0 * F(3);
+            const double s2 = 2.0 * F(0) - 5.0 * F(1) + 4.0 * F(2) - F(3);
+            const double s3 = -F(0) + 3.0 * F(1) - 3.0 * F(2) + F(3);
             // the right-side beta_3 carries the 781/720 term unsquared (teno.py:168-169)
-            beta[3] = (1.0 / 36.0) * s1 * s1 + (13.0 / 12.0) * s2 * s2 + (781.0 / 720.0) * s3;
-            eno[3] = (3.0 / 12.0) * F(0) + (13.0 / 12.0) * F(1) + (-5.0 / 12.0) * F(2) + (1.0 / 12.0) * F(3);
+            beta[3] = std::fma((1.0 / 36.0) * s1, s1, std::fma((13.0 / 12.0) * s2, s2, (781.0 / 720.0) * s3));
+            eno[3] = std::fma(1.0 / 12.0, F(3), std::fma(-5.0 / 12.0, F(2), std::fma(13.0 / 12.0, F(1), (3.0 / 12.0) * F(0))));
         }
     } else {
-        // stencils [0,1,2], [-1,0,1], [1,2,3], [-2,-1,0,1]
-        beta[0] = (1.0 / 4.0) * (F(0) - F(2)) * (F(0) - F(2)) + (13.0 / 12.0) * (F(0) - 2 * F(1) + F(2)) * (F(0) - 2 * F(1) + F(2));
-        beta[1] = (1.0 / 4.0) * (F(-1) - 4 * F(0) + 3 * F(1)) * (F(-1) - 4 * F(0) + 3 * F(1)) + (13.0 / 12.0) * (F(-1) - 2 * F(0) + F(1)) * (F(-1) - 2 * F(0) + F(1));
-        beta[2] = (1.0 / 4.0) * (3 * F(1) - 4 * F(2) + F(3)) * (3 * F(1) - 4 * F(2) + F(3)) + (13.0 / 12.0) * (F(1) - 2 * F(2) + F(3)) * (F(1) - 2 * F(2) + F(3));
-        eno[0] = (2.0 / 6.0) * F(0) + (5.0 / 6.0) * F(1) + (-1.0 / 6.0) * F(2);
-        eno[1] = (-1.0 / 6.0) * F(-1) + (5.0 / 6.0) * F(0) + (2.0 / 6.0) * F(1);
-        eno[2] = (11.0 / 6.0) * F(1) + (-7.0 / 6.0) * F(2) + (1.0 / 3.0) * F(3);
+        // stencils [0,1,2], [-1,0,1], [1,2,3], [-2,-1,0,1]  (teno.py:100-107, 117-123, 170-180)
+        beta[0] = js_sum(F(0) - 2.0 * F(1) + F(2), F(0) - F(2));
+        beta[1] = js_sum(F(-1) - 2.0 * F(0) + F(1), F(-1) - 4.0 * F(0) + 3.0 * F(1));
+        beta[2] = js_sum(F(1) - 2.0 * F(2) + F(3), 3.0 * F(1) - 4.0 * F(2) + F(3));
+        eno[0] = std::fma(-1.0 / 6.0, F(2), std::fma(5.0 / 6.0, F(1), (2.0 / 6.0) * F(0)));
+        eno[1] = std::fma(2.0 / 6.0, F(1), std::fma(5.0 / 6.0, F(0), (-1.0 / 6.0) * F(-1)));
+        eno[2] = std::fma(1.0 / 3.0, F(3), std::fma(-7.0 / 6.0, F(2), (11.0 / 6.0) * F(1)));
         if (ns == 4) {
-            const double s1 = -2 * F(-2) + 9 * F(-1) - 18 * F(0) + 11 * F(1);
-            const double s2 = -1 * F(-2) + 4 * F(-1) - 5 * F(0) + 2 * F(1);
-            const double s3 = -1 * F(-2) + 3 * F(-1) - 3 * F(0) + F(1);
-            beta[3] = (1.0 / 36.0) * s1 * s1 + (13.0 / 12.0) * s2 * s2 + (781.0 / 720.0) * s3 * s3;
-            eno[3] = (1.0 / 12.0) * F(-2) + (-5.0 / 12.0) * F(-1) + (13.0 / 12.0) * F(0) + (3.0 / 12.0) * F(1);
+            const double s1 = -2.0 * F(-2) + 9.0 * F(-1) - 18.0 * F(0) + 11.0 * F(1);
+            const double s2 = -F(-2) + 4.0 * F(-1) - 5.0 * F(0) + 2.0 * F(1);
+            const double s3 = -F(-2) + 3.0 * F(-1) - 3.0 * F(0) + F(1);
+            beta[3] = std::fma((1.0 / 36.0) * s1, s1, std::fma((13.0 / 12.0) * s2, s2, ((781.0 / 720.0) * s3) * s3));
+            eno[3] = std::fma(3.0 / 12.0, F(1), std::fma(13.0 / 12.0, F(0), std::fma(-5.0 / 12.0, F(-1), (1.0 / 12.0) * F(-2))));
         }
     }
     double tau;
-    if (ns == 3) tau = std::fabs(beta[0] - beta[2]);                                         // Teno5 (:210)
-    else tau = std::fabs(beta[3] - (1.0 / 6.0) * (beta[0] + beta[2] - 4 * beta[1]));           // Teno6 (:228)
+    if (ns == 3) tau = std::fabs(beta[0] - beta[2]);                                          // Teno5 (:210)
+    else tau = std::fabs(beta[3] - (1.0 / 6.0) * (beta[0] + beta[2] - 4.0 * beta[1]));         // Teno6 (:228)
     double alpha[4], asum = 0.0;
     for (int r = 0; r < ns; ++r) {
         const double inv_beta = 1.0 / (tp.eps + beta[r]);
-        alpha[r] = pow6(1.0 + tau * inv_beta);                                                 // (C + tau/beta)**6
+        alpha[r] = pow6(1.0 + tau * inv_beta);                                                  // (C + tau/beta)**6
         asum += alpha[r];
     }
     const double inv_asum = 1.0 / asum;
     double delta[4], wsum = 0.0;
     for (int r = 0; r < ns; ++r) {
-        delta[r] = (alpha[r] * inv_asum < tp.ct) ? 0.0 : 1.0;                                 // cut-off (:457-477)
+        delta[r] = (alpha[r] * inv_asum < tp.ct) ? 0.0 : 1.0;                                  // cut-off (:457-477)
         wsum += dopt[r] * delta[r];
     }
     const double inv_wsum = 1.0 / wsum;
     double rec = 0.0;
-    for (int r = 0; r < ns; ++r) rec += dopt[r] * delta[r] * inv_wsum * eno[r];
+    for (int r = 0; r < ns; ++r) rec = std::fma(dopt[r] * delta[r] * inv_wsum, eno[r], rec);
     return rec;
 }
 
@@ -472,16 +477,16 @@ void reconstruct_direction(osb_ctx* c, int dir) {
             double fp[6], fm[6];
             for (int s = 0; s < 6; ++s) {
                 double cs = 0.0, cf = 0.0;
-                for (int m = 0; m < nv; ++m) { cs += e.L[kk][m] * qv[s][m]; cf += e.L[kk][m] * F[s][m]; }
-                fp[s] = 0.5 * (cf + lam[kk] * cs);
-                fm[s] = 0.5 * (cf - lam[kk] * cs);
+                for (int m = 0; m < nv; ++m) { cs = std::fma(e.L[kk][m], qv[s][m], cs); cf = std::fma(e.L[kk][m], F[s][m], cf); }
+                fp[s] = 0.5 * std::fma(lam[kk], cs, cf);   // f+ = (CF + lambda CS)/2
+                fm[s] = 0.5 * std::fma(-lam[kk], cs, cf);  // f- = (CF - lambda CS)/2
             }
             if (teno) recon[kk] = teno_side(fp, +1, tp) + teno_side(fm, -1, tp);
             else recon[kk] = wenoz_right(fp, weps) + wenoz_left(fm, weps);
         }
         for (int v = 0; v < nv; ++v) {
             double s = 0.0;
-            for (int kk = 0; kk < nv; ++kk) s += e.Rm[v][kk] * recon[kk];
+            for (int kk = 0; kk < nv; ++kk) s = std::fma(e.Rm[v][kk], recon[kk], s);
             wk[(size_t)v * c->Ptot + x0] = s;
         }
     });
@@ -510,11 +515,11 @@ void shock_residual(osb_ctx* c) {
 // ---------------------------------------------------------------------------
 template <class G>
 inline double d1(G f, double invd) {
-    return ((1.0 / 12.0) * (f(-2) - f(2)) + (2.0 / 3.0) * (f(1) - f(-1))) * invd;
+    return std::fma(1.0 / 12.0, f(-2) - f(2), (2.0 / 3.0) * (f(1) - f(-1))) * invd;
 }
 template <class G>
 inline double d2(G f, double inv2d) {
-    return ((-1.0 / 12.0) * (f(-2) + f(2)) + (4.0 / 3.0) * (f(-1) + f(1)) - (5.0 / 2.0) * f(0)) * inv2d;
+    return std::fma(-1.0 / 12.0, f(-2) + f(2), std::fma(4.0 / 3.0, f(-1) + f(1), (-5.0 / 2.0) * f(0))) * inv2d;
 }
 
 // Central convective residual, written first (StoreSome.py:138-228; scheme.py:321-350).
@@ -938,8 +943,8 @@ int osb_step(osb_ctx* c, int nsteps) {
                     const size_t x = c->I(i, j, k);
                     for (int v = 0; v < nv; ++v) {
                         const size_t y = (size_t)v * Pt + x;
-                        c->rkold[y] = rkA[s] * c->rkold[y] + dt * c->R[y];
-                        c->q[y] = rkB[s] * c->rkold[y] + c->q[y];
+                        c->rkold[y] = std::fma(rkA[s], c->rkold[y], dt * c->R[y]);
+                        c->q[y] = std::fma(rkB[s], c->rkold[y], c->q[y]);
                     }
                 });
                 apply_bcs(c);
@@ -958,8 +963,8 @@ int osb_step(osb_ctx* c, int nsteps) {
                     const size_t x = c->I(i, j, k);
                     for (int v = 0; v < nv; ++v) {
                         const size_t y = (size_t)v * Pt + x;
-                        c->q[y] = c->rkold[y] + dt * rknew_c[s] * c->R[y];
-                        c->rkold[y] = c->rkold[y] + dt * rkold_c[s] * c->R[y];
+                        c->q[y] = std::fma(dt * rknew_c[s], c->R[y], c->rkold[y]);
+                        c->rkold[y] = std::fma(dt * rkold_c[s], c->R[y], c->rkold[y]);
                     }
                 });
                 apply_bcs(c);
