@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the OpenSBLI explicit-solver hot path on one MI355X.
+
+A "step" is one RK3 time step (three RK stages) over the whole grid with the
+state resident in HBM.  value = grid-point RK-stage updates per second
+(GPts*stage/s) = npoints * 3 * K / t.  Default workload: the 3-D viscous
+Taylor-Green vortex at 128^3 (Ma=0.1, Re=1600, Pr=0.71), 4th-order central
+Feiereisen split form + constant-viscosity terms, RK3-LS dt=5e-3, FP64.
+
+The roofline entry uses the library's own HIP-event timing of its dominant
+kernel family over the timed region (events recorded on the context's
+stream); the CPU baseline runs the oracle on a bounded sample of the same
+workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GPts·stage/s FP64 vs mixed-prec at stated L2 error; ensemble p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+# compulsory bytes per grid-point RK stage: read q, read du, write du, write q (FP64 state)
+BYTES_PER_PT_STAGE = {1: 96.0, 2: 128.0, 3: 160.0}
+
+
+def make_problem(name, precision):
+    from opensbli_amd import native as nat, problems as P
+    prec = nat.PREC_MIXED if precision == "mixed" else nat.PREC_FP64
+    if name == "tgv_central_128":
+        return P.tgv(n=128, precision=prec)
+    if name == "tgv_teno5_256":
+        return P.tgv(n=256, steps=100, scheme="teno", minf=1.25, cfl=0.5, dt=0.0, teno_ct=1e-5, precision=prec)
+    if name == "sod_1024":
+        return P.sod(precision=prec)
+    if name.startswith("kh_"):
+        return P.kelvin_helmholtz(n=int(name.split("_")[1]), precision=prec)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def ensure_built():
+    from opensbli_amd import native as nat
+    if not os.path.exists(nat.HIP_LIBRARY):
+        import subprocess
+        subprocess.check_call(["make", "-C", os.path.join(ROOT, "opensbli_amd", "csrc")])
+    oracle = os.path.join(ROOT, "oracle", "libosb_oracle.so")
+    if not os.path.exists(oracle):
+        import subprocess
+        subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle")])
+    return oracle
+
+
+def time_gpu(problem, steps, warmup):
+    import torch
+    from opensbli_amd import native as nat
+    hip = nat.load_hip()
+    with hip.context(problem.desc) as c:
+        c.set_state(problem.q0)
+        c.step(warmup)
+        c.sync()
+        c.profile_enable(True)
+        torch.cuda.synchronize()
+        c.sync()
+        t0 = time.perf_counter()
+        c.step(steps)
+        c.sync()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        prof = c.profile_read()
+        c.profile_enable(False)
+        state = c.get_state()
+    return t1 - t0, prof, state
+
+
+def cpu_baseline(oracle_path, problem, budget_s=15.0):
+    from opensbli_amd import native as nat
+    lib = nat.Library(oracle_path)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    with lib.context(problem.desc) as c:
+        c.set_state(problem.q0)
+        t0 = time.perf_counter()
+        c.step(1)
+        t1 = time.perf_counter()
+        per = max(t1 - t0, 1e-6)
+        n = max(1, min(problem.steps, int(budget_s / per)))
+        t0 = time.perf_counter()
+        c.step(n)
+        t1 = time.perf_counter()
+    value = problem.desc.npoints * 3.0 * n / (t1 - t0) / 1e9
+    return {"value": value, "unit": "GPts·stage/s", "cores": cores, "kind": "port",
+            "sample": f"{problem.name}: {n} of the workload's steps after 1 warm-up step, oracle OpenMP build, {cores} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="tgv_central_128")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the timed run, copy a 256 MiB FP64 buffer 5 times (PMC byte-counter calibration)")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
+              "the multi-GPU decomposition is design-only in this build.", file=sys.stderr)
+        sys.exit(2)
+
+    oracle_path = ensure_built()
+    problem = make_problem(args.workload, args.precision)
+    dt_wall, prof, state = time_gpu(problem, args.steps, args.warmup)
+    import numpy as np
+    if not np.all(np.isfinite(state)):
+        raise SystemExit("non-finite state after the timed steps")
+    npts = problem.desc.npoints
+    value = npts * 3.0 * args.steps / dt_wall / 1e9
+
+    # roofline of the dominant kernel family: algorithmic bytes of one RK stage over
+    # the grid (read q, read du, write du, write q) per launch of a per-stage kernel
+    fam, (ms, nlaunch) = max(prof.items(), key=lambda kv: kv[1][0])
+    avg_s = ms / 1e3 / max(nlaunch, 1)
+    ndim = problem.desc.ndim
+    bytes_alg = BYTES_PER_PT_STAGE[ndim] * npts
+    achieved = bytes_alg / avg_s / 1e9
+    stage_s = dt_wall / (3.0 * args.steps)
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "kernel": fam, "kernel_avg_us": avg_s * 1e6, "kernel_launches": nlaunch,
+                "bytes_alg_per_launch": bytes_alg,
+                "stage_achieved_GBs": bytes_alg / stage_s / 1e9}
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "GPts·stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt_wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f64-state/f32-flux",
+        "data": "synthetic (analytic initial field)",
+        "config": {"workload": problem.name, "grid": list(problem.desc.n), "precision": args.precision,
+                   "scheme": args.workload.split("_")[1], "steps_timed": args.steps},
+        "roofline": roofline,
+        "kernel_ms": {k: v[0] for k, v in prof.items() if v[1]},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(oracle_path, problem, args.cpu_budget)
+    if args.copy_probe:
+        import torch
+        a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        out["copy_probe_bytes"] = a.numel() * 8
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

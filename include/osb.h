@@ -143,6 +143,14 @@ int  osb_tgv_diagnostics(osb_ctx* ctx, double out[4]);
  * returns OSB_E_NONFINITE if any interior rho is not finite.                */
 int  osb_nan_check(osb_ctx* ctx);
 
+/* Kernel timing (HIP library only).  While enabled, every kernel launch of
+ * the context is bracketed by a pair of hipEvents on the context's stream and
+ * its duration is accumulated per kernel family.  osb_profile_read returns
+ * family `idx` (0-based); OSB_E_INVALID past the last family.  Replaces the
+ * OPS per-kernel timers (ops_timing_output, opsc.py:826-836, 1018-1024).    */
+int  osb_profile_enable(osb_ctx* ctx, int enable);
+int  osb_profile_read(osb_ctx* ctx, int idx, char* name, int name_len, double* total_ms, int64_t* launches);
+
 /* Device buffers (HIP library only): pointer to the FP64 conservative
  * variable `var` (padded layout) for zero-copy interop.                     */
 int  osb_state_device_ptr(osb_ctx* ctx, int var, void** out);

@@ -11,6 +11,7 @@
 // context's stream; nothing synchronises except the calls that return data.
 
 #include "osb_kernels.hpp"
+#include "osb_central_fused.hpp"
 #include "../../include/osb.h"
 
 #include <algorithm>
@@ -66,6 +67,14 @@ struct osb_ctx {
     double last_dt = 0.0;
     bool have_prev = false;
     std::string err;
+
+    // kernel timing (osb_profile_*)
+    bool prof_on = false;
+    struct Rec { int fam; hipEvent_t a, b; };
+    std::vector<Rec> prof_pending;
+    std::vector<hipEvent_t> prof_pool;
+    double prof_ms[16] = {0};
+    int64_t prof_n[16] = {0};
 };
 
 namespace {
@@ -85,6 +94,45 @@ int set_err(osb_ctx* c, int code, const std::string& m) {
 
 inline unsigned nblocks(long work) { return (unsigned)((work + BLOCK - 1) / BLOCK); }
 
+// kernel families reported by osb_profile_read
+enum ProfFamily { PF_FACE_FLUX, PF_FLUX_RESIDUAL, PF_CONSTITUENT, PF_GRADIENTS, PF_CENTRAL, PF_VISCOUS,
+                  PF_RK, PF_BC, PF_REDUCE, PF_CENTRAL_FUSED, PF_COUNT };
+const char* const kFamilyName[PF_COUNT] = {"face_flux", "flux_residual", "constituent", "gradients", "central_convective",
+                                           "viscous", "rk_update", "boundary", "reduction", "central_fused"};
+
+hipEvent_t pool_event(osb_ctx* c) {
+    if (!c->prof_pool.empty()) { hipEvent_t e = c->prof_pool.back(); c->prof_pool.pop_back(); return e; }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+// brackets the launches issued in its scope with events when timing is on
+struct Timed {
+    osb_ctx* c; int fam; hipEvent_t a = nullptr;
+    Timed(osb_ctx* ctx, int f) : c(ctx), fam(f) {
+        if (c->prof_on) { a = pool_event(c); (void)hipEventRecord(a, c->stream); }
+    }
+    ~Timed() {
+        if (c->prof_on && a) {
+            hipEvent_t b = pool_event(c);
+            (void)hipEventRecord(b, c->stream);
+            c->prof_pending.push_back({fam, a, b});
+        }
+    }
+};
+
+void prof_resolve(osb_ctx* c) {
+    for (auto& r : c->prof_pending) {
+        float ms = 0.f;
+        (void)hipEventSynchronize(r.b);
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) { c->prof_ms[r.fam] += ms; c->prof_n[r.fam] += 1; }
+        c->prof_pool.push_back(r.a);
+        c->prof_pool.push_back(r.b);
+    }
+    c->prof_pending.clear();
+}
+
 long interior_points(const Geom& g) { return (long)g.n[0] * g.n[1] * g.n[2]; }
 
 long box_points(const Geom& g, int ext) {
@@ -100,6 +148,7 @@ template <int NDIM, int DIR, typename T>
 void launch_flux_t(osb_ctx* c, const double* q) {
     long work = interior_points(c->g) / c->g.n[DIR] * (long)(c->g.n[DIR] + 1);
     double* F = c->F + (long)DIR * c->nv * c->g.Ptot;
+    Timed tm(c, PF_FACE_FLUX);
     switch (c->sch) {
         case SCH_WENOZ: k_face_flux<NDIM, DIR, SCH_WENOZ, T><<<nblocks(work), BLOCK, 0, c->stream>>>(q, F, c->g, c->c); break;
         case SCH_TENO5: k_face_flux<NDIM, DIR, SCH_TENO5, T><<<nblocks(work), BLOCK, 0, c->stream>>>(q, F, c->g, c->c); break;
@@ -120,19 +169,26 @@ void rhs_nd(osb_ctx* c, const double* q) {
         launch_flux<NDIM, 0>(c, q);
         if (NDIM > 1) launch_flux<NDIM, (NDIM > 1 ? 1 : 0)>(c, q);
         if (NDIM > 2) launch_flux<NDIM, (NDIM > 2 ? 2 : 0)>(c, q);
+        Timed tm(c, PF_FLUX_RESIDUAL);
         k_flux_residual<NDIM><<<nblocks(nin), BLOCK, 0, c->stream>>>(c->F, c->R, c->g, c->c);
     }
     if (c->central || c->viscous) {
-        k_constituent<NDIM><<<nblocks(box_points(c->g, 2)), BLOCK, 0, c->stream>>>(q, c->cr, c->g, c->c, 2);
+        {
+            Timed tm(c, PF_CONSTITUENT);
+            k_constituent<NDIM><<<nblocks(box_points(c->g, 2)), BLOCK, 0, c->stream>>>(q, c->cr, c->g, c->c, 2);
+        }
+        Timed tm(c, PF_GRADIENTS);
         k_gradients<NDIM><<<nblocks(box_points(c->g, 2)), BLOCK, 0, c->stream>>>(c->cr, c->gr, c->g, c->c);
     }
     if (c->central) {
+        Timed tm(c, PF_CENTRAL);
         if (c->d.conv_scheme == OSB_CONV_CENTRAL_SKEW)
             k_central_convective<NDIM, true><<<nblocks(nin), BLOCK, 0, c->stream>>>(q, c->cr, c->gr, c->R, c->g, c->c);
         else
             k_central_convective<NDIM, false><<<nblocks(nin), BLOCK, 0, c->stream>>>(q, c->cr, c->gr, c->R, c->g, c->c);
     }
     if (c->viscous) {
+        Timed tm(c, PF_VISCOUS);
         if (c->d.viscous == OSB_VISC_SUTHERLAND)
             k_viscous<NDIM, true><<<nblocks(nin), BLOCK, 0, c->stream>>>(c->cr, c->gr, c->R, c->g, c->c);
         else
@@ -149,6 +205,7 @@ void rhs(osb_ctx* c, const double* q) {
 }
 
 void apply_bcs(osb_ctx* c, double* q) {
+    Timed tm(c, PF_BC);
     const Geom& g = c->g;
     for (int dir = 0; dir < g.ndim; ++dir) {
         long work = 2L * (g.halo + 1);
@@ -162,6 +219,7 @@ void apply_bcs(osb_ctx* c, double* q) {
 }
 
 void launch_wavespeed(osb_ctx* c, const double* q) {
+    Timed tm(c, PF_REDUCE);
     k_zero_scalar<<<1, 64, 0, c->stream>>>(c->scal);
     const long nin = interior_points(c->g);
     const unsigned nb = std::min<unsigned>(nblocks(nin), 2048u);
@@ -171,6 +229,39 @@ void launch_wavespeed(osb_ctx* c, const double* q) {
         default: k_wavespeed<3><<<nb, BLOCK, 0, c->stream>>>(q, c->scal, c->g, c->c); break;
     }
 }
+
+template <int CONV, int VISC, bool FLUXRES, int TIME>
+void launch_fused_t(osb_ctx* c, const double* q, double* qn, double A, double B) {
+    const Geom& g = c->g;
+    dim3 grid((g.n[0] + CF_TX - 1) / CF_TX, (g.n[1] + CF_TY - 1) / CF_TY, (g.n[2] + CF_TZ - 1) / CF_TZ);
+    k_central_fused<CONV, VISC, FLUXRES, TIME><<<grid, CF_THREADS, 0, c->stream>>>(q, c->F, c->du, qn, c->scal, g, c->c, A, B);
+}
+
+template <int CONV, bool FLUXRES, int TIME>
+void launch_fused_v(osb_ctx* c, const double* q, double* qn, double A, double B) {
+    switch (c->d.viscous) {
+        case OSB_VISC_NONE: if (!FLUXRES) launch_fused_t<CONV, 0, FLUXRES, TIME>(c, q, qn, A, B); break;
+        case OSB_VISC_CONSTANT: launch_fused_t<CONV, 1, FLUXRES, TIME>(c, q, qn, A, B); break;
+        default: launch_fused_t<CONV, 2, FLUXRES, TIME>(c, q, qn, A, B); break;
+    }
+}
+
+// One RK stage through the fused kernel: 3-D central schemes, and 3-D
+// shock-capturing schemes with viscous terms (face fluxes first).
+template <int TIME>
+void fused_stage(osb_ctx* c, const double* q, double* qn, double A, double B) {
+    if (c->shock) {
+        launch_flux<3, 0>(c, q);
+        launch_flux<3, 1>(c, q);
+        launch_flux<3, 2>(c, q);
+    }
+    Timed tm(c, PF_CENTRAL_FUSED);
+    if (c->shock) launch_fused_v<CF_NONE, true, TIME>(c, q, qn, A, B);
+    else if (c->d.conv_scheme == OSB_CONV_CENTRAL_SKEW) launch_fused_v<CF_SKEW, false, TIME>(c, q, qn, A, B);
+    else launch_fused_v<CF_FEIEREISEN, false, TIME>(c, q, qn, A, B);
+}
+
+bool use_fused(const osb_ctx* c) { return c->g.ndim == 3 && (c->central || c->viscous); }
 
 int check_launch(osb_ctx* c) {
     hipError_t e = hipGetLastError();
@@ -189,9 +280,14 @@ int do_step(osb_ctx* c) {
         const double rkA[3] = {0, (-5.0 / 9.0), (-153.0 / 128.0)};
         const double rkB[3] = {(1.0 / 3.0), (15.0 / 16.0), (8.0 / 15.0)};
         for (int s = 0; s < 3; ++s) {
-            rhs(c, c->q_cur);
             double* qn = (c->q_cur == c->qa) ? c->qb : c->qa;
-            k_rkls<<<nblocks(nin), BLOCK, 0, c->stream>>>(c->q_cur, qn, c->du, c->R, c->scal, g, c->nv, rkA[s], rkB[s]);
+            if (use_fused(c)) {
+                fused_stage<1>(c, c->q_cur, qn, rkA[s], rkB[s]);
+            } else {
+                rhs(c, c->q_cur);
+                Timed tm(c, PF_RK);
+                k_rkls<<<nblocks(nin), BLOCK, 0, c->stream>>>(c->q_cur, qn, c->du, c->R, c->scal, g, c->nv, rkA[s], rkB[s]);
+            }
             apply_bcs(c, qn);
             c->q_prev = c->q_cur;
             c->q_cur = qn;
@@ -201,9 +297,14 @@ int do_step(osb_ctx* c) {
         const double rknew[3] = {2.0 / 3.0, 5.0 / 12.0, 3.0 / 5.0};
         k_copy_interior<<<nblocks(nin), BLOCK, 0, c->stream>>>(c->q_cur, c->du, g, c->nv);
         for (int s = 0; s < 3; ++s) {
-            rhs(c, c->q_cur);
             double* qn = (c->q_cur == c->qa) ? c->qb : c->qa;
-            k_rk3_stage<<<nblocks(nin), BLOCK, 0, c->stream>>>(qn, c->du, c->R, c->scal, g, c->nv, rknew[s], rkold[s]);
+            if (use_fused(c)) {
+                fused_stage<2>(c, c->q_cur, qn, rknew[s], rkold[s]);
+            } else {
+                rhs(c, c->q_cur);
+                Timed tm(c, PF_RK);
+                k_rk3_stage<<<nblocks(nin), BLOCK, 0, c->stream>>>(qn, c->du, c->R, c->scal, g, c->nv, rknew[s], rkold[s]);
+            }
             apply_bcs(c, qn);
             c->q_prev = c->q_cur;
             c->q_cur = qn;
@@ -260,6 +361,8 @@ void free_ctx(osb_ctx* c) {
     double* bufs[] = {c->qa, c->qb, c->du, c->R, c->F, c->cr, c->gr, c->scal, c->staging, c->part};
     for (double* b : bufs) if (b) (void)hipFree(b);
     if (c->nan_cnt) (void)hipFree(c->nan_cnt);
+    for (auto& r : c->prof_pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    for (hipEvent_t e : c->prof_pool) (void)hipEventDestroy(e);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -502,6 +605,27 @@ int osb_nan_check(osb_ctx* c) {
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (cnt) return set_err(c, OSB_E_NONFINITE, "non-finite rho at " + std::to_string(cnt) + " points");
     return check_launch(c);
+}
+
+int osb_profile_enable(osb_ctx* c, int enable) {
+    if (!c) return OSB_E_INVALID;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    prof_resolve(c);
+    for (int i = 0; i < PF_COUNT; ++i) { c->prof_ms[i] = 0.0; c->prof_n[i] = 0; }
+    c->prof_on = enable != 0;
+    return OSB_OK;
+}
+
+int osb_profile_read(osb_ctx* c, int idx, char* name, int name_len, double* total_ms, int64_t* launches) {
+    if (!c || idx < 0 || idx >= PF_COUNT) return OSB_E_INVALID;
+    prof_resolve(c);
+    if (name && name_len > 0) {
+        std::strncpy(name, kFamilyName[idx], (size_t)name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (total_ms) *total_ms = c->prof_ms[idx];
+    if (launches) *launches = c->prof_n[idx];
+    return OSB_OK;
 }
 
 int osb_state_device_ptr(osb_ctx* c, int var, void** out) {

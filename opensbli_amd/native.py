@@ -208,6 +208,10 @@ class Library:
         L.osb_nan_check.argtypes = [vp]
         L.osb_state_device_ptr.argtypes = [vp, C.c_int, C.POINTER(vp)]
         L.osb_set_stream.argtypes = [vp, vp]
+        self.has_profile = hasattr(L, "osb_profile_enable")
+        if self.has_profile:
+            L.osb_profile_enable.argtypes = [vp, C.c_int]
+            L.osb_profile_read.argtypes = [vp, C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         self.has_batch = hasattr(L, "osb_batch_create")
         if self.has_batch:
             L.osb_batch_create.argtypes = [C.POINTER(ProblemDescC), C.c_int, C.POINTER(vp)]
@@ -328,6 +332,25 @@ class Context:
         p = C.c_void_p()
         self._check(self.lib.lib.osb_state_device_ptr(self.h, int(var), C.byref(p)))
         return p.value
+
+    def profile_enable(self, on: bool = True):
+        """Start (and clear) or stop the library's per-kernel-family HIP-event timing."""
+        self._check(self.lib.lib.osb_profile_enable(self.h, 1 if on else 0))
+
+    def profile_read(self):
+        """{family: (total_ms, launches)} accumulated since profile_enable(True)."""
+        out = {}
+        i = 0
+        while True:
+            name = C.create_string_buffer(64)
+            ms = C.c_double()
+            n = C.c_int64()
+            rc = self.lib.lib.osb_profile_read(self.h, i, name, 64, C.byref(ms), C.byref(n))
+            if rc != OK:
+                break
+            out[name.value.decode()] = (ms.value, n.value)
+            i += 1
+        return out
 
     def set_stream(self, stream_handle: int):
         self._check(self.lib.lib.osb_set_stream(self.h, C.c_void_p(stream_handle)))
