@@ -17,29 +17,42 @@
 
 namespace osb {
 
-constexpr int CF_TX = 16, CF_TY = 4, CF_TZ = 4;
-constexpr int CF_BX = CF_TX + 4, CF_BY = CF_TY + 4, CF_BZ = CF_TZ + 4;
-constexpr int CF_BOX = CF_BX * CF_BY * CF_BZ;          // 1280
-constexpr int CF_THREADS = CF_TX * CF_TY * CF_TZ;      // 256
+// tile shapes offered to the autotuner: {TX, TY, TZ}
+struct CfTile { int tx, ty, tz; };
+constexpr CfTile CF_TILES[] = {{16, 4, 4}, {8, 8, 4}, {32, 4, 2}};
+constexpr int CF_NTILES = 3;
 
 enum CfConv { CF_NONE = 0, CF_FEIEREISEN = 1, CF_SKEW = 2 };
 
 // CONV: CfConv; VISC: 0 none, 1 constant, 2 Sutherland; FLUXRES: start R from
 // the face-flux difference (shock-capturing schemes); TIME: OSB_TIME_RKLS3 (1)
 // or OSB_TIME_RK3 (2).
-template <int CONV, int VISC, bool FLUXRES, int TIME>
-__global__ void __launch_bounds__(CF_THREADS) k_central_fused(const double* __restrict__ q, const double* __restrict__ F,
+template <int CF_TX, int CF_TY, int CF_TZ, int CONV, int VISC, bool FLUXRES, int TIME>
+__global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_fused(const double* __restrict__ q, const double* __restrict__ F,
                                                               double* __restrict__ reg, double* __restrict__ qout,
                                                               const double* __restrict__ scal, Geom g, Consts c,
-                                                              double A, double B) {
+                                                              double A, double B, int xcd_remap) {
     constexpr int NV = 5;
+    constexpr int CF_BX = CF_TX + 4, CF_BY = CF_TY + 4, CF_BZ = CF_TZ + 4;
+    constexpr int CF_BOX = CF_BX * CF_BY * CF_BZ;
+    constexpr int CF_THREADS = CF_TX * CF_TY * CF_TZ;
     __shared__ double s_u[3][CF_BOX];
     __shared__ double s_p[CF_BOX];
     __shared__ double s_T[CF_BOX];
     __shared__ double s_E[CF_BOX];
     __shared__ double s_mu[VISC == 2 ? CF_BOX : 1];
 
-    const int i0 = blockIdx.x * CF_TX, j0 = blockIdx.y * CF_TY, k0 = blockIdx.z * CF_TZ;
+    // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs, so
+    // workgroup w runs on XCD w % 8.  Giving XCD x the contiguous range of
+    // tiles [x*T/8, (x+1)*T/8) (x fastest, then y, then z) keeps neighbouring
+    // tiles -- which share halo points -- in the same L2.
+    const int ntx = gridDim.x, nty = gridDim.y;
+    const int ntiles = ntx * nty * gridDim.z;
+    const int w = blockIdx.x + ntx * (blockIdx.y + nty * blockIdx.z);
+    int tile = w;
+    if (xcd_remap && (ntiles & 7) == 0) tile = (w & 7) * (ntiles >> 3) + (w >> 3);
+    const int tbx = tile % ntx, tby = (tile / ntx) % nty, tbz = tile / (ntx * nty);
+    const int i0 = tbx * CF_TX, j0 = tby * CF_TY, k0 = tbz * CF_TZ;
     const long P = g.Ptot;
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
 

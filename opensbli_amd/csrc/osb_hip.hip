@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -67,6 +68,8 @@ struct osb_ctx {
     double last_dt = 0.0;
     bool have_prev = false;
     std::string err;
+    int fused_tile = 0;             // index into CF_TILES (autotuned)
+    int xcd_remap = 0;              // XCD-aware workgroup-to-tile order (OSB_XCD_REMAP=1)
 
     // kernel timing (osb_profile_*)
     bool prof_on = false;
@@ -230,11 +233,21 @@ void launch_wavespeed(osb_ctx* c, const double* q) {
     }
 }
 
+template <int TX, int TY, int TZ, int CONV, int VISC, bool FLUXRES, int TIME>
+void launch_fused_tile(osb_ctx* c, const double* q, double* qn, double A, double B) {
+    const Geom& g = c->g;
+    dim3 grid((g.n[0] + TX - 1) / TX, (g.n[1] + TY - 1) / TY, (g.n[2] + TZ - 1) / TZ);
+    k_central_fused<TX, TY, TZ, CONV, VISC, FLUXRES, TIME><<<grid, TX * TY * TZ, 0, c->stream>>>(
+        q, c->F, c->du, qn, c->scal, g, c->c, A, B, c->xcd_remap);
+}
+
 template <int CONV, int VISC, bool FLUXRES, int TIME>
 void launch_fused_t(osb_ctx* c, const double* q, double* qn, double A, double B) {
-    const Geom& g = c->g;
-    dim3 grid((g.n[0] + CF_TX - 1) / CF_TX, (g.n[1] + CF_TY - 1) / CF_TY, (g.n[2] + CF_TZ - 1) / CF_TZ);
-    k_central_fused<CONV, VISC, FLUXRES, TIME><<<grid, CF_THREADS, 0, c->stream>>>(q, c->F, c->du, qn, c->scal, g, c->c, A, B);
+    switch (c->fused_tile) {
+        case 1: launch_fused_tile<8, 8, 4, CONV, VISC, FLUXRES, TIME>(c, q, qn, A, B); break;
+        case 2: launch_fused_tile<32, 4, 2, CONV, VISC, FLUXRES, TIME>(c, q, qn, A, B); break;
+        default: launch_fused_tile<16, 4, 4, CONV, VISC, FLUXRES, TIME>(c, q, qn, A, B); break;
+    }
 }
 
 template <int CONV, bool FLUXRES, int TIME>
@@ -458,6 +471,8 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     if (r) { free_ctx(c); return r; }
     c->q_cur = c->qa;
     c->q_prev = c->qb;
+    if (const char* e = std::getenv("OSB_FUSED_TILE")) c->fused_tile = std::max(0, std::min(CF_NTILES - 1, std::atoi(e)));
+    if (const char* e = std::getenv("OSB_XCD_REMAP")) c->xcd_remap = std::atoi(e) != 0;
     *out = c;
     return OSB_OK;
 }

@@ -792,6 +792,22 @@ double max_wavespeed(osb_ctx* c) {
     return m;
 }
 
+// RK point updates.  Low-storage RK3 (rk_LS.py:121-124, 204-215):
+// du = A*du + dt*R; q = q + B*du.  Classic RK3 (rk_sbli.py:69-76, 129-159):
+// q = qold + dt*rknew*R; qold = qold + dt*rkold*R.
+const double RKLS_A[3] = {0, (-5.0 / 9.0), (-153.0 / 128.0)};
+const double RKLS_B[3] = {(1.0 / 3.0), (15.0 / 16.0), (8.0 / 15.0)};
+const double RK3_OLD[3] = {1.0 / 4.0, 3.0 / 20.0, 3.0 / 5.0};
+const double RK3_NEW[3] = {2.0 / 3.0, 5.0 / 12.0, 3.0 / 5.0};
+inline void rkls_point(int s, double dt, double R, double& du, double& q) {
+    du = std::fma(RKLS_A[s], du, dt * R);
+    q = std::fma(RKLS_B[s], du, q);
+}
+inline void rk3_point(int s, double dt, double R, double& qold, double& q) {
+    q = std::fma(dt * RK3_NEW[s], R, qold);
+    qold = std::fma(dt * RK3_OLD[s], R, qold);
+}
+
 int validate(const osb_problem_desc* d, std::string& why) {
     if (!d) { why = "null descriptor"; return OSB_E_INVALID; }
     if (d->struct_size != sizeof(osb_problem_desc) || d->abi_version != OSB_ABI_VERSION) { why = "descriptor size/abi mismatch"; return OSB_E_INVALID; }
@@ -934,25 +950,19 @@ int osb_step(osb_ctx* c, int nsteps) {
         if (c->d.cfl > 0) dt = c->d.cfl / max_wavespeed(c);
         c->last_dt = dt;
         if (c->d.time_scheme == OSB_TIME_RKLS3) {
-            // rk_LS.py:121-124, 204-215
-            const double rkA[3] = {0, (-5.0 / 9.0), (-153.0 / 128.0)};
-            const double rkB[3] = {(1.0 / 3.0), (15.0 / 16.0), (8.0 / 15.0)};
             for (int s = 0; s < 3; ++s) {
                 compute_rhs(c);
                 for_range(in, [&](int i, int j, int k) {
                     const size_t x = c->I(i, j, k);
                     for (int v = 0; v < nv; ++v) {
                         const size_t y = (size_t)v * Pt + x;
-                        c->rkold[y] = std::fma(rkA[s], c->rkold[y], dt * c->R[y]);
-                        c->q[y] = std::fma(rkB[s], c->rkold[y], c->q[y]);
+                        rkls_point(s, dt, c->R[y], c->rkold[y], c->q[y]);
                     }
                 });
                 apply_bcs(c);
             }
         } else {
-            // rk_sbli.py:69-76, 129-159: save q, then per stage q = qold + dt*rknew*R; qold += dt*rkold*R
-            const double rkold_c[3] = {1.0 / 4.0, 3.0 / 20.0, 3.0 / 5.0};
-            const double rknew_c[3] = {2.0 / 3.0, 5.0 / 12.0, 3.0 / 5.0};
+            // rk_sbli.py:129-159: save q ("Save equations"), then the stage updates
             for_range(in, [&](int i, int j, int k) {
                 const size_t x = c->I(i, j, k);
                 for (int v = 0; v < nv; ++v) c->rkold[(size_t)v * Pt + x] = c->q[(size_t)v * Pt + x];
@@ -963,8 +973,7 @@ int osb_step(osb_ctx* c, int nsteps) {
                     const size_t x = c->I(i, j, k);
                     for (int v = 0; v < nv; ++v) {
                         const size_t y = (size_t)v * Pt + x;
-                        c->q[y] = std::fma(dt * rknew_c[s], c->R[y], c->rkold[y]);
-                        c->rkold[y] = std::fma(dt * rkold_c[s], c->R[y], c->rkold[y]);
+                        rk3_point(s, dt, c->R[y], c->rkold[y], c->q[y]);
                     }
                 });
                 apply_bcs(c);
@@ -1036,5 +1045,64 @@ int osb_nan_check(osb_ctx* c) {
 
 int osb_state_device_ptr(osb_ctx* c, int, void**) { return fail(c, OSB_E_UNSUPPORTED, "oracle has no device buffers"); }
 int osb_set_stream(osb_ctx* c, void*) { return fail(c, OSB_E_UNSUPPORTED, "oracle has no streams"); }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Test hooks (oracle only): direct access to the pieces that the tests pin
+// against closed forms and textbook identities.
+// ---------------------------------------------------------------------------
+extern "C" {
+
+// ENO coefficients c[side][r][j] and optimal weights d[side][r] of WENO5
+int osb_oracle_weno5_tables(double* c18, double* d6) {
+    for (int s = 0; s < 2; ++s)
+        for (int r = 0; r < 3; ++r) {
+            d6[s * 3 + r] = W5.dopt[s][r];
+            for (int j = 0; j < 3; ++j) c18[(s * 3 + r) * 3 + j] = W5.c[s][r][j];
+        }
+    return OSB_OK;
+}
+
+// one reconstruction: scheme 0 WENO-Z, 5 TENO5, 6 TENO6; side +1 (f+, offsets
+// -2..2) or -1 (f-, offsets -1..3); f[0..5] at offsets -2..3
+double osb_oracle_reconstruct(int scheme, int side, const double* f, double eps, double ct) {
+    if (scheme == 0) return side > 0 ? wenoz_right(f, eps) : wenoz_left(f, eps);
+    TenoParams tp{scheme, eps, ct};
+    return teno_side(f, side, tp);
+}
+
+// eigensystem for direction dir: L (left eigenvectors as rows) and R (right
+// eigenvectors as columns), row-major nv x nv
+int osb_oracle_eigensystem(int ndim, int dir, double gama, double rho, const double* u, double a, double* L, double* R) {
+    Eig e;
+    eigensystem(ndim, dir, gama - 1.0, rho, u, a, e);
+    const int nv = ndim + 2;
+    for (int i = 0; i < nv; ++i)
+        for (int j = 0; j < nv; ++j) { L[i * nv + j] = e.L[i][j]; R[i * nv + j] = e.Rm[i][j]; }
+    return OSB_OK;
+}
+
+// scalar RK step of dq/dt = lam*q with the solver's stage updates
+double osb_oracle_rk_scalar(int scheme, double lam, double dt, double q0) {
+    double q = q0, reg = 0.0;
+    if (scheme == OSB_TIME_RKLS3) {
+        for (int s = 0; s < 3; ++s) rkls_point(s, dt, lam * q, reg, q);
+    } else {
+        reg = q;
+        for (int s = 0; s < 3; ++s) rk3_point(s, dt, lam * q, reg, q);
+    }
+    return q;
+}
+
+// the padded conservative state (nv x P2 x P1 x P0 with 5 halo layers in
+// active directions), after the boundary conditions
+int osb_oracle_padded_state(osb_ctx* c, double* out, int* dims) {
+    if (!c) return OSB_E_INVALID;
+    apply_bcs(c);
+    dims[0] = c->P[0]; dims[1] = c->P[1]; dims[2] = c->P[2]; dims[3] = HALO;
+    if (out) std::copy(c->q.begin(), c->q.end(), out);
+    return OSB_OK;
+}
 
 }  // extern "C"

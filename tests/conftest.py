@@ -46,3 +46,9 @@ def hip():
     from opensbli_amd import native as nat
     _ensure(HIP_SO, os.path.join(ROOT, "opensbli_amd", "csrc"))
     return nat.load_hip()
+
+
+@pytest.fixture(scope="session")
+def hip_so():
+    """Path of the built HIP library (built here if missing; loading needs no GPU)."""
+    return _ensure(HIP_SO, os.path.join(ROOT, "opensbli_amd", "csrc"))

@@ -1,0 +1,337 @@
+"""CPU tests that pin the oracle (oracle/osb_oracle.cpp) against closed forms,
+textbook identities and the known answers the reference holds.
+
+Each check is chosen so that a plausible slip (a wrong coefficient, sign,
+index or transposed operand) fails it."""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+
+from opensbli_amd import native as nat, problems as P
+from helpers import rel_errors, run
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def hooks(oracle):
+    L = oracle.lib
+    dp = C.POINTER(C.c_double)
+    L.osb_oracle_weno5_tables.argtypes = [dp, dp]
+    L.osb_oracle_reconstruct.argtypes = [C.c_int, C.c_int, dp, C.c_double, C.c_double]
+    L.osb_oracle_reconstruct.restype = C.c_double
+    L.osb_oracle_eigensystem.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, dp, C.c_double, dp, dp]
+    L.osb_oracle_rk_scalar.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
+    L.osb_oracle_rk_scalar.restype = C.c_double
+    L.osb_oracle_padded_state.argtypes = [C.c_void_p, dp, C.POINTER(C.c_int)]
+    return L
+
+
+def _arr(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+# --------------------------------------------------------------------------------
+# a1: WENO5 tables -- Shu (1997) Table 2.1 and the optimal weights
+# --------------------------------------------------------------------------------
+def test_weno5_eno_coefficients_match_shu_table(hooks):
+    c, cp = _arr(np.zeros(18))
+    d, dpp = _arr(np.zeros(6))
+    hooks.osb_oracle_weno5_tables(cp, dpp)
+    c = c.reshape(2, 3, 3)
+    shu = {-1: [11 / 6, -7 / 6, 1 / 3], 0: [1 / 3, 5 / 6, -1 / 6], 1: [-1 / 6, 5 / 6, 1 / 3], 2: [1 / 3, -7 / 6, 11 / 6]}
+    # right reconstruction (side +1): stencil r starts at i-r -> c_rj
+    for r in range(3):
+        assert np.allclose(c[0, r], shu[r], rtol=0, atol=1e-15)
+    # left reconstruction (side -1), evaluated at the left edge of cell i+1 -> c_{r-1,j}
+    for r in range(3):
+        assert np.allclose(c[1, r], shu[r - 1], rtol=0, atol=1e-15)
+    assert np.allclose(d[:3], [0.3, 0.6, 0.1]) and np.allclose(d[3:], [0.1, 0.6, 0.3])
+
+
+def test_weno5_linear_weights_give_fifth_order_stencil(hooks):
+    """sum_r d_r c_r placed on points i-2..i+2 is the 5th-order upwind stencil
+    (2, -13, 47, 27, -3)/60."""
+    c, cp = _arr(np.zeros(18))
+    d, dpp = _arr(np.zeros(6))
+    hooks.osb_oracle_weno5_tables(cp, dpp)
+    c = c.reshape(2, 3, 3)
+    w = np.zeros(5)
+    for r in range(3):  # right: stencil r covers offsets -r..2-r -> index 2-r..4-r
+        w[2 - r:5 - r] += d[r] * c[0, r]
+    assert np.allclose(w, np.array([2, -13, 47, 27, -3]) / 60.0, rtol=0, atol=1e-15)
+
+
+def _cell_avgs(F, x_edges):
+    return (F(x_edges[1:]) - F(x_edges[:-1])) / (x_edges[1:] - x_edges[:-1])
+
+
+def _recon(hooks, scheme, side, f, eps=1e-40, ct=1e-5):
+    a, ap = _arr(f)
+    return hooks.osb_oracle_reconstruct(scheme, side, ap, eps, ct)
+
+
+@pytest.mark.parametrize("scheme", [0, 5, 6])
+@pytest.mark.parametrize("side", [1, -1])
+def test_reconstruction_exact_for_quadratics(hooks, scheme, side):
+    """All candidate stencils reproduce a quadratic primitive's interface value,
+    whatever the non-linear weights."""
+    h = 0.1
+    edges = h * (np.arange(7) - 3.0)     # cells at offsets -2..3; interface i+1/2 at x = 0
+    Fp = lambda x: 0.3 * x ** 3 - 0.7 * x ** 2 + 1.1 * x  # primitive of a quadratic
+    f = _cell_avgs(Fp, edges)
+    exact = 0.9 * 0.0 - 1.4 * 0.0 + 1.1   # derivative of the primitive at x = 0
+    assert abs(_recon(hooks, scheme, side, f) - exact) < 1e-13
+
+
+@pytest.mark.parametrize("side", [1, -1])
+def test_wenoz_fifth_order_convergence(hooks, side):
+    errs = []
+    for h in (0.08, 0.04, 0.02):
+        x0 = 0.3
+        edges = x0 + h * (np.arange(7) - 3.0)
+        f = _cell_avgs(lambda x: -np.cos(x) / 1.0, edges)   # averages of sin
+        errs.append(abs(_recon(hooks, 0, side, f) - math.sin(x0)))
+    assert errs[0] / errs[1] > 24 and errs[1] / errs[2] > 24, errs
+
+
+def test_teno5_smooth_data_uses_linear_weights(hooks):
+    """Smooth data keeps every TENO5 stencil (delta_r = 1): the result is the
+    d-weighted (11/20, 4/10, 1/20) combination of Shu's ENO candidates."""
+    h = 0.05
+    edges = 0.2 + h * (np.arange(7) - 3.0)
+    f = _cell_avgs(lambda x: -np.cos(x), edges)
+    F = lambda o: f[o + 2]
+    shu = {-1: [11 / 6, -7 / 6, 1 / 3], 0: [1 / 3, 5 / 6, -1 / 6], 1: [-1 / 6, 5 / 6, 1 / 3], 2: [1 / 3, -7 / 6, 11 / 6]}
+    # right side, teno.py stencils [-1,0,1], [0,1,2], [-2,-1,0] = Shu r = 1, 0, 2
+    cand = [np.dot(shu[1], [F(-1), F(0), F(1)]), np.dot(shu[0], [F(0), F(1), F(2)]), np.dot(shu[2], [F(-2), F(-1), F(0)])]
+    expect = 11 / 20 * cand[0] + 4 / 10 * cand[1] + 1 / 20 * cand[2]
+    assert abs(_recon(hooks, 5, 1, f) - expect) < 1e-14
+
+
+def test_teno5_cutoff_drops_stencils_across_a_jump(hooks):
+    """Step between offsets 0 and 1: only the fully upwind stencil [-2,-1,0]
+    is smooth, so TENO5 returns its ENO value (constant 1) exactly."""
+    f = np.array([1.0, 1.0, 1.0, 0.0, 0.0, 0.0])
+    assert abs(_recon(hooks, 5, 1, f) - 1.0) < 1e-15
+    # left side mirror: jump between offsets 0 and 1, upwind stencil [1,2,3] holds 0
+    assert _recon(hooks, 5, -1, f) == 0.0
+
+
+def test_wenoz_nonoscillatory_across_a_jump(hooks):
+    f = np.array([1.0, 1.0, 1.0, 0.0, 0.0, 0.0])
+    v = _recon(hooks, 0, 1, f)
+    assert abs(v - 1.0) < 1e-6 and v <= 1.0 + 1e-12
+
+
+# --------------------------------------------------------------------------------
+# a3: eigensystems -- L R = I and A R = R Lambda for the Euler flux Jacobian
+# --------------------------------------------------------------------------------
+def _euler_flux(q, ndim, d, gama):
+    rho = q[0]
+    u = q[1:1 + ndim] / rho
+    E = q[-1]
+    p = (gama - 1) * (E - 0.5 * rho * np.dot(u, u))
+    F = np.empty(ndim + 2)
+    F[0] = q[1 + d]
+    for m in range(ndim):
+        F[1 + m] = q[1 + m] * u[d] + (p if m == d else 0.0)
+    F[-1] = (E + p) * u[d]
+    return F
+
+
+@pytest.mark.parametrize("ndim,d", [(1, 0), (2, 0), (2, 1), (3, 0), (3, 1), (3, 2)])
+def test_eigensystem_diagonalises_flux_jacobian(hooks, ndim, d):
+    rng = np.random.default_rng(7 + 10 * ndim + d)
+    gama = 1.4
+    nv = ndim + 2
+    for _ in range(5):
+        rho = rng.uniform(0.3, 2.0)
+        u = rng.uniform(-1.0, 1.0, 3)
+        p = rng.uniform(0.2, 3.0)
+        a = math.sqrt(gama * p / rho)
+        uu, up = _arr(u)
+        L, Lp = _arr(np.zeros(nv * nv))
+        R, Rp = _arr(np.zeros(nv * nv))
+        hooks.osb_oracle_eigensystem(ndim, d, gama, rho, up, a, Lp, Rp)
+        L = L.reshape(nv, nv)
+        R = R.reshape(nv, nv)
+        assert np.allclose(L @ R, np.eye(nv), atol=2e-12)
+        q = np.concatenate([[rho], rho * u[:ndim], [p / (gama - 1) + 0.5 * rho * np.dot(u[:ndim], u[:ndim])]])
+        A = np.empty((nv, nv))
+        for j in range(nv):
+            hstep = 1e-6 * max(1.0, abs(q[j]))
+            e = np.zeros(nv); e[j] = hstep
+            A[:, j] = (_euler_flux(q + e, ndim, d, gama) - _euler_flux(q - e, ndim, d, gama)) / (2 * hstep)
+        ud = u[d]
+        ev = [ud - a, ud, ud + a] if ndim == 1 else [ud] * ndim + [ud + a, ud - a]
+        assert np.allclose(A @ R, R @ np.diag(ev), atol=1e-7 * max(1.0, np.abs(A).max()))
+
+
+# --------------------------------------------------------------------------------
+# a7: RK stability polynomials
+# --------------------------------------------------------------------------------
+@pytest.mark.parametrize("scheme", [nat.TIME_RKLS3, nat.TIME_RK3])
+def test_rk3_stability_polynomial(hooks, scheme):
+    for z in (-0.3, -1.1, 0.2, -2.0):
+        got = hooks.osb_oracle_rk_scalar(scheme, z, 1.0, 1.0)
+        assert abs(got - (1 + z + z * z / 2 + z ** 3 / 6)) < 1e-15 * max(1.0, abs(got))
+
+
+# --------------------------------------------------------------------------------
+# a8: boundary-condition index maps on small arrays
+# --------------------------------------------------------------------------------
+def _padded(hooks, ctx):
+    dims = (C.c_int * 4)()
+    hooks.osb_oracle_padded_state(ctx.h, None, dims)
+    P0, P1, P2, H = dims[0], dims[1], dims[2], dims[3]
+    out = np.zeros(ctx.desc.nv * P0 * P1 * P2)
+    hooks.osb_oracle_padded_state(ctx.h, out.ctypes.data_as(C.POINTER(C.c_double)), dims)
+    return out.reshape(ctx.desc.nv, P2, P1, P0), H
+
+
+@pytest.mark.parametrize("bc", [nat.BC_ZERO_GRADIENT, nat.BC_PERIODIC, nat.BC_INVISCID_WALL, nat.BC_SYMMETRY])
+def test_bc_index_maps_1d(oracle, hooks, bc):
+    n = 10
+    desc = nat.ProblemDesc(ndim=1, n=[n], delta=[0.1], conv_scheme=nat.CONV_WENO_Z, bc=[[bc, bc]], dt=1e-3)
+    rng = np.random.default_rng(3)
+    q = np.empty((3, 1, 1, n))
+    q[0] = rng.uniform(1, 2, n); q[1] = rng.uniform(-1, 1, n); q[2] = rng.uniform(3, 4, n)
+    with oracle.context(desc) as c:
+        c.set_state(q)
+        Q, H = _padded(hooks, c)
+    Q = Q[:, 0, 0]
+    at = lambda v, i: Q[v, i + H]
+    h = 4   # WENO5 boundary halo depth
+    for v in range(3):
+        sgn = -1.0 if (v == 1 and bc in (nat.BC_INVISCID_WALL, nat.BC_SYMMETRY)) else 1.0
+        for t in range(1, h + 1):
+            if bc == nat.BC_PERIODIC:
+                assert at(v, -t) == q[v, 0, 0, n - t] and at(v, n - 1 + t) == q[v, 0, 0, t - 1]
+            else:
+                assert at(v, -t) == sgn * at(v, t) and at(v, n - 1 + t) == sgn * at(v, n - 1 - t)
+        if bc == nat.BC_ZERO_GRADIENT:
+            assert at(v, 0) == q[v, 0, 0, 1] and at(v, n - 1) == q[v, 0, 0, n - 2]
+        elif bc == nat.BC_INVISCID_WALL:
+            expect0 = 0.0 if v == 1 else q[v, 0, 0, 1]
+            expect1 = 0.0 if v == 1 else q[v, 0, 0, n - 2]
+            assert at(v, 0) == expect0 and at(v, n - 1) == expect1
+        else:
+            assert at(v, 0) == q[v, 0, 0, 0] and at(v, n - 1) == q[v, 0, 0, n - 1]
+
+
+# --------------------------------------------------------------------------------
+# a4/a6: free-stream preservation, conservation, shift invariance, derivative order
+# --------------------------------------------------------------------------------
+@pytest.mark.parametrize("maker", [
+    lambda: P.sod(n=32),
+    lambda: P.kelvin_helmholtz(n=16),
+    lambda: P.riemann_2d(n=16),
+    lambda: P.tgv(n=12, scheme="teno", minf=1.25),
+    lambda: P.tgv_teno6_validation(n=12),
+    lambda: P.tgv(n=12),
+    lambda: P.tg_sym(n=12),
+])
+def test_free_stream_preserved_exactly(oracle, maker):
+    pr = maker()
+    # velocity only along directions whose boundaries are periodic or outflow
+    # (walls and symmetry planes reflect the normal momentum)
+    open_bc = (nat.BC_PERIODIC, nat.BC_ZERO_GRADIENT)
+    vel = [0.2 + 0.1 * m if all(b in open_bc for b in pr.desc.bc[m]) else 0.0 for m in range(pr.desc.ndim)]
+    q = np.empty(pr.desc.shape)
+    q[0] = 1.3
+    for m in range(pr.desc.ndim):
+        q[1 + m] = 1.3 * vel[m]
+    q[-1] = 2.5 / 0.4 + 0.5 * 1.3 * sum(v * v for v in vel)
+    with oracle.context(pr.desc) as c:
+        c.set_state(q)
+        r = c.residual()
+    if pr.desc.viscous == nat.VISC_NONE:
+        assert np.all(r == 0.0), np.abs(r).max()
+    else:
+        # the 4th-order second-derivative weights do not sum to exactly zero in
+        # binary floating point: a constant temperature leaves a rounding residue
+        assert np.abs(r).max() < 1e-14, np.abs(r).max()
+
+
+def test_periodic_conservation_weno(oracle):
+    pr = P.kelvin_helmholtz(n=32, steps=20)
+    pr.desc.bc = [[nat.BC_PERIODIC] * 2, [nat.BC_PERIODIC] * 2]
+    q1, _ = run(oracle, pr)
+    for v in range(4):
+        s0, s1 = pr.q0[v].sum(), q1[v].sum()
+        assert abs(s1 - s0) <= 2e-13 * max(1.0, np.abs(pr.q0[v]).sum()), (v, s0, s1)
+
+
+def test_periodic_shift_invariance_bitwise(oracle):
+    pr = P.kelvin_helmholtz(n=32, steps=5)
+    pr.desc.bc = [[nat.BC_PERIODIC] * 2, [nat.BC_PERIODIC] * 2]
+    qa, _ = run(oracle, pr)
+    shifted = np.roll(pr.q0, 7, axis=3)
+    qb, _ = run(oracle, pr, state=shifted)
+    assert np.array_equal(np.roll(qa, 7, axis=3), qb)
+
+
+def test_central_pressure_gradient_fourth_order(oracle):
+    """Quiescent flow with a pressure wave: the Feiereisen momentum residual is
+    -dp/dx by the 4th-order central stencil (scheme.py:83-87)."""
+    errs = []
+    for n in (16, 32):
+        delta = 2 * math.pi / n
+        desc = nat.ProblemDesc(ndim=3, n=[n, 8, 8], delta=[delta, delta, delta], conv_scheme=nat.CONV_CENTRAL_FEIEREISEN,
+                               order=4, viscous=nat.VISC_NONE, dt=1e-3, bc=[[nat.BC_PERIODIC] * 2] * 3)
+        x = delta * np.arange(n)
+        p = 1.0 + 0.1 * np.sin(x)
+        q = np.zeros(desc.shape)
+        q[0] = 1.0
+        q[4] = (p / 0.4)[None, None, :]
+        with oracle.context(desc) as c:
+            c.set_state(q)
+            r = c.residual()
+        errs.append(np.abs(r[1, 0, 0] + 0.1 * np.cos(x)).max())
+        assert np.abs(r[0]).max() == 0.0
+    assert errs[0] / errs[1] > 14, errs
+
+
+# --------------------------------------------------------------------------------
+# known answers held by the reference
+# --------------------------------------------------------------------------------
+def test_sod_against_exact_solution(oracle):
+    """apps/Sod_shock_tube/reference.txt: exact density at t = 0.2 (1000 points)."""
+    pr = P.sod()
+    q, st = run(oracle, pr)
+    assert abs(st.time - 0.2) < 1e-12
+    ref = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "sod_exact_t0.2.txt"))
+    x = pr.desc.delta[0] * np.arange(pr.desc.n[0])
+    rho = np.interp(ref[:, 0], x, q[0, 0, 0])
+    l1 = np.mean(np.abs(rho - ref[:, 1]))
+    assert l1 < 1e-3, l1
+
+
+def test_tg_sym_known_answer(oracle):
+    """tests/verification_apps/tg_sym/check_opensbli_output.py:29-34: w at interior
+    (8,8,8) after one step is -2.2438363795512375e-08.  The value is a ~1e-8
+    residue of cancelling O(1e-13) rounding-level terms; the oracle matches it
+    to 8e-9 relative (rtol 2e-8 here)."""
+    pr = P.tg_sym()
+    q, _ = run(oracle, pr)
+    w = q[3] / q[0]
+    assert math.isclose(w[8, 8, 8], -2.2438363795512375e-08, rel_tol=2e-8)
+
+
+@pytest.mark.slow
+def test_teno6_readme_validation(oracle):
+    """apps/performance_benchmark/README.MD:46-47 (128^3, 500 iterations, ~20 min on 8 cores)."""
+    pr = P.tgv_teno6_validation()
+    with oracle.context(pr.desc) as c:
+        c.set_state(pr.q0)
+        c.step(pr.steps)
+        d = c.tgv_diagnostics()
+        st = c.stats()
+    assert f"{st.time:.5e}" == "8.46250e-01"
+    for v, r in zip(d, [2.65363e+05, 1.16991e+05, 1.79666e+06, 2.09715e+06]):
+        assert f"{v:.5e}" == f"{r:.5e}"
