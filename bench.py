@@ -5,7 +5,9 @@ A "step" is one RK3 time step (three RK stages) over the whole grid with the
 state resident in HBM.  value = grid-point RK-stage updates per second
 (GPts*stage/s) = npoints * 3 * K / t.  Default workload: the 3-D viscous
 Taylor-Green vortex at 128^3 (Ma=0.1, Re=1600, Pr=0.71), 4th-order central
-Feiereisen split form + constant-viscosity terms, RK3-LS dt=5e-3, FP64.
+Feiereisen split form + constant-viscosity terms, RK3-LS, FP64.  dt is the
+reference TGV app's 0.003385 (apps/taylor_green_vortex/taylor_green_vortex.py:12):
+5e-3 exceeds the RK3/central-4 acoustic stability limit at Ma=0.1 on 128^3.
 
 The roofline entry uses the library's own HIP-event timing of its dominant
 kernel family over the timed region (events recorded on the context's
@@ -25,6 +27,25 @@ METRIC = "GPts·stage/s FP64 vs mixed-prec at stated L2 error; ensemble p50/p99 
 HBM_PEAK_GBS = 8000.0
 # compulsory bytes per grid-point RK stage: read q, read du, write du, write q (FP64 state)
 BYTES_PER_PT_STAGE = {1: 96.0, 2: 128.0, 3: 160.0}
+
+
+# HBM bytes per launch of the dominant kernel measured with rocprofv3 PMC
+# FETCH_SIZE/WRITE_SIZE (calibrated against a copy of known size); the
+# summaries hold a line "traffic_bytes_per_launch: N"
+PMC_PROFILES = {("central_fused", "tgv_central_128", "fp64"): "central_fused_tgv128_pmc.md"}
+
+
+def pmc_traffic(family, workload, precision):
+    name = PMC_PROFILES.get((family, workload, precision))
+    if not name:
+        return None
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
 
 
 def make_problem(name, precision):
@@ -130,7 +151,7 @@ def main():
     achieved = bytes_alg / avg_s / 1e9
     stage_s = dt_wall / (3.0 * args.steps)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic(fam, args.workload, args.precision),
                 "kernel": fam, "kernel_avg_us": avg_s * 1e6, "kernel_launches": nlaunch,
                 "bytes_alg_per_launch": bytes_alg,
                 "stage_achieved_GBs": bytes_alg / stage_s / 1e9}

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define OSB_ABI_VERSION 1
+#define OSB_ABI_VERSION 2
 
 /* error codes */
 #define OSB_OK            0
@@ -56,7 +56,7 @@ extern "C" {
 #define OSB_VISC_SUTHERLAND 2         /* tau = (mu/Re)(...), mu = 1.4042 T^1.5/(T+0.40417) */
 
 /* time integration */
-#define OSB_TIME_RKLS3 1              /* RungeKuttaLS(3)  rk_LS.py:17   */
+#define OSB_TIME_RKLS3 1              /* RungeKuttaLS  rk_LS.py:17 (2N-storage; see rk_stages) */
 #define OSB_TIME_RK3   2              /* RungeKutta(3)    rk_sbli.py:16 */
 
 /* boundary conditions (opensbli/core/boundary_conditions/) */
@@ -94,6 +94,14 @@ typedef struct osb_problem_desc {
     double   shock_filter_control; /* LLF wave-speed scale, shock_capturing.py:1014 */
     double   dt;               /* constant time step (used when cfl <= 0)          */
     double   cfl;              /* > 0: dt = cfl / max(sum_d (|u_d|+a)/Delta_d)     */
+    int32_t  rk_stages;        /* OSB_TIME_RKLS3 only.  0: RungeKuttaLS(3), the
+                                  Williamson coefficients of rk_LS.py:121-124;
+                                  1..5: stages with rk_a/rk_b below (RungeKuttaLS
+                                  formulation='SSP' 3/4/5 stages, or order 4,
+                                  rk_LS.py:97-132)                                 */
+    int32_t  reserved0;
+    double   rk_a[5];          /* rkA: du = rk_a[s]*du + dt*R                       */
+    double   rk_b[5];          /* rkB: q  = q + rk_b[s]*du                          */
     int32_t  reserved[8];
 } osb_problem_desc;
 

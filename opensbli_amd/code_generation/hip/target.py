@@ -1,0 +1,292 @@
+"""HIP back end for an OpenSBLI algorithm.
+
+``HIPTarget(alg, ...)`` is called where a problem script called
+``OPSC(alg)`` (opensbli/code_generation/opsc.py:408) followed by
+``substitute_simulation_parameters(names, values)``
+(opensbli/utilities/helperfunctions.py:150).  Instead of printing an OPS C
+program it reads the block's schemes, boundary conditions and physics, checks
+that the HIP kernels implement them, fills an ``osb_problem_desc`` and drives
+``libosb_hip.so`` through the C ABI.
+
+The block may be built from ``opensbli_amd.surface`` or from the reference's
+own classes: only class names and the attributes named below are read.
+
+Parameters come as the same name -> value mapping the reference pastes into
+the generated C (``'Delta0block0': '2*M_PI/block0np0'``); values may be
+numbers or C expressions over earlier names, evaluated in order with C
+arithmetic (int/int divides as integers).
+"""
+from __future__ import annotations
+
+import ast
+import math
+import operator
+from typing import Callable, Dict, Mapping, Optional, Union
+
+import numpy as np
+
+from ... import native as nat
+
+
+# --------------------------------------------------------------------------
+# parameters: name -> number or C expression
+# --------------------------------------------------------------------------
+_FUNCS = {"ceil": math.ceil, "floor": math.floor, "sqrt": math.sqrt, "pow": math.pow, "exp": math.exp,
+          "log": math.log, "sin": math.sin, "cos": math.cos, "tan": math.tan, "fabs": math.fabs,
+          "abs": abs, "fmin": min, "fmax": max}
+_CONSTS = {"M_PI": math.pi, "pi": math.pi}
+
+
+def _c_div(a, b):
+    if isinstance(a, int) and isinstance(b, int):
+        q = abs(a) // abs(b)
+        return q if (a >= 0) == (b >= 0) else -q
+    return a / b
+
+
+_BINOPS = {ast.Add: operator.add, ast.Sub: operator.sub, ast.Mult: operator.mul, ast.Div: _c_div,
+           ast.Pow: operator.pow, ast.Mod: operator.mod}
+
+
+def _eval_node(node, env):
+    if isinstance(node, ast.Expression):
+        return _eval_node(node.body, env)
+    if isinstance(node, ast.Constant) and isinstance(node.value, (int, float)):
+        return node.value
+    if isinstance(node, ast.Name):
+        if node.id in env:
+            return env[node.id]
+        if node.id in _CONSTS:
+            return _CONSTS[node.id]
+        raise KeyError(f"unknown name {node.id!r} in a simulation parameter")
+    if isinstance(node, ast.BinOp) and type(node.op) in _BINOPS:
+        return _BINOPS[type(node.op)](_eval_node(node.left, env), _eval_node(node.right, env))
+    if isinstance(node, ast.UnaryOp) and isinstance(node.op, (ast.USub, ast.UAdd)):
+        v = _eval_node(node.operand, env)
+        return -v if isinstance(node.op, ast.USub) else v
+    if isinstance(node, ast.Call) and isinstance(node.func, ast.Name) and node.func.id in _FUNCS:
+        return _FUNCS[node.func.id](*[_eval_node(a, env) for a in node.args])
+    raise ValueError(f"unsupported expression in a simulation parameter: {ast.dump(node)}")
+
+
+def evaluate_parameters(parameters: Mapping[str, Union[str, float, int]]) -> Dict[str, float]:
+    """Evaluate ``substitute_simulation_parameters``-style values in order."""
+    env: Dict[str, float] = {}
+    for name, value in parameters.items():
+        if isinstance(value, (int, float)):
+            env[name] = value
+        else:
+            env[name] = _eval_node(ast.parse(str(value).strip(), mode="eval"), env)
+    return env
+
+
+# --------------------------------------------------------------------------
+# plan: block -> osb_problem_desc
+# --------------------------------------------------------------------------
+def _kinds(obj) -> set:
+    """Class names along the MRO (so subclasses such as StoreSome count as Central)."""
+    return {k.__name__ for k in type(obj).__mro__}
+
+
+_BC_IDS = {"PeriodicBC": nat.BC_PERIODIC, "ZeroGradientOutletBC": nat.BC_ZERO_GRADIENT,
+           "InviscidWallBC": nat.BC_INVISCID_WALL, "SymmetryBC": nat.BC_SYMMETRY}
+
+
+def _averaging(scheme) -> int:
+    avg = getattr(scheme, "averaging", None)
+    if avg is None:
+        return nat.AVG_SIMPLE            # shock_capturing.py:776-777
+    kinds = _kinds(avg)
+    if "RoeAverage" in kinds:
+        return nat.AVG_ROE
+    if "SimpleAverage" in kinds:
+        return nat.AVG_SIMPLE
+    raise NotImplementedError(f"averaging {type(avg).__name__} is not implemented by the HIP kernels")
+
+
+def _viscous_model(physics) -> int:
+    if physics is None:
+        return nat.VISC_NONE
+    visc = physics.get("viscosity") if isinstance(physics, Mapping) else getattr(physics, "viscosity", None)
+    if visc in (None, "inviscid", "none"):
+        return nat.VISC_NONE
+    if visc == "constant":
+        return nat.VISC_CONSTANT
+    if visc in ("sutherland", "dynamic", "variable"):
+        return nat.VISC_SUTHERLAND
+    raise NotImplementedError(f"viscosity {visc!r} is not implemented by the HIP kernels")
+
+
+def _convective_form(physics) -> str:
+    if physics is None:
+        return "Feiereisen"
+    if isinstance(physics, Mapping):
+        return physics.get("convective", physics.get("split_type", "Feiereisen"))
+    return getattr(physics, "split_type", None) or type(getattr(physics, "split", None)).__name__
+
+
+def plan_problem(alg_or_block, parameters: Mapping, physics=None, precision: str = "fp64",
+                 cfl: Optional[float] = None) -> nat.ProblemDesc:
+    """Select the kernel plan for one block and return its descriptor.
+
+    ``physics``: an ``NS_Split`` object (its ``split_type`` and ``viscosity``)
+    or a dict ``{"convective": "Feiereisen"|"Skew", "viscosity":
+    "constant"|"sutherland"|None}``; it supplies what the reference keeps in
+    the SymPy equations.  Combinations the kernels do not implement raise
+    ``NotImplementedError`` naming the offending object."""
+    block = alg_or_block
+    if hasattr(alg_or_block, "blocks"):
+        if len(alg_or_block.blocks) != 1:
+            raise NotImplementedError("the HIP back end drives one block per algorithm")
+        block = alg_or_block.blocks[0]
+    p = evaluate_parameters(parameters)
+    ndim = int(block.ndim)
+    n = [int(p[f"block0np{d}"]) for d in range(ndim)]
+    delta = [float(p[f"Delta{d}block0"]) for d in range(ndim)]
+
+    shock, central, time = None, None, None
+    for sch in block.discretisation_schemes.values():
+        kinds = _kinds(sch)
+        if kinds & {"LFWeno", "LFTeno"}:
+            if shock is not None:
+                raise NotImplementedError("one shock-capturing scheme per block")
+            shock = sch
+        elif "Central" in kinds:
+            central = sch
+        elif kinds & {"RungeKuttaLS", "RungeKutta"}:
+            time = sch
+        else:
+            raise NotImplementedError(f"scheme {type(sch).__name__} is not implemented by the HIP kernels")
+    if time is None:
+        raise NotImplementedError("no temporal scheme on the block")
+
+    desc = dict(ndim=ndim, n=n, delta=delta)
+    if shock is not None:
+        kinds = _kinds(shock)
+        if getattr(shock, "flux_type", "LLF") != "LLF":
+            raise NotImplementedError(f"{type(shock).__name__}(flux_type={shock.flux_type!r}): only LLF")
+        if "LFWeno" in kinds:
+            if shock.order != 5 or str(getattr(shock, "formulation", "JS")).upper() != "Z":
+                raise NotImplementedError(
+                    f"LFWeno(order={shock.order}, formulation={shock.formulation!r}): only order 5, 'Z'")
+            desc.update(conv_scheme=nat.CONV_WENO_Z, order=5)
+        else:
+            if shock.order not in (5, 6) or getattr(shock, "formulation", None) is not None:
+                raise NotImplementedError(f"LFTeno(order={shock.order}): orders 5 and 6 without a sensor")
+            desc.update(conv_scheme=nat.CONV_TENO, order=int(shock.order))
+            if "TENO_CT" in p:
+                desc["teno_ct"] = float(p["TENO_CT"])
+            if "eps" in p:
+                desc["teno_eps"] = float(p["eps"])
+        desc["averaging"] = _averaging(shock)
+        desc["viscous"] = _viscous_model(physics)
+        if desc["viscous"] != nat.VISC_NONE and (central is None or central.order != 4):
+            raise NotImplementedError("viscous terms need a Central(4) scheme on the block")
+    else:
+        if central is None:
+            raise NotImplementedError("no spatial scheme on the block")
+        if central.order != 4:
+            raise NotImplementedError(f"{type(central).__name__}(order={central.order}): only order 4")
+        form = _convective_form(physics)
+        if form == "Feiereisen":
+            desc["conv_scheme"] = nat.CONV_CENTRAL_FEIEREISEN
+        elif form in ("Skew", "skew"):
+            desc["conv_scheme"] = nat.CONV_CENTRAL_SKEW
+        else:
+            raise NotImplementedError(f"convective split form {form!r}: Feiereisen and Skew are implemented")
+        desc.update(order=4, viscous=_viscous_model(physics))
+
+    tk = _kinds(time)
+    if "RungeKuttaLS" in tk:
+        desc["time_scheme"] = nat.TIME_RKLS3
+        a = list(getattr(time, "rk_a", []))
+        b = list(getattr(time, "rk_b", []))
+        williamson = time.order == 3 and getattr(time, "formulation", None) != "SSP"
+        if not williamson:
+            if not a:
+                raise NotImplementedError(f"RungeKuttaLS(order={time.order}) without rk_a/rk_b coefficients")
+            desc.update(rk_a=a, rk_b=b)
+    else:
+        if time.order != 3:
+            raise NotImplementedError(f"RungeKutta(order={time.order}): only order 3")
+        desc["time_scheme"] = nat.TIME_RK3
+
+    bc = [[0, 0] for _ in range(ndim)]
+    for pair in block.boundary_types:
+        for b in pair:
+            name = next((k for k in _kinds(b) if k in _BC_IDS), None)
+            if name is None:
+                raise NotImplementedError(f"boundary condition {type(b).__name__} is not implemented by the HIP kernels")
+            bc[b.direction][b.side] = _BC_IDS[name]
+    if any(v == 0 for pair in bc for v in pair):
+        raise ValueError("every direction needs a boundary condition on both sides")
+    desc["bc"] = bc
+
+    for key, name in (("gama", "gama"), ("minf", "Minf"), ("re", "Re"), ("pr", "Pr")):
+        if name in p:
+            desc[key] = float(p[name])
+    if cfl is not None:
+        desc["cfl"] = float(cfl)
+    elif "cfl" in p:
+        desc["cfl"] = float(p["cfl"])
+    if "dt" in p:
+        desc["dt"] = float(p["dt"])
+    desc["precision"] = nat.PREC_MIXED if precision == "mixed" else nat.PREC_FP64
+    return nat.ProblemDesc(**desc)
+
+
+# --------------------------------------------------------------------------
+# the target
+# --------------------------------------------------------------------------
+class HIPTarget:
+    """Runs an OpenSBLI algorithm on the HIP kernels.
+
+    >>> target = HIPTarget(alg, parameters=simulation_parameters, physics=NS)
+    >>> target.initialise(q0)          # array (nv, n2, n1, n0), or f(x0, x1, x2)
+    >>> target.run()                   # niter steps, state stays on the GPU
+    >>> q = target.state()
+
+    ``library`` defaults to ``libosb_hip.so``; any library exporting the same
+    C ABI can be passed (the tests pass the CPU oracle)."""
+
+    def __init__(self, alg, precision: str = "fp64", parameters: Optional[Mapping] = None, physics=None,
+                 cfl: Optional[float] = None, library: Optional[nat.Library] = None):
+        self.parameters = evaluate_parameters(parameters or {})
+        self.desc = plan_problem(alg, parameters or {}, physics=physics, precision=precision, cfl=cfl)
+        self.niter = int(self.parameters.get("niter", 0))
+        self.lib = library if library is not None else nat.load_hip()
+        self.ctx = self.lib.context(self.desc)
+
+    # grid coordinates of the interior points, x_d = Delta_d * i_d
+    def coordinates(self):
+        n = list(self.desc.n) + [1] * (3 - len(self.desc.n))
+        dl = list(self.desc.delta) + [1.0] * (3 - len(self.desc.delta))
+        axes = [dl[d] * np.arange(n[d], dtype=np.float64) for d in range(3)]
+        z, y, x = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+        return x, y, z
+
+    def initialise(self, q0: Union[np.ndarray, Callable]):
+        if callable(q0):
+            q0 = np.asarray(q0(*self.coordinates()), dtype=np.float64)
+        self.ctx.set_state(np.asarray(q0, dtype=np.float64).reshape(self.desc.shape))
+        return self
+
+    def run(self, niter: Optional[int] = None):
+        steps = self.niter if niter is None else int(niter)
+        self.ctx.step(steps)
+        return self
+
+    def state(self) -> np.ndarray:
+        return self.ctx.get_state()
+
+    def stats(self):
+        return self.ctx.stats()
+
+    def close(self):
+        self.ctx.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -68,6 +68,8 @@ struct osb_ctx {
     double last_dt = 0.0;
     bool have_prev = false;
     std::string err;
+    int rk_stages = 3;              // RK-LS stages and coefficients (rk_LS.py:97-132)
+    double rkA[5] = {0}, rkB[5] = {0};
     int fused_tile = 0;             // index into CF_TILES (autotuned)
     int xcd_remap = 0;              // XCD-aware workgroup-to-tile order (OSB_XCD_REMAP=1)
 
@@ -290,16 +292,14 @@ int do_step(osb_ctx* c) {
         k_dt_finalize<<<1, 64, 0, c->stream>>>(c->scal, c->d.cfl);
     }
     if (c->d.time_scheme == OSB_TIME_RKLS3) {
-        const double rkA[3] = {0, (-5.0 / 9.0), (-153.0 / 128.0)};
-        const double rkB[3] = {(1.0 / 3.0), (15.0 / 16.0), (8.0 / 15.0)};
-        for (int s = 0; s < 3; ++s) {
+        for (int s = 0; s < c->rk_stages; ++s) {
             double* qn = (c->q_cur == c->qa) ? c->qb : c->qa;
             if (use_fused(c)) {
-                fused_stage<1>(c, c->q_cur, qn, rkA[s], rkB[s]);
+                fused_stage<1>(c, c->q_cur, qn, c->rkA[s], c->rkB[s]);
             } else {
                 rhs(c, c->q_cur);
                 Timed tm(c, PF_RK);
-                k_rkls<<<nblocks(nin), BLOCK, 0, c->stream>>>(c->q_cur, qn, c->du, c->R, c->scal, g, c->nv, rkA[s], rkB[s]);
+                k_rkls<<<nblocks(nin), BLOCK, 0, c->stream>>>(c->q_cur, qn, c->du, c->R, c->scal, g, c->nv, c->rkA[s], c->rkB[s]);
             }
             apply_bcs(c, qn);
             c->q_prev = c->q_cur;
@@ -345,6 +345,10 @@ int validate(const osb_problem_desc* d, std::string& why) {
     if (d->viscous != OSB_VISC_NONE && d->ndim < 2) { why = "viscous terms need ndim >= 2"; return OSB_E_UNSUPPORTED; }
     if (d->time_scheme != OSB_TIME_RKLS3 && d->time_scheme != OSB_TIME_RK3) { why = "unknown time scheme"; return OSB_E_UNSUPPORTED; }
     if (d->precision != OSB_PREC_FP64 && d->precision != OSB_PREC_MIXED) { why = "unknown precision"; return OSB_E_UNSUPPORTED; }
+    if (d->rk_stages < 0 || d->rk_stages > 5) { why = "rk_stages must be 0..5"; return OSB_E_INVALID; }
+    if (d->rk_stages > 0 && d->time_scheme != OSB_TIME_RKLS3) { why = "rk_stages applies to OSB_TIME_RKLS3 only"; return OSB_E_INVALID; }
+    for (int s = 0; s < d->rk_stages; ++s)
+        if (!std::isfinite(d->rk_a[s]) || !std::isfinite(d->rk_b[s])) { why = "non-finite RK coefficient"; return OSB_E_INVALID; }
     for (int i = 0; i < d->ndim; ++i) {
         for (int s = 0; s < 2; ++s) {
             const int b = d->bc[i][s];
@@ -406,6 +410,16 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     c->central = !c->shock;
     c->viscous = desc->viscous != OSB_VISC_NONE;
     c->mixed = desc->precision == OSB_PREC_MIXED;
+    if (desc->rk_stages == 0) {
+        // RungeKuttaLS(3) Williamson coefficients (rk_LS.py:121-124)
+        const double A[3] = {0, (-5.0 / 9.0), (-153.0 / 128.0)};
+        const double B[3] = {(1.0 / 3.0), (15.0 / 16.0), (8.0 / 15.0)};
+        c->rk_stages = 3;
+        for (int s = 0; s < 3; ++s) { c->rkA[s] = A[s]; c->rkB[s] = B[s]; }
+    } else {
+        c->rk_stages = desc->rk_stages;
+        for (int s = 0; s < desc->rk_stages; ++s) { c->rkA[s] = desc->rk_a[s]; c->rkB[s] = desc->rk_b[s]; }
+    }
     c->sch = desc->conv_scheme == OSB_CONV_WENO_Z ? SCH_WENOZ : (desc->order == 6 ? SCH_TENO6 : SCH_TENO5);
 
     Geom& g = c->g;

@@ -19,7 +19,7 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 OK = 0
 E_INVALID = -1
@@ -79,6 +79,10 @@ class ProblemDescC(C.Structure):
         ("shock_filter_control", C.c_double),
         ("dt", C.c_double),
         ("cfl", C.c_double),
+        ("rk_stages", C.c_int32),
+        ("reserved0", C.c_int32),
+        ("rk_a", C.c_double * 5),
+        ("rk_b", C.c_double * 5),
         ("reserved", C.c_int32 * 8),
     ]
 
@@ -119,6 +123,9 @@ class ProblemDesc:
     shock_filter_control: float = 1.0
     dt: float = 0.0
     cfl: float = 0.0
+    # RungeKuttaLS coefficients; empty = the 3-stage Williamson scheme
+    rk_a: Sequence[float] = ()
+    rk_b: Sequence[float] = ()
 
     @property
     def nv(self) -> int:
@@ -165,6 +172,12 @@ class ProblemDesc:
         d.shock_filter_control = self.shock_filter_control
         d.dt = self.dt
         d.cfl = self.cfl
+        if len(self.rk_a) != len(self.rk_b) or len(self.rk_a) > 5:
+            raise ValueError("rk_a and rk_b need the same length, at most 5")
+        d.rk_stages = len(self.rk_a)
+        for s, (a, b) in enumerate(zip(self.rk_a, self.rk_b)):
+            d.rk_a[s] = float(a)
+            d.rk_b[s] = float(b)
         return d
 
 

@@ -1,0 +1,363 @@
+"""OpenSBLI problem-definition objects understood by the HIP back end.
+
+The class names, constructor arguments and the attributes the back end reads
+follow the reference, so a problem script written against OpenSBLI selects
+the same kernels here.  Each class records the choices a script makes; it
+does not build SymPy equations.  ``code_generation.hip.HIPTarget`` turns a
+block built from these objects -- or from the reference's own objects, whose
+class names and attributes it reads the same way -- into an
+``osb_problem_desc``.
+
+Only the configurations that the HIP kernels implement are accepted by the
+back end; everything else raises ``NotImplementedError`` there, naming the
+offending object.
+"""
+from __future__ import annotations
+
+from typing import Dict, List, Optional, Sequence
+
+
+# --------------------------------------------------------------------------
+# characteristic averaging (opensbli/schemes/spatial/averaging.py:16-124)
+# --------------------------------------------------------------------------
+class Averaging:
+    def __init__(self, locations: Sequence[int]):
+        self.locations = list(locations)
+
+
+class SimpleAverage(Averaging):
+    """Arithmetic mean of the states at the two locations (averaging.py:31)."""
+
+
+class RoeAverage(Averaging):
+    """Roe average (averaging.py:65)."""
+
+    def __init__(self, locations: Sequence[int], physics=None):
+        super().__init__(locations)
+        self.physics = physics
+
+
+# --------------------------------------------------------------------------
+# spatial schemes
+# --------------------------------------------------------------------------
+class Scheme:
+    def __init__(self, name: str, order: int):
+        self.name = name
+        self.order = order
+
+
+class LFWeno(Scheme):
+    """Local Lax-Friedrichs characteristic WENO (opensbli/schemes/spatial/weno.py:475)."""
+
+    def __init__(self, order, physics=None, averaging=None, shock_filter=None, formulation="JS",
+                 conservative=True, flux_type="LLF", flux_split=True, species=None):
+        if flux_type not in ("LLF", "GLF"):
+            raise ValueError("Please select either LLF or GLF for the flux-splitting.")
+        if order % 2 == 0:
+            raise ValueError(f"Please set an odd-order for the WENO scheme, currently {order} is not supported")
+        super().__init__("WenoDerivative", order)
+        self.physics = physics
+        self.averaging = averaging
+        self.shock_filter = shock_filter
+        self.formulation = formulation
+        self.conservative = conservative
+        self.flux_type = flux_type
+        self.flux_split = flux_split
+        self.species = species
+
+
+class LFTeno(Scheme):
+    """Local Lax-Friedrichs characteristic TENO (opensbli/schemes/spatial/teno.py:480)."""
+
+    def __init__(self, order, formulation=None, physics=None, averaging=None, sensor=None, store_sensor=False,
+                 conservative=True, flux_type="LLF", flux_split=True, passive_scalar=False):
+        if flux_type not in ("LLF", "GLF"):
+            raise ValueError("Please select either LLF or GLF for the flux-splitting.")
+        super().__init__("TenoDerivative", order)
+        self.formulation = formulation
+        self.physics = physics
+        self.averaging = averaging
+        self.sensor_array = sensor
+        self.store_sensor = store_sensor
+        self.conservative = conservative
+        self.flux_type = flux_type
+        self.flux_split = flux_split
+        self.passive_scalar = passive_scalar
+
+
+class Central(Scheme):
+    """Central differences (opensbli/schemes/spatial/scheme.py:64)."""
+
+    def __init__(self, order):
+        super().__init__("CentralDerivative", order)
+        self.points = list(range(-(order // 2), order // 2 + 1))
+
+
+class StoreSome(Central):
+    """Central differences storing the derivatives of some functions
+    (opensbli/optimisations/StoreSome.py:19)."""
+
+    def __init__(self, order, der_fns_to_store, merged=True, group_stored=False, level=1):
+        super().__init__(order)
+        self.fns = der_fns_to_store
+        self.merged = merged
+        self.group_stored = group_stored
+
+
+# --------------------------------------------------------------------------
+# temporal schemes
+# --------------------------------------------------------------------------
+class RungeKuttaLS(Scheme):
+    """2N-storage Runge-Kutta (opensbli/schemes/temporal/rk_LS.py:17).
+
+    ``rk_a``/``rk_b`` are the stage coefficients of rk_LS.py:97-132: the
+    Williamson 3-stage scheme, Ruuth's SSP 3rd-order schemes with 3, 4 or 5
+    stages, and the Carpenter-Kennedy 4th-order 5-stage scheme.  The values
+    are published coefficients and must be the same numbers."""
+
+    def __init__(self, order, stages=None, formulation=None):
+        super().__init__("RungeKutta", order)
+        self.formulation = formulation
+        if order == 3:
+            if formulation == "SSP":
+                n = 3 if stages is None else stages
+                tables = {
+                    3: ([0.0, -2.915493957701923, 0.0],
+                        [0.924574112262461, 0.287712943868770, 0.626538293270800]),
+                    4: ([0.0, -0.449336503268844, 0.0, -4.661555711601366],
+                        [1.086620745813428, 0.854115548251602, -1.576604558206099, -0.278475500113052]),
+                    5: ([0.0, -4.344339134485095, 0.0, -3.770024161386381, -0.046347284573284],
+                        [0.713497331193829, 0.133505249805329, 0.713497331193829, 0.149579395628565,
+                         0.384471116121269]),
+                }
+                if n not in tables:
+                    raise ValueError("The 3rd order RK-SSP is defined for 3, 4, or 5 stages.")
+                self.rk_a, self.rk_b = tables[n]
+            else:
+                n = 3
+                self.rk_a = [0.0, -5.0 / 9.0, -153.0 / 128.0]
+                self.rk_b = [1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0]
+        elif order == 4:
+            n = 5
+            self.rk_a = [0.0, -0.4178904745, -1.192151694643, -1.697784692471, -1.514183444257]
+            self.rk_b = [0.1496590219993, 0.3792103129999, 0.8229550293869, 0.6994504559488, 0.1530572479681]
+        else:
+            raise NotImplementedError("Only 3rd and 4th order RK schemes are currently implemented.")
+        self.stages = n
+        self.n_stages = n
+
+    @property
+    def is_default(self) -> bool:
+        """True for the Williamson 3-stage scheme (the descriptor's default)."""
+        return self.order == 3 and self.formulation != "SSP"
+
+
+class RungeKutta(Scheme):
+    """Classic 3-stage RK3 with a saved q (opensbli/schemes/temporal/rk_sbli.py:16)."""
+
+    def __init__(self, order, constant_dt=None):
+        if order != 3:
+            raise NotImplementedError("RungeKutta is implemented for order 3")
+        super().__init__("RungeKutta", order)
+        self.constant_dt = constant_dt
+
+
+# --------------------------------------------------------------------------
+# boundary conditions (opensbli/core/boundary_conditions/)
+# --------------------------------------------------------------------------
+class BoundaryConditionBase:
+    def __init__(self, direction, side, plane=True):
+        self.direction = direction
+        self.side = side
+        self.full_plane = bool(plane)
+
+
+class PeriodicBC(BoundaryConditionBase):
+    """periodic.py:7"""
+
+    def __init__(self, direction, side, halos=None, full_depth=False, corners=True, plane=True):
+        super().__init__(direction, side, plane)
+        self.halos = halos
+        self.corners = corners
+
+
+class ZeroGradientOutletBC(BoundaryConditionBase):
+    """zero_gradient_outlet.py:5"""
+
+    def __init__(self, direction, side, scheme=None, plane=True):
+        super().__init__(direction, side, plane)
+        self.scheme = scheme
+
+
+class InviscidWallBC(BoundaryConditionBase):
+    """inviscid_wall.py:6"""
+
+    def __init__(self, direction, side, scheme=None, plane=True, include_corner_halos=True):
+        super().__init__(direction, side, plane)
+        self.scheme = scheme
+
+
+class SymmetryBC(BoundaryConditionBase):
+    """symmetry.py:6"""
+
+    def __init__(self, direction, side, scheme=None, plane=True):
+        super().__init__(direction, side, plane)
+        self.scheme = scheme
+
+
+class DirichletBC(BoundaryConditionBase):
+    """dirichlet.py; recorded so a script constructs, rejected by the HIP back end."""
+
+    def __init__(self, direction, side, equations, scheme=None, plane=True):
+        super().__init__(direction, side, plane)
+        self.equations = equations
+
+
+# --------------------------------------------------------------------------
+# equations, physics and the block
+# --------------------------------------------------------------------------
+class NS_Split:
+    """Split forms of the Navier-Stokes convective terms
+    (opensbli/physical_models/split_forms.py:295).  ``viscosity`` is 'constant'
+    (tau = (1/Re)(...)), 'inviscid' or None (no diffusive terms), or a
+    variable-viscosity choice ('dynamic', 'sutherland'): tau = (mu/Re)(...)
+    with the Sutherland law mu = 1.4042 T^1.5/(T + 0.40417)."""
+
+    def __init__(self, split_type, ndim, constants=(), coordinate_symbol="x", conservative=True, viscosity=None,
+                 energy_formulation="none", debug=False):
+        known = ("Divergence", "Blaisdell", "Feiereisen", "Jameson", "Kok", "KGP", "KEEP")
+        if split_type not in known:
+            raise NotImplementedError(
+                "Only Divergence, Blasidell, Feiereisen, Jameson, Kok, KGP, and KEEP splitting methods are implemented.")
+        self.split_type = split_type
+        self.ndim = ndim
+        self.viscosity = viscosity
+        self.inviscid = viscosity == "inviscid"
+        self.conservative = conservative
+        self.energy_formulation = energy_formulation
+        self.mass = ("NS_Split", split_type, "mass")
+        self.momentum = ("NS_Split", split_type, "momentum")
+        self.energy = ("NS_Split", split_type, "energy")
+
+
+class EinsteinEquation:
+    """Stand-in for the index expander: returns the equation text unchanged."""
+
+    def expand(self, equation, ndim, coordinate_symbol, substitutions, constants):
+        return equation
+
+
+class _EquationClass:
+    def __init__(self):
+        self.equations: List = []
+
+    def add_equations(self, eqs):
+        self.equations += list(eqs) if isinstance(eqs, (list, tuple)) else [eqs]
+
+
+class SimulationEquations(_EquationClass):
+    """opensbli/equation_types/opensbliequations.py"""
+
+
+class ConstituentRelations(_EquationClass):
+    """opensbli/equation_types/opensbliequations.py:491"""
+
+
+class GridBasedInitialisation(_EquationClass):
+    """opensbli/initialisation/gridbasedinit.py:16 (the HIP back end takes
+    the initial field as an array or a function of the grid coordinates)."""
+
+
+class iohdf5:
+    """opensbli/core/io_hdf5.py:22 -- records the arrays a script asks to write."""
+
+    def __init__(self, arrays=None, save_every=None, **kwargs):
+        self.arrays = list(arrays or [])
+        self.save_every = save_every
+        self.kwargs = {k.lower(): (v.lower() if isinstance(v, str) else v) for k, v in kwargs.items()} \
+            or {"iotype": "write"}
+
+    def add_arrays(self, arrays):
+        self.arrays += list(arrays)
+
+
+class Double:
+    """SimulationDataType choice: FP64 everywhere."""
+
+
+class FloatC:
+    """SimulationDataType choice of the reference for FP32 storage; the HIP
+    back end offers instead the FP32-flux/FP64-state 'mixed' precision."""
+
+
+class SimulationDataType:
+    """opensbli/core/datatypes.py:3"""
+    dtype_ = Double
+
+    @staticmethod
+    def set_datatype(types):
+        SimulationDataType.dtype_ = types
+
+    @staticmethod
+    def dtype():
+        return SimulationDataType.dtype_
+
+
+class SimulationBlock:
+    """One structured block (opensbli/core/block.py:83)."""
+
+    def __init__(self, ndim, block_number=0):
+        self.ndim = ndim
+        self.blocknumber = block_number
+        self.block_number = block_number
+        self.boundary_types: List = []
+        self.discretisation_schemes: Dict[str, Scheme] = {}
+        self.list_of_equation_classes: List = []
+        self.InputOutput: List = []
+
+    def set_block_boundaries(self, bclist):
+        flat = _flatten(bclist)
+        counts = [0, 0, 0]
+        for bc in flat:
+            counts[bc.direction] += 1
+        for d in range(self.ndim):
+            if counts[d] != 2:
+                raise AssertionError(f"direction {d} needs two boundary conditions, got {counts[d]}")
+        it = iter(flat)
+        self.boundary_types = list(zip(it, it))
+
+    def set_discretisation_schemes(self, schemes: Dict[str, Scheme]):
+        self.discretisation_schemes = schemes
+
+    def set_equations(self, list_of_equations):
+        self.list_of_equation_classes += list(list_of_equations)
+
+    def setio(self, list_of_io):
+        self.InputOutput += _flatten([list_of_io])
+
+    def discretise(self):
+        return None
+
+
+class TraditionalAlgorithmRK:
+    """opensbli/code_generation/algorithm/algorithm.py:334 (single block)."""
+
+    def __init__(self, blocks, simulation_monitor=None):
+        blocks = _flatten([blocks])
+        if len(blocks) != 1:
+            raise NotImplementedError("one block per algorithm")
+        self.blocks = blocks
+        self.simulation_monitor = simulation_monitor
+
+
+def _flatten(x) -> list:
+    out = []
+    for e in x:
+        if isinstance(e, (list, tuple)):
+            out += _flatten(e)
+        else:
+            out.append(e)
+    return out
+
+
+__all__ = [n for n in dir() if not n.startswith("_")]

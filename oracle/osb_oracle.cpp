@@ -58,6 +58,8 @@ struct osb_ctx {
     int64_t iter = 0;
     double last_dt = 0.0;
     std::string err;
+    int rk_stages = 3;                    // RK-LS stages and coefficients
+    double rkA[5] = {0}, rkB[5] = {0};
 
     size_t I(int i, int j, int k) const {
         return (size_t)(i + off[0]) + (size_t)P[0] * ((size_t)(j + off[1]) + (size_t)P[1] * (size_t)(k + off[2]));
@@ -795,13 +797,15 @@ double max_wavespeed(osb_ctx* c) {
 // RK point updates.  Low-storage RK3 (rk_LS.py:121-124, 204-215):
 // du = A*du + dt*R; q = q + B*du.  Classic RK3 (rk_sbli.py:69-76, 129-159):
 // q = qold + dt*rknew*R; qold = qold + dt*rkold*R.
+// RungeKuttaLS(3) Williamson coefficients (rk_LS.py:121-124)
 const double RKLS_A[3] = {0, (-5.0 / 9.0), (-153.0 / 128.0)};
 const double RKLS_B[3] = {(1.0 / 3.0), (15.0 / 16.0), (8.0 / 15.0)};
 const double RK3_OLD[3] = {1.0 / 4.0, 3.0 / 20.0, 3.0 / 5.0};
 const double RK3_NEW[3] = {2.0 / 3.0, 5.0 / 12.0, 3.0 / 5.0};
-inline void rkls_point(int s, double dt, double R, double& du, double& q) {
-    du = std::fma(RKLS_A[s], du, dt * R);
-    q = std::fma(RKLS_B[s], du, q);
+// rk_LS.py:204-215: du = rkA*du + dt*R; q = q + rkB*du
+inline void rkls_point(double A, double B, double dt, double R, double& du, double& q) {
+    du = std::fma(A, du, dt * R);
+    q = std::fma(B, du, q);
 }
 inline void rk3_point(int s, double dt, double R, double& qold, double& q) {
     q = std::fma(dt * RK3_NEW[s], R, qold);
@@ -826,6 +830,10 @@ int validate(const osb_problem_desc* d, std::string& why) {
     if (d->viscous != OSB_VISC_NONE && d->ndim < 2) { why = "viscous terms need ndim >= 2"; return OSB_E_UNSUPPORTED; }
     if (d->time_scheme != OSB_TIME_RKLS3 && d->time_scheme != OSB_TIME_RK3) { why = "unknown time scheme"; return OSB_E_UNSUPPORTED; }
     if (d->precision != OSB_PREC_FP64 && d->precision != OSB_PREC_MIXED) { why = "unknown precision"; return OSB_E_UNSUPPORTED; }
+    if (d->rk_stages < 0 || d->rk_stages > 5) { why = "rk_stages must be 0..5"; return OSB_E_INVALID; }
+    if (d->rk_stages > 0 && d->time_scheme != OSB_TIME_RKLS3) { why = "rk_stages applies to OSB_TIME_RKLS3 only"; return OSB_E_INVALID; }
+    for (int s = 0; s < d->rk_stages; ++s)
+        if (!std::isfinite(d->rk_a[s]) || !std::isfinite(d->rk_b[s])) { why = "non-finite RK coefficient"; return OSB_E_INVALID; }
     for (int i = 0; i < d->ndim; ++i) {
         for (int s = 0; s < 2; ++s) {
             const int b = d->bc[i][s];
@@ -869,6 +877,13 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     c->central = !c->shock;
     c->viscous = desc->viscous != OSB_VISC_NONE;
     c->depth = c->shock ? 4 : 2;          // WenoHalos/TenoHalos [-3,4] -> 4; CentralHalos(4) -> 2
+    if (desc->rk_stages == 0) {
+        c->rk_stages = 3;
+        for (int s = 0; s < 3; ++s) { c->rkA[s] = RKLS_A[s]; c->rkB[s] = RKLS_B[s]; }
+    } else {
+        c->rk_stages = desc->rk_stages;
+        for (int s = 0; s < desc->rk_stages; ++s) { c->rkA[s] = desc->rk_a[s]; c->rkB[s] = desc->rk_b[s]; }
+    }
     const size_t nvP = (size_t)c->nv * c->Ptot;
     c->q.assign(nvP, 0.0);
     c->rkold.assign(nvP, 0.0);
@@ -950,13 +965,14 @@ int osb_step(osb_ctx* c, int nsteps) {
         if (c->d.cfl > 0) dt = c->d.cfl / max_wavespeed(c);
         c->last_dt = dt;
         if (c->d.time_scheme == OSB_TIME_RKLS3) {
-            for (int s = 0; s < 3; ++s) {
+            for (int s = 0; s < c->rk_stages; ++s) {
                 compute_rhs(c);
+                const double A = c->rkA[s], B = c->rkB[s];
                 for_range(in, [&](int i, int j, int k) {
                     const size_t x = c->I(i, j, k);
                     for (int v = 0; v < nv; ++v) {
                         const size_t y = (size_t)v * Pt + x;
-                        rkls_point(s, dt, c->R[y], c->rkold[y], c->q[y]);
+                        rkls_point(A, B, dt, c->R[y], c->rkold[y], c->q[y]);
                     }
                 });
                 apply_bcs(c);
@@ -1084,10 +1100,13 @@ int osb_oracle_eigensystem(int ndim, int dir, double gama, double rho, const dou
 }
 
 // scalar RK step of dq/dt = lam*q with the solver's stage updates
-double osb_oracle_rk_scalar(int scheme, double lam, double dt, double q0) {
+// stages = 0: the default coefficients of `scheme`; else RK-LS with A/B given
+double osb_oracle_rk_scalar(int scheme, int stages, const double* A, const double* B, double lam, double dt, double q0) {
     double q = q0, reg = 0.0;
-    if (scheme == OSB_TIME_RKLS3) {
-        for (int s = 0; s < 3; ++s) rkls_point(s, dt, lam * q, reg, q);
+    if (scheme == OSB_TIME_RKLS3 && stages > 0) {
+        for (int s = 0; s < stages; ++s) rkls_point(A[s], B[s], dt, lam * q, reg, q);
+    } else if (scheme == OSB_TIME_RKLS3) {
+        for (int s = 0; s < 3; ++s) rkls_point(RKLS_A[s], RKLS_B[s], dt, lam * q, reg, q);
     } else {
         reg = q;
         for (int s = 0; s < 3; ++s) rk3_point(s, dt, lam * q, reg, q);

@@ -103,6 +103,38 @@ def test_riemann_small(hip, oracle):
     _compare(hip, oracle, P.riemann_2d(n=64, steps=50))
 
 
+@pytest.mark.parametrize("order,stages,formulation", [(3, 5, "SSP"), (4, None, None)])
+def test_rkls_variants(hip, oracle, order, stages, formulation):
+    """RungeKuttaLS SSP(5,3) and the 4th-order 5-stage scheme through the
+    generic 2N-storage path (KH, and the fused 3-D central stage kernel)."""
+    from opensbli_amd.surface import RungeKuttaLS
+    rk = RungeKuttaLS(order, stages=stages, formulation=formulation)
+    for pr in (P.kelvin_helmholtz(n=64, steps=20), P.tgv(n=24, steps=4)):
+        pr.desc.rk_a, pr.desc.rk_b = rk.rk_a, rk.rk_b
+        pr.name += f"_rk{order}_{len(rk.rk_a)}"
+        _compare(hip, oracle, pr)
+
+
+def test_hip_target_default_library(oracle):
+    """HIPTarget drives libosb_hip.so by default (the drop-in for OPSC(alg))."""
+    from opensbli_amd.code_generation.hip import HIPTarget
+    from opensbli_amd.surface import (LFWeno, RungeKuttaLS, SimpleAverage, SimulationBlock,
+                                      TraditionalAlgorithmRK, ZeroGradientOutletBC)
+    _announce("HIPTarget sod 256 x 100")
+    block = SimulationBlock(1)
+    block.set_block_boundaries([ZeroGradientOutletBC(0, 0), ZeroGradientOutletBC(0, 1)])
+    block.set_discretisation_schemes({"w": LFWeno(5, formulation="Z", averaging=SimpleAverage([0, 1])),
+                                      "rk": RungeKuttaLS(3)})
+    params = {"gama": "1.4", "dt": "0.0002", "niter": "100", "block0np0": "256", "Delta0block0": "1.0/(block0np0-1)"}
+    pr = P.sod(n=256, steps=100)
+    with HIPTarget(TraditionalAlgorithmRK(block), parameters=params) as t:
+        assert t.lib.path.endswith("libosb_hip.so")
+        q = t.initialise(pr.q0).run().state()
+    want, _ = run(oracle, pr)
+    l2, li = rel_errors(q, want)
+    assert l2.max() <= TOL and li.max() <= TOL
+
+
 def test_tg_sym_known_answer(hip, oracle):
     """Reference pin: w at interior (8,8,8) after one step,
     tests/verification_apps/tg_sym/check_opensbli_output.py:31-34."""

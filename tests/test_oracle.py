@@ -24,7 +24,7 @@ def hooks(oracle):
     L.osb_oracle_reconstruct.argtypes = [C.c_int, C.c_int, dp, C.c_double, C.c_double]
     L.osb_oracle_reconstruct.restype = C.c_double
     L.osb_oracle_eigensystem.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, dp, C.c_double, dp, dp]
-    L.osb_oracle_rk_scalar.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
+    L.osb_oracle_rk_scalar.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, C.c_double, C.c_double]
     L.osb_oracle_rk_scalar.restype = C.c_double
     L.osb_oracle_padded_state.argtypes = [C.c_void_p, dp, C.POINTER(C.c_int)]
     return L
@@ -178,8 +178,27 @@ def test_eigensystem_diagonalises_flux_jacobian(hooks, ndim, d):
 @pytest.mark.parametrize("scheme", [nat.TIME_RKLS3, nat.TIME_RK3])
 def test_rk3_stability_polynomial(hooks, scheme):
     for z in (-0.3, -1.1, 0.2, -2.0):
-        got = hooks.osb_oracle_rk_scalar(scheme, z, 1.0, 1.0)
+        got = hooks.osb_oracle_rk_scalar(scheme, 0, None, None, z, 1.0, 1.0)
         assert abs(got - (1 + z + z * z / 2 + z ** 3 / 6)) < 1e-15 * max(1.0, abs(got))
+
+
+@pytest.mark.parametrize("formulation,stages,order", [(None, None, 3), ("SSP", 3, 3), ("SSP", 4, 3), ("SSP", 5, 3),
+                                                      (None, None, 4)])
+def test_rkls_variants_order_conditions(hooks, formulation, stages, order):
+    """The amplification polynomial of every RungeKuttaLS variant (rk_LS.py:97-132)
+    matches exp(z) through z^order: its Taylor coefficients are 1/k!."""
+    from opensbli_amd.surface import RungeKuttaLS
+    rk = RungeKuttaLS(order, stages=stages, formulation=formulation)
+    A, Ap = _arr(rk.rk_a)
+    B, Bp = _arr(rk.rk_b)
+    m = len(rk.rk_a)
+    zs = np.linspace(-0.5, 0.5, m + 1)
+    vals = [hooks.osb_oracle_rk_scalar(nat.TIME_RKLS3, m, Ap, Bp, z, 1.0, 1.0) for z in zs]
+    coef = np.linalg.solve(np.vander(zs, m + 1, increasing=True), vals)
+    for k in range(order + 1):
+        assert abs(coef[k] - 1.0 / math.factorial(k)) < 1e-9, (k, coef)
+    if m > order:   # and not through z^(order+1): the order is exactly `order`
+        assert abs(coef[order + 1] - 1.0 / math.factorial(order + 1)) > 1e-6
 
 
 # --------------------------------------------------------------------------------

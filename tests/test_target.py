@@ -1,0 +1,161 @@
+"""The Python problem-definition surface and the HIP back end's plan.
+
+Each test builds a block the way a reference problem script does
+(apps/Sod_shock_tube/Sod_shock_tube.py, apps/taylor_green_vortex/taylor_green_vortex.py,
+apps/compressible_taylor_green_vortex/compressible_TGV.py,
+apps/kelvin_helmholtz/kelvin_helmholtz.py) and checks the descriptor the
+back end derives.  The run tests drive the CPU oracle through HIPTarget's
+``library`` argument (the product default is libosb_hip.so)."""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import run
+from opensbli_amd import native as nat, problems as P
+from opensbli_amd.code_generation.hip import HIPTarget, evaluate_parameters, plan_problem
+from opensbli_amd.surface import (
+    Central, DirichletBC, InviscidWallBC, LFTeno, LFWeno, NS_Split, PeriodicBC, RoeAverage, RungeKutta,
+    RungeKuttaLS, SimpleAverage, SimulationBlock, StoreSome, SymmetryBC, TraditionalAlgorithmRK,
+    ZeroGradientOutletBC)
+
+
+def _block(ndim, schemes, bcs):
+    block = SimulationBlock(ndim, block_number=0)
+    block.set_block_boundaries(bcs)
+    block.set_discretisation_schemes({s.name + str(i): s for i, s in enumerate(schemes)})
+    block.discretise()
+    return TraditionalAlgorithmRK(block)
+
+
+def _same(a: nat.ProblemDesc, b: nat.ProblemDesc):
+    ca, cb = a.to_c(), b.to_c()
+    central = a.conv_scheme in (nat.CONV_CENTRAL_FEIEREISEN, nat.CONV_CENTRAL_SKEW)
+    for name, _ in nat.ProblemDescC._fields_:
+        if central and name == "averaging":
+            continue        # characteristic averaging applies to shock capturing only
+        va, vb = getattr(ca, name), getattr(cb, name)
+        if hasattr(va, "__len__"):
+            va, vb = np.array(va), np.array(vb)
+            assert np.array_equal(va, vb), name
+        else:
+            assert va == vb or (isinstance(va, float) and math.isclose(va, vb, rel_tol=1e-15)), (name, va, vb)
+
+
+def test_parameters_c_semantics():
+    p = evaluate_parameters({"block0np0": "128", "Delta0block0": "2*M_PI/block0np0", "niter": "ceil(0.2/0.0002)",
+                             "half": "1/2", "x": 3, "y": "x*x - 1"})
+    assert p["Delta0block0"] == 2 * math.pi / 128 and p["niter"] == 1000
+    assert p["half"] == 0 and p["y"] == 8
+
+
+def test_sod_plan_matches_config():
+    alg = _block(1, [LFWeno(5, formulation="Z", averaging=SimpleAverage([0, 1]), flux_type="LLF"), RungeKuttaLS(3)],
+                 [ZeroGradientOutletBC(0, 0), ZeroGradientOutletBC(0, 1)])
+    params = {"gama": "1.4", "dt": "0.0002", "niter": "ceil(0.2/0.0002)", "block0np0": "1024",
+              "Delta0block0": "1.0/(block0np0-1)"}
+    _same(plan_problem(alg, params), P.sod().desc)
+
+
+def test_tgv_central_plan_matches_config():
+    NS = NS_Split("Feiereisen", 3, ["Re", "Pr", "gama", "Minf"], viscosity="constant")
+    alg = _block(3, [StoreSome(4, "u0 u1 u2"), RungeKuttaLS(3)],
+                 [PeriodicBC(d, s) for d in range(3) for s in range(2)])
+    params = {"gama": "1.4", "Minf": "0.1", "Re": "1600.0", "Pr": "0.71", "dt": "0.003385", "niter": "200"}
+    for d in range(3):
+        params[f"block0np{d}"] = "128"
+        params[f"Delta{d}block0"] = f"2*M_PI/block0np{d}"
+    _same(plan_problem(alg, params, physics=NS), P.tgv().desc)
+    _same(plan_problem(alg, params, physics=NS, precision="mixed"), P.tgv(precision=nat.PREC_MIXED).desc)
+
+
+def test_teno_tgv_plan_matches_config():
+    alg = _block(3, [LFTeno(5, averaging=RoeAverage([0, 1]), flux_type="LLF"), Central(4), RungeKuttaLS(3)],
+                 [PeriodicBC(d, s) for d in range(3) for s in range(2)])
+    params = {"gama": "1.4", "Minf": "1.25", "Re": "1600.0", "Pr": "0.71", "dt": "0.003385", "TENO_CT": "1e-5",
+              "eps": "1e-40"}
+    for d in range(3):
+        params[f"block0np{d}"] = "256"
+        params[f"Delta{d}block0"] = f"2*M_PI/block0np{d}"
+    got = plan_problem(alg, params, physics={"viscosity": "sutherland"}, cfl=0.5)
+    want = P.tgv(n=256, steps=100, scheme="teno", minf=1.25, cfl=0.5, dt=0.003385, teno_ct=1e-5)
+    _same(got, want.desc)
+
+
+def test_kh_and_tg_sym_plans():
+    alg = _block(2, [LFWeno(5, formulation="Z", averaging=RoeAverage([0, 1])), RungeKuttaLS(3)],
+                 [PeriodicBC(0, 0), PeriodicBC(0, 1), InviscidWallBC(1, 0), InviscidWallBC(1, 1)])
+    params = {"gama": "1.4", "block0np0": "256", "block0np1": "256", "Delta0block0": "1.0/block0np0",
+              "Delta1block0": "1.0/block0np1", "dt": "0.2/256"}
+    _same(plan_problem(alg, params), P.kelvin_helmholtz(n=256).desc)
+
+    alg = _block(3, [Central(4), RungeKutta(3)], [SymmetryBC(d, s) for d in range(3) for s in range(2)])
+    params = {"gama": "1.4", "Minf": "0.1", "Re": "800.0", "Pr": "0.71", "dt": "0.005"}
+    for d in range(3):
+        params[f"block0np{d}"] = "33"
+        params[f"Delta{d}block0"] = f"M_PI/(block0np{d}-1)"
+    _same(plan_problem(alg, params, physics={"convective": "Skew", "viscosity": "constant"}), P.tg_sym().desc)
+
+
+@pytest.mark.parametrize("bad,match", [
+    (lambda: [LFWeno(5, formulation="JS"), RungeKuttaLS(3)], "formulation"),
+    (lambda: [LFWeno(7, formulation="Z"), RungeKuttaLS(3)], "order"),
+    (lambda: [LFTeno(5, flux_type="GLF"), RungeKuttaLS(3)], "LLF"),
+    (lambda: [LFWeno(5, formulation="Z")], "temporal"),
+])
+def test_unsupported_configurations_raise(bad, match):
+    alg = _block(1, bad(), [ZeroGradientOutletBC(0, 0), ZeroGradientOutletBC(0, 1)])
+    with pytest.raises(NotImplementedError, match=match):
+        plan_problem(alg, {"block0np0": 64, "Delta0block0": 0.01, "dt": 1e-3})
+
+
+def test_dirichlet_rejected():
+    alg = _block(1, [LFWeno(5, formulation="Z"), RungeKuttaLS(3)], [DirichletBC(0, 0, []), ZeroGradientOutletBC(0, 1)])
+    with pytest.raises(NotImplementedError, match="DirichletBC"):
+        plan_problem(alg, {"block0np0": 64, "Delta0block0": 0.01, "dt": 1e-3})
+
+
+def test_rk_ssp_plan_and_run(oracle):
+    """RungeKuttaLS(3, formulation='SSP') (the Sod app's choice) carries its
+    coefficients through the descriptor; a run through HIPTarget equals the
+    same run through the library directly."""
+    rk = RungeKuttaLS(3, formulation="SSP")
+    alg = _block(1, [LFWeno(5, formulation="Z"), rk], [ZeroGradientOutletBC(0, 0), ZeroGradientOutletBC(0, 1)])
+    params = {"gama": "1.4", "dt": "0.0002", "niter": "50", "block0np0": "200", "Delta0block0": "1.0/(block0np0-1)"}
+    desc = plan_problem(alg, params)
+    assert list(desc.rk_a) == rk.rk_a and desc.to_c().rk_stages == 3
+    pr = P.sod(n=200, steps=50)
+    with HIPTarget(alg, parameters=params, library=oracle) as t:
+        t.initialise(pr.q0).run()
+        q_target = t.state()
+        assert t.stats().iter == 50
+    pr.desc.rk_a, pr.desc.rk_b = rk.rk_a, rk.rk_b
+    with oracle.context(pr.desc) as c:
+        c.set_state(pr.q0)
+        c.step(50)
+        assert np.array_equal(c.get_state(), q_target)
+    # and it differs from the default (Williamson) scheme's result
+    q_default, _ = run(oracle, P.sod(n=200, steps=50))
+    assert not np.array_equal(q_default, q_target)
+
+
+def test_target_initialises_from_a_function(oracle):
+    NS = NS_Split("Feiereisen", 3, [], viscosity="constant")
+    alg = _block(3, [StoreSome(4, "u0 u1 u2"), RungeKuttaLS(3)], [PeriodicBC(d, s) for d in range(3) for s in range(2)])
+    n = 16
+    params = {"gama": 1.4, "Minf": 0.1, "Re": 1600.0, "Pr": 0.71, "dt": 0.003385 * 128 / n, "niter": 2}
+    for d in range(3):
+        params[f"block0np{d}"] = n
+        params[f"Delta{d}block0"] = f"2*M_PI/block0np{d}"
+    pr = P.tgv(n=n, steps=2)
+
+    def field(x, y, z):
+        return pr.q0       # the initial field on the grid the target reports
+
+    with HIPTarget(alg, parameters=params, physics=NS, library=oracle) as t:
+        x, y, z = t.coordinates()
+        assert x.shape == (n, n, n) and x[0, 0, 1] == 2 * math.pi / n and z[1, 0, 0] == 2 * math.pi / n
+        t.initialise(field).run()
+        q = t.state()
+    want, _ = run(oracle, pr)
+    assert np.array_equal(q, want)
