@@ -117,6 +117,7 @@ typedef struct osb_stats {
 
 typedef struct osb_ctx osb_ctx;
 typedef struct osb_batch osb_batch;
+typedef struct osb_pool osb_pool;
 
 /* Lifecycle.  Replaces OPSC(alg) + ops_init/ops_decl_block/ops_decl_dat
  * (opsc.py:408, opsc.py:838-960): one context owns one block's arrays.      */
@@ -124,6 +125,8 @@ int  osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out);
 int  osb_ctx_destroy(osb_ctx* ctx);
 const char* osb_last_error(const osb_ctx* ctx);   /* ctx may be NULL: last create error */
 int  osb_abi_version(void);
+/* Device bytes a context for `desc` allocates at creation (admission control). */
+int  osb_estimate_bytes(const osb_problem_desc* desc, int64_t* out);
 
 /* State transfer.  Replaces GridBasedInitialisation / ops_decl_dat_hdf5
  * (initialisation/gridbasedinit.py:16, opsc.py:962-977).                   */
@@ -174,6 +177,25 @@ int  osb_batch_get_state(osb_batch* b, int i, double* host, int layout);
 int  osb_batch_step(osb_batch* b, int nsteps);
 int  osb_batch_sync(osb_batch* b);
 const char* osb_batch_last_error(const osb_batch* b);
+
+/* Serving pool (HIP library only): continuous batching of independent solves.
+ * osb_pool_admit creates a solve if `max_solves` slots and the device-memory
+ * budget allow it (else OSB_E_NOMEM: the caller keeps it queued).  Each
+ * osb_pool_advance enqueues, without waiting, the next chunk of at most
+ * `chunk` steps of every solve whose previous chunk has finished: solves of
+ * one shock-capturing signature advance together through the batched
+ * launches (a solve admitted meanwhile joins the next chunk); the others run
+ * on side streams.  osb_pool_poll lists finished solves; fetch their state,
+ * then release the slot and its memory.  No counterpart in the reference.   */
+int osb_pool_create(int64_t hbm_budget_bytes, int max_solves, int nstreams, osb_pool** out);
+int osb_pool_destroy(osb_pool* p);
+int osb_pool_admit(osb_pool* p, const osb_problem_desc* desc, const double* host_q0, int nsteps, int* slot);
+int osb_pool_advance(osb_pool* p, int chunk);
+int osb_pool_poll(osb_pool* p, int* done_slots, int cap, int* ndone);
+int osb_pool_fetch(osb_pool* p, int slot, double* host, osb_stats* st);
+int osb_pool_release(osb_pool* p, int slot);
+int osb_pool_usage(osb_pool* p, int64_t* used_bytes, int* running, int* done);
+const char* osb_pool_last_error(const osb_pool* p);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,132 @@
+// osb_batch.hpp -- batched kernels: one launch advances every instance of a
+// batch group (same dimension, scheme, precision and time scheme; any grid
+// sizes).  Each instance's work is a contiguous run of virtual blocks; the
+// launch is persistent (grid-stride over virtual blocks), and its grid size is
+// a tuning parameter.  The per-point work is the same device code as the
+// single-block kernels, so results are bitwise identical to stepping each
+// instance alone.
+#pragma once
+
+#include "osb_kernels.hpp"
+
+namespace osb {
+
+struct BItem {
+    double* q[2];          // state ping-pong; the input of a stage is q[parity]
+    double* du;            // RK-LS register (q_old for the classic RK3)
+    double* F;             // interface fluxes, ndim x nv x Ptot
+    double* scal;          // [0] wave-speed bits, [1] dt, [2] device time
+    Geom g;
+    Consts c;
+    int bc[3][2];
+    double cfl;            // > 0: dt from the wave-speed reduction
+    double rkA[5], rkB[5]; // stage coefficients (classic RK3: rknew, rkold)
+};
+
+// index of the item owning virtual block vb: largest i with offs[i] <= vb
+__device__ __forceinline__ int find_item(const int* __restrict__ offs, int n, int vb) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (offs[mid] <= vb) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+#define OSB_BATCH_LOOP                                                   \
+    const int total_ = offs[n];                                          \
+    for (int vb = blockIdx.x; vb < total_; vb += gridDim.x)
+
+template <int NDIM, int DIR, int SCH, typename T>
+__global__ void __launch_bounds__(BLOCK) kb_face_flux(const BItem* __restrict__ items, const int* __restrict__ offs, int n,
+                                                      int parity) {
+    OSB_BATCH_LOOP {
+        const int i = find_item(offs, n, vb);
+        const BItem& it = items[i];
+        const long t = (long)(vb - offs[i]) * BLOCK + threadIdx.x;
+        // fluxes of direction DIR occupy the DIR-th nv x Ptot block of F
+        face_flux_item<NDIM, DIR, SCH, T>(t, it.q[parity], it.F + (long)DIR * (NDIM + 2) * it.g.Ptot, it.g, it.c);
+    }
+}
+
+template <int NDIM, int TIME>
+__global__ void __launch_bounds__(BLOCK) kb_resid_rk(const BItem* __restrict__ items, const int* __restrict__ offs, int n,
+                                                     int parity, int stage) {
+    OSB_BATCH_LOOP {
+        const int i = find_item(offs, n, vb);
+        const BItem& it = items[i];
+        const long t = (long)(vb - offs[i]) * BLOCK + threadIdx.x;
+        resid_rk_item<NDIM, TIME>(t, it.F, it.q[parity], it.q[parity ^ 1], it.du, it.scal[1], it.g, it.c,
+                                  it.rkA[stage], it.rkB[stage]);
+    }
+}
+
+template <int DIR>
+__global__ void __launch_bounds__(BLOCK) kb_bc(const BItem* __restrict__ items, const int* __restrict__ offs, int n,
+                                               int parity) {
+    OSB_BATCH_LOOP {
+        const int i = find_item(offs, n, vb);
+        const BItem& it = items[i];
+        const long t = (long)(vb - offs[i]) * BLOCK + threadIdx.x;
+        bc_item<DIR>(t, it.q[parity], it.g, it.g.ndim + 2, it.bc[DIR][0], it.bc[DIR][1]);
+    }
+}
+
+// classic RK3: q_old = q at the start of the step (interior)
+static __global__ void __launch_bounds__(BLOCK) kb_save_state(const BItem* __restrict__ items, const int* __restrict__ offs, int n,
+                                                       int parity) {
+    OSB_BATCH_LOOP {
+        const int i = find_item(offs, n, vb);
+        const BItem& it = items[i];
+        const Geom& g = it.g;
+        const long t = (long)(vb - offs[i]) * BLOCK + threadIdx.x;
+        const int e[3] = {g.n[0], g.n[1], g.n[2]};
+        const int lo[3] = {0, 0, 0};
+        if (t >= (long)e[0] * e[1] * e[2]) continue;
+        int a, b, c;
+        unflatten(t, e, lo, a, b, c);
+        const long x = gidx(g, a, b, c);
+        for (int v = 0; v < g.ndim + 2; ++v) it.du[(long)v * g.Ptot + x] = it.q[parity][(long)v * g.Ptot + x];
+    }
+}
+
+// CFL: reset, per-instance max wave speed, dt = cfl / max and time += dt
+static __global__ void kb_zero_wavespeed(const BItem* __restrict__ items, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && items[i].cfl > 0) reinterpret_cast<unsigned long long*>(items[i].scal)[0] = 0ull;
+}
+
+template <int NDIM>
+__global__ void __launch_bounds__(BLOCK) kb_wavespeed(const BItem* __restrict__ items, const int* __restrict__ offs, int n,
+                                                      int parity) {
+    OSB_BATCH_LOOP {
+        const int i = find_item(offs, n, vb);
+        const BItem& it = items[i];
+        const Geom& g = it.g;
+        const long t = (long)(vb - offs[i]) * BLOCK + threadIdx.x;
+        const int e[3] = {g.n[0], g.n[1], g.n[2]};
+        const int lo[3] = {0, 0, 0};
+        double m = 0.0;
+        if (it.cfl > 0 && t < (long)e[0] * e[1] * e[2]) {
+            int a, b, c;
+            unflatten(t, e, lo, a, b, c);
+            m = wavespeed_point<NDIM>(it.q[parity], gidx(g, a, b, c), g, it.c);
+        }
+        if (it.cfl > 0) block_max_to(m, it.scal);     // uniform over the block
+    }
+}
+
+static __global__ void kb_dt_finalize(const BItem* __restrict__ items, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !(items[i].cfl > 0)) return;
+    double* scal = items[i].scal;
+    const double m = __longlong_as_double((long long)reinterpret_cast<unsigned long long*>(scal)[0]);
+    const double dt = items[i].cfl / m;
+    scal[1] = dt;
+    scal[2] += dt;
+}
+
+#undef OSB_BATCH_LOOP
+
+}  // namespace osb

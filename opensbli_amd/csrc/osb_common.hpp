@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <type_traits>
 
 namespace osb {
 
@@ -184,7 +185,7 @@ __device__ __forceinline__ void eigensystem(T gm1, T rho, const T* u, T a, T (&L
     for (int i = 0; i < NV; ++i)
 #pragma unroll
         for (int j = 0; j < NV; ++j) { L[i][j] = T(0); R[i][j] = T(0); }
-    if (NDIM == 1) {
+    if constexpr (NDIM == 1) {
         const T u0 = u[0];
         const T inva = T(1) / a;
         const T H = (a * a) * (T(1) / gm1) + (u0 * u0) * T(0.5);
@@ -202,8 +203,7 @@ __device__ __forceinline__ void eigensystem(T gm1, T rho, const T* u, T a, T (&L
         L[2][0] = u0 * (-T(2) * H + a * u0 + u0 * u0) * T(0.5) * inva * invD;
         L[2][1] = (H - a * u0 - (u0 * u0) * T(0.5)) * inva * invD;
         L[2][2] = invD;
-        return;
-    }
+    } else {
     constexpr T k0 = DIR == 0 ? 1 : 0, k1 = DIR == 1 ? 1 : 0, k2 = DIR == 2 ? 1 : 0;
     T q2 = T(0);
 #pragma unroll
@@ -216,7 +216,7 @@ __device__ __forceinline__ void eigensystem(T gm1, T rho, const T* u, T a, T (&L
     const T bta = T(INV_SQRT2) * invrho * inva;
     const T igm1 = T(1) / gm1;
     const T a2 = a * a;
-    if (NDIM == 2) {
+    if constexpr (NDIM == 2) {
         const T u0 = u[0], u1 = u[1];
         R[0][0] = 1; R[0][2] = alpha; R[0][3] = alpha;
         R[1][0] = u0; R[1][1] = k1 * rho; R[1][2] = alpha * (u0 + k0 * a); R[1][3] = alpha * (u0 - k0 * a);
@@ -230,8 +230,7 @@ __device__ __forceinline__ void eigensystem(T gm1, T rho, const T* u, T a, T (&L
         L[2][2] = bta * (k1 * a - gm1 * u1); L[2][3] = bta * gm1;
         L[3][0] = bta * (phi_sq + a * theta); L[3][1] = -bta * (k0 * a + gm1 * u0);
         L[3][2] = -bta * (k1 * a + gm1 * u1); L[3][3] = bta * gm1;
-        return;
-    }
+    } else {
     const T u0 = u[0], u1 = u[1], u2 = u[2];
     R[0][0] = k0; R[0][1] = k1; R[0][2] = k2; R[0][3] = alpha; R[0][4] = alpha;
     R[1][0] = k0 * u0; R[1][1] = k1 * u0 - k2 * rho; R[1][2] = k2 * u0 + k1 * rho;
@@ -271,6 +270,8 @@ __device__ __forceinline__ void eigensystem(T gm1, T rho, const T* u, T a, T (&L
     L[4][2] = -bta * (gm1 * u1 + k1 * a);
     L[4][3] = -bta * (gm1 * u2 + k2 * a);
     L[4][4] = bta * gm1;
+    }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -282,12 +283,19 @@ template <int NDIM, int DIR, int SCH, typename T>
 __device__ __forceinline__ void face_flux(const double* __restrict__ q, long Ptot, long st, long x0,
                                           const Consts& c, double (&out)[NDIM + 2]) {
     constexpr int NV = NDIM + 2;
+    // T = double: the reference's arithmetic.  T = float (mixed precision): the
+    // characteristic work runs on deviations from the state and flux at point
+    // i (s = 2), formed in FP64 and then rounded to FP32, so FP32 holds only
+    // the local variation.  WENO-Z and TENO are translation invariant
+    // (rec(f + c) = rec(f) + c: smoothness indicators see differences, the
+    // candidate weights sum to one), and f+ + f- = L F, so the interface flux
+    // is F_i + R rec(deviations) with the FP64 F_i added back at the end.
+    constexpr bool DEV = !std::is_same<T, double>::value;
     T qs[6][NV];
     T F[6][NV];
-    T ud[6];
+    double base_q[NV], base_F[NV];
     double rho23[2], u23[2][NDIM], p23[2], E23[2], a23[2];
-#pragma unroll
-    for (int s = 0; s < 6; ++s) {
+    auto point = [&](const int s) {
         const long xs = x0 + (long)(s - 2) * st;
         double v[NV];
 #pragma unroll
@@ -299,13 +307,22 @@ __device__ __forceinline__ void face_flux(const double* __restrict__ q, long Pto
         for (int d = 0; d < NDIM; ++d) { uu[d] = v[1 + d] / r; ke -= 0.5 * (uu[d] * uu[d]) * r; }
         const double p = c.gamma_m1 * (ke + v[NV - 1]);
         const double u_dir = uu[DIR];
+        double f[NV];
+        f[0] = v[1 + DIR];
 #pragma unroll
-        for (int m = 0; m < NV; ++m) qs[s][m] = T(v[m]);
-        ud[s] = T(u_dir);
-        F[s][0] = T(v[1 + DIR]);
+        for (int d = 0; d < NDIM; ++d) f[1 + d] = v[1 + d] * u_dir + (d == DIR ? p : 0.0);
+        f[NV - 1] = (p + v[NV - 1]) * u_dir;
+        if constexpr (DEV) {
+            if (s == 2) {
 #pragma unroll
-        for (int d = 0; d < NDIM; ++d) F[s][1 + d] = T(v[1 + d] * u_dir + (d == DIR ? p : 0.0));
-        F[s][NV - 1] = T((p + v[NV - 1]) * u_dir);
+                for (int m = 0; m < NV; ++m) { base_q[m] = v[m]; base_F[m] = f[m]; }
+            }
+#pragma unroll
+            for (int m = 0; m < NV; ++m) { qs[s][m] = T(v[m] - base_q[m]); F[s][m] = T(f[m] - base_F[m]); }
+        } else {
+#pragma unroll
+            for (int m = 0; m < NV; ++m) { qs[s][m] = T(v[m]); F[s][m] = T(f[m]); }
+        }
         if (s == 2 || s == 3) {
             const int w = s - 2;
             rho23[w] = r; p23[w] = p; E23[w] = v[NV - 1];
@@ -313,7 +330,13 @@ __device__ __forceinline__ void face_flux(const double* __restrict__ q, long Pto
             for (int d = 0; d < NDIM; ++d) u23[w][d] = uu[d];
             a23[w] = sqrt(c.gama * p / r);
         }
-    }
+    };
+    point(2);              // the base point first (mixed precision subtracts it)
+    point(0);
+    point(1);
+    point(3);
+    point(4);
+    point(5);
     // averaged state (averaging.py:31-124)
     T rho_av, u_av[NDIM], a_av;
     if (c.avg_roe) {
@@ -370,8 +393,13 @@ __device__ __forceinline__ void face_flux(const double* __restrict__ q, long Pto
 #pragma unroll
         for (int m = 0; m < NV; ++m) acc[m] = fma(R[m][k], rec, acc[m]);
     }
+    if (DEV) {
 #pragma unroll
-    for (int m = 0; m < NV; ++m) out[m] = double(acc[m]);
+        for (int m = 0; m < NV; ++m) out[m] = base_F[m] + double(acc[m]);
+    } else {
+#pragma unroll
+        for (int m = 0; m < NV; ++m) out[m] = double(acc[m]);
+    }
 }
 
 }  // namespace osb

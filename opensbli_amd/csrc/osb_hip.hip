@@ -10,9 +10,7 @@
 // all SoA with `halo` ghost layers, x fastest.  Every launch goes to the
 // context's stream; nothing synchronises except the calls that return data.
 
-#include "osb_kernels.hpp"
-#include "osb_central_fused.hpp"
-#include "../../include/osb.h"
+#include "osb_runtime.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -24,68 +22,12 @@
 
 using namespace osb;
 
-namespace {
-
-std::string g_create_error;
-
-struct DevBuf {
-    double* p = nullptr;
-    size_t bytes = 0;
-};
-
-}  // namespace
-
-struct osb_ctx {
-    osb_problem_desc d{};
-    Geom g{};
-    Consts c{};
-    int nv = 0;
-    bool shock = false, central = false, viscous = false;
-    int sch = SCH_WENOZ;
-    bool mixed = false;
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    int device = 0;
-
-    double* qa = nullptr;
-    double* qb = nullptr;
-    double* q_cur = nullptr;        // current state
-    double* q_prev = nullptr;       // input of the last stage of the last step
-    double* du = nullptr;
-    double* R = nullptr;
-    double* F = nullptr;
-    double* cr = nullptr;
-    double* gr = nullptr;
-    double* scal = nullptr;         // [0] max wavespeed bits, [1] dt, [2] device time, [3..6] reduction out
-    double* staging = nullptr;      // interior-layout transfer buffer
-    double* part = nullptr;         // reduction partials
-    unsigned int* nan_cnt = nullptr;
-    int part_blocks = 0;
-    size_t bytes = 0;
-
-    double time = 0.0;
-    int64_t iter = 0;
-    double last_dt = 0.0;
-    bool have_prev = false;
-    std::string err;
-    int rk_stages = 3;              // RK-LS stages and coefficients (rk_LS.py:97-132)
-    double rkA[5] = {0}, rkB[5] = {0};
-    int fused_tile = 0;             // index into CF_TILES (autotuned)
-    int xcd_remap = 0;              // XCD-aware workgroup-to-tile order (OSB_XCD_REMAP=1)
-
-    // kernel timing (osb_profile_*)
-    bool prof_on = false;
-    struct Rec { int fam; hipEvent_t a, b; };
-    std::vector<Rec> prof_pending;
-    std::vector<hipEvent_t> prof_pool;
-    double prof_ms[16] = {0};
-    int64_t prof_n[16] = {0};
-};
+std::string g_osb_create_error;
 
 namespace {
 
 int set_err(osb_ctx* c, int code, const std::string& m) {
-    if (c) c->err = m; else g_create_error = m;
+    if (c) c->err = m; else g_osb_create_error = m;
     return code;
 }
 
@@ -201,6 +143,26 @@ void rhs_nd(osb_ctx* c, const double* q) {
     }
 }
 
+// Shock capturing without viscous terms: face fluxes, then one kernel that
+// differences them and applies the RK update (no residual array).
+template <int NDIM, int TIME>
+void shock_stage_nd(osb_ctx* c, const double* q, double* qn, double A, double B) {
+    launch_flux<NDIM, 0>(c, q);
+    if (NDIM > 1) launch_flux<NDIM, (NDIM > 1 ? 1 : 0)>(c, q);
+    if (NDIM > 2) launch_flux<NDIM, (NDIM > 2 ? 2 : 0)>(c, q);
+    Timed tm(c, PF_RK);
+    k_resid_rk<NDIM, TIME><<<nblocks(interior_points(c->g)), BLOCK, 0, c->stream>>>(c->F, q, qn, c->du, c->scal, c->g, c->c, A, B);
+}
+
+template <int TIME>
+void shock_stage(osb_ctx* c, const double* q, double* qn, double A, double B) {
+    switch (c->g.ndim) {
+        case 1: shock_stage_nd<1, TIME>(c, q, qn, A, B); break;
+        case 2: shock_stage_nd<2, TIME>(c, q, qn, A, B); break;
+        default: shock_stage_nd<3, TIME>(c, q, qn, A, B); break;
+    }
+}
+
 void rhs(osb_ctx* c, const double* q) {
     switch (c->g.ndim) {
         case 1: rhs_nd<1>(c, q); break;
@@ -277,6 +239,7 @@ void fused_stage(osb_ctx* c, const double* q, double* qn, double A, double B) {
 }
 
 bool use_fused(const osb_ctx* c) { return c->g.ndim == 3 && (c->central || c->viscous); }
+bool use_shock_stage(const osb_ctx* c) { return c->shock && !c->central && !c->viscous; }
 
 int check_launch(osb_ctx* c) {
     hipError_t e = hipGetLastError();
@@ -296,6 +259,8 @@ int do_step(osb_ctx* c) {
             double* qn = (c->q_cur == c->qa) ? c->qb : c->qa;
             if (use_fused(c)) {
                 fused_stage<1>(c, c->q_cur, qn, c->rkA[s], c->rkB[s]);
+            } else if (use_shock_stage(c)) {
+                shock_stage<1>(c, c->q_cur, qn, c->rkA[s], c->rkB[s]);
             } else {
                 rhs(c, c->q_cur);
                 Timed tm(c, PF_RK);
@@ -313,6 +278,8 @@ int do_step(osb_ctx* c) {
             double* qn = (c->q_cur == c->qa) ? c->qb : c->qa;
             if (use_fused(c)) {
                 fused_stage<2>(c, c->q_cur, qn, rknew[s], rkold[s]);
+            } else if (use_shock_stage(c)) {
+                shock_stage<2>(c, c->q_cur, qn, rknew[s], rkold[s]);
             } else {
                 rhs(c, c->q_cur);
                 Timed tm(c, PF_RK);
@@ -373,6 +340,72 @@ int dalloc(osb_ctx* c, double** p, size_t count) {
     return OSB_OK;
 }
 
+Geom make_geom(const osb_problem_desc* desc) {
+    const int nd = desc->ndim;
+    const bool shock = desc->conv_scheme == OSB_CONV_WENO_Z || desc->conv_scheme == OSB_CONV_TENO;
+    Geom g{};
+    g.ndim = nd;
+    g.halo = shock ? 4 : 2;     // symmetric scheme halo: WENO5/TENO [-3,4] -> 4, Central(4) -> 2
+    for (int i = 0; i < 3; ++i) {
+        g.n[i] = desc->n[i];
+        g.P[i] = i < nd ? desc->n[i] + 2 * g.halo : 1;
+        g.o[i] = i < nd ? g.halo : 0;
+    }
+    g.s[0] = 1;
+    g.s[1] = g.P[0];
+    g.s[2] = (long)g.P[0] * g.P[1];
+    g.Ptot = (long)g.P[0] * g.P[1] * g.P[2];
+    return g;
+}
+
+constexpr int PART_BLOCKS = 1024;
+
+// device arrays (in doubles) a context allocates at creation.  The residual
+// and the constituent-relation / gradient arrays are needed for stepping only
+// by the unfused central path (ndim < 3); otherwise osb_residual and
+// osb_tgv_diagnostics allocate them on first use.
+struct AllocPlan {
+    size_t q = 0, R = 0, F = 0, cr = 0, gr = 0, staging = 0;
+    size_t bytes() const {
+        return sizeof(double) * (3 * q + R + F + cr + gr + 16 + staging + (size_t)PART_BLOCKS * 4) + sizeof(unsigned int);
+    }
+};
+
+AllocPlan plan_alloc(const osb_problem_desc* d) {
+    const Geom g = make_geom(d);
+    const int nd = d->ndim, nv = nd + 2;
+    const bool shock = d->conv_scheme == OSB_CONV_WENO_Z || d->conv_scheme == OSB_CONV_TENO;
+    const bool central = !shock, viscous = d->viscous != OSB_VISC_NONE;
+    const bool fused = nd == 3 && (central || viscous);
+    const bool unfused_central = (central || viscous) && !fused;
+    const size_t P = (size_t)g.Ptot;
+    AllocPlan a;
+    a.q = (size_t)nv * P;
+    a.R = unfused_central ? a.q : 0;
+    a.F = shock ? (size_t)nd * a.q : 0;
+    a.cr = unfused_central ? (size_t)(nd + 3) * P : 0;
+    a.gr = unfused_central ? (size_t)nd * nd * P : 0;
+    a.staging = (size_t)nv * (size_t)g.n[0] * g.n[1] * g.n[2];
+    return a;
+}
+
+int ensure_buf(osb_ctx* c, double** p, size_t count) {
+    if (*p) return OSB_OK;
+    int r = dalloc(c, p, count);
+    if (r) c->err = g_osb_create_error;
+    return r;
+}
+
+// arrays of the unfused right-hand side (osb_residual on any path)
+int ensure_rhs_arrays(osb_ctx* c) {
+    const size_t P = (size_t)c->g.Ptot;
+    const int nd = c->g.ndim;
+    int r = ensure_buf(c, &c->R, (size_t)c->nv * P);
+    if (!r && (c->central || c->viscous)) r = ensure_buf(c, &c->cr, (size_t)(nd + 3) * P);
+    if (!r && (c->central || c->viscous)) r = ensure_buf(c, &c->gr, (size_t)nd * nd * P);
+    return r;
+}
+
 void free_ctx(osb_ctx* c) {
     if (!c) return;
     double* bufs[] = {c->qa, c->qb, c->du, c->R, c->F, c->cr, c->gr, c->scal, c->staging, c->part};
@@ -390,7 +423,16 @@ extern "C" {
 
 int osb_abi_version(void) { return OSB_ABI_VERSION; }
 
-const char* osb_last_error(const osb_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+int osb_estimate_bytes(const osb_problem_desc* desc, int64_t* out) {
+    if (!out) return set_err(nullptr, OSB_E_INVALID, "null output pointer");
+    std::string why;
+    int rc = validate(desc, why);
+    if (rc) return set_err(nullptr, rc, why);
+    *out = (int64_t)plan_alloc(desc).bytes();
+    return OSB_OK;
+}
+
+const char* osb_last_error(const osb_ctx* ctx) { return ctx ? ctx->err.c_str() : g_osb_create_error.c_str(); }
 
 int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     if (!out) return set_err(nullptr, OSB_E_INVALID, "null output pointer");
@@ -422,18 +464,8 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     }
     c->sch = desc->conv_scheme == OSB_CONV_WENO_Z ? SCH_WENOZ : (desc->order == 6 ? SCH_TENO6 : SCH_TENO5);
 
-    Geom& g = c->g;
-    g.ndim = nd;
-    g.halo = c->shock ? 4 : 2;     // symmetric scheme halo: WENO5/TENO [-3,4] -> 4, Central(4) -> 2
-    for (int i = 0; i < 3; ++i) {
-        g.n[i] = desc->n[i];
-        g.P[i] = i < nd ? desc->n[i] + 2 * g.halo : 1;
-        g.o[i] = i < nd ? g.halo : 0;
-    }
-    g.s[0] = 1;
-    g.s[1] = g.P[0];
-    g.s[2] = (long)g.P[0] * g.P[1];
-    g.Ptot = (long)g.P[0] * g.P[1] * g.P[2];
+    c->g = make_geom(desc);
+    const Geom& g = c->g;
 
     Consts& k = c->c;
     k.gama = desc->gama;
@@ -454,18 +486,18 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     k.avg_roe = desc->averaging == OSB_AVG_ROE;
     k.viscous = desc->viscous;
 
-    const size_t P = (size_t)g.Ptot, nvP = (size_t)c->nv * P;
+    const AllocPlan ap = plan_alloc(desc);
     int r = OSB_OK;
-    if (!r) r = dalloc(c, &c->qa, nvP);
-    if (!r) r = dalloc(c, &c->qb, nvP);
-    if (!r) r = dalloc(c, &c->du, nvP);
-    if (!r) r = dalloc(c, &c->R, nvP);
-    if (!r && c->shock) r = dalloc(c, &c->F, (size_t)nd * nvP);
-    if (!r && (c->central || c->viscous)) r = dalloc(c, &c->cr, (size_t)(nd + 3) * P);
-    if (!r && (c->central || c->viscous)) r = dalloc(c, &c->gr, (size_t)nd * nd * P);
+    if (!r) r = dalloc(c, &c->qa, ap.q);
+    if (!r) r = dalloc(c, &c->qb, ap.q);
+    if (!r) r = dalloc(c, &c->du, ap.q);
+    if (!r && ap.R) r = dalloc(c, &c->R, ap.R);
+    if (!r && ap.F) r = dalloc(c, &c->F, ap.F);
+    if (!r && ap.cr) r = dalloc(c, &c->cr, ap.cr);
+    if (!r && ap.gr) r = dalloc(c, &c->gr, ap.gr);
     if (!r) r = dalloc(c, &c->scal, 16);
-    if (!r) r = dalloc(c, &c->staging, (size_t)c->nv * interior_points(g));
-    c->part_blocks = 1024;
+    if (!r) r = dalloc(c, &c->staging, ap.staging);
+    c->part_blocks = PART_BLOCKS;
     if (!r) r = dalloc(c, &c->part, (size_t)c->part_blocks * 4);
     if (!r) {
         hipError_t e = hipMalloc((void**)&c->nan_cnt, sizeof(unsigned int));
@@ -550,6 +582,8 @@ int osb_apply_bcs(osb_ctx* c) {
 
 int osb_residual(osb_ctx* c, double* host_out) {
     if (!c || !host_out) return OSB_E_INVALID;
+    int r = ensure_rhs_arrays(c);
+    if (r) return r;
     apply_bcs(c, c->q_cur);
     rhs(c, c->q_cur);
     const long npts = interior_points(c->g);
@@ -610,7 +644,9 @@ int osb_get_stats(osb_ctx* c, osb_stats* out) {
 int osb_tgv_diagnostics(osb_ctx* c, double out[4]) {
     if (!c || !out) return OSB_E_INVALID;
     if (c->g.ndim != 3) return set_err(c, OSB_E_UNSUPPORTED, "TGV diagnostics need ndim == 3");
-    if (!c->cr) return set_err(c, OSB_E_UNSUPPORTED, "TGV diagnostics need a viscous configuration");
+    if (!(c->central || c->viscous)) return set_err(c, OSB_E_UNSUPPORTED, "TGV diagnostics need a viscous configuration");
+    int r = ensure_buf(c, &c->cr, (size_t)(c->g.ndim + 3) * c->g.Ptot);
+    if (r) return r;
     // velocity and mu from the state at the start of the last stage (the
     // reference's constituent-relation arrays), rho from the current state
     const double* qcr = c->have_prev ? c->q_prev : c->q_cur;
@@ -660,77 +696,6 @@ int osb_profile_read(osb_ctx* c, int idx, char* name, int name_len, double* tota
 int osb_state_device_ptr(osb_ctx* c, int var, void** out) {
     if (!c || !out || var < 0 || var >= c->nv) return OSB_E_INVALID;
     *out = (void*)(c->q_cur + (long)var * c->g.Ptot);
-    return OSB_OK;
-}
-
-}  // extern "C"
-
-// ---------------------------------------------------------------------------
-// Batched ensembles
-// ---------------------------------------------------------------------------
-struct osb_batch {
-    std::vector<osb_ctx*> ctx;
-    std::string err;
-};
-
-extern "C" {
-
-const char* osb_batch_last_error(const osb_batch* b) { return b ? b->err.c_str() : g_create_error.c_str(); }
-
-int osb_batch_create(const osb_problem_desc* descs, int n, osb_batch** out) {
-    if (!out || !descs || n < 1) return set_err(nullptr, OSB_E_INVALID, "bad batch arguments");
-    *out = nullptr;
-    osb_batch* b = new osb_batch();
-    for (int i = 0; i < n; ++i) {
-        osb_ctx* c = nullptr;
-        int r = osb_ctx_create(&descs[i], &c);
-        if (r) {
-            for (osb_ctx* x : b->ctx) osb_ctx_destroy(x);
-            delete b;
-            return r;
-        }
-        b->ctx.push_back(c);
-    }
-    *out = b;
-    return OSB_OK;
-}
-
-int osb_batch_destroy(osb_batch* b) {
-    if (!b) return OSB_OK;
-    for (osb_ctx* c : b->ctx) osb_ctx_destroy(c);
-    delete b;
-    return OSB_OK;
-}
-
-int osb_batch_set_state(osb_batch* b, int i, const double* host, int layout) {
-    if (!b || i < 0 || i >= (int)b->ctx.size()) return OSB_E_INVALID;
-    int r = osb_set_state(b->ctx[i], host, layout);
-    if (r) b->err = b->ctx[i]->err;
-    return r;
-}
-
-int osb_batch_get_state(osb_batch* b, int i, double* host, int layout) {
-    if (!b || i < 0 || i >= (int)b->ctx.size()) return OSB_E_INVALID;
-    int r = osb_get_state(b->ctx[i], host, layout);
-    if (r) b->err = b->ctx[i]->err;
-    return r;
-}
-
-int osb_batch_step(osb_batch* b, int nsteps) {
-    if (!b || nsteps < 0) return OSB_E_INVALID;
-    for (osb_ctx* c : b->ctx) {
-        int r = osb_step(c, nsteps);
-        if (r) { b->err = c->err; return r; }
-    }
-    return OSB_OK;
-}
-
-int osb_batch_sync(osb_batch* b) {
-    if (!b) return OSB_E_INVALID;
-    for (osb_ctx* c : b->ctx) {
-        int r = osb_sync(c);
-        if (r) { b->err = c->err; return r; }
-    }
     return OSB_OK;
 }
 

@@ -23,13 +23,13 @@ __device__ __forceinline__ void unflatten(long t, const int (&e)[3], const int (
 // interior across (TENO6/opensbli.cpp:263-316 ranges).
 // ---------------------------------------------------------------------------
 template <int NDIM, int DIR, int SCH, typename T>
-__global__ void __launch_bounds__(BLOCK) k_face_flux(const double* __restrict__ q, double* __restrict__ F, Geom g, Consts c) {
+__device__ __forceinline__ void face_flux_item(long t, const double* __restrict__ q, double* __restrict__ F, const Geom& g,
+                                               const Consts& c) {
     constexpr int NV = NDIM + 2;
     int e[3] = {g.n[0], g.n[1], g.n[2]};
     e[DIR] += 1;
     int lo[3] = {0, 0, 0};
     lo[DIR] = -1;
-    const long t = gtid();
     if (t >= (long)e[0] * e[1] * e[2]) return;
     int i, j, k;
     unflatten(t, e, lo, i, j, k);
@@ -38,6 +38,30 @@ __global__ void __launch_bounds__(BLOCK) k_face_flux(const double* __restrict__ 
     face_flux<NDIM, DIR, SCH, T>(q, g.Ptot, g.s[DIR], x0, c, out);
 #pragma unroll
     for (int v = 0; v < NV; ++v) F[(long)v * g.Ptot + x0] = out[v];
+}
+template <int NDIM, int DIR>
+__host__ __device__ inline long face_flux_work(const Geom& g) {
+    long w = 1;
+    for (int d = 0; d < 3; ++d) w *= (long)g.n[d] + (d == DIR ? 1 : 0);
+    return w;
+}
+
+template <int NDIM, int DIR, int SCH, typename T>
+__global__ void __launch_bounds__(BLOCK) k_face_flux(const double* __restrict__ q, double* __restrict__ F, Geom g, Consts c) {
+    face_flux_item<NDIM, DIR, SCH, T>(gtid(), q, F, g, c);
+}
+
+// flux difference at interior point x: -sum_d (F_d(x) - F_d(x - s_d)) / Delta_d
+template <int NDIM>
+__device__ __forceinline__ double flux_difference(const double* __restrict__ F, long nvP, long P, int v, long x, const Geom& g,
+                                                  const Consts& c) {
+    double acc = 0.0;
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) {
+        const double* w = F + d * nvP + (long)v * P;
+        acc -= (w[x] - w[x - g.s[d]]) * c.invd[d];
+    }
+    return acc;
 }
 
 // R_v = -sum_d (F_d(i) - F_d(i-1)) / Delta_d   (shock_capturing.py:23-36)
@@ -53,15 +77,45 @@ __global__ void __launch_bounds__(BLOCK) k_flux_residual(const double* __restric
     const long x = gidx(g, i, j, k);
     const long nvP = (long)NV * g.Ptot;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        double acc = 0.0;
+    for (int v = 0; v < NV; ++v) R[(long)v * g.Ptot + x] = flux_difference<NDIM>(F, nvP, g.Ptot, v, x, g, c);
+}
+
+// Flux-difference residual fused with the RK stage update (no residual array):
+// TIME 1, RK-LS (rk_LS.py:204-215): du = A du + dt R; q_out = q_in + B du
+// TIME 2, classic RK3 (rk_sbli.py:129-159): q_out = qold + dt*A*R; qold += dt*B*R  (du holds qold)
+template <int NDIM, int TIME>
+__device__ __forceinline__ void resid_rk_item(long t, const double* __restrict__ F, const double* __restrict__ qin,
+                                              double* __restrict__ qout, double* __restrict__ du, double dt, const Geom& g,
+                                              const Consts& c, double A, double B) {
+    constexpr int NV = NDIM + 2;
+    const int e[3] = {g.n[0], g.n[1], g.n[2]};
+    const int lo[3] = {0, 0, 0};
+    if (t >= (long)e[0] * e[1] * e[2]) return;
+    int i, j, k;
+    unflatten(t, e, lo, i, j, k);
+    const long x = gidx(g, i, j, k);
+    const long nvP = (long)NV * g.Ptot;
 #pragma unroll
-        for (int d = 0; d < NDIM; ++d) {
-            const double* w = F + d * nvP + (long)v * g.Ptot;
-            acc -= (w[x] - w[x - g.s[d]]) * c.invd[d];
+    for (int v = 0; v < NV; ++v) {
+        const double r = flux_difference<NDIM>(F, nvP, g.Ptot, v, x, g, c);
+        const long y = (long)v * g.Ptot + x;
+        if (TIME == 1) {
+            const double d = fma(A, du[y], dt * r);
+            du[y] = d;
+            qout[y] = fma(B, d, qin[y]);
+        } else {
+            const double o = du[y];
+            qout[y] = fma(dt * A, r, o);
+            du[y] = fma(dt * B, r, o);
         }
-        R[(long)v * g.Ptot + x] = acc;
     }
+}
+
+template <int NDIM, int TIME>
+__global__ void __launch_bounds__(BLOCK) k_resid_rk(const double* __restrict__ F, const double* __restrict__ qin,
+                                                    double* __restrict__ qout, double* __restrict__ du,
+                                                    const double* __restrict__ scal, Geom g, Consts c, double A, double B) {
+    resid_rk_item<NDIM, TIME>(gtid(), F, qin, qout, du, scal[1], g, c, A, B);
 }
 
 // Constituent relations u_i, p, T, mu on the box [-ext, n+ext)
@@ -286,7 +340,7 @@ __global__ void __launch_bounds__(BLOCK) k_viscous(const double* __restrict__ cr
 }
 
 // Low-storage RK3 stage (rk_LS.py:204-215): du = A du + dt R; q_out = q_in + B du
-__global__ void __launch_bounds__(BLOCK) k_rkls(const double* __restrict__ qin, double* __restrict__ qout, double* __restrict__ du,
+static __global__ void __launch_bounds__(BLOCK) k_rkls(const double* __restrict__ qin, double* __restrict__ qout, double* __restrict__ du,
                                                 const double* __restrict__ R, const double* __restrict__ scal, Geom g, int nv,
                                                 double A, double B) {
     const int e[3] = {g.n[0], g.n[1], g.n[2]};
@@ -306,7 +360,7 @@ __global__ void __launch_bounds__(BLOCK) k_rkls(const double* __restrict__ qin, 
 }
 
 // Classic RK3 (rk_sbli.py:129-159): q = qold + dt*rknew*R; qold += dt*rkold*R
-__global__ void __launch_bounds__(BLOCK) k_rk3_stage(double* __restrict__ qout, double* __restrict__ qold,
+static __global__ void __launch_bounds__(BLOCK) k_rk3_stage(double* __restrict__ qout, double* __restrict__ qold,
                                                      const double* __restrict__ R, const double* __restrict__ scal, Geom g, int nv,
                                                      double rknew, double rkold) {
     const int e[3] = {g.n[0], g.n[1], g.n[2]};
@@ -325,7 +379,7 @@ __global__ void __launch_bounds__(BLOCK) k_rk3_stage(double* __restrict__ qout, 
     }
 }
 
-__global__ void __launch_bounds__(BLOCK) k_copy_interior(const double* __restrict__ src, double* __restrict__ dst, Geom g, int nv) {
+static __global__ void __launch_bounds__(BLOCK) k_copy_interior(const double* __restrict__ src, double* __restrict__ dst, Geom g, int nv) {
     const int e[3] = {g.n[0], g.n[1], g.n[2]};
     const int lo[3] = {0, 0, 0};
     const long t = gtid();
@@ -340,7 +394,15 @@ __global__ void __launch_bounds__(BLOCK) k_copy_interior(const double* __restric
 // (t = 0 is the boundary point), side 1 likewise, over the halo-extended
 // range of the other directions.  bc0/bc1: OSB_BC_* of the two sides.
 template <int DIR>
-__global__ void __launch_bounds__(BLOCK) k_bc(double* __restrict__ q, Geom g, int nv, int bc0, int bc1) {
+__host__ __device__ inline long bc_work(const Geom& g) {
+    long w = 2L * (g.halo + 1);
+    for (int d = 0; d < g.ndim; ++d)
+        if (d != DIR) w *= (g.n[d] + 2 * g.halo);
+    return w;
+}
+
+template <int DIR>
+__device__ __forceinline__ void bc_item(long t, double* __restrict__ q, const Geom& g, int nv, int bc0, int bc1) {
     const int h = g.halo;
     int e[3], lo[3];
     for (int d = 0; d < 3; ++d) {
@@ -350,7 +412,6 @@ __global__ void __launch_bounds__(BLOCK) k_bc(double* __restrict__ q, Geom g, in
     }
     e[DIR] = 2 * (h + 1);
     lo[DIR] = 0;
-    const long t = gtid();
     if (t >= (long)e[0] * e[1] * e[2]) return;
     int ii[3];
     unflatten(t, e, lo, ii[0], ii[1], ii[2]);
@@ -386,8 +447,16 @@ __global__ void __launch_bounds__(BLOCK) k_bc(double* __restrict__ q, Geom g, in
     }
 }
 
+// Boundary conditions along DIR (bc_core.py:128-202): side 0 layers t=0..h
+// (t = 0 is the boundary point), side 1 likewise, over the halo-extended
+// range of the other directions.  bc0/bc1: OSB_BC_* of the two sides.
+template <int DIR>
+__global__ void __launch_bounds__(BLOCK) k_bc(double* __restrict__ q, Geom g, int nv, int bc0, int bc1) {
+    bc_item<DIR>(gtid(), q, g, nv, bc0, bc1);
+}
+
 // host interior layout (v, k, j, i) <-> padded device layout
-__global__ void __launch_bounds__(BLOCK) k_scatter(const double* __restrict__ src, double* __restrict__ q, Geom g, int nv) {
+static __global__ void __launch_bounds__(BLOCK) k_scatter(const double* __restrict__ src, double* __restrict__ q, Geom g, int nv) {
     const long npts = (long)g.n[0] * g.n[1] * g.n[2];
     const long t = gtid();
     if (t >= npts * nv) return;
@@ -399,7 +468,7 @@ __global__ void __launch_bounds__(BLOCK) k_scatter(const double* __restrict__ sr
     unflatten(r, e, lo, i, j, k);
     q[(long)v * g.Ptot + gidx(g, i, j, k)] = src[t];
 }
-__global__ void __launch_bounds__(BLOCK) k_gather(const double* __restrict__ q, double* __restrict__ dst, Geom g, int nv) {
+static __global__ void __launch_bounds__(BLOCK) k_gather(const double* __restrict__ q, double* __restrict__ dst, Geom g, int nv) {
     const long npts = (long)g.n[0] * g.n[1] * g.n[2];
     const long t = gtid();
     if (t >= npts * nv) return;
@@ -427,35 +496,31 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-__global__ void k_zero_scalar(double* scal) {
+static __global__ void k_zero_scalar(double* scal) {
     if (threadIdx.x == 0 && blockIdx.x == 0) { reinterpret_cast<unsigned long long*>(scal)[0] = 0ull; }
 }
 
 // max over interior points of sum_d (|u_d| + a)/Delta_d; stored as the bit
 // pattern of a non-negative double in scal[0] (order-preserving for >= 0)
 template <int NDIM>
-__global__ void __launch_bounds__(BLOCK) k_wavespeed(const double* __restrict__ q, double* scal, Geom g, Consts c) {
+__device__ __forceinline__ double wavespeed_point(const double* __restrict__ q, long x, const Geom& g, const Consts& c) {
     constexpr int NV = NDIM + 2;
+    const double r = q[x];
+    double uu[NDIM], ke = 0.0;
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) { uu[d] = q[(long)(1 + d) * g.Ptot + x] / r; ke -= 0.5 * (uu[d] * uu[d]) * r; }
+    const double p = c.gamma_m1 * (ke + q[(long)(NV - 1) * g.Ptot + x]);
+    const double a = sqrt(c.gama * p / r);
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) s += (fabs(uu[d]) + a) * c.invd[d];
+    return s;
+}
+
+// block max of m, then one atomicMax on the bit pattern (order-preserving for
+// non-negative doubles, so the result does not depend on the order)
+__device__ __forceinline__ void block_max_to(double m, double* scal) {
     __shared__ double red[BLOCK / 64];
-    const int e[3] = {g.n[0], g.n[1], g.n[2]};
-    const int lo[3] = {0, 0, 0};
-    const long tot = (long)e[0] * e[1] * e[2];
-    double m = 0.0;
-    for (long t = gtid(); t < tot; t += (long)gridDim.x * blockDim.x) {
-        int i, j, k;
-        unflatten(t, e, lo, i, j, k);
-        const long x = gidx(g, i, j, k);
-        const double r = q[x];
-        double uu[NDIM], ke = 0.0;
-#pragma unroll
-        for (int d = 0; d < NDIM; ++d) { uu[d] = q[(long)(1 + d) * g.Ptot + x] / r; ke -= 0.5 * (uu[d] * uu[d]) * r; }
-        const double p = c.gamma_m1 * (ke + q[(long)(NV - 1) * g.Ptot + x]);
-        const double a = sqrt(c.gama * p / r);
-        double s = 0.0;
-#pragma unroll
-        for (int d = 0; d < NDIM; ++d) s += (fabs(uu[d]) + a) * c.invd[d];
-        m = fmax(m, s);
-    }
     m = wave_max(m);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
@@ -464,10 +529,25 @@ __global__ void __launch_bounds__(BLOCK) k_wavespeed(const double* __restrict__ 
         for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = fmax(b, red[w]);
         atomicMax(reinterpret_cast<unsigned long long*>(scal), (unsigned long long)__double_as_longlong(b));
     }
+    __syncthreads();
+}
+
+template <int NDIM>
+__global__ void __launch_bounds__(BLOCK) k_wavespeed(const double* __restrict__ q, double* scal, Geom g, Consts c) {
+    const int e[3] = {g.n[0], g.n[1], g.n[2]};
+    const int lo[3] = {0, 0, 0};
+    const long tot = (long)e[0] * e[1] * e[2];
+    double m = 0.0;
+    for (long t = gtid(); t < tot; t += (long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        unflatten(t, e, lo, i, j, k);
+        m = fmax(m, wavespeed_point<NDIM>(q, gidx(g, i, j, k), g, c));
+    }
+    block_max_to(m, scal);
 }
 
 // dt = cfl / max wave speed; time += dt
-__global__ void k_dt_finalize(double* scal, double cfl) {
+static __global__ void k_dt_finalize(double* scal, double cfl) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         const double m = __longlong_as_double((long long)reinterpret_cast<unsigned long long*>(scal)[0]);
         const double dt = cfl / m;
@@ -477,7 +557,7 @@ __global__ void k_dt_finalize(double* scal, double cfl) {
 }
 
 // Taylor-Green reductions: per-block partial sums (deterministic two-pass)
-__global__ void __launch_bounds__(BLOCK) k_tgv_partials(const double* __restrict__ qrho, const double* __restrict__ cr,
+static __global__ void __launch_bounds__(BLOCK) k_tgv_partials(const double* __restrict__ qrho, const double* __restrict__ cr,
                                                         double* __restrict__ part, Geom g, Consts c) {
     __shared__ double red[4][BLOCK / 64];
     const int e[3] = {g.n[0], g.n[1], g.n[2]};
@@ -517,7 +597,7 @@ __global__ void __launch_bounds__(BLOCK) k_tgv_partials(const double* __restrict
     }
 }
 
-__global__ void k_sum_partials(const double* __restrict__ part, double* __restrict__ out, int nblocks) {
+static __global__ void k_sum_partials(const double* __restrict__ part, double* __restrict__ out, int nblocks) {
     if (blockIdx.x == 0 && threadIdx.x < 4) {
         double s = 0.0;
         for (int b = 0; b < nblocks; ++b) s += part[(long)b * 4 + threadIdx.x];
@@ -525,7 +605,7 @@ __global__ void k_sum_partials(const double* __restrict__ part, double* __restri
     }
 }
 
-__global__ void __launch_bounds__(BLOCK) k_nan_count(const double* __restrict__ q, unsigned int* cnt, Geom g) {
+static __global__ void __launch_bounds__(BLOCK) k_nan_count(const double* __restrict__ q, unsigned int* cnt, Geom g) {
     const int e[3] = {g.n[0], g.n[1], g.n[2]};
     const int lo[3] = {0, 0, 0};
     const long t = gtid();

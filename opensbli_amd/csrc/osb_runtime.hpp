@@ -1,0 +1,61 @@
+// osb_runtime.hpp -- the per-block context shared by the single-block C ABI
+// (osb_hip.hip) and the batch / pool runtime (osb_batch.hip).
+#pragma once
+
+#include "osb_kernels.hpp"
+#include "osb_central_fused.hpp"
+#include "../../include/osb.h"
+
+#include <string>
+#include <vector>
+
+// message of the last failed create call (osb_last_error(NULL))
+extern std::string g_osb_create_error;
+
+struct osb_ctx {
+    osb_problem_desc d{};
+    osb::Geom g{};
+    osb::Consts c{};
+    int nv = 0;
+    bool shock = false, central = false, viscous = false;
+    int sch = osb::SCH_WENOZ;
+    bool mixed = false;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int device = 0;
+
+    double* qa = nullptr;
+    double* qb = nullptr;
+    double* q_cur = nullptr;        // current state
+    double* q_prev = nullptr;       // input of the last stage of the last step
+    double* du = nullptr;
+    double* R = nullptr;
+    double* F = nullptr;
+    double* cr = nullptr;
+    double* gr = nullptr;
+    double* scal = nullptr;         // [0] max wavespeed bits, [1] dt, [2] device time, [3..6] reduction out
+    double* staging = nullptr;      // interior-layout transfer buffer
+    double* part = nullptr;         // reduction partials
+    unsigned int* nan_cnt = nullptr;
+    int part_blocks = 0;
+    size_t bytes = 0;
+
+    double time = 0.0;
+    int64_t iter = 0;
+    double last_dt = 0.0;
+    bool have_prev = false;
+    std::string err;
+    int rk_stages = 3;              // RK-LS stages and coefficients (rk_LS.py:97-132)
+    double rkA[5] = {0}, rkB[5] = {0};
+    int fused_tile = 0;             // index into CF_TILES (autotuned)
+    int xcd_remap = 0;              // XCD-aware workgroup-to-tile order (OSB_XCD_REMAP=1)
+
+    // kernel timing (osb_profile_*)
+    bool prof_on = false;
+    struct Rec { int fam; hipEvent_t a, b; };
+    std::vector<Rec> prof_pending;
+    std::vector<hipEvent_t> prof_pool;
+    double prof_ms[16] = {0};
+    int64_t prof_n[16] = {0};
+};
+

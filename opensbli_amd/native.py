@@ -235,6 +235,23 @@ class Library:
             L.osb_batch_sync.argtypes = [vp]
             L.osb_batch_last_error.argtypes = [vp]
             L.osb_batch_last_error.restype = C.c_char_p
+            L.osb_batch_get_stats.argtypes = [vp, C.c_int, C.POINTER(StatsC)]
+            L.osb_batch_autotune.argtypes = [vp]
+            L.osb_batch_describe.argtypes = [vp, C.c_char_p, C.c_int]
+        L.osb_estimate_bytes.argtypes = [C.POINTER(ProblemDescC), C.POINTER(C.c_int64)]
+        L.osb_estimate_bytes.restype = C.c_int
+        self.has_pool = hasattr(L, "osb_pool_create")
+        if self.has_pool:
+            L.osb_pool_create.argtypes = [C.c_int64, C.c_int, C.c_int, C.POINTER(vp)]
+            L.osb_pool_destroy.argtypes = [vp]
+            L.osb_pool_admit.argtypes = [vp, C.POINTER(ProblemDescC), dptr, C.c_int, C.POINTER(C.c_int)]
+            L.osb_pool_advance.argtypes = [vp, C.c_int]
+            L.osb_pool_poll.argtypes = [vp, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]
+            L.osb_pool_fetch.argtypes = [vp, C.c_int, dptr, C.POINTER(StatsC)]
+            L.osb_pool_release.argtypes = [vp, C.c_int]
+            L.osb_pool_usage.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+            L.osb_pool_last_error.argtypes = [vp]
+            L.osb_pool_last_error.restype = C.c_char_p
         if L.osb_abi_version() != ABI_VERSION:
             raise RuntimeError(f"{path}: ABI version {L.osb_abi_version()} != {ABI_VERSION}")
 
@@ -243,6 +260,18 @@ class Library:
 
     def batch(self, descs: Sequence[ProblemDesc]) -> "Batch":
         return Batch(self, descs)
+
+    def pool(self, hbm_budget_bytes: int, max_solves: int = 64, nstreams: int = 4) -> "Pool":
+        return Pool(self, hbm_budget_bytes, max_solves, nstreams)
+
+    def estimate_bytes(self, desc: ProblemDesc) -> int:
+        cd = desc.to_c()
+        out = C.c_int64()
+        rc = self.lib.osb_estimate_bytes(C.byref(cd), C.byref(out))
+        if rc != OK:
+            msg = self.lib.osb_last_error(None)
+            raise OSBError(rc, msg.decode() if msg else "")
+        return out.value
 
 
 _hip_lib: Optional[Library] = None
@@ -417,8 +446,102 @@ class Batch:
         self._check(self.lib.lib.osb_batch_get_state(self.h, int(i), out.reshape(-1), LAYOUT_INTERIOR))
         return out
 
+    def stats(self, i: int) -> StatsC:
+        st = StatsC()
+        st.struct_size = C.sizeof(StatsC)
+        self._check(self.lib.lib.osb_batch_get_stats(self.h, int(i), C.byref(st)))
+        return st
+
+    def autotune(self):
+        self._check(self.lib.lib.osb_batch_autotune(self.h))
+
+    def describe(self) -> dict:
+        import json
+        buf = C.create_string_buffer(1 << 16)
+        self._check(self.lib.lib.osb_batch_describe(self.h, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
     def step(self, nsteps: int = 1):
         self._check(self.lib.lib.osb_batch_step(self.h, int(nsteps)))
 
     def sync(self):
         self._check(self.lib.lib.osb_batch_sync(self.h))
+
+
+class Pool:
+    """An osb_pool: continuous batching of independent solves (HIP library only)."""
+
+    def __init__(self, lib: Library, hbm_budget_bytes: int, max_solves: int = 64, nstreams: int = 4):
+        if not lib.has_pool:
+            raise OSBError(E_UNSUPPORTED, f"{lib.path} has no pool entry points")
+        self.lib = lib
+        h = C.c_void_p()
+        rc = lib.lib.osb_pool_create(int(hbm_budget_bytes), int(max_solves), int(nstreams), C.byref(h))
+        if rc != OK:
+            msg = lib.lib.osb_last_error(None)
+            raise OSBError(rc, msg.decode() if msg else "")
+        self.h = h
+        self.max_solves = max_solves
+        self.descs = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lib.osb_pool_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != OK:
+            msg = self.lib.lib.osb_pool_last_error(self.h)
+            raise OSBError(rc, msg.decode() if msg else "")
+
+    def admit(self, desc: ProblemDesc, q0: np.ndarray, nsteps: int) -> Optional[int]:
+        """Slot of the admitted solve, or None when slots or memory are short."""
+        cd = desc.to_c()
+        q0 = np.ascontiguousarray(q0, dtype=np.float64).reshape(-1)
+        slot = C.c_int(-1)
+        rc = self.lib.lib.osb_pool_admit(self.h, C.byref(cd), q0, int(nsteps), C.byref(slot))
+        if rc == E_NOMEM:
+            return None
+        self._check(rc)
+        self.descs[slot.value] = desc
+        return slot.value
+
+    def advance(self, chunk: int):
+        self._check(self.lib.lib.osb_pool_advance(self.h, int(chunk)))
+
+    def poll(self):
+        buf = (C.c_int * self.max_solves)()
+        n = C.c_int()
+        self._check(self.lib.lib.osb_pool_poll(self.h, buf, self.max_solves, C.byref(n)))
+        return [buf[i] for i in range(min(n.value, self.max_solves))]
+
+    def fetch(self, slot: int):
+        desc = self.descs[slot]
+        out = np.empty(desc.shape, dtype=np.float64)
+        st = StatsC()
+        st.struct_size = C.sizeof(StatsC)
+        self._check(self.lib.lib.osb_pool_fetch(self.h, int(slot), out.reshape(-1), C.byref(st)))
+        return out, st
+
+    def release(self, slot: int):
+        self._check(self.lib.lib.osb_pool_release(self.h, int(slot)))
+        self.descs.pop(slot, None)
+
+    def usage(self):
+        used = C.c_int64()
+        running = C.c_int()
+        done = C.c_int()
+        self._check(self.lib.lib.osb_pool_usage(self.h, C.byref(used), C.byref(running), C.byref(done)))
+        return used.value, running.value, done.value

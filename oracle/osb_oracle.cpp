@@ -856,6 +856,24 @@ extern "C" {
 
 int osb_abi_version(void) { return OSB_ABI_VERSION; }
 
+// host bytes of the arrays a context allocates (the oracle's own layout)
+int osb_estimate_bytes(const osb_problem_desc* desc, int64_t* out) {
+    if (!out) return fail(nullptr, OSB_E_INVALID, "null output pointer");
+    std::string why;
+    int rc = validate(desc, why);
+    if (rc) return fail(nullptr, rc, why);
+    const int nd = desc->ndim, nv = nd + 2;
+    size_t P = 1;
+    for (int i = 0; i < nd; ++i) P *= (size_t)(desc->n[i] + 2 * HALO);
+    const bool shock = desc->conv_scheme == OSB_CONV_WENO_Z || desc->conv_scheme == OSB_CONV_TENO;
+    const bool cv = !shock || desc->viscous != OSB_VISC_NONE;
+    size_t doubles = 3 * (size_t)nv * P + (size_t)(nd + 2) * P;
+    if (cv) doubles += 2 * P + (size_t)nd * nd * P;
+    if (shock) doubles += (size_t)nd * nv * P;
+    *out = (int64_t)(doubles * sizeof(double));
+    return OSB_OK;
+}
+
 const char* osb_last_error(const osb_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
 
 int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {

@@ -46,8 +46,8 @@ def test_hip_library_exports_every_declared_function(hip_so):
 
 
 def test_oracle_exports_the_single_block_api(oracle):
-    """The oracle restates one block; timing and batching are HIP-library only."""
-    single = [f for f in declared_functions() if not f.startswith(("osb_batch_", "osb_profile_"))]
+    """The oracle restates one block; timing, batching and the pool are HIP-library only."""
+    single = [f for f in declared_functions() if not f.startswith(("osb_batch_", "osb_profile_", "osb_pool_"))]
     missing = [f for f in single if f not in exported(oracle.path)]
     assert not missing, missing
 
