@@ -115,6 +115,99 @@ def cpu_baseline(oracle_path, problem, budget_s=15.0):
             "sample": f"{problem.name}: {n} of the workload's steps after 1 warm-up step, oracle OpenMP build, {cores} threads"}
 
 
+def _rel_l2(a, b):
+    import numpy as np
+    per_var = [float(np.linalg.norm(a[v] - b[v]) / max(np.linalg.norm(b[v]), 1e-300)) for v in range(b.shape[0])]
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b)), per_var
+
+
+def ensemble_bench(steps, count=64, seed=20261203):
+    """KH ensemble: plain FP64 (each instance alone, one after another), then
+    the batched FP64 and batched mixed forms; mixed error against the FP64
+    results (the FP64 path is bitwise equal to the oracle)."""
+    import numpy as np
+    import torch
+    from opensbli_amd import native as nat, problems as P
+    hip = nat.load_hip()
+    probs = P.kh_ensemble(count=count, seed=seed, steps=steps, precision=nat.PREC_FP64)
+    work = sum(p.desc.npoints * 3.0 * steps for p in probs)
+    sizes = sorted({p.desc.n[0] for p in probs})
+    # plain FP64 path: one context per instance, stepped alone
+    t_plain = 0.0
+    for p in probs:
+        with hip.context(p.desc) as c:
+            c.set_state(p.q0)
+            c.step(1)
+            c.sync()
+            t0 = time.perf_counter()
+            c.step(steps)
+            c.sync()
+            t_plain += time.perf_counter() - t0
+    res = {}
+    finals = {}
+    for label, prec in (("batched_fp64", nat.PREC_FP64), ("batched_mixed", nat.PREC_MIXED)):
+        descs = []
+        for p in probs:
+            d = nat.ProblemDesc(**{**p.desc.__dict__, "precision": prec})
+            descs.append(d)
+        with hip.batch(descs) as b:
+            for i, p in enumerate(probs):
+                b.set_state(i, p.q0)
+            b.step(1)                      # autotune + graph capture + one step
+            b.sync()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            b.step(steps)
+            b.sync()
+            t1 = time.perf_counter()
+            res[label] = work / (t1 - t0) / 1e9
+            res[label + "_describe"] = b.describe()
+            finals[label] = [b.get_state(i) for i in range(len(probs))]
+    worst_state, worst_var = 0.0, 0.0
+    for a, bref in zip(finals["batched_mixed"], finals["batched_fp64"]):
+        st, pv = _rel_l2(a, bref)
+        worst_state = max(worst_state, st)
+        worst_var = max(worst_var, max(pv))
+    tuning = res["batched_mixed_describe"]["groups"][0]["tuning"]
+    return {
+        "workload": f"kh_ensemble_{count}", "instances": count, "sizes": sizes, "seed": seed,
+        "steps_timed": steps, "work_gpts_stage": work / 1e9,
+        "plain_fp64": work / t_plain / 1e9,
+        "batched_fp64": res["batched_fp64"],
+        "batched_mixed": res["batched_mixed"],
+        "speedup_batched_mixed_vs_plain_fp64": res["batched_mixed"] * t_plain / work * 1e9,
+        "mixed_rel_l2_vs_fp64_worst_instance": {"state": worst_state, "per_variable_max": worst_var},
+        "autotuned_face_flux_grid": {k: v["grid"] for k, v in tuning.items() if k.startswith("face_flux")},
+    }
+
+
+def serving_bench(count=64, rate=50.0, seed=20261203, budget_gib=8.0, max_solves=32, chunk=25):
+    from opensbli_amd import native as nat, serving as S
+    hip = nat.load_hip()
+    reqs = S.make_stream(count=count, rate=rate, seed=seed)
+    rep = S.serve(hip, reqs, int(budget_gib * 2**30), max_solves=max_solves, nstreams=4, chunk=chunk)
+    return {"workload": f"serving_{count}", "requests": count, "rate_per_s": rate, "seed": seed,
+            "hbm_budget_gib": budget_gib, "max_solves": max_solves, "chunk_steps": chunk,
+            "p50_time_to_solution_s": rep.p50, "p99_time_to_solution_s": rep.p99,
+            "aggregate_gpts_stage_per_s": rep.throughput, "makespan_s": rep.makespan,
+            "peak_running": rep.peak_running, "peak_device_gib": rep.peak_bytes / 2**30}
+
+
+def mixed_headline(problem_name, steps, warmup):
+    """The headline workload in mixed precision, with its error against FP64."""
+    from opensbli_amd import native as nat
+    pm = make_problem(problem_name, "mixed")
+    dt_m, _, qm = time_gpu(pm, steps, warmup)
+    pf = make_problem(problem_name, "fp64")
+    hip = nat.load_hip()
+    with hip.context(pf.desc) as c:
+        c.set_state(pf.q0)
+        c.step(warmup + steps)
+        qf = c.get_state()
+    st, pv = _rel_l2(qm, qf)
+    return {"value": pm.desc.npoints * 3.0 * steps / dt_m / 1e9, "rel_l2_vs_fp64": {"state": st, "per_variable": pv}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +217,10 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the mixed-precision, KH-ensemble and serving lines added to the default run")
+    ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--serving-requests", type=int, default=64)
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed run, copy a 256 MiB FP64 buffer 5 times (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -134,6 +231,16 @@ def main():
         sys.exit(2)
 
     oracle_path = ensure_built()
+    if args.workload in ("kh_ensemble", "serving"):
+        extra = ensemble_bench(args.ensemble_steps) if args.workload == "kh_ensemble" else \
+            serving_bench(args.serving_requests)
+        value = extra["batched_mixed"] if args.workload == "kh_ensemble" else extra["aggregate_gpts_stage_per_s"]
+        print(json.dumps({"metric": METRIC, "value": value, "unit": "GPts·stage/s", "n_gpus": 1,
+                          "steps": args.ensemble_steps, "warmup": 1, "higher_is_better": True, "scaling": "weak",
+                          "vs_baseline": None, "dtype": "f64-state/f32-flux" if args.workload == "kh_ensemble" else "f64",
+                          "data": "synthetic (seeded)", "config": {"workload": extra["workload"]}, "detail": extra}),
+              flush=True)
+        return
     problem = make_problem(args.workload, args.precision)
     dt_wall, prof, state = time_gpu(problem, args.steps, args.warmup)
     import numpy as np
@@ -175,6 +282,10 @@ def main():
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(oracle_path, problem, args.cpu_budget)
+    if not args.no_extras and args.workload == "tgv_central_128" and args.precision == "fp64":
+        out["mixed_precision"] = mixed_headline(args.workload, args.steps, args.warmup)
+        out["ensemble"] = ensemble_bench(args.ensemble_steps)
+        out["serving"] = serving_bench(args.serving_requests)
     if args.copy_probe:
         import torch
         a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
