@@ -11,6 +11,7 @@
 // context's stream; nothing synchronises except the calls that return data.
 
 #include "osb_runtime.hpp"
+#include "osb_central_march.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -223,6 +224,30 @@ void launch_fused_v(osb_ctx* c, const double* q, double* qn, double A, double B)
     }
 }
 
+template <int TX, int TY, int CONV, int VISC, bool FLUXRES, int TIME>
+void launch_march_tile(osb_ctx* c, const double* q, double* qn, double A, double B) {
+    const Geom& g = c->g;
+    const int kz = c->march_kz;
+    dim3 grid((g.n[0] + TX - 1) / TX, (g.n[1] + TY - 1) / TY, (g.n[2] + kz - 1) / kz);
+    k_central_march<TX, TY, CONV, VISC, FLUXRES, TIME><<<grid, TX * TY, 0, c->stream>>>(
+        q, c->F, c->du, qn, c->scal, g, c->c, A, B, kz);
+}
+
+template <int CONV, int VISC, bool FLUXRES, int TIME>
+void launch_march_t(osb_ctx* c, const double* q, double* qn, double A, double B) {
+    if (c->march_tile == 1) launch_march_tile<32, 8, CONV, VISC, FLUXRES, TIME>(c, q, qn, A, B);
+    else launch_march_tile<16, 16, CONV, VISC, FLUXRES, TIME>(c, q, qn, A, B);
+}
+
+template <int CONV, bool FLUXRES, int TIME>
+void launch_march_v(osb_ctx* c, const double* q, double* qn, double A, double B) {
+    switch (c->d.viscous) {
+        case OSB_VISC_NONE: if (!FLUXRES) launch_march_t<CONV, 0, FLUXRES, TIME>(c, q, qn, A, B); break;
+        case OSB_VISC_CONSTANT: launch_march_t<CONV, 1, FLUXRES, TIME>(c, q, qn, A, B); break;
+        default: launch_march_t<CONV, 2, FLUXRES, TIME>(c, q, qn, A, B); break;
+    }
+}
+
 // One RK stage through the fused kernel: 3-D central schemes, and 3-D
 // shock-capturing schemes with viscous terms (face fluxes first).
 template <int TIME>
@@ -233,6 +258,12 @@ void fused_stage(osb_ctx* c, const double* q, double* qn, double A, double B) {
         launch_flux<3, 2>(c, q);
     }
     Timed tm(c, PF_CENTRAL_FUSED);
+    if (c->march) {
+        if (c->shock) launch_march_v<CF_NONE, true, TIME>(c, q, qn, A, B);
+        else if (c->d.conv_scheme == OSB_CONV_CENTRAL_SKEW) launch_march_v<CF_SKEW, false, TIME>(c, q, qn, A, B);
+        else launch_march_v<CF_FEIEREISEN, false, TIME>(c, q, qn, A, B);
+        return;
+    }
     if (c->shock) launch_fused_v<CF_NONE, true, TIME>(c, q, qn, A, B);
     else if (c->d.conv_scheme == OSB_CONV_CENTRAL_SKEW) launch_fused_v<CF_SKEW, false, TIME>(c, q, qn, A, B);
     else launch_fused_v<CF_FEIEREISEN, false, TIME>(c, q, qn, A, B);
@@ -519,6 +550,9 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     c->q_prev = c->qb;
     if (const char* e = std::getenv("OSB_FUSED_TILE")) c->fused_tile = std::max(0, std::min(CF_NTILES - 1, std::atoi(e)));
     if (const char* e = std::getenv("OSB_XCD_REMAP")) c->xcd_remap = std::atoi(e) != 0;
+    if (const char* e = std::getenv("OSB_CENTRAL_KERNEL")) c->march = std::strcmp(e, "march") == 0;
+    if (const char* e = std::getenv("OSB_MARCH_KZ")) c->march_kz = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("OSB_MARCH_TILE")) c->march_tile = std::atoi(e);
     *out = c;
     return OSB_OK;
 }

@@ -48,7 +48,10 @@ struct osb_ctx {
     int rk_stages = 3;              // RK-LS stages and coefficients (rk_LS.py:97-132)
     double rkA[5] = {0}, rkB[5] = {0};
     int fused_tile = 0;             // index into CF_TILES (autotuned)
-    int xcd_remap = 0;              // XCD-aware workgroup-to-tile order (OSB_XCD_REMAP=1)
+    int xcd_remap = 1;              // XCD-aware workgroup-to-tile order (OSB_XCD_REMAP=0 turns it off)
+    bool march = false;             // 3-D central stage: 3-D tiles, or the z-march kernel (OSB_CENTRAL_KERNEL=march)
+    int march_kz = 16;              // planes per z-march workgroup
+    int march_tile = 0;             // 0: 16x16 columns, 1: 32x8
 
     // kernel timing (osb_profile_*)
     bool prof_on = false;
