@@ -75,6 +75,9 @@ def ensure_built():
 
 
 def time_gpu(problem, steps, warmup):
+    """Wall time of `steps` steps (captured time steps replayed as hipGraphs),
+    then the same number of steps again, launched eagerly with HIP events
+    around every kernel, for the per-kernel durations."""
     import torch
     from opensbli_amd import native as nat
     hip = nat.load_hip()
@@ -82,17 +85,18 @@ def time_gpu(problem, steps, warmup):
         c.set_state(problem.q0)
         c.step(warmup)
         c.sync()
-        c.profile_enable(True)
         torch.cuda.synchronize()
-        c.sync()
         t0 = time.perf_counter()
         c.step(steps)
         c.sync()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
+        state = c.get_state()
+        c.profile_enable(True)
+        c.step(steps)
+        c.sync()
         prof = c.profile_read()
         c.profile_enable(False)
-        state = c.get_state()
     return t1 - t0, prof, state
 
 

@@ -59,9 +59,29 @@ __device__ __forceinline__ double js_last(double a, double b, double c) { return
 template <typename T>
 __device__ __forceinline__ T eno3(T c0, T c1, T c2, T x0, T x1, T x2) { return fma(c2, x2, fma(c1, x1, c0 * x0)); }
 
+// Reciprocals of the non-linear weights.  FP64 path: IEEE division, as the
+// oracle.  Mixed path: the hardware reciprocal refined by two Newton steps
+// (an FP64-accurate reciprocal without the scaled division sequence).
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return fma(fma(-x, r, 1.0), r, r);
+}
+template <typename T>
+__device__ __forceinline__ double wrecip(double x) {
+    if constexpr (std::is_same<T, double>::value) return 1.0 / x;
+    else return rcp_nr(x);
+}
+template <typename T>
+__device__ __forceinline__ double wdiv(double a, double b) {
+    if constexpr (std::is_same<T, double>::value) return a / b;
+    else return a * rcp_nr(b);
+}
+
+template <typename T>
 __device__ __forceinline__ double zalpha(double d, double t2, double b, double eps) {
     const double eb = eps + b;
-    return d + (d * t2) / (eb * eb);
+    return d + wdiv<T>(d * t2, eb * eb);
 }
 
 // WENO-Z, order 5 (weno.py:290-345): tau5 = |b0 - b2|,
@@ -72,10 +92,10 @@ __device__ __forceinline__ T wenoz_right(const T* f, double eps) {
     const double b1 = js_centre(f[1], f[2], f[3]);    // stencil (i-1, i, i+1), d = 6/10
     const double b2 = js_last(f[0], f[1], f[2]);      // stencil (i-2, i-1, i), d = 1/10
     const double tau = fabs(b0 - b2), t2 = tau * tau;
-    const double a0 = zalpha(3.0 / 10.0, t2, b0, eps);
-    const double a1 = zalpha(6.0 / 10.0, t2, b1, eps);
-    const double a2 = zalpha(1.0 / 10.0, t2, b2, eps);
-    const double inv = 1.0 / (a0 + a1 + a2);
+    const double a0 = zalpha<T>(3.0 / 10.0, t2, b0, eps);
+    const double a1 = zalpha<T>(6.0 / 10.0, t2, b1, eps);
+    const double a2 = zalpha<T>(1.0 / 10.0, t2, b2, eps);
+    const double inv = wrecip<T>(a0 + a1 + a2);
     const T e0 = eno3<T>(T(1.0 / 3.0), T(5.0 / 6.0), T(-1.0 / 6.0), f[2], f[3], f[4]);
     const T e1 = eno3<T>(T(-1.0 / 6.0), T(5.0 / 6.0), T(1.0 / 3.0), f[1], f[2], f[3]);
     const T e2 = eno3<T>(T(1.0 / 3.0), T(-7.0 / 6.0), T(11.0 / 6.0), f[0], f[1], f[2]);
@@ -87,10 +107,10 @@ __device__ __forceinline__ T wenoz_left(const T* f, double eps) {
     const double b1 = js_centre(f[2], f[3], f[4]);    // (i, i+1, i+2),   d = 6/10
     const double b2 = js_last(f[1], f[2], f[3]);      // (i-1, i, i+1),   d = 3/10
     const double tau = fabs(b0 - b2), t2 = tau * tau;
-    const double a0 = zalpha(1.0 / 10.0, t2, b0, eps);
-    const double a1 = zalpha(6.0 / 10.0, t2, b1, eps);
-    const double a2 = zalpha(3.0 / 10.0, t2, b2, eps);
-    const double inv = 1.0 / (a0 + a1 + a2);
+    const double a0 = zalpha<T>(1.0 / 10.0, t2, b0, eps);
+    const double a1 = zalpha<T>(6.0 / 10.0, t2, b1, eps);
+    const double a2 = zalpha<T>(3.0 / 10.0, t2, b2, eps);
+    const double inv = wrecip<T>(a0 + a1 + a2);
     const T e0 = eno3<T>(T(11.0 / 6.0), T(-7.0 / 6.0), T(1.0 / 3.0), f[3], f[4], f[5]);
     const T e1 = eno3<T>(T(1.0 / 3.0), T(5.0 / 6.0), T(-1.0 / 6.0), f[2], f[3], f[4]);
     const T e2 = eno3<T>(T(-1.0 / 6.0), T(5.0 / 6.0), T(1.0 / 3.0), f[1], f[2], f[3]);
@@ -147,18 +167,18 @@ __device__ __forceinline__ T teno(const T* f, double eps, double ct) {
     double alpha[4], asum = 0.0;
 #pragma unroll
     for (int r = 0; r < NS; ++r) {
-        const double inv_beta = 1.0 / (eps + beta[r]);
+        const double inv_beta = wrecip<T>(eps + beta[r]);
         alpha[r] = pow6(1.0 + tau * inv_beta);
         asum += alpha[r];
     }
-    const double inv_asum = 1.0 / asum;
+    const double inv_asum = wrecip<T>(asum);
     double delta[4], wsum = 0.0;
 #pragma unroll
     for (int r = 0; r < NS; ++r) {
         delta[r] = (alpha[r] * inv_asum < ct) ? 0.0 : 1.0;
         wsum += dopt[r] * delta[r];
     }
-    const double inv_wsum = 1.0 / wsum;
+    const double inv_wsum = wrecip<T>(wsum);
     T rec = T(0);
 #pragma unroll
     for (int r = 0; r < NS; ++r) rec = fma(T(dopt[r] * delta[r] * inv_wsum), eno[r], rec);
@@ -177,6 +197,13 @@ __device__ __forceinline__ T recon_pair(const T* fp, const T* fm, const Consts& 
 // out with the unit normal e_DIR folded in at compile time.
 // L[k][m]: left eigenvectors (rows), R[m][k]: right eigenvectors (columns).
 // ---------------------------------------------------------------------------
+// 1/x for the eigenvector entries: exact in FP64, the hardware reciprocal in FP32
+template <typename T>
+__device__ __forceinline__ T frecip(T x) {
+    if constexpr (std::is_same<T, double>::value) return T(1) / x;
+    else return __builtin_amdgcn_rcpf(x);
+}
+
 template <int NDIM, int DIR, typename T>
 __device__ __forceinline__ void eigensystem(T gm1, T rho, const T* u, T a, T (&L)[NDIM + 2][NDIM + 2],
                                             T (&R)[NDIM + 2][NDIM + 2]) {
@@ -187,10 +214,10 @@ __device__ __forceinline__ void eigensystem(T gm1, T rho, const T* u, T a, T (&L
         for (int j = 0; j < NV; ++j) { L[i][j] = T(0); R[i][j] = T(0); }
     if constexpr (NDIM == 1) {
         const T u0 = u[0];
-        const T inva = T(1) / a;
-        const T H = (a * a) * (T(1) / gm1) + (u0 * u0) * T(0.5);
+        const T inva = frecip<T>(a);
+        const T H = (a * a) * frecip<T>(gm1) + (u0 * u0) * T(0.5);
         const T D = T(2) * H - u0 * u0;
-        const T invD = T(1) / D;
+        const T invD = frecip<T>(D);
         R[0][0] = 1; R[0][1] = 1; R[0][2] = 1;
         R[1][0] = u0 - a; R[1][1] = u0; R[1][2] = u0 + a;
         R[2][0] = H - u0 * a; R[2][1] = (u0 * u0) * T(0.5); R[2][2] = H + u0 * a;
@@ -210,11 +237,11 @@ __device__ __forceinline__ void eigensystem(T gm1, T rho, const T* u, T a, T (&L
     for (int d = 0; d < NDIM; ++d) q2 += u[d] * u[d];
     const T phi_sq = gm1 * (q2 * T(0.5));
     const T theta = u[DIR];
-    const T inva = T(1) / a, invrho = T(1) / rho;
+    const T inva = frecip<T>(a), invrho = frecip<T>(rho);
     const T inva2 = inva * inva;
     const T alpha = rho * T(INV_SQRT2) * inva;
     const T bta = T(INV_SQRT2) * invrho * inva;
-    const T igm1 = T(1) / gm1;
+    const T igm1 = frecip<T>(gm1);
     const T a2 = a * a;
     if constexpr (NDIM == 2) {
         const T u0 = u[0], u1 = u[1];
@@ -303,8 +330,12 @@ __device__ __forceinline__ void face_flux(const double* __restrict__ q, long Pto
         const double r = v[0];
         double uu[NDIM];
         double ke = 0.0;
+        const double inv_r = DEV ? rcp_nr(r) : 0.0;      // mixed precision: reciprocal, not division
 #pragma unroll
-        for (int d = 0; d < NDIM; ++d) { uu[d] = v[1 + d] / r; ke -= 0.5 * (uu[d] * uu[d]) * r; }
+        for (int d = 0; d < NDIM; ++d) {
+            uu[d] = DEV ? v[1 + d] * inv_r : v[1 + d] / r;
+            ke -= 0.5 * (uu[d] * uu[d]) * r;
+        }
         const double p = c.gamma_m1 * (ke + v[NV - 1]);
         const double u_dir = uu[DIR];
         double f[NV];
@@ -328,7 +359,7 @@ __device__ __forceinline__ void face_flux(const double* __restrict__ q, long Pto
             rho23[w] = r; p23[w] = p; E23[w] = v[NV - 1];
 #pragma unroll
             for (int d = 0; d < NDIM; ++d) u23[w][d] = uu[d];
-            a23[w] = sqrt(c.gama * p / r);
+            a23[w] = DEV ? sqrt(c.gama * p * inv_r) : sqrt(c.gama * p / r);
         }
     };
     point(2);              // the base point first (mixed precision subtracts it)
@@ -341,12 +372,12 @@ __device__ __forceinline__ void face_flux(const double* __restrict__ q, long Pto
     T rho_av, u_av[NDIM], a_av;
     if (c.avg_roe) {
         const double sl = sqrt(rho23[0]), sr = sqrt(rho23[1]);
-        const double inv = 1.0 / (sl + sr);
+        const double inv = wrecip<T>(sl + sr);
         double ke = 0.0;
         double ua[NDIM];
 #pragma unroll
         for (int d = 0; d < NDIM; ++d) { ua[d] = (sl * u23[0][d] + sr * u23[1][d]) * inv; ke += ua[d] * ua[d]; }
-        const double Hr = ((p23[0] + E23[0]) / sl + (p23[1] + E23[1]) / sr) * inv;
+        const double Hr = (wdiv<T>(p23[0] + E23[0], sl) + wdiv<T>(p23[1] + E23[1], sr)) * inv;
         rho_av = T(sqrt(rho23[0] * rho23[1]));
 #pragma unroll
         for (int d = 0; d < NDIM; ++d) u_av[d] = T(ua[d]);

@@ -437,8 +437,14 @@ int ensure_rhs_arrays(osb_ctx* c) {
     return r;
 }
 
+void drop_graphs(osb_ctx* c) {
+    for (auto& e : c->gexec)
+        if (e) { (void)hipGraphExecDestroy(e); e = nullptr; }
+}
+
 void free_ctx(osb_ctx* c) {
     if (!c) return;
+    drop_graphs(c);
     double* bufs[] = {c->qa, c->qb, c->du, c->R, c->F, c->cr, c->gr, c->scal, c->staging, c->part};
     for (double* b : bufs) if (b) (void)hipFree(b);
     if (c->nan_cnt) (void)hipFree(c->nan_cnt);
@@ -551,6 +557,7 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     if (const char* e = std::getenv("OSB_FUSED_TILE")) c->fused_tile = std::max(0, std::min(CF_NTILES - 1, std::atoi(e)));
     if (const char* e = std::getenv("OSB_XCD_REMAP")) c->xcd_remap = std::atoi(e) != 0;
     if (const char* e = std::getenv("OSB_CENTRAL_KERNEL")) c->march = std::strcmp(e, "march") == 0;
+    if (const char* e = std::getenv("OSB_GRAPH")) c->use_graph = std::atoi(e) != 0;
     if (const char* e = std::getenv("OSB_MARCH_KZ")) c->march_kz = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("OSB_MARCH_TILE")) c->march_tile = std::atoi(e);
     *out = c;
@@ -569,6 +576,7 @@ int osb_set_stream(osb_ctx* c, void* s) {
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (c->own_stream) { (void)hipStreamDestroy(c->stream); c->own_stream = false; }
     c->stream = (hipStream_t)s;
+    drop_graphs(c);                        // recapture on the new stream
     return OSB_OK;
 }
 
@@ -639,10 +647,37 @@ int osb_max_wavespeed(osb_ctx* c, double* out) {
     return check_launch(c);
 }
 
+// One time step replayed from a captured hipGraph (one graph per starting
+// buffer).  The graph holds exactly the launches of do_step, so results are
+// those of the eager path; per-launch timing needs the eager path.
+int graph_step(osb_ctx* c) {
+    const int p = c->q_cur == c->qa ? 0 : 1;
+    if (!c->gexec[p]) {
+        hipGraph_t graph = nullptr;
+        HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int r = do_step(c);
+        hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (r) { if (graph) (void)hipGraphDestroy(graph); return r; }
+        if (e != hipSuccess) return set_err(c, OSB_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&c->gexec[p], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return set_err(c, OSB_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        c->g_cur[p] = c->q_cur;            // do_step advanced the host view
+        c->g_prev[p] = c->q_prev;
+    } else {
+        c->q_cur = c->g_cur[p];
+        c->q_prev = c->g_prev[p];
+        c->have_prev = true;
+    }
+    HIP_TRY(c, hipGraphLaunch(c->gexec[p], c->stream));
+    return OSB_OK;
+}
+
 int osb_step(osb_ctx* c, int nsteps) {
     if (!c || nsteps < 0) return OSB_E_INVALID;
+    const bool graphs = c->use_graph && !c->prof_on;
     for (int i = 0; i < nsteps; ++i) {
-        int r = do_step(c);
+        int r = graphs ? graph_step(c) : do_step(c);
         if (r) return r;
         c->iter += 1;
         if (!(c->d.cfl > 0)) { c->time += c->d.dt; c->last_dt = c->d.dt; }

@@ -53,6 +53,12 @@ struct osb_ctx {
     int march_kz = 16;              // planes per z-march workgroup
     int march_tile = 0;             // 0: 16x16 columns, 1: 32x8
 
+    // captured time steps: graph p starts from buffer qa (p = 0) or qb (p = 1)
+    bool use_graph = true;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    double* g_cur[2] = {nullptr, nullptr};     // q_cur / q_prev after the captured step
+    double* g_prev[2] = {nullptr, nullptr};
+
     // kernel timing (osb_profile_*)
     bool prof_on = false;
     struct Rec { int fam; hipEvent_t a, b; };

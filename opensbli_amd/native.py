@@ -278,10 +278,11 @@ _hip_lib: Optional[Library] = None
 
 
 def load_hip() -> Library:
-    """Load the HIP product library.  Raises if it has not been built."""
+    """Load the HIP product library (OSB_HIP_LIBRARY overrides the in-tree
+    path, e.g. for a differently tuned build).  Raises if it has not been built."""
     global _hip_lib
     if _hip_lib is None:
-        _hip_lib = Library(HIP_LIBRARY)
+        _hip_lib = Library(os.environ.get("OSB_HIP_LIBRARY") or HIP_LIBRARY)
     return _hip_lib
 
 

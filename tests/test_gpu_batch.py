@@ -124,3 +124,26 @@ def test_pool_results_same_alone_and_in_the_mix(hip):
         want, st = run(hip, r.problem, steps=r.steps)
         assert r.iters == r.steps == st.iter
         assert np.array_equal(r.result, want), r.problem.name
+
+
+@pytest.mark.parametrize("maker", [lambda: P.sod(n=256, steps=7), lambda: P.kelvin_helmholtz(n=48, steps=7),
+                                   lambda: P.tgv(n=16, steps=7),
+                                   lambda: P.tgv(n=16, steps=7, scheme="teno", minf=1.25, cfl=0.5, dt=0.0)])
+def test_graph_replay_equals_eager(hip, maker, monkeypatch):
+    """Captured time steps replayed several times, with eager work and
+    synchronisation between the replays, against the eager launches."""
+    pr = maker()
+    _announce(f"graph replay {pr.name}")
+    out = {}
+    for graphs in ("1", "0"):
+        monkeypatch.setenv("OSB_GRAPH", graphs)
+        with hip.context(pr.desc) as c:
+            c.set_state(pr.q0)
+            snaps = []
+            for k in (1, 2, 1, 3):
+                c.step(k)
+                snaps.append(c.get_state())      # eager gather + sync between replays
+            out[graphs] = (snaps, c.stats().time)
+    for a, b in zip(out["1"][0], out["0"][0]):
+        assert np.array_equal(a, b)
+    assert out["1"][1] == out["0"][1]
