@@ -302,47 +302,54 @@ __device__ __forceinline__ void eigensystem(T gm1, T rho, const T* u, T a, T (&L
 }
 
 // ---------------------------------------------------------------------------
-// Characteristic LLF flux at interface x0 + 1/2 along DIR
-// (shock_capturing.py:786-891, 957-1028).  q: FP64 state, `st` the stride
-// along DIR.  Result in `out` (FP64).
+// Primitive variables and the physical flux along DIR of one point
+// (constituent relations + flux vector, shock_capturing.py:786-836).  FAST
+// (mixed precision) uses a refined reciprocal of rho instead of divisions.
 // ---------------------------------------------------------------------------
-template <int NDIM, int DIR, int SCH, typename T>
-__device__ __forceinline__ void face_flux(const double* __restrict__ q, long Ptot, long st, long x0,
-                                          const Consts& c, double (&out)[NDIM + 2]) {
+template <int NDIM, int DIR, bool FAST>
+__device__ __forceinline__ void point_prims(const double (&v)[NDIM + 2], const Consts& c, double (&uu)[NDIM], double& p,
+                                            double (&f)[NDIM + 2], double& a) {
     constexpr int NV = NDIM + 2;
-    // T = double: the reference's arithmetic.  T = float (mixed precision): the
-    // characteristic work runs on deviations from the state and flux at point
-    // i (s = 2), formed in FP64 and then rounded to FP32, so FP32 holds only
-    // the local variation.  WENO-Z and TENO are translation invariant
-    // (rec(f + c) = rec(f) + c: smoothness indicators see differences, the
-    // candidate weights sum to one), and f+ + f- = L F, so the interface flux
-    // is F_i + R rec(deviations) with the FP64 F_i added back at the end.
+    const double r = v[0];
+    double ke = 0.0;
+    const double inv_r = FAST ? rcp_nr(r) : 0.0;
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) {
+        uu[d] = FAST ? v[1 + d] * inv_r : v[1 + d] / r;
+        ke -= 0.5 * (uu[d] * uu[d]) * r;
+    }
+    p = c.gamma_m1 * (ke + v[NV - 1]);
+    const double u_dir = uu[DIR];
+    f[0] = v[1 + DIR];
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) f[1 + d] = v[1 + d] * u_dir + (d == DIR ? p : 0.0);
+    f[NV - 1] = (p + v[NV - 1]) * u_dir;
+    a = FAST ? sqrt(c.gama * p * inv_r) : sqrt(c.gama * p / r);
+}
+
+// ---------------------------------------------------------------------------
+// Characteristic LLF flux at interface i+1/2 along DIR from the six stencil
+// points i-2..i+3 (shock_capturing.py:786-891, 957-1028).
+//   pt(s, v, f, uu, p, a): FP64 state, flux and primitives of stencil point s.
+// T = double: the reference's arithmetic.  T = float (mixed precision): the
+// characteristic work runs on deviations from the state and flux at point
+// i (s = 2), formed in FP64 and then rounded to FP32, so FP32 holds only
+// the local variation.  WENO-Z and TENO are translation invariant
+// (rec(f + c) = rec(f) + c: smoothness indicators see differences, the
+// candidate weights sum to one), and f+ + f- = L F, so the interface flux
+// is F_i + R rec(deviations) with the FP64 F_i added back at the end.
+// ---------------------------------------------------------------------------
+template <int NDIM, int DIR, int SCH, typename T, class PT>
+__device__ __forceinline__ void face_core(PT&& pt, const Consts& c, double (&out)[NDIM + 2]) {
+    constexpr int NV = NDIM + 2;
     constexpr bool DEV = !std::is_same<T, double>::value;
     T qs[6][NV];
     T F[6][NV];
     double base_q[NV], base_F[NV];
     double rho23[2], u23[2][NDIM], p23[2], E23[2], a23[2];
     auto point = [&](const int s) {
-        const long xs = x0 + (long)(s - 2) * st;
-        double v[NV];
-#pragma unroll
-        for (int m = 0; m < NV; ++m) v[m] = q[(long)m * Ptot + xs];
-        const double r = v[0];
-        double uu[NDIM];
-        double ke = 0.0;
-        const double inv_r = DEV ? rcp_nr(r) : 0.0;      // mixed precision: reciprocal, not division
-#pragma unroll
-        for (int d = 0; d < NDIM; ++d) {
-            uu[d] = DEV ? v[1 + d] * inv_r : v[1 + d] / r;
-            ke -= 0.5 * (uu[d] * uu[d]) * r;
-        }
-        const double p = c.gamma_m1 * (ke + v[NV - 1]);
-        const double u_dir = uu[DIR];
-        double f[NV];
-        f[0] = v[1 + DIR];
-#pragma unroll
-        for (int d = 0; d < NDIM; ++d) f[1 + d] = v[1 + d] * u_dir + (d == DIR ? p : 0.0);
-        f[NV - 1] = (p + v[NV - 1]) * u_dir;
+        double v[NV], f[NV], uu[NDIM], p, a;
+        pt(s, v, f, uu, p, a);
         if constexpr (DEV) {
             if (s == 2) {
 #pragma unroll
@@ -356,10 +363,10 @@ __device__ __forceinline__ void face_flux(const double* __restrict__ q, long Pto
         }
         if (s == 2 || s == 3) {
             const int w = s - 2;
-            rho23[w] = r; p23[w] = p; E23[w] = v[NV - 1];
+            rho23[w] = v[0]; p23[w] = p; E23[w] = v[NV - 1];
 #pragma unroll
             for (int d = 0; d < NDIM; ++d) u23[w][d] = uu[d];
-            a23[w] = DEV ? sqrt(c.gama * p * inv_r) : sqrt(c.gama * p / r);
+            a23[w] = a;
         }
     };
     point(2);              // the base point first (mixed precision subtracts it)
@@ -431,6 +438,23 @@ __device__ __forceinline__ void face_flux(const double* __restrict__ q, long Pto
 #pragma unroll
         for (int m = 0; m < NV; ++m) out[m] = double(acc[m]);
     }
+}
+
+// face flux at interface x0 + 1/2 along DIR reading the six points from the
+// state (q: FP64 state, `st` the stride along DIR)
+template <int NDIM, int DIR, int SCH, typename T>
+__device__ __forceinline__ void face_flux(const double* __restrict__ q, long Ptot, long st, long x0,
+                                          const Consts& c, double (&out)[NDIM + 2]) {
+    constexpr int NV = NDIM + 2;
+    constexpr bool FAST = !std::is_same<T, double>::value;
+    face_core<NDIM, DIR, SCH, T>(
+        [&](int s, double (&v)[NV], double (&f)[NV], double (&uu)[NDIM], double& p, double& a) {
+            const long xs = x0 + (long)(s - 2) * st;
+#pragma unroll
+            for (int m = 0; m < NV; ++m) v[m] = q[(long)m * Ptot + xs];
+            point_prims<NDIM, DIR, FAST>(v, c, uu, p, f, a);
+        },
+        c, out);
 }
 
 }  // namespace osb

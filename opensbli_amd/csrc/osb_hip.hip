@@ -94,9 +94,18 @@ long box_points(const Geom& g, int ext) {
 // ---------------------------------------------------------------------------
 template <int NDIM, int DIR, typename T>
 void launch_flux_t(osb_ctx* c, const double* q) {
-    long work = interior_points(c->g) / c->g.n[DIR] * (long)(c->g.n[DIR] + 1);
     double* F = c->F + (long)DIR * c->nv * c->g.Ptot;
     Timed tm(c, PF_FACE_FLUX);
+    if (c->face_tiled) {
+        const unsigned nt = (unsigned)face_tiles<NDIM, DIR>(c->g);
+        switch (c->sch) {
+            case SCH_WENOZ: k_face_flux_tile<NDIM, DIR, SCH_WENOZ, T><<<nt, BLOCK, 0, c->stream>>>(q, F, c->g, c->c); break;
+            case SCH_TENO5: k_face_flux_tile<NDIM, DIR, SCH_TENO5, T><<<nt, BLOCK, 0, c->stream>>>(q, F, c->g, c->c); break;
+            default: k_face_flux_tile<NDIM, DIR, SCH_TENO6, T><<<nt, BLOCK, 0, c->stream>>>(q, F, c->g, c->c); break;
+        }
+        return;
+    }
+    long work = interior_points(c->g) / c->g.n[DIR] * (long)(c->g.n[DIR] + 1);
     switch (c->sch) {
         case SCH_WENOZ: k_face_flux<NDIM, DIR, SCH_WENOZ, T><<<nblocks(work), BLOCK, 0, c->stream>>>(q, F, c->g, c->c); break;
         case SCH_TENO5: k_face_flux<NDIM, DIR, SCH_TENO5, T><<<nblocks(work), BLOCK, 0, c->stream>>>(q, F, c->g, c->c); break;
@@ -558,6 +567,7 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     if (const char* e = std::getenv("OSB_XCD_REMAP")) c->xcd_remap = std::atoi(e) != 0;
     if (const char* e = std::getenv("OSB_CENTRAL_KERNEL")) c->march = std::strcmp(e, "march") == 0;
     if (const char* e = std::getenv("OSB_GRAPH")) c->use_graph = std::atoi(e) != 0;
+    if (const char* e = std::getenv("OSB_FACE_KERNEL")) c->face_tiled = std::strcmp(e, "tile") == 0;
     if (const char* e = std::getenv("OSB_MARCH_KZ")) c->march_kz = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("OSB_MARCH_TILE")) c->march_tile = std::atoi(e);
     *out = c;

@@ -53,6 +53,8 @@ struct osb_ctx {
     int march_kz = 16;              // planes per z-march workgroup
     int march_tile = 0;             // 0: 16x16 columns, 1: 32x8
 
+    bool face_tiled = false;        // face fluxes one thread per face from L2/HBM; OSB_FACE_KERNEL=tile: LDS-tiled variant
+
     // captured time steps: graph p starts from buffer qa (p = 0) or qb (p = 1)
     bool use_graph = true;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
