@@ -84,6 +84,32 @@ __device__ __forceinline__ double zalpha(double d, double t2, double b, double e
     return d + wdiv<T>(d * t2, eb * eb);
 }
 
+// Non-linear WENO-Z weights omega_r (times 1/sum) from the smoothness
+// indicators.  FP64 path: alpha_r = d_r + d_r tau^2/((eps+b_r)(eps+b_r)) and
+// omega_r = alpha_r / sum(alpha), as printed by the reference.  Mixed path:
+// the same weights with the three alpha denominators e_r = (eps+b_r)^2
+// cleared -- omega_r = d_r (e_r + tau^2) e_s e_t / sum -- so one reciprocal
+// replaces four divisions (e_r >= eps^2 = 1e-80 keeps the products in range).
+template <typename T>
+__device__ __forceinline__ void wenoz_weights(double d0, double d1, double d2, double b0, double b1, double b2, double eps,
+                                              double& w0, double& w1, double& w2) {
+    const double tau = fabs(b0 - b2), t2 = tau * tau;
+    if constexpr (std::is_same<T, double>::value) {
+        const double a0 = zalpha<T>(d0, t2, b0, eps);
+        const double a1 = zalpha<T>(d1, t2, b1, eps);
+        const double a2 = zalpha<T>(d2, t2, b2, eps);
+        const double inv = 1.0 / (a0 + a1 + a2);
+        w0 = a0 * inv; w1 = a1 * inv; w2 = a2 * inv;
+    } else {
+        const double e0 = (eps + b0) * (eps + b0), e1 = (eps + b1) * (eps + b1), e2 = (eps + b2) * (eps + b2);
+        const double n0 = d0 * (e0 + t2) * (e1 * e2);
+        const double n1 = d1 * (e1 + t2) * (e0 * e2);
+        const double n2 = d2 * (e2 + t2) * (e0 * e1);
+        const double inv = rcp_nr(n0 + n1 + n2);
+        w0 = n0 * inv; w1 = n1 * inv; w2 = n2 * inv;
+    }
+}
+
 // WENO-Z, order 5 (weno.py:290-345): tau5 = |b0 - b2|,
 // alpha_r = d_r + d_r tau^2/((eps+b_r)(eps+b_r)), omega_r = alpha_r / sum(alpha)
 template <typename T>
@@ -91,30 +117,24 @@ __device__ __forceinline__ T wenoz_right(const T* f, double eps) {
     const double b0 = js_first(f[2], f[3], f[4]);     // stencil (i, i+1, i+2), d = 3/10
     const double b1 = js_centre(f[1], f[2], f[3]);    // stencil (i-1, i, i+1), d = 6/10
     const double b2 = js_last(f[0], f[1], f[2]);      // stencil (i-2, i-1, i), d = 1/10
-    const double tau = fabs(b0 - b2), t2 = tau * tau;
-    const double a0 = zalpha<T>(3.0 / 10.0, t2, b0, eps);
-    const double a1 = zalpha<T>(6.0 / 10.0, t2, b1, eps);
-    const double a2 = zalpha<T>(1.0 / 10.0, t2, b2, eps);
-    const double inv = wrecip<T>(a0 + a1 + a2);
+    double w0, w1, w2;
+    wenoz_weights<T>(3.0 / 10.0, 6.0 / 10.0, 1.0 / 10.0, b0, b1, b2, eps, w0, w1, w2);
     const T e0 = eno3<T>(T(1.0 / 3.0), T(5.0 / 6.0), T(-1.0 / 6.0), f[2], f[3], f[4]);
     const T e1 = eno3<T>(T(-1.0 / 6.0), T(5.0 / 6.0), T(1.0 / 3.0), f[1], f[2], f[3]);
     const T e2 = eno3<T>(T(1.0 / 3.0), T(-7.0 / 6.0), T(11.0 / 6.0), f[0], f[1], f[2]);
-    return fma(T(a2 * inv), e2, fma(T(a1 * inv), e1, T(a0 * inv) * e0));
+    return fma(T(w2), e2, fma(T(w1), e1, T(w0) * e0));
 }
 template <typename T>
 __device__ __forceinline__ T wenoz_left(const T* f, double eps) {
     const double b0 = js_first(f[3], f[4], f[5]);     // (i+1, i+2, i+3), d = 1/10
     const double b1 = js_centre(f[2], f[3], f[4]);    // (i, i+1, i+2),   d = 6/10
     const double b2 = js_last(f[1], f[2], f[3]);      // (i-1, i, i+1),   d = 3/10
-    const double tau = fabs(b0 - b2), t2 = tau * tau;
-    const double a0 = zalpha<T>(1.0 / 10.0, t2, b0, eps);
-    const double a1 = zalpha<T>(6.0 / 10.0, t2, b1, eps);
-    const double a2 = zalpha<T>(3.0 / 10.0, t2, b2, eps);
-    const double inv = wrecip<T>(a0 + a1 + a2);
+    double w0, w1, w2;
+    wenoz_weights<T>(1.0 / 10.0, 6.0 / 10.0, 3.0 / 10.0, b0, b1, b2, eps, w0, w1, w2);
     const T e0 = eno3<T>(T(11.0 / 6.0), T(-7.0 / 6.0), T(1.0 / 3.0), f[3], f[4], f[5]);
     const T e1 = eno3<T>(T(1.0 / 3.0), T(5.0 / 6.0), T(-1.0 / 6.0), f[2], f[3], f[4]);
     const T e2 = eno3<T>(T(-1.0 / 6.0), T(5.0 / 6.0), T(1.0 / 3.0), f[1], f[2], f[3]);
-    return fma(T(a2 * inv), e2, fma(T(a1 * inv), e1, T(a0 * inv) * e0));
+    return fma(T(w2), e2, fma(T(w1), e1, T(w0) * e0));
 }
 
 // (C + tau/beta)**6 printed as six repeated multiplies (opsc.py:222-241)
