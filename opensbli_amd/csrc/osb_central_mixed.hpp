@@ -1,0 +1,271 @@
+// osb_central_mixed.hpp -- mixed-precision RK stage of the 3-D central path.
+//
+// Same tiling and phases as k_central_fused, with the derivative work in
+// FP32: every field staged in LDS is stored as the FP32 deviation from a
+// reference value (the FP64 state and primitives of the tile's first point),
+// so FP32 holds only the variation across the tile.  Derivatives of a field
+// are derivatives of its deviation (the stencil weights sum to zero); a
+// derivative of a product a*b expands exactly as
+//   D(ab) = a_ref D(b') + b_ref D(a') + D(a'b').
+// The constituent relations are evaluated in FP64, the right-hand side is
+// accumulated in FP32, and the RK update (du, q) is FP64.
+#pragma once
+
+#include "osb_kernels.hpp"
+#include "osb_central_fused.hpp"
+
+namespace osb {
+
+template <class A>
+__device__ __forceinline__ float d1f(A f, float invd) {
+    return fmaf(1.0f / 12.0f, f(-2) - f(2), (2.0f / 3.0f) * (f(1) - f(-1))) * invd;
+}
+template <class A>
+__device__ __forceinline__ float d2f(A f, float inv2d) {
+    return fmaf(-1.0f / 12.0f, f(-2) + f(2), fmaf(4.0f / 3.0f, f(-1) + f(1), (-5.0f / 2.0f) * f(0))) * inv2d;
+}
+
+// CONV: CF_FEIEREISEN or CF_NONE (with FLUXRES: the shock-capturing flux
+// difference, kept in FP64).  VISC: 0, 1 constant, 2 Sutherland.
+template <int CF_TX, int CF_TY, int CF_TZ, int CONV, int VISC, bool FLUXRES, int TIME>
+__global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const double* __restrict__ q, const double* __restrict__ F,
+                                                              double* __restrict__ reg, double* __restrict__ qout,
+                                                              const double* __restrict__ scal, Geom g, Consts c,
+                                                              double A, double B, int xcd_remap) {
+    static_assert(CONV == CF_FEIEREISEN || CONV == CF_NONE, "mixed central kernel: Feiereisen or viscous-only");
+    constexpr int NV = 5;
+    constexpr int CF_BX = CF_TX + 4, CF_BY = CF_TY + 4, CF_BZ = CF_TZ + 4;
+    constexpr int CF_BOX = CF_BX * CF_BY * CF_BZ;
+    constexpr int CF_THREADS = CF_TX * CF_TY * CF_TZ;
+    constexpr bool HAS_CONV = CONV != CF_NONE;
+    constexpr bool SUTH = VISC == 2;
+    __shared__ float s_u[3][CF_BOX];
+    __shared__ float s_T[VISC ? CF_BOX : 1];
+    __shared__ float s_mu[SUTH ? CF_BOX : 1];
+    __shared__ float s_p[HAS_CONV ? CF_BOX : 1];
+    __shared__ float s_E[HAS_CONV ? CF_BOX : 1];
+    __shared__ float s_r[HAS_CONV ? 4 : 1][HAS_CONV ? CF_BOX : 1];    // rho u0..2, rhoE
+
+    const int ntx = gridDim.x, nty = gridDim.y;
+    const int ntiles = ntx * nty * gridDim.z;
+    const int w = blockIdx.x + ntx * (blockIdx.y + nty * blockIdx.z);
+    int tile = w;
+    if (xcd_remap && (ntiles & 7) == 0) tile = (w & 7) * (ntiles >> 3) + (w >> 3);
+    const int tbx = tile % ntx, tby = (tile / ntx) % nty, tbz = tile / (ntx * nty);
+    const int i0 = tbx * CF_TX, j0 = tby * CF_TY, k0 = tbz * CF_TZ;
+    const long P = g.Ptot;
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+
+    // reference state: the tile's first point (in the interior, identical in every thread)
+    double ref_u[3], ref_p, ref_T, ref_E, ref_mu, ref_r[4];
+    {
+        const long x = gidx(g, i0, j0, k0);
+        const double r = q[x];
+        double ke = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            ref_r[d] = q[(long)(1 + d) * P + x];
+            ref_u[d] = ref_r[d] / r;
+            ke -= 0.5 * (ref_u[d] * ref_u[d]) * r;
+        }
+        ref_r[3] = q[4 * P + x];
+        ref_p = c.gamma_m1 * (ke + ref_r[3]);
+        ref_T = (c.minf * c.minf) * c.gama * ref_p / r;
+        ref_E = ref_r[3] / r;
+        ref_mu = SUTH ? 1.4042 * ref_T * sqrt(ref_T) / (0.40417 + ref_T) : 1.0;
+    }
+
+    // ---- phase 1: constituent relations (FP64) -> FP32 deviations in LDS ----
+    for (int b = threadIdx.x; b < CF_BOX; b += CF_THREADS) {
+        const int bx = b % CF_BX, by = (b / CF_BX) % CF_BY, bz = b / (CF_BX * CF_BY);
+        const int i = i0 + bx - 2, j = j0 + by - 2, k = k0 + bz - 2;
+        const int out = (bx < 2 || bx >= CF_TX + 2) + (by < 2 || by >= CF_TY + 2) + (bz < 2 || bz >= CF_TZ + 2);
+        if (out == 3) continue;
+        if (i >= n0 + 2 || j >= n1 + 2 || k >= n2 + 2) continue;
+        const long x = gidx(g, i, j, k);
+        const double r = q[x];
+        const double inv_r = rcp_nr(r);
+        double ke = 0.0;
+        double rv[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            rv[d] = q[(long)(1 + d) * P + x];
+            const double u = rv[d] * inv_r;
+            s_u[d][b] = float(u - ref_u[d]);
+            ke -= 0.5 * (u * u) * r;
+        }
+        if (out <= 1) {
+            const double rhoE = q[4 * P + x];
+            const double p = c.gamma_m1 * (ke + rhoE);
+            if (VISC) {
+                const double TT = (c.minf * c.minf) * c.gama * p * inv_r;
+                s_T[VISC ? b : 0] = float(TT - ref_T);
+                if (SUTH) s_mu[SUTH ? b : 0] = float(1.4042 * TT * sqrt(TT) / (0.40417 + TT) - ref_mu);
+            }
+            if (HAS_CONV) {
+                s_p[HAS_CONV ? b : 0] = float(p - ref_p);
+                s_E[HAS_CONV ? b : 0] = float(rhoE * inv_r - ref_E);
+#pragma unroll
+                for (int d = 0; d < 3; ++d) s_r[HAS_CONV ? d : 0][HAS_CONV ? b : 0] = float(rv[d] - ref_r[d]);
+                s_r[HAS_CONV ? 3 : 0][HAS_CONV ? b : 0] = float(rhoE - ref_r[3]);
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 2: FP32 right-hand side, FP64 RK update --------------------
+    const int tx = threadIdx.x % CF_TX, ty = (threadIdx.x / CF_TX) % CF_TY, tz = threadIdx.x / (CF_TX * CF_TY);
+    const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
+    if (i >= n0 || j >= n1 || k >= n2) return;
+    const long x = gidx(g, i, j, k);
+    const int l = (tx + 2) + CF_BX * ((ty + 2) + CF_BY * (tz + 2));
+    const int ls[3] = {1, CF_BX, CF_BX * CF_BY};
+    const float invd[3] = {float(c.invd[0]), float(c.invd[1]), float(c.invd[2])};
+    const float inv2d[3] = {float(c.inv2d[0]), float(c.inv2d[1]), float(c.inv2d[2])};
+
+    double Rd[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};       // FP64 part (face-flux difference)
+    if (FLUXRES) {
+        const long nvP = (long)NV * P;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double acc = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double* wv = F + d * nvP + (long)v * P;
+                acc -= (wv[x] - wv[x - g.s[d]]) * c.invd[d];
+            }
+            Rd[v] = acc;
+        }
+    }
+    float Rv[NV] = {0.f, 0.f, 0.f, 0.f, 0.f};
+
+    // point values (full) and gradients of u
+    float uc[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) uc[m] = float(ref_u[m]) + s_u[m][l];
+    float G[3][3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int jd = 0; jd < 3; ++jd) G[m][jd] = d1f([&](int s) { return s_u[m][l + s * ls[jd]]; }, invd[jd]);
+
+    if (HAS_CONV) {
+        const float* su = nullptr;
+        (void)su;
+        float div_rhou = 0.f, eterm = 0.f;
+        float conv[3] = {0.f, 0.f, 0.f};
+        float rc[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) rc[v] = float(ref_r[v]) + s_r[HAS_CONV ? v : 0][HAS_CONV ? l : 0];
+#pragma unroll
+        for (int jd = 0; jd < 3; ++jd) {
+            const int lj = ls[jd];
+            const float inv = invd[jd];
+            const float urj = float(ref_u[jd]);
+            auto ud = [&](int s) { return s_u[jd][l + s * lj]; };                    // u_j deviation
+            auto rd = [&](int v, int s) { return s_r[HAS_CONV ? v : 0][HAS_CONV ? l + s * lj : 0]; };
+            const float d_rhouj = d1f([&](int s) { return rd(jd, s); }, inv);
+            div_rhou += d_rhouj;
+            const float d_uj = G[jd][jd];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                // D(rho u_m u_j) = ref(rho u_m) D(u_j') + ref(u_j) D(rho u_m') + D(rho u_m' u_j')
+                const float d_rm = d1f([&](int s) { return rd(m, s); }, inv);
+                const float d_cross = d1f([&](int s) { return rd(m, s) * ud(s); }, inv);
+                const float d_prod = float(ref_r[m]) * d_uj + urj * d_rm + d_cross;
+                conv[m] += d_prod + rc[jd] * G[m][jd];
+            }
+            const float d_rE = d1f([&](int s) { return rd(3, s); }, inv);
+            const float d_Eu = float(ref_r[3]) * d_uj + urj * d_rE + d1f([&](int s) { return rd(3, s) * ud(s); }, inv);
+            const float d_E = d1f([&](int s) { return s_E[HAS_CONV ? l + s * lj : 0]; }, inv);
+            const float d_pp = d1f([&](int s) { return s_p[HAS_CONV ? l + s * lj : 0]; }, inv);
+            const float d_pu = float(ref_p) * d_uj + urj * d_pp + d1f([&](int s) { return s_p[HAS_CONV ? l + s * lj : 0] * ud(s); }, inv);
+            eterm += -0.5f * (d_Eu + rc[jd] * d_E) - d_pu;
+        }
+        const float Ec = float(ref_E) + s_E[HAS_CONV ? l : 0];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            const int lm = ls[m];
+            const float dp = d1f([&](int s) { return s_p[HAS_CONV ? l + s * lm : 0]; }, invd[m]);
+            Rv[1 + m] = -dp - 0.5f * (conv[m] + uc[m] * div_rhou);
+        }
+        Rv[0] = -div_rhou;
+        Rv[4] = eterm - 0.5f * Ec * div_rhou;
+    }
+
+    if (VISC != 0) {
+        float D2u[3][3], dmu[3], dT[3];
+        float lapT = 0.f;
+#pragma unroll
+        for (int jd = 0; jd < 3; ++jd) {
+            const int lj = ls[jd];
+            dmu[jd] = SUTH ? d1f([&](int s) { return s_mu[SUTH ? l + s * lj : 0]; }, invd[jd]) : 0.f;
+            dT[jd] = d1f([&](int s) { return s_T[VISC ? l + s * lj : 0]; }, invd[jd]);
+            lapT += d2f([&](int s) { return s_T[VISC ? l + s * lj : 0]; }, inv2d[jd]);
+#pragma unroll
+            for (int m = 0; m < 3; ++m) D2u[m][jd] = d2f([&](int s) { return s_u[m][l + s * lj]; }, inv2d[jd]);
+        }
+        float div = 0.f;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) div += G[m][m];
+        const float muv = SUTH ? float(ref_mu) + s_mu[SUTH ? l : 0] : 1.f;
+        float Eacc = 0.f;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            float Lm = (4.0f / 3.0f) * D2u[m][m];
+#pragma unroll
+            for (int jd = 0; jd < 3; ++jd) {
+                if (jd == m) continue;
+                const int a = jd < m ? jd : m, b = jd < m ? m : jd;
+                const int la = ls[a], lb = ls[b];
+                auto ga = [&](int s) {
+                    const int lc = l + s * lb;
+                    return d1f([&](int r) { return s_u[jd][lc + r * la]; }, invd[a]);
+                };
+                Lm += D2u[m][jd] + (1.0f / 3.0f) * d1f(ga, invd[b]);
+            }
+            float S[3];
+#pragma unroll
+            for (int jd = 0; jd < 3; ++jd) S[jd] = G[m][jd] + G[jd][m] - (m == jd ? (2.0f / 3.0f) * div : 0.f);
+            float mom = muv * Lm;
+            if (SUTH) {
+#pragma unroll
+                for (int jd = 0; jd < 3; ++jd) mom += S[jd] * dmu[jd];
+            }
+            Rv[1 + m] += float(c.invRe) * mom;
+            float e = uc[m] * mom;
+#pragma unroll
+            for (int jd = 0; jd < 3; ++jd) e += muv * S[jd] * G[m][jd];
+            Eacc += e;
+        }
+        float heat = muv * lapT;
+        if (SUTH) {
+#pragma unroll
+            for (int jd = 0; jd < 3; ++jd) heat += dT[jd] * dmu[jd];
+        }
+        Rv[4] += float(c.invRe) * Eacc + float(c.cheat) * heat;
+    }
+
+    const double dt = scal[1];
+    if (TIME == 1) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const long y = (long)v * P + x;
+            const double R = Rd[v] + double(Rv[v]);
+            const double d = fma(A, reg[y], dt * R);
+            reg[y] = d;
+            qout[y] = fma(B, d, q[y]);
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const long y = (long)v * P + x;
+            const double R = Rd[v] + double(Rv[v]);
+            const double o = reg[y];
+            qout[y] = fma(dt * A, R, o);
+            reg[y] = fma(dt * B, R, o);
+        }
+    }
+}
+
+}  // namespace osb

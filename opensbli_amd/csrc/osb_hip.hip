@@ -12,6 +12,7 @@
 
 #include "osb_runtime.hpp"
 #include "osb_central_march.hpp"
+#include "osb_central_mixed.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -257,6 +258,23 @@ void launch_march_v(osb_ctx* c, const double* q, double* qn, double A, double B)
     }
 }
 
+template <int CONV, int VISC, bool FLUXRES, int TIME>
+void launch_mixed_t(osb_ctx* c, const double* q, double* qn, double A, double B) {
+    const Geom& g = c->g;
+    dim3 grid((g.n[0] + 15) / 16, (g.n[1] + 3) / 4, (g.n[2] + 3) / 4);
+    k_central_mixed<16, 4, 4, CONV, VISC, FLUXRES, TIME><<<grid, 256, 0, c->stream>>>(
+        q, c->F, c->du, qn, c->scal, g, c->c, A, B, c->xcd_remap);
+}
+
+template <int CONV, bool FLUXRES, int TIME>
+void launch_mixed_v(osb_ctx* c, const double* q, double* qn, double A, double B) {
+    switch (c->d.viscous) {
+        case OSB_VISC_NONE: if (!FLUXRES) launch_mixed_t<CONV, 0, FLUXRES, TIME>(c, q, qn, A, B); break;
+        case OSB_VISC_CONSTANT: launch_mixed_t<CONV, 1, FLUXRES, TIME>(c, q, qn, A, B); break;
+        default: launch_mixed_t<CONV, 2, FLUXRES, TIME>(c, q, qn, A, B); break;
+    }
+}
+
 // One RK stage through the fused kernel: 3-D central schemes, and 3-D
 // shock-capturing schemes with viscous terms (face fluxes first).
 template <int TIME>
@@ -267,6 +285,12 @@ void fused_stage(osb_ctx* c, const double* q, double* qn, double A, double B) {
         launch_flux<3, 2>(c, q);
     }
     Timed tm(c, PF_CENTRAL_FUSED);
+    if (c->mixed && c->d.conv_scheme != OSB_CONV_CENTRAL_SKEW) {
+        // FP32 derivatives on deviations (the skew form stays FP64)
+        if (c->shock) launch_mixed_v<CF_NONE, true, TIME>(c, q, qn, A, B);
+        else launch_mixed_v<CF_FEIEREISEN, false, TIME>(c, q, qn, A, B);
+        return;
+    }
     if (c->march) {
         if (c->shock) launch_march_v<CF_NONE, true, TIME>(c, q, qn, A, B);
         else if (c->d.conv_scheme == OSB_CONV_CENTRAL_SKEW) launch_march_v<CF_SKEW, false, TIME>(c, q, qn, A, B);
