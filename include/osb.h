@@ -165,6 +165,10 @@ int  osb_profile_read(osb_ctx* ctx, int idx, char* name, int name_len, double* t
 /* Device buffers (HIP library only): pointer to the FP64 conservative
  * variable `var` (padded layout) for zero-copy interop.                     */
 int  osb_state_device_ptr(osb_ctx* ctx, int var, void** out);
+/* device-to-device state transfer in the interior layout (e.g. a torch
+ * tensor's data_ptr(); the caller makes sure the source is ready) */
+int  osb_set_state_device(osb_ctx* ctx, const double* dev, int layout);
+int  osb_get_state_device(osb_ctx* ctx, double* dev, int layout);
 int  osb_set_stream(osb_ctx* ctx, void* hip_stream);
 
 /* Batched ensembles: many instances of possibly different sizes advanced by

@@ -796,6 +796,33 @@ int osb_profile_read(osb_ctx* c, int idx, char* name, int name_len, double* tota
     return OSB_OK;
 }
 
+// Device-to-device state transfer (interior layout, nv x n2 x n1 x n0 FP64 in
+// device memory, e.g. a torch tensor's data_ptr()).  The caller makes sure the
+// source is ready; the call returns when the copy is done.
+int osb_set_state_device(osb_ctx* c, const double* dev, int layout) {
+    if (!c || !dev) return OSB_E_INVALID;
+    if (layout != OSB_LAYOUT_INTERIOR) return set_err(c, OSB_E_UNSUPPORTED, "only the interior layout is accepted");
+    const long npts = interior_points(c->g);
+    HIP_TRY(c, hipMemsetAsync(c->qa, 0, sizeof(double) * c->nv * c->g.Ptot, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->du, 0, sizeof(double) * c->nv * c->g.Ptot, c->stream));
+    k_scatter<<<nblocks(npts * c->nv), BLOCK, 0, c->stream>>>(dev, c->qa, c->g, c->nv);
+    c->q_cur = c->qa;
+    c->q_prev = c->qb;
+    c->have_prev = false;
+    apply_bcs(c, c->q_cur);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return check_launch(c);
+}
+
+int osb_get_state_device(osb_ctx* c, double* dev, int layout) {
+    if (!c || !dev) return OSB_E_INVALID;
+    if (layout != OSB_LAYOUT_INTERIOR) return set_err(c, OSB_E_UNSUPPORTED, "only the interior layout is returned");
+    const long npts = interior_points(c->g);
+    k_gather<<<nblocks(npts * c->nv), BLOCK, 0, c->stream>>>(c->q_cur, dev, c->g, c->nv);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return check_launch(c);
+}
+
 int osb_state_device_ptr(osb_ctx* c, int var, void** out) {
     if (!c || !out || var < 0 || var >= c->nv) return OSB_E_INVALID;
     *out = (void*)(c->q_cur + (long)var * c->g.Ptot);

@@ -221,6 +221,9 @@ class Library:
         L.osb_nan_check.argtypes = [vp]
         L.osb_state_device_ptr.argtypes = [vp, C.c_int, C.POINTER(vp)]
         L.osb_set_stream.argtypes = [vp, vp]
+        if hasattr(L, "osb_set_state_device"):
+            L.osb_set_state_device.argtypes = [vp, vp, C.c_int]
+            L.osb_get_state_device.argtypes = [vp, vp, C.c_int]
         self.has_profile = hasattr(L, "osb_profile_enable")
         if self.has_profile:
             L.osb_profile_enable.argtypes = [vp, C.c_int]
@@ -370,6 +373,13 @@ class Context:
 
     def nan_check(self) -> int:
         return self.lib.lib.osb_nan_check(self.h)
+
+    def set_state_device(self, ptr: int):
+        """State from device memory (interior layout, FP64), e.g. tensor.data_ptr()."""
+        self._check(self.lib.lib.osb_set_state_device(self.h, C.c_void_p(ptr), LAYOUT_INTERIOR))
+
+    def get_state_device(self, ptr: int):
+        self._check(self.lib.lib.osb_get_state_device(self.h, C.c_void_p(ptr), LAYOUT_INTERIOR))
 
     def device_ptr(self, var: int) -> int:
         p = C.c_void_p()

@@ -149,3 +149,20 @@ def test_graph_replay_equals_eager(hip, maker, monkeypatch):
     for a, b in zip(out["1"][0], out["0"][0]):
         assert np.array_equal(a, b)
     assert out["1"][1] == out["0"][1]
+
+
+def test_device_state_roundtrip_with_torch(hip):
+    """osb_set_state_device / osb_get_state_device with torch tensors as glue."""
+    import torch
+    pr = P.kelvin_helmholtz(n=48, steps=4)
+    _announce("device state transfer")
+    src = torch.from_numpy(pr.q0.copy()).cuda()
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    with hip.context(pr.desc) as c:
+        c.set_state_device(src.data_ptr())
+        c.step(pr.steps)
+        c.get_state_device(dst.data_ptr())
+        host = c.get_state()
+    want, _ = run(hip, pr)
+    assert np.array_equal(dst.cpu().numpy(), want) and np.array_equal(host, want)
