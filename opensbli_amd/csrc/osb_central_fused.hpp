@@ -19,8 +19,8 @@ namespace osb {
 
 // tile shapes offered to the autotuner: {TX, TY, TZ}
 struct CfTile { int tx, ty, tz; };
-constexpr CfTile CF_TILES[] = {{16, 4, 4}, {8, 8, 4}, {32, 4, 2}};
-constexpr int CF_NTILES = 3;
+constexpr CfTile CF_TILES[] = {{16, 4, 4}, {8, 8, 4}, {32, 4, 2}, {16, 8, 4}, {32, 4, 4}};
+constexpr int CF_NTILES = 5;
 
 enum CfConv { CF_NONE = 0, CF_FEIEREISEN = 1, CF_SKEW = 2 };
 

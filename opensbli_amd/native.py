@@ -285,6 +285,14 @@ def load_hip() -> Library:
     path, e.g. for a differently tuned build).  Raises if it has not been built."""
     global _hip_lib
     if _hip_lib is None:
+        # One HIP runtime per process: PyTorch-ROCm ships its own libamdhip64
+        # with the same soname, and torch cannot initialise on a runtime that
+        # was loaded before it.  Load torch's first (when installed); the
+        # library then binds to it.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _hip_lib = Library(os.environ.get("OSB_HIP_LIBRARY") or HIP_LIBRARY)
     return _hip_lib
 
