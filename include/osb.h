@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define OSB_ABI_VERSION 2
+#define OSB_ABI_VERSION 3
 
 /* error codes */
 #define OSB_OK            0
@@ -55,6 +55,10 @@ extern "C" {
 #define OSB_VISC_CONSTANT   1         /* tau = (1.0/Re)(...)                         */
 #define OSB_VISC_SUTHERLAND 2         /* tau = (mu/Re)(...), mu = 1.4042 T^1.5/(T+0.40417) */
 
+/* flux splitting of the shock-capturing schemes (LFWeno/LFTeno flux_type) */
+#define OSB_FLUX_LLF 0                /* local: lambda = max over {i, i+1}, shock_capturing.py:1000-1028 */
+#define OSB_FLUX_GLF 1                /* global: lambda = domain max per stage, shock_capturing.py:416-438, :838-857 */
+
 /* time integration */
 #define OSB_TIME_RKLS3 1              /* RungeKuttaLS  rk_LS.py:17 (2N-storage; see rk_stages) */
 #define OSB_TIME_RK3   2              /* RungeKutta(3)    rk_sbli.py:16 */
@@ -64,6 +68,12 @@ extern "C" {
 #define OSB_BC_ZERO_GRADIENT 2        /* ZeroGradientOutletBC  zero_gradient_outlet.py:5  */
 #define OSB_BC_INVISCID_WALL 3        /* InviscidWallBC        inviscid_wall.py:6         */
 #define OSB_BC_SYMMETRY      4        /* SymmetryBC            symmetry.py:6              */
+#define OSB_BC_EXTRAPOLATION0 5       /* ExtrapolationBC(order=0) extrapolation.py:36-40:
+                                         boundary point and halos take q[1] (side 0)        */
+#define OSB_BC_EXTRAPOLATION1 6       /* ExtrapolationBC(order=1) extrapolation.py:41-54:
+                                         q[-t] = 2 q[-t+1] - q[-t+2], t = 1..h, in order    */
+#define OSB_BC_DIRICHLET     7        /* DirichletBC dirichlet.py:28-41 with constant
+                                         values: boundary point and halos = bc_value     */
 
 /* arithmetic of the flux / derivative evaluation */
 #define OSB_PREC_FP64  1
@@ -99,9 +109,11 @@ typedef struct osb_problem_desc {
                                   1..5: stages with rk_a/rk_b below (RungeKuttaLS
                                   formulation='SSP' 3/4/5 stages, or order 4,
                                   rk_LS.py:97-132)                                 */
-    int32_t  reserved0;
+    int32_t  flux_type;        /* OSB_FLUX_* (shock capturing only)                */
     double   rk_a[5];          /* rkA: du = rk_a[s]*du + dt*R                       */
     double   rk_b[5];          /* rkB: q  = q + rk_b[s]*du                          */
+    double   bc_value[3][2][5];/* OSB_BC_DIRICHLET: conservative values per face,
+                                  (rho, rhou0.., rhoE); unused entries ignored     */
     int32_t  reserved[8];
 } osb_problem_desc;
 

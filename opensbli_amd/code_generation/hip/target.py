@@ -21,6 +21,7 @@ from __future__ import annotations
 import ast
 import math
 import operator
+import re
 from typing import Callable, Dict, Mapping, Optional, Union
 
 import numpy as np
@@ -89,7 +90,65 @@ def _kinds(obj) -> set:
 
 
 _BC_IDS = {"PeriodicBC": nat.BC_PERIODIC, "ZeroGradientOutletBC": nat.BC_ZERO_GRADIENT,
-           "InviscidWallBC": nat.BC_INVISCID_WALL, "SymmetryBC": nat.BC_SYMMETRY}
+           "InviscidWallBC": nat.BC_INVISCID_WALL, "SymmetryBC": nat.BC_SYMMETRY,
+           "ExtrapolationBC": None, "DirichletBC": nat.BC_DIRICHLET}
+
+
+def _conservative_names(ndim: int):
+    return ["rho"] + [f"rhou{d}" for d in range(ndim)] + ["rhoE"]
+
+
+def _equation_parts(eq):
+    """(assigned name, right-hand side text) of one boundary equation: a SymPy
+    / OpenSBLIEq equation (``lhs``/``rhs``) or the text ``Eq(DataObject(rho), d)``."""
+    if hasattr(eq, "lhs") and hasattr(eq, "rhs"):
+        # block datasets print as name_B0[x0, ...]; the variable is the name
+        plain = lambda e: re.sub(r"\b([A-Za-z_]\w*?)_B\d+\[[^\]]*\]", r"\1", str(e))
+        return plain(eq.lhs), plain(eq.rhs)
+    node = ast.parse(str(eq).strip(), mode="eval").body
+    if not (isinstance(node, ast.Call) and isinstance(node.func, ast.Name) and node.func.id == "Eq"
+            and len(node.args) == 2):
+        raise NotImplementedError(f"Dirichlet equation {eq!r}: expected Eq(lhs, rhs)")
+    lhs = node.args[0]
+    if (isinstance(lhs, ast.Call) and isinstance(lhs.func, ast.Name)
+            and lhs.func.id in ("DataObject", "GridVariable") and len(lhs.args) == 1
+            and isinstance(lhs.args[0], ast.Name)):
+        name = lhs.args[0].id
+    elif isinstance(lhs, ast.Name):
+        name = lhs.id
+    else:
+        raise NotImplementedError(f"Dirichlet equation {eq!r}: unsupported left-hand side")
+    return name, ast.unparse(node.args[1])
+
+
+def dirichlet_values(bc, ndim: int, parameters: Mapping[str, float]):
+    """Conservative values that DirichletBC ``bc`` (dirichlet.py:28-41) holds on
+    its face, evaluated in order on the host with the simulation parameters in
+    scope.  Every conservative variable must be assigned a constant."""
+    names = _conservative_names(ndim)
+    eqs = bc.equations
+    if isinstance(eqs, Mapping):
+        vals = {str(k): float(v) for k, v in eqs.items()}
+    else:
+        env = dict(parameters)
+        vals = {}
+        for eq in eqs:
+            name, rhs = _equation_parts(eq)
+            text = rhs.replace("DataObject(", "(").replace("GridVariable(", "(")
+            try:
+                env[name] = _eval_node(ast.parse(text, mode="eval"), env)
+            except (KeyError, ValueError) as e:
+                raise NotImplementedError(
+                    f"DirichletBC(direction={bc.direction}, side={bc.side}): {name} = {rhs} is not a constant "
+                    f"({e}); only constant boundary values are implemented") from None
+            if name in names:
+                vals[name] = float(env[name])
+    missing = [m for m in names if m not in vals]
+    if missing:
+        raise NotImplementedError(
+            f"DirichletBC(direction={bc.direction}, side={bc.side}) leaves {missing} unset; "
+            "the HIP kernels hold every conservative variable")
+    return [vals[m] for m in names]
 
 
 def _averaging(scheme) -> int:
@@ -212,15 +271,28 @@ def plan_problem(alg_or_block, parameters: Mapping, physics=None, precision: str
         desc["time_scheme"] = nat.TIME_RK3
 
     bc = [[0, 0] for _ in range(ndim)]
+    bc_value = {}
     for pair in block.boundary_types:
         for b in pair:
             name = next((k for k in _kinds(b) if k in _BC_IDS), None)
             if name is None:
                 raise NotImplementedError(f"boundary condition {type(b).__name__} is not implemented by the HIP kernels")
-            bc[b.direction][b.side] = _BC_IDS[name]
+            if name == "ExtrapolationBC":
+                if b.order not in (0, 1):
+                    raise NotImplementedError(f"ExtrapolationBC(order={b.order}): orders 0 and 1")
+                bc[b.direction][b.side] = nat.BC_EXTRAPOLATION0 if b.order == 0 else nat.BC_EXTRAPOLATION1
+            else:
+                bc[b.direction][b.side] = _BC_IDS[name]
+            if name == "DirichletBC":
+                bc_value[(b.direction, b.side)] = dirichlet_values(b, ndim, p)
+            if name in ("ExtrapolationBC", "DirichletBC") and (central is not None or shock is None):
+                raise NotImplementedError(
+                    f"{name} with central derivatives needs the Carpenter boundary closure "
+                    "(Carpenter_scheme.py), which the HIP kernels do not implement")
     if any(v == 0 for pair in bc for v in pair):
         raise ValueError("every direction needs a boundary condition on both sides")
     desc["bc"] = bc
+    desc["bc_value"] = bc_value
 
     for key, name in (("gama", "gama"), ("minf", "Minf"), ("re", "Re"), ("pr", "Pr")):
         if name in p:

@@ -168,7 +168,7 @@ int upload_items(osb_batch* b) {
             x.scal = c->scal;
             x.g = c->g;
             x.c = c->c;
-            for (int d = 0; d < 3; ++d) { x.bc[d][0] = c->d.bc[d][0]; x.bc[d][1] = c->d.bc[d][1]; }
+            for (int d = 0; d < 3; ++d) x.bc[d] = bc_sides(c->d, d);
             x.cfl = c->d.cfl;
             if (c->d.time_scheme == OSB_TIME_RKLS3) {
                 for (int s = 0; s < c->rk_stages; ++s) { x.rkA[s] = c->rkA[s]; x.rkB[s] = c->rkB[s]; }
@@ -635,7 +635,7 @@ int launch_group_chunk(osb_pool* p, PoolGroup& pg, const std::vector<int>& membe
         x.q[0] = c->q_cur;
         x.q[1] = c->q_cur == c->qa ? c->qb : c->qa;
         x.du = c->du; x.F = c->F; x.scal = c->scal; x.g = c->g; x.c = c->c;
-        for (int d = 0; d < 3; ++d) { x.bc[d][0] = c->d.bc[d][0]; x.bc[d][1] = c->d.bc[d][1]; }
+        for (int d = 0; d < 3; ++d) x.bc[d] = bc_sides(c->d, d);
         x.cfl = c->d.cfl;
         g.any_cfl = g.any_cfl || c->d.cfl > 0;
         if (c->d.time_scheme == OSB_TIME_RKLS3) {

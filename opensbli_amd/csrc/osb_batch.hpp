@@ -18,7 +18,7 @@ struct BItem {
     double* scal;          // [0] wave-speed bits, [1] dt, [2] device time
     Geom g;
     Consts c;
-    int bc[3][2];
+    BcSides bc[3];         // boundary types and Dirichlet values per direction
     double cfl;            // > 0: dt from the wave-speed reduction
     double rkA[5], rkB[5]; // stage coefficients (classic RK3: rknew, rkold)
 };
@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(BLOCK) kb_bc(const BItem* __restrict__ items, 
         const int i = find_item(offs, n, vb);
         const BItem& it = items[i];
         const long t = (long)(vb - offs[i]) * BLOCK + threadIdx.x;
-        bc_item<DIR>(t, it.q[parity], it.g, it.g.ndim + 2, it.bc[DIR][0], it.bc[DIR][1]);
+        bc_item<DIR>(t, it.q[parity], it.g, it.g.ndim + 2, it.bc[DIR]);
     }
 }
 

@@ -7,6 +7,8 @@
 #include <cstdint>
 #include <type_traits>
 
+#include "../../include/osb.h"
+
 namespace osb {
 
 constexpr double INV_SQRT2 = 0.707106781186547;   // sqrt(2.0) printed to 15 s.f. (euler_eigensystem.py:109,139)

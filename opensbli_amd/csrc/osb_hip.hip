@@ -189,10 +189,10 @@ void apply_bcs(osb_ctx* c, double* q) {
         long work = 2L * (g.halo + 1);
         for (int d = 0; d < g.ndim; ++d)
             if (d != dir) work *= (g.n[d] + 2 * g.halo);
-        const int b0 = c->d.bc[dir][0], b1 = c->d.bc[dir][1];
-        if (dir == 0) k_bc<0><<<nblocks(work), BLOCK, 0, c->stream>>>(q, g, c->nv, b0, b1);
-        else if (dir == 1) k_bc<1><<<nblocks(work), BLOCK, 0, c->stream>>>(q, g, c->nv, b0, b1);
-        else k_bc<2><<<nblocks(work), BLOCK, 0, c->stream>>>(q, g, c->nv, b0, b1);
+        const BcSides b = bc_sides(c->d, dir);
+        if (dir == 0) k_bc<0><<<nblocks(work), BLOCK, 0, c->stream>>>(q, g, c->nv, b);
+        else if (dir == 1) k_bc<1><<<nblocks(work), BLOCK, 0, c->stream>>>(q, g, c->nv, b);
+        else k_bc<2><<<nblocks(work), BLOCK, 0, c->stream>>>(q, g, c->nv, b);
     }
 }
 
@@ -378,6 +378,8 @@ int validate(const osb_problem_desc* d, std::string& why) {
     if (d->viscous != OSB_VISC_NONE && d->ndim < 2) { why = "viscous terms need ndim >= 2"; return OSB_E_UNSUPPORTED; }
     if (d->time_scheme != OSB_TIME_RKLS3 && d->time_scheme != OSB_TIME_RK3) { why = "unknown time scheme"; return OSB_E_UNSUPPORTED; }
     if (d->precision != OSB_PREC_FP64 && d->precision != OSB_PREC_MIXED) { why = "unknown precision"; return OSB_E_UNSUPPORTED; }
+    if (d->flux_type != OSB_FLUX_LLF && d->flux_type != OSB_FLUX_GLF) { why = "unknown flux type"; return OSB_E_UNSUPPORTED; }
+    if (d->flux_type == OSB_FLUX_GLF && cs != OSB_CONV_WENO_Z && cs != OSB_CONV_TENO) { why = "GLF applies to shock capturing only"; return OSB_E_INVALID; }
     if (d->rk_stages < 0 || d->rk_stages > 5) { why = "rk_stages must be 0..5"; return OSB_E_INVALID; }
     if (d->rk_stages > 0 && d->time_scheme != OSB_TIME_RKLS3) { why = "rk_stages applies to OSB_TIME_RKLS3 only"; return OSB_E_INVALID; }
     for (int s = 0; s < d->rk_stages; ++s)
@@ -385,7 +387,14 @@ int validate(const osb_problem_desc* d, std::string& why) {
     for (int i = 0; i < d->ndim; ++i) {
         for (int s = 0; s < 2; ++s) {
             const int b = d->bc[i][s];
-            if (b < OSB_BC_PERIODIC || b > OSB_BC_SYMMETRY) { why = "unknown boundary condition"; return OSB_E_UNSUPPORTED; }
+            if (b < OSB_BC_PERIODIC || b > OSB_BC_DIRICHLET) { why = "unknown boundary condition"; return OSB_E_UNSUPPORTED; }
+            if (b >= OSB_BC_EXTRAPOLATION0 && (cs == OSB_CONV_CENTRAL_FEIEREISEN || cs == OSB_CONV_CENTRAL_SKEW || d->viscous != OSB_VISC_NONE)) {
+                why = "extrapolation/Dirichlet boundaries with central derivatives need the Carpenter closure (Carpenter_scheme.py), not built";
+                return OSB_E_UNSUPPORTED;
+            }
+            if (b == OSB_BC_DIRICHLET)
+                for (int v = 0; v < d->ndim + 2; ++v)
+                    if (!std::isfinite(d->bc_value[i][s][v])) { why = "non-finite Dirichlet value"; return OSB_E_INVALID; }
         }
         if ((d->bc[i][0] == OSB_BC_PERIODIC) != (d->bc[i][1] == OSB_BC_PERIODIC)) { why = "periodic boundaries must be paired"; return OSB_E_INVALID; }
     }

@@ -506,9 +506,16 @@ static __global__ void __launch_bounds__(BLOCK) k_copy_interior(const double* __
     for (int v = 0; v < nv; ++v) dst[(long)v * g.Ptot + x] = src[(long)v * g.Ptot + x];
 }
 
+// The two faces of one direction: OSB_BC_* type and, for OSB_BC_DIRICHLET,
+// the conservative values written there.
+struct BcSides {
+    int type[2];
+    double val[2][5];
+};
+
 // Boundary conditions along DIR (bc_core.py:128-202): side 0 layers t=0..h
 // (t = 0 is the boundary point), side 1 likewise, over the halo-extended
-// range of the other directions.  bc0/bc1: OSB_BC_* of the two sides.
+// range of the other directions.
 template <int DIR>
 __host__ __device__ inline long bc_work(const Geom& g) {
     long w = 2L * (g.halo + 1);
@@ -518,7 +525,7 @@ __host__ __device__ inline long bc_work(const Geom& g) {
 }
 
 template <int DIR>
-__device__ __forceinline__ void bc_item(long t, double* __restrict__ q, const Geom& g, int nv, int bc0, int bc1) {
+__device__ __forceinline__ void bc_item(long t, double* __restrict__ q, const Geom& g, int nv, const BcSides& b) {
     const int h = g.halo;
     int e[3], lo[3];
     for (int d = 0; d < 3; ++d) {
@@ -534,25 +541,48 @@ __device__ __forceinline__ void bc_item(long t, double* __restrict__ q, const Ge
     const int layer = ii[DIR];
     const int side = layer / (h + 1);
     const int tt = layer % (h + 1);
-    const int type = side == 0 ? bc0 : bc1;
+    const int type = b.type[side];
     const int n = g.n[DIR];
+    const int bnd = side == 0 ? 0 : n - 1;          // boundary point
+    const int inw = side == 0 ? 1 : -1;             // towards the interior (set_side_factor)
+    int a[3] = {ii[0], ii[1], ii[2]};
+    auto at = [&](int i) { a[DIR] = i; return gidx(g, a[0], a[1], a[2]); };
+    if (type == OSB_BC_EXTRAPOLATION1) {
+        // extrapolation.py:41-54: layer t reads layers t-1 and t-2, so one
+        // thread walks the layers of its line in order
+        if (tt != 0) return;
+        for (int s = 1; s <= h; ++s) {
+            const long x0 = at(bnd - inw * s), x1 = at(bnd - inw * (s - 1)), x2 = at(bnd - inw * (s - 2));
+            for (int v = 0; v < nv; ++v) {
+                double* qv = q + (long)v * g.Ptot;
+                qv[x0] = 2.0 * qv[x1] - qv[x2];
+            }
+        }
+        return;
+    }
+    if (type == OSB_BC_DIRICHLET) {                 // dirichlet.py:28-41
+        const long xd = at(bnd - inw * tt);
+        for (int v = 0; v < nv; ++v) q[(long)v * g.Ptot + xd] = b.val[side][v];
+        return;
+    }
     int dst, src;
     bool negate = false, zero = false;
     if (tt == 0) {
-        if (type != 2 && type != 3) return;            // zero-gradient / inviscid wall modify the boundary point
-        dst = side == 0 ? 0 : n - 1;
-        src = side == 0 ? 1 : n - 2;
-        zero = (type == 3);
+        // zero-gradient, inviscid wall and order-0 extrapolation modify the boundary point
+        if (type != OSB_BC_ZERO_GRADIENT && type != OSB_BC_INVISCID_WALL && type != OSB_BC_EXTRAPOLATION0) return;
+        dst = bnd;
+        src = bnd + inw;
+        zero = (type == OSB_BC_INVISCID_WALL);
+    } else if (type == OSB_BC_EXTRAPOLATION0) {     // extrapolation.py:36-40
+        dst = bnd - inw * tt;
+        src = bnd + inw;
     } else {
-        if (side == 0) { dst = -tt; src = (type == 1) ? n - tt : tt; }
-        else { dst = n - 1 + tt; src = (type == 1) ? tt - 1 : n - 1 - tt; }
-        negate = (type == 3 || type == 4);
+        if (side == 0) { dst = -tt; src = (type == OSB_BC_PERIODIC) ? n - tt : tt; }
+        else { dst = n - 1 + tt; src = (type == OSB_BC_PERIODIC) ? tt - 1 : n - 1 - tt; }
+        negate = (type == OSB_BC_INVISCID_WALL || type == OSB_BC_SYMMETRY);
     }
-    int a[3] = {ii[0], ii[1], ii[2]};
-    a[DIR] = dst;
-    const long xd = gidx(g, a[0], a[1], a[2]);
-    a[DIR] = src;
-    const long xs = gidx(g, a[0], a[1], a[2]);
+    const long xd = at(dst);
+    const long xs = at(src);
     for (int v = 0; v < nv; ++v) {
         double val = q[(long)v * g.Ptot + xs];
         if (v == 1 + DIR) {
@@ -563,12 +593,9 @@ __device__ __forceinline__ void bc_item(long t, double* __restrict__ q, const Ge
     }
 }
 
-// Boundary conditions along DIR (bc_core.py:128-202): side 0 layers t=0..h
-// (t = 0 is the boundary point), side 1 likewise, over the halo-extended
-// range of the other directions.  bc0/bc1: OSB_BC_* of the two sides.
 template <int DIR>
-__global__ void __launch_bounds__(BLOCK) k_bc(double* __restrict__ q, Geom g, int nv, int bc0, int bc1) {
-    bc_item<DIR>(gtid(), q, g, nv, bc0, bc1);
+__global__ void __launch_bounds__(BLOCK) k_bc(double* __restrict__ q, Geom g, int nv, BcSides b) {
+    bc_item<DIR>(gtid(), q, g, nv, b);
 }
 
 // host interior layout (v, k, j, i) <-> padded device layout

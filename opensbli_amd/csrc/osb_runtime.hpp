@@ -70,3 +70,14 @@ struct osb_ctx {
     int64_t prof_n[16] = {0};
 };
 
+// the two faces of direction dir as the BC kernels take them
+inline osb::BcSides bc_sides(const osb_problem_desc& d, int dir) {
+    osb::BcSides b;
+    for (int s = 0; s < 2; ++s) {
+        b.type[s] = d.bc[dir][s];
+        for (int v = 0; v < 5; ++v) b.val[s][v] = d.bc_value[dir][s][v];
+    }
+    return b;
+}
+
+

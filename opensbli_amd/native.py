@@ -19,7 +19,7 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 OK = 0
 E_INVALID = -1
@@ -47,6 +47,12 @@ BC_PERIODIC = 1
 BC_ZERO_GRADIENT = 2
 BC_INVISCID_WALL = 3
 BC_SYMMETRY = 4
+BC_EXTRAPOLATION0 = 5
+BC_EXTRAPOLATION1 = 6
+BC_DIRICHLET = 7
+
+FLUX_LLF = 0
+FLUX_GLF = 1
 
 PREC_FP64 = 1
 PREC_MIXED = 2
@@ -80,9 +86,10 @@ class ProblemDescC(C.Structure):
         ("dt", C.c_double),
         ("cfl", C.c_double),
         ("rk_stages", C.c_int32),
-        ("reserved0", C.c_int32),
+        ("flux_type", C.c_int32),
         ("rk_a", C.c_double * 5),
         ("rk_b", C.c_double * 5),
+        ("bc_value", ((C.c_double * 5) * 2) * 3),
         ("reserved", C.c_int32 * 8),
     ]
 
@@ -126,6 +133,9 @@ class ProblemDesc:
     # RungeKuttaLS coefficients; empty = the 3-stage Williamson scheme
     rk_a: Sequence[float] = ()
     rk_b: Sequence[float] = ()
+    flux_type: int = FLUX_LLF
+    # BC_DIRICHLET faces: {(direction, side): conservative values (rho, rhou.., rhoE)}
+    bc_value: dict = field(default_factory=dict)
 
     @property
     def nv(self) -> int:
@@ -178,6 +188,13 @@ class ProblemDesc:
         for s, (a, b) in enumerate(zip(self.rk_a, self.rk_b)):
             d.rk_a[s] = float(a)
             d.rk_b[s] = float(b)
+        d.flux_type = int(self.flux_type)
+        for (i, s), vals in self.bc_value.items():
+            vals = [float(v) for v in vals]
+            if len(vals) != self.nv:
+                raise ValueError(f"bc_value[{i},{s}] needs {self.nv} conservative values")
+            for v, x in enumerate(vals):
+                d.bc_value[i][s][v] = x
         return d
 
 

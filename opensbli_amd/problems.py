@@ -39,7 +39,12 @@ class Problem:
 # ---------------------------------------------------------------------------
 # 1-D Sod shock tube (apps/Sod_shock_tube/Sod_shock_tube.py:72-81)
 # ---------------------------------------------------------------------------
-def sod(n: int = 1024, steps: int = 1000, dt: float = 2.0e-4, precision: int = nat.PREC_FP64) -> Problem:
+def sod(n: int = 1024, steps: int = 1000, dt: float = 2.0e-4, precision: int = nat.PREC_FP64,
+        boundaries: str = "zero_gradient") -> Problem:
+    """``boundaries``: 'zero_gradient' (apps/Sod_shock_tube), 'dirichlet' (the
+    left and right states held at both ends,
+    tests/verification_apps/sod_shock_tube/verify_sod_shock_tube.py:96-109),
+    'extrapolation' (order 0) or 'linear' (order-1 extrapolation)."""
     gama = 1.4
     delta = 1.0 / (n - 1)
     x = delta * np.arange(n, dtype=np.float64)
@@ -55,6 +60,16 @@ def sod(n: int = 1024, steps: int = 1000, dt: float = 2.0e-4, precision: int = n
         ndim=1, n=[n, 1, 1], delta=[delta, 1.0, 1.0], conv_scheme=nat.CONV_WENO_Z, order=5,
         averaging=nat.AVG_SIMPLE, time_scheme=nat.TIME_RKLS3, precision=precision,
         bc=[[nat.BC_ZERO_GRADIENT, nat.BC_ZERO_GRADIENT]], gama=gama, dt=dt)
+    if boundaries == "dirichlet":
+        desc.bc = [[nat.BC_DIRICHLET, nat.BC_DIRICHLET]]
+        desc.bc_value = {(0, 0): [1.0, 1.0 * 0.0, 1.0 / (gama - 1.0) + 0.5 * 1.0 * (0.0 ** 2.0)],
+                         (0, 1): [0.125, 0.125 * 0.0, 0.1 / (gama - 1.0) + 0.5 * 0.125 * (0.0 ** 2.0)]}
+    elif boundaries == "extrapolation":
+        desc.bc = [[nat.BC_EXTRAPOLATION0, nat.BC_EXTRAPOLATION0]]
+    elif boundaries == "linear":
+        desc.bc = [[nat.BC_EXTRAPOLATION1, nat.BC_EXTRAPOLATION1]]
+    elif boundaries != "zero_gradient":
+        raise ValueError(f"unknown Sod boundaries {boundaries!r}")
     return Problem("sod", desc, q, steps)
 
 
@@ -199,7 +214,11 @@ def kh_ensemble(count: int = 64, seed: int = 20261203, sizes=(128, 256, 512, 102
 # 2-D four-quadrant Riemann problem (serving stream)
 # ---------------------------------------------------------------------------
 def riemann_2d(n: int = 256, states=None, steps: int = 100, precision: int = nat.PREC_FP64,
-               dt: Optional[float] = None) -> Problem:
+               dt: Optional[float] = None, boundaries: str = "zero_gradient") -> Problem:
+    """``boundaries``: 'zero_gradient' on all four faces, or 'mixed': x0 = 0
+    Dirichlet at the lower-left state, x0 = 1 linear extrapolation, x1 = 0
+    order-0 extrapolation, x1 = 1 symmetry (every face type that reads or writes
+    the corner halos of the other direction)."""
     gama = 1.4
     if states is None:
         # (rho, u, v, p) for quadrants (x>.5,y>.5), (x<.5,y>.5), (x<.5,y<.5), (x>.5,y<.5)
@@ -226,4 +245,11 @@ def riemann_2d(n: int = 256, states=None, steps: int = 100, precision: int = nat
         ndim=2, n=[n, n, 1], delta=[delta, delta, 1.0], conv_scheme=nat.CONV_WENO_Z, order=5,
         averaging=nat.AVG_ROE, time_scheme=nat.TIME_RKLS3, precision=precision,
         bc=[[nat.BC_ZERO_GRADIENT] * 2] * 2, gama=gama, dt=dt)
+    if boundaries == "mixed":
+        desc.bc = [[nat.BC_DIRICHLET, nat.BC_EXTRAPOLATION1], [nat.BC_EXTRAPOLATION0, nat.BC_SYMMETRY]]
+        r_, u_, v_, p_ = states[2]
+        desc.bc_value = {(0, 0): [r_, r_ * u_, r_ * v_, p_ / (gama - 1.0) + 0.5 * r_ * (u_ * u_ + v_ * v_)]}
+        return Problem(f"riemann_mixed_{n}", desc, q, steps)
+    if boundaries != "zero_gradient":
+        raise ValueError(f"unknown boundaries {boundaries!r}")
     return Problem(f"riemann_{n}", desc, q, steps)

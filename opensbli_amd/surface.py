@@ -205,12 +205,31 @@ class SymmetryBC(BoundaryConditionBase):
         self.scheme = scheme
 
 
+class ExtrapolationBC(BoundaryConditionBase):
+    """extrapolation.py:8.  order 0: the boundary point and the halos take the
+    first interior point; order 1: linear extrapolation into the halos."""
+
+    def __init__(self, direction, side, order, scheme=None, plane=True):
+        super().__init__(direction, side, plane)
+        if order not in (0, 1):
+            raise ValueError("Only zeroth order and linear extrapolation currently implemented.")
+        self.order = order
+        self.scheme = scheme
+
+
 class DirichletBC(BoundaryConditionBase):
-    """dirichlet.py; recorded so a script constructs, rejected by the HIP back end."""
+    """dirichlet.py:6.  ``equations`` as the reference's problem scripts write
+    them -- ``"Eq(GridVariable(d), 1.0)"`` ... ``"Eq(DataObject(rho), d)"``, as
+    text or as SymPy equations -- or a mapping {conservative name: value}.  The
+    HIP back end evaluates them once on the host (simulation parameters such as
+    ``gama`` in scope) and holds every conservative variable at the value on the
+    boundary point and this side's halos; values that depend on the grid
+    position are refused there."""
 
     def __init__(self, direction, side, equations, scheme=None, plane=True):
         super().__init__(direction, side, plane)
         self.equations = equations
+        self.scheme = scheme
 
 
 # --------------------------------------------------------------------------

@@ -743,6 +743,24 @@ void apply_bcs(osb_ctx* c) {
                     cp(0, to, true, false);                                // boundary point from q[1]
                 } else if (type == OSB_BC_SYMMETRY) {
                     for (int t = 1; t <= h; ++t) cp(-(long)to * t, (long)to * t, false, true);
+                } else if (type == OSB_BC_EXTRAPOLATION0) {
+                    // extrapolation.py:36-40: points 0, -1, .., -h (side 0) take q[1]
+                    for (int t = 0; t <= h; ++t) cp(-(long)to * t, to, false, false);
+                } else if (type == OSB_BC_EXTRAPOLATION1) {
+                    // extrapolation.py:41-54: q[-t] = 2.0*q[-t+1] - q[-t+2], t = 1..h in
+                    // order, so each layer reads the one written before it
+                    for (int t = 1; t <= h; ++t)
+                        for (int v = 0; v < nv; ++v) {
+                            double* qv = c->q.data() + (size_t)v * Pt;
+                            const double one = qv[(long)xb - (long)to * (t - 1) * (long)st];
+                            const double two = qv[(long)xb - (long)to * (t - 2) * (long)st];
+                            qv[(long)xb - (long)to * t * (long)st] = 2.0 * one - two;
+                        }
+                } else if (type == OSB_BC_DIRICHLET) {
+                    // dirichlet.py:28-41: the boundary point and the halo range of this side
+                    for (int t = 0; t <= h; ++t)
+                        for (int v = 0; v < nv; ++v)
+                            c->q[(size_t)v * Pt + (size_t)((long)xb - (long)to * t * (long)st)] = c->d.bc_value[dir][side][v];
                 }
             });
         }
@@ -830,6 +848,8 @@ int validate(const osb_problem_desc* d, std::string& why) {
     if (d->viscous != OSB_VISC_NONE && d->ndim < 2) { why = "viscous terms need ndim >= 2"; return OSB_E_UNSUPPORTED; }
     if (d->time_scheme != OSB_TIME_RKLS3 && d->time_scheme != OSB_TIME_RK3) { why = "unknown time scheme"; return OSB_E_UNSUPPORTED; }
     if (d->precision != OSB_PREC_FP64 && d->precision != OSB_PREC_MIXED) { why = "unknown precision"; return OSB_E_UNSUPPORTED; }
+    if (d->flux_type != OSB_FLUX_LLF && d->flux_type != OSB_FLUX_GLF) { why = "unknown flux type"; return OSB_E_UNSUPPORTED; }
+    if (d->flux_type == OSB_FLUX_GLF && cs != OSB_CONV_WENO_Z && cs != OSB_CONV_TENO) { why = "GLF applies to shock capturing only"; return OSB_E_INVALID; }
     if (d->rk_stages < 0 || d->rk_stages > 5) { why = "rk_stages must be 0..5"; return OSB_E_INVALID; }
     if (d->rk_stages > 0 && d->time_scheme != OSB_TIME_RKLS3) { why = "rk_stages applies to OSB_TIME_RKLS3 only"; return OSB_E_INVALID; }
     for (int s = 0; s < d->rk_stages; ++s)
@@ -837,7 +857,14 @@ int validate(const osb_problem_desc* d, std::string& why) {
     for (int i = 0; i < d->ndim; ++i) {
         for (int s = 0; s < 2; ++s) {
             const int b = d->bc[i][s];
-            if (b < OSB_BC_PERIODIC || b > OSB_BC_SYMMETRY) { why = "unknown boundary condition"; return OSB_E_UNSUPPORTED; }
+            if (b < OSB_BC_PERIODIC || b > OSB_BC_DIRICHLET) { why = "unknown boundary condition"; return OSB_E_UNSUPPORTED; }
+            if (b >= OSB_BC_EXTRAPOLATION0 && (cs == OSB_CONV_CENTRAL_FEIEREISEN || cs == OSB_CONV_CENTRAL_SKEW || d->viscous != OSB_VISC_NONE)) {
+                why = "extrapolation/Dirichlet boundaries with central derivatives need the Carpenter closure (Carpenter_scheme.py), not built";
+                return OSB_E_UNSUPPORTED;
+            }
+            if (b == OSB_BC_DIRICHLET)
+                for (int v = 0; v < d->ndim + 2; ++v)
+                    if (!std::isfinite(d->bc_value[i][s][v])) { why = "non-finite Dirichlet value"; return OSB_E_INVALID; }
         }
         if ((d->bc[i][0] == OSB_BC_PERIODIC) != (d->bc[i][1] == OSB_BC_PERIODIC)) { why = "periodic boundaries must be paired"; return OSB_E_INVALID; }
     }

@@ -28,6 +28,9 @@ def _ensemble():
         pr.name += f"_p{prec}"
         probs.append(pr)
     probs.append(P.sod(n=128, steps=5))
+    probs.append(P.sod(n=96, steps=5, boundaries="dirichlet"))
+    probs.append(P.sod(n=80, steps=5, boundaries="linear"))
+    probs.append(P.riemann_2d(n=36, steps=5, boundaries="mixed"))
     r = P.riemann_2d(n=40, steps=5)
     r.desc.cfl, r.desc.dt = 0.4, 0.0                  # device CFL inside the batch
     probs.append(r)
@@ -75,7 +78,8 @@ def test_batch_bitwise_equals_single(hip, graphs, monkeypatch):
 
 
 def test_batch_fp64_matches_oracle(hip, oracle):
-    probs = [P.kelvin_helmholtz(n=48, steps=20), P.riemann_2d(n=48, steps=20), P.sod(n=200, steps=20)]
+    probs = [P.kelvin_helmholtz(n=48, steps=20), P.riemann_2d(n=48, steps=20), P.sod(n=200, steps=20),
+             P.riemann_2d(n=40, steps=20, boundaries="mixed"), P.sod(n=160, steps=20, boundaries="dirichlet")]
     _announce("batch vs oracle")
     with hip.batch([p.desc for p in probs]) as b:
         for i, p in enumerate(probs):

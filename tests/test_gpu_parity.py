@@ -103,6 +103,36 @@ def test_riemann_small(hip, oracle):
     _compare(hip, oracle, P.riemann_2d(n=64, steps=50))
 
 
+# --- extrapolation and Dirichlet faces (extrapolation.py:29-58, dirichlet.py:28-41) ---
+@pytest.mark.parametrize("boundaries", ["dirichlet", "extrapolation", "linear"])
+def test_sod_boundary_variants(hip, oracle, boundaries):
+    pr = P.sod(boundaries=boundaries)
+    pr.name += "_" + boundaries
+    a, b = _compare(hip, oracle, pr)
+    assert np.array_equal(a, b)
+
+
+def test_riemann_mixed_boundaries(hip, oracle):
+    _residual_compare(hip, oracle, P.riemann_2d(n=48, boundaries="mixed"))
+    a, b = _compare(hip, oracle, P.riemann_2d(n=64, steps=60, boundaries="mixed"))
+    assert np.array_equal(a, b)
+
+
+def test_3d_shock_extrapolation_dirichlet(hip, oracle):
+    """3-D inviscid WENO-Z: extrapolation in x0, Dirichlet/linear in x1, periodic
+    x2 -- the BC kernels over both transverse halo ranges."""
+    pr = P.tgv(n=16, steps=8, scheme="weno", minf=1.25)
+    pr.desc.viscous = nat.VISC_NONE
+    pr.desc.bc = [[nat.BC_EXTRAPOLATION0, nat.BC_EXTRAPOLATION1], [nat.BC_DIRICHLET, nat.BC_EXTRAPOLATION1],
+                  [nat.BC_PERIODIC, nat.BC_PERIODIC]]
+    q0 = pr.q0[:, 0, 0, 0]
+    pr.desc.bc_value = {(1, 0): list(q0)}
+    pr.name += "_extrap_dirichlet"
+    _residual_compare(hip, oracle, pr)
+    a, b = _compare(hip, oracle, pr)
+    assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("order,stages,formulation", [(3, 5, "SSP"), (4, None, None)])
 def test_rkls_variants(hip, oracle, order, stages, formulation):
     """RungeKuttaLS SSP(5,3) and the 4th-order 5-stage scheme through the

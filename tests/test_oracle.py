@@ -243,6 +243,80 @@ def test_bc_index_maps_1d(oracle, hooks, bc):
             assert at(v, 0) == q[v, 0, 0, 0] and at(v, n - 1) == q[v, 0, 0, n - 1]
 
 
+def _bc_state_1d(oracle, hooks, bc0, bc1, q, values=None):
+    n = q.shape[-1]
+    desc = nat.ProblemDesc(ndim=1, n=[n], delta=[0.1], conv_scheme=nat.CONV_WENO_Z, bc=[[bc0, bc1]], dt=1e-3,
+                           bc_value=values or {})
+    with oracle.context(desc) as c:
+        c.set_state(q)
+        Q, H = _padded(hooks, c)
+    return Q[:, 0, 0], H
+
+
+def test_bc_extrapolation_order0_index_map(oracle, hooks):
+    """extrapolation.py:36-40: points 0, -1, .., -h take q[1]; side 1 mirrors about N-1."""
+    n, h = 10, 4
+    q = np.random.default_rng(5).uniform(1, 2, (3, 1, 1, n))
+    Q, H = _bc_state_1d(oracle, hooks, nat.BC_EXTRAPOLATION0, nat.BC_EXTRAPOLATION0, q)
+    for v in range(3):
+        for t in range(0, h + 1):
+            assert Q[v, H - t] == q[v, 0, 0, 1]
+            assert Q[v, H + n - 1 + t] == q[v, 0, 0, n - 2]
+        assert np.array_equal(Q[v, H + 1:H + n - 1], q[v, 0, 0, 1:n - 1])
+
+
+def test_bc_extrapolation_order1_index_map(oracle, hooks):
+    """extrapolation.py:41-54: q[-t] = 2 q[-t+1] - q[-t+2] in order t = 1..h; the
+    boundary point is kept.  Linear data continue exactly (integers), random data
+    follow the recursion bit for bit (a closed form 2q[0]-q[1], t(q[0]-q[1]) + q[0]
+    would round differently)."""
+    n, h = 10, 4
+    i = np.arange(n, dtype=np.float64)
+    lin = np.stack([3.0 + 2.0 * i, -5.0 + 1.0 * i, 7.0 - 3.0 * i]).reshape(3, 1, 1, n)
+    Q, H = _bc_state_1d(oracle, hooks, nat.BC_EXTRAPOLATION1, nat.BC_EXTRAPOLATION1, lin)
+    ext = np.arange(-h, n + h, dtype=np.float64)
+    assert np.array_equal(Q[:, H - h:H + n + h], np.stack([3.0 + 2.0 * ext, -5.0 + 1.0 * ext, 7.0 - 3.0 * ext]))
+    q = np.random.default_rng(6).uniform(1, 2, (3, 1, 1, n))
+    Q, H = _bc_state_1d(oracle, hooks, nat.BC_EXTRAPOLATION1, nat.BC_EXTRAPOLATION1, q)
+    for v in range(3):
+        lo, hi = list(q[v, 0, 0, :2][::-1]), list(q[v, 0, 0, -2:])   # [q1, q0], [q_{n-2}, q_{n-1}]
+        for t in range(1, h + 1):
+            lo.append(2.0 * lo[-1] - lo[-2])
+            hi.append(2.0 * hi[-1] - hi[-2])
+            assert Q[v, H - t] == lo[-1] and Q[v, H + n - 1 + t] == hi[-1]
+        assert Q[v, H] == q[v, 0, 0, 0] and Q[v, H + n - 1] == q[v, 0, 0, n - 1]
+
+
+def test_bc_dirichlet_index_map(oracle, hooks):
+    """dirichlet.py:28-41: the boundary point and this side's halos take the values;
+    the opposite side is independent (mixed with a zero-gradient outlet)."""
+    n, h = 10, 4
+    q = np.random.default_rng(7).uniform(1, 2, (3, 1, 1, n))
+    vals = {(0, 0): [1.5, -0.25, 4.0]}
+    Q, H = _bc_state_1d(oracle, hooks, nat.BC_DIRICHLET, nat.BC_ZERO_GRADIENT, q, vals)
+    for v in range(3):
+        for t in range(0, h + 1):
+            assert Q[v, H - t] == vals[(0, 0)][v]
+            if t > 0:
+                assert Q[v, H + n - 1 + t] == q[v, 0, 0, n - 1 - t]
+        assert Q[v, H + n - 1] == q[v, 0, 0, n - 2]
+        assert np.array_equal(Q[v, H + 1:H + n - 1], q[v, 0, 0, 1:n - 1])
+
+
+def test_bc_dirichlet_and_extrapolation_rejected_with_central(oracle):
+    """The Carpenter one-sided closure (Carpenter_scheme.py) that these boundaries switch
+    on for central derivatives is not built: the descriptor is refused, not run wrongly."""
+    pr = P.tgv(n=12)
+    pr.desc.bc = [[nat.BC_EXTRAPOLATION0, nat.BC_EXTRAPOLATION0]] + [[nat.BC_PERIODIC] * 2] * 2
+    with pytest.raises(nat.OSBError) as e:
+        oracle.context(pr.desc)
+    assert e.value.code == nat.E_UNSUPPORTED and "Carpenter" in str(e.value)
+    pr.desc.bc = [[nat.BC_DIRICHLET, nat.BC_DIRICHLET]] + [[nat.BC_PERIODIC] * 2] * 2
+    pr.desc.bc_value = {(0, 0): [1.0, 0.0, 0.0, 0.0, 2.5], (0, 1): [1.0, 0.0, 0.0, 0.0, float("nan")]}
+    with pytest.raises(nat.OSBError):
+        oracle.context(pr.desc)
+
+
 # --------------------------------------------------------------------------------
 # a4/a6: free-stream preservation, conservation, shift invariance, derivative order
 # --------------------------------------------------------------------------------
@@ -319,9 +393,13 @@ def test_central_pressure_gradient_fourth_order(oracle):
 # --------------------------------------------------------------------------------
 # known answers held by the reference
 # --------------------------------------------------------------------------------
-def test_sod_against_exact_solution(oracle):
-    """apps/Sod_shock_tube/reference.txt: exact density at t = 0.2 (1000 points)."""
-    pr = P.sod()
+@pytest.mark.parametrize("boundaries", ["zero_gradient", "dirichlet", "extrapolation", "linear"])
+def test_sod_against_exact_solution(oracle, boundaries):
+    """apps/Sod_shock_tube/reference.txt: exact density at t = 0.2 (1000 points).
+    The waves stay clear of both ends until t = 0.2, so every boundary treatment
+    (zero-gradient as in apps/Sod_shock_tube, Dirichlet as in
+    verify_sod_shock_tube.py:96-109, both extrapolation orders) must agree with it."""
+    pr = P.sod(boundaries=boundaries)
     q, st = run(oracle, pr)
     assert abs(st.time - 0.2) < 1e-12
     ref = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "sod_exact_t0.2.txt"))

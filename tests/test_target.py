@@ -15,7 +15,7 @@ from helpers import run
 from opensbli_amd import native as nat, problems as P
 from opensbli_amd.code_generation.hip import HIPTarget, evaluate_parameters, plan_problem
 from opensbli_amd.surface import (
-    Central, DirichletBC, InviscidWallBC, LFTeno, LFWeno, NS_Split, PeriodicBC, RoeAverage, RungeKutta,
+    Central, DirichletBC, ExtrapolationBC, InviscidWallBC, LFTeno, LFWeno, NS_Split, PeriodicBC, RoeAverage, RungeKutta,
     RungeKuttaLS, SimpleAverage, SimulationBlock, StoreSome, SymmetryBC, TraditionalAlgorithmRK,
     ZeroGradientOutletBC)
 
@@ -109,10 +109,60 @@ def test_unsupported_configurations_raise(bad, match):
         plan_problem(alg, {"block0np0": 64, "Delta0block0": 0.01, "dt": 1e-3})
 
 
-def test_dirichlet_rejected():
-    alg = _block(1, [LFWeno(5, formulation="Z"), RungeKuttaLS(3)], [DirichletBC(0, 0, []), ZeroGradientOutletBC(0, 1)])
-    with pytest.raises(NotImplementedError, match="DirichletBC"):
-        plan_problem(alg, {"block0np0": 64, "Delta0block0": 0.01, "dt": 1e-3})
+# the left and right states of tests/verification_apps/sod_shock_tube/verify_sod_shock_tube.py:96-103
+_SOD_LEFT = ["Eq(GridVariable(d), 1.0)", "Eq(GridVariable(u0), 0.0)", "Eq(GridVariable(p), 1.0)",
+             "Eq(DataObject(rho), d)", "Eq(DataObject(rhou0), d*u0)",
+             "Eq(DataObject(rhoE), p/(gama-1.0) + 0.5* d *(u0**2.0))"]
+_SOD_RIGHT = ["Eq(GridVariable(d), 0.125)", "Eq(GridVariable(u0), 0.0)", "Eq(GridVariable(p), 0.1)"] + _SOD_LEFT[3:]
+_SOD_PARAMS = {"gama": "1.4", "dt": "0.0002", "niter": "ceil(0.2/0.0002)", "block0np0": "1024",
+               "Delta0block0": "1.0/(block0np0-1)"}
+
+
+def test_dirichlet_plan_from_script_equations(oracle):
+    """The Sod verification app's Dirichlet equations, evaluated on the host, give
+    the descriptor of the Dirichlet Sod problem; a run through HIPTarget equals it."""
+    weno = LFWeno(5, formulation="Z", averaging=SimpleAverage([0, 1]), flux_type="LLF")
+    alg = _block(1, [weno, RungeKuttaLS(3)], [DirichletBC(0, 0, _SOD_LEFT), DirichletBC(0, 1, _SOD_RIGHT)])
+    want = P.sod(boundaries="dirichlet")
+    got = plan_problem(alg, _SOD_PARAMS)
+    _same(got, want.desc)
+    assert got.bc_value[(0, 1)] == [0.125, 0.0, 0.1 / (1.4 - 1.0)]
+    t = HIPTarget(alg, parameters=dict(_SOD_PARAMS, niter="20"), library=oracle)
+    t.initialise(want.q0)
+    q = t.run().state()
+    t.close()
+    want.steps = 20
+    q_ref, _ = run(oracle, want)
+    assert np.array_equal(q, q_ref)
+
+
+def test_extrapolation_plan():
+    weno = LFWeno(5, formulation="Z", averaging=SimpleAverage([0, 1]), flux_type="LLF")
+    alg = _block(1, [weno, RungeKuttaLS(3)], [ExtrapolationBC(0, 0, order=1), ExtrapolationBC(0, 1, order=0)])
+    got = plan_problem(alg, _SOD_PARAMS)
+    assert [list(b) for b in got.bc] == [[nat.BC_EXTRAPOLATION1, nat.BC_EXTRAPOLATION0]]
+    with pytest.raises(ValueError):
+        ExtrapolationBC(0, 0, order=2)
+
+
+@pytest.mark.parametrize("bcs,match", [
+    (lambda: [DirichletBC(0, 0, []), ZeroGradientOutletBC(0, 1)], "unset"),
+    (lambda: [DirichletBC(0, 0, ["Eq(DataObject(rho), DataObject(x0))"] + _SOD_LEFT[1:]), ZeroGradientOutletBC(0, 1)],
+     "not a constant"),
+])
+def test_dirichlet_unsupported_forms_raise(bcs, match):
+    alg = _block(1, [LFWeno(5, formulation="Z"), RungeKuttaLS(3)], bcs())
+    with pytest.raises(NotImplementedError, match=match):
+        plan_problem(alg, _SOD_PARAMS)
+
+
+def test_extrapolation_with_central_derivatives_raises():
+    NS = NS_Split("Feiereisen", 2, ["Re", "Pr", "gama", "Minf"], viscosity="constant")
+    alg = _block(2, [StoreSome(4, "u0 u1"), RungeKuttaLS(3)],
+                 [ExtrapolationBC(0, 0, order=0), ExtrapolationBC(0, 1, order=0), PeriodicBC(1, 0), PeriodicBC(1, 1)])
+    with pytest.raises(NotImplementedError, match="Carpenter"):
+        plan_problem(alg, {"block0np0": 16, "block0np1": 16, "Delta0block0": 0.1, "Delta1block0": 0.1, "dt": 1e-3},
+                     physics=NS)
 
 
 def test_rk_ssp_plan_and_run(oracle):
