@@ -222,8 +222,10 @@ def plan_problem(alg_or_block, parameters: Mapping, physics=None, precision: str
     desc = dict(ndim=ndim, n=n, delta=delta)
     if shock is not None:
         kinds = _kinds(shock)
-        if getattr(shock, "flux_type", "LLF") != "LLF":
-            raise NotImplementedError(f"{type(shock).__name__}(flux_type={shock.flux_type!r}): only LLF")
+        ft = getattr(shock, "flux_type", None) or "LLF"      # shock_capturing.py:780-781
+        if ft not in ("LLF", "GLF"):
+            raise NotImplementedError(f"{type(shock).__name__}(flux_type={ft!r}): LLF and GLF are implemented")
+        desc["flux_type"] = nat.FLUX_GLF if ft == "GLF" else nat.FLUX_LLF
         if "LFWeno" in kinds:
             if shock.order != 5 or str(getattr(shock, "formulation", "JS")).upper() != "Z":
                 raise NotImplementedError(

@@ -36,6 +36,7 @@ const char* const kBFamName[BF_COUNT] = {"face_flux_x", "face_flux_y", "face_flu
 struct Group {
     int ndim = 0, sch = 0, mixed = 0, time = 0, stages = 0;
     bool any_cfl = false;
+    bool any_glf = false;
     std::vector<int> members;
     std::vector<int> offs;                 // BF_COUNT x (n + 1) virtual-block offsets
     BItem* d_items = nullptr;
@@ -131,6 +132,7 @@ int build_groups(osb_batch* b) {
         }
         gp->members.push_back(i);
         gp->any_cfl = gp->any_cfl || c->d.cfl > 0;
+        gp->any_glf = gp->any_glf || c->c.glf != nullptr;
         gp->tiled = c->face_tiled;
     }
     for (auto& gr : b->groups) {
@@ -242,6 +244,10 @@ void enqueue_group_step_nd(hipStream_t st, const Group& gr, int parity) {
     if (gr.time == OSB_TIME_RK3)
         kb_save_state<<<grid_of(gr, BF_SAVE), BLOCK, 0, st>>>(gr.d_items, gr.dof(BF_SAVE), n, parity);
     for (int s = 0; s < gr.stages; ++s) {
+        if (gr.any_glf) {
+            kb_glf_zero<<<(n + 63) / 64, 64, 0, st>>>(gr.d_items, n);
+            kb_glf<NDIM><<<grid_of(gr, BF_WAVE), BLOCK, 0, st>>>(gr.d_items, gr.dof(BF_WAVE), n, parity);
+        }
         for (int d = 0; d < NDIM; ++d) launch_face_dir<NDIM>(st, gr, d, parity, grid_of(gr, BF_FACE0 + d));
         if (gr.time == OSB_TIME_RK3)
             kb_resid_rk<NDIM, 2><<<grid_of(gr, BF_RESID), BLOCK, 0, st>>>(gr.d_items, gr.dof(BF_RESID), n, parity, s);
@@ -603,6 +609,7 @@ int launch_group_chunk(osb_pool* p, PoolGroup& pg, const std::vector<int>& membe
     for (int sl : members) g.members.push_back(sl);
     g.offs.assign((size_t)BF_COUNT * (n + 1), 0);
     g.any_cfl = false;
+    g.any_glf = false;
     for (int f = 0; f < BF_COUNT; ++f) {
         long acc = 0;
         for (int m = 0; m < n; ++m) {
@@ -638,6 +645,7 @@ int launch_group_chunk(osb_pool* p, PoolGroup& pg, const std::vector<int>& membe
         for (int d = 0; d < 3; ++d) x.bc[d] = bc_sides(c->d, d);
         x.cfl = c->d.cfl;
         g.any_cfl = g.any_cfl || c->d.cfl > 0;
+        g.any_glf = g.any_glf || c->c.glf != nullptr;
         if (c->d.time_scheme == OSB_TIME_RKLS3) {
             for (int s = 0; s < c->rk_stages; ++s) { x.rkA[s] = c->rkA[s]; x.rkB[s] = c->rkB[s]; }
         } else {

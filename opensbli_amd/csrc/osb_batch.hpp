@@ -128,6 +128,37 @@ __global__ void __launch_bounds__(BLOCK) kb_wavespeed(const BItem* __restrict__ 
     }
 }
 
+// GLF wave speeds of every GLF item (it.c.glf set), as k_glf_zero / k_glf
+static __global__ void kb_glf_zero(const BItem* __restrict__ items, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !items[i].c.glf) return;
+    unsigned long long* z = reinterpret_cast<unsigned long long*>(const_cast<double*>(items[i].c.glf));
+    for (int r = 0; r < 9; ++r) z[r] = 0ull;
+}
+
+template <int NDIM>
+__global__ void __launch_bounds__(BLOCK) kb_glf(const BItem* __restrict__ items, const int* __restrict__ offs, int n,
+                                                int parity) {
+    OSB_BATCH_LOOP {
+        const int i = find_item(offs, n, vb);
+        const BItem& it = items[i];
+        const Geom& g = it.g;
+        const long t = (long)(vb - offs[i]) * BLOCK + threadIdx.x;
+        const int e[3] = {g.n[0], g.n[1], g.n[2]};
+        const int lo[3] = {0, 0, 0};
+        double m[3 * NDIM];
+#pragma unroll
+        for (int r = 0; r < 3 * NDIM; ++r) m[r] = 0.0;
+        const bool on = it.c.glf != nullptr;          // uniform over the block: one item per virtual block
+        if (on && t < (long)e[0] * e[1] * e[2]) {
+            int a, b, c;
+            unflatten(t, e, lo, a, b, c);
+            glf_point<NDIM>(it.q[parity], gidx(g, a, b, c), g.Ptot, it.c, m);
+        }
+        if (on) glf_block_to<NDIM>(m, const_cast<double*>(it.c.glf));
+    }
+}
+
 static __global__ void kb_dt_finalize(const BItem* __restrict__ items, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || !(items[i].cfl > 0)) return;

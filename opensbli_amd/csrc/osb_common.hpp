@@ -41,6 +41,9 @@ struct Consts {
     double cfl, dt;
     int avg_roe;
     int viscous;           // 0 none, 1 constant, 2 Sutherland
+    // GLF splitting: per direction d, glf[3d..3d+2] = max over the grid of
+    // |u_d|, |u_d + a|, |u_d - a| for the current stage; nullptr: LLF
+    const double* glf;
 };
 
 // ---------------------------------------------------------------------------
@@ -419,9 +422,23 @@ __device__ __forceinline__ void face_core(PT&& pt, const Consts& c, double (&out
     }
     T L[NV][NV], R[NV][NV];
     eigensystem<NDIM, DIR, T>(T(c.gamma_m1), rho_av, u_av, a_av, L, R);
-    // local Lax-Friedrichs wave speeds over offsets {0,1} (shock_capturing.py:1000-1028)
+    // wave speeds: global (GLF, the stage's grid maxima in the eigenvalue order
+    // of the direction, shock_capturing.py:838-857) or local Lax-Friedrichs over
+    // offsets {0,1} (shock_capturing.py:1000-1028)
     T lam[NV];
-    {
+    if (c.glf) {
+        const double* gl = c.glf + 3 * DIR;
+        if (NDIM == 1) {
+            lam[0] = T(gl[2]);
+            lam[1] = T(gl[0]);
+            lam[2] = T(gl[1]);
+        } else {
+#pragma unroll
+            for (int d = 0; d < NDIM; ++d) lam[d] = T(gl[0]);
+            lam[NDIM] = T(gl[1]);
+            lam[NDIM + 1] = T(gl[2]);
+        }
+    } else {
         const double ua = u23[0][DIR], ub = u23[1][DIR];
         if (NDIM == 1) {
             lam[0] = T(c.sfc * fmax(fabs(ua - a23[0]), fabs(ub - a23[1])));

@@ -120,10 +120,25 @@ void launch_flux(osb_ctx* c, const double* q) {
     else launch_flux_t<NDIM, DIR, double>(c, q);
 }
 
+// GLF: the stage's global wave speeds, before the face fluxes that read them
+void launch_glf(osb_ctx* c, const double* q) {
+    if (!c->c.glf) return;
+    Timed tm(c, PF_REDUCE);
+    double* glf = c->scal + OSB_SCAL_GLF;
+    k_glf_zero<<<1, 64, 0, c->stream>>>(glf);
+    const unsigned nb = std::min<unsigned>(nblocks(interior_points(c->g)), 2048u);
+    switch (c->g.ndim) {
+        case 1: k_glf<1><<<nb, BLOCK, 0, c->stream>>>(q, glf, c->g, c->c); break;
+        case 2: k_glf<2><<<nb, BLOCK, 0, c->stream>>>(q, glf, c->g, c->c); break;
+        default: k_glf<3><<<nb, BLOCK, 0, c->stream>>>(q, glf, c->g, c->c); break;
+    }
+}
+
 template <int NDIM>
 void rhs_nd(osb_ctx* c, const double* q) {
     const long nin = interior_points(c->g);
     if (c->shock) {
+        launch_glf(c, q);
         launch_flux<NDIM, 0>(c, q);
         if (NDIM > 1) launch_flux<NDIM, (NDIM > 1 ? 1 : 0)>(c, q);
         if (NDIM > 2) launch_flux<NDIM, (NDIM > 2 ? 2 : 0)>(c, q);
@@ -158,6 +173,7 @@ void rhs_nd(osb_ctx* c, const double* q) {
 // differences them and applies the RK update (no residual array).
 template <int NDIM, int TIME>
 void shock_stage_nd(osb_ctx* c, const double* q, double* qn, double A, double B) {
+    launch_glf(c, q);
     launch_flux<NDIM, 0>(c, q);
     if (NDIM > 1) launch_flux<NDIM, (NDIM > 1 ? 1 : 0)>(c, q);
     if (NDIM > 2) launch_flux<NDIM, (NDIM > 2 ? 2 : 0)>(c, q);
@@ -282,6 +298,7 @@ void launch_mixed_v(osb_ctx* c, const double* q, double* qn, double A, double B)
 template <int TIME>
 void fused_stage(osb_ctx* c, const double* q, double* qn, double A, double B) {
     if (c->shock) {
+        launch_glf(c, q);
         launch_flux<3, 0>(c, q);
         launch_flux<3, 1>(c, q);
         launch_flux<3, 2>(c, q);
@@ -442,7 +459,7 @@ constexpr int PART_BLOCKS = 1024;
 struct AllocPlan {
     size_t q = 0, R = 0, F = 0, cr = 0, gr = 0, staging = 0;
     size_t bytes() const {
-        return sizeof(double) * (3 * q + R + F + cr + gr + 16 + staging + (size_t)PART_BLOCKS * 4) + sizeof(unsigned int);
+        return sizeof(double) * (3 * q + R + F + cr + gr + OSB_SCAL_SIZE + staging + (size_t)PART_BLOCKS * 4) + sizeof(unsigned int);
     }
 };
 
@@ -566,6 +583,7 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     k.dt = desc->dt;
     k.avg_roe = desc->averaging == OSB_AVG_ROE;
     k.viscous = desc->viscous;
+    k.glf = nullptr;
 
     const AllocPlan ap = plan_alloc(desc);
     int r = OSB_OK;
@@ -576,7 +594,8 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     if (!r && ap.F) r = dalloc(c, &c->F, ap.F);
     if (!r && ap.cr) r = dalloc(c, &c->cr, ap.cr);
     if (!r && ap.gr) r = dalloc(c, &c->gr, ap.gr);
-    if (!r) r = dalloc(c, &c->scal, 16);
+    if (!r) r = dalloc(c, &c->scal, OSB_SCAL_SIZE);
+    if (!r && c->shock && desc->flux_type == OSB_FLUX_GLF) k.glf = c->scal + OSB_SCAL_GLF;
     if (!r) r = dalloc(c, &c->staging, ap.staging);
     c->part_blocks = PART_BLOCKS;
     if (!r) r = dalloc(c, &c->part, (size_t)c->part_blocks * 4);

@@ -689,6 +689,63 @@ __global__ void __launch_bounds__(BLOCK) k_wavespeed(const double* __restrict__ 
     block_max_to(m, scal);
 }
 
+// GLF wave speeds ("Global wave-speed reductions", weno.py:515-519,
+// shock_capturing.py:838-857): per direction the maxima over the grid of
+// |u_d|, |u_d + a|, |u_d - a|, from the same point primitives as the face
+// fluxes.  Stored as bit patterns of non-negative doubles (order-preserving
+// for atomicMax); k_glf_zero clears them first.
+static __global__ void k_glf_zero(double* glf) {
+    if (blockIdx.x == 0 && threadIdx.x < 9) reinterpret_cast<unsigned long long*>(glf)[threadIdx.x] = 0ull;
+}
+
+template <int NDIM>
+__device__ __forceinline__ void glf_point(const double* __restrict__ q, long x, long Ptot, const Consts& c,
+                                          double (&m)[3 * NDIM]) {
+    double v[NDIM + 2], f[NDIM + 2], uu[NDIM], p, a;
+#pragma unroll
+    for (int k = 0; k < NDIM + 2; ++k) v[k] = q[(long)k * Ptot + x];
+    point_prims<NDIM, 0, false>(v, c, uu, p, f, a);
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) {
+        m[3 * d] = fmax(m[3 * d], fabs(uu[d]));
+        m[3 * d + 1] = fmax(m[3 * d + 1], fabs(uu[d] + a));
+        m[3 * d + 2] = fmax(m[3 * d + 2], fabs(uu[d] - a));
+    }
+}
+
+template <int NDIM>
+__device__ __forceinline__ void glf_block_to(double (&m)[3 * NDIM], double* glf) {
+    __shared__ double red[3 * NDIM][BLOCK / 64];
+#pragma unroll
+    for (int r = 0; r < 3 * NDIM; ++r) {
+        const double w = wave_max(m[r]);
+        if ((threadIdx.x & 63) == 0) red[r][threadIdx.x >> 6] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3 * NDIM) {
+        double b = red[threadIdx.x][0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = fmax(b, red[threadIdx.x][w]);
+        atomicMax(reinterpret_cast<unsigned long long*>(glf) + threadIdx.x, (unsigned long long)__double_as_longlong(b));
+    }
+    __syncthreads();
+}
+
+template <int NDIM>
+__global__ void __launch_bounds__(BLOCK) k_glf(const double* __restrict__ q, double* glf, Geom g, Consts c) {
+    const int e[3] = {g.n[0], g.n[1], g.n[2]};
+    const int lo[3] = {0, 0, 0};
+    const long tot = (long)e[0] * e[1] * e[2];
+    double m[3 * NDIM];
+#pragma unroll
+    for (int r = 0; r < 3 * NDIM; ++r) m[r] = 0.0;
+    for (long t = gtid(); t < tot; t += (long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        unflatten(t, e, lo, i, j, k);
+        glf_point<NDIM>(q, gidx(g, i, j, k), g.Ptot, c, m);
+    }
+    glf_block_to<NDIM>(m, glf);
+}
+
 // dt = cfl / max wave speed; time += dt
 static __global__ void k_dt_finalize(double* scal, double cfl) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {

@@ -9,6 +9,10 @@
 #include <string>
 #include <vector>
 
+// per-context device scalars (see osb_ctx::scal)
+constexpr int OSB_SCAL_SIZE = 32;
+constexpr int OSB_SCAL_GLF = 16;
+
 // message of the last failed create call (osb_last_error(NULL))
 extern std::string g_osb_create_error;
 
@@ -33,7 +37,8 @@ struct osb_ctx {
     double* F = nullptr;
     double* cr = nullptr;
     double* gr = nullptr;
-    double* scal = nullptr;         // [0] max wavespeed bits, [1] dt, [2] device time, [3..6] reduction out
+    double* scal = nullptr;         // OSB_SCAL_SIZE: [0] max wavespeed bits, [1] dt, [2] device time,
+                                    // [4..7] TGV reduction out, [OSB_SCAL_GLF..+9) GLF wave speeds
     double* staging = nullptr;      // interior-layout transfer buffer
     double* part = nullptr;         // reduction partials
     unsigned int* nan_cnt = nullptr;

@@ -60,6 +60,7 @@ struct osb_ctx {
     std::string err;
     int rk_stages = 3;                    // RK-LS stages and coefficients
     double rkA[5] = {0}, rkB[5] = {0};
+    double glf[3][3] = {};                // GLF: max |u_d|, |u_d + a|, |u_d - a| over the grid, per stage
 
     size_t I(int i, int j, int k) const {
         return (size_t)(i + off[0]) + (size_t)P[0] * ((size_t)(j + off[1]) + (size_t)P[1] * (size_t)(k + off[2]));
@@ -464,7 +465,17 @@ void reconstruct_direction(osb_ctx* c, int dir) {
         eigensystem(nd, dir, gamma_m1, rho_av, u_av, a_av, e);
         // local LF wave speeds: max over offsets {0, 1} of |eigenvalue| (shock_capturing.py:1000-1028)
         double lam[5];
-        if (nd == 1) {
+        if (c->d.flux_type == OSB_FLUX_GLF) {
+            // global wave speeds of this stage (shock_capturing.py:838-857): the
+            // reductions follow the eigenvalue order of the direction
+            const double* gl = c->glf[dir];
+            if (nd == 1) { lam[0] = gl[2]; lam[1] = gl[0]; lam[2] = gl[1]; }
+            else {
+                for (int m = 0; m < nd; ++m) lam[m] = gl[0];
+                lam[nd] = gl[1];
+                lam[nd + 1] = gl[2];
+            }
+        } else if (nd == 1) {
             lam[0] = sfc * std::max(std::fabs(ud[2] - aa[2]), std::fabs(ud[3] - aa[3]));
             lam[1] = sfc * std::max(std::fabs(ud[2]), std::fabs(ud[3]));
             lam[2] = sfc * std::max(std::fabs(ud[2] + aa[2]), std::fabs(ud[3] + aa[3]));
@@ -768,6 +779,29 @@ void apply_bcs(osb_ctx* c) {
 }
 
 // ---------------------------------------------------------------------------
+// "Global wave-speed reductions" kernel of GLF splitting (weno.py:515-519,
+// teno.py:520-524; shock_capturing.py:416-438, :838-857): per direction the
+// maxima over the grid [0, N) of |u_d|, |u_d + a|, |u_d - a| from the
+// constituent relations of this stage.
+// ---------------------------------------------------------------------------
+void glf_reductions(osb_ctx* c) {
+    const Range r = make_range(c, 0);
+    for (int d = 0; d < c->ndim; ++d) {
+        double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+        const double* ud = c->u.data() + (size_t)d * c->Ptot;
+        for (int k = r.lo[2]; k < r.hi[2]; ++k)          // serial: a max is order-free anyway
+            for (int j = r.lo[1]; j < r.hi[1]; ++j)
+                for (int i = r.lo[0]; i < r.hi[0]; ++i) {
+                    const size_t x = c->I(i, j, k);
+                    m0 = std::max(m0, std::fabs(ud[x]));
+                    m1 = std::max(m1, std::fabs(ud[x] + c->a[x]));
+                    m2 = std::max(m2, std::fabs(ud[x] - c->a[x]));
+                }
+        c->glf[d][0] = m0; c->glf[d][1] = m1; c->glf[d][2] = m2;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Right-hand side at the current state (halos already filled).
 // ---------------------------------------------------------------------------
 void compute_rhs(osb_ctx* c) {
@@ -779,6 +813,7 @@ void compute_rhs(osb_ctx* c) {
     }
     constituent_relations(c, rcr, c->viscous || c->central);
     if (c->shock) {
+        if (c->d.flux_type == OSB_FLUX_GLF) glf_reductions(c);
         for (int dir = 0; dir < c->ndim; ++dir) reconstruct_direction(c, dir);
         shock_residual(c);
     }
@@ -1168,6 +1203,14 @@ int osb_oracle_padded_state(osb_ctx* c, double* out, int* dims) {
     apply_bcs(c);
     dims[0] = c->P[0]; dims[1] = c->P[1]; dims[2] = c->P[2]; dims[3] = HALO;
     if (out) std::copy(c->q.begin(), c->q.end(), out);
+    return OSB_OK;
+}
+
+// GLF wave speeds of the last right-hand side: out[3d..3d+2] = max |u_d|, |u_d + a|, |u_d - a|
+int osb_oracle_glf(const osb_ctx* c, double* out) {
+    if (!c || !out) return OSB_E_INVALID;
+    for (int d = 0; d < 3; ++d)
+        for (int r = 0; r < 3; ++r) out[3 * d + r] = c->glf[d][r];
     return OSB_OK;
 }
 

@@ -31,6 +31,14 @@ def _ensemble():
     probs.append(P.sod(n=96, steps=5, boundaries="dirichlet"))
     probs.append(P.sod(n=80, steps=5, boundaries="linear"))
     probs.append(P.riemann_2d(n=36, steps=5, boundaries="mixed"))
+    kg = P.kelvin_helmholtz(n=40, steps=5)
+    kg.desc.flux_type = nat.FLUX_GLF                  # GLF next to LLF items of the same group
+    kg.name += "_glf"
+    probs.append(kg)
+    kgm = P.kelvin_helmholtz(n=44, steps=5, precision=nat.PREC_MIXED)
+    kgm.desc.flux_type = nat.FLUX_GLF
+    kgm.name += "_glf_mixed"
+    probs.append(kgm)
     r = P.riemann_2d(n=40, steps=5)
     r.desc.cfl, r.desc.dt = 0.4, 0.0                  # device CFL inside the batch
     probs.append(r)
@@ -96,12 +104,16 @@ def test_batch_fp64_matches_oracle(hip, oracle):
 def test_mixed_precision_error_vs_fp64_oracle(hip, oracle):
     """FP32 flux / derivative evaluation over the FP64 state: relative L2
     against the FP64 oracle, per conservative variable: KH 128^2 x 100 steps,
-    TENO5 + viscous TGV 24^3 x 20 (CFL), Sod 1024 x 1000 and the central
-    viscous TGV 32^3 x 50 (the stated bar is 1e-5)."""
+    TENO5 + viscous TGV 24^3 x 20 (CFL), Sod 1024 x 1000, the central
+    viscous TGV 32^3 x 50 and KH 128^2 with GLF splitting (the stated bar is 1e-5)."""
     cases = [(P.kelvin_helmholtz(n=128, steps=100, precision=nat.PREC_MIXED), 100),
              (P.tgv(n=24, steps=20, scheme="teno", minf=1.25, cfl=0.5, dt=0.0, precision=nat.PREC_MIXED), 20),
              (P.sod(n=1024, steps=1000, precision=nat.PREC_MIXED), 1000),
              (P.tgv(n=32, steps=50, precision=nat.PREC_MIXED), 50)]
+    kg = P.kelvin_helmholtz(n=128, steps=100, precision=nat.PREC_MIXED)
+    kg.desc.flux_type = nat.FLUX_GLF
+    kg.name += "_glf"
+    cases.append((kg, 100))
     for pr, steps in cases:
         _announce(f"mixed {pr.name} {steps} steps")
         q, _ = run(hip, pr, steps=steps)

@@ -118,6 +118,33 @@ def test_riemann_mixed_boundaries(hip, oracle):
     assert np.array_equal(a, b)
 
 
+# --- GLF flux splitting (shock_capturing.py:416-438, :838-857) --------------------
+def _glf(pr):
+    pr.desc.flux_type = nat.FLUX_GLF
+    pr.name += "_glf"
+    return pr
+
+
+def test_glf_sod_teno_dirichlet(hip, oracle):
+    """The shock-capturing branch of verify_sod_shock_tube.py: LFTeno(5, GLF), Dirichlet ends."""
+    pr = _glf(P.sod(boundaries="dirichlet"))
+    pr.desc.conv_scheme = nat.CONV_TENO
+    a, b = _compare(hip, oracle, pr)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("maker", [
+    lambda: P.kelvin_helmholtz(n=64, steps=40),
+    lambda: P.riemann_2d(n=48, steps=40, boundaries="mixed"),
+    lambda: P.tgv(n=16, steps=4, scheme="teno", minf=1.25),            # fused face fluxes + viscous
+])
+def test_glf_parity(hip, oracle, maker):
+    pr = _glf(maker())
+    _residual_compare(hip, oracle, pr)
+    a, b = _compare(hip, oracle, pr)
+    assert np.array_equal(a, b)
+
+
 def test_3d_shock_extrapolation_dirichlet(hip, oracle):
     """3-D inviscid WENO-Z: extrapolation in x0, Dirichlet/linear in x1, periodic
     x2 -- the BC kernels over both transverse halo ranges."""

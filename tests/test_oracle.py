@@ -318,6 +318,97 @@ def test_bc_dirichlet_and_extrapolation_rejected_with_central(oracle):
 
 
 # --------------------------------------------------------------------------------
+# a3/a10: GLF flux splitting (shock_capturing.py:416-438, :838-857)
+# --------------------------------------------------------------------------------
+def _uniform_wave_state(ndim, n, rng):
+    """rho in {1/2, 1, 2, 4}, fixed velocity, p/rho fixed: u and a come out bit-identical
+    at every point (power-of-two scalings round alike), so each |eigenvalue| is one
+    value over the grid."""
+    u = [0.5, -0.25, 0.125][:ndim]
+    shape = (1,) * (3 - ndim) + (n,) * ndim
+    rho = rng.choice([0.5, 1.0, 2.0, 4.0], size=shape)
+    q = np.empty((ndim + 2,) + shape)
+    q[0] = rho
+    for d in range(ndim):
+        q[1 + d] = rho * u[d]
+    q[-1] = rho * (0.5 * sum(x * x for x in u) + 2.5)
+    return q
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+@pytest.mark.parametrize("scheme", [nat.CONV_WENO_Z, nat.CONV_TENO])
+def test_glf_equals_llf_when_wave_speeds_are_uniform(oracle, ndim, scheme):
+    """With every |u_d|, |u_d +- a| the same at all points, the grid maxima equal
+    the two-point maxima: GLF and LLF residuals agree bit for bit.  A GLF speed
+    taken for the wrong characteristic or direction (u_d differ per direction,
+    |u - a| != |u + a|) or scaled by shock_filter_control breaks it."""
+    n = 12 if ndim < 3 else 8
+    q = _uniform_wave_state(ndim, n, np.random.default_rng(11 + ndim))
+    res = []
+    for ft in (nat.FLUX_LLF, nat.FLUX_GLF):
+        desc = nat.ProblemDesc(ndim=ndim, n=[n] * ndim, delta=[0.1, 0.2, 0.3][:ndim], conv_scheme=scheme,
+                               bc=[[nat.BC_PERIODIC] * 2] * ndim, dt=1e-3, flux_type=ft)
+        with oracle.context(desc) as c:
+            c.set_state(q.reshape(desc.shape))
+            res.append(c.residual())
+    assert np.abs(res[0]).max() > 1e-3          # the density jumps give a non-trivial residual
+    assert np.array_equal(res[0], res[1])
+
+
+def test_glf_wave_speeds_are_interior_maxima(oracle, hooks):
+    """The reductions run over the grid [0, N) only (EV_kernel.set_grid_range,
+    weno.py:517-519): linear extrapolation of a steep boundary layer puts faster
+    states in the halos, and they do not enter the maxima."""
+    L = oracle.lib
+    L.osb_oracle_glf.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    pr = P.riemann_2d(n=24, steps=1)
+    pr.desc.flux_type = nat.FLUX_GLF
+    pr.desc.bc = [[nat.BC_EXTRAPOLATION1, nat.BC_ZERO_GRADIENT], [nat.BC_ZERO_GRADIENT, nat.BC_ZERO_GRADIENT]]
+    q0 = pr.q0.copy()
+    r, g = q0[0, 0, :, 0], pr.desc.gama               # u jumps by 3 at the x0 = 0 face, same p
+    u0, v0 = q0[1, 0, :, 0] / r, q0[2, 0, :, 0] / r
+    p0 = (g - 1.0) * (q0[3, 0, :, 0] - 0.5 * r * (u0 ** 2 + v0 ** 2))
+    q0[1, 0, :, 0] = r * (u0 + 3.0)
+    q0[3, 0, :, 0] = p0 / (g - 1.0) + 0.5 * r * ((u0 + 3.0) ** 2 + v0 ** 2)
+    out = np.zeros(9)
+    with oracle.context(pr.desc) as c:
+        c.set_state(q0)
+        c.residual()
+        L.osb_oracle_glf(c.h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        Q, H = _padded(hooks, c)
+    n = pr.desc.n[0]
+    g = pr.desc.gama
+    def speeds(q):
+        u = [q[1] / q[0], q[2] / q[0]]
+        p = (g - 1.0) * (q[3] - 0.5 * q[0] * (u[0] ** 2 + u[1] ** 2))
+        return u, np.sqrt(g * p / q[0])
+    u, a = speeds(Q[:, 0, H:H + n, H:H + n])
+    for d in range(2):
+        want = [np.abs(u[d]).max(), np.abs(u[d] + a).max(), np.abs(u[d] - a).max()]
+        assert np.allclose(out[3 * d:3 * d + 3], want, rtol=1e-14, atol=0), (d, out, want)
+    uh = Q[1, 0, H:H + n, H - 4:H] / Q[0, 0, H:H + n, H - 4:H]   # the x0 < 0 halos the BC fills (WENO5: 4)
+    assert np.abs(uh).max() > 3.0 * out[0]
+
+
+@pytest.mark.parametrize("scheme", [nat.CONV_WENO_Z, nat.CONV_TENO])
+def test_sod_glf_against_exact_solution(oracle, scheme):
+    """LFTeno(order=5, flux_type='GLF') with Dirichlet ends, the shock-capturing
+    branch of tests/verification_apps/sod_shock_tube/verify_sod_shock_tube.py,
+    at the Sod app's N and dt: exact density at t = 0.2."""
+    pr = P.sod(boundaries="dirichlet")
+    pr.desc.conv_scheme = scheme
+    pr.desc.flux_type = nat.FLUX_GLF
+    q, st = run(oracle, pr)
+    ref = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "sod_exact_t0.2.txt"))
+    x = pr.desc.delta[0] * np.arange(pr.desc.n[0])
+    l1 = np.mean(np.abs(np.interp(ref[:, 0], x, q[0, 0, 0]) - ref[:, 1]))
+    assert l1 < 1e-3, l1
+    llf = P.sod(boundaries="dirichlet")
+    llf.desc.conv_scheme = scheme
+    assert not np.array_equal(q, run(oracle, llf)[0])     # GLF is a different splitting
+
+
+# --------------------------------------------------------------------------------
 # a4/a6: free-stream preservation, conservation, shift invariance, derivative order
 # --------------------------------------------------------------------------------
 @pytest.mark.parametrize("maker", [

@@ -100,7 +100,6 @@ def test_kh_and_tg_sym_plans():
 @pytest.mark.parametrize("bad,match", [
     (lambda: [LFWeno(5, formulation="JS"), RungeKuttaLS(3)], "formulation"),
     (lambda: [LFWeno(7, formulation="Z"), RungeKuttaLS(3)], "order"),
-    (lambda: [LFTeno(5, flux_type="GLF"), RungeKuttaLS(3)], "LLF"),
     (lambda: [LFWeno(5, formulation="Z")], "temporal"),
 ])
 def test_unsupported_configurations_raise(bad, match):
@@ -134,6 +133,18 @@ def test_dirichlet_plan_from_script_equations(oracle):
     want.steps = 20
     q_ref, _ = run(oracle, want)
     assert np.array_equal(q, q_ref)
+
+
+def test_glf_plan():
+    """LFTeno(order=5, averaging=SimpleAverage, flux_type='GLF'), the shock-capturing
+    branch of verify_sod_shock_tube.py:112-116."""
+    teno = LFTeno(order=5, averaging=SimpleAverage([0, 1]), flux_type="GLF")
+    alg = _block(1, [teno, RungeKuttaLS(3)], [DirichletBC(0, 0, _SOD_LEFT), DirichletBC(0, 1, _SOD_RIGHT)])
+    got = plan_problem(alg, _SOD_PARAMS)
+    assert got.flux_type == nat.FLUX_GLF and got.conv_scheme == nat.CONV_TENO
+    weno = LFWeno(5, formulation="Z", averaging=SimpleAverage([0, 1]))
+    alg = _block(1, [weno, RungeKuttaLS(3)], [ZeroGradientOutletBC(0, 0), ZeroGradientOutletBC(0, 1)])
+    assert plan_problem(alg, _SOD_PARAMS).flux_type == nat.FLUX_LLF
 
 
 def test_extrapolation_plan():
