@@ -21,6 +21,7 @@ from __future__ import annotations
 import ast
 import math
 import operator
+import os
 import re
 from typing import Callable, Dict, Mapping, Optional, Union
 
@@ -309,6 +310,25 @@ def plan_problem(alg_or_block, parameters: Mapping, physics=None, precision: str
     return nat.ProblemDesc(**desc)
 
 
+def _io_kwargs(io) -> Dict:
+    """iohdf5 keyword choices: the mirror's ``kwargs`` (lower-cased like
+    io_hdf5.py:33-38) or the reference object's."""
+    kw = getattr(io, "kwargs", None) or {}
+    return {str(k).lower(): (v.lower() if isinstance(v, str) else v) for k, v in kw.items()}
+
+
+def _io_filename(io) -> str:
+    """Output file of a writer outside the loop (io_hdf5.py:190-206)."""
+    name = str(_io_kwargs(io).get("name") or "opensbli_output")
+    return name if name.endswith(".h5") else f"{name}.h5"
+
+
+def _save_every(io) -> int:
+    v = getattr(io, "save_every", None)
+    v = getattr(v, "_value", v)           # the reference keeps a ConstantObject
+    return int(v) if v else 0
+
+
 # --------------------------------------------------------------------------
 # the target
 # --------------------------------------------------------------------------
@@ -324,12 +344,24 @@ class HIPTarget:
     C ABI can be passed (the tests pass the CPU oracle)."""
 
     def __init__(self, alg, precision: str = "fp64", parameters: Optional[Mapping] = None, physics=None,
-                 cfl: Optional[float] = None, library: Optional[nat.Library] = None):
+                 cfl: Optional[float] = None, library: Optional[nat.Library] = None, output_dir: str = "."):
         self.parameters = evaluate_parameters(parameters or {})
         self.desc = plan_problem(alg, parameters or {}, physics=physics, precision=precision, cfl=cfl)
         self.niter = int(self.parameters.get("niter", 0))
         self.lib = library if library is not None else nat.load_hip()
         self.ctx = self.lib.context(self.desc)
+        self.output_dir = output_dir
+        block = alg.blocks[0] if hasattr(alg, "blocks") else alg
+        self.block_number = int(getattr(block, "blocknumber", 0) or 0)
+        ios = list(getattr(block, "InputOutput", []) or [])
+        self.writers = [io for io in ios if str(_io_kwargs(io).get("iotype", "write")).lower() == "write"]
+        self.restart_file = None
+        for io in ios:
+            kw = _io_kwargs(io)
+            if str(kw.get("iotype", "")).lower() == "read" and str(kw.get("restart_simulation", "")).lower() in (
+                    "true", "1", "yes"):
+                self.restart_file = os.path.join(output_dir, "restart.h5")      # opsc.py:962-977
+        self.start_iter = 0
 
     # grid coordinates of the interior points, x_d = Delta_d * i_d
     def coordinates(self):
@@ -345,9 +377,47 @@ class HIPTarget:
         self.ctx.set_state(np.asarray(q0, dtype=np.float64).reshape(self.desc.shape))
         return self
 
+    def restart(self, path: Optional[str] = None):
+        """State, start iteration and simulation time from an HDF5 restart file."""
+        from ... import io_hdf5
+        self.start_iter, _ = io_hdf5.restart(self.ctx, path or self.restart_file or "restart.h5", self.block_number)
+        return self
+
+    def write_hdf5(self, path: str, iteration: Optional[int] = None):
+        """The block's conservative variables and the simulation constants in
+        the OPS HDF5 layout (ops_fetch_dat_hdf5_file + write_constants)."""
+        from ... import io_hdf5
+        consts = {k: v for k, v in self.parameters.items() if isinstance(v, (int, float)) and k not in ("iter",)}
+        ints = {k for k, v in consts.items() if isinstance(v, int) and not isinstance(v, bool)}
+        io_hdf5.write_state(path, self.ctx, consts, ints, iteration=iteration, block_number=self.block_number)
+
     def run(self, niter: Optional[int] = None):
+        """niter steps; the block's iohdf5 writers run where the generated time
+        loop places them (algorithm.py:476-485): with save_every after loop
+        index it when (it + 1) % save_every == 0 or it == 0, as
+        opensbli_output_%06d.h5 % (it + 1), and at the end as
+        opensbli_output.h5 (or the writer's name)."""
         steps = self.niter if niter is None else int(niter)
-        self.ctx.step(steps)
+        if self.restart_file is not None and self.ctx.stats().iter == 0:
+            self.restart()
+        for w in self.writers:                       # position='init': before the time loop only
+            if _io_kwargs(w).get("position") == "init" and self.ctx.stats().iter == 0:
+                self.write_hdf5(os.path.join(self.output_dir, _io_filename(w)))
+        every = sorted({_save_every(w) for w in self.writers if _save_every(w)})
+        it = int(self.ctx.stats().iter)              # loop index of the next iteration
+        stop = it + steps
+        if not every:
+            self.ctx.step(steps)
+        while every and it < stop:
+            # next loop index j >= it that writes: (j + 1) % e == 0, or j == 0
+            j = min([stop - 1, 0 if it == 0 else stop] + [-(-(it + 1) // e) * e - 1 for e in every])
+            self.ctx.step(j - it + 1)
+            if j == 0 or any((j + 1) % e == 0 for e in every):
+                self.write_hdf5(os.path.join(self.output_dir, "opensbli_output_%06d.h5" % (j + 1)), iteration=j)
+            it = j + 1
+        for w in self.writers:
+            if _io_kwargs(w).get("position") != "init":
+                self.write_hdf5(os.path.join(self.output_dir, _io_filename(w)))
         return self
 
     def state(self) -> np.ndarray:

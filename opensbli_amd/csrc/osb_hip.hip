@@ -646,12 +646,17 @@ int osb_set_stream(osb_ctx* c, void* s) {
 
 int osb_set_state(osb_ctx* c, const double* host, int layout) {
     if (!c || !host) return OSB_E_INVALID;
-    if (layout != OSB_LAYOUT_INTERIOR) return set_err(c, OSB_E_UNSUPPORTED, "only the interior layout is accepted");
+    if (layout != OSB_LAYOUT_INTERIOR && layout != OSB_LAYOUT_PADDED) return set_err(c, OSB_E_INVALID, "unknown layout");
     const long npts = interior_points(c->g);
-    HIP_TRY(c, hipMemcpyAsync(c->staging, host, sizeof(double) * npts * c->nv, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, hipMemsetAsync(c->qa, 0, sizeof(double) * c->nv * c->g.Ptot, c->stream));
     HIP_TRY(c, hipMemsetAsync(c->du, 0, sizeof(double) * c->nv * c->g.Ptot, c->stream));
-    k_scatter<<<nblocks(npts * c->nv), BLOCK, 0, c->stream>>>(c->staging, c->qa, c->g, c->nv);
+    if (layout == OSB_LAYOUT_PADDED) {
+        // the whole padded array; the boundary conditions below refill the halos
+        HIP_TRY(c, hipMemcpyAsync(c->qa, host, sizeof(double) * c->nv * c->g.Ptot, hipMemcpyHostToDevice, c->stream));
+    } else {
+        HIP_TRY(c, hipMemcpyAsync(c->staging, host, sizeof(double) * npts * c->nv, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipMemsetAsync(c->qa, 0, sizeof(double) * c->nv * c->g.Ptot, c->stream));
+        k_scatter<<<nblocks(npts * c->nv), BLOCK, 0, c->stream>>>(c->staging, c->qa, c->g, c->nv);
+    }
     c->q_cur = c->qa;
     c->q_prev = c->qb;
     c->have_prev = false;
@@ -662,7 +667,12 @@ int osb_set_state(osb_ctx* c, const double* host, int layout) {
 
 int osb_get_state(osb_ctx* c, double* host, int layout) {
     if (!c || !host) return OSB_E_INVALID;
-    if (layout != OSB_LAYOUT_INTERIOR) return set_err(c, OSB_E_UNSUPPORTED, "only the interior layout is returned");
+    if (layout != OSB_LAYOUT_INTERIOR && layout != OSB_LAYOUT_PADDED) return set_err(c, OSB_E_INVALID, "unknown layout");
+    if (layout == OSB_LAYOUT_PADDED) {
+        HIP_TRY(c, hipMemcpyAsync(host, c->q_cur, sizeof(double) * c->nv * c->g.Ptot, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        return check_launch(c);
+    }
     const long npts = interior_points(c->g);
     k_gather<<<nblocks(npts * c->nv), BLOCK, 0, c->stream>>>(c->q_cur, c->staging, c->g, c->nv);
     HIP_TRY(c, hipMemcpyAsync(host, c->staging, sizeof(double) * npts * c->nv, hipMemcpyDeviceToHost, c->stream));

@@ -352,15 +352,25 @@ class Context:
             raise OSBError(rc, msg.decode() if msg else "")
 
     # state ---------------------------------------------------------------
-    def set_state(self, q: np.ndarray):
-        q = np.ascontiguousarray(q, dtype=np.float64)
-        if q.size != self.desc.nv * self.desc.npoints:
-            raise ValueError(f"state has {q.size} values, expected {self.desc.nv * self.desc.npoints}")
-        self._check(self.lib.lib.osb_set_state(self.h, q.reshape(-1), LAYOUT_INTERIOR))
+    def padded_shape(self):
+        """(nv, P2, P1, P0) of the padded layout: n + 2h in active directions,
+        h = the library's stored halo depth (``stats().halo``)."""
+        h = int(self.stats().halo)
+        n = list(self.desc.n) + [1] * (3 - len(self.desc.n))
+        P = [n[d] + 2 * h if d < self.desc.ndim else 1 for d in range(3)]
+        return (self.desc.nv, P[2], P[1], P[0])
 
-    def get_state(self) -> np.ndarray:
-        out = np.empty(self.desc.shape, dtype=np.float64)
-        self._check(self.lib.lib.osb_get_state(self.h, out.reshape(-1), LAYOUT_INTERIOR))
+    def set_state(self, q: np.ndarray, layout: int = LAYOUT_INTERIOR):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        want = int(np.prod(self.padded_shape())) if layout == LAYOUT_PADDED else self.desc.nv * self.desc.npoints
+        if q.size != want:
+            raise ValueError(f"state has {q.size} values, expected {want}")
+        self._check(self.lib.lib.osb_set_state(self.h, q.reshape(-1), int(layout)))
+
+    def get_state(self, layout: int = LAYOUT_INTERIOR) -> np.ndarray:
+        shape = self.padded_shape() if layout == LAYOUT_PADDED else self.desc.shape
+        out = np.empty(shape, dtype=np.float64)
+        self._check(self.lib.lib.osb_get_state(self.h, out.reshape(-1), int(layout)))
         return out
 
     def set_time(self, t: float, it: int):

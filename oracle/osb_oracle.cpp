@@ -982,9 +982,14 @@ int osb_ctx_destroy(osb_ctx* ctx) { delete ctx; return OSB_OK; }
 
 int osb_set_state(osb_ctx* c, const double* host, int layout) {
     if (!c || !host) return OSB_E_INVALID;
-    if (layout != OSB_LAYOUT_INTERIOR) return fail(c, OSB_E_UNSUPPORTED, "oracle accepts the interior layout only");
-    std::fill(c->q.begin(), c->q.end(), 0.0);
+    if (layout != OSB_LAYOUT_INTERIOR && layout != OSB_LAYOUT_PADDED) return fail(c, OSB_E_INVALID, "unknown layout");
     std::fill(c->rkold.begin(), c->rkold.end(), 0.0);
+    if (layout == OSB_LAYOUT_PADDED) {            // halos are refilled by the boundary conditions
+        std::copy(host, host + c->q.size(), c->q.begin());
+        apply_bcs(c);
+        return OSB_OK;
+    }
+    std::fill(c->q.begin(), c->q.end(), 0.0);
     const size_t npts = (size_t)c->n[0] * c->n[1] * c->n[2];
     for (int v = 0; v < c->nv; ++v)
         for (int k = 0; k < c->n[2]; ++k)
@@ -997,7 +1002,11 @@ int osb_set_state(osb_ctx* c, const double* host, int layout) {
 
 int osb_get_state(osb_ctx* c, double* host, int layout) {
     if (!c || !host) return OSB_E_INVALID;
-    if (layout != OSB_LAYOUT_INTERIOR) return fail(c, OSB_E_UNSUPPORTED, "oracle returns the interior layout only");
+    if (layout != OSB_LAYOUT_INTERIOR && layout != OSB_LAYOUT_PADDED) return fail(c, OSB_E_INVALID, "unknown layout");
+    if (layout == OSB_LAYOUT_PADDED) {
+        std::copy(c->q.begin(), c->q.end(), host);
+        return OSB_OK;
+    }
     const size_t npts = (size_t)c->n[0] * c->n[1] * c->n[2];
     for (int v = 0; v < c->nv; ++v)
         for (int k = 0; k < c->n[2]; ++k)

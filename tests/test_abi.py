@@ -16,8 +16,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "osb.h")
 
 
-def declared_functions():
-    text = open(HEADER).read()
+def declared_functions(header=HEADER):
+    text = open(header).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(osb_[a-z0-9_]+)\s*\(", text)))
 
@@ -50,6 +50,18 @@ def test_oracle_exports_the_single_block_api(oracle):
     single = [f for f in declared_functions() if not f.startswith(("osb_batch_", "osb_profile_", "osb_pool_"))]
     missing = [f for f in single if f not in exported(oracle.path)]
     assert not missing, missing
+
+
+def test_io_library_exports_every_declared_function():
+    """libosb_io.so (HDF5 output/restart) exports what include/osb_io.h declares."""
+    from opensbli_amd import io_hdf5
+    if not os.path.exists(io_hdf5.IO_LIBRARY):
+        pytest.skip("libosb_io.so not built (no HDF5 C library)")
+    fns = declared_functions(os.path.join(ROOT, "include", "osb_io.h"))
+    assert "osb_io_write" in fns and "osb_io_read_dat" in fns
+    missing = [f for f in fns if f not in exported(io_hdf5.IO_LIBRARY)]
+    assert not missing, missing
+    io_hdf5.load_io()
 
 
 def test_hip_library_loads_without_a_gpu(hip_so):

@@ -218,3 +218,35 @@ def test_tgv_teno6_reference_validation(hip):
     assert f"{st.time:.5e}" == "8.46250e-01"
     for v, r in zip(d, ref):
         assert f"{v:.5e}" == f"{r:.5e}", (d, ref)
+
+
+# --- HDF5 output and restart (libosb_io.so; opensbli/core/io_hdf5.py:165-250) ------
+def test_hdf5_restart_on_the_gpu(hip, oracle, tmp_path):
+    """The GPU state written in the OPS layout, read back by a restarted GPU
+    context, continues bit for bit; its interior equals the oracle's file."""
+    from opensbli_amd import io_hdf5 as io
+    try:
+        io.load_io()
+    except FileNotFoundError as e:
+        pytest.skip(str(e))
+    for pr in (P.kelvin_helmholtz(n=64, steps=20), P.tgv(n=24, steps=6)):
+        _announce(f"hdf5 restart {pr.name}")
+        k = pr.steps // 2
+        straight, st = run(hip, pr, pr.steps)
+        paths = {}
+        for name, lib in (("hip", hip), ("oracle", oracle)):
+            paths[name] = str(tmp_path / f"{pr.name}_{name}.h5")
+            with lib.context(pr.desc) as c:
+                c.set_state(pr.q0)
+                c.step(k)
+                io.write_state(paths[name], c)
+        qa, ia, ta = io.read_state(paths["hip"], pr.desc)
+        qb, ib, tb = io.read_state(paths["oracle"], pr.desc)
+        assert np.array_equal(qa, qb) and ia == ib == k and ta == tb
+        with hip.context(pr.desc) as c:
+            io.restart(c, paths["hip"])
+            c.step(pr.steps - k)
+            q = c.get_state()
+            s2 = c.stats()
+        assert np.array_equal(q, straight)
+        assert s2.iter == st.iter and math.isclose(s2.time, st.time, rel_tol=1e-15)
