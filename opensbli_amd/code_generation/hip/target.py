@@ -329,6 +329,52 @@ def _save_every(io) -> int:
     return int(v) if v else 0
 
 
+def initial_fields(block, parameters: Mapping[str, float], desc: nat.ProblemDesc) -> np.ndarray:
+    """Evaluate the block's GridBasedInitialisation equations in order over the
+    grid points [0, N) on the host (opensbli/initialisation/gridbasedinit.py:16-86)
+    and return the conservative state (nv, n2, n1, n0).  Grid indexes enter as
+    idx0..idx2 (the reference's idx[d]); simulation parameters such as gama and
+    Delta0block0 are in scope.  The halo points the reference also fills are
+    rewritten by the boundary conditions before the first stage, so only the
+    interior is evaluated."""
+    import sympy as sp
+    eqs = []
+    for ec in getattr(block, "list_of_equation_classes", []):
+        if "GridBasedInitialisation" in _kinds(ec):
+            eqs += list(ec.equations)
+    if not eqs:
+        raise ValueError("the block has no GridBasedInitialisation equations")
+    from ... import surface
+    local = {"block": block, "DataObject": surface.DataObject, "GridVariable": surface.GridVariable, "Eq": sp.Eq}
+    n = list(desc.n) + [1] * (3 - len(desc.n))
+    shape = (n[2], n[1], n[0])
+    grids = np.meshgrid(np.arange(n[2]), np.arange(n[1]), np.arange(n[0]), indexing="ij")
+    env: Dict[str, object] = {k: v for k, v in parameters.items() if isinstance(v, (int, float))}
+    for d in range(desc.ndim):
+        env[f"idx{d}"] = grids[2 - d].astype(np.float64)
+    plain = lambda e: re.sub(r"\b([A-Za-z_]\w*?)_B\d+\[[^\]]*\]", r"\1", str(e))
+    for eq in eqs:
+        if isinstance(eq, str):
+            eq = sp.parse_expr(eq, local_dict=local)
+        rhs = eq.rhs
+        # the reference's grid index idx[d] (an Indexed) -> idx<d>
+        rhs = rhs.xreplace({a: sp.Symbol(f"idx{int(a.indices[0])}") for a in rhs.atoms(sp.Indexed)
+                            if str(a.base.label).startswith("idx")})
+        syms = sorted(rhs.free_symbols, key=str)
+        names = [plain(s_) for s_ in syms]
+        missing = [m for m in names if m not in env]
+        if missing:
+            raise NotImplementedError(f"initial condition {eq}: unknown names {missing}")
+        f = sp.lambdify(syms, rhs, modules="numpy")
+        env[plain(eq.lhs)] = f(*[env[m] for m in names])
+    q = np.empty(desc.shape, dtype=np.float64)
+    for v, nm in enumerate(_conservative_names(desc.ndim)):
+        if nm not in env:
+            raise NotImplementedError(f"the initial equations do not set {nm} (conservative variables are required)")
+        q[v] = np.broadcast_to(np.asarray(env[nm], dtype=np.float64), shape)
+    return q
+
+
 # --------------------------------------------------------------------------
 # the target
 # --------------------------------------------------------------------------
@@ -352,6 +398,7 @@ class HIPTarget:
         self.ctx = self.lib.context(self.desc)
         self.output_dir = output_dir
         block = alg.blocks[0] if hasattr(alg, "blocks") else alg
+        self.block = block
         self.block_number = int(getattr(block, "blocknumber", 0) or 0)
         ios = list(getattr(block, "InputOutput", []) or [])
         self.writers = [io for io in ios if str(_io_kwargs(io).get("iotype", "write")).lower() == "write"]
@@ -371,8 +418,12 @@ class HIPTarget:
         z, y, x = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
         return x, y, z
 
-    def initialise(self, q0: Union[np.ndarray, Callable]):
-        if callable(q0):
+    def initialise(self, q0: Union[np.ndarray, Callable, None] = None):
+        """Initial state: an array (nv, n2, n1, n0), a function of the grid
+        coordinates, or (None) the block's GridBasedInitialisation equations."""
+        if q0 is None:
+            q0 = initial_fields(self.block, self.parameters, self.desc)
+        elif callable(q0):
             q0 = np.asarray(q0(*self.coordinates()), dtype=np.float64)
         self.ctx.set_state(np.asarray(q0, dtype=np.float64).reshape(self.desc.shape))
         return self

@@ -283,8 +283,31 @@ class ConstituentRelations(_EquationClass):
 
 
 class GridBasedInitialisation(_EquationClass):
-    """opensbli/initialisation/gridbasedinit.py:16 (the HIP back end takes
-    the initial field as an array or a function of the grid coordinates)."""
+    """opensbli/initialisation/gridbasedinit.py:16.  The equations as a
+    problem script writes them (SymPy equations from ``parse_expr`` with
+    ``block``, ``DataObject`` and ``GridVariable`` in the local dict, or the
+    equation strings themselves); the HIP back end evaluates them in order
+    over the grid on the host (``HIPTarget.initialise()`` without an array)."""
+
+
+def _symbol(name):
+    try:
+        import sympy
+    except ImportError as e:                          # pragma: no cover - SymPy is in the image
+        raise ImportError("equation objects need SymPy") from e
+    return sympy.Symbol(str(name))
+
+
+def DataObject(name):
+    """A named grid array in equations (opensbli/core/opensbliobjects.py);
+    a SymPy symbol of that name here."""
+    return _symbol(name)
+
+
+def GridVariable(name):
+    """A point-local temporary in equations (opensbli/core/grid.py:214); a
+    SymPy symbol of that name here."""
+    return _symbol(name)
 
 
 class iohdf5:
@@ -333,6 +356,16 @@ class SimulationBlock:
         self.discretisation_schemes: Dict[str, Scheme] = {}
         self.list_of_equation_classes: List = []
         self.InputOutput: List = []
+
+    @property
+    def deltas(self):
+        """Grid spacings Delta{d}block{b} (opensbli/core/grid.py:155)."""
+        return [_symbol(f"Delta{d}block{self.blocknumber}") for d in range(self.ndim)]
+
+    @property
+    def grid_indexes(self):
+        """Grid index of each direction (opensbli/core/grid.py:176), idx0..idx2."""
+        return [_symbol(f"idx{d}") for d in range(self.ndim)]
 
     def set_block_boundaries(self, bclist):
         flat = _flatten(bclist)

@@ -13,7 +13,7 @@ import pytest
 
 from helpers import run
 from opensbli_amd import native as nat, problems as P
-from opensbli_amd.code_generation.hip import HIPTarget, evaluate_parameters, plan_problem
+from opensbli_amd.code_generation.hip import HIPTarget, evaluate_parameters, initial_fields, plan_problem
 from opensbli_amd.surface import (
     Central, DirichletBC, ExtrapolationBC, InviscidWallBC, LFTeno, LFWeno, NS_Split, PeriodicBC, RoeAverage, RungeKutta,
     RungeKuttaLS, SimpleAverage, SimulationBlock, StoreSome, SymmetryBC, TraditionalAlgorithmRK,
@@ -220,3 +220,78 @@ def test_target_initialises_from_a_function(oracle):
         q = t.state()
     want, _ = run(oracle, pr)
     assert np.array_equal(q, want)
+
+
+# --------------------------------------------------------------------------------
+# GridBasedInitialisation equations as the problem scripts write them
+# --------------------------------------------------------------------------------
+def _script_block(ndim, eq_strings, bcs, schemes, parse=True):
+    from sympy.parsing.sympy_parser import parse_expr
+    from opensbli_amd.surface import DataObject, GridBasedInitialisation, GridVariable
+    block = SimulationBlock(ndim, block_number=0)
+    local_dict = {"block": block, "GridVariable": GridVariable, "DataObject": DataObject}
+    initial = GridBasedInitialisation()
+    initial.add_equations([parse_expr(e, local_dict=local_dict) for e in eq_strings] if parse else eq_strings)
+    block.set_equations([initial])
+    block.set_block_boundaries(bcs)
+    block.set_discretisation_schemes({f"s{i}": s for i, s in enumerate(schemes)})
+    return TraditionalAlgorithmRK(block)
+
+
+# apps/Sod_shock_tube/Sod_shock_tube.py:73-81
+_SOD_INIT = ["Eq(DataObject(x0), block.deltas[0]*block.grid_indexes[0])",
+             "Eq(GridVariable(d), Piecewise((1.0, DataObject(x0) < 0.5), (0.125, True)))",
+             "Eq(GridVariable(p), Piecewise((1.0, DataObject(x0) < 0.5), (0.10, True)))",
+             "Eq(GridVariable(u0), Piecewise((0.0, DataObject(x0) < 0.5),(0, True)))",
+             "Eq(DataObject(rho), d)", "Eq(DataObject(rhou0), d*u0)",
+             "Eq(DataObject(rhoE), p/(gama-1.0) + 0.5* d *(u0**2.0))"]
+# apps/taylor_green_vortex/taylor_green_vortex.py:84-100
+_TGV_INIT = ["Eq(GridVariable(x0), block.deltas[0]*block.grid_indexes[0])",
+             "Eq(GridVariable(x1), block.deltas[1]*block.grid_indexes[1])",
+             "Eq(GridVariable(x2), block.deltas[2]*block.grid_indexes[2])",
+             "Eq(GridVariable(u0),sin(x0)*cos(x1)*cos(x2))", "Eq(GridVariable(u1),-cos(x0)*sin(x1)*cos(x2))",
+             "Eq(GridVariable(u2), 0.0)",
+             "Eq(GridVariable(p), 1.0/(gama*Minf*Minf)+ (1.0/16.0) * (cos(2.0*x0)+cos(2.0*x1))*(2.0 + cos(2.0*x2)))",
+             "Eq(GridVariable(r), gama*Minf*Minf*p)", "Eq(DataObject(rho), r)", "Eq(DataObject(rhou0), r*u0)",
+             "Eq(DataObject(rhou1), r*u1)", "Eq(DataObject(rhou2), r*u2)",
+             "Eq(DataObject(rhoE), p/(gama-1) + 0.5* r *(u0**2+ u1**2 + u2**2))"]
+
+
+@pytest.mark.parametrize("parse", [True, False])
+def test_sod_script_initial_condition(oracle, parse):
+    """The Sod script's equations (parsed by SymPy as the script does, or given as
+    text) reproduce the shock-tube state exactly, and a run from them equals
+    the prepared problem's run."""
+    alg = _script_block(1, _SOD_INIT, [ZeroGradientOutletBC(0, 0), ZeroGradientOutletBC(0, 1)],
+                        [LFWeno(5, formulation="Z", averaging=SimpleAverage([0, 1])), RungeKuttaLS(3)], parse)
+    params = dict(_SOD_PARAMS, niter="30")
+    t = HIPTarget(alg, parameters=params, library=oracle)
+    want = P.sod(steps=30)
+    assert np.array_equal(initial_fields(alg.blocks[0], t.parameters, t.desc), want.q0)
+    q = t.initialise().run().state()
+    t.close()
+    assert np.array_equal(q, run(oracle, want)[0])
+
+
+def test_tgv_script_initial_condition(oracle):
+    alg = _script_block(3, _TGV_INIT, [PeriodicBC(d, s) for d in range(3) for s in range(2)],
+                        [StoreSome(4, "u0 u1 u2"), RungeKuttaLS(3)])
+    params = {"gama": "1.4", "Minf": "0.1", "Re": "1600.0", "Pr": "0.71", "dt": "0.003385", "niter": "2"}
+    for d in range(3):
+        params[f"block0np{d}"] = "16"
+        params[f"Delta{d}block0"] = f"2*M_PI/block0np{d}"
+    NS = NS_Split("Feiereisen", 3, ["Re", "Pr", "gama", "Minf"], viscosity="constant")
+    t = HIPTarget(alg, parameters=params, physics=NS, library=oracle)
+    q0 = initial_fields(alg.blocks[0], t.parameters, t.desc)
+    want = P.tgv(n=16).q0
+    assert np.max(np.abs(q0 - want) / np.maximum(np.abs(want), 1.0)) < 1e-15
+    t.close()
+
+
+def test_initial_condition_unknown_names_raise(oracle):
+    alg = _script_block(1, ["Eq(DataObject(rho), q_unknown)"], [ZeroGradientOutletBC(0, 0), ZeroGradientOutletBC(0, 1)],
+                        [LFWeno(5, formulation="Z"), RungeKuttaLS(3)], parse=False)
+    t = HIPTarget(alg, parameters=_SOD_PARAMS, library=oracle)
+    with pytest.raises(NotImplementedError, match="q_unknown"):
+        t.initialise()
+    t.close()
