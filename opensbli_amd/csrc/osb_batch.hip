@@ -37,6 +37,7 @@ struct Group {
     int ndim = 0, sch = 0, mixed = 0, time = 0, stages = 0;
     bool any_cfl = false;
     bool any_glf = false;
+    bool fused = false;                    // last direction's faces fused with the update (kb_face_resid)
     std::vector<int> members;
     std::vector<int> offs;                 // BF_COUNT x (n + 1) virtual-block offsets
     BItem* d_items = nullptr;
@@ -103,7 +104,13 @@ long family_work(const osb_ctx* c, int f, bool tiled) {
         case BF_BC0: return bc_work<0>(g);
         case BF_BC1: return g.ndim > 1 ? bc_work<1>(g) : 0;
         case BF_BC2: return g.ndim > 2 ? bc_work<2>(g) : 0;
-        default: return (long)g.n[0] * g.n[1] * g.n[2];     // resid_rk, wavespeed, save_state
+        case BF_RESID:
+            if (c->shock_fused) {                            // fused last-direction tiles
+                const long t = g.ndim == 1 ? fused_tiles<1>(g) : (g.ndim == 2 ? fused_tiles<2>(g) : fused_tiles<3>(g));
+                return t * BLOCK;
+            }
+            return (long)g.n[0] * g.n[1] * g.n[2];
+        default: return (long)g.n[0] * g.n[1] * g.n[2];     // wavespeed, save_state
     }
 }
 
@@ -124,11 +131,12 @@ int build_groups(osb_batch* b) {
         Group* gp = nullptr;
         for (auto& gr : b->groups)
             if (gr.ndim == c->g.ndim && gr.sch == c->sch && gr.mixed == (int)c->mixed && gr.time == c->d.time_scheme &&
-                gr.stages == stages) { gp = &gr; break; }
+                gr.stages == stages && gr.fused == c->shock_fused) { gp = &gr; break; }
         if (!gp) {
             b->groups.emplace_back();
             gp = &b->groups.back();
             gp->ndim = c->g.ndim; gp->sch = c->sch; gp->mixed = c->mixed; gp->time = c->d.time_scheme; gp->stages = stages;
+            gp->fused = c->shock_fused;
         }
         gp->members.push_back(i);
         gp->any_cfl = gp->any_cfl || c->d.cfl > 0;
@@ -225,6 +233,28 @@ void launch_face_dir(hipStream_t st, const Group& gr, int dir, int parity, unsig
     else launch_face_p<NDIM, (NDIM > 2 ? 2 : 0)>(st, gr, parity, grid);
 }
 
+template <int NDIM, int TIME, typename T>
+void launch_face_resid(hipStream_t st, const Group& gr, int parity, int stage) {
+    const unsigned grid = grid_of(gr, BF_RESID);
+    switch (gr.sch) {
+        case SCH_WENOZ:
+            kb_face_resid<NDIM, SCH_WENOZ, T, TIME><<<grid, BLOCK, 0, st>>>(gr.d_items, gr.dof(BF_RESID), gr.n(), parity, stage);
+            break;
+        case SCH_TENO5:
+            kb_face_resid<NDIM, SCH_TENO5, T, TIME><<<grid, BLOCK, 0, st>>>(gr.d_items, gr.dof(BF_RESID), gr.n(), parity, stage);
+            break;
+        default:
+            kb_face_resid<NDIM, SCH_TENO6, T, TIME><<<grid, BLOCK, 0, st>>>(gr.d_items, gr.dof(BF_RESID), gr.n(), parity, stage);
+            break;
+    }
+}
+
+template <int NDIM, int TIME>
+void launch_face_resid_p(hipStream_t st, const Group& gr, int parity, int stage) {
+    if (gr.mixed) launch_face_resid<NDIM, TIME, float>(st, gr, parity, stage);
+    else launch_face_resid<NDIM, TIME, double>(st, gr, parity, stage);
+}
+
 template <int NDIM>
 void launch_bcs(hipStream_t st, const Group& gr, int parity) {
     kb_bc<0><<<grid_of(gr, BF_BC0), BLOCK, 0, st>>>(gr.d_items, gr.dof(BF_BC0), gr.n(), parity);
@@ -248,11 +278,17 @@ void enqueue_group_step_nd(hipStream_t st, const Group& gr, int parity) {
             kb_glf_zero<<<(n + 63) / 64, 64, 0, st>>>(gr.d_items, n);
             kb_glf<NDIM><<<grid_of(gr, BF_WAVE), BLOCK, 0, st>>>(gr.d_items, gr.dof(BF_WAVE), n, parity);
         }
-        for (int d = 0; d < NDIM; ++d) launch_face_dir<NDIM>(st, gr, d, parity, grid_of(gr, BF_FACE0 + d));
-        if (gr.time == OSB_TIME_RK3)
-            kb_resid_rk<NDIM, 2><<<grid_of(gr, BF_RESID), BLOCK, 0, st>>>(gr.d_items, gr.dof(BF_RESID), n, parity, s);
-        else
-            kb_resid_rk<NDIM, 1><<<grid_of(gr, BF_RESID), BLOCK, 0, st>>>(gr.d_items, gr.dof(BF_RESID), n, parity, s);
+        if (gr.fused) {
+            for (int d = 0; d + 1 < NDIM; ++d) launch_face_dir<NDIM>(st, gr, d, parity, grid_of(gr, BF_FACE0 + d));
+            if (gr.time == OSB_TIME_RK3) launch_face_resid_p<NDIM, 2>(st, gr, parity, s);
+            else launch_face_resid_p<NDIM, 1>(st, gr, parity, s);
+        } else {
+            for (int d = 0; d < NDIM; ++d) launch_face_dir<NDIM>(st, gr, d, parity, grid_of(gr, BF_FACE0 + d));
+            if (gr.time == OSB_TIME_RK3)
+                kb_resid_rk<NDIM, 2><<<grid_of(gr, BF_RESID), BLOCK, 0, st>>>(gr.d_items, gr.dof(BF_RESID), n, parity, s);
+            else
+                kb_resid_rk<NDIM, 1><<<grid_of(gr, BF_RESID), BLOCK, 0, st>>>(gr.d_items, gr.dof(BF_RESID), n, parity, s);
+        }
         launch_bcs<NDIM>(st, gr, parity ^ 1);
         parity ^= 1;
     }
@@ -288,7 +324,7 @@ int autotune(osb_batch* b) {
     B_TRY(b, hipEventCreate(&e0));
     B_TRY(b, hipEventCreate(&e1));
     for (auto& gr : b->groups) {
-        for (int d = 0; d < gr.ndim; ++d) {
+        for (int d = 0; d < gr.ndim - (gr.fused ? 1 : 0); ++d) {     // the fused family changes du: not timed
             const int f = BF_FACE0 + d;
             const int tot = gr.total(f);
             const int cands[5] = {0, 8192, 4096, 2048, 1024};
@@ -561,13 +597,15 @@ int signature_group(osb_pool* p, const osb_ctx* c) {
     const int stages = c->d.time_scheme == OSB_TIME_RKLS3 ? c->rk_stages : 3;
     for (int i = 0; i < (int)p->groups.size(); ++i) {
         const Group& g = p->groups[i].g;
-        if (g.ndim == c->g.ndim && g.sch == c->sch && g.mixed == (int)c->mixed && g.time == c->d.time_scheme && g.stages == stages)
+        if (g.ndim == c->g.ndim && g.sch == c->sch && g.mixed == (int)c->mixed && g.time == c->d.time_scheme && g.stages == stages &&
+            g.fused == c->shock_fused)
             return i;
     }
     p->groups.emplace_back();
     PoolGroup& pg = p->groups.back();
     pg.g.ndim = c->g.ndim; pg.g.sch = c->sch; pg.g.mixed = c->mixed; pg.g.time = c->d.time_scheme; pg.g.stages = stages;
     pg.g.tiled = c->face_tiled;
+    pg.g.fused = c->shock_fused;
     pg.stream = p->streams[p->next_stream++ % p->streams.size()];
     if (hipEventCreateWithFlags(&pg.ev, hipEventDisableTiming) != hipSuccess) pg.ev = nullptr;
     return (int)p->groups.size() - 1;

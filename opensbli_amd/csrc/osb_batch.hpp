@@ -61,6 +61,18 @@ __global__ void __launch_bounds__(BLOCK) kb_face_flux_tile(const BItem* __restri
     }
 }
 
+// last direction's faces fused with the update: one virtual block is one tile
+template <int NDIM, int SCH, typename T, int TIME>
+__global__ void __launch_bounds__(BLOCK) kb_face_resid(const BItem* __restrict__ items, const int* __restrict__ offs, int n,
+                                                       int parity, int stage) {
+    OSB_BATCH_LOOP {
+        const int i = find_item(offs, n, vb);
+        const BItem& it = items[i];
+        face_resid_tile<NDIM, SCH, T, TIME>(vb - offs[i], it.q[parity], it.F, it.q[parity ^ 1], it.du, it.scal[1], it.g, it.c,
+                                            it.rkA[stage], it.rkB[stage]);
+    }
+}
+
 template <int NDIM, int TIME>
 __global__ void __launch_bounds__(BLOCK) kb_resid_rk(const BItem* __restrict__ items, const int* __restrict__ offs, int n,
                                                      int parity, int stage) {

@@ -45,9 +45,10 @@ inline unsigned nblocks(long work) { return (unsigned)((work + BLOCK - 1) / BLOC
 
 // kernel families reported by osb_profile_read
 enum ProfFamily { PF_FACE_FLUX, PF_FLUX_RESIDUAL, PF_CONSTITUENT, PF_GRADIENTS, PF_CENTRAL, PF_VISCOUS,
-                  PF_RK, PF_BC, PF_REDUCE, PF_CENTRAL_FUSED, PF_COUNT };
+                  PF_RK, PF_BC, PF_REDUCE, PF_CENTRAL_FUSED, PF_FACE_RESID, PF_COUNT };
 const char* const kFamilyName[PF_COUNT] = {"face_flux", "flux_residual", "constituent", "gradients", "central_convective",
-                                           "viscous", "rk_update", "boundary", "reduction", "central_fused"};
+                                           "viscous", "rk_update", "boundary", "reduction", "central_fused",
+                                           "face_resid_rk"};
 
 hipEvent_t pool_event(osb_ctx* c) {
     if (!c->prof_pool.empty()) { hipEvent_t e = c->prof_pool.back(); c->prof_pool.pop_back(); return e; }
@@ -171,9 +172,34 @@ void rhs_nd(osb_ctx* c, const double* q) {
 
 // Shock capturing without viscous terms: face fluxes, then one kernel that
 // differences them and applies the RK update (no residual array).
+template <int NDIM, int TIME, typename T>
+void launch_face_resid(osb_ctx* c, const double* q, double* qn, double A, double B) {
+    Timed tm(c, PF_FACE_RESID);
+    const unsigned nt = (unsigned)fused_tiles<NDIM>(c->g);
+    switch (c->sch) {
+        case SCH_WENOZ:
+            k_face_resid<NDIM, SCH_WENOZ, T, TIME><<<nt, BLOCK, 0, c->stream>>>(q, c->F, qn, c->du, c->scal, c->g, c->c, A, B);
+            break;
+        case SCH_TENO5:
+            k_face_resid<NDIM, SCH_TENO5, T, TIME><<<nt, BLOCK, 0, c->stream>>>(q, c->F, qn, c->du, c->scal, c->g, c->c, A, B);
+            break;
+        default:
+            k_face_resid<NDIM, SCH_TENO6, T, TIME><<<nt, BLOCK, 0, c->stream>>>(q, c->F, qn, c->du, c->scal, c->g, c->c, A, B);
+            break;
+    }
+}
+
 template <int NDIM, int TIME>
 void shock_stage_nd(osb_ctx* c, const double* q, double* qn, double A, double B) {
     launch_glf(c, q);
+    if (c->shock_fused) {
+        // the other directions' fluxes to F, then the last direction fused with the update
+        if (NDIM > 1) launch_flux<NDIM, 0>(c, q);
+        if (NDIM > 2) launch_flux<NDIM, (NDIM > 2 ? 1 : 0)>(c, q);
+        if (c->mixed) launch_face_resid<NDIM, TIME, float>(c, q, qn, A, B);
+        else launch_face_resid<NDIM, TIME, double>(c, q, qn, A, B);
+        return;
+    }
     launch_flux<NDIM, 0>(c, q);
     if (NDIM > 1) launch_flux<NDIM, (NDIM > 1 ? 1 : 0)>(c, q);
     if (NDIM > 2) launch_flux<NDIM, (NDIM > 2 ? 2 : 0)>(c, q);
@@ -622,6 +648,8 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     if (const char* e = std::getenv("OSB_CENTRAL_KERNEL")) c->march = std::strcmp(e, "march") == 0;
     if (const char* e = std::getenv("OSB_GRAPH")) c->use_graph = std::atoi(e) != 0;
     if (const char* e = std::getenv("OSB_FACE_KERNEL")) c->face_tiled = std::strcmp(e, "tile") == 0;
+    if (const char* e = std::getenv("OSB_SHOCK_FUSED")) c->shock_fused = std::strcmp(e, "0") != 0;
+    if (c->face_tiled) c->shock_fused = false;      // the LDS-tiled face variant keeps the separate update
     if (const char* e = std::getenv("OSB_MARCH_KZ")) c->march_kz = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("OSB_MARCH_TILE")) c->march_tile = std::atoi(e);
     *out = c;

@@ -144,22 +144,24 @@ __device__ __forceinline__ void face_flux_tile(long tile, const double* __restri
     int i, j, k;
     coords(w, lf + 2, i, j, k);     // point f (face f + 1/2 is stored at point f)
     const bool line_ok = DIR == 0 ? (j < g.n[1]) : (i < g.n[0]);
-    if (!line_ok || f >= nd) return;
-    double out[NV];
-    face_core<NDIM, DIR, SCH, T>(
-        [&](int s, double (&v)[NV], double (&ff)[NV], double (&uu)[NDIM], double& p, double& a) {
-            const int sl = slot(w, lf + s);
+    // no early return: in the persistent batched loop this thread has further tiles
+    if (line_ok && f < nd) {
+        double out[NV];
+        face_core<NDIM, DIR, SCH, T>(
+            [&](int s, double (&v)[NV], double (&ff)[NV], double (&uu)[NDIM], double& p, double& a) {
+                const int sl = slot(w, lf + s);
 #pragma unroll
-            for (int m = 0; m < NV; ++m) { v[m] = s_v[m][sl]; ff[m] = s_f[m][sl]; }
+                for (int m = 0; m < NV; ++m) { v[m] = s_v[m][sl]; ff[m] = s_f[m][sl]; }
 #pragma unroll
-            for (int d = 0; d < NDIM; ++d) uu[d] = s_uu[d][sl];
-            p = s_p[sl];
-            a = s_a[sl];
-        },
-        c, out);
-    const long x = gidx(g, i, j, k);
+                for (int d = 0; d < NDIM; ++d) uu[d] = s_uu[d][sl];
+                p = s_p[sl];
+                a = s_a[sl];
+            },
+            c, out);
+        const long x = gidx(g, i, j, k);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) F[(long)v * g.Ptot + x] = out[v];
+        for (int v = 0; v < NV; ++v) F[(long)v * g.Ptot + x] = out[v];
+    }
 }
 
 template <int NDIM, int DIR, int SCH, typename T>
@@ -232,6 +234,105 @@ __global__ void __launch_bounds__(BLOCK) k_resid_rk(const double* __restrict__ F
                                                     double* __restrict__ qout, double* __restrict__ du,
                                                     const double* __restrict__ scal, Geom g, Consts c, double A, double B) {
     resid_rk_item<NDIM, TIME>(gtid(), F, qin, qout, du, scal[1], g, c, A, B);
+}
+
+// ---------------------------------------------------------------------------
+// Last direction's face fluxes fused with the flux-difference residual and
+// the RK update.  A workgroup owns FW lines along D = NDIM-1 and FL faces on
+// each (faces j0-1 .. j0+FL-2, consecutive threads across the lines).  Each
+// thread evaluates one face as k_face_flux does and leaves it in LDS; the
+// threads of faces 1..FL-1 then update the points between them, with the
+// fluxes of the other directions read from F.  One face in FL is evaluated
+// twice (by two neighbouring tiles); the D-direction fluxes never reach HBM.
+// The arithmetic is that of face_flux + resid_rk_item, term for term.
+// ---------------------------------------------------------------------------
+template <int NDIM>
+struct FusedTile {
+    static constexpr int D = NDIM - 1;
+    static constexpr int FL = NDIM == 1 ? BLOCK : 16;   // faces per line
+    static constexpr int FW = NDIM == 1 ? 1 : 16;       // lines (consecutive x for D > 0)
+    static constexpr int PL = FL - 1;                   // points per line
+    static_assert(FL * FW == BLOCK, "one face per thread");
+};
+
+template <int NDIM>
+__host__ __device__ inline long fused_tiles(const Geom& g) {
+    using FT = FusedTile<NDIM>;
+    constexpr int D = NDIM - 1;
+    const long nseg = (g.n[D] + FT::PL - 1) / FT::PL;
+    if (NDIM == 1) return nseg;
+    const long ncol = (g.n[0] + FT::FW - 1) / FT::FW;
+    return ncol * nseg * (NDIM == 3 ? g.n[1] : 1);
+}
+
+template <int NDIM, int SCH, typename T, int TIME>
+__device__ __forceinline__ void face_resid_tile(long tile, const double* __restrict__ q, const double* __restrict__ F,
+                                                double* __restrict__ qout, double* __restrict__ du, double dt, const Geom& g,
+                                                const Consts& c, double A, double B) {
+    constexpr int NV = NDIM + 2;
+    using FT = FusedTile<NDIM>;
+    constexpr int D = FT::D, FW = FT::FW, PL = FT::PL;
+    __shared__ double s_F[NV][BLOCK];
+    const long nseg = (g.n[D] + PL - 1) / PL;
+    int col0 = 0, j0, other = 0;
+    if (NDIM == 1) {
+        j0 = (int)tile * PL;
+    } else {
+        const long ncol = (g.n[0] + FW - 1) / FW;
+        col0 = (int)(tile % ncol) * FW;
+        const long r = tile / ncol;
+        j0 = (int)(r % nseg) * PL;
+        other = (int)(r / nseg);          // k for NDIM 2 (always 0), j for NDIM 3
+    }
+    const int tid = (int)threadIdx.x;
+    const int w = tid % FW, lf = tid / FW;
+    const int f = j0 - 1 + lf;            // face f + 1/2, stored at point f
+    int i, j, k;
+    if (NDIM == 1) { i = f; j = 0; k = 0; }
+    else if (NDIM == 2) { i = col0 + w; j = f; k = 0; }
+    else { i = col0 + w; j = other; k = f; }
+    const bool ok = (NDIM == 1 || col0 + w < g.n[0]) && f < g.n[D];
+    const long x = ok ? gidx(g, i, j, k) : 0;
+    __syncthreads();                      // the previous tile of a persistent loop is done with LDS
+    if (ok) {
+        double out[NV];
+        face_flux<NDIM, D, SCH, T>(q, g.Ptot, g.s[D], x, c, out);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) s_F[v][tid] = out[v];
+    }
+    __syncthreads();
+    // point f lies between faces lf - 1 and lf (no early return: the persistent
+    // batched loop calls this again and every thread must reach its barriers)
+    if (ok && lf > 0) {
+        const long nvP = (long)NV * g.Ptot;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double acc = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const double* wd = F + d * nvP + (long)v * g.Ptot;
+                acc -= (wd[x] - wd[x - g.s[d]]) * c.invd[d];
+            }
+            acc -= (s_F[v][tid] - s_F[v][tid - FW]) * c.invd[D];
+            const long y = (long)v * g.Ptot + x;
+            if (TIME == 1) {
+                const double dd = fma(A, du[y], dt * acc);
+                du[y] = dd;
+                qout[y] = fma(B, dd, q[y]);
+            } else {
+                const double o = du[y];
+                qout[y] = fma(dt * A, acc, o);
+                du[y] = fma(dt * B, acc, o);
+            }
+        }
+    }
+}
+
+template <int NDIM, int SCH, typename T, int TIME>
+__global__ void __launch_bounds__(BLOCK) k_face_resid(const double* __restrict__ q, const double* __restrict__ F,
+                                                      double* __restrict__ qout, double* __restrict__ du,
+                                                      const double* __restrict__ scal, Geom g, Consts c, double A, double B) {
+    face_resid_tile<NDIM, SCH, T, TIME>(blockIdx.x, q, F, qout, du, scal[1], g, c, A, B);
 }
 
 // Constituent relations u_i, p, T, mu on the box [-ext, n+ext)
