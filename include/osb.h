@@ -149,6 +149,11 @@ int  osb_set_time(osb_ctx* ctx, double time, int64_t iter);
 /* Stepping.  Replaces the generated time loop
  * (algorithm/algorithm.py:387-502; TENO6/opensbli.cpp:206-432).             */
 int  osb_step(osb_ctx* ctx, int nsteps);
+/* One RK stage of the current step, for debugging: the body of the stage loop
+ * (rk_LS.py:229-233 -- the RHS, the stage update, the BCs).  Stages run in
+ * order 0..stages-1; stage 0 also computes the CFL dt, and the iteration count
+ * and time advance after the last.  osb_step refuses a partly advanced step. */
+int  osb_stage(osb_ctx* ctx, int stage);
 int  osb_apply_bcs(osb_ctx* ctx);                        /* ops_halo_transfer + BC kernels */
 int  osb_residual(osb_ctx* ctx, double* host_out);       /* one RHS evaluation, interior layout */
 int  osb_max_wavespeed(osb_ctx* ctx, double* out);       /* max_points sum_d (|u_d|+a)/Delta_d */

@@ -356,49 +356,49 @@ int check_launch(osb_ctx* c) {
     return OSB_OK;
 }
 
-int do_step(osb_ctx* c) {
+// RK stage s of the current step (stage 0 also does the step's start: the
+// CFL dt, the classic RK3 saved state), ending with the boundary conditions.
+void do_stage(osb_ctx* c, int s) {
     const Geom& g = c->g;
     const long nin = interior_points(g);
-    if (c->d.cfl > 0) {
+    if (s == 0 && c->d.cfl > 0) {
         launch_wavespeed(c, c->q_cur);
         k_dt_finalize<<<1, 64, 0, c->stream>>>(c->scal, c->d.cfl);
     }
+    double* qn = (c->q_cur == c->qa) ? c->qb : c->qa;
     if (c->d.time_scheme == OSB_TIME_RKLS3) {
-        for (int s = 0; s < c->rk_stages; ++s) {
-            double* qn = (c->q_cur == c->qa) ? c->qb : c->qa;
-            if (use_fused(c)) {
-                fused_stage<1>(c, c->q_cur, qn, c->rkA[s], c->rkB[s]);
-            } else if (use_shock_stage(c)) {
-                shock_stage<1>(c, c->q_cur, qn, c->rkA[s], c->rkB[s]);
-            } else {
-                rhs(c, c->q_cur);
-                Timed tm(c, PF_RK);
-                k_rkls<<<nblocks(nin), BLOCK, 0, c->stream>>>(c->q_cur, qn, c->du, c->R, c->scal, g, c->nv, c->rkA[s], c->rkB[s]);
-            }
-            apply_bcs(c, qn);
-            c->q_prev = c->q_cur;
-            c->q_cur = qn;
+        if (use_fused(c)) {
+            fused_stage<1>(c, c->q_cur, qn, c->rkA[s], c->rkB[s]);
+        } else if (use_shock_stage(c)) {
+            shock_stage<1>(c, c->q_cur, qn, c->rkA[s], c->rkB[s]);
+        } else {
+            rhs(c, c->q_cur);
+            Timed tm(c, PF_RK);
+            k_rkls<<<nblocks(nin), BLOCK, 0, c->stream>>>(c->q_cur, qn, c->du, c->R, c->scal, g, c->nv, c->rkA[s], c->rkB[s]);
         }
     } else {
         const double rkold[3] = {1.0 / 4.0, 3.0 / 20.0, 3.0 / 5.0};
         const double rknew[3] = {2.0 / 3.0, 5.0 / 12.0, 3.0 / 5.0};
-        k_copy_interior<<<nblocks(nin), BLOCK, 0, c->stream>>>(c->q_cur, c->du, g, c->nv);
-        for (int s = 0; s < 3; ++s) {
-            double* qn = (c->q_cur == c->qa) ? c->qb : c->qa;
-            if (use_fused(c)) {
-                fused_stage<2>(c, c->q_cur, qn, rknew[s], rkold[s]);
-            } else if (use_shock_stage(c)) {
-                shock_stage<2>(c, c->q_cur, qn, rknew[s], rkold[s]);
-            } else {
-                rhs(c, c->q_cur);
-                Timed tm(c, PF_RK);
-                k_rk3_stage<<<nblocks(nin), BLOCK, 0, c->stream>>>(qn, c->du, c->R, c->scal, g, c->nv, rknew[s], rkold[s]);
-            }
-            apply_bcs(c, qn);
-            c->q_prev = c->q_cur;
-            c->q_cur = qn;
+        if (s == 0) k_copy_interior<<<nblocks(nin), BLOCK, 0, c->stream>>>(c->q_cur, c->du, g, c->nv);
+        if (use_fused(c)) {
+            fused_stage<2>(c, c->q_cur, qn, rknew[s], rkold[s]);
+        } else if (use_shock_stage(c)) {
+            shock_stage<2>(c, c->q_cur, qn, rknew[s], rkold[s]);
+        } else {
+            rhs(c, c->q_cur);
+            Timed tm(c, PF_RK);
+            k_rk3_stage<<<nblocks(nin), BLOCK, 0, c->stream>>>(qn, c->du, c->R, c->scal, g, c->nv, rknew[s], rkold[s]);
         }
     }
+    apply_bcs(c, qn);
+    c->q_prev = c->q_cur;
+    c->q_cur = qn;
+}
+
+int stage_count(const osb_ctx* c) { return c->d.time_scheme == OSB_TIME_RKLS3 ? c->rk_stages : 3; }
+
+int do_step(osb_ctx* c) {
+    for (int s = 0; s < stage_count(c); ++s) do_stage(c, s);
     c->have_prev = true;
     return check_launch(c);
 }
@@ -688,6 +688,7 @@ int osb_set_state(osb_ctx* c, const double* host, int layout) {
     c->q_cur = c->qa;
     c->q_prev = c->qb;
     c->have_prev = false;
+    c->next_stage = 0;
     apply_bcs(c, c->q_cur);
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return check_launch(c);
@@ -777,6 +778,7 @@ int graph_step(osb_ctx* c) {
 
 int osb_step(osb_ctx* c, int nsteps) {
     if (!c || nsteps < 0) return OSB_E_INVALID;
+    if (c->next_stage != 0) return set_err(c, OSB_E_INVALID, "a step is partly advanced by osb_stage; finish its stages first");
     const bool graphs = c->use_graph && !c->prof_on;
     for (int i = 0; i < nsteps; ++i) {
         int r = graphs ? graph_step(c) : do_step(c);
@@ -785,6 +787,21 @@ int osb_step(osb_ctx* c, int nsteps) {
         if (!(c->d.cfl > 0)) { c->time += c->d.dt; c->last_dt = c->d.dt; }
     }
     return OSB_OK;
+}
+
+int osb_stage(osb_ctx* c, int s) {
+    if (!c) return OSB_E_INVALID;
+    if (s < 0 || s >= stage_count(c)) return set_err(c, OSB_E_INVALID, "stage index out of range");
+    if (s != c->next_stage)
+        return set_err(c, OSB_E_INVALID, "stages run in order: expected stage " + std::to_string(c->next_stage));
+    do_stage(c, s);                       // eager: single stages are not captured
+    c->next_stage = (s + 1) % stage_count(c);
+    if (c->next_stage == 0) {
+        c->have_prev = true;
+        c->iter += 1;
+        if (!(c->d.cfl > 0)) { c->time += c->d.dt; c->last_dt = c->d.dt; }
+    }
+    return check_launch(c);
 }
 
 int osb_sync(osb_ctx* c) {
@@ -877,6 +894,7 @@ int osb_set_state_device(osb_ctx* c, const double* dev, int layout) {
     c->q_cur = c->qa;
     c->q_prev = c->qb;
     c->have_prev = false;
+    c->next_stage = 0;
     apply_bcs(c, c->q_cur);
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return check_launch(c);

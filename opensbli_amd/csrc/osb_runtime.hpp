@@ -50,6 +50,7 @@ struct osb_ctx {
     double last_dt = 0.0;
     bool have_prev = false;
     std::string err;
+    int next_stage = 0;             // osb_stage: the stage the current step continues with
     int rk_stages = 3;              // RK-LS stages and coefficients (rk_LS.py:97-132)
     double rkA[5] = {0}, rkB[5] = {0};
     int fused_tile = 0;             // index into CF_TILES (autotuned)

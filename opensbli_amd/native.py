@@ -229,6 +229,7 @@ class Library:
         L.osb_get_state.argtypes = [vp, dptr, C.c_int]
         L.osb_set_time.argtypes = [vp, C.c_double, C.c_int64]
         L.osb_step.argtypes = [vp, C.c_int]
+        L.osb_stage.argtypes = [vp, C.c_int]
         L.osb_apply_bcs.argtypes = [vp]
         L.osb_residual.argtypes = [vp, dptr]
         L.osb_max_wavespeed.argtypes = [vp, C.POINTER(C.c_double)]
@@ -378,6 +379,10 @@ class Context:
 
     def step(self, nsteps: int = 1):
         self._check(self.lib.lib.osb_step(self.h, int(nsteps)))
+
+    def stage(self, s: int):
+        """One RK stage of the current step (osb_stage), in order."""
+        self._check(self.lib.lib.osb_stage(self.h, int(s)))
 
     def apply_bcs(self):
         self._check(self.lib.lib.osb_apply_bcs(self.h))

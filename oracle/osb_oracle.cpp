@@ -61,6 +61,8 @@ struct osb_ctx {
     int rk_stages = 3;                    // RK-LS stages and coefficients
     double rkA[5] = {0}, rkB[5] = {0};
     double glf[3][3] = {};                // GLF: max |u_d|, |u_d + a|, |u_d - a| over the grid, per stage
+    int next_stage = 0;                   // osb_stage: the stage the current step continues with
+    double stage_dt = 0.0;                // dt of the current step
 
     size_t I(int i, int j, int k) const {
         return (size_t)(i + off[0]) + (size_t)P[0] * ((size_t)(j + off[1]) + (size_t)P[1] * (size_t)(k + off[2]));
@@ -983,6 +985,7 @@ int osb_ctx_destroy(osb_ctx* ctx) { delete ctx; return OSB_OK; }
 int osb_set_state(osb_ctx* c, const double* host, int layout) {
     if (!c || !host) return OSB_E_INVALID;
     if (layout != OSB_LAYOUT_INTERIOR && layout != OSB_LAYOUT_PADDED) return fail(c, OSB_E_INVALID, "unknown layout");
+    c->next_stage = 0;
     std::fill(c->rkold.begin(), c->rkold.end(), 0.0);
     if (layout == OSB_LAYOUT_PADDED) {            // halos are refilled by the boundary conditions
         std::copy(host, host + c->q.size(), c->q.begin());
@@ -1043,50 +1046,67 @@ int osb_max_wavespeed(osb_ctx* c, double* out) {
     return OSB_OK;
 }
 
-int osb_step(osb_ctx* c, int nsteps) {
-    if (!c || nsteps < 0) return OSB_E_INVALID;
+// RK stage s of the current step; the step's start (BCs, dt, the RK3 saved
+// state) happens at s = 0 and its end (iteration count, time) after the last.
+void run_stage(osb_ctx* c, int s) {
     const int nv = c->nv;
     const size_t Pt = c->Ptot;
     Range in = make_range(c, 0);
-    for (int it = 0; it < nsteps; ++it) {
+    const int nst = c->d.time_scheme == OSB_TIME_RKLS3 ? c->rk_stages : 3;
+    if (s == 0) {
         apply_bcs(c);                                        // temporal_start BC kernels (algorithm.py:462)
-        double dt = c->d.dt;
-        if (c->d.cfl > 0) dt = c->d.cfl / max_wavespeed(c);
-        c->last_dt = dt;
-        if (c->d.time_scheme == OSB_TIME_RKLS3) {
-            for (int s = 0; s < c->rk_stages; ++s) {
-                compute_rhs(c);
-                const double A = c->rkA[s], B = c->rkB[s];
-                for_range(in, [&](int i, int j, int k) {
-                    const size_t x = c->I(i, j, k);
-                    for (int v = 0; v < nv; ++v) {
-                        const size_t y = (size_t)v * Pt + x;
-                        rkls_point(A, B, dt, c->R[y], c->rkold[y], c->q[y]);
-                    }
-                });
-                apply_bcs(c);
-            }
-        } else {
+        c->stage_dt = c->d.cfl > 0 ? c->d.cfl / max_wavespeed(c) : c->d.dt;
+        c->last_dt = c->stage_dt;
+        if (c->d.time_scheme == OSB_TIME_RK3)
             // rk_sbli.py:129-159: save q ("Save equations"), then the stage updates
             for_range(in, [&](int i, int j, int k) {
                 const size_t x = c->I(i, j, k);
                 for (int v = 0; v < nv; ++v) c->rkold[(size_t)v * Pt + x] = c->q[(size_t)v * Pt + x];
             });
-            for (int s = 0; s < 3; ++s) {
-                compute_rhs(c);
-                for_range(in, [&](int i, int j, int k) {
-                    const size_t x = c->I(i, j, k);
-                    for (int v = 0; v < nv; ++v) {
-                        const size_t y = (size_t)v * Pt + x;
-                        rk3_point(s, dt, c->R[y], c->rkold[y], c->q[y]);
-                    }
-                });
-                apply_bcs(c);
+    }
+    const double dt = c->stage_dt;
+    compute_rhs(c);
+    if (c->d.time_scheme == OSB_TIME_RKLS3) {
+        const double A = c->rkA[s], B = c->rkB[s];
+        for_range(in, [&](int i, int j, int k) {
+            const size_t x = c->I(i, j, k);
+            for (int v = 0; v < nv; ++v) {
+                const size_t y = (size_t)v * Pt + x;
+                rkls_point(A, B, dt, c->R[y], c->rkold[y], c->q[y]);
             }
-        }
+        });
+    } else {
+        for_range(in, [&](int i, int j, int k) {
+            const size_t x = c->I(i, j, k);
+            for (int v = 0; v < nv; ++v) {
+                const size_t y = (size_t)v * Pt + x;
+                rk3_point(s, dt, c->R[y], c->rkold[y], c->q[y]);
+            }
+        });
+    }
+    apply_bcs(c);
+    c->next_stage = (s + 1) % nst;
+    if (c->next_stage == 0) {
         c->iter += 1;
         c->time += dt;
     }
+}
+
+int osb_step(osb_ctx* c, int nsteps) {
+    if (!c || nsteps < 0) return OSB_E_INVALID;
+    if (c->next_stage != 0) return fail(c, OSB_E_INVALID, "a step is partly advanced by osb_stage; finish its stages first");
+    const int nst = c->d.time_scheme == OSB_TIME_RKLS3 ? c->rk_stages : 3;
+    for (int it = 0; it < nsteps; ++it)
+        for (int s = 0; s < nst; ++s) run_stage(c, s);
+    return OSB_OK;
+}
+
+int osb_stage(osb_ctx* c, int s) {
+    if (!c) return OSB_E_INVALID;
+    const int nst = c->d.time_scheme == OSB_TIME_RKLS3 ? c->rk_stages : 3;
+    if (s < 0 || s >= nst) return fail(c, OSB_E_INVALID, "stage index out of range");
+    if (s != c->next_stage) return fail(c, OSB_E_INVALID, "stages run in order: expected stage " + std::to_string(c->next_stage));
+    run_stage(c, s);
     return OSB_OK;
 }
 

@@ -156,3 +156,50 @@ def test_nan_check_reports_nonfinite(oracle):
     with oracle.context(pr.desc) as c:
         c.set_state(pr.q0)
         assert c.nan_check() == nat.OK
+
+
+@pytest.mark.parametrize("maker", [
+    lambda: P.sod(n=64, steps=3),
+    lambda: P.kelvin_helmholtz(n=16, steps=2),
+    lambda: P.tgv(n=12, steps=2, scheme="teno", minf=1.25, cfl=0.5, dt=0.0),
+])
+def test_stages_equal_steps(oracle, maker):
+    """osb_stage over every stage of every step equals osb_step bit for bit
+    (iteration count and time included); out-of-order stages and a step
+    started mid-way are refused."""
+    pr = maker()
+    nst = 3
+    with oracle.context(pr.desc) as a, oracle.context(pr.desc) as b:
+        a.set_state(pr.q0)
+        b.set_state(pr.q0)
+        a.step(pr.steps)
+        for _ in range(pr.steps):
+            for s in range(nst):
+                b.stage(s)
+        assert np.array_equal(a.get_state(), b.get_state())
+        sa, sb = a.stats(), b.stats()
+        assert sa.iter == sb.iter and sa.time == sb.time
+        with pytest.raises(nat.OSBError):
+            b.stage(1)
+        b.stage(0)
+        with pytest.raises(nat.OSBError):
+            b.step(1)
+        b.stage(1)
+        b.stage(2)
+        b.step(1)
+
+
+def test_first_rkls_stage_is_b0_dt_residual(oracle):
+    """RK-LS stage 0 from a fresh state (du = 0, A0 = 0): q1 = q0 + B0 dt R(q0),
+    B0 = 1/3 (rk_LS.py:121-124)."""
+    pr = P.kelvin_helmholtz(n=16, steps=1)
+    with oracle.context(pr.desc) as c:
+        c.set_state(pr.q0)
+        R = c.residual()
+        c.set_state(pr.q0)
+        c.stage(0)
+        q1 = c.get_state()
+    want = pr.q0 + (1.0 / 3.0) * (pr.desc.dt * R)
+    inner = (slice(None), slice(None), slice(1, -1))      # the wall rows are rewritten by the BC after the stage
+    assert np.allclose(q1[inner], want[inner], rtol=1e-15, atol=1e-15 * np.abs(pr.q0).max())
+    assert not np.array_equal(q1, pr.q0)

@@ -250,3 +250,19 @@ def test_hdf5_restart_on_the_gpu(hip, oracle, tmp_path):
             s2 = c.stats()
         assert np.array_equal(q, straight)
         assert s2.iter == st.iter and math.isclose(s2.time, st.time, rel_tol=1e-15)
+
+
+def test_stage_by_stage_equals_step(hip, oracle):
+    """osb_stage on the GPU: stage-by-stage equals osb_step (graphs) and the oracle's stages."""
+    for pr in (P.kelvin_helmholtz(n=48, steps=2), P.tgv(n=16, steps=2)):
+        _announce(f"stages {pr.name}")
+        with hip.context(pr.desc) as a, hip.context(pr.desc) as b, oracle.context(pr.desc) as o:
+            for c in (a, b, o):
+                c.set_state(pr.q0)
+            a.step(pr.steps)
+            for _ in range(pr.steps):
+                for s in range(3):
+                    b.stage(s)
+                    o.stage(s)
+            qa, qb, qo = a.get_state(), b.get_state(), o.get_state()
+        assert np.array_equal(qa, qb) and np.array_equal(qb, qo)
