@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(BLOCK) kb_face_resid(const BItem* __restrict__
     OSB_BATCH_LOOP {
         const int i = find_item(offs, n, vb);
         const BItem& it = items[i];
-        face_resid_tile<NDIM, SCH, T, TIME>(vb - offs[i], it.q[parity], it.F, it.q[parity ^ 1], it.du, it.scal[1], it.g, it.c,
+        face_resid_tile<NDIM, SCH, T, TIME, FUSED_FW>(vb - offs[i], it.q[parity], it.F, it.q[parity ^ 1], it.du, it.scal[1], it.g, it.c,
                                             it.rkA[stage], it.rkB[stage]);
     }
 }

@@ -172,21 +172,27 @@ void rhs_nd(osb_ctx* c, const double* q) {
 
 // Shock capturing without viscous terms: face fluxes, then one kernel that
 // differences them and applies the RK update (no residual array).
+template <int NDIM, int TIME, typename T, int W>
+void launch_face_resid_w(osb_ctx* c, const double* q, double* qn, double A, double B) {
+    const unsigned nt = (unsigned)fused_tiles<NDIM, W>(c->g);
+    switch (c->sch) {
+        case SCH_WENOZ:
+            k_face_resid<NDIM, SCH_WENOZ, T, TIME, W><<<nt, BLOCK, 0, c->stream>>>(q, c->F, qn, c->du, c->scal, c->g, c->c, A, B);
+            break;
+        case SCH_TENO5:
+            k_face_resid<NDIM, SCH_TENO5, T, TIME, W><<<nt, BLOCK, 0, c->stream>>>(q, c->F, qn, c->du, c->scal, c->g, c->c, A, B);
+            break;
+        default:
+            k_face_resid<NDIM, SCH_TENO6, T, TIME, W><<<nt, BLOCK, 0, c->stream>>>(q, c->F, qn, c->du, c->scal, c->g, c->c, A, B);
+            break;
+    }
+}
+
 template <int NDIM, int TIME, typename T>
 void launch_face_resid(osb_ctx* c, const double* q, double* qn, double A, double B) {
     Timed tm(c, PF_FACE_RESID);
-    const unsigned nt = (unsigned)fused_tiles<NDIM>(c->g);
-    switch (c->sch) {
-        case SCH_WENOZ:
-            k_face_resid<NDIM, SCH_WENOZ, T, TIME><<<nt, BLOCK, 0, c->stream>>>(q, c->F, qn, c->du, c->scal, c->g, c->c, A, B);
-            break;
-        case SCH_TENO5:
-            k_face_resid<NDIM, SCH_TENO5, T, TIME><<<nt, BLOCK, 0, c->stream>>>(q, c->F, qn, c->du, c->scal, c->g, c->c, A, B);
-            break;
-        default:
-            k_face_resid<NDIM, SCH_TENO6, T, TIME><<<nt, BLOCK, 0, c->stream>>>(q, c->F, qn, c->du, c->scal, c->g, c->c, A, B);
-            break;
-    }
+    if (NDIM == 1 || c->fused_fw != 16) launch_face_resid_w<NDIM, TIME, T, FUSED_FW>(c, q, qn, A, B);
+    else launch_face_resid_w<NDIM, TIME, T, 16>(c, q, qn, A, B);
 }
 
 template <int NDIM, int TIME>
@@ -649,6 +655,10 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     if (const char* e = std::getenv("OSB_GRAPH")) c->use_graph = std::atoi(e) != 0;
     if (const char* e = std::getenv("OSB_FACE_KERNEL")) c->face_tiled = std::strcmp(e, "tile") == 0;
     if (const char* e = std::getenv("OSB_SHOCK_FUSED")) c->shock_fused = std::strcmp(e, "0") != 0;
+    if (const char* e = std::getenv("OSB_FUSED_FW")) {
+        const int w = std::atoi(e);
+        if (w == 16 || w == 32) c->fused_fw = w;
+    }
     if (c->face_tiled) c->shock_fused = false;      // the LDS-tiled face variant keeps the separate update
     if (const char* e = std::getenv("OSB_MARCH_KZ")) c->march_kz = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("OSB_MARCH_TILE")) c->march_tile = std::atoi(e);

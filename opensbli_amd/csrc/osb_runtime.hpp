@@ -60,6 +60,7 @@ struct osb_ctx {
     int march_tile = 0;             // 0: 16x16 columns, 1: 32x8
 
     bool face_tiled = false;        // face fluxes one thread per face from L2/HBM; OSB_FACE_KERNEL=tile: LDS-tiled variant
+    int fused_fw = 32;              // lines per fused tile: 32 or 16 (OSB_FUSED_FW)
     bool shock_fused = true;        // shock-only stages: last direction's faces fused with the update (OSB_SHOCK_FUSED=0: separate)
 
     // captured time steps: graph p starts from buffer qa (p = 0) or qb (p = 1)
