@@ -167,6 +167,12 @@ int  osb_get_stats(osb_ctx* ctx, osb_stats* out);
  * reference's post-process kernel placement (TENO6/opensbli.cpp:433-455).   */
 int  osb_tgv_diagnostics(osb_ctx* ctx, double out[4]);
 
+/* Point probes of the state (SimulationMonitor, simulation_monitors.py:40-265:
+ * the monitor_%d_<name> reduction kernels over one grid point).  Probe p reads
+ * conservative variable var[p] (0 = rho ... nv-1 = rhoE) at interior grid
+ * index ijk[3p..3p+2] (unused directions 0) into out[p].                     */
+int  osb_probe(osb_ctx* ctx, int n, const int32_t* ijk, const int32_t* var, double* out);
+
 /* NaN check of rho (ops_NaNcheck, helperfunctions.py:201-214):
  * returns OSB_E_NONFINITE if any interior rho is not finite.                */
 int  osb_nan_check(osb_ctx* ctx);

@@ -409,6 +409,20 @@ class HIPTarget:
                     "true", "1", "yes"):
                 self.restart_file = os.path.join(output_dir, "restart.h5")      # opsc.py:962-977
         self.start_iter = 0
+        self.monitor = getattr(alg, "simulation_monitor", None)
+        self._probes = None
+        if self.monitor is not None:
+            names = _conservative_names(self.desc.ndim)
+            pts, var = [], []
+            for a, loc in zip(self.monitor.arrays, self.monitor.probe_locations):
+                if not isinstance(loc, tuple):
+                    raise NotImplementedError(f"SimulationMonitor scalar monitor {a!r}: only point probes are implemented")
+                nm = re.sub(r"_B\d+$", "", str(a))
+                if nm not in names:
+                    raise NotImplementedError(f"SimulationMonitor array {a!r}: conservative variables only ({names})")
+                pts.append(tuple(int(x) for x in loc))
+                var.append(names.index(nm))
+            self._probes = (pts, var)
 
     # grid coordinates of the interior points, x_d = Delta_d * i_d
     def coordinates(self):
@@ -455,21 +469,51 @@ class HIPTarget:
             if _io_kwargs(w).get("position") == "init" and self.ctx.stats().iter == 0:
                 self.write_hdf5(os.path.join(self.output_dir, _io_filename(w)))
         every = sorted({_save_every(w) for w in self.writers if _save_every(w)})
+        mon = self.monitor
+        periods = every + ([mon.frequency] if mon is not None else [])
         it = int(self.ctx.stats().iter)              # loop index of the next iteration
         stop = it + steps
-        if not every:
+        if not periods:
             self.ctx.step(steps)
-        while every and it < stop:
-            # next loop index j >= it that writes: (j + 1) % e == 0, or j == 0
-            j = min([stop - 1, 0 if it == 0 else stop] + [-(-(it + 1) // e) * e - 1 for e in every])
+        while periods and it < stop:
+            # next loop index j >= it with in-loop work: (j + 1) % period == 0, or j == 0
+            j = min([stop - 1, 0 if it == 0 else stop] + [-(-(it + 1) // e) * e - 1 for e in periods])
             self.ctx.step(j - it + 1)
-            if j == 0 or any((j + 1) % e == 0 for e in every):
+            if mon is not None and (j == 0 or (j + 1) % mon.frequency == 0):
+                self._monitor(j)                     # the monitor precedes the in-loop IO (algorithm.py:452-485)
+            if every and (j == 0 or any((j + 1) % e == 0 for e in every)):
                 self.write_hdf5(os.path.join(self.output_dir, "opensbli_output_%06d.h5" % (j + 1)), iteration=j)
             it = j + 1
         for w in self.writers:
             if _io_kwargs(w).get("position") != "init":
                 self.write_hdf5(os.path.join(self.output_dir, _io_filename(w)))
         return self
+
+    def _monitor(self, j: int):
+        """One monitor line after loop index j (simulation_monitors.py:205-257):
+        iteration j + 1, the simulation time, then each probe, in %.<p>e; the
+        header first when j == 0.  NaN_check first runs ops_NaNcheck."""
+        mon = self.monitor
+        if mon.NaN_check is not None:
+            rc = self.ctx.nan_check()
+            if rc:
+                raise nat.OSBError(rc, f"NaN check failed after iteration {j + 1}")
+        pts, var = self._probes
+        vals = self.ctx.probe(pts, var) if pts else []
+        p = mon.fp_precision
+        lines = []
+        if j == 0:
+            heads = ["Iteration", "Time"] + [f"{re.sub(r'_B[0-9]+$', '', str(a))}_B{self.block_number}{loc}"
+                                             for a, loc in zip(mon.arrays, mon.probe_locations)]
+            lines.append(", ".join(heads))
+        t = self.ctx.stats().time
+        lines.append(", ".join([str(j + 1), f"{t:.{p}e}"] + [f"{v:.{p}e}" for v in vals]))
+        if mon.output_file:
+            path = os.path.join(self.output_dir, f"block{self.block_number}_{mon.output_file}")
+            with open(path, "a") as f:
+                f.write("\n".join(lines) + "\n")
+        else:
+            print("\n".join(lines), flush=True)
 
     def state(self) -> np.ndarray:
         return self.ctx.get_state()

@@ -519,8 +519,10 @@ int osb_batch_describe(osb_batch* b, char* buf, int len) {
         bool first = true;
         for (int f = 0; f < BF_COUNT; ++f) {
             if (gr.total(f) == 0 || (f >= BF_FACE0 + gr.ndim && f <= BF_FACE2)) continue;
+            if (gr.fused && f == BF_FACE0 + gr.ndim - 1) continue;          // evaluated inside face_resid_rk
+            const char* name = (gr.fused && f == BF_RESID) ? "face_resid_rk" : kBFamName[f];
             std::snprintf(tmp, sizeof(tmp), "%s\"%s\": {\"virtual_blocks\": %d, \"grid\": %u, \"us\": %.2f}", first ? "" : ", ",
-                          kBFamName[f], gr.total(f), grid_of(gr, f), (double)gr.tuned_us[f]);
+                          name, gr.total(f), grid_of(gr, f), (double)gr.tuned_us[f]);
             s += tmp;
             first = false;
         }

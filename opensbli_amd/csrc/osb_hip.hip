@@ -858,6 +858,22 @@ int osb_tgv_diagnostics(osb_ctx* c, double out[4]) {
     return check_launch(c);
 }
 
+int osb_probe(osb_ctx* c, int n, const int32_t* ijk, const int32_t* var, double* out) {
+    if (!c || n < 0 || (n > 0 && (!ijk || !var || !out))) return OSB_E_INVALID;
+    for (int p = 0; p < n; ++p) {
+        for (int d = 0; d < 3; ++d)
+            if (ijk[3 * p + d] < 0 || ijk[3 * p + d] >= c->g.n[d]) return set_err(c, OSB_E_INVALID, "probe outside the grid");
+        if (var[p] < 0 || var[p] >= c->nv) return set_err(c, OSB_E_INVALID, "probe variable out of range");
+    }
+    // one 8-byte copy per probe from the current state (monitor frequency is low)
+    for (int p = 0; p < n; ++p) {
+        const long x = (long)var[p] * c->g.Ptot + gidx(c->g, ijk[3 * p], ijk[3 * p + 1], ijk[3 * p + 2]);
+        HIP_TRY(c, hipMemcpyAsync(out + p, c->q_cur + x, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return check_launch(c);
+}
+
 int osb_nan_check(osb_ctx* c) {
     if (!c) return OSB_E_INVALID;
     HIP_TRY(c, hipMemsetAsync(c->nan_cnt, 0, sizeof(unsigned int), c->stream));

@@ -237,6 +237,7 @@ class Library:
         L.osb_get_stats.argtypes = [vp, C.POINTER(StatsC)]
         L.osb_tgv_diagnostics.argtypes = [vp, C.POINTER(C.c_double)]
         L.osb_nan_check.argtypes = [vp]
+        L.osb_probe.argtypes = [vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_double)]
         L.osb_state_device_ptr.argtypes = [vp, C.c_int, C.POINTER(vp)]
         L.osb_set_stream.argtypes = [vp, vp]
         if hasattr(L, "osb_set_state_device"):
@@ -413,6 +414,21 @@ class Context:
 
     def nan_check(self) -> int:
         return self.lib.lib.osb_nan_check(self.h)
+
+    def probe(self, points: Sequence[Sequence[int]], variables: Sequence[int]) -> np.ndarray:
+        """Values of conservative variables[p] at interior grid points[p] (i, j, k)."""
+        n = len(points)
+        if len(variables) != n:
+            raise ValueError("one variable per probe point")
+        ijk = (C.c_int32 * max(3 * n, 1))()
+        for p, pt in enumerate(points):
+            pt = list(pt) + [0] * (3 - len(pt))
+            for d in range(3):
+                ijk[3 * p + d] = int(pt[d])
+        var = (C.c_int32 * max(n, 1))(*[int(v) for v in variables])
+        out = (C.c_double * max(n, 1))()
+        self._check(self.lib.lib.osb_probe(self.h, n, ijk, var, out))
+        return np.array(list(out)[:n])
 
     def set_state_device(self, ptr: int):
         """State from device memory (interior layout, FP64), e.g. tensor.data_ptr()."""

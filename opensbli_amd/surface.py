@@ -391,6 +391,31 @@ class SimulationBlock:
         return None
 
 
+class SimulationMonitor:
+    """opensbli/core/diagnostics/simulation_monitors.py:40.  ``arrays`` name the
+    probed datasets (``"rho_B0"``, ``"rho"`` or DataObjects), ``probe_locations``
+    the (i, j, k) grid index of each.  The HIP back end prints them every
+    ``print_frequency`` iterations and after the first, as the generated
+    program does, to ``block<b>_<output_file>`` (``output_file=None``: stdout)."""
+
+    def __init__(self, arrays, probe_locations, blocks, print_frequency=100, OPS_V2=True, fp_precision=12,
+                 NaN_check=None, output_file="output.log"):
+        arrays, probe_locations = list(arrays), list(probe_locations)
+        if len(arrays) != len(probe_locations):
+            raise ValueError("The number of arrays must equal the number of probe locations.")
+        self.blocks = _flatten([blocks])
+        nd = self.blocks[0].ndim
+        for loc in probe_locations:
+            if isinstance(loc, tuple) and len(loc) != nd:
+                raise ValueError("The number of probe indices (i,j,k) must match the dimensions of the problem (block.ndim).")
+        self.arrays = arrays
+        self.probe_locations = probe_locations
+        self.frequency = int(print_frequency)
+        self.fp_precision = int(fp_precision)
+        self.NaN_check = NaN_check
+        self.output_file = output_file
+
+
 class TraditionalAlgorithmRK:
     """opensbli/code_generation/algorithm/algorithm.py:334 (single block)."""
 

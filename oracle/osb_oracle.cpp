@@ -1159,6 +1159,17 @@ int osb_tgv_diagnostics(osb_ctx* c, double out[4]) {
     return OSB_OK;
 }
 
+int osb_probe(osb_ctx* c, int n, const int32_t* ijk, const int32_t* var, double* out) {
+    if (!c || n < 0 || (n > 0 && (!ijk || !var || !out))) return OSB_E_INVALID;
+    for (int p = 0; p < n; ++p) {
+        for (int d = 0; d < 3; ++d)
+            if (ijk[3 * p + d] < 0 || ijk[3 * p + d] >= c->n[d]) return fail(c, OSB_E_INVALID, "probe outside the grid");
+        if (var[p] < 0 || var[p] >= c->nv) return fail(c, OSB_E_INVALID, "probe variable out of range");
+        out[p] = c->q[(size_t)var[p] * c->Ptot + c->I(ijk[3 * p], ijk[3 * p + 1], ijk[3 * p + 2])];
+    }
+    return OSB_OK;
+}
+
 int osb_nan_check(osb_ctx* c) {
     if (!c) return OSB_E_INVALID;
     for (int k = 0; k < c->n[2]; ++k)

@@ -266,3 +266,15 @@ def test_stage_by_stage_equals_step(hip, oracle):
                     o.stage(s)
             qa, qb, qo = a.get_state(), b.get_state(), o.get_state()
         assert np.array_equal(qa, qb) and np.array_equal(qb, qo)
+
+
+def test_probes_on_the_gpu(hip):
+    pr = P.tgv(n=16, steps=2)
+    _announce("probes tgv16")
+    with hip.context(pr.desc) as c:
+        c.set_state(pr.q0)
+        c.step(2)
+        q = c.get_state()
+        pts = [(0, 0, 0), (15, 3, 7), (8, 15, 1)]
+        v = c.probe(pts, [0, 4, 2])
+    assert list(v) == [q[0, 0, 0, 0], q[4, 7, 3, 15], q[2, 1, 15, 8]]

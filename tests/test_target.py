@@ -295,3 +295,47 @@ def test_initial_condition_unknown_names_raise(oracle):
     with pytest.raises(NotImplementedError, match="q_unknown"):
         t.initialise()
     t.close()
+
+
+# --------------------------------------------------------------------------------
+# SimulationMonitor probes (simulation_monitors.py:40-265)
+# --------------------------------------------------------------------------------
+def test_simulation_monitor_lines(oracle, tmp_path):
+    """Probes printed after the first iteration and every print_frequency, with
+    the generated program's header and %.<p>e format; each value is the state
+    at that iteration."""
+    from opensbli_amd.surface import SimulationMonitor
+    block = SimulationBlock(2, block_number=0)
+    block.set_block_boundaries([PeriodicBC(0, 0), PeriodicBC(0, 1), InviscidWallBC(1, 0), InviscidWallBC(1, 1)])
+    weno = LFWeno(5, formulation="Z", averaging=RoeAverage([0, 1]))
+    block.set_discretisation_schemes({"weno": weno, "rk": RungeKuttaLS(3)})
+    SM = SimulationMonitor(["rho_B0", "rhou1"], [(3, 4), (8, 8)], block, print_frequency=5, fp_precision=10,
+                           NaN_check="rho")
+    alg = TraditionalAlgorithmRK(block, simulation_monitor=SM)
+    pr = P.kelvin_helmholtz(n=16)
+    params = {"gama": "1.4", "dt": repr(pr.desc.dt), "niter": "12", "block0np0": "16", "block0np1": "16",
+              "Delta0block0": repr(pr.desc.delta[0]), "Delta1block0": repr(pr.desc.delta[1])}
+    t = HIPTarget(alg, parameters=params, library=oracle, output_dir=str(tmp_path))
+    t.initialise(pr.q0).run()
+    t.close()
+    lines = open(tmp_path / "block0_output.log").read().splitlines()
+    assert lines[0] == "Iteration, Time, rho_B0(3, 4), rhou1_B0(8, 8)"
+    assert [ln.split(", ")[0] for ln in lines[1:]] == ["1", "5", "10"]
+    for ln in lines[1:]:
+        k = int(ln.split(", ")[0])
+        q, st = run(oracle, pr, k)
+        want = [f"{st.time:.10e}", f"{q[0, 0, 4, 3]:.10e}", f"{q[2, 0, 8, 8]:.10e}"]
+        assert ln.split(", ")[1:] == want, (ln, want)
+
+
+def test_probe_bounds_checked(oracle):
+    pr = P.kelvin_helmholtz(n=16)
+    with oracle.context(pr.desc) as c:
+        c.set_state(pr.q0)
+        q = c.get_state()                    # the wall rows hold the BC's boundary points
+        v = c.probe([(0, 0), (15, 15), (7, 3)], [0, 3, 2])
+        assert list(v) == [q[0, 0, 0, 0], q[3, 0, 15, 15], q[2, 0, 3, 7]]
+        with pytest.raises(nat.OSBError):
+            c.probe([(16, 0)], [0])
+        with pytest.raises(nat.OSBError):
+            c.probe([(1, 1)], [4])
