@@ -190,11 +190,17 @@ def serving_bench(count=64, rate=50.0, seed=20261203, budget_gib=8.0, max_solves
     hip = nat.load_hip()
     reqs = S.make_stream(count=count, rate=rate, seed=seed)
     rep = S.serve(hip, reqs, int(budget_gib * 2**30), max_solves=max_solves, nstreams=4, chunk=chunk)
-    return {"workload": f"serving_{count}", "requests": count, "rate_per_s": rate, "seed": seed,
-            "hbm_budget_gib": budget_gib, "max_solves": max_solves, "chunk_steps": chunk,
-            "p50_time_to_solution_s": rep.p50, "p99_time_to_solution_s": rep.p99,
-            "aggregate_gpts_stage_per_s": rep.throughput, "makespan_s": rep.makespan,
-            "peak_running": rep.peak_running, "peak_device_gib": rep.peak_bytes / 2**30}
+    out = {"workload": f"serving_{count}", "requests": count, "rate_per_s": rate, "seed": seed,
+           "hbm_budget_gib": budget_gib, "max_solves": max_solves, "chunk_steps": chunk, "policy": "fifo",
+           "p50_time_to_solution_s": rep.p50, "p99_time_to_solution_s": rep.p99,
+           "aggregate_gpts_stage_per_s": rep.throughput, "makespan_s": rep.makespan,
+           "peak_running": rep.peak_running, "peak_device_gib": rep.peak_bytes / 2**30}
+    # the same stream admitted least-work-first
+    reqs = S.make_stream(count=count, rate=rate, seed=seed)
+    sj = S.serve(hip, reqs, int(budget_gib * 2**30), max_solves=max_solves, nstreams=4, chunk=chunk, policy="sjf")
+    out["sjf"] = {"p50_time_to_solution_s": sj.p50, "p99_time_to_solution_s": sj.p99,
+                  "aggregate_gpts_stage_per_s": sj.throughput, "makespan_s": sj.makespan}
+    return out
 
 
 def mixed_headline(problem_name, steps, warmup):

@@ -94,9 +94,18 @@ class ServeReport:
 
 
 def serve(lib: nat.Library, requests: List[Request], hbm_budget_bytes: int, max_solves: int = 32,
-          nstreams: int = 4, chunk: int = 25, keep_results: bool = False, realtime: bool = True) -> ServeReport:
+          nstreams: int = 4, chunk: int = 25, keep_results: bool = False, realtime: bool = True,
+          policy: str = "fifo") -> ServeReport:
     """Run the stream through an osb_pool.  With realtime=True a request can be
-    admitted only once its arrival time has passed on the wall clock."""
+    admitted only once its arrival time has passed on the wall clock.
+
+    ``policy`` orders the waiting requests for admission: "fifo" (arrival
+    order) or "sjf" (least work first -- grid-point RK-stage updates -- which
+    lowers the median time-to-solution when the queue is long).  A request that
+    does not fit the remaining budget does not block smaller ones behind it
+    under "sjf"."""
+    if policy not in ("fifo", "sjf"):
+        raise ValueError(f"unknown admission policy {policy!r}")
     pending = sorted(requests, key=lambda r: r.arrival)
     waiting: List[Request] = []
     running = {}
@@ -109,16 +118,22 @@ def serve(lib: nat.Library, requests: List[Request], hbm_budget_bytes: int, max_
             now = time.perf_counter() - t0
             while pending and (not realtime or pending[0].arrival <= now):
                 waiting.append(pending.pop(0))
-            # admit in arrival order while slots and memory allow
-            while waiting:
-                r = waiting[0]
+            # admit in policy order while slots and memory allow
+            if policy == "sjf":
+                waiting.sort(key=lambda r: (r.work, r.arrival))
+            k = 0
+            while k < len(waiting):
+                r = waiting[k]
                 slot = pool.admit(r.problem.desc, r.problem.q0, r.steps)
                 if slot is None:
-                    break
+                    if policy == "fifo" or len(running) + k >= max_solves:
+                        break
+                    k += 1                           # too big for the budget left: try a smaller one
+                    continue
                 r.slot = slot
                 r.admitted = time.perf_counter() - t0
                 running[slot] = r
-                waiting.pop(0)
+                waiting.pop(k)
             used, nrun, _ = pool.usage()
             peak_running = max(peak_running, nrun)
             peak_bytes = max(peak_bytes, used)

@@ -126,15 +126,17 @@ def test_mixed_precision_error_vs_fp64_oracle(hip, oracle):
         assert l2.max() <= 1e-5, l2
 
 
-def test_pool_results_same_alone_and_in_the_mix(hip):
+@pytest.mark.parametrize("policy", ["fifo", "sjf"])
+def test_pool_results_same_alone_and_in_the_mix(hip, policy):
     """Requests of both kinds through the pool with a budget that forces
-    queueing and joining mid-run; each result equals the same solve alone."""
+    queueing and joining mid-run; each result equals the same solve alone
+    (arrival-order and least-work-first admission)."""
     from opensbli_amd import serving as S
     reqs = S.make_stream(count=10, rate=400.0, seed=7, riemann_sizes=(32, 48, 64), tgv_sizes=(16, 24),
                          steps_range=(10, 40))
     budget = max(hip.estimate_bytes(r.problem.desc) for r in reqs) * 3
     _announce(f"pool: {len(reqs)} requests, budget {budget} B")
-    rep = S.serve(hip, reqs, budget, max_solves=4, nstreams=2, chunk=7, keep_results=True)
+    rep = S.serve(hip, reqs, budget, max_solves=4, nstreams=2, chunk=7, keep_results=True, policy=policy)
     print(f"   p50 {rep.p50 * 1e3:.1f} ms p99 {rep.p99 * 1e3:.1f} ms, peak running {rep.peak_running}", flush=True)
     assert len(rep.requests) == len(reqs)
     assert rep.peak_running >= 2
