@@ -45,6 +45,7 @@ extern "C" {
 #define OSB_CONV_TENO              2  /* LFTeno(order=5|6)                          teno.py:480   */
 #define OSB_CONV_CENTRAL_FEIEREISEN 3 /* NS_Split('Feiereisen') + StoreSome(4)      split_forms.py:111 */
 #define OSB_CONV_CENTRAL_SKEW      4  /* Skew(...) convective terms + Central(4)    parsing.py:96 */
+#define OSB_CONV_CENTRAL_KGP       5  /* NS_Split('KGP', energy_formulation='enthalpy') split_forms.py:178-234 */
 
 /* characteristic averaging (opensbli/schemes/spatial/averaging.py) */
 #define OSB_AVG_SIMPLE 1              /* SimpleAverage  averaging.py:31 */

@@ -254,8 +254,17 @@ def plan_problem(alg_or_block, parameters: Mapping, physics=None, precision: str
             desc["conv_scheme"] = nat.CONV_CENTRAL_FEIEREISEN
         elif form in ("Skew", "skew"):
             desc["conv_scheme"] = nat.CONV_CENTRAL_SKEW
+        elif form == "KGP":
+            if isinstance(physics, Mapping):
+                ef = physics.get("energy_formulation", "none")
+            else:        # the mirror keeps it on NS_Split, the reference on its split object
+                ef = getattr(physics, "energy_formulation", None) or \
+                    getattr(getattr(physics, "split", None), "energy_formulation", "none")
+            if ef != "enthalpy":
+                raise NotImplementedError(f"NS_Split('KGP', energy_formulation={ef!r}): the enthalpy formulation is implemented")
+            desc["conv_scheme"] = nat.CONV_CENTRAL_KGP
         else:
-            raise NotImplementedError(f"convective split form {form!r}: Feiereisen and Skew are implemented")
+            raise NotImplementedError(f"convective split form {form!r}: Feiereisen, KGP (enthalpy) and Skew are implemented")
         desc.update(order=4, viscous=_viscous_model(physics))
 
     tk = _kinds(time)

@@ -157,9 +157,11 @@ void rhs_nd(osb_ctx* c, const double* q) {
     if (c->central) {
         Timed tm(c, PF_CENTRAL);
         if (c->d.conv_scheme == OSB_CONV_CENTRAL_SKEW)
-            k_central_convective<NDIM, true><<<nblocks(nin), BLOCK, 0, c->stream>>>(q, c->cr, c->gr, c->R, c->g, c->c);
+            k_central_convective<NDIM, CC_SKEW><<<nblocks(nin), BLOCK, 0, c->stream>>>(q, c->cr, c->gr, c->R, c->g, c->c);
+        else if (c->d.conv_scheme == OSB_CONV_CENTRAL_KGP)
+            k_central_convective<NDIM, CC_KGP><<<nblocks(nin), BLOCK, 0, c->stream>>>(q, c->cr, c->gr, c->R, c->g, c->c);
         else
-            k_central_convective<NDIM, false><<<nblocks(nin), BLOCK, 0, c->stream>>>(q, c->cr, c->gr, c->R, c->g, c->c);
+            k_central_convective<NDIM, CC_FEIEREISEN><<<nblocks(nin), BLOCK, 0, c->stream>>>(q, c->cr, c->gr, c->R, c->g, c->c);
     }
     if (c->viscous) {
         Timed tm(c, PF_VISCOUS);
@@ -353,7 +355,10 @@ void fused_stage(osb_ctx* c, const double* q, double* qn, double A, double B) {
     else launch_fused_v<CF_FEIEREISEN, false, TIME>(c, q, qn, A, B);
 }
 
-bool use_fused(const osb_ctx* c) { return c->g.ndim == 3 && (c->central || c->viscous); }
+// the fused 3-D stage kernel implements the Feiereisen and skew forms; KGP runs the separate kernels
+bool use_fused(const osb_ctx* c) {
+    return c->g.ndim == 3 && (c->central || c->viscous) && c->d.conv_scheme != OSB_CONV_CENTRAL_KGP;
+}
 bool use_shock_stage(const osb_ctx* c) { return c->shock && !c->central && !c->viscous; }
 
 int check_launch(osb_ctx* c) {
@@ -418,10 +423,12 @@ int validate(const osb_problem_desc* d, std::string& why) {
         if (i >= d->ndim && d->n[i] != 1) { why = "inactive directions must have n == 1"; return OSB_E_INVALID; }
     }
     const int cs = d->conv_scheme;
-    if (cs != OSB_CONV_WENO_Z && cs != OSB_CONV_TENO && cs != OSB_CONV_CENTRAL_FEIEREISEN && cs != OSB_CONV_CENTRAL_SKEW) { why = "unknown conv_scheme"; return OSB_E_UNSUPPORTED; }
+    const bool central_cs = cs == OSB_CONV_CENTRAL_FEIEREISEN || cs == OSB_CONV_CENTRAL_SKEW || cs == OSB_CONV_CENTRAL_KGP;
+    if (cs != OSB_CONV_WENO_Z && cs != OSB_CONV_TENO && !central_cs) { why = "unknown conv_scheme"; return OSB_E_UNSUPPORTED; }
+    if (cs == OSB_CONV_CENTRAL_KGP && d->precision == OSB_PREC_MIXED) { why = "mixed precision is not built for the KGP split form"; return OSB_E_UNSUPPORTED; }
     if (cs == OSB_CONV_WENO_Z && d->order != 5) { why = "WENO-Z supports order 5"; return OSB_E_UNSUPPORTED; }
     if (cs == OSB_CONV_TENO && d->order != 5 && d->order != 6) { why = "TENO supports orders 5 and 6"; return OSB_E_UNSUPPORTED; }
-    if ((cs == OSB_CONV_CENTRAL_FEIEREISEN || cs == OSB_CONV_CENTRAL_SKEW) && d->order != 4) { why = "central scheme supports order 4"; return OSB_E_UNSUPPORTED; }
+    if (central_cs && d->order != 4) { why = "central scheme supports order 4"; return OSB_E_UNSUPPORTED; }
     if ((cs == OSB_CONV_WENO_Z || cs == OSB_CONV_TENO) && d->averaging != OSB_AVG_SIMPLE && d->averaging != OSB_AVG_ROE) { why = "unknown averaging"; return OSB_E_UNSUPPORTED; }
     if (d->viscous != OSB_VISC_NONE && d->viscous != OSB_VISC_CONSTANT && d->viscous != OSB_VISC_SUTHERLAND) { why = "unknown viscous model"; return OSB_E_UNSUPPORTED; }
     if (d->viscous != OSB_VISC_NONE && d->ndim < 2) { why = "viscous terms need ndim >= 2"; return OSB_E_UNSUPPORTED; }
@@ -437,7 +444,7 @@ int validate(const osb_problem_desc* d, std::string& why) {
         for (int s = 0; s < 2; ++s) {
             const int b = d->bc[i][s];
             if (b < OSB_BC_PERIODIC || b > OSB_BC_DIRICHLET) { why = "unknown boundary condition"; return OSB_E_UNSUPPORTED; }
-            if (b >= OSB_BC_EXTRAPOLATION0 && (cs == OSB_CONV_CENTRAL_FEIEREISEN || cs == OSB_CONV_CENTRAL_SKEW || d->viscous != OSB_VISC_NONE)) {
+            if (b >= OSB_BC_EXTRAPOLATION0 && (central_cs || d->viscous != OSB_VISC_NONE)) {
                 why = "extrapolation/Dirichlet boundaries with central derivatives need the Carpenter closure (Carpenter_scheme.py), not built";
                 return OSB_E_UNSUPPORTED;
             }
@@ -500,7 +507,7 @@ AllocPlan plan_alloc(const osb_problem_desc* d) {
     const int nd = d->ndim, nv = nd + 2;
     const bool shock = d->conv_scheme == OSB_CONV_WENO_Z || d->conv_scheme == OSB_CONV_TENO;
     const bool central = !shock, viscous = d->viscous != OSB_VISC_NONE;
-    const bool fused = nd == 3 && (central || viscous);
+    const bool fused = nd == 3 && (central || viscous) && d->conv_scheme != OSB_CONV_CENTRAL_KGP;
     const bool unfused_central = (central || viscous) && !fused;
     const size_t P = (size_t)g.Ptot;
     AllocPlan a;

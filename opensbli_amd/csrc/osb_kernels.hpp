@@ -411,7 +411,9 @@ __global__ void __launch_bounds__(BLOCK) k_gradients(const double* __restrict__ 
 // SKEW = false: Feiereisen form (split_forms.py:111-144) with StoreSome gradients.
 // SKEW = true : Skew(ab) = 1/2(D(ab) + b D(a) + a D(b)) terms of the tg_sym app
 //               (parsing.py:96-112; verify_tg_sym.py:32-36).
-template <int NDIM, bool SKEW>
+enum CcForm { CC_FEIEREISEN = 0, CC_SKEW = 1, CC_KGP = 2 };
+
+template <int NDIM, int FORM>
 __global__ void __launch_bounds__(BLOCK) k_central_convective(const double* __restrict__ q, const double* __restrict__ cr,
                                                               const double* __restrict__ gr, double* __restrict__ R, Geom g, Consts c) {
     constexpr int NV = NDIM + 2;
@@ -437,7 +439,35 @@ __global__ void __launch_bounds__(BLOCK) k_central_convective(const double* __re
         const double* rhou_j = q + (long)(1 + jd) * P;
         const double* u_j = cr + (long)jd * P;
         auto at = [&](const double* a, int s) { return a[x + s * st]; };
-        if (!SKEW) {
+        if (FORM == CC_KGP) {
+            // KGP, enthalpy splitting, all coefficients 1/4 (split_forms.py:178-234): the
+            // same term sequence as the CPU oracle; H = (rhoE + p)/rho at each stencil point
+            const double ujc = at(u_j, 0), rhoujc = at(rhou_j, 0);
+            const double gjj = gr[(long)(jd * NDIM + jd) * P + x];
+            auto Hs = [&](int s) { return (at(rhoE, s) + at(pp, s)) / at(rho, s); };
+            const double d_rhouj = d1([&](int s) { return at(rhou_j, s); }, inv);
+            const double d_rho = d1([&](int s) { return at(rho, s); }, inv);
+            Rrho += 0.5 * d_rhouj + 0.5 * (ujc * d_rho + rho[x] * gjj);
+#pragma unroll
+            for (int m = 0; m < NDIM; ++m) {
+                const double* rhou_m = q + (long)(1 + m) * P;
+                const double* u_m = cr + (long)m * P;
+                const double umc = at(u_m, 0), rhoumc = at(rhou_m, 0);
+                const double g_mj = gr[(long)(m * NDIM + jd) * P + x];
+                const double d_rhouj_um = d1([&](int s) { return at(u_m, s) * at(rhou_j, s); }, inv);
+                const double d_rhoum = d1([&](int s) { return at(rhou_m, s); }, inv);
+                const double d_ujum = d1([&](int s) { return at(u_j, s) * at(u_m, s); }, inv);
+                conv[m] += 0.25 * ((((((d_rhouj_um + umc * d_rhouj) + rhoujc * g_mj) + ujc * d_rhoum) + rhoumc * gjj) +
+                                    rho[x] * d_ujum) + (umc * ujc) * d_rho);
+            }
+            const double Hc = Hs(0);
+            const double d_Hrhouj = d1([&](int s) { return Hs(s) * at(rhou_j, s); }, inv);
+            const double d_H = d1(Hs, inv);
+            const double d_Hrho = d1([&](int s) { return Hs(s) * at(rho, s); }, inv);
+            const double d_Huj = d1([&](int s) { return Hs(s) * at(u_j, s); }, inv);
+            eterm += 0.25 * ((((((d_Hrhouj + Hc * d_rhouj) + rhoujc * d_H) + ujc * d_Hrho) + (rho[x] * Hc) * gjj) +
+                              rho[x] * d_Huj) + (ujc * Hc) * d_rho);
+        } else if (FORM == CC_FEIEREISEN) {
             const double d_rhouj = d1([&](int s) { return at(rhou_j, s); }, inv);
             div_rhou += d_rhouj;
 #pragma unroll
@@ -474,11 +504,14 @@ __global__ void __launch_bounds__(BLOCK) k_central_convective(const double* __re
         const long st = g.s[m];
         const double dp = d1([&](int s) { return pp[x + s * st]; }, c.invd[m]);
         double rm;
-        if (!SKEW) rm = -dp - 0.5 * (conv[m] + cr[(long)m * P + x] * div_rhou);
+        if (FORM == CC_FEIEREISEN) rm = -dp - 0.5 * (conv[m] + cr[(long)m * P + x] * div_rhou);
         else rm = -dp - conv[m];
         R[(long)(1 + m) * P + x] = rm;
     }
-    if (!SKEW) {
+    if (FORM == CC_KGP) {
+        R[x] = -Rrho;
+        R[(long)(NV - 1) * P + x] = -eterm;
+    } else if (FORM == CC_FEIEREISEN) {
         R[x] = -div_rhou;
         R[(long)(NV - 1) * P + x] = eterm - 0.5 * (rhoE[x] / rho[x]) * div_rhou;
     } else {

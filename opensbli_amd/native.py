@@ -32,6 +32,7 @@ CONV_WENO_Z = 1
 CONV_TENO = 2
 CONV_CENTRAL_FEIEREISEN = 3
 CONV_CENTRAL_SKEW = 4
+CONV_CENTRAL_KGP = 5
 
 AVG_SIMPLE = 1
 AVG_ROE = 2

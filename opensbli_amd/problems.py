@@ -118,6 +118,7 @@ def tgv(n: int = 128, steps: int = 200, dt: Optional[float] = None, minf: float 
     """Periodic [0, 2pi)^3 Taylor-Green vortex.
 
     scheme='central': NS_Split('Feiereisen', viscosity='constant') + StoreSome(4, 'u0 u1 u2')
+    scheme='kgp'    : NS_Split('KGP', viscosity='constant', energy_formulation='enthalpy') + StoreSome(4)
     scheme='teno'   : LFTeno(order, averaging=RoeAverage) + central Sutherland viscous terms
     """
     gama = 1.4
@@ -127,6 +128,8 @@ def tgv(n: int = 128, steps: int = 200, dt: Optional[float] = None, minf: float 
     q = _tgv_fields((n, n, n), (delta, delta, delta), gama, minf)
     if scheme == "central":
         conv, order, visc = nat.CONV_CENTRAL_FEIEREISEN, 4, nat.VISC_CONSTANT
+    elif scheme == "kgp":
+        conv, order, visc = nat.CONV_CENTRAL_KGP, 4, nat.VISC_CONSTANT
     elif scheme == "teno":
         conv, order, visc = nat.CONV_TENO, teno_order, nat.VISC_SUTHERLAND
     elif scheme == "weno":

@@ -542,6 +542,7 @@ void central_convective(osb_ctx* c) {
     const int nd = c->ndim;
     const size_t Pt = c->Ptot;
     const bool skew = c->d.conv_scheme == OSB_CONV_CENTRAL_SKEW;
+    const bool kgp = c->d.conv_scheme == OSB_CONV_CENTRAL_KGP;
     Range r = make_range(c, 0);
     const double* q = c->q.data();
     for_range(r, [&](int i, int j, int k) {
@@ -559,7 +560,36 @@ void central_convective(osb_ctx* c) {
             const double* u_j = c->u.data() + (size_t)jd * Pt;
             const double* rhoEa = q + (size_t)(nd + 1) * Pt;
             const double* pa = c->p.data();
-            if (!skew) {
+            if (kgp) {
+                // KGP with enthalpy splitting, alpha = beta = gamma = delta = 1/4
+                // (split_forms.py:178-234; generated form in
+                // apps/performance_benchmark/Central4-WENO5/opensbliblock00_kernels.h:157-382):
+                // H = (rhoE + p)/rho, D_j u_j from the stored gradients
+                const double ujc = at(u_j, 0), rhoujc = at(rhou_j, 0);
+                const double gjj = c->g[(size_t)(jd * nd + jd) * Pt + x];
+                auto Hs = [&](int s) { return (at(rhoEa, s) + at(pa, s)) / at(q, s); };
+                const double d_rhouj = d1([&](int s) { return at(rhou_j, s); }, invd);
+                const double d_rho = d1([&](int s) { return at(q, s); }, invd);
+                Rrho += 0.5 * d_rhouj + 0.5 * (ujc * d_rho + rho * gjj);
+                for (int m = 0; m < nd; ++m) {
+                    const double* rhou_m = q + (size_t)(1 + m) * Pt;
+                    const double* u_m = c->u.data() + (size_t)m * Pt;
+                    const double umc = at(u_m, 0), rhoumc = at(rhou_m, 0);
+                    const double g_mj = c->g[(size_t)(m * nd + jd) * Pt + x];
+                    const double d_rhouj_um = d1([&](int s) { return at(u_m, s) * at(rhou_j, s); }, invd);
+                    const double d_rhoum = d1([&](int s) { return at(rhou_m, s); }, invd);
+                    const double d_ujum = d1([&](int s) { return at(u_j, s) * at(u_m, s); }, invd);
+                    sum_conv[m] += 0.25 * ((((((d_rhouj_um + umc * d_rhouj) + rhoujc * g_mj) + ujc * d_rhoum) + rhoumc * gjj) +
+                                            rho * d_ujum) + (umc * ujc) * d_rho);
+                }
+                const double Hc = Hs(0);
+                const double d_Hrhouj = d1([&](int s) { return Hs(s) * at(rhou_j, s); }, invd);
+                const double d_H = d1(Hs, invd);
+                const double d_Hrho = d1([&](int s) { return Hs(s) * at(q, s); }, invd);
+                const double d_Huj = d1([&](int s) { return Hs(s) * at(u_j, s); }, invd);
+                eterm += 0.25 * ((((((d_Hrhouj + Hc * d_rhouj) + rhoujc * d_H) + ujc * d_Hrho) + (rho * Hc) * gjj) + rho * d_Huj) +
+                                 (ujc * Hc) * d_rho);
+            } else if (!skew) {
                 // Feiereisen (split_forms.py:111-144)
                 const double d_rhouj = d1([&](int s) { return at(rhou_j, s); }, invd);
                 div_rhou += d_rhouj;
@@ -598,14 +628,17 @@ void central_convective(osb_ctx* c) {
             const double invd = 1.0 / c->d.delta[m];
             const double* pa = c->p.data();
             const double dp = d1([&](int s) { return pa[x + (long)s * (long)st]; }, invd);
-            if (!skew) {
+            if (!skew && !kgp) {
                 const double um = c->u[(size_t)m * Pt + x];
                 Rm[m] = -dp - 0.5 * (sum_conv[m] + um * div_rhou);
             } else {
                 Rm[m] = -dp - sum_conv[m];
             }
         }
-        if (!skew) {
+        if (kgp) {
+            Rrho = -Rrho;
+            RE = -eterm;
+        } else if (!skew) {
             Rrho = -div_rhou;
             RE = eterm - 0.5 * (rhoE / rho) * div_rhou;
         } else {
@@ -876,10 +909,11 @@ int validate(const osb_problem_desc* d, std::string& why) {
         if (i >= d->ndim && d->n[i] != 1) { why = "inactive directions must have n == 1"; return OSB_E_INVALID; }
     }
     const int cs = d->conv_scheme;
-    if (cs != OSB_CONV_WENO_Z && cs != OSB_CONV_TENO && cs != OSB_CONV_CENTRAL_FEIEREISEN && cs != OSB_CONV_CENTRAL_SKEW) { why = "unknown conv_scheme"; return OSB_E_UNSUPPORTED; }
+    const bool central_cs = cs == OSB_CONV_CENTRAL_FEIEREISEN || cs == OSB_CONV_CENTRAL_SKEW || cs == OSB_CONV_CENTRAL_KGP;
+    if (cs != OSB_CONV_WENO_Z && cs != OSB_CONV_TENO && !central_cs) { why = "unknown conv_scheme"; return OSB_E_UNSUPPORTED; }
     if (cs == OSB_CONV_WENO_Z && d->order != 5) { why = "WENO-Z supports order 5"; return OSB_E_UNSUPPORTED; }
     if (cs == OSB_CONV_TENO && d->order != 5 && d->order != 6) { why = "TENO supports orders 5 and 6"; return OSB_E_UNSUPPORTED; }
-    if ((cs == OSB_CONV_CENTRAL_FEIEREISEN || cs == OSB_CONV_CENTRAL_SKEW) && d->order != 4) { why = "central scheme supports order 4"; return OSB_E_UNSUPPORTED; }
+    if (central_cs && d->order != 4) { why = "central scheme supports order 4"; return OSB_E_UNSUPPORTED; }
     if ((cs == OSB_CONV_WENO_Z || cs == OSB_CONV_TENO) && d->averaging != OSB_AVG_SIMPLE && d->averaging != OSB_AVG_ROE) { why = "unknown averaging"; return OSB_E_UNSUPPORTED; }
     if (d->viscous != OSB_VISC_NONE && d->viscous != OSB_VISC_CONSTANT && d->viscous != OSB_VISC_SUTHERLAND) { why = "unknown viscous model"; return OSB_E_UNSUPPORTED; }
     if (d->viscous != OSB_VISC_NONE && d->ndim < 2) { why = "viscous terms need ndim >= 2"; return OSB_E_UNSUPPORTED; }
@@ -895,7 +929,7 @@ int validate(const osb_problem_desc* d, std::string& why) {
         for (int s = 0; s < 2; ++s) {
             const int b = d->bc[i][s];
             if (b < OSB_BC_PERIODIC || b > OSB_BC_DIRICHLET) { why = "unknown boundary condition"; return OSB_E_UNSUPPORTED; }
-            if (b >= OSB_BC_EXTRAPOLATION0 && (cs == OSB_CONV_CENTRAL_FEIEREISEN || cs == OSB_CONV_CENTRAL_SKEW || d->viscous != OSB_VISC_NONE)) {
+            if (b >= OSB_BC_EXTRAPOLATION0 && (central_cs || d->viscous != OSB_VISC_NONE)) {
                 why = "extrapolation/Dirichlet boundaries with central derivatives need the Carpenter closure (Carpenter_scheme.py), not built";
                 return OSB_E_UNSUPPORTED;
             }
@@ -908,7 +942,7 @@ int validate(const osb_problem_desc* d, std::string& why) {
     if (!(d->cfl > 0) && !(d->dt > 0)) { why = "dt > 0 or cfl > 0 required"; return OSB_E_INVALID; }
     if (!(d->gama > 1.0)) { why = "gama > 1 required"; return OSB_E_INVALID; }
     if (d->viscous != OSB_VISC_NONE && (!(d->re > 0) || !(d->pr > 0) || !(d->minf > 0))) { why = "Re, Pr, Minf > 0 required"; return OSB_E_INVALID; }
-    if (cs == OSB_CONV_CENTRAL_FEIEREISEN || cs == OSB_CONV_CENTRAL_SKEW) {
+    if (central_cs) {
         if (d->ndim < 2) { why = "central split forms need ndim >= 2"; return OSB_E_UNSUPPORTED; }
     }
     return OSB_OK;

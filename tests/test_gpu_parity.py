@@ -73,6 +73,29 @@ def test_residual_tgv_teno6(hip, oracle):
     _residual_compare(hip, oracle, P.tgv_teno6_validation(n=24))
 
 
+def test_residual_tgv_kgp(hip, oracle):
+    _residual_compare(hip, oracle, P.tgv(n=24, scheme="kgp"))
+
+
+def test_kgp_runs(hip, oracle):
+    """KGP (enthalpy) with constant-viscosity terms: the separate central kernels in 3-D and 2-D."""
+    a, b = _compare(hip, oracle, P.tgv(n=24, steps=5, scheme="kgp"))
+    assert np.array_equal(a, b)
+    # a smooth periodic 2-D field with density and pressure variation
+    n, gama = 40, 1.4
+    h = 2 * math.pi / n
+    X, Y = np.meshgrid(h * np.arange(n), h * np.arange(n), indexing="xy")
+    rho = 1.0 + 0.2 * np.sin(X) * np.cos(2 * Y)
+    u, v = 0.3 * np.sin(X + 0.5) * np.cos(Y), -0.2 * np.cos(X) * np.sin(Y - 0.3)
+    p = 1.0 + 0.1 * np.cos(X - Y)
+    q = np.stack([rho, rho * u, rho * v, p / (gama - 1.0) + 0.5 * rho * (u * u + v * v)])[:, None]
+    desc = nat.ProblemDesc(ndim=2, n=[n, n], delta=[h, h], conv_scheme=nat.CONV_CENTRAL_KGP, order=4,
+                           viscous=nat.VISC_CONSTANT, re=100.0, minf=0.3, bc=[[nat.BC_PERIODIC] * 2] * 2,
+                           gama=gama, dt=2e-3)
+    a, b = _compare(hip, oracle, P.Problem("kgp_2d", desc, q, 20))
+    assert np.array_equal(a, b)
+
+
 def test_residual_tg_sym(hip, oracle):
     _residual_compare(hip, oracle, P.tg_sym())
 

@@ -418,6 +418,7 @@ def test_sod_glf_against_exact_solution(oracle, scheme):
     lambda: P.tgv(n=12, scheme="teno", minf=1.25),
     lambda: P.tgv_teno6_validation(n=12),
     lambda: P.tgv(n=12),
+    lambda: P.tgv(n=12, scheme="kgp"),
     lambda: P.tg_sym(n=12),
 ])
 def test_free_stream_preserved_exactly(oracle, maker):
@@ -523,3 +524,49 @@ def test_teno6_readme_validation(oracle):
     assert f"{st.time:.5e}" == "8.46250e-01"
     for v, r in zip(d, [2.65363e+05, 1.16991e+05, 1.79666e+06, 2.09715e+06]):
         assert f"{v:.5e}" == f"{r:.5e}"
+
+
+# --------------------------------------------------------------------------------
+# a6: KGP split form (split_forms.py:178-234, enthalpy formulation)
+# --------------------------------------------------------------------------------
+def _smooth_periodic_2d(n, conv):
+    gama = 1.4
+    h = 2 * math.pi / n
+    x = h * np.arange(n)
+    X, Y = np.meshgrid(x, x, indexing="xy")
+    rho = 1.0 + 0.2 * np.sin(X) * np.cos(2 * Y)
+    u = 0.3 * np.sin(X + 0.5) * np.cos(Y)
+    v = -0.2 * np.cos(X) * np.sin(Y - 0.3)
+    p = 1.0 + 0.1 * np.cos(X - Y)
+    q = np.stack([rho, rho * u, rho * v, p / (gama - 1.0) + 0.5 * rho * (u * u + v * v)])[:, None]
+    desc = nat.ProblemDesc(ndim=2, n=[n, n], delta=[h, h], conv_scheme=conv, order=4,
+                           bc=[[nat.BC_PERIODIC] * 2] * 2, gama=gama, dt=1e-3)
+    return desc, q
+
+
+def test_kgp_is_conservative(oracle):
+    """Every KGP product pair f D(g) + g D(f) telescopes on a periodic grid, so the
+    residual of each conservative variable sums to zero; a wrong coefficient or a
+    transposed gradient leaves an unpaired term."""
+    desc, q = _smooth_periodic_2d(24, nat.CONV_CENTRAL_KGP)
+    with oracle.context(desc) as c:
+        c.set_state(q)
+        R = c.residual()
+    for v in range(4):
+        assert abs(R[v].sum()) <= 1e-13 * np.abs(R[v]).sum(), (v, R[v].sum(), np.abs(R[v]).sum())
+    assert np.abs(R).max() > 1e-2
+
+
+def test_kgp_converges_to_the_divergence_form(oracle):
+    """KGP and the Feiereisen form discretise the same convective operator: their
+    residuals differ by O(h^4) on a smooth field."""
+    diffs = []
+    for n in (32, 64):
+        res = {}
+        for conv in (nat.CONV_CENTRAL_KGP, nat.CONV_CENTRAL_FEIEREISEN):
+            desc, q = _smooth_periodic_2d(n, conv)
+            with oracle.context(desc) as c:
+                c.set_state(q)
+                res[conv] = c.residual()
+        diffs.append(np.abs(res[nat.CONV_CENTRAL_KGP] - res[nat.CONV_CENTRAL_FEIEREISEN]).max())
+    assert diffs[0] / diffs[1] > 13, diffs          # 4th order: 16 in the limit

@@ -30,7 +30,7 @@ def _block(ndim, schemes, bcs):
 
 def _same(a: nat.ProblemDesc, b: nat.ProblemDesc):
     ca, cb = a.to_c(), b.to_c()
-    central = a.conv_scheme in (nat.CONV_CENTRAL_FEIEREISEN, nat.CONV_CENTRAL_SKEW)
+    central = a.conv_scheme in (nat.CONV_CENTRAL_FEIEREISEN, nat.CONV_CENTRAL_SKEW, nat.CONV_CENTRAL_KGP)
     for name, _ in nat.ProblemDescC._fields_:
         if central and name == "averaging":
             continue        # characteristic averaging applies to shock capturing only
@@ -339,3 +339,17 @@ def test_probe_bounds_checked(oracle):
             c.probe([(16, 0)], [0])
         with pytest.raises(nat.OSBError):
             c.probe([(1, 1)], [4])
+
+
+def test_kgp_plan():
+    """NS_Split('KGP', energy_formulation='enthalpy') (the Central4-WENO5 benchmark's
+    convective form) selects the KGP kernels; the 'none' formulation is refused."""
+    params = {"gama": "1.4", "Minf": "0.1", "Re": "1600.0", "Pr": "0.71", "dt": "0.003385", "niter": "2"}
+    for d in range(3):
+        params[f"block0np{d}"] = "16"
+        params[f"Delta{d}block0"] = f"2*M_PI/block0np{d}"
+    alg = _block(3, [StoreSome(4, "u0 u1 u2"), RungeKuttaLS(3)], [PeriodicBC(d, s) for d in range(3) for s in range(2)])
+    NS = NS_Split("KGP", 3, ["Re", "Pr", "gama", "Minf"], viscosity="constant", energy_formulation="enthalpy")
+    _same(plan_problem(alg, params, physics=NS), P.tgv(n=16, scheme="kgp", dt=0.003385).desc)
+    with pytest.raises(NotImplementedError, match="enthalpy"):
+        plan_problem(alg, params, physics=NS_Split("KGP", 3, [], viscosity="constant"))
