@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define OSB_ABI_VERSION 3
+#define OSB_ABI_VERSION 4
 
 /* error codes */
 #define OSB_OK            0
@@ -59,6 +59,12 @@ extern "C" {
 /* flux splitting of the shock-capturing schemes (LFWeno/LFTeno flux_type) */
 #define OSB_FLUX_LLF 0                /* local: lambda = max over {i, i+1}, shock_capturing.py:1000-1028 */
 #define OSB_FLUX_GLF 1                /* global: lambda = domain max per stage, shock_capturing.py:416-438, :838-857 */
+
+/* non-linear filter step after each full RK step (WENOFilter, filters/WENO_filter.py:406) */
+#define OSB_FILTER_NONE  0
+#define OSB_FILTER_WENOZ 1            /* WENOFilter(order=5, formulation='Z', flux_type='LLF'):
+                                         q += dt * (-div of the sensor-weighted dissipative
+                                         part of characteristic WENO-Z, RoeAverage)   */
 
 /* time integration */
 #define OSB_TIME_RKLS3 1              /* RungeKuttaLS  rk_LS.py:17 (2N-storage; see rk_stages) */
@@ -115,7 +121,8 @@ typedef struct osb_problem_desc {
     double   rk_b[5];          /* rkB: q  = q + rk_b[s]*du                          */
     double   bc_value[3][2][5];/* OSB_BC_DIRICHLET: conservative values per face,
                                   (rho, rhou0.., rhoE); unused entries ignored     */
-    int32_t  reserved[8];
+    int32_t  shock_filter;     /* OSB_FILTER_*                                     */
+    int32_t  reserved[7];
 } osb_problem_desc;
 
 typedef struct osb_stats {

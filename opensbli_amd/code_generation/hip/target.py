@@ -303,6 +303,24 @@ def plan_problem(alg_or_block, parameters: Mapping, physics=None, precision: str
                     "(Carpenter_scheme.py), which the HIP kernels do not implement")
     if any(v == 0 for pair in bc for v in pair):
         raise ValueError("every direction needs a boundary condition on both sides")
+
+    # non-linear filters among the equation classes (WENO_filter.py:406-443)
+    filters = [ec for ec in getattr(block, "list_of_equation_classes", []) if type(ec).__name__.endswith("Filter")]
+    for f in filters:
+        if type(f).__name__ != "WENOFilter":
+            raise NotImplementedError(f"{type(f).__name__} is not implemented by the HIP kernels")
+        if f.order != 5 or str(f.formulation).upper() != "Z" or f.flux_type != "LLF" or f.optimize \
+                or f.metrics is not None or f.species is not None:
+            raise NotImplementedError(
+                f"WENOFilter(order={f.order}, formulation={f.formulation!r}, flux_type={f.flux_type!r}, "
+                f"optimize={f.optimize}, metrics={f.metrics!r}): order 5, 'Z', 'LLF', optimize=False, "
+                "uniform mesh only")
+    if len(filters) > 1:
+        raise NotImplementedError("one WENO filter per block")
+    if filters:
+        if desc.get("flux_type") == nat.FLUX_GLF:
+            raise NotImplementedError("a WENO filter over a GLF base scheme: the kernels pair the filter with LLF")
+        desc["shock_filter"] = nat.FILTER_WENOZ
     desc["bc"] = bc
     desc["bc_value"] = bc_value
 

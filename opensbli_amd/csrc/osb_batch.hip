@@ -53,7 +53,7 @@ struct Group {
     int total(int f) const { return offs[(size_t)f * (members.size() + 1) + members.size()]; }
 };
 
-bool batchable(const osb_ctx* c) { return c->shock && !c->central && !c->viscous; }
+bool batchable(const osb_ctx* c) { return c->shock && !c->central && !c->viscous && c->d.shock_filter == OSB_FILTER_NONE; }
 
 inline unsigned vblocks(long work) { return (unsigned)((work + BLOCK - 1) / BLOCK); }
 

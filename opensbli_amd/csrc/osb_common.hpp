@@ -115,27 +115,36 @@ __device__ __forceinline__ void wenoz_weights(double d0, double d1, double d2, d
     }
 }
 
+// shock sensor of the WENO filter (weno.py:443-459, normalised):
+// sum_r |omega_r/d_r - 1| / (|1/min(d) - 1| + (k - 1)) = (...) / 11 for WENO5
+__device__ __forceinline__ double weno_sensor(double d0, double d1, double d2, double w0, double w1, double w2) {
+    return (fabs(w0 * (1.0 / d0) - 1.0) + fabs(w1 * (1.0 / d1) - 1.0) + fabs(w2 * (1.0 / d2) - 1.0)) / 11.0;
+}
+
 // WENO-Z, order 5 (weno.py:290-345): tau5 = |b0 - b2|,
-// alpha_r = d_r + d_r tau^2/((eps+b_r)(eps+b_r)), omega_r = alpha_r / sum(alpha)
+// alpha_r = d_r + d_r tau^2/((eps+b_r)(eps+b_r)), omega_r = alpha_r / sum(alpha).
+// sensor (FP64 only): the filter's shock sensor of these weights.
 template <typename T>
-__device__ __forceinline__ T wenoz_right(const T* f, double eps) {
+__device__ __forceinline__ T wenoz_right(const T* f, double eps, double* sensor = nullptr) {
     const double b0 = js_first(f[2], f[3], f[4]);     // stencil (i, i+1, i+2), d = 3/10
     const double b1 = js_centre(f[1], f[2], f[3]);    // stencil (i-1, i, i+1), d = 6/10
     const double b2 = js_last(f[0], f[1], f[2]);      // stencil (i-2, i-1, i), d = 1/10
     double w0, w1, w2;
     wenoz_weights<T>(3.0 / 10.0, 6.0 / 10.0, 1.0 / 10.0, b0, b1, b2, eps, w0, w1, w2);
+    if (sensor) *sensor = weno_sensor(3.0 / 10.0, 6.0 / 10.0, 1.0 / 10.0, w0, w1, w2);
     const T e0 = eno3<T>(T(1.0 / 3.0), T(5.0 / 6.0), T(-1.0 / 6.0), f[2], f[3], f[4]);
     const T e1 = eno3<T>(T(-1.0 / 6.0), T(5.0 / 6.0), T(1.0 / 3.0), f[1], f[2], f[3]);
     const T e2 = eno3<T>(T(1.0 / 3.0), T(-7.0 / 6.0), T(11.0 / 6.0), f[0], f[1], f[2]);
     return fma(T(w2), e2, fma(T(w1), e1, T(w0) * e0));
 }
 template <typename T>
-__device__ __forceinline__ T wenoz_left(const T* f, double eps) {
+__device__ __forceinline__ T wenoz_left(const T* f, double eps, double* sensor = nullptr) {
     const double b0 = js_first(f[3], f[4], f[5]);     // (i+1, i+2, i+3), d = 1/10
     const double b1 = js_centre(f[2], f[3], f[4]);    // (i, i+1, i+2),   d = 6/10
     const double b2 = js_last(f[1], f[2], f[3]);      // (i-1, i, i+1),   d = 3/10
     double w0, w1, w2;
     wenoz_weights<T>(1.0 / 10.0, 6.0 / 10.0, 3.0 / 10.0, b0, b1, b2, eps, w0, w1, w2);
+    if (sensor) *sensor = weno_sensor(1.0 / 10.0, 6.0 / 10.0, 3.0 / 10.0, w0, w1, w2);
     const T e0 = eno3<T>(T(11.0 / 6.0), T(-7.0 / 6.0), T(1.0 / 3.0), f[3], f[4], f[5]);
     const T e1 = eno3<T>(T(1.0 / 3.0), T(5.0 / 6.0), T(-1.0 / 6.0), f[2], f[3], f[4]);
     const T e2 = eno3<T>(T(-1.0 / 6.0), T(5.0 / 6.0), T(1.0 / 3.0), f[1], f[2], f[3]);
@@ -364,7 +373,7 @@ __device__ __forceinline__ void point_prims(const double (&v)[NDIM + 2], const C
 // candidate weights sum to one), and f+ + f- = L F, so the interface flux
 // is F_i + R rec(deviations) with the FP64 F_i added back at the end.
 // ---------------------------------------------------------------------------
-template <int NDIM, int DIR, int SCH, typename T, class PT>
+template <int NDIM, int DIR, int SCH, typename T, class PT, bool FILT = false>
 __device__ __forceinline__ void face_core(PT&& pt, const Consts& c, double (&out)[NDIM + 2]) {
     constexpr int NV = NDIM + 2;
     constexpr bool DEV = !std::is_same<T, double>::value;
@@ -457,7 +466,7 @@ __device__ __forceinline__ void face_core(PT&& pt, const Consts& c, double (&out
     for (int m = 0; m < NV; ++m) acc[m] = T(0);
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        T fp[6], fm[6];
+        T fp[6], fm[6], cfv[6];
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
             T cs = T(0), cf = T(0);
@@ -465,8 +474,21 @@ __device__ __forceinline__ void face_core(PT&& pt, const Consts& c, double (&out
             for (int m = 0; m < NV; ++m) { cs = fma(L[k][m], qs[s][m], cs); cf = fma(L[k][m], F[s][m], cf); }
             fp[s] = T(0.5) * fma(lam[k], cs, cf);
             fm[s] = T(0.5) * fma(-lam[k], cs, cf);
+            cfv[s] = cf;
         }
-        const T rec = recon_pair<SCH, T>(fp, fm, c);
+        T rec;
+        if constexpr (FILT) {
+            // WENO filter (WENO_filter.py:406-470, shock_capturing.py:554-581): the dissipative part
+            // of WENO-Z -- its interface value less the 6th-order central interpolation of the
+            // characteristic fluxes -- times the sensor max(right, left); FP64
+            double s_r = 0.0, s_l = 0.0;
+            const double r0 = wenoz_right<double>(fp, c.weno_eps, &s_r) + wenoz_left<double>(fm, c.weno_eps, &s_l);
+            const double central = (((((cfv[0] - 8.0 * cfv[1]) + 37.0 * cfv[2]) + 37.0 * cfv[3]) - 8.0 * cfv[4]) + cfv[5]) *
+                                   (1.0 / 60.0);
+            rec = fmax(s_r, s_l) * (r0 - central);
+        } else {
+            rec = recon_pair<SCH, T>(fp, fm, c);
+        }
 #pragma unroll
         for (int m = 0; m < NV; ++m) acc[m] = fma(R[m][k], rec, acc[m]);
     }
@@ -481,19 +503,20 @@ __device__ __forceinline__ void face_core(PT&& pt, const Consts& c, double (&out
 
 // face flux at interface x0 + 1/2 along DIR reading the six points from the
 // state (q: FP64 state, `st` the stride along DIR)
-template <int NDIM, int DIR, int SCH, typename T>
+template <int NDIM, int DIR, int SCH, typename T, bool FILT = false>
 __device__ __forceinline__ void face_flux(const double* __restrict__ q, long Ptot, long st, long x0,
                                           const Consts& c, double (&out)[NDIM + 2]) {
     constexpr int NV = NDIM + 2;
     constexpr bool FAST = !std::is_same<T, double>::value;
-    face_core<NDIM, DIR, SCH, T>(
+    static_assert(!FILT || !FAST, "the WENO filter is FP64");
+    auto pt = 
         [&](int s, double (&v)[NV], double (&f)[NV], double (&uu)[NDIM], double& p, double& a) {
             const long xs = x0 + (long)(s - 2) * st;
 #pragma unroll
             for (int m = 0; m < NV; ++m) v[m] = q[(long)m * Ptot + xs];
             point_prims<NDIM, DIR, FAST>(v, c, uu, p, f, a);
-        },
-        c, out);
+        };
+    face_core<NDIM, DIR, SCH, T, decltype(pt)&, FILT>(pt, c, out);
 }
 
 }  // namespace osb

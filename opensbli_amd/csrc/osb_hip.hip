@@ -45,10 +45,10 @@ inline unsigned nblocks(long work) { return (unsigned)((work + BLOCK - 1) / BLOC
 
 // kernel families reported by osb_profile_read
 enum ProfFamily { PF_FACE_FLUX, PF_FLUX_RESIDUAL, PF_CONSTITUENT, PF_GRADIENTS, PF_CENTRAL, PF_VISCOUS,
-                  PF_RK, PF_BC, PF_REDUCE, PF_CENTRAL_FUSED, PF_FACE_RESID, PF_COUNT };
+                  PF_RK, PF_BC, PF_REDUCE, PF_CENTRAL_FUSED, PF_FACE_RESID, PF_FILTER, PF_COUNT };
 const char* const kFamilyName[PF_COUNT] = {"face_flux", "flux_residual", "constituent", "gradients", "central_convective",
                                            "viscous", "rk_update", "boundary", "reduction", "central_fused",
-                                           "face_resid_rk"};
+                                           "face_resid_rk", "weno_filter"};
 
 hipEvent_t pool_event(osb_ctx* c) {
     if (!c->prof_pool.empty()) { hipEvent_t e = c->prof_pool.back(); c->prof_pool.pop_back(); return e; }
@@ -367,11 +367,39 @@ int check_launch(osb_ctx* c) {
     return OSB_OK;
 }
 
+// WENO filter after the full step, in place on q (WENO_filter.py:406-536)
+template <int NDIM>
+void filter_nd(osb_ctx* c, double* q) {
+    Consts fc = c->c;
+    fc.avg_roe = 1;                       // the filter's LFWeno uses RoeAverage (WENO_filter.py:438)
+    fc.glf = nullptr;
+    const Geom& g = c->g;
+    Timed tm(c, PF_FILTER);
+    constexpr int D1 = NDIM > 1 ? 1 : 0, D2 = NDIM > 2 ? 2 : 0;
+    const long P = (long)c->nv * g.Ptot;
+    k_filter_face<NDIM, 0><<<nblocks(face_flux_work<NDIM, 0>(g)), BLOCK, 0, c->stream>>>(q, c->F, g, fc);
+    if (NDIM > 1) k_filter_face<NDIM, D1><<<nblocks(face_flux_work<NDIM, D1>(g)), BLOCK, 0, c->stream>>>(q, c->F + P, g, fc);
+    if (NDIM > 2) k_filter_face<NDIM, D2><<<nblocks(face_flux_work<NDIM, D2>(g)), BLOCK, 0, c->stream>>>(q, c->F + 2 * P, g, fc);
+    k_filter_update<NDIM><<<nblocks(interior_points(g)), BLOCK, 0, c->stream>>>(c->F, q, c->scal, g, c->c);
+}
+
+void apply_filter(osb_ctx* c, double* q) {
+    switch (c->g.ndim) {
+        case 1: filter_nd<1>(c, q); break;
+        case 2: filter_nd<2>(c, q); break;
+        default: filter_nd<3>(c, q); break;
+    }
+}
+
 // RK stage s of the current step (stage 0 also does the step's start: the
 // CFL dt, the classic RK3 saved state), ending with the boundary conditions.
+// With the WENO filter the step starts with the boundary conditions (the
+// filter leaves the halos stale, algorithm.py:462) and ends with the filter.
 void do_stage(osb_ctx* c, int s) {
     const Geom& g = c->g;
     const long nin = interior_points(g);
+    const bool filter = c->d.shock_filter == OSB_FILTER_WENOZ;
+    if (s == 0 && filter) apply_bcs(c, c->q_cur);
     if (s == 0 && c->d.cfl > 0) {
         launch_wavespeed(c, c->q_cur);
         k_dt_finalize<<<1, 64, 0, c->stream>>>(c->scal, c->d.cfl);
@@ -404,6 +432,8 @@ void do_stage(osb_ctx* c, int s) {
     apply_bcs(c, qn);
     c->q_prev = c->q_cur;
     c->q_cur = qn;
+    const int nst = c->d.time_scheme == OSB_TIME_RKLS3 ? c->rk_stages : 3;
+    if (filter && s == nst - 1) apply_filter(c, c->q_cur);
 }
 
 int stage_count(const osb_ctx* c) { return c->d.time_scheme == OSB_TIME_RKLS3 ? c->rk_stages : 3; }
@@ -435,6 +465,9 @@ int validate(const osb_problem_desc* d, std::string& why) {
     if (d->time_scheme != OSB_TIME_RKLS3 && d->time_scheme != OSB_TIME_RK3) { why = "unknown time scheme"; return OSB_E_UNSUPPORTED; }
     if (d->precision != OSB_PREC_FP64 && d->precision != OSB_PREC_MIXED) { why = "unknown precision"; return OSB_E_UNSUPPORTED; }
     if (d->flux_type != OSB_FLUX_LLF && d->flux_type != OSB_FLUX_GLF) { why = "unknown flux type"; return OSB_E_UNSUPPORTED; }
+    if (d->shock_filter != OSB_FILTER_NONE && d->shock_filter != OSB_FILTER_WENOZ) { why = "unknown shock filter"; return OSB_E_UNSUPPORTED; }
+    if (d->shock_filter == OSB_FILTER_WENOZ && d->precision != OSB_PREC_FP64) { why = "the WENO filter runs in FP64 only"; return OSB_E_UNSUPPORTED; }
+    if (d->shock_filter == OSB_FILTER_WENOZ && d->flux_type != OSB_FLUX_LLF) { why = "the WENO filter runs with LLF splitting of the base scheme only"; return OSB_E_UNSUPPORTED; }
     if (d->flux_type == OSB_FLUX_GLF && cs != OSB_CONV_WENO_Z && cs != OSB_CONV_TENO) { why = "GLF applies to shock capturing only"; return OSB_E_INVALID; }
     if (d->rk_stages < 0 || d->rk_stages > 5) { why = "rk_stages must be 0..5"; return OSB_E_INVALID; }
     if (d->rk_stages > 0 && d->time_scheme != OSB_TIME_RKLS3) { why = "rk_stages applies to OSB_TIME_RKLS3 only"; return OSB_E_INVALID; }
@@ -476,7 +509,9 @@ Geom make_geom(const osb_problem_desc* desc) {
     const bool shock = desc->conv_scheme == OSB_CONV_WENO_Z || desc->conv_scheme == OSB_CONV_TENO;
     Geom g{};
     g.ndim = nd;
-    g.halo = shock ? 4 : 2;     // symmetric scheme halo: WENO5/TENO [-3,4] -> 4, Central(4) -> 2
+    // symmetric scheme halo: WENO5/TENO [-3,4] -> 4, Central(4) -> 2; the WENO
+    // filter's reconstruction needs the WENO halo
+    g.halo = (shock || desc->shock_filter == OSB_FILTER_WENOZ) ? 4 : 2;
     for (int i = 0; i < 3; ++i) {
         g.n[i] = desc->n[i];
         g.P[i] = i < nd ? desc->n[i] + 2 * g.halo : 1;
@@ -513,7 +548,7 @@ AllocPlan plan_alloc(const osb_problem_desc* d) {
     AllocPlan a;
     a.q = (size_t)nv * P;
     a.R = unfused_central ? a.q : 0;
-    a.F = shock ? (size_t)nd * a.q : 0;
+    a.F = (shock || d->shock_filter == OSB_FILTER_WENOZ) ? (size_t)nd * a.q : 0;
     a.cr = unfused_central ? (size_t)(nd + 3) * P : 0;
     a.gr = unfused_central ? (size_t)nd * nd * P : 0;
     a.staging = (size_t)nv * (size_t)g.n[0] * g.n[1] * g.n[2];

@@ -51,6 +51,53 @@ __global__ void __launch_bounds__(BLOCK) k_face_flux(const double* __restrict__ 
     face_flux_item<NDIM, DIR, SCH, T>(gtid(), q, F, g, c);
 }
 
+// WENO filter fluxes along DIR (WENO_filter.py:406-470): faces [-1, n) as the
+// reconstruction kernels of the scheme; c carries RoeAverage and LLF.
+template <int NDIM, int DIR>
+__global__ void __launch_bounds__(BLOCK) k_filter_face(const double* __restrict__ q, double* __restrict__ F, Geom g, Consts c) {
+    constexpr int NV = NDIM + 2;
+    int e[3] = {g.n[0], g.n[1], g.n[2]};
+    e[DIR] += 1;
+    int lo[3] = {0, 0, 0};
+    lo[DIR] = -1;
+    const long t = gtid();
+    if (t >= (long)e[0] * e[1] * e[2]) return;
+    int i, j, k;
+    unflatten(t, e, lo, i, j, k);
+    const long x0 = gidx(g, i, j, k);
+    double out[NV];
+    face_flux<NDIM, DIR, SCH_WENOZ, double, true>(q, g.Ptot, g.s[DIR], x0, c, out);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) F[(long)v * g.Ptot + x0] = out[v];
+}
+
+// filter application (WENO_filter.py:479-536): q += dt * (-sum_d (F_d(i) - F_d(i-1)) / Delta_d), in place
+template <int NDIM>
+__global__ void __launch_bounds__(BLOCK) k_filter_update(const double* __restrict__ F, double* __restrict__ q,
+                                                         const double* __restrict__ scal, Geom g, Consts c) {
+    constexpr int NV = NDIM + 2;
+    const int e[3] = {g.n[0], g.n[1], g.n[2]};
+    const int lo[3] = {0, 0, 0};
+    const long t = gtid();
+    if (t >= (long)e[0] * e[1] * e[2]) return;
+    int i, j, k;
+    unflatten(t, e, lo, i, j, k);
+    const long x = gidx(g, i, j, k);
+    const long nvP = (long)NV * g.Ptot;
+    const double dt = scal[1];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        double acc = 0.0;
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) {
+            const double* w = F + d * nvP + (long)v * g.Ptot;
+            acc -= (w[x] - w[x - g.s[d]]) * c.invd[d];
+        }
+        const long y = (long)v * g.Ptot + x;
+        q[y] = q[y] + dt * acc;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Interface fluxes along DIR, tiled: a workgroup owns FL consecutive faces on
 // each of FW lines.  Phase 1 evaluates the primitives and physical flux of

@@ -19,7 +19,7 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 OK = 0
 E_INVALID = -1
@@ -54,6 +54,9 @@ BC_DIRICHLET = 7
 
 FLUX_LLF = 0
 FLUX_GLF = 1
+
+FILTER_NONE = 0
+FILTER_WENOZ = 1
 
 PREC_FP64 = 1
 PREC_MIXED = 2
@@ -91,7 +94,8 @@ class ProblemDescC(C.Structure):
         ("rk_a", C.c_double * 5),
         ("rk_b", C.c_double * 5),
         ("bc_value", ((C.c_double * 5) * 2) * 3),
-        ("reserved", C.c_int32 * 8),
+        ("shock_filter", C.c_int32),
+        ("reserved", C.c_int32 * 7),
     ]
 
 
@@ -135,6 +139,7 @@ class ProblemDesc:
     rk_a: Sequence[float] = ()
     rk_b: Sequence[float] = ()
     flux_type: int = FLUX_LLF
+    shock_filter: int = FILTER_NONE
     # BC_DIRICHLET faces: {(direction, side): conservative values (rho, rhou.., rhoE)}
     bc_value: dict = field(default_factory=dict)
 
@@ -190,6 +195,7 @@ class ProblemDesc:
             d.rk_a[s] = float(a)
             d.rk_b[s] = float(b)
         d.flux_type = int(self.flux_type)
+        d.shock_filter = int(self.shock_filter)
         for (i, s), vals in self.bc_value.items():
             vals = [float(v) for v in vals]
             if len(vals) != self.nv:

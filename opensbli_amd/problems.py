@@ -167,6 +167,48 @@ def tg_sym(n: int = 33, steps: int = 1) -> Problem:
 
 
 # ---------------------------------------------------------------------------
+# Periodic entropy wave under the WENO filter (opensbli/filters/WENO_filter.py:406)
+# ---------------------------------------------------------------------------
+def entropy_wave(n: int = 64, ndim: int = 2, profile: str = "step", steps: int = 20, filtered: bool = True,
+                 scheme: str = "central", velocity: float = 1.0, cfl: float = 0.0) -> Problem:
+    """A density profile advected at constant velocity and pressure on the
+    periodic unit cube: 'step' (a top hat, density 1 / 0.5 / 1 over x <
+    0.25, 0.25 <= x < 0.75, x >= 0.75) or 'sine' (1 + 0.2 sin 2 pi x).  The
+    profile varies along x; in 2-D and 3-D the other directions advect at
+    the same speed with the profile constant along them.  ``scheme``:
+    'central' (Central(4) Feiereisen, inviscid -- the base scheme the filter
+    is designed for), 'kgp' (the enthalpy split form) or 'weno' (WENO-Z,
+    simple averaging).  ``filtered``: the WENO-Z filter after each step."""
+    gama, pr0 = 1.4, 1.0
+    delta = 1.0 / n
+    x = delta * np.arange(n, dtype=np.float64)
+    if profile == "step":
+        d1 = np.where((x >= 0.25) & (x < 0.75), 0.5, 1.0)
+    elif profile == "sine":
+        d1 = 1.0 + 0.2 * _axis(lambda v: math.sin(2.0 * math.pi * v), x)
+    else:
+        raise ValueError(f"unknown profile {profile!r}")
+    shape = [1, 1, 1]
+    for d in range(ndim):
+        shape[2 - d] = n
+    rho = np.broadcast_to(d1[None, None, :], tuple(shape)).copy()
+    nv = ndim + 2
+    q = np.empty((nv,) + tuple(shape))
+    q[0] = rho
+    for d in range(ndim):
+        q[1 + d] = rho * velocity
+    q[nv - 1] = pr0 / (gama - 1.0) + 0.5 * rho * (ndim * velocity * velocity)
+    conv = {"central": nat.CONV_CENTRAL_FEIEREISEN, "kgp": nat.CONV_CENTRAL_KGP, "weno": nat.CONV_WENO_Z}[scheme]
+    desc = nat.ProblemDesc(
+        ndim=ndim, n=[n if d < ndim else 1 for d in range(3)], delta=[delta if d < ndim else 1.0 for d in range(3)],
+        conv_scheme=conv, order=5 if scheme == "weno" else 4, averaging=nat.AVG_SIMPLE,
+        viscous=nat.VISC_NONE, time_scheme=nat.TIME_RKLS3, precision=nat.PREC_FP64,
+        bc=[[nat.BC_PERIODIC, nat.BC_PERIODIC]] * ndim, gama=gama, dt=0.0 if cfl > 0 else 0.2 * delta, cfl=cfl,
+        shock_filter=nat.FILTER_WENOZ if filtered else nat.FILTER_NONE)
+    return Problem(f"entropy_{profile}_{ndim}d_{n}_{scheme}{'_filtered' if filtered else ''}", desc, q, steps)
+
+
+# ---------------------------------------------------------------------------
 # 2-D Kelvin-Helmholtz instance (config: periodic x, slip walls y)
 # ---------------------------------------------------------------------------
 def kelvin_helmholtz(n: int = 128, amplitude: float = 0.02, mode: int = 2, steps: int = 500,

@@ -391,6 +391,30 @@ class SimulationBlock:
         return None
 
 
+class WENOFilter:
+    """opensbli/filters/WENO_filter.py:406 -- the non-linear WENO filter applied
+    after each full time step of a non-dissipative base scheme: the
+    dissipative part of a WENO-Z reconstruction in characteristic space (its
+    interface flux less the 6th-order central interpolation of the
+    characteristic fluxes) scaled by the WENO weight sensor, with Roe
+    averaging, and q <- q + dt * (filter residual).  ``equation_classes`` goes
+    to ``block.set_equations`` as in the reference.  The HIP kernels
+    implement order 5, formulation 'Z', flux_type 'LLF', uniform meshes
+    (metrics=None) and optimize=False (no Ducros hybrid selection)."""
+
+    def __init__(self, block, order, metrics=None, flux_type="LLF", airfoil=False, formulation="Z", optimize=False,
+                 species=None):
+        self.block = block
+        self.order = int(order)
+        self.metrics = metrics
+        self.flux_type = flux_type
+        self.airfoil = airfoil
+        self.formulation = formulation
+        self.optimize = optimize
+        self.species = species
+        self.equation_classes = [self]
+
+
 class SimulationMonitor:
     """opensbli/core/diagnostics/simulation_monitors.py:40.  ``arrays`` name the
     probed datasets (``"rho_B0"``, ``"rho"`` or DataObjects), ``probe_locations``

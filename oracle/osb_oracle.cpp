@@ -316,7 +316,13 @@ inline double zalpha(double d, double t2, double b, double eps) {
 }
 
 // WENO-Z (weno.py:290-345): tau5 = |b0 - b2|.
-double wenoz_right(const double* f, double eps) {
+// shock sensor of the WENO filter (weno.py:443-459, normalised): sum_r |omega_r/d_r - 1| /
+// (|1/min(d) - 1| + (k - 1)) = (...) / 11 for WENO5
+inline double weno_sensor(const double* d, double w0, double w1, double w2) {
+    return (std::fabs(w0 * (1.0 / d[0]) - 1.0) + std::fabs(w1 * (1.0 / d[1]) - 1.0) + std::fabs(w2 * (1.0 / d[2]) - 1.0)) / 11.0;
+}
+
+double wenoz_right(const double* f, double eps, double* sensor = nullptr) {
     // points f[0]=i-2 ... f[4]=i+2 ; r=0 -> (i,i+1,i+2), r=1 -> (i-1,i,i+1), r=2 -> (i-2,i-1,i)
     const double b0 = js_first(f[2], f[3], f[4]);
     const double b1 = js_centre(f[1], f[2], f[3]);
@@ -328,9 +334,10 @@ double wenoz_right(const double* f, double eps) {
     const double e0 = eno3(W5.c[0][0], f[2], f[3], f[4]);
     const double e1 = eno3(W5.c[0][1], f[1], f[2], f[3]);
     const double e2 = eno3(W5.c[0][2], f[0], f[1], f[2]);
+    if (sensor) *sensor = weno_sensor(d, a0 * inv, a1 * inv, a2 * inv);
     return std::fma(a2 * inv, e2, std::fma(a1 * inv, e1, (a0 * inv) * e0));
 }
-double wenoz_left(const double* f, double eps) {
+double wenoz_left(const double* f, double eps, double* sensor = nullptr) {
     // points f[1]=i-1 ... f[5]=i+3 ; r=0 -> (i+1,i+2,i+3), r=1 -> (i,i+1,i+2), r=2 -> (i-1,i,i+1)
     const double b0 = js_first(f[3], f[4], f[5]);
     const double b1 = js_centre(f[2], f[3], f[4]);
@@ -342,6 +349,7 @@ double wenoz_left(const double* f, double eps) {
     const double e0 = eno3(W5.c[1][0], f[3], f[4], f[5]);
     const double e1 = eno3(W5.c[1][1], f[2], f[3], f[4]);
     const double e2 = eno3(W5.c[1][2], f[1], f[2], f[3]);
+    if (sensor) *sensor = weno_sensor(d, a0 * inv, a1 * inv, a2 * inv);
     return std::fma(a2 * inv, e2, std::fma(a1 * inv, e1, (a0 * inv) * e0));
 }
 
@@ -415,7 +423,7 @@ double teno_side(const double* f, int side, const TenoParams& tp) {
 // Writes the interface flux at i+1/2 into wk_dir[v] at index i over
 // [-1, N+1) along dir and [0, N) across (TENO6/opensbli.cpp:263-316).
 // ---------------------------------------------------------------------------
-void reconstruct_direction(osb_ctx* c, int dir) {
+void reconstruct_direction(osb_ctx* c, int dir, bool filter = false) {
     const int nd = c->ndim, nv = c->nv;
     const double gama = c->d.gama, gamma_m1 = gama - 1.0;
     const double sfc = c->d.shock_filter_control;
@@ -445,7 +453,7 @@ void reconstruct_direction(osb_ctx* c, int dir) {
         // averaged state between offsets 0 and 1 (averaging.py:31-124)
         const size_t xa = x0, xb = x0 + st;
         double rho_av, u_av[3] = {0, 0, 0}, a_av;
-        if (c->d.averaging == OSB_AVG_ROE) {
+        if (filter || c->d.averaging == OSB_AVG_ROE) {   // the filter's scheme averages by Roe (WENO_filter.py:438)
             const double rl = c->q[xa], rr = c->q[xb];
             const double srl = std::sqrt(rl), srr = std::sqrt(rr);
             rho_av = std::sqrt(rl * rr);
@@ -489,14 +497,24 @@ void reconstruct_direction(osb_ctx* c, int dir) {
         }
         double recon[5];
         for (int kk = 0; kk < nv; ++kk) {
-            double fp[6], fm[6];
+            double fp[6], fm[6], cfv[6];
             for (int s = 0; s < 6; ++s) {
                 double cs = 0.0, cf = 0.0;
                 for (int m = 0; m < nv; ++m) { cs = std::fma(e.L[kk][m], qv[s][m], cs); cf = std::fma(e.L[kk][m], F[s][m], cf); }
                 fp[s] = 0.5 * std::fma(lam[kk], cs, cf);   // f+ = (CF + lambda CS)/2
                 fm[s] = 0.5 * std::fma(-lam[kk], cs, cf);  // f- = (CF - lambda CS)/2
+                cfv[s] = cf;
             }
-            if (teno) recon[kk] = teno_side(fp, +1, tp) + teno_side(fm, -1, tp);
+            if (filter) {
+                // WENO filter (WENO_filter.py:406-470, shock_capturing.py:554-581): the dissipative
+                // part of WENO-Z -- its interface value less the 6th-order central interpolation
+                // of the characteristic fluxes -- times the sensor max(right, left)
+                double s_r = 0.0, s_l = 0.0;
+                const double rec = wenoz_right(fp, weps, &s_r) + wenoz_left(fm, weps, &s_l);
+                const double central = (((((cfv[0] - 8.0 * cfv[1]) + 37.0 * cfv[2]) + 37.0 * cfv[3]) - 8.0 * cfv[4]) + cfv[5]) *
+                                       (1.0 / 60.0);
+                recon[kk] = std::max(s_r, s_l) * (rec - central);
+            } else if (teno) recon[kk] = teno_side(fp, +1, tp) + teno_side(fm, -1, tp);
             else recon[kk] = wenoz_right(fp, weps) + wenoz_left(fm, weps);
         }
         for (int v = 0; v < nv; ++v) {
@@ -814,6 +832,32 @@ void apply_bcs(osb_ctx* c) {
 }
 
 // ---------------------------------------------------------------------------
+// WENO filter step after a full RK step (WENO_filter.py:406-536; placed in the
+// time loop after the stage loop, algorithm.py:428-450): constituent relations
+// of the current state, the filtered interface fluxes per direction, and
+// q += dt * R_filter with R_filter = -sum_d (F_d(i) - F_d(i-1)) / Delta_d.
+// ---------------------------------------------------------------------------
+void apply_weno_filter(osb_ctx* c, double dt) {
+    Range rcr = make_range(c, 3);
+    for (int d = 0; d < c->ndim; ++d) rcr.hi[d] = c->n[d] + 4;
+    constituent_relations(c, rcr, false);
+    for (int dir = 0; dir < c->ndim; ++dir) reconstruct_direction(c, dir, true);
+    const int nd = c->ndim, nv = c->nv;
+    Range r = make_range(c, 0);
+    for_range(r, [&](int i, int j, int k) {
+        const size_t x = c->I(i, j, k);
+        for (int v = 0; v < nv; ++v) {
+            double acc = 0.0;
+            for (int dir = 0; dir < nd; ++dir) {
+                const double* w = c->wk.data() + ((size_t)dir * nv + v) * c->Ptot;
+                acc -= (w[x] - w[x - c->stride(dir)]) * (1.0 / c->d.delta[dir]);
+            }
+            c->q[(size_t)v * c->Ptot + x] = c->q[(size_t)v * c->Ptot + x] + dt * acc;
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------
 // "Global wave-speed reductions" kernel of GLF splitting (weno.py:515-519,
 // teno.py:520-524; shock_capturing.py:416-438, :838-857): per direction the
 // maxima over the grid [0, N) of |u_d|, |u_d + a|, |u_d - a| from the
@@ -920,6 +964,9 @@ int validate(const osb_problem_desc* d, std::string& why) {
     if (d->time_scheme != OSB_TIME_RKLS3 && d->time_scheme != OSB_TIME_RK3) { why = "unknown time scheme"; return OSB_E_UNSUPPORTED; }
     if (d->precision != OSB_PREC_FP64 && d->precision != OSB_PREC_MIXED) { why = "unknown precision"; return OSB_E_UNSUPPORTED; }
     if (d->flux_type != OSB_FLUX_LLF && d->flux_type != OSB_FLUX_GLF) { why = "unknown flux type"; return OSB_E_UNSUPPORTED; }
+    if (d->shock_filter != OSB_FILTER_NONE && d->shock_filter != OSB_FILTER_WENOZ) { why = "unknown shock filter"; return OSB_E_UNSUPPORTED; }
+    if (d->shock_filter == OSB_FILTER_WENOZ && d->precision != OSB_PREC_FP64) { why = "the WENO filter runs in FP64 only"; return OSB_E_UNSUPPORTED; }
+    if (d->shock_filter == OSB_FILTER_WENOZ && d->flux_type != OSB_FLUX_LLF) { why = "the WENO filter runs with LLF splitting of the base scheme only"; return OSB_E_UNSUPPORTED; }
     if (d->flux_type == OSB_FLUX_GLF && cs != OSB_CONV_WENO_Z && cs != OSB_CONV_TENO) { why = "GLF applies to shock capturing only"; return OSB_E_INVALID; }
     if (d->rk_stages < 0 || d->rk_stages > 5) { why = "rk_stages must be 0..5"; return OSB_E_INVALID; }
     if (d->rk_stages > 0 && d->time_scheme != OSB_TIME_RKLS3) { why = "rk_stages applies to OSB_TIME_RKLS3 only"; return OSB_E_INVALID; }
@@ -992,7 +1039,9 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     c->shock = desc->conv_scheme == OSB_CONV_WENO_Z || desc->conv_scheme == OSB_CONV_TENO;
     c->central = !c->shock;
     c->viscous = desc->viscous != OSB_VISC_NONE;
-    c->depth = c->shock ? 4 : 2;          // WenoHalos/TenoHalos [-3,4] -> 4; CentralHalos(4) -> 2
+    // WenoHalos/TenoHalos [-3,4] -> 4; CentralHalos(4) -> 2; the WENO filter adds
+    // its WENO halos to the block's (WENO_filter.py:440-441)
+    c->depth = (c->shock || desc->shock_filter == OSB_FILTER_WENOZ) ? 4 : 2;
     if (desc->rk_stages == 0) {
         c->rk_stages = 3;
         for (int s = 0; s < 3; ++s) { c->rkA[s] = RKLS_A[s]; c->rkB[s] = RKLS_B[s]; }
@@ -1008,7 +1057,7 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     c->p.assign(c->Ptot, 0.0);
     c->a.assign(c->Ptot, 0.0);
     if (c->viscous || c->central) { c->T.assign(c->Ptot, 0.0); c->mu.assign(c->Ptot, 0.0); }
-    if (c->shock) c->wk.assign((size_t)c->ndim * nvP, 0.0);
+    if (c->shock || desc->shock_filter == OSB_FILTER_WENOZ) c->wk.assign((size_t)c->ndim * nvP, 0.0);
     if (c->viscous || c->central) c->g.assign((size_t)c->ndim * c->ndim * c->Ptot, 0.0);
     *out = c;
     return OSB_OK;
@@ -1121,6 +1170,7 @@ void run_stage(osb_ctx* c, int s) {
     apply_bcs(c);
     c->next_stage = (s + 1) % nst;
     if (c->next_stage == 0) {
+        if (c->d.shock_filter == OSB_FILTER_WENOZ) apply_weno_filter(c, dt);
         c->iter += 1;
         c->time += dt;
     }

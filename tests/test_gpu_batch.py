@@ -58,6 +58,10 @@ def _ensemble():
     probs.append(w3)
     probs.append(P.tgv(n=16, steps=5))                # fused central path (stepped alone)
     probs.append(P.tgv(n=16, steps=5, scheme="teno", minf=1.25, cfl=0.5, dt=0.0))
+    sf = P.sod(n=112, steps=5)
+    sf.desc.shock_filter = nat.FILTER_WENOZ           # filtered contexts are stepped alone
+    sf.name += "_filtered"
+    probs.append(sf)
     return probs
 
 
@@ -76,7 +80,7 @@ def test_batch_bitwise_equals_single(hip, graphs, monkeypatch):
         print("  ", info, flush=True)
         got = [b.get_state(i) for i in range(len(probs))]
         stats = [b.stats(i) for i in range(len(probs))]
-    assert info["solo"] == 2 and len(info["groups"]) >= 5
+    assert info["solo"] == 3 and len(info["groups"]) >= 5
     for p, q, st in zip(probs, got, stats):
         _announce(f"  compare {p.name}")
         want, wst = run(hip, p, steps=5)

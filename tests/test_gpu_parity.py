@@ -301,3 +301,51 @@ def test_probes_on_the_gpu(hip):
         pts = [(0, 0, 0), (15, 3, 7), (8, 15, 1)]
         v = c.probe(pts, [0, 4, 2])
     assert list(v) == [q[0, 0, 0, 0], q[4, 7, 3, 15], q[2, 1, 15, 8]]
+
+
+# --- WENO filter after each full step (WENO_filter.py:406-536) ----------------------
+def _filtered(pr, **kw):
+    d = nat.ProblemDesc(**{**pr.desc.__dict__, "shock_filter": nat.FILTER_WENOZ, **kw})
+    return P.Problem(pr.name + "_filtered", d, pr.q0, pr.steps)
+
+
+@pytest.mark.parametrize("maker", [
+    lambda: P.entropy_wave(n=64, profile="step", steps=20),                          # 2-D central, unfused
+    lambda: P.entropy_wave(n=48, profile="step", steps=10, scheme="kgp"),            # 2-D KGP
+    lambda: _filtered(P.tgv(n=16, steps=5, scheme="kgp")),                           # 3-D KGP, viscous
+    lambda: _filtered(P.tgv(n=16, steps=5, minf=1.25, cfl=0.5, dt=0.0)),              # 3-D fused central, CFL
+    lambda: _filtered(P.tg_sym(n=17, steps=3)),                                      # classic RK3, symmetry walls
+    lambda: _filtered(P.sod(n=200, steps=20)),                                       # 1-D WENO base, simple average
+    lambda: _filtered(P.kelvin_helmholtz(n=48, steps=10)),                           # 2-D WENO base, walls
+])
+def test_weno_filter_parity(hip, oracle, maker):
+    a, b = _compare(hip, oracle, maker())
+    assert np.array_equal(a, b)
+
+
+def test_weno_filter_stages_and_graphs(hip, oracle, monkeypatch):
+    """The filter runs after the last stage: stage-by-stage equals whole steps,
+    and captured steps replayed equal eager launches."""
+    pr = P.entropy_wave(n=32, profile="step", steps=3)
+    _announce("filter stages / graph replay")
+    out = {}
+    for graphs in ("1", "0"):
+        monkeypatch.setenv("OSB_GRAPH", graphs)
+        with hip.context(pr.desc) as c:
+            c.set_state(pr.q0)
+            snaps = []
+            for k in (1, 2, 1):
+                c.step(k)
+                snaps.append(c.get_state())
+            out[graphs] = snaps
+    with hip.context(pr.desc) as c, oracle.context(pr.desc) as o:
+        for x in (c, o):
+            x.set_state(pr.q0)
+        for _ in range(4):
+            for s in range(3):
+                c.stage(s)
+                o.stage(s)
+        qs, qo = c.get_state(), o.get_state()
+    for a, b in zip(out["1"], out["0"]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(qs, out["0"][-1]) and np.array_equal(qs, qo)

@@ -570,3 +570,113 @@ def test_kgp_converges_to_the_divergence_form(oracle):
                 res[conv] = c.residual()
         diffs.append(np.abs(res[nat.CONV_CENTRAL_KGP] - res[nat.CONV_CENTRAL_FEIEREISEN]).max())
     assert diffs[0] / diffs[1] > 13, diffs          # 4th order: 16 in the limit
+
+
+# --- WENO filter (opensbli/filters/WENO_filter.py:406-536) ------------------------
+def _tv_x(q):
+    return np.abs(np.diff(q[0][0, 0])).sum()
+
+
+def test_weno_filter_removes_central_oscillations(oracle):
+    """A top hat advected by the non-dissipative central scheme rings (total
+    variation ~5x the initial one after 40 steps); with the filter after each
+    step the variation stays near the initial one and the density within the
+    initial bounds up to a small overshoot.  A sign error in the filter's
+    flux difference or in its dissipative part amplifies instead."""
+    raw, _ = run(oracle, P.entropy_wave(n=64, profile="step", steps=40, filtered=False))
+    pr = P.entropy_wave(n=64, profile="step", steps=40, filtered=True)
+    q, _ = run(oracle, pr)
+    tv0 = _tv_x(pr.q0)
+    assert _tv_x(raw) > 4.0 * tv0
+    assert _tv_x(q) < 1.1 * tv0
+    assert q[0].min() > 0.49 and q[0].max() < 1.01
+
+
+def test_weno_filter_is_conservative(oracle):
+    """The filter adds a flux difference: on a periodic grid every conservative
+    variable's total is unchanged by it (to rounding)."""
+    for prof in ("step", "sine"):
+        a, _ = run(oracle, P.entropy_wave(n=48, profile=prof, steps=6, filtered=False))
+        b, _ = run(oracle, P.entropy_wave(n=48, profile=prof, steps=6, filtered=True))
+        assert np.abs(a - b).max() > 1e-9
+        for v in range(a.shape[0]):
+            assert abs(b[v].sum() - a[v].sum()) <= 1e-12 * np.abs(a[v]).sum(), v
+
+
+def test_weno_filter_vanishes_on_smooth_data(oracle):
+    """On a smooth wave the sensor and the WENO-minus-central part are both
+    small: the change the filter makes in one step falls at least 4th order
+    with the grid spacing (5th in the limit)."""
+    inc = []
+    for n in (32, 64, 128):
+        a, _ = run(oracle, P.entropy_wave(n=n, profile="sine", steps=1, filtered=False))
+        b, _ = run(oracle, P.entropy_wave(n=n, profile="sine", steps=1, filtered=True))
+        inc.append(np.abs(a - b).max())
+    assert inc[0] / inc[1] > 16 and inc[1] / inc[2] > 16, inc
+
+
+def test_weno_filter_keeps_a_uniform_state(oracle):
+    """Free stream: every candidate stencil, the central interpolation and the
+    split fluxes agree, so the filter leaves a uniform state at rounding."""
+    pr = P.entropy_wave(n=24, profile="sine", steps=3, filtered=True)
+    pr.q0[0] = 1.0
+    pr.q0[1] = pr.q0[2] = 1.0
+    pr.q0[3] = 1.0 / 0.4 + 1.0
+    q, _ = run(oracle, pr)
+    assert np.abs(q - pr.q0).max() <= 1e-13
+
+
+def test_weno_filter_leaves_a_stationary_contact(oracle):
+    """u = 0 and uniform pressure: the entropy characteristic's LLF speed is 0
+    and the acoustic characteristic variables do not vary across the jump, so
+    the filter's WENO and central parts agree and the contact stays put."""
+    pr = P.entropy_wave(n=40, profile="step", steps=5, filtered=True, velocity=0.0)
+    q, _ = run(oracle, pr)
+    assert np.abs(q - pr.q0).max() <= 1e-14
+
+
+def test_weno_filter_keeps_mirror_symmetry(oracle):
+    """A pressure and density top hat at rest, symmetric about x = 1/2: the
+    acoustic waves leave symmetrically and the filtered state keeps rho(i) =
+    rho(n-1-i), rhou(i) = -rhou(n-1-i); a shifted face index or swapped
+    left/right reconstructions break the symmetry.  WENO-Z's epsilon is
+    raised from 1e-40 to 1e-16 here: with 1e-40 the sensor is ill-conditioned
+    where the smoothness indicators are at rounding level, and mirrored
+    evaluation orders then differ by up to ~1e-10."""
+    pr = P.entropy_wave(n=40, profile="step", steps=5, filtered=True, velocity=0.0)
+    pr.desc.weno_eps = 1e-16
+    rho = pr.q0[0].copy()
+    pr.q0[3] = np.where(rho < 0.75, 0.5, 1.0) / 0.4            # p = 0.5 inside the hat, 1 outside
+    ref, _ = run(oracle, pr, steps=5)
+    raw, _ = run(oracle, P.Problem(pr.name, nat.ProblemDesc(**{**pr.desc.__dict__, "shock_filter": nat.FILTER_NONE}),
+                                   pr.q0, 5))
+    assert np.abs(ref - raw).max() > 1e-6                           # the filter acted
+    line = ref[:, 0, 0, :]
+    np.testing.assert_allclose(line[0], line[0][::-1], rtol=0, atol=1e-11)
+    np.testing.assert_allclose(line[1], -line[1][::-1], rtol=0, atol=1e-11)
+    np.testing.assert_allclose(line[3], line[3][::-1], rtol=0, atol=1e-11)
+
+
+def test_weno_filter_direction_symmetry(oracle):
+    """The same wave along x and along y (velocity components and momenta
+    swapped) filters to transposed states: the directions are treated alike
+    (epsilon 1e-16 as in the mirror test; the eigenvector formulas differ per
+    direction, so the match is to rounding, not bitwise)."""
+    pr = P.entropy_wave(n=32, profile="step", steps=4, filtered=True)
+    pr.desc.weno_eps = 1e-16
+    qx, _ = run(oracle, pr)
+    qy0 = pr.q0.transpose(0, 1, 3, 2)[[0, 2, 1, 3]].copy()
+    qy, _ = run(oracle, pr, state=qy0)
+    np.testing.assert_allclose(qy, qx.transpose(0, 1, 3, 2)[[0, 2, 1, 3]], rtol=0, atol=1e-11)
+    assert np.array_equal(qx[:, :, 0, :], qx[:, :, 5, :])             # rows stay identical
+
+
+def test_weno_filter_validation(oracle):
+    pr = P.entropy_wave(n=16)
+    for kw in (dict(precision=nat.PREC_MIXED), dict(shock_filter=7)):
+        with pytest.raises(nat.OSBError):
+            oracle.context(nat.ProblemDesc(**{**pr.desc.__dict__, **kw}))
+    w = P.sod(n=64)
+    with pytest.raises(nat.OSBError):
+        oracle.context(nat.ProblemDesc(**{**w.desc.__dict__, "flux_type": nat.FLUX_GLF,
+                                          "shock_filter": nat.FILTER_WENOZ}))

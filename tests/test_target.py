@@ -353,3 +353,48 @@ def test_kgp_plan():
     _same(plan_problem(alg, params, physics=NS), P.tgv(n=16, scheme="kgp", dt=0.003385).desc)
     with pytest.raises(NotImplementedError, match="enthalpy"):
         plan_problem(alg, params, physics=NS_Split("KGP", 3, [], viscosity="constant"))
+
+
+def _filtered_wave_alg(**filter_kw):
+    from opensbli_amd.surface import WENOFilter
+    block = SimulationBlock(2, block_number=0)
+    block.set_block_boundaries([PeriodicBC(d, s) for d in range(2) for s in range(2)])
+    block.set_discretisation_schemes({"central": Central(4), "rk": RungeKuttaLS(3)})
+    WF = WENOFilter(block, order=5, formulation="Z", flux_type="LLF", airfoil=False, optimize=False, **filter_kw)
+    block.set_equations(WF.equation_classes)
+    block.discretise()
+    return TraditionalAlgorithmRK(block)
+
+
+def test_weno_filter_plan_and_run(oracle):
+    """WENOFilter(block, order=5, formulation='Z', flux_type='LLF') in the
+    block's equation classes (apps/Sod_shock_tube/Sod_shock_tube.py:138-139)
+    turns the filter on; the run equals the descriptor's."""
+    params = {"gama": "1.4", "dt": "0.2/32", "niter": "3", "block0np0": "32", "block0np1": "32",
+              "Delta0block0": "1.0/block0np0", "Delta1block0": "1.0/block0np1"}
+    alg = _filtered_wave_alg()
+    pr = P.entropy_wave(n=32, steps=3)
+    desc = plan_problem(alg, params, physics={"convective": "Feiereisen", "viscosity": None})
+    _same(desc, pr.desc)
+    t = HIPTarget(alg, parameters=params, physics={"convective": "Feiereisen", "viscosity": None}, library=oracle)
+    q = t.initialise(pr.q0).run().state()
+    t.close()
+    want, _ = run(oracle, pr)
+    assert np.array_equal(q, want)
+
+
+@pytest.mark.parametrize("kw,match", [(dict(order=3), "order 5"), (dict(formulation="JS"), "'Z'"),
+                                      (dict(flux_type="HLLC"), "LLF"), (dict(optimize=True), "optimize"),
+                                      (dict(metrics=object()), "uniform")])
+def test_weno_filter_unsupported_options_raise(kw, match):
+    from opensbli_amd.surface import WENOFilter
+    block = SimulationBlock(2, block_number=0)
+    block.set_block_boundaries([PeriodicBC(d, s) for d in range(2) for s in range(2)])
+    block.set_discretisation_schemes({"central": Central(4), "rk": RungeKuttaLS(3)})
+    args = dict(order=5, formulation="Z", flux_type="LLF", optimize=False, metrics=None)
+    args.update(kw)
+    block.set_equations(WENOFilter(block, **args).equation_classes)
+    params = {"gama": "1.4", "dt": "0.001", "block0np0": "16", "block0np1": "16", "Delta0block0": "0.1",
+              "Delta1block0": "0.1"}
+    with pytest.raises(NotImplementedError, match=match):
+        plan_problem(TraditionalAlgorithmRK(block), params, physics={"convective": "Feiereisen", "viscosity": None})
