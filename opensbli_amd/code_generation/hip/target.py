@@ -297,10 +297,10 @@ def plan_problem(alg_or_block, parameters: Mapping, physics=None, precision: str
                 bc[b.direction][b.side] = _BC_IDS[name]
             if name == "DirichletBC":
                 bc_value[(b.direction, b.side)] = dirichlet_values(b, ndim, p)
-            if name in ("ExtrapolationBC", "DirichletBC") and (central is not None or shock is None):
-                raise NotImplementedError(
-                    f"{name} with central derivatives needs the Carpenter boundary closure "
-                    "(Carpenter_scheme.py), which the HIP kernels do not implement")
+            if name in ("ExtrapolationBC", "DirichletBC") and getattr(b, "modification_scheme", None) is not None \
+                    and type(b.modification_scheme).__name__ != "Carpenter":
+                raise NotImplementedError(f"{name}(scheme={type(b.modification_scheme).__name__}): the kernels "
+                                          "implement the default Carpenter closure")
     if any(v == 0 for pair in bc for v in pair):
         raise ValueError("every direction needs a boundary condition on both sides")
 
@@ -321,6 +321,10 @@ def plan_problem(alg_or_block, parameters: Mapping, physics=None, precision: str
         if desc.get("flux_type") == nat.FLUX_GLF:
             raise NotImplementedError("a WENO filter over a GLF base scheme: the kernels pair the filter with LLF")
         desc["shock_filter"] = nat.FILTER_WENOZ
+        for d in range(ndim):
+            key = f"inv_rfact{d}_block0"         # the filter's 1/Delta (opensblifunctions.py:577-591)
+            if key in p and not math.isclose(float(p[key]), 1.0 / delta[d], rel_tol=1e-12):
+                raise NotImplementedError(f"{key} = {p[key]}: the filter kernels scale by 1/Delta{d}block0")
     desc["bc"] = bc
     desc["bc_value"] = bc_value
 

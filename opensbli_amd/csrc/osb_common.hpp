@@ -44,6 +44,9 @@ struct Consts {
     // GLF splitting: per direction d, glf[3d..3d+2] = max over the grid of
     // |u_d|, |u_d + a|, |u_d - a| for the current stage; nullptr: LLF
     const double* glf;
+    // Carpenter boundary closure of the central derivatives per direction
+    // (bit 0: side 0, bit 1: side 1), set for extrapolation and Dirichlet faces
+    int carp[3];
 };
 
 // ---------------------------------------------------------------------------

@@ -355,9 +355,11 @@ void fused_stage(osb_ctx* c, const double* q, double* qn, double A, double B) {
     else launch_fused_v<CF_FEIEREISEN, false, TIME>(c, q, qn, A, B);
 }
 
-// the fused 3-D stage kernel implements the Feiereisen and skew forms; KGP runs the separate kernels
+// the fused 3-D stage kernel implements the Feiereisen and skew forms without
+// the Carpenter boundary closure; KGP and closure faces run the separate kernels
 bool use_fused(const osb_ctx* c) {
-    return c->g.ndim == 3 && (c->central || c->viscous) && c->d.conv_scheme != OSB_CONV_CENTRAL_KGP;
+    return c->g.ndim == 3 && (c->central || c->viscous) && c->d.conv_scheme != OSB_CONV_CENTRAL_KGP &&
+           !(c->c.carp[0] | c->c.carp[1] | c->c.carp[2]);
 }
 bool use_shock_stage(const osb_ctx* c) { return c->shock && !c->central && !c->viscous; }
 
@@ -477,8 +479,8 @@ int validate(const osb_problem_desc* d, std::string& why) {
         for (int s = 0; s < 2; ++s) {
             const int b = d->bc[i][s];
             if (b < OSB_BC_PERIODIC || b > OSB_BC_DIRICHLET) { why = "unknown boundary condition"; return OSB_E_UNSUPPORTED; }
-            if (b >= OSB_BC_EXTRAPOLATION0 && (central_cs || d->viscous != OSB_VISC_NONE)) {
-                why = "extrapolation/Dirichlet boundaries with central derivatives need the Carpenter closure (Carpenter_scheme.py), not built";
+            if (b >= OSB_BC_EXTRAPOLATION0 && (central_cs || d->viscous != OSB_VISC_NONE) && d->precision != OSB_PREC_FP64) {
+                why = "the Carpenter boundary closure of the central derivatives runs in FP64 only";
                 return OSB_E_UNSUPPORTED;
             }
             if (b == OSB_BC_DIRICHLET)
@@ -490,7 +492,6 @@ int validate(const osb_problem_desc* d, std::string& why) {
     if (!(d->cfl > 0) && !(d->dt > 0)) { why = "dt > 0 or cfl > 0 required"; return OSB_E_INVALID; }
     if (!(d->gama > 1.0)) { why = "gama > 1 required"; return OSB_E_INVALID; }
     if (d->viscous != OSB_VISC_NONE && (!(d->re > 0) || !(d->pr > 0) || !(d->minf > 0))) { why = "Re, Pr, Minf > 0 required"; return OSB_E_INVALID; }
-    if ((cs == OSB_CONV_CENTRAL_FEIEREISEN || cs == OSB_CONV_CENTRAL_SKEW) && d->ndim < 2) { why = "central split forms need ndim >= 2"; return OSB_E_UNSUPPORTED; }
     return OSB_OK;
 }
 
@@ -542,7 +543,9 @@ AllocPlan plan_alloc(const osb_problem_desc* d) {
     const int nd = d->ndim, nv = nd + 2;
     const bool shock = d->conv_scheme == OSB_CONV_WENO_Z || d->conv_scheme == OSB_CONV_TENO;
     const bool central = !shock, viscous = d->viscous != OSB_VISC_NONE;
-    const bool fused = nd == 3 && (central || viscous) && d->conv_scheme != OSB_CONV_CENTRAL_KGP;
+    bool carp = false;
+    for (int i = 0; i < nd; ++i) carp = carp || d->bc[i][0] >= OSB_BC_EXTRAPOLATION0 || d->bc[i][1] >= OSB_BC_EXTRAPOLATION0;
+    const bool fused = nd == 3 && (central || viscous) && d->conv_scheme != OSB_CONV_CENTRAL_KGP && !carp;
     const bool unfused_central = (central || viscous) && !fused;
     const size_t P = (size_t)g.Ptot;
     AllocPlan a;
@@ -658,6 +661,11 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     k.avg_roe = desc->averaging == OSB_AVG_ROE;
     k.viscous = desc->viscous;
     k.glf = nullptr;
+    for (int i = 0; i < 3; ++i) {
+        k.carp[i] = 0;
+        for (int sd = 0; sd < 2; ++sd)
+            if (i < desc->ndim && desc->bc[i][sd] >= OSB_BC_EXTRAPOLATION0) k.carp[i] |= 1 << sd;
+    }
 
     const AllocPlan ap = plan_alloc(desc);
     int r = OSB_OK;

@@ -427,6 +427,55 @@ __device__ __forceinline__ double d2(A f, double inv2d) {
     return fma(-1.0 / 12.0, f(-2) + f(2), fma(4.0 / 3.0, f(-1) + f(1), (-5.0 / 2.0) * f(0))) * inv2d;
 }
 
+// Carpenter's 4th-order boundary closure (Carpenter_scheme.py:6-125) at the
+// points next to extrapolation / Dirichlet faces (opensblifunctions.py:486-534):
+// first derivative at r = 0..3 points from the face from the 6 points nearest
+// it (negated on side 1), second derivative at r = 0, 1 from 5 points.  The
+// coefficients are al4^-1 ar4 of Carpenter_scheme.py:97-121, correctly rounded.
+__device__ __forceinline__ double carp_c1(int r, int j) {
+    constexpr double C[4][6] = {
+        {-11.0 / 6.0, 3.0, -1.5, 1.0 / 3.0, 0.0, 0.0},
+        {-0.3762836775133548, -0.3224849328821596, 0.719443173328853, 0.03941685243994666, -0.0658051057710398, 0.005713690397754591},
+        {0.11344647038423929, -0.7912455927658653, 0.19718433388773493, 0.521455851089594, -0.0367146847001282, -0.004126377895574793},
+        {-0.009325979850499882, 0.12193715322406413, -0.7278221477245911, 0.045103322334387165, 0.6521410848612417, -0.08203343284460206}};
+    return C[r][j];
+}
+__device__ __forceinline__ double carp_c2(int r, int j) {
+    constexpr double C[2][5] = {{35.0 / 12.0, -104.0 / 12.0, 114.0 / 12.0, -56.0 / 12.0, 11.0 / 12.0},
+                                {11.0 / 12.0, -20.0 / 12.0, 6.0 / 12.0, 4.0 / 12.0, -1.0 / 12.0}};
+    return C[r][j];
+}
+// dirn = +1 on side 0 (points j - r from x), -1 on side 1 (r - j)
+template <class A>
+__device__ __forceinline__ double carp1(A f, int r, int dirn) {
+    double s = carp_c1(r, 0) * f(-dirn * r);
+    for (int j = 1; j < 6; ++j) s = fma(carp_c1(r, j), f(dirn * (j - r)), s);
+    return s;
+}
+template <class A>
+__device__ __forceinline__ double carp2(A f, int r, int dirn) {
+    double s = carp_c2(r, 0) * f(-dirn * r);
+    for (int j = 1; j < 5; ++j) s = fma(carp_c2(r, j), f(dirn * (j - r)), s);
+    return s;
+}
+// central derivatives at index idx of n along a direction with closure mask
+template <class A>
+__device__ __forceinline__ double d1c(A f, double invd, int idx, int n, int mask) {
+    if (mask) {
+        if ((mask & 1) && idx < 4) return carp1(f, idx, 1) * invd;
+        if ((mask & 2) && idx >= n - 4) return -carp1(f, n - 1 - idx, -1) * invd;
+    }
+    return d1(f, invd);
+}
+template <class A>
+__device__ __forceinline__ double d2c(A f, double inv2d, int idx, int n, int mask) {
+    if (mask) {
+        if ((mask & 1) && idx < 2) return carp2(f, idx, 1) * inv2d;
+        if ((mask & 2) && idx >= n - 2) return carp2(f, n - 1 - idx, -1) * inv2d;
+    }
+    return d2(f, inv2d);
+}
+
 // Stored velocity gradients g[m*NDIM+j] = D_j u_m on [0,n) along j and
 // [-2, n+2) across (StoreSome.py:87-135; TENO6/opensbli.cpp:340-384).
 template <int NDIM>
@@ -449,7 +498,7 @@ __global__ void __launch_bounds__(BLOCK) k_gradients(const double* __restrict__ 
 #pragma unroll
         for (int m = 0; m < NDIM; ++m) {
             const double* um = cr + (long)m * g.Ptot;
-            gr[(long)(m * NDIM + jd) * g.Ptot + x] = d1([&](int s) { return um[x + s * st]; }, c.invd[jd]);
+            gr[(long)(m * NDIM + jd) * g.Ptot + x] = d1c([&](int s) { return um[x + s * st]; }, c.invd[jd], ii[jd], g.n[jd], c.carp[jd]);
         }
     }
 }
@@ -470,6 +519,7 @@ __global__ void __launch_bounds__(BLOCK) k_central_convective(const double* __re
     if (t >= (long)e[0] * e[1] * e[2]) return;
     int i, j, k;
     unflatten(t, e, lo, i, j, k);
+    const int ii[3] = {i, j, k};
     const long x = gidx(g, i, j, k);
     const long P = g.Ptot;
     const double* rho = q;
@@ -483,6 +533,7 @@ __global__ void __launch_bounds__(BLOCK) k_central_convective(const double* __re
     for (int jd = 0; jd < NDIM; ++jd) {
         const long st = g.s[jd];
         const double inv = c.invd[jd];
+        const int id = ii[jd], nn = g.n[jd], cm = c.carp[jd];
         const double* rhou_j = q + (long)(1 + jd) * P;
         const double* u_j = cr + (long)jd * P;
         auto at = [&](const double* a, int s) { return a[x + s * st]; };
@@ -492,8 +543,8 @@ __global__ void __launch_bounds__(BLOCK) k_central_convective(const double* __re
             const double ujc = at(u_j, 0), rhoujc = at(rhou_j, 0);
             const double gjj = gr[(long)(jd * NDIM + jd) * P + x];
             auto Hs = [&](int s) { return (at(rhoE, s) + at(pp, s)) / at(rho, s); };
-            const double d_rhouj = d1([&](int s) { return at(rhou_j, s); }, inv);
-            const double d_rho = d1([&](int s) { return at(rho, s); }, inv);
+            const double d_rhouj = d1c([&](int s) { return at(rhou_j, s); }, inv, id, nn, cm);
+            const double d_rho = d1c([&](int s) { return at(rho, s); }, inv, id, nn, cm);
             Rrho += 0.5 * d_rhouj + 0.5 * (ujc * d_rho + rho[x] * gjj);
 #pragma unroll
             for (int m = 0; m < NDIM; ++m) {
@@ -501,55 +552,55 @@ __global__ void __launch_bounds__(BLOCK) k_central_convective(const double* __re
                 const double* u_m = cr + (long)m * P;
                 const double umc = at(u_m, 0), rhoumc = at(rhou_m, 0);
                 const double g_mj = gr[(long)(m * NDIM + jd) * P + x];
-                const double d_rhouj_um = d1([&](int s) { return at(u_m, s) * at(rhou_j, s); }, inv);
-                const double d_rhoum = d1([&](int s) { return at(rhou_m, s); }, inv);
-                const double d_ujum = d1([&](int s) { return at(u_j, s) * at(u_m, s); }, inv);
+                const double d_rhouj_um = d1c([&](int s) { return at(u_m, s) * at(rhou_j, s); }, inv, id, nn, cm);
+                const double d_rhoum = d1c([&](int s) { return at(rhou_m, s); }, inv, id, nn, cm);
+                const double d_ujum = d1c([&](int s) { return at(u_j, s) * at(u_m, s); }, inv, id, nn, cm);
                 conv[m] += 0.25 * ((((((d_rhouj_um + umc * d_rhouj) + rhoujc * g_mj) + ujc * d_rhoum) + rhoumc * gjj) +
                                     rho[x] * d_ujum) + (umc * ujc) * d_rho);
             }
             const double Hc = Hs(0);
-            const double d_Hrhouj = d1([&](int s) { return Hs(s) * at(rhou_j, s); }, inv);
-            const double d_H = d1(Hs, inv);
-            const double d_Hrho = d1([&](int s) { return Hs(s) * at(rho, s); }, inv);
-            const double d_Huj = d1([&](int s) { return Hs(s) * at(u_j, s); }, inv);
+            const double d_Hrhouj = d1c([&](int s) { return Hs(s) * at(rhou_j, s); }, inv, id, nn, cm);
+            const double d_H = d1c(Hs, inv, id, nn, cm);
+            const double d_Hrho = d1c([&](int s) { return Hs(s) * at(rho, s); }, inv, id, nn, cm);
+            const double d_Huj = d1c([&](int s) { return Hs(s) * at(u_j, s); }, inv, id, nn, cm);
             eterm += 0.25 * ((((((d_Hrhouj + Hc * d_rhouj) + rhoujc * d_H) + ujc * d_Hrho) + (rho[x] * Hc) * gjj) +
                               rho[x] * d_Huj) + (ujc * Hc) * d_rho);
         } else if (FORM == CC_FEIEREISEN) {
-            const double d_rhouj = d1([&](int s) { return at(rhou_j, s); }, inv);
+            const double d_rhouj = d1c([&](int s) { return at(rhou_j, s); }, inv, id, nn, cm);
             div_rhou += d_rhouj;
 #pragma unroll
             for (int m = 0; m < NDIM; ++m) {
                 const double* rhou_m = q + (long)(1 + m) * P;
-                const double d_prod = d1([&](int s) { return at(rhou_m, s) * at(u_j, s); }, inv);
+                const double d_prod = d1c([&](int s) { return at(rhou_m, s) * at(u_j, s); }, inv, id, nn, cm);
                 conv[m] += d_prod + at(rhou_j, 0) * gr[(long)(m * NDIM + jd) * P + x];
             }
-            const double d_Eu = d1([&](int s) { return at(rhoE, s) * at(u_j, s); }, inv);
-            const double d_E = d1([&](int s) { return at(rhoE, s) / at(rho, s); }, inv);
-            const double d_pu = d1([&](int s) { return at(pp, s) * at(u_j, s); }, inv);
+            const double d_Eu = d1c([&](int s) { return at(rhoE, s) * at(u_j, s); }, inv, id, nn, cm);
+            const double d_E = d1c([&](int s) { return at(rhoE, s) / at(rho, s); }, inv, id, nn, cm);
+            const double d_pu = d1c([&](int s) { return at(pp, s) * at(u_j, s); }, inv, id, nn, cm);
             eterm += -0.5 * (d_Eu + at(rhou_j, 0) * d_E) - d_pu;
         } else {
             const double ujc = at(u_j, 0);
-            const double d_rhou = d1([&](int s) { return at(rho, s) * at(u_j, s); }, inv);
-            const double d_rho = d1([&](int s) { return at(rho, s); }, inv);
-            const double d_u = d1([&](int s) { return at(u_j, s); }, inv);
+            const double d_rhou = d1c([&](int s) { return at(rho, s) * at(u_j, s); }, inv, id, nn, cm);
+            const double d_rho = d1c([&](int s) { return at(rho, s); }, inv, id, nn, cm);
+            const double d_u = d1c([&](int s) { return at(u_j, s); }, inv, id, nn, cm);
             Rrho -= 0.5 * (d_rhou + ujc * d_rho + rho[x] * d_u);
 #pragma unroll
             for (int m = 0; m < NDIM; ++m) {
                 const double* rhou_m = q + (long)(1 + m) * P;
-                const double d_prod = d1([&](int s) { return at(rhou_m, s) * at(u_j, s); }, inv);
-                const double d_rm = d1([&](int s) { return at(rhou_m, s); }, inv);
+                const double d_prod = d1c([&](int s) { return at(rhou_m, s) * at(u_j, s); }, inv, id, nn, cm);
+                const double d_rm = d1c([&](int s) { return at(rhou_m, s); }, inv, id, nn, cm);
                 conv[m] += 0.5 * (d_prod + ujc * d_rm + at(rhou_m, 0) * d_u);
             }
-            const double d_Eu = d1([&](int s) { return at(rhoE, s) * at(u_j, s); }, inv);
-            const double d_rE = d1([&](int s) { return at(rhoE, s); }, inv);
-            const double d_pu = d1([&](int s) { return at(pp, s) * at(u_j, s); }, inv);
+            const double d_Eu = d1c([&](int s) { return at(rhoE, s) * at(u_j, s); }, inv, id, nn, cm);
+            const double d_rE = d1c([&](int s) { return at(rhoE, s); }, inv, id, nn, cm);
+            const double d_pu = d1c([&](int s) { return at(pp, s) * at(u_j, s); }, inv, id, nn, cm);
             eterm += -0.5 * (d_Eu + ujc * d_rE + rhoE[x] * d_u) - d_pu;
         }
     }
 #pragma unroll
     for (int m = 0; m < NDIM; ++m) {
         const long st = g.s[m];
-        const double dp = d1([&](int s) { return pp[x + s * st]; }, c.invd[m]);
+        const double dp = d1c([&](int s) { return pp[x + s * st]; }, c.invd[m], ii[m], g.n[m], c.carp[m]);
         double rm;
         if (FORM == CC_FEIEREISEN) rm = -dp - 0.5 * (conv[m] + cr[(long)m * P + x] * div_rhou);
         else rm = -dp - conv[m];
@@ -580,6 +631,7 @@ __global__ void __launch_bounds__(BLOCK) k_viscous(const double* __restrict__ cr
     if (t >= (long)e[0] * e[1] * e[2]) return;
     int i, j, k;
     unflatten(t, e, lo, i, j, k);
+    const int ii[3] = {i, j, k};
     const long x = gidx(g, i, j, k);
     const long P = g.Ptot;
     const double* Ta = cr + (long)(NDIM + 1) * P;
@@ -593,13 +645,14 @@ __global__ void __launch_bounds__(BLOCK) k_viscous(const double* __restrict__ cr
 #pragma unroll
     for (int jd = 0; jd < NDIM; ++jd) {
         const long st = g.s[jd];
-        dmu[jd] = SUTH ? d1([&](int s) { return mua[x + s * st]; }, c.invd[jd]) : 0.0;
-        dT[jd] = d1([&](int s) { return Ta[x + s * st]; }, c.invd[jd]);
-        lapT += d2([&](int s) { return Ta[x + s * st]; }, c.inv2d[jd]);
+        const int id = ii[jd], nn = g.n[jd], cm = c.carp[jd];
+        dmu[jd] = SUTH ? d1c([&](int s) { return mua[x + s * st]; }, c.invd[jd], id, nn, cm) : 0.0;
+        dT[jd] = d1c([&](int s) { return Ta[x + s * st]; }, c.invd[jd], id, nn, cm);
+        lapT += d2c([&](int s) { return Ta[x + s * st]; }, c.inv2d[jd], id, nn, cm);
 #pragma unroll
         for (int m = 0; m < NDIM; ++m) {
             const double* um = cr + (long)m * P;
-            D2u[m][jd] = d2([&](int s) { return um[x + s * st]; }, c.inv2d[jd]);
+            D2u[m][jd] = d2c([&](int s) { return um[x + s * st]; }, c.inv2d[jd], id, nn, cm);
         }
     }
     double div = 0.0;
@@ -616,7 +669,7 @@ __global__ void __launch_bounds__(BLOCK) k_viscous(const double* __restrict__ cr
             const int a = jd < m ? jd : m, b = jd < m ? m : jd;
             const double* ga = gr + (long)(jd * NDIM + a) * P;
             const long sb = g.s[b];
-            const double mixed = d1([&](int s) { return ga[x + s * sb]; }, c.invd[b]);
+            const double mixed = d1c([&](int s) { return ga[x + s * sb]; }, c.invd[b], ii[b], g.n[b], c.carp[b]);
             Lm += D2u[m][jd] + (1.0 / 3.0) * mixed;
         }
         double S[NDIM];

@@ -73,6 +73,24 @@ def sod(n: int = 1024, steps: int = 1000, dt: float = 2.0e-4, precision: int = n
     return Problem("sod", desc, q, steps)
 
 
+def sod_filtered(n: int = 200, steps: int = 1000, dt: float = 2.0e-4) -> Problem:
+    """The Sod verification app's default method
+    (tests/verification_apps/sod_shock_tube/verify_sod_shock_tube.py:37-191):
+    Central(4) Feiereisen split (StoreSome 'u0'), inviscid, the WENO-Z filter
+    after each step, RungeKuttaLS(3, 'SSP'), Dirichlet faces holding the left
+    and right states (so the central derivatives next to them use Carpenter's
+    closure), N = 200, dt = 2e-4, 1000 steps."""
+    pr = sod(n=n, steps=steps, dt=dt, boundaries="dirichlet")
+    # RungeKuttaLS(3, formulation='SSP'): the 3-stage 2N-storage SSP coefficients
+    from .surface import RungeKuttaLS
+    ssp = RungeKuttaLS(3, formulation="SSP")
+    d = pr.desc
+    d.conv_scheme, d.order = nat.CONV_CENTRAL_FEIEREISEN, 4
+    d.rk_a, d.rk_b = list(ssp.rk_a), list(ssp.rk_b)
+    d.shock_filter = nat.FILTER_WENOZ
+    return Problem(f"sod_filtered_{n}", d, pr.q0, steps)
+
+
 # ---------------------------------------------------------------------------
 # 3-D Taylor-Green vortex (apps/taylor_green_vortex/taylor_green_vortex.py,
 # apps/compressible_taylor_green_vortex/compressible_TGV.py)

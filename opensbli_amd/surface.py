@@ -205,6 +205,12 @@ class SymmetryBC(BoundaryConditionBase):
         self.scheme = scheme
 
 
+class Carpenter:
+    """opensbli/core/boundary_conditions/Carpenter_scheme.py:6 -- the 4th-order
+    one-sided closure the central derivatives switch to next to extrapolation
+    and Dirichlet faces (the default ``modification_scheme`` of those BCs)."""
+
+
 class ExtrapolationBC(BoundaryConditionBase):
     """extrapolation.py:8.  order 0: the boundary point and the halos take the
     first interior point; order 1: linear extrapolation into the halos."""
@@ -214,7 +220,7 @@ class ExtrapolationBC(BoundaryConditionBase):
         if order not in (0, 1):
             raise ValueError("Only zeroth order and linear extrapolation currently implemented.")
         self.order = order
-        self.scheme = scheme
+        self.modification_scheme = scheme if scheme else Carpenter()
 
 
 class DirichletBC(BoundaryConditionBase):
@@ -229,7 +235,7 @@ class DirichletBC(BoundaryConditionBase):
     def __init__(self, direction, side, equations, scheme=None, plane=True):
         super().__init__(direction, side, plane)
         self.equations = equations
-        self.scheme = scheme
+        self.modification_scheme = scheme if scheme else Carpenter()
 
 
 # --------------------------------------------------------------------------

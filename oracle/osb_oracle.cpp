@@ -49,6 +49,7 @@ struct osb_ctx {
     size_t Ptot = 0;
     int depth = 0;                 // boundary halo depth (symmetric max of scheme halos)
     bool shock = false, central = false, viscous = false;
+    int carp[3] = {0, 0, 0};       // Carpenter closure per direction: bit 0 side 0, bit 1 side 1
 
     std::vector<double> q, rkold, R;
     std::vector<double> u, p, a, T, mu;   // constituent relations
@@ -555,6 +556,52 @@ inline double d2(G f, double inv2d) {
     return std::fma(-1.0 / 12.0, f(-2) + f(2), std::fma(4.0 / 3.0, f(-1) + f(1), (-5.0 / 2.0) * f(0))) * inv2d;
 }
 
+// Carpenter's 4th-order boundary closure (Carpenter_scheme.py:6-125; Carpenter,
+// Nordstrom & Gottlieb, J. Comput. Phys. 148 (1999)), replacing the central
+// derivatives at the grid points next to faces whose boundary condition is a
+// ModifyCentralDerivative (ExtrapolationBC, DirichletBC; bc_core.py:15-18,
+// opensblifunctions.py:486-534).  First derivative at boundary point r = 0..3
+// (counted from the face): sum_j C1[r][j] f(point j) / Delta over points
+// j = 0..5 from the face, negated on side 1; second derivative at r = 0, 1:
+// sum_j C2[r][j] f(point j) / Delta^2 over j = 0..4.  C1 = al4^-1 ar4 of
+// Carpenter_scheme.py:97-121 (row 0 simplified to the exact rationals as
+// there), here correctly rounded from a 40-digit evaluation.
+const double CARP_D1[4][6] = {
+    {-11.0 / 6.0, 3.0, -1.5, 1.0 / 3.0, 0.0, 0.0},
+    {-0.3762836775133548, -0.3224849328821596, 0.719443173328853, 0.03941685243994666, -0.0658051057710398, 0.005713690397754591},
+    {0.11344647038423929, -0.7912455927658653, 0.19718433388773493, 0.521455851089594, -0.0367146847001282, -0.004126377895574793},
+    {-0.009325979850499882, 0.12193715322406413, -0.7278221477245911, 0.045103322334387165, 0.6521410848612417, -0.08203343284460206}};
+const double CARP_D2[2][5] = {{35.0 / 12.0, -104.0 / 12.0, 114.0 / 12.0, -56.0 / 12.0, 11.0 / 12.0},
+                              {11.0 / 12.0, -20.0 / 12.0, 6.0 / 12.0, 4.0 / 12.0, -1.0 / 12.0}};
+
+// the closure sums; dirn = +1 on side 0 (points j - r from x), -1 on side 1 (r - j)
+template <class G>
+inline double carp1(G f, int r, int dirn) {
+    double s = CARP_D1[r][0] * f(-dirn * r);
+    for (int j = 1; j < 6; ++j) s = std::fma(CARP_D1[r][j], f(dirn * (j - r)), s);
+    return s;
+}
+template <class G>
+inline double carp2(G f, int r, int dirn) {
+    double s = CARP_D2[r][0] * f(-dirn * r);
+    for (int j = 1; j < 5; ++j) s = std::fma(CARP_D2[r][j], f(dirn * (j - r)), s);
+    return s;
+}
+// first / second derivative at index idx of n along a direction whose faces
+// carry the closure where mask bit 0 (side 0) / bit 1 (side 1) is set
+template <class G>
+inline double d1c(G f, double invd, int idx, int n, int mask) {
+    if ((mask & 1) && idx < 4) return carp1(f, idx, 1) * invd;
+    if ((mask & 2) && idx >= n - 4) return -carp1(f, n - 1 - idx, -1) * invd;
+    return d1(f, invd);
+}
+template <class G>
+inline double d2c(G f, double inv2d, int idx, int n, int mask) {
+    if ((mask & 1) && idx < 2) return carp2(f, idx, 1) * inv2d;
+    if ((mask & 2) && idx >= n - 2) return carp2(f, n - 1 - idx, -1) * inv2d;
+    return d2(f, inv2d);
+}
+
 // Central convective residual, written first (StoreSome.py:138-228; scheme.py:321-350).
 void central_convective(osb_ctx* c) {
     const int nd = c->ndim;
@@ -574,6 +621,7 @@ void central_convective(osb_ctx* c) {
             const size_t st = c->stride(jd);
             const double invd = 1.0 / c->d.delta[jd];
             auto at = [&](const double* arr, int s) { return arr[x + (long)s * (long)st]; };
+            const int id = jd == 0 ? i : (jd == 1 ? j : k), nn = c->n[jd], cm = c->carp[jd];
             const double* rhou_j = q + (size_t)(1 + jd) * Pt;
             const double* u_j = c->u.data() + (size_t)jd * Pt;
             const double* rhoEa = q + (size_t)(nd + 1) * Pt;
@@ -586,58 +634,58 @@ void central_convective(osb_ctx* c) {
                 const double ujc = at(u_j, 0), rhoujc = at(rhou_j, 0);
                 const double gjj = c->g[(size_t)(jd * nd + jd) * Pt + x];
                 auto Hs = [&](int s) { return (at(rhoEa, s) + at(pa, s)) / at(q, s); };
-                const double d_rhouj = d1([&](int s) { return at(rhou_j, s); }, invd);
-                const double d_rho = d1([&](int s) { return at(q, s); }, invd);
+                const double d_rhouj = d1c([&](int s) { return at(rhou_j, s); }, invd, id, nn, cm);
+                const double d_rho = d1c([&](int s) { return at(q, s); }, invd, id, nn, cm);
                 Rrho += 0.5 * d_rhouj + 0.5 * (ujc * d_rho + rho * gjj);
                 for (int m = 0; m < nd; ++m) {
                     const double* rhou_m = q + (size_t)(1 + m) * Pt;
                     const double* u_m = c->u.data() + (size_t)m * Pt;
                     const double umc = at(u_m, 0), rhoumc = at(rhou_m, 0);
                     const double g_mj = c->g[(size_t)(m * nd + jd) * Pt + x];
-                    const double d_rhouj_um = d1([&](int s) { return at(u_m, s) * at(rhou_j, s); }, invd);
-                    const double d_rhoum = d1([&](int s) { return at(rhou_m, s); }, invd);
-                    const double d_ujum = d1([&](int s) { return at(u_j, s) * at(u_m, s); }, invd);
+                    const double d_rhouj_um = d1c([&](int s) { return at(u_m, s) * at(rhou_j, s); }, invd, id, nn, cm);
+                    const double d_rhoum = d1c([&](int s) { return at(rhou_m, s); }, invd, id, nn, cm);
+                    const double d_ujum = d1c([&](int s) { return at(u_j, s) * at(u_m, s); }, invd, id, nn, cm);
                     sum_conv[m] += 0.25 * ((((((d_rhouj_um + umc * d_rhouj) + rhoujc * g_mj) + ujc * d_rhoum) + rhoumc * gjj) +
                                             rho * d_ujum) + (umc * ujc) * d_rho);
                 }
                 const double Hc = Hs(0);
-                const double d_Hrhouj = d1([&](int s) { return Hs(s) * at(rhou_j, s); }, invd);
-                const double d_H = d1(Hs, invd);
-                const double d_Hrho = d1([&](int s) { return Hs(s) * at(q, s); }, invd);
-                const double d_Huj = d1([&](int s) { return Hs(s) * at(u_j, s); }, invd);
+                const double d_Hrhouj = d1c([&](int s) { return Hs(s) * at(rhou_j, s); }, invd, id, nn, cm);
+                const double d_H = d1c(Hs, invd, id, nn, cm);
+                const double d_Hrho = d1c([&](int s) { return Hs(s) * at(q, s); }, invd, id, nn, cm);
+                const double d_Huj = d1c([&](int s) { return Hs(s) * at(u_j, s); }, invd, id, nn, cm);
                 eterm += 0.25 * ((((((d_Hrhouj + Hc * d_rhouj) + rhoujc * d_H) + ujc * d_Hrho) + (rho * Hc) * gjj) + rho * d_Huj) +
                                  (ujc * Hc) * d_rho);
             } else if (!skew) {
                 // Feiereisen (split_forms.py:111-144)
-                const double d_rhouj = d1([&](int s) { return at(rhou_j, s); }, invd);
+                const double d_rhouj = d1c([&](int s) { return at(rhou_j, s); }, invd, id, nn, cm);
                 div_rhou += d_rhouj;
                 for (int m = 0; m < nd; ++m) {
                     const double* rhou_m = q + (size_t)(1 + m) * Pt;
-                    const double d_prod = d1([&](int s) { return at(rhou_m, s) * at(u_j, s); }, invd);
+                    const double d_prod = d1c([&](int s) { return at(rhou_m, s) * at(u_j, s); }, invd, id, nn, cm);
                     const double g_mj = c->g[(size_t)(m * nd + jd) * Pt + x];   // stored D_j u_m
                     sum_conv[m] += d_prod + at(rhou_j, 0) * g_mj;
                 }
-                const double d_rhoEu = d1([&](int s) { return at(rhoEa, s) * at(u_j, s); }, invd);
-                const double d_E = d1([&](int s) { return at(rhoEa, s) / at(q, s); }, invd);
-                const double d_pu = d1([&](int s) { return at(pa, s) * at(u_j, s); }, invd);
+                const double d_rhoEu = d1c([&](int s) { return at(rhoEa, s) * at(u_j, s); }, invd, id, nn, cm);
+                const double d_E = d1c([&](int s) { return at(rhoEa, s) / at(q, s); }, invd, id, nn, cm);
+                const double d_pu = d1c([&](int s) { return at(pa, s) * at(u_j, s); }, invd, id, nn, cm);
                 eterm += -0.5 * (d_rhoEu + at(rhou_j, 0) * d_E) - d_pu;
             } else {
                 // Skew(ab, x) = 1/2 (D(ab) + b D(a) + a D(b))  (parsing.py:96-112), tg_sym equations
                 const double* rhoa = q;
                 const double ujc = at(u_j, 0);
-                const double d_rhou = d1([&](int s) { return at(rhoa, s) * at(u_j, s); }, invd);
-                const double d_rho = d1([&](int s) { return at(rhoa, s); }, invd);
-                const double d_u = d1([&](int s) { return at(u_j, s); }, invd);
+                const double d_rhou = d1c([&](int s) { return at(rhoa, s) * at(u_j, s); }, invd, id, nn, cm);
+                const double d_rho = d1c([&](int s) { return at(rhoa, s); }, invd, id, nn, cm);
+                const double d_u = d1c([&](int s) { return at(u_j, s); }, invd, id, nn, cm);
                 Rrho -= 0.5 * (d_rhou + ujc * d_rho + rho * d_u);
                 for (int m = 0; m < nd; ++m) {
                     const double* rhou_m = q + (size_t)(1 + m) * Pt;
-                    const double d_prod = d1([&](int s) { return at(rhou_m, s) * at(u_j, s); }, invd);
-                    const double d_rm = d1([&](int s) { return at(rhou_m, s); }, invd);
+                    const double d_prod = d1c([&](int s) { return at(rhou_m, s) * at(u_j, s); }, invd, id, nn, cm);
+                    const double d_rm = d1c([&](int s) { return at(rhou_m, s); }, invd, id, nn, cm);
                     sum_conv[m] += 0.5 * (d_prod + ujc * d_rm + at(rhou_m, 0) * d_u);
                 }
-                const double d_Eu = d1([&](int s) { return at(rhoEa, s) * at(u_j, s); }, invd);
-                const double d_rE = d1([&](int s) { return at(rhoEa, s); }, invd);
-                const double d_pu = d1([&](int s) { return at(pa, s) * at(u_j, s); }, invd);
+                const double d_Eu = d1c([&](int s) { return at(rhoEa, s) * at(u_j, s); }, invd, id, nn, cm);
+                const double d_rE = d1c([&](int s) { return at(rhoEa, s); }, invd, id, nn, cm);
+                const double d_pu = d1c([&](int s) { return at(pa, s) * at(u_j, s); }, invd, id, nn, cm);
                 eterm += -0.5 * (d_Eu + ujc * d_rE + rhoE * d_u) - d_pu;
             }
         }
@@ -645,7 +693,8 @@ void central_convective(osb_ctx* c) {
             const size_t st = c->stride(m);
             const double invd = 1.0 / c->d.delta[m];
             const double* pa = c->p.data();
-            const double dp = d1([&](int s) { return pa[x + (long)s * (long)st]; }, invd);
+            const int id = m == 0 ? i : (m == 1 ? j : k);
+            const double dp = d1c([&](int s) { return pa[x + (long)s * (long)st]; }, invd, id, c->n[m], c->carp[m]);
             if (!skew && !kgp) {
                 const double um = c->u[(size_t)m * Pt + x];
                 Rm[m] = -dp - 0.5 * (sum_conv[m] + um * div_rhou);
@@ -685,7 +734,8 @@ void stored_gradients(osb_ctx* c) {
             double* gm = c->g.data() + (size_t)(m * nd + jd) * Pt;
             for_range(rr, [&](int i, int j, int k) {
                 const size_t x = c->I(i, j, k);
-                gm[x] = d1([&](int s) { return um[x + (long)s * (long)st]; }, invd);
+                const int id = jd == 0 ? i : (jd == 1 ? j : k);
+                gm[x] = d1c([&](int s) { return um[x + (long)s * (long)st]; }, invd, id, c->n[jd], c->carp[jd]);
             });
         }
     }
@@ -715,12 +765,13 @@ void viscous_terms(osb_ctx* c) {
             const double invd = 1.0 / c->d.delta[jd];
             const double inv2d = 1.0 / (c->d.delta[jd] * c->d.delta[jd]);
             auto at = [&](const double* arr, int s) { return arr[x + (long)s * (long)st]; };
-            if (suth) dmu[jd] = d1([&](int s) { return at(c->mu.data(), s); }, invd);
-            dT[jd] = d1([&](int s) { return at(c->T.data(), s); }, invd);
-            lapT += d2([&](int s) { return at(c->T.data(), s); }, inv2d);
+            const int id = jd == 0 ? i : (jd == 1 ? j : k), nn = c->n[jd], cm = c->carp[jd];
+            if (suth) dmu[jd] = d1c([&](int s) { return at(c->mu.data(), s); }, invd, id, nn, cm);
+            dT[jd] = d1c([&](int s) { return at(c->T.data(), s); }, invd, id, nn, cm);
+            lapT += d2c([&](int s) { return at(c->T.data(), s); }, inv2d, id, nn, cm);
             for (int m = 0; m < nd; ++m) {
                 const double* um = c->u.data() + (size_t)m * Pt;
-                D2u[m][jd] = d2([&](int s) { return at(um, s); }, inv2d);
+                D2u[m][jd] = d2c([&](int s) { return at(um, s); }, inv2d, id, nn, cm);
             }
         }
         // mixed(m, a, b) = D_b(D_a u_m), a < b
@@ -728,7 +779,8 @@ void viscous_terms(osb_ctx* c) {
             if (a > b) std::swap(a, b);
             const double* ga = c->g.data() + (size_t)(m * nd + a) * Pt;
             const size_t st = c->stride(b);
-            return d1([&](int s) { return ga[x + (long)s * (long)st]; }, 1.0 / c->d.delta[b]);
+            const int id = b == 0 ? i : (b == 1 ? j : k);
+            return d1c([&](int s) { return ga[x + (long)s * (long)st]; }, 1.0 / c->d.delta[b], id, c->n[b], c->carp[b]);
         };
         double div = 0.0;
         for (int m = 0; m < nd; ++m) div += G[m][m];
@@ -976,8 +1028,8 @@ int validate(const osb_problem_desc* d, std::string& why) {
         for (int s = 0; s < 2; ++s) {
             const int b = d->bc[i][s];
             if (b < OSB_BC_PERIODIC || b > OSB_BC_DIRICHLET) { why = "unknown boundary condition"; return OSB_E_UNSUPPORTED; }
-            if (b >= OSB_BC_EXTRAPOLATION0 && (central_cs || d->viscous != OSB_VISC_NONE)) {
-                why = "extrapolation/Dirichlet boundaries with central derivatives need the Carpenter closure (Carpenter_scheme.py), not built";
+            if (b >= OSB_BC_EXTRAPOLATION0 && (central_cs || d->viscous != OSB_VISC_NONE) && d->precision != OSB_PREC_FP64) {
+                why = "the Carpenter boundary closure of the central derivatives runs in FP64 only";
                 return OSB_E_UNSUPPORTED;
             }
             if (b == OSB_BC_DIRICHLET)
@@ -989,9 +1041,6 @@ int validate(const osb_problem_desc* d, std::string& why) {
     if (!(d->cfl > 0) && !(d->dt > 0)) { why = "dt > 0 or cfl > 0 required"; return OSB_E_INVALID; }
     if (!(d->gama > 1.0)) { why = "gama > 1 required"; return OSB_E_INVALID; }
     if (d->viscous != OSB_VISC_NONE && (!(d->re > 0) || !(d->pr > 0) || !(d->minf > 0))) { why = "Re, Pr, Minf > 0 required"; return OSB_E_INVALID; }
-    if (central_cs) {
-        if (d->ndim < 2) { why = "central split forms need ndim >= 2"; return OSB_E_UNSUPPORTED; }
-    }
     return OSB_OK;
 }
 
@@ -1039,6 +1088,9 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     c->shock = desc->conv_scheme == OSB_CONV_WENO_Z || desc->conv_scheme == OSB_CONV_TENO;
     c->central = !c->shock;
     c->viscous = desc->viscous != OSB_VISC_NONE;
+    for (int d = 0; d < c->ndim; ++d)
+        for (int sd = 0; sd < 2; ++sd)
+            if (desc->bc[d][sd] >= OSB_BC_EXTRAPOLATION0) c->carp[d] |= 1 << sd;
     // WenoHalos/TenoHalos [-3,4] -> 4; CentralHalos(4) -> 2; the WENO filter adds
     // its WENO halos to the block's (WENO_filter.py:440-441)
     c->depth = (c->shock || desc->shock_filter == OSB_FILTER_WENOZ) ? 4 : 2;
@@ -1335,6 +1387,16 @@ int osb_oracle_glf(const osb_ctx* c, double* out) {
     if (!c || !out) return OSB_E_INVALID;
     for (int d = 0; d < 3; ++d)
         for (int r = 0; r < 3; ++r) out[3 * d + r] = c->glf[d][r];
+    return OSB_OK;
+}
+
+// test hook: the Carpenter closure tables (4 x 6 first-derivative rows, then 2 x 5)
+int osb_oracle_carpenter(double* out) {
+    if (!out) return OSB_E_INVALID;
+    for (int r = 0; r < 4; ++r)
+        for (int j = 0; j < 6; ++j) out[6 * r + j] = CARP_D1[r][j];
+    for (int r = 0; r < 2; ++r)
+        for (int j = 0; j < 5; ++j) out[24 + 5 * r + j] = CARP_D2[r][j];
     return OSB_OK;
 }
 

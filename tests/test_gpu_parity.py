@@ -349,3 +349,31 @@ def test_weno_filter_stages_and_graphs(hip, oracle, monkeypatch):
     for a, b in zip(out["1"], out["0"]):
         assert np.array_equal(a, b)
     assert np.array_equal(qs, out["0"][-1]) and np.array_equal(qs, qo)
+
+
+# --- Carpenter boundary closure next to extrapolation / Dirichlet faces ------------
+def _faces(pr, bc, values=None, name=""):
+    d = nat.ProblemDesc(**{**pr.desc.__dict__, "bc": bc, "bc_value": values or {}})
+    return P.Problem(pr.name + name, d, pr.q0, pr.steps)
+
+
+_EXT3 = [[nat.BC_EXTRAPOLATION0, nat.BC_EXTRAPOLATION1], [nat.BC_PERIODIC] * 2, [nat.BC_EXTRAPOLATION1, nat.BC_EXTRAPOLATION0]]
+
+
+def test_residual_closure_tgv(hip, oracle):
+    _residual_compare(hip, oracle, _faces(P.tgv(n=20), _EXT3, name="_ext"))
+
+
+@pytest.mark.parametrize("maker", [
+    lambda: P.sod_filtered(n=200, steps=100),                                       # 1-D central + filter + Dirichlet
+    lambda: _faces(P.tgv(n=16, steps=4), _EXT3, name="_ext"),                       # 3-D Feiereisen viscous, unfused
+    lambda: _faces(P.tgv(n=16, steps=4, scheme="kgp"),
+                   [[nat.BC_DIRICHLET] * 2, [nat.BC_PERIODIC] * 2, [nat.BC_PERIODIC] * 2],
+                   {(0, 0): [1.0, 0.0, 0.0, 0.0, 179.0], (0, 1): [1.0, 0.0, 0.0, 0.0, 179.0]}, "_dir"),   # 3-D KGP
+    lambda: _faces(P.tgv(n=16, steps=4, scheme="teno", minf=1.25, cfl=0.5, dt=0.0), _EXT3, name="_ext"),  # shock + viscous
+    lambda: _faces(P.entropy_wave(n=40, profile="sine", steps=6, filtered=False),
+                   [[nat.BC_EXTRAPOLATION1] * 2, [nat.BC_PERIODIC] * 2], name="_ext"),                  # 2-D inviscid
+])
+def test_closure_parity(hip, oracle, maker):
+    a, b = _compare(hip, oracle, maker())
+    assert np.array_equal(a, b)

@@ -167,13 +167,34 @@ def test_dirichlet_unsupported_forms_raise(bcs, match):
         plan_problem(alg, _SOD_PARAMS)
 
 
-def test_extrapolation_with_central_derivatives_raises():
-    NS = NS_Split("Feiereisen", 2, ["Re", "Pr", "gama", "Minf"], viscosity="constant")
-    alg = _block(2, [StoreSome(4, "u0 u1"), RungeKuttaLS(3)],
-                 [ExtrapolationBC(0, 0, order=0), ExtrapolationBC(0, 1, order=0), PeriodicBC(1, 0), PeriodicBC(1, 1)])
-    with pytest.raises(NotImplementedError, match="Carpenter"):
-        plan_problem(alg, {"block0np0": 16, "block0np1": 16, "Delta0block0": 0.1, "Delta1block0": 0.1, "dt": 1e-3},
-                     physics=NS)
+def test_sod_verification_app_plan_and_run(oracle):
+    """verify_sod_shock_tube.py's default branch built with the surface: StoreSome(4,
+    'u0'), NS_Split('Feiereisen', inviscid), DirichletBC faces from the script's
+    equations, RungeKuttaLS(3, 'SSP'), WENOFilter(order=5, 'Z', 'LLF'); the plan
+    equals problems.sod_filtered and the run through HIPTarget equals its run."""
+    from opensbli_amd.surface import WENOFilter
+    block = SimulationBlock(1, block_number=0)
+    left = ["Eq(GridVariable(d), 1.0)", "Eq(GridVariable(u0), 0.0)", "Eq(GridVariable(p), 1.0)",
+            "Eq(DataObject(rho), d)", "Eq(DataObject(rhou0), d*u0)", "Eq(DataObject(rhoE), p/(gama-1.0) + 0.5* d *(u0**2.0))"]
+    right = ["Eq(GridVariable(d), 0.125)", "Eq(GridVariable(u0), 0.0)", "Eq(GridVariable(p), 0.1)",
+             "Eq(DataObject(rho), d)", "Eq(DataObject(rhou0), d*u0)", "Eq(DataObject(rhoE), p/(gama-1.0) + 0.5* d *(u0**2.0))"]
+    block.set_block_boundaries([DirichletBC(0, 0, left), DirichletBC(0, 1, right)])
+    block.set_equations(WENOFilter(block, order=5, formulation="Z", flux_type="LLF", airfoil=False,
+                                   optimize=False).equation_classes)
+    block.set_discretisation_schemes({"cent": StoreSome(4, "u0"), "rk": RungeKuttaLS(3, formulation="SSP")})
+    block.discretise()
+    alg = TraditionalAlgorithmRK(block)
+    params = {"gama": "1.4", "dt": "0.0002", "niter": "40", "block0np0": "200", "Delta0block0": "1.0/(block0np0-1)",
+              "eps": "1.0e-16", "inv_rfact0_block0": "1.0/Delta0block0", "lambda0_TVD": "dt/Delta0block0",
+              "TENO_CT": "1.0e-5"}
+    NS = NS_Split("Feiereisen", 1, ["gama", "Minf"], viscosity="inviscid", energy_formulation="enthalpy")
+    pr = P.sod_filtered(n=200, steps=40)
+    _same(plan_problem(alg, params, physics=NS), pr.desc)
+    t = HIPTarget(alg, parameters=params, physics=NS, library=oracle)
+    q = t.initialise(pr.q0).run().state()
+    t.close()
+    want, _ = run(oracle, pr)
+    assert np.array_equal(q, want)
 
 
 def test_rk_ssp_plan_and_run(oracle):
