@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define OSB_ABI_VERSION 4
+#define OSB_ABI_VERSION 5
 
 /* error codes */
 #define OSB_OK            0
@@ -65,6 +65,11 @@ extern "C" {
 #define OSB_FILTER_WENOZ 1            /* WENOFilter(order=5, formulation='Z', flux_type='LLF'):
                                          q += dt * (-div of the sensor-weighted dissipative
                                          part of characteristic WENO-Z, RoeAverage)   */
+
+/* hybrid selection of the WENO filter (WENOFilter(optimize=...)) */
+#define OSB_SENSOR_NONE   0           /* optimize=False: every face and point         */
+#define OSB_SENSOR_DUCROS 1           /* optimize=True: modified Ducros sensor kappa,
+                                         shock_sensors.py:16-59                       */
 
 /* time integration */
 #define OSB_TIME_RKLS3 1              /* RungeKuttaLS  rk_LS.py:17 (2N-storage; see rk_stages) */
@@ -122,7 +127,15 @@ typedef struct osb_problem_desc {
     double   bc_value[3][2][5];/* OSB_BC_DIRICHLET: conservative values per face,
                                   (rho, rhou0.., rhoE); unused entries ignored     */
     int32_t  shock_filter;     /* OSB_FILTER_*                                     */
-    int32_t  reserved[7];
+    int32_t  filter_order;     /* WENO filter order: 5, or 3 (WENO3-Z); 0 means 5   */
+    int32_t  filter_sensor;    /* OSB_SENSOR_*: the filter's hybrid selection      */
+    int32_t  reserved0;
+    double   ducros_check;     /* OSB_SENSOR_DUCROS: a face is reconstructed where
+                                  max kappa over its stencil > ducros_check
+                                  (WENO_filter.py:561-584, 'Ducros_check')         */
+    double   ducros_select;    /* ... and a point filtered where max kappa over
+                                  it and its +-1, +2 neighbours >= ducros_select
+                                  (WENO_filter.py:486-500, 'Ducros_select')        */
 } osb_problem_desc;
 
 typedef struct osb_stats {

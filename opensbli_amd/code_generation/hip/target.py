@@ -309,18 +309,25 @@ def plan_problem(alg_or_block, parameters: Mapping, physics=None, precision: str
     for f in filters:
         if type(f).__name__ != "WENOFilter":
             raise NotImplementedError(f"{type(f).__name__} is not implemented by the HIP kernels")
-        if f.order != 5 or str(f.formulation).upper() != "Z" or f.flux_type != "LLF" or f.optimize \
+        if f.order not in (3, 5) or str(f.formulation).upper() != "Z" or f.flux_type != "LLF" \
                 or f.metrics is not None or f.species is not None:
             raise NotImplementedError(
                 f"WENOFilter(order={f.order}, formulation={f.formulation!r}, flux_type={f.flux_type!r}, "
-                f"optimize={f.optimize}, metrics={f.metrics!r}): order 5, 'Z', 'LLF', optimize=False, "
-                "uniform mesh only")
+                f"metrics={f.metrics!r}): orders 3 and 5, 'Z', 'LLF', uniform mesh only")
+        if f.optimize and ndim < 2:
+            raise NotImplementedError("WENOFilter(optimize=True): the Ducros sensor needs ndim >= 2")
     if len(filters) > 1:
         raise NotImplementedError("one WENO filter per block")
     if filters:
         if desc.get("flux_type") == nat.FLUX_GLF:
             raise NotImplementedError("a WENO filter over a GLF base scheme: the kernels pair the filter with LLF")
         desc["shock_filter"] = nat.FILTER_WENOZ
+        f = filters[0]
+        desc["filter_order"] = int(f.order)
+        if f.optimize:                       # Ducros hybrid selection (WENO_filter.py:455-458, 561-584)
+            desc["filter_sensor"] = nat.SENSOR_DUCROS
+            desc["ducros_check"] = float(p.get("Ducros_check", 0.05))
+            desc["ducros_select"] = float(p.get("Ducros_select", 0.05))
         for d in range(ndim):
             key = f"inv_rfact{d}_block0"         # the filter's 1/Delta (opensblifunctions.py:577-591)
             if key in p and not math.isclose(float(p[key]), 1.0 / delta[d], rel_tol=1e-12):

@@ -47,6 +47,8 @@ struct Consts {
     // Carpenter boundary closure of the central derivatives per direction
     // (bit 0: side 0, bit 1: side 1), set for extrapolation and Dirichlet faces
     int carp[3];
+    // WENO filter: Ducros thresholds of the hybrid selection
+    double ducros_check, ducros_select;
 };
 
 // ---------------------------------------------------------------------------
@@ -152,6 +154,36 @@ __device__ __forceinline__ T wenoz_left(const T* f, double eps, double* sensor =
     const T e1 = eno3<T>(T(1.0 / 3.0), T(5.0 / 6.0), T(-1.0 / 6.0), f[2], f[3], f[4]);
     const T e2 = eno3<T>(T(-1.0 / 6.0), T(5.0 / 6.0), T(1.0 / 3.0), f[1], f[2], f[3]);
     return fma(T(w2), e2, fma(T(w1), e1, T(w0) * e0));
+}
+
+// WENO3-Z for the WENO filter of order 3 (weno.py:36-136, 290-345 with k = 2:
+// tau = |b0 - b1|, p = 2; generated form in
+// apps/performance_benchmark/Central4-WENO5/opensbliblock00_kernels.h:930-990),
+// FP64, f[s] at offset s-2; beta_r is the squared difference over stencil r and
+// the sensor sum_r |omega_r/d_r - 1| / 3 (weno.py:443-459).
+__device__ __forceinline__ double wenoz3_right(const double* f, double eps, double* sensor) {
+    const double db0 = f[3] - f[2], db1 = f[2] - f[1];          // (i, i+1) d = 2/3; (i-1, i) d = 1/3
+    const double b0 = db0 * db0, b1 = db1 * db1;
+    const double tau = fabs(b0 - b1), t2 = tau * tau;
+    const double a0 = zalpha<double>(2.0 / 3.0, t2, b0, eps), a1 = zalpha<double>(1.0 / 3.0, t2, b1, eps);
+    const double inv = 1.0 / (a0 + a1);
+    const double w0 = a0 * inv, w1 = a1 * inv;
+    *sensor = (fabs(3.0 * w1 - 1.0) + fabs(1.5 * w0 - 1.0)) * (1.0 / 3.0);
+    const double e0 = 0.5 * f[2] + 0.5 * f[3];
+    const double e1 = fma(-0.5, f[1], 1.5 * f[2]);
+    return fma(w1, e1, w0 * e0);
+}
+__device__ __forceinline__ double wenoz3_left(const double* f, double eps, double* sensor) {
+    const double db0 = f[4] - f[3], db1 = f[3] - f[2];          // (i+1, i+2) d = 1/3; (i, i+1) d = 2/3
+    const double b0 = db0 * db0, b1 = db1 * db1;
+    const double tau = fabs(b0 - b1), t2 = tau * tau;
+    const double a0 = zalpha<double>(1.0 / 3.0, t2, b0, eps), a1 = zalpha<double>(2.0 / 3.0, t2, b1, eps);
+    const double inv = 1.0 / (a0 + a1);
+    const double w0 = a0 * inv, w1 = a1 * inv;
+    *sensor = (fabs(3.0 * w0 - 1.0) + fabs(1.5 * w1 - 1.0)) * (1.0 / 3.0);
+    const double e0 = fma(-0.5, f[4], 1.5 * f[3]);
+    const double e1 = 0.5 * f[2] + 0.5 * f[3];
+    return fma(w1, e1, w0 * e0);
 }
 
 // (C + tau/beta)**6 printed as six repeated multiplies (opsc.py:222-241)
@@ -376,7 +408,7 @@ __device__ __forceinline__ void point_prims(const double (&v)[NDIM + 2], const C
 // candidate weights sum to one), and f+ + f- = L F, so the interface flux
 // is F_i + R rec(deviations) with the FP64 F_i added back at the end.
 // ---------------------------------------------------------------------------
-template <int NDIM, int DIR, int SCH, typename T, class PT, bool FILT = false>
+template <int NDIM, int DIR, int SCH, typename T, class PT, int FILT = 0>
 __device__ __forceinline__ void face_core(PT&& pt, const Consts& c, double (&out)[NDIM + 2]) {
     constexpr int NV = NDIM + 2;
     constexpr bool DEV = !std::is_same<T, double>::value;
@@ -480,7 +512,7 @@ __device__ __forceinline__ void face_core(PT&& pt, const Consts& c, double (&out
             cfv[s] = cf;
         }
         T rec;
-        if constexpr (FILT) {
+        if constexpr (FILT == 5) {
             // WENO filter (WENO_filter.py:406-470, shock_capturing.py:554-581): the dissipative part
             // of WENO-Z -- its interface value less the 6th-order central interpolation of the
             // characteristic fluxes -- times the sensor max(right, left); FP64
@@ -488,6 +520,12 @@ __device__ __forceinline__ void face_core(PT&& pt, const Consts& c, double (&out
             const double r0 = wenoz_right<double>(fp, c.weno_eps, &s_r) + wenoz_left<double>(fm, c.weno_eps, &s_l);
             const double central = (((((cfv[0] - 8.0 * cfv[1]) + 37.0 * cfv[2]) + 37.0 * cfv[3]) - 8.0 * cfv[4]) + cfv[5]) *
                                    (1.0 / 60.0);
+            rec = fmax(s_r, s_l) * (r0 - central);
+        } else if constexpr (FILT == 3) {
+            // order 3: WENO3-Z less the 4th-order central interpolation (-1, 7, 7, -1)/12
+            double s_r = 0.0, s_l = 0.0;
+            const double r0 = wenoz3_right(fp, c.weno_eps, &s_r) + wenoz3_left(fm, c.weno_eps, &s_l);
+            const double central = (7.0 * (cfv[2] + cfv[3]) - (cfv[1] + cfv[4])) * (1.0 / 12.0);
             rec = fmax(s_r, s_l) * (r0 - central);
         } else {
             rec = recon_pair<SCH, T>(fp, fm, c);
@@ -506,7 +544,7 @@ __device__ __forceinline__ void face_core(PT&& pt, const Consts& c, double (&out
 
 // face flux at interface x0 + 1/2 along DIR reading the six points from the
 // state (q: FP64 state, `st` the stride along DIR)
-template <int NDIM, int DIR, int SCH, typename T, bool FILT = false>
+template <int NDIM, int DIR, int SCH, typename T, int FILT = 0>
 __device__ __forceinline__ void face_flux(const double* __restrict__ q, long Ptot, long st, long x0,
                                           const Consts& c, double (&out)[NDIM + 2]) {
     constexpr int NV = NDIM + 2;

@@ -369,20 +369,39 @@ int check_launch(osb_ctx* c) {
     return OSB_OK;
 }
 
-// WENO filter after the full step, in place on q (WENO_filter.py:406-536)
-template <int NDIM>
-void filter_nd(osb_ctx* c, double* q) {
+// WENO filter after the full step, in place on q (WENO_filter.py:406-536):
+// the Ducros sensor (HYB), the filter fluxes per direction, the update
+template <int NDIM, int ORDER, bool HYB>
+void filter_nd_t(osb_ctx* c, double* q) {
     Consts fc = c->c;
     fc.avg_roe = 1;                       // the filter's LFWeno uses RoeAverage (WENO_filter.py:438)
     fc.glf = nullptr;
     const Geom& g = c->g;
     Timed tm(c, PF_FILTER);
+    if constexpr (HYB && NDIM >= 2) {
+        double sd = 0.0;
+        for (int d = 0; d < NDIM; ++d) sd += c->d.delta[d] * c->d.delta[d];
+        k_ducros<NDIM><<<nblocks(interior_points(g)), BLOCK, 0, c->stream>>>(q, c->kappa, g, fc, std::sqrt(sd));
+    }
     constexpr int D1 = NDIM > 1 ? 1 : 0, D2 = NDIM > 2 ? 2 : 0;
     const long P = (long)c->nv * g.Ptot;
-    k_filter_face<NDIM, 0><<<nblocks(face_flux_work<NDIM, 0>(g)), BLOCK, 0, c->stream>>>(q, c->F, g, fc);
-    if (NDIM > 1) k_filter_face<NDIM, D1><<<nblocks(face_flux_work<NDIM, D1>(g)), BLOCK, 0, c->stream>>>(q, c->F + P, g, fc);
-    if (NDIM > 2) k_filter_face<NDIM, D2><<<nblocks(face_flux_work<NDIM, D2>(g)), BLOCK, 0, c->stream>>>(q, c->F + 2 * P, g, fc);
-    k_filter_update<NDIM><<<nblocks(interior_points(g)), BLOCK, 0, c->stream>>>(c->F, q, c->scal, g, c->c);
+    k_filter_face<NDIM, 0, ORDER, HYB><<<nblocks(face_flux_work<NDIM, 0>(g)), BLOCK, 0, c->stream>>>(q, c->kappa, c->F, g, fc);
+    if (NDIM > 1)
+        k_filter_face<NDIM, D1, ORDER, HYB><<<nblocks(face_flux_work<NDIM, D1>(g)), BLOCK, 0, c->stream>>>(q, c->kappa, c->F + P, g, fc);
+    if (NDIM > 2)
+        k_filter_face<NDIM, D2, ORDER, HYB><<<nblocks(face_flux_work<NDIM, D2>(g)), BLOCK, 0, c->stream>>>(q, c->kappa, c->F + 2 * P, g, fc);
+    k_filter_update<NDIM, HYB><<<nblocks(interior_points(g)), BLOCK, 0, c->stream>>>(c->F, c->kappa, q, c->scal, g, fc);
+}
+
+template <int NDIM>
+void filter_nd(osb_ctx* c, double* q) {
+    if (c->d.filter_order == 3) {
+        if (c->d.filter_sensor == OSB_SENSOR_DUCROS) filter_nd_t<NDIM, 3, true>(c, q);
+        else filter_nd_t<NDIM, 3, false>(c, q);
+    } else {
+        if (c->d.filter_sensor == OSB_SENSOR_DUCROS) filter_nd_t<NDIM, 5, true>(c, q);
+        else filter_nd_t<NDIM, 5, false>(c, q);
+    }
 }
 
 void apply_filter(osb_ctx* c, double* q) {
@@ -470,6 +489,12 @@ int validate(const osb_problem_desc* d, std::string& why) {
     if (d->shock_filter != OSB_FILTER_NONE && d->shock_filter != OSB_FILTER_WENOZ) { why = "unknown shock filter"; return OSB_E_UNSUPPORTED; }
     if (d->shock_filter == OSB_FILTER_WENOZ && d->precision != OSB_PREC_FP64) { why = "the WENO filter runs in FP64 only"; return OSB_E_UNSUPPORTED; }
     if (d->shock_filter == OSB_FILTER_WENOZ && d->flux_type != OSB_FLUX_LLF) { why = "the WENO filter runs with LLF splitting of the base scheme only"; return OSB_E_UNSUPPORTED; }
+    if (d->shock_filter == OSB_FILTER_WENOZ) {
+        if (d->filter_order != 0 && d->filter_order != 3 && d->filter_order != 5) { why = "WENO filter orders 3 and 5"; return OSB_E_UNSUPPORTED; }
+        if (d->filter_sensor != OSB_SENSOR_NONE && d->filter_sensor != OSB_SENSOR_DUCROS) { why = "unknown filter sensor"; return OSB_E_UNSUPPORTED; }
+        if (d->filter_sensor == OSB_SENSOR_DUCROS && d->ndim < 2) { why = "the Ducros sensor needs ndim >= 2 (shock_sensors.py:42-43)"; return OSB_E_UNSUPPORTED; }
+        if (d->filter_sensor == OSB_SENSOR_DUCROS && (!std::isfinite(d->ducros_check) || !std::isfinite(d->ducros_select))) { why = "non-finite Ducros thresholds"; return OSB_E_INVALID; }
+    }
     if (d->flux_type == OSB_FLUX_GLF && cs != OSB_CONV_WENO_Z && cs != OSB_CONV_TENO) { why = "GLF applies to shock capturing only"; return OSB_E_INVALID; }
     if (d->rk_stages < 0 || d->rk_stages > 5) { why = "rk_stages must be 0..5"; return OSB_E_INVALID; }
     if (d->rk_stages > 0 && d->time_scheme != OSB_TIME_RKLS3) { why = "rk_stages applies to OSB_TIME_RKLS3 only"; return OSB_E_INVALID; }
@@ -532,9 +557,10 @@ constexpr int PART_BLOCKS = 1024;
 // by the unfused central path (ndim < 3); otherwise osb_residual and
 // osb_tgv_diagnostics allocate them on first use.
 struct AllocPlan {
-    size_t q = 0, R = 0, F = 0, cr = 0, gr = 0, staging = 0;
+    size_t q = 0, R = 0, F = 0, kappa = 0, cr = 0, gr = 0, staging = 0;
     size_t bytes() const {
-        return sizeof(double) * (3 * q + R + F + cr + gr + OSB_SCAL_SIZE + staging + (size_t)PART_BLOCKS * 4) + sizeof(unsigned int);
+        return sizeof(double) * (3 * q + R + F + kappa + cr + gr + OSB_SCAL_SIZE + staging + (size_t)PART_BLOCKS * 4) +
+               sizeof(unsigned int);
     }
 };
 
@@ -552,6 +578,7 @@ AllocPlan plan_alloc(const osb_problem_desc* d) {
     a.q = (size_t)nv * P;
     a.R = unfused_central ? a.q : 0;
     a.F = (shock || d->shock_filter == OSB_FILTER_WENOZ) ? (size_t)nd * a.q : 0;
+    a.kappa = (d->shock_filter == OSB_FILTER_WENOZ && d->filter_sensor == OSB_SENSOR_DUCROS) ? P : 0;
     a.cr = unfused_central ? (size_t)(nd + 3) * P : 0;
     a.gr = unfused_central ? (size_t)nd * nd * P : 0;
     a.staging = (size_t)nv * (size_t)g.n[0] * g.n[1] * g.n[2];
@@ -583,7 +610,7 @@ void drop_graphs(osb_ctx* c) {
 void free_ctx(osb_ctx* c) {
     if (!c) return;
     drop_graphs(c);
-    double* bufs[] = {c->qa, c->qb, c->du, c->R, c->F, c->cr, c->gr, c->scal, c->staging, c->part};
+    double* bufs[] = {c->qa, c->qb, c->du, c->R, c->F, c->kappa, c->cr, c->gr, c->scal, c->staging, c->part};
     for (double* b : bufs) if (b) (void)hipFree(b);
     if (c->nan_cnt) (void)hipFree(c->nan_cnt);
     for (auto& r : c->prof_pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
@@ -661,6 +688,8 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     k.avg_roe = desc->averaging == OSB_AVG_ROE;
     k.viscous = desc->viscous;
     k.glf = nullptr;
+    k.ducros_check = desc->ducros_check;
+    k.ducros_select = desc->ducros_select;
     for (int i = 0; i < 3; ++i) {
         k.carp[i] = 0;
         for (int sd = 0; sd < 2; ++sd)
@@ -674,6 +703,7 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     if (!r) r = dalloc(c, &c->du, ap.q);
     if (!r && ap.R) r = dalloc(c, &c->R, ap.R);
     if (!r && ap.F) r = dalloc(c, &c->F, ap.F);
+    if (!r && ap.kappa) r = dalloc(c, &c->kappa, ap.kappa);      // zeroed: the halos stay 0
     if (!r && ap.cr) r = dalloc(c, &c->cr, ap.cr);
     if (!r && ap.gr) r = dalloc(c, &c->gr, ap.gr);
     if (!r) r = dalloc(c, &c->scal, OSB_SCAL_SIZE);

@@ -52,9 +52,13 @@ __global__ void __launch_bounds__(BLOCK) k_face_flux(const double* __restrict__ 
 }
 
 // WENO filter fluxes along DIR (WENO_filter.py:406-470): faces [-1, n) as the
-// reconstruction kernels of the scheme; c carries RoeAverage and LLF.
-template <int NDIM, int DIR>
-__global__ void __launch_bounds__(BLOCK) k_filter_face(const double* __restrict__ q, double* __restrict__ F, Geom g, Consts c) {
+// reconstruction kernels of the scheme; c carries RoeAverage and LLF.  ORDER 5
+// or 3; HYB: the Ducros hybrid selection (WENO_filter.py:561-584) -- a face is
+// reconstructed only where max kappa over offsets -3..2 along DIR exceeds
+// ducros_check, else its fluxes are zero.
+template <int NDIM, int DIR, int ORDER, bool HYB>
+__global__ void __launch_bounds__(BLOCK) k_filter_face(const double* __restrict__ q, const double* __restrict__ kappa,
+                                                       double* __restrict__ F, Geom g, Consts c) {
     constexpr int NV = NDIM + 2;
     int e[3] = {g.n[0], g.n[1], g.n[2]};
     e[DIR] += 1;
@@ -66,15 +70,69 @@ __global__ void __launch_bounds__(BLOCK) k_filter_face(const double* __restrict_
     unflatten(t, e, lo, i, j, k);
     const long x0 = gidx(g, i, j, k);
     double out[NV];
-    face_flux<NDIM, DIR, SCH_WENOZ, double, true>(q, g.Ptot, g.s[DIR], x0, c, out);
+    bool active = true;
+    if (HYB) {
+        const long st = g.s[DIR];
+        double km = kappa[x0];
+#pragma unroll
+        for (int o = -3; o <= 2; ++o) km = fmax(km, kappa[x0 + o * st]);
+        active = km > c.ducros_check;
+    }
+    if (active) {
+        face_flux<NDIM, DIR, SCH_WENOZ, double, ORDER>(q, g.Ptot, g.s[DIR], x0, c, out);
+    } else {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) out[v] = 0.0;
+    }
 #pragma unroll
     for (int v = 0; v < NV; ++v) F[(long)v * g.Ptot + x0] = out[v];
 }
 
-// filter application (WENO_filter.py:479-536): q += dt * (-sum_d (F_d(i) - F_d(i-1)) / Delta_d), in place
+// Modified Ducros sensor (shock_sensors.py:16-59; generated
+// Central4-WENO5/opensbliblock00_kernels.h:572-617) over [0, n):
+// kappa = div^2 (1/2 - 1/2 tanh(2.5 + 500 div / sqrt(sum Delta^2))) / (div^2 + |curl u|^2 + 1e-40)
+// with central derivatives of u = rhou/rho; the halos of kappa stay zero.
 template <int NDIM>
-__global__ void __launch_bounds__(BLOCK) k_filter_update(const double* __restrict__ F, double* __restrict__ q,
-                                                         const double* __restrict__ scal, Geom g, Consts c) {
+__global__ void __launch_bounds__(BLOCK) k_ducros(const double* __restrict__ q, double* __restrict__ kappa, Geom g, Consts c,
+                                                  double sd) {
+    const int e[3] = {g.n[0], g.n[1], g.n[2]};
+    const int lo[3] = {0, 0, 0};
+    const long t = gtid();
+    if (t >= (long)e[0] * e[1] * e[2]) return;
+    int i, j, k;
+    unflatten(t, e, lo, i, j, k);
+    const int ii[3] = {i, j, k};
+    const long x = gidx(g, i, j, k);
+    const long P = g.Ptot;
+    double G[NDIM][NDIM];
+#pragma unroll
+    for (int m = 0; m < NDIM; ++m)
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) {
+            const long st = g.s[d];
+            G[m][d] = d1c([&](int s) { const long y = x + s * st; return q[(long)(1 + m) * P + y] / q[y]; }, c.invd[d], ii[d],
+                          g.n[d], c.carp[d]);
+        }
+    double div = 0.0;
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) div += G[d][d];
+    double w2;
+    if constexpr (NDIM == 2) {
+        const double wz = G[1][0] - G[0][1];
+        w2 = wz * wz;
+    } else {
+        const double wx = G[2][1] - G[1][2], wy = G[0][2] - G[2][0], wz = G[1][0] - G[0][1];
+        w2 = (wx * wx + wy * wy) + wz * wz;
+    }
+    const double th = 0.5 - 0.5 * tanh(2.5 + (500.0 * div) / sd);
+    kappa[x] = ((div * div) * th) / ((div * div + w2) + 1.0e-40);
+}
+
+// filter application (WENO_filter.py:479-536): q += dt * (-sum_d (F_d(i) - F_d(i-1)) / Delta_d), in
+// place; HYB: only where max kappa over the point and its -1, +1, +2 neighbours reaches ducros_select
+template <int NDIM, bool HYB>
+__global__ void __launch_bounds__(BLOCK) k_filter_update(const double* __restrict__ F, const double* __restrict__ kappa,
+                                                         double* __restrict__ q, const double* __restrict__ scal, Geom g, Consts c) {
     constexpr int NV = NDIM + 2;
     const int e[3] = {g.n[0], g.n[1], g.n[2]};
     const int lo[3] = {0, 0, 0};
@@ -83,6 +141,15 @@ __global__ void __launch_bounds__(BLOCK) k_filter_update(const double* __restric
     int i, j, k;
     unflatten(t, e, lo, i, j, k);
     const long x = gidx(g, i, j, k);
+    if (HYB) {
+        double km = kappa[x];
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) {
+            const long st = g.s[d];
+            km = fmax(km, fmax(kappa[x - st], fmax(kappa[x + st], kappa[x + 2 * st])));
+        }
+        if (!(km >= c.ducros_select)) return;
+    }
     const long nvP = (long)NV * g.Ptot;
     const double dt = scal[1];
 #pragma unroll

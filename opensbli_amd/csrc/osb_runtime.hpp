@@ -35,6 +35,7 @@ struct osb_ctx {
     double* du = nullptr;
     double* R = nullptr;
     double* F = nullptr;
+    double* kappa = nullptr;        // WENO filter's Ducros sensor over [0,n), zero halos
     double* cr = nullptr;
     double* gr = nullptr;
     double* scal = nullptr;         // OSB_SCAL_SIZE: [0] max wavespeed bits, [1] dt, [2] device time,

@@ -19,7 +19,7 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 OK = 0
 E_INVALID = -1
@@ -58,6 +58,9 @@ FLUX_GLF = 1
 FILTER_NONE = 0
 FILTER_WENOZ = 1
 
+SENSOR_NONE = 0
+SENSOR_DUCROS = 1
+
 PREC_FP64 = 1
 PREC_MIXED = 2
 
@@ -95,7 +98,11 @@ class ProblemDescC(C.Structure):
         ("rk_b", C.c_double * 5),
         ("bc_value", ((C.c_double * 5) * 2) * 3),
         ("shock_filter", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("filter_order", C.c_int32),
+        ("filter_sensor", C.c_int32),
+        ("reserved0", C.c_int32),
+        ("ducros_check", C.c_double),
+        ("ducros_select", C.c_double),
     ]
 
 
@@ -140,6 +147,11 @@ class ProblemDesc:
     rk_b: Sequence[float] = ()
     flux_type: int = FLUX_LLF
     shock_filter: int = FILTER_NONE
+    # WENO filter: order 5 or 3; SENSOR_DUCROS = WENOFilter(optimize=True)
+    filter_order: int = 5
+    filter_sensor: int = SENSOR_NONE
+    ducros_check: float = 0.05
+    ducros_select: float = 0.05
     # BC_DIRICHLET faces: {(direction, side): conservative values (rho, rhou.., rhoE)}
     bc_value: dict = field(default_factory=dict)
 
@@ -196,6 +208,10 @@ class ProblemDesc:
             d.rk_b[s] = float(b)
         d.flux_type = int(self.flux_type)
         d.shock_filter = int(self.shock_filter)
+        d.filter_order = int(self.filter_order)
+        d.filter_sensor = int(self.filter_sensor)
+        d.ducros_check = float(self.ducros_check)
+        d.ducros_select = float(self.ducros_select)
         for (i, s), vals in self.bc_value.items():
             vals = [float(v) for v in vals]
             if len(vals) != self.nv:

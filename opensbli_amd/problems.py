@@ -162,6 +162,24 @@ def tgv(n: int = 128, steps: int = 200, dt: Optional[float] = None, minf: float 
     return Problem(f"tgv_{scheme}_{n}", desc, q, steps)
 
 
+def tgv_central4_weno5(n: int = 128, steps: int = 500, dt: Optional[float] = None) -> Problem:
+    """The reference's other performance-benchmark method
+    (apps/performance_benchmark/Central4-WENO5/opensbli.cpp:36-110,
+    apps/performance_benchmark/README.MD:27-29): KGP cubic split (enthalpy
+    form), Sutherland viscous terms, RK-LS3, and after each step the WENO3-Z
+    filter with RoeAverage, LLF and the Ducros hybrid selection
+    (Ducros_check = Ducros_select = 0.05).  dt = 2 * 0.00084625 at the
+    validation size 128^3 ("va"), 0.00084625 * 256 / n otherwise ("ss")."""
+    if dt is None:
+        dt = 0.00084625 * 2.0 if n == 128 else 0.00084625 * 256.0 / n
+    pr = tgv(n=n, steps=steps, dt=dt, minf=1.25, re=1600.0, pr=0.71, scheme="kgp")
+    d = pr.desc
+    d.viscous = nat.VISC_SUTHERLAND
+    d.shock_filter, d.filter_order, d.filter_sensor = nat.FILTER_WENOZ, 3, nat.SENSOR_DUCROS
+    d.ducros_check = d.ducros_select = 0.05
+    return Problem(f"tgv_central4_weno5_{n}", d, pr.q0, steps)
+
+
 def tgv_teno6_validation(n: int = 128, steps: int = 500) -> Problem:
     """The reference's performance-benchmark validation run ("va"):
     apps/performance_benchmark/TENO6/opensbli.cpp:36-48, 94-140 -- TENO6,

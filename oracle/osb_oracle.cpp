@@ -54,6 +54,7 @@ struct osb_ctx {
     std::vector<double> q, rkold, R;
     std::vector<double> u, p, a, T, mu;   // constituent relations
     std::vector<double> wk;               // face fluxes, nv per direction
+    std::vector<double> kappa;            // WENO filter's Ducros sensor, [0,N); 0 in the halos
     std::vector<double> g;                // stored velocity gradients g[i*ndim+j] = d u_i / d x_j
     double time = 0.0;
     int64_t iter = 0;
@@ -354,6 +355,39 @@ double wenoz_left(const double* f, double eps, double* sensor = nullptr) {
     return std::fma(a2 * inv, e2, std::fma(a1 * inv, e1, (a0 * inv) * e0));
 }
 
+// WENO3-Z, used by the WENO filter of order 3 (weno.py:36-136, 290-345 with
+// k = 2: tau = |beta_0 - beta_1|, p = 2; generated form in
+// apps/performance_benchmark/Central4-WENO5/opensbliblock00_kernels.h:930-990).
+// f[s] is the split value at offset s-2 from point i; beta_r is the squared
+// difference over stencil r (the generated 1/4 scaling is the split's 1/2
+// squared).  The sensor is sum_r |omega_r/d_r - 1| / 3 (weno.py:443-459).
+double wenoz3_right(const double* f, double eps, double* sensor) {
+    // r=0 -> (i, i+1), d = 2/3; r=1 -> (i-1, i), d = 1/3
+    const double db0 = f[3] - f[2], db1 = f[2] - f[1];
+    const double b0 = db0 * db0, b1 = db1 * db1;
+    const double tau = std::fabs(b0 - b1), t2 = tau * tau;
+    const double a0 = zalpha(2.0 / 3.0, t2, b0, eps), a1 = zalpha(1.0 / 3.0, t2, b1, eps);
+    const double inv = 1.0 / (a0 + a1);
+    const double w0 = a0 * inv, w1 = a1 * inv;
+    *sensor = (std::fabs(3.0 * w1 - 1.0) + std::fabs(1.5 * w0 - 1.0)) * (1.0 / 3.0);
+    const double e0 = 0.5 * f[2] + 0.5 * f[3];
+    const double e1 = std::fma(-0.5, f[1], 1.5 * f[2]);
+    return std::fma(w1, e1, w0 * e0);
+}
+double wenoz3_left(const double* f, double eps, double* sensor) {
+    // r=0 -> (i+1, i+2), d = 1/3; r=1 -> (i, i+1), d = 2/3
+    const double db0 = f[4] - f[3], db1 = f[3] - f[2];
+    const double b0 = db0 * db0, b1 = db1 * db1;
+    const double tau = std::fabs(b0 - b1), t2 = tau * tau;
+    const double a0 = zalpha(1.0 / 3.0, t2, b0, eps), a1 = zalpha(2.0 / 3.0, t2, b1, eps);
+    const double inv = 1.0 / (a0 + a1);
+    const double w0 = a0 * inv, w1 = a1 * inv;
+    *sensor = (std::fabs(3.0 * w0 - 1.0) + std::fabs(1.5 * w1 - 1.0)) * (1.0 / 3.0);
+    const double e0 = std::fma(-0.5, f[4], 1.5 * f[3]);
+    const double e1 = 0.5 * f[2] + 0.5 * f[3];
+    return std::fma(w1, e1, w0 * e0);
+}
+
 // TENO5 / TENO6 (teno.py:59-217, 419-477).
 struct TenoParams { int order; double eps, ct; };
 // (C + tau/beta)**6 printed as six repeated multiplies (opsc.py:222-241)
@@ -435,8 +469,19 @@ void reconstruct_direction(osb_ctx* c, int dir, bool filter = false) {
     const bool teno = c->d.conv_scheme == OSB_CONV_TENO;
     TenoParams tp{c->d.order, c->d.teno_eps, c->d.teno_ct};
     const double weps = c->d.weno_eps;
+    const bool hybrid = filter && c->d.filter_sensor == OSB_SENSOR_DUCROS;
     for_range(r, [&](int i, int j, int k) {
         const size_t x0 = c->I(i, j, k);
+        if (hybrid) {
+            // WENO_filter.py:561-584: reconstruct only where max kappa over offsets
+            // -3..2 along dir exceeds Ducros_check, else zero fluxes
+            double km = c->kappa[x0];
+            for (int o = -3; o <= 2; ++o) km = std::max(km, c->kappa[x0 + (size_t)((long)o * (long)st)]);
+            if (!(km > c->d.ducros_check)) {
+                for (int v = 0; v < nv; ++v) wk[(size_t)v * c->Ptot + x0] = 0.0;
+                return;
+            }
+        }
         // point data at offsets -2..3
         double qv[6][5], F[6][5], ud[6], aa[6];
         for (int s = 0; s < 6; ++s) {
@@ -510,10 +555,15 @@ void reconstruct_direction(osb_ctx* c, int dir, bool filter = false) {
                 // WENO filter (WENO_filter.py:406-470, shock_capturing.py:554-581): the dissipative
                 // part of WENO-Z -- its interface value less the 6th-order central interpolation
                 // of the characteristic fluxes -- times the sensor max(right, left)
-                double s_r = 0.0, s_l = 0.0;
-                const double rec = wenoz_right(fp, weps, &s_r) + wenoz_left(fm, weps, &s_l);
-                const double central = (((((cfv[0] - 8.0 * cfv[1]) + 37.0 * cfv[2]) + 37.0 * cfv[3]) - 8.0 * cfv[4]) + cfv[5]) *
-                                       (1.0 / 60.0);
+                double s_r = 0.0, s_l = 0.0, rec, central;
+                if (c->d.filter_order == 3) {
+                    // WENO3-Z less the 4th-order central interpolation (-1, 7, 7, -1)/12
+                    rec = wenoz3_right(fp, weps, &s_r) + wenoz3_left(fm, weps, &s_l);
+                    central = (7.0 * (cfv[2] + cfv[3]) - (cfv[1] + cfv[4])) * (1.0 / 12.0);
+                } else {
+                    rec = wenoz_right(fp, weps, &s_r) + wenoz_left(fm, weps, &s_l);
+                    central = (((((cfv[0] - 8.0 * cfv[1]) + 37.0 * cfv[2]) + 37.0 * cfv[3]) - 8.0 * cfv[4]) + cfv[5]) * (1.0 / 60.0);
+                }
                 recon[kk] = std::max(s_r, s_l) * (rec - central);
             } else if (teno) recon[kk] = teno_side(fp, +1, tp) + teno_side(fm, -1, tp);
             else recon[kk] = wenoz_right(fp, weps) + wenoz_left(fm, weps);
@@ -889,15 +939,63 @@ void apply_bcs(osb_ctx* c) {
 // of the current state, the filtered interface fluxes per direction, and
 // q += dt * R_filter with R_filter = -sum_d (F_d(i) - F_d(i-1)) / Delta_d.
 // ---------------------------------------------------------------------------
+// Modified Ducros sensor (shock_sensors.py:16-59; generated
+// Central4-WENO5/opensbliblock00_kernels.h:572-617) over [0, N):
+// kappa = div^2 (1/2 - 1/2 tanh(2.5 + 500 div / sqrt(sum_d Delta_d^2))) /
+//         (div^2 + |curl u|^2 + 1e-40), central derivatives of u.
+void ducros_sensor(osb_ctx* c) {
+    const int nd = c->ndim;
+    const size_t Pt = c->Ptot;
+    double sd = 0.0;
+    for (int d = 0; d < nd; ++d) sd += c->d.delta[d] * c->d.delta[d];
+    sd = std::sqrt(sd);
+    Range r = make_range(c, 0);
+    for_range(r, [&](int i, int j, int k) {
+        const size_t x = c->I(i, j, k);
+        const int ii[3] = {i, j, k};
+        double G[3][3] = {{0}};              // G[m][d] = D_d u_m
+        for (int m = 0; m < nd; ++m)
+            for (int d = 0; d < nd; ++d) {
+                const double* um = c->u.data() + (size_t)m * Pt;
+                const size_t st = c->stride(d);
+                G[m][d] = d1c([&](int s) { return um[x + (long)s * (long)st]; }, 1.0 / c->d.delta[d], ii[d], c->n[d], c->carp[d]);
+            }
+        double div = 0.0;
+        for (int d = 0; d < nd; ++d) div += G[d][d];
+        double w2;
+        if (nd == 2) {
+            const double wz = G[1][0] - G[0][1];
+            w2 = wz * wz;
+        } else {
+            const double wx = G[2][1] - G[1][2], wy = G[0][2] - G[2][0], wz = G[1][0] - G[0][1];
+            w2 = (wx * wx + wy * wy) + wz * wz;
+        }
+        const double th = 0.5 - 0.5 * std::tanh(2.5 + (500.0 * div) / sd);
+        c->kappa[x] = ((div * div) * th) / ((div * div + w2) + 1.0e-40);
+    });
+}
+
 void apply_weno_filter(osb_ctx* c, double dt) {
     Range rcr = make_range(c, 3);
     for (int d = 0; d < c->ndim; ++d) rcr.hi[d] = c->n[d] + 4;
     constituent_relations(c, rcr, false);
+    const bool hybrid = c->d.filter_sensor == OSB_SENSOR_DUCROS;
+    if (hybrid) ducros_sensor(c);
     for (int dir = 0; dir < c->ndim; ++dir) reconstruct_direction(c, dir, true);
     const int nd = c->ndim, nv = c->nv;
     Range r = make_range(c, 0);
     for_range(r, [&](int i, int j, int k) {
         const size_t x = c->I(i, j, k);
+        if (hybrid) {
+            // WENO_filter.py:486-500: filter where max kappa over the point and its
+            // -1, +1, +2 neighbours in every direction reaches Ducros_select
+            double km = c->kappa[x];
+            for (int d = 0; d < nd; ++d) {
+                const long st = (long)c->stride(d);
+                km = std::max(km, std::max(c->kappa[x - st], std::max(c->kappa[x + st], c->kappa[x + 2 * st])));
+            }
+            if (!(km >= c->d.ducros_select)) return;
+        }
         for (int v = 0; v < nv; ++v) {
             double acc = 0.0;
             for (int dir = 0; dir < nd; ++dir) {
@@ -1019,6 +1117,12 @@ int validate(const osb_problem_desc* d, std::string& why) {
     if (d->shock_filter != OSB_FILTER_NONE && d->shock_filter != OSB_FILTER_WENOZ) { why = "unknown shock filter"; return OSB_E_UNSUPPORTED; }
     if (d->shock_filter == OSB_FILTER_WENOZ && d->precision != OSB_PREC_FP64) { why = "the WENO filter runs in FP64 only"; return OSB_E_UNSUPPORTED; }
     if (d->shock_filter == OSB_FILTER_WENOZ && d->flux_type != OSB_FLUX_LLF) { why = "the WENO filter runs with LLF splitting of the base scheme only"; return OSB_E_UNSUPPORTED; }
+    if (d->shock_filter == OSB_FILTER_WENOZ) {
+        if (d->filter_order != 0 && d->filter_order != 3 && d->filter_order != 5) { why = "WENO filter orders 3 and 5"; return OSB_E_UNSUPPORTED; }
+        if (d->filter_sensor != OSB_SENSOR_NONE && d->filter_sensor != OSB_SENSOR_DUCROS) { why = "unknown filter sensor"; return OSB_E_UNSUPPORTED; }
+        if (d->filter_sensor == OSB_SENSOR_DUCROS && d->ndim < 2) { why = "the Ducros sensor needs ndim >= 2 (shock_sensors.py:42-43)"; return OSB_E_UNSUPPORTED; }
+        if (d->filter_sensor == OSB_SENSOR_DUCROS && (!std::isfinite(d->ducros_check) || !std::isfinite(d->ducros_select))) { why = "non-finite Ducros thresholds"; return OSB_E_INVALID; }
+    }
     if (d->flux_type == OSB_FLUX_GLF && cs != OSB_CONV_WENO_Z && cs != OSB_CONV_TENO) { why = "GLF applies to shock capturing only"; return OSB_E_INVALID; }
     if (d->rk_stages < 0 || d->rk_stages > 5) { why = "rk_stages must be 0..5"; return OSB_E_INVALID; }
     if (d->rk_stages > 0 && d->time_scheme != OSB_TIME_RKLS3) { why = "rk_stages applies to OSB_TIME_RKLS3 only"; return OSB_E_INVALID; }
@@ -1110,6 +1214,7 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     c->a.assign(c->Ptot, 0.0);
     if (c->viscous || c->central) { c->T.assign(c->Ptot, 0.0); c->mu.assign(c->Ptot, 0.0); }
     if (c->shock || desc->shock_filter == OSB_FILTER_WENOZ) c->wk.assign((size_t)c->ndim * nvP, 0.0);
+    if (desc->shock_filter == OSB_FILTER_WENOZ && desc->filter_sensor == OSB_SENSOR_DUCROS) c->kappa.assign(c->Ptot, 0.0);
     if (c->viscous || c->central) c->g.assign((size_t)c->ndim * c->ndim * c->Ptot, 0.0);
     *out = c;
     return OSB_OK;
@@ -1387,6 +1492,20 @@ int osb_oracle_glf(const osb_ctx* c, double* out) {
     if (!c || !out) return OSB_E_INVALID;
     for (int d = 0; d < 3; ++d)
         for (int r = 0; r < 3; ++r) out[3 * d + r] = c->glf[d][r];
+    return OSB_OK;
+}
+
+// test hook: the Ducros sensor of the current state over the interior (x fastest)
+int osb_oracle_kappa(osb_ctx* c, double* out) {
+    if (!c || !out || c->ndim < 2) return OSB_E_INVALID;
+    if (c->kappa.empty()) c->kappa.assign(c->Ptot, 0.0);
+    Range rcr = make_range(c, 3);
+    constituent_relations(c, rcr, false);
+    ducros_sensor(c);
+    size_t o = 0;
+    for (int k = 0; k < c->n[2]; ++k)
+        for (int j = 0; j < c->n[1]; ++j)
+            for (int i = 0; i < c->n[0]; ++i) out[o++] = c->kappa[c->I(i, j, k)];
     return OSB_OK;
 }
 

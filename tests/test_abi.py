@@ -89,7 +89,7 @@ def test_descriptor_struct_layout():
 #define S(f) printf("%s %zu\n", #f, offsetof(osb_stats, f));
 int main(void) {
   printf("size %zu\n", sizeof(osb_problem_desc));
-  O(ndim) O(n) O(delta) O(conv_scheme) O(bc) O(gama) O(weno_eps) O(teno_ct) O(dt) O(cfl) O(rk_stages) O(flux_type) O(rk_a) O(rk_b) O(bc_value) O(shock_filter) O(reserved)
+  O(ndim) O(n) O(delta) O(conv_scheme) O(bc) O(gama) O(weno_eps) O(teno_ct) O(dt) O(cfl) O(rk_stages) O(flux_type) O(rk_a) O(rk_b) O(bc_value) O(shock_filter) O(filter_order) O(filter_sensor) O(ducros_check) O(ducros_select)
   printf("ssize %zu\n", sizeof(osb_stats));
   S(iter) S(time) S(last_dt) S(halo) S(nv) S(device_bytes)
   return 0;
@@ -105,7 +105,8 @@ int main(void) {
     assert c_off["size"] == C.sizeof(nat.ProblemDescC)
     assert c_off["ssize"] == C.sizeof(nat.StatsC)
     for f in ("ndim", "n", "delta", "conv_scheme", "bc", "gama", "weno_eps", "teno_ct", "dt", "cfl",
-              "rk_stages", "flux_type", "rk_a", "rk_b", "bc_value", "shock_filter", "reserved"):
+              "rk_stages", "flux_type", "rk_a", "rk_b", "bc_value", "shock_filter", "filter_order", "filter_sensor",
+              "ducros_check", "ducros_select"):
         assert getattr(nat.ProblemDescC, f).offset == c_off[f], f
     for f in ("iter", "time", "last_dt", "halo", "nv", "device_bytes"):
         assert getattr(nat.StatsC, f).offset == c_off[f], f

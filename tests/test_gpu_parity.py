@@ -377,3 +377,29 @@ def test_residual_closure_tgv(hip, oracle):
 def test_closure_parity(hip, oracle, maker):
     a, b = _compare(hip, oracle, maker())
     assert np.array_equal(a, b)
+
+
+# --- WENO3-Z filter with the Ducros hybrid selection (Central4-WENO5) -----------------
+def _kgp2d_smooth(n=40, steps=12):
+    gama = 1.4
+    h = 2 * math.pi / n
+    X, Y = np.meshgrid(h * np.arange(n), h * np.arange(n), indexing="xy")
+    rho = 1.0 + 0.2 * np.sin(X) * np.cos(2 * Y)
+    u, v = 0.6 * np.sin(X + 0.5) * np.cos(Y), -0.4 * np.cos(X) * np.sin(Y - 0.3)
+    p = 1.0 + 0.1 * np.cos(X - Y)
+    q = np.stack([rho, rho * u, rho * v, p / (gama - 1.0) + 0.5 * rho * (u * u + v * v)])[:, None]
+    desc = nat.ProblemDesc(ndim=2, n=[n, n], delta=[h, h], conv_scheme=nat.CONV_CENTRAL_KGP, order=4,
+                           bc=[[nat.BC_PERIODIC] * 2] * 2, gama=gama, dt=5e-3, shock_filter=nat.FILTER_WENOZ,
+                           filter_order=3, filter_sensor=nat.SENSOR_DUCROS)
+    return P.Problem("kgp2d_weno3_ducros", desc, q, steps)
+
+
+@pytest.mark.parametrize("maker", [
+    lambda: P.tgv_central4_weno5(n=24, steps=10),                                      # the benchmark's method, 3-D
+    lambda: _kgp2d_smooth(),                                                            # 2-D hybrid
+    lambda: P.Problem("entropy_o3", nat.ProblemDesc(**{**P.entropy_wave(n=48, steps=10).desc.__dict__, "filter_order": 3}),
+                      P.entropy_wave(n=48, steps=10).q0, 10),                           # order 3, every point
+])
+def test_weno3_ducros_filter_parity(hip, oracle, maker):
+    a, b = _compare(hip, oracle, maker())
+    assert np.array_equal(a, b)

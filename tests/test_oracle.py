@@ -786,3 +786,102 @@ def test_closure_converges_on_a_non_periodic_wave(oracle):
         dpdx = 0.01 * 2 * np.pi * np.cos(2 * np.pi * x + 0.3)
         res.append(np.abs(R[1, 0, 0, :4] + dpdx[:4]).max())      # -dp/dx at the closure points
     assert res[0] / res[1] > 7 and res[1] / res[2] > 7, res
+
+
+# --- WENO3-Z filter with the Ducros hybrid selection (Central4-WENO5 benchmark) -----
+def _order3(pr, sensor=nat.SENSOR_NONE, check=0.05, select=0.05):
+    d = nat.ProblemDesc(**{**pr.desc.__dict__, "filter_order": 3, "filter_sensor": sensor, "ducros_check": check,
+                           "ducros_select": select})
+    return P.Problem(pr.name + "_o3", d, pr.q0, pr.steps)
+
+
+def test_weno3_filter_properties(oracle):
+    """Order 3: conservative on a periodic grid, a stationary contact kept, the
+    ringing central top hat held to TV <= 1.1x the initial one, and the change on
+    a smooth wave falling 2nd order in the max norm (WENO3-Z's accuracy at the
+    wave's critical points; a sign error in the central part gives no decay)."""
+    a, _ = run(oracle, P.entropy_wave(n=48, profile="step", steps=6, filtered=False))
+    b, _ = run(oracle, _order3(P.entropy_wave(n=48, profile="step", steps=6, filtered=True)))
+    assert np.abs(a - b).max() > 1e-9
+    for v in range(a.shape[0]):
+        assert abs(b[v].sum() - a[v].sum()) <= 1e-12 * np.abs(a[v]).sum(), v
+    st = _order3(P.entropy_wave(n=40, profile="step", steps=5, filtered=True, velocity=0.0))
+    q, _ = run(oracle, st)
+    assert np.abs(q - st.q0).max() <= 1e-14
+    pr = _order3(P.entropy_wave(n=64, profile="step", steps=40, filtered=True))
+    q, _ = run(oracle, pr)
+    assert _tv_x(q) < 1.1 * _tv_x(pr.q0) and q[0].min() > 0.49 and q[0].max() < 1.01
+    inc = []
+    for n in (32, 64, 128):
+        a, _ = run(oracle, P.entropy_wave(n=n, profile="sine", steps=1, filtered=False))
+        b, _ = run(oracle, _order3(P.entropy_wave(n=n, profile="sine", steps=1, filtered=True)))
+        inc.append(np.abs(a - b).max())
+    assert inc[0] / inc[1] > 3.5 and inc[1] / inc[2] > 3.5, inc
+
+
+def _kappa(oracle, desc, q):
+    L = oracle.lib
+    L.osb_oracle_kappa.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    out = np.zeros(desc.npoints)
+    with oracle.context(desc) as c:
+        c.set_state(q)
+        assert L.osb_oracle_kappa(c.h, out.ctypes.data_as(C.POINTER(C.c_double))) == 0
+    return out.reshape(desc.shape[1:])
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_ducros_sensor_closed_forms(oracle, ndim):
+    """kappa = div^2 (1/2 - 1/2 tanh(2.5 + 500 div/sqrt(sum Delta^2))) / (div^2 + |curl u|^2 + 1e-40)
+    (shock_sensors.py:16-59) on linear velocity fields, whose central and closure
+    derivatives are exact (order-1 extrapolation faces keep the halos linear):
+    a rotation gives 0 (to rounding), a strong compression 1, a strong expansion 0, and a
+    compressing vortex the closed form with the tanh argument near 0."""
+    n, h = 12, 0.1
+    sd = math.sqrt(ndim * h * h)
+    ax = [h * np.arange(n)] * ndim
+    grids = np.meshgrid(*ax[::-1], indexing="ij")[::-1]          # grids[d] varies along direction d
+    desc = nat.ProblemDesc(ndim=ndim, n=[n] * ndim, delta=[h] * ndim, conv_scheme=nat.CONV_CENTRAL_FEIEREISEN, order=4,
+                           bc=[[nat.BC_EXTRAPOLATION1] * 2] * ndim, gama=1.4, dt=1e-3, shock_filter=nat.FILTER_WENOZ,
+                           filter_order=3, filter_sensor=nat.SENSOR_DUCROS)
+
+    def state(u):
+        q = np.zeros(desc.shape)
+        q[0] = 1.0
+        for d in range(ndim):
+            q[1 + d] = u[d]
+        q[-1] = 1.0 / 0.4 + 0.5 * sum(ud * ud for ud in u)
+        return q
+
+    x, y = grids[0], grids[1]
+    z = grids[2] if ndim == 3 else 0.0 * x
+    zero = 0.0 * x
+    k = _kappa(oracle, desc, state([-y, x] + ([zero] if ndim == 3 else [])))
+    assert np.abs(k).max() <= 1e-30           # div at rounding level: the closure rows sum to ~1e-17
+    k = _kappa(oracle, desc, state([-1.0 * x, -1.0 * y] + ([-1.0 * z] if ndim == 3 else [])))
+    np.testing.assert_allclose(k, 1.0, rtol=0, atol=1e-14)
+    k = _kappa(oracle, desc, state([1.0 * x, 1.0 * y] + ([1.0 * z] if ndim == 3 else [])))
+    assert np.abs(k).max() <= 1e-14
+    a, w, b = -0.4 * sd / 500.0, 0.3 * sd / 500.0, -0.2 * sd / 500.0
+    u = [a * x - w * y, w * x + a * y] + ([b * z] if ndim == 3 else [])
+    div = 2 * a + (b if ndim == 3 else 0.0)
+    want = div * div * (0.5 - 0.5 * math.tanh(2.5 + 500.0 * div / sd)) / (div * div + 4 * w * w + 1e-40)
+    np.testing.assert_allclose(_kappa(oracle, desc, state(u)), want, rtol=1e-9)
+
+
+def test_ducros_hybrid_thresholds(oracle):
+    """With thresholds below every kappa the hybrid filter is the plain order-3
+    filter, bit for bit; above every kappa (kappa <= 1) it does nothing.  (The
+    point selection multiplies each point's whole update by 0 or 1, as
+    WENO_filter.py:486-536 does, so the hybrid filter is not conservative.)"""
+    base = P.tgv_central4_weno5(n=16, steps=4)
+    plain = P.Problem("p", nat.ProblemDesc(**{**base.desc.__dict__, "filter_sensor": nat.SENSOR_NONE}), base.q0, 4)
+    none = P.Problem("n", nat.ProblemDesc(**{**base.desc.__dict__, "shock_filter": nat.FILTER_NONE}), base.q0, 4)
+    on = P.Problem("on", nat.ProblemDesc(**{**base.desc.__dict__, "ducros_check": -1.0, "ducros_select": -1.0}), base.q0, 4)
+    off = P.Problem("off", nat.ProblemDesc(**{**base.desc.__dict__, "ducros_check": 2.0, "ducros_select": 2.0}), base.q0, 4)
+    qp, _ = run(oracle, plain)
+    qn, _ = run(oracle, none)
+    qon, _ = run(oracle, on)
+    qoff, _ = run(oracle, off)
+    qh, _ = run(oracle, base)
+    assert np.array_equal(qon, qp) and np.array_equal(qoff, qn)
+    assert not np.array_equal(qh, qn) and not np.array_equal(qh, qp)      # the 0.05 selection is partial

@@ -404,9 +404,8 @@ def test_weno_filter_plan_and_run(oracle):
     assert np.array_equal(q, want)
 
 
-@pytest.mark.parametrize("kw,match", [(dict(order=3), "order 5"), (dict(formulation="JS"), "'Z'"),
-                                      (dict(flux_type="HLLC"), "LLF"), (dict(optimize=True), "optimize"),
-                                      (dict(metrics=object()), "uniform")])
+@pytest.mark.parametrize("kw,match", [(dict(order=7), "orders 3 and 5"), (dict(formulation="JS"), "'Z'"),
+                                      (dict(flux_type="HLLC"), "LLF"), (dict(metrics=object()), "uniform")])
 def test_weno_filter_unsupported_options_raise(kw, match):
     from opensbli_amd.surface import WENOFilter
     block = SimulationBlock(2, block_number=0)
@@ -419,3 +418,26 @@ def test_weno_filter_unsupported_options_raise(kw, match):
               "Delta1block0": "0.1"}
     with pytest.raises(NotImplementedError, match=match):
         plan_problem(TraditionalAlgorithmRK(block), params, physics={"convective": "Feiereisen", "viscosity": None})
+
+
+def test_central4_weno5_plan():
+    """The Central4-WENO5 benchmark's objects (apps/compressible_taylor_green_vortex/
+    compressible_TGV.py:216 form, order 3 and optimize=True as the benchmark's
+    generated kernels): NS_Split('KGP', enthalpy, Sutherland), StoreSome(4),
+    RungeKuttaLS(3), WENOFilter(order=3, 'Z', 'LLF', optimize=True) plan to
+    problems.tgv_central4_weno5."""
+    from opensbli_amd.surface import WENOFilter
+    block = SimulationBlock(3, block_number=0)
+    block.set_block_boundaries([PeriodicBC(d, s) for d in range(3) for s in range(2)])
+    block.set_discretisation_schemes({"cent": StoreSome(4, "u0 u1 u2 T"), "rk": RungeKuttaLS(3)})
+    block.set_equations(WENOFilter(block, order=3, formulation="Z", flux_type="LLF", airfoil=False, metrics=None,
+                                   optimize=True).equation_classes)
+    block.discretise()
+    params = {"gama": "1.4", "Minf": "1.25", "Re": "1600.0", "Pr": "0.71", "dt": "0.00084625*2.0", "niter": "500",
+              "Ducros_check": "0.05", "Ducros_select": "0.05"}
+    for d in range(3):
+        params[f"block0np{d}"] = "16"
+        params[f"Delta{d}block0"] = f"2.0*M_PI/block0np{d}"
+    NS = NS_Split("KGP", 3, ["Re", "Pr", "gama", "Minf"], viscosity="dynamic", energy_formulation="enthalpy")
+    desc = plan_problem(TraditionalAlgorithmRK(block), params, physics=NS)
+    _same(desc, P.tgv_central4_weno5(n=16, dt=0.00084625 * 2.0).desc)
