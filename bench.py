@@ -57,6 +57,10 @@ def make_problem(name, precision):
         return P.tgv(n=256, steps=100, scheme="teno", minf=1.25, cfl=0.5, dt=0.0, teno_ct=1e-5, precision=prec)
     if name == "sod_1024":
         return P.sod(precision=prec)
+    if name == "tgv_teno6_128":
+        return P.tgv_teno6_validation()
+    if name == "tgv_central4weno5_128":
+        return P.tgv_central4_weno5()
     if name.startswith("kh_"):
         return P.kelvin_helmholtz(n=int(name.split("_")[1]), precision=prec)
     raise SystemExit(f"unknown workload {name}")
@@ -203,6 +207,19 @@ def serving_bench(count=64, rate=50.0, seed=20261203, budget_gib=8.0, max_solves
     return out
 
 
+def reference_benchmarks(steps=20, warmup=2):
+    """The reference's own performance-benchmark methods at their validation
+    size (apps/performance_benchmark/{TENO6,Central4-WENO5}, 128^3, FP64):
+    grid-point RK-stage updates per second over `steps` captured steps."""
+    out = {}
+    for name in ("tgv_teno6_128", "tgv_central4weno5_128"):
+        pr = make_problem(name, "fp64")
+        dt, prof, _ = time_gpu(pr, steps, warmup)
+        out[name] = {"value": pr.desc.npoints * 3.0 * steps / dt / 1e9, "unit": "GPts*stage/s", "steps_timed": steps,
+                     "kernel_ms_per_step": {k: v[0] / steps for k, v in prof.items() if v[1]}}
+    return out
+
+
 def mixed_headline(problem_name, steps, warmup):
     """The headline workload in mixed precision, with its error against FP64."""
     from opensbli_amd import native as nat
@@ -296,6 +313,7 @@ def main():
         out["mixed_precision"] = mixed_headline(args.workload, args.steps, args.warmup)
         out["ensemble"] = ensemble_bench(args.ensemble_steps)
         out["serving"] = serving_bench(args.serving_requests)
+        out["reference_benchmarks"] = reference_benchmarks()
     if args.copy_probe:
         import torch
         a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
