@@ -4,7 +4,8 @@
 // constituent relations of the tile plus a 2-point halo box in LDS:
 //   u0,u1,u2                at tile, faces and edges of the box (the edges feed
 //                           the mixed second derivatives D_b(D_a u))
-//   p, T, E=rhoE/rho, mu    at tile and faces (single-direction stencils)
+//   p, T, E=rhoE/rho, mu    at tile and faces (single-direction stencils);
+//                           the KGP form stores H=(rhoE+p)/rho in place of E
 // each evaluated once per box point.  Phase 2 gives every point its full
 // right-hand side -- split-form convective terms, viscous stresses and heat
 // conduction, or the interface-flux difference of a shock-capturing scheme --
@@ -22,7 +23,7 @@ struct CfTile { int tx, ty, tz; };
 constexpr CfTile CF_TILES[] = {{16, 4, 4}, {8, 8, 4}, {32, 4, 2}, {16, 8, 4}, {32, 4, 4}};
 constexpr int CF_NTILES = 5;
 
-enum CfConv { CF_NONE = 0, CF_FEIEREISEN = 1, CF_SKEW = 2 };
+enum CfConv { CF_NONE = 0, CF_FEIEREISEN = 1, CF_SKEW = 2, CF_KGP = 3 };
 
 // CONV: CfConv; VISC: 0 none, 1 constant, 2 Sutherland; FLUXRES: start R from
 // the face-flux difference (shock-capturing schemes); TIME: OSB_TIME_RKLS3 (1)
@@ -78,7 +79,7 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_fused(const d
             const double TT = (c.minf * c.minf) * c.gama * p / r;
             s_p[b] = p;
             s_T[b] = TT;
-            s_E[b] = rhoE / r;
+            s_E[b] = CONV == CF_KGP ? (rhoE + p) / r : rhoE / r;
             if (VISC == 2) s_mu[VISC == 2 ? b : 0] = 1.4042 * TT * sqrt(TT) / (0.40417 + TT);
         }
     }
@@ -140,6 +141,33 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_fused(const d
                 const double d_E = d1([&](int s) { return s_E[l + s * lj]; }, inv);
                 const double d_pu = d1([&](int s) { return s_p[l + s * lj] * uj(s); }, inv);
                 eterm += -0.5 * (d_Eu + qat(rhou_j, 0) * d_E) - d_pu;
+            } else if (CONV == CF_KGP) {
+                // KGP, enthalpy splitting, all coefficients 1/4 (split_forms.py:178-234): the term
+                // sequence of the CPU oracle, with u and H = (rhoE + p)/rho from LDS
+                const double ujc = uj(0), rhoujc = qat(rhou_j, 0);
+                const double gjj = G[jd][jd];
+                auto Hs = [&](int s) { return s_E[l + s * lj]; };
+                const double d_rhouj = d1([&](int s) { return qat(rhou_j, s); }, inv);
+                const double d_rho = d1([&](int s) { return qat(q, s); }, inv);
+                Rrho += 0.5 * d_rhouj + 0.5 * (ujc * d_rho + rho * gjj);
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    const double* rhou_m = q + (long)(1 + m) * P;
+                    auto um = [&](int s) { return s_u[m][l + s * lj]; };
+                    const double umc = um(0), rhoumc = qat(rhou_m, 0);
+                    const double d_rhouj_um = d1([&](int s) { return um(s) * qat(rhou_j, s); }, inv);
+                    const double d_rhoum = d1([&](int s) { return qat(rhou_m, s); }, inv);
+                    const double d_ujum = d1([&](int s) { return uj(s) * um(s); }, inv);
+                    conv[m] += 0.25 * ((((((d_rhouj_um + umc * d_rhouj) + rhoujc * G[m][jd]) + ujc * d_rhoum) + rhoumc * gjj) +
+                                        rho * d_ujum) + (umc * ujc) * d_rho);
+                }
+                const double Hc = Hs(0);
+                const double d_Hrhouj = d1([&](int s) { return Hs(s) * qat(rhou_j, s); }, inv);
+                const double d_H = d1(Hs, inv);
+                const double d_Hrho = d1([&](int s) { return Hs(s) * qat(q, s); }, inv);
+                const double d_Huj = d1([&](int s) { return Hs(s) * uj(s); }, inv);
+                eterm += 0.25 * ((((((d_Hrhouj + Hc * d_rhouj) + rhoujc * d_H) + ujc * d_Hrho) + (rho * Hc) * gjj) + rho * d_Huj) +
+                                 (ujc * Hc) * d_rho);
             } else {
                 const double ujc = uj(0);
                 const double d_rhou = d1([&](int s) { return qat(q, s) * uj(s); }, inv);
@@ -169,6 +197,9 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_fused(const d
         if (CONV == CF_FEIEREISEN) {
             Rv[0] = -div_rhou;
             Rv[4] = eterm - 0.5 * (rhoE / rho) * div_rhou;
+        } else if (CONV == CF_KGP) {
+            Rv[0] = -Rrho;
+            Rv[4] = -eterm;
         } else {
             Rv[0] = Rrho;
             Rv[4] = eterm;

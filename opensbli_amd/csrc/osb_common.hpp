@@ -439,11 +439,11 @@ __device__ __forceinline__ void face_core(PT&& pt, const Consts& c, double (&out
         }
     };
     point(2);              // the base point first (mixed precision subtracts it)
-    point(0);
+    if (FILT != 3) point(0);
     point(1);
     point(3);
     point(4);
-    point(5);
+    if (FILT != 3) point(5);   // the order-3 filter reads offsets -1..2 only
     // averaged state (averaging.py:31-124)
     T rho_av, u_av[NDIM], a_av;
     if (c.avg_roe) {
@@ -502,8 +502,9 @@ __device__ __forceinline__ void face_core(PT&& pt, const Consts& c, double (&out
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
         T fp[6], fm[6], cfv[6];
+        constexpr int S0 = FILT == 3 ? 1 : 0, S1 = FILT == 3 ? 5 : 6;
 #pragma unroll
-        for (int s = 0; s < 6; ++s) {
+        for (int s = S0; s < S1; ++s) {
             T cs = T(0), cf = T(0);
 #pragma unroll
             for (int m = 0; m < NV; ++m) { cs = fma(L[k][m], qs[s][m], cs); cf = fma(L[k][m], F[s][m], cf); }

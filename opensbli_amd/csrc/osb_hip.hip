@@ -344,6 +344,10 @@ void fused_stage(osb_ctx* c, const double* q, double* qn, double A, double B) {
         else launch_mixed_v<CF_FEIEREISEN, false, TIME>(c, q, qn, A, B);
         return;
     }
+    if (c->d.conv_scheme == OSB_CONV_CENTRAL_KGP) {           // tile kernel only
+        launch_fused_v<CF_KGP, false, TIME>(c, q, qn, A, B);
+        return;
+    }
     if (c->march) {
         if (c->shock) launch_march_v<CF_NONE, true, TIME>(c, q, qn, A, B);
         else if (c->d.conv_scheme == OSB_CONV_CENTRAL_SKEW) launch_march_v<CF_SKEW, false, TIME>(c, q, qn, A, B);
@@ -355,11 +359,10 @@ void fused_stage(osb_ctx* c, const double* q, double* qn, double A, double B) {
     else launch_fused_v<CF_FEIEREISEN, false, TIME>(c, q, qn, A, B);
 }
 
-// the fused 3-D stage kernel implements the Feiereisen and skew forms without
-// the Carpenter boundary closure; KGP and closure faces run the separate kernels
+// the fused 3-D stage kernel implements the Feiereisen, skew and KGP forms
+// without the Carpenter boundary closure; closure faces run the separate kernels
 bool use_fused(const osb_ctx* c) {
-    return c->g.ndim == 3 && (c->central || c->viscous) && c->d.conv_scheme != OSB_CONV_CENTRAL_KGP &&
-           !(c->c.carp[0] | c->c.carp[1] | c->c.carp[2]);
+    return c->g.ndim == 3 && (c->central || c->viscous) && !(c->c.carp[0] | c->c.carp[1] | c->c.carp[2]);
 }
 bool use_shock_stage(const osb_ctx* c) { return c->shock && !c->central && !c->viscous; }
 
@@ -571,7 +574,7 @@ AllocPlan plan_alloc(const osb_problem_desc* d) {
     const bool central = !shock, viscous = d->viscous != OSB_VISC_NONE;
     bool carp = false;
     for (int i = 0; i < nd; ++i) carp = carp || d->bc[i][0] >= OSB_BC_EXTRAPOLATION0 || d->bc[i][1] >= OSB_BC_EXTRAPOLATION0;
-    const bool fused = nd == 3 && (central || viscous) && d->conv_scheme != OSB_CONV_CENTRAL_KGP && !carp;
+    const bool fused = nd == 3 && (central || viscous) && !carp;
     const bool unfused_central = (central || viscous) && !fused;
     const size_t P = (size_t)g.Ptot;
     AllocPlan a;
