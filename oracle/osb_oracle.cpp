@@ -640,15 +640,25 @@ inline double carp2(G f, int r, int dirn) {
 // first / second derivative at index idx of n along a direction whose faces
 // carry the closure where mask bit 0 (side 0) / bit 1 (side 1) is set
 template <class G>
-inline double d1c(G f, double invd, int idx, int n, int mask) {
+[[gnu::noinline]] double d1c_edge(G f, double invd, int idx, int n, int mask) {
     if ((mask & 1) && idx < 4) return carp1(f, idx, 1) * invd;
     if ((mask & 2) && idx >= n - 4) return -carp1(f, n - 1 - idx, -1) * invd;
     return d1(f, invd);
 }
 template <class G>
-inline double d2c(G f, double inv2d, int idx, int n, int mask) {
+[[gnu::noinline]] double d2c_edge(G f, double inv2d, int idx, int n, int mask) {
     if ((mask & 1) && idx < 2) return carp2(f, idx, 1) * inv2d;
     if ((mask & 2) && idx >= n - 2) return carp2(f, n - 1 - idx, -1) * inv2d;
+    return d2(f, inv2d);
+}
+template <class G>
+inline double d1c(G f, double invd, int idx, int n, int mask) {
+    if (__builtin_expect(mask != 0, 0)) return d1c_edge(f, invd, idx, n, mask);
+    return d1(f, invd);
+}
+template <class G>
+inline double d2c(G f, double inv2d, int idx, int n, int mask) {
+    if (__builtin_expect(mask != 0, 0)) return d2c_edge(f, inv2d, idx, n, mask);
     return d2(f, inv2d);
 }
 
