@@ -81,7 +81,10 @@ int berr(osb_batch* b, int code, const std::string& m) {
 #define B_TRY(b, expr)                                                                        \
     do {                                                                                      \
         hipError_t e_ = (expr);                                                               \
-        if (e_ != hipSuccess) return berr((b), OSB_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+        if (e_ != hipSuccess) {                                                               \
+            (void)hipGetLastError(); /* clear the runtime's sticky error */                   \
+            return berr((b), OSB_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_));  \
+        }                                                                                     \
     } while (0)
 
 template <int DIR>
@@ -592,7 +595,10 @@ int perr(osb_pool* p, int code, const std::string& m) {
 #define P_TRY(p, expr)                                                                        \
     do {                                                                                      \
         hipError_t e_ = (expr);                                                               \
-        if (e_ != hipSuccess) return perr((p), OSB_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+        if (e_ != hipSuccess) {                                                               \
+            (void)hipGetLastError(); /* clear the runtime's sticky error */                   \
+            return perr((p), OSB_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_));  \
+        }                                                                                     \
     } while (0)
 
 int signature_group(osb_pool* p, const osb_ctx* c) {

@@ -526,9 +526,16 @@ int validate(const osb_problem_desc* d, std::string& why) {
 int dalloc(osb_ctx* c, double** p, size_t count) {
     const size_t bytes = count * sizeof(double);
     hipError_t e = hipMalloc((void**)p, bytes);
-    if (e != hipSuccess) return set_err(nullptr, OSB_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();      // clear the runtime's sticky error so later launch checks see only their own
+        return set_err(nullptr, OSB_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
     e = hipMemset(*p, 0, bytes);
-    if (e != hipSuccess) return set_err(nullptr, OSB_E_DEVICE, std::string("hipMemset: ") + hipGetErrorString(e));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return set_err(nullptr, OSB_E_DEVICE, std::string("hipMemset: ") + hipGetErrorString(e));
+    }
     c->bytes += bytes;
     return OSB_OK;
 }
@@ -716,7 +723,7 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     if (!r) r = dalloc(c, &c->part, (size_t)c->part_blocks * 4);
     if (!r) {
         hipError_t e = hipMalloc((void**)&c->nan_cnt, sizeof(unsigned int));
-        if (e != hipSuccess) r = set_err(nullptr, OSB_E_NOMEM, "hipMalloc nan counter");
+        if (e != hipSuccess) { c->nan_cnt = nullptr; (void)hipGetLastError(); r = set_err(nullptr, OSB_E_NOMEM, "hipMalloc nan counter"); }
     }
     if (!r) {
         hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

@@ -403,3 +403,28 @@ def _kgp2d_smooth(n=40, steps=12):
 def test_weno3_ducros_filter_parity(hip, oracle, maker):
     a, b = _compare(hip, oracle, maker())
     assert np.array_equal(a, b)
+
+
+# --- error paths that stay on the host side of every kernel ---------------------------
+def test_out_of_memory_is_an_error_not_a_crash(hip):
+    """A grid far beyond HBM fails context creation with OSB_E_NOMEM (every
+    partial allocation released); the library keeps working afterwards, and
+    zero / negative step counts behave as documented."""
+    # descriptor only (no host state): 2048^3 points, ~3 x 345 GB of state buffers
+    big = nat.ProblemDesc(**{**P.tgv(n=16).desc.__dict__, "n": [2048] * 3, "delta": [2 * math.pi / 2048] * 3})
+    _announce("oversized context")
+    assert hip.estimate_bytes(big) > 900 * 2 ** 30
+    with pytest.raises(nat.OSBError) as e:
+        hip.context(big)
+    assert e.value.code == nat.E_NOMEM
+    pr = P.kelvin_helmholtz(n=32, steps=2)
+    with hip.context(pr.desc) as c:
+        c.set_state(pr.q0)
+        c.step(0)
+        assert c.stats().iter == 0
+        with pytest.raises(nat.OSBError):
+            c.step(-1)
+        c.step(2)
+        q = c.get_state()
+    want, _ = run(hip, pr)
+    assert np.array_equal(q, want)
