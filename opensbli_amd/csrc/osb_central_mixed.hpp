@@ -25,14 +25,16 @@ __device__ __forceinline__ float d2f(A f, float inv2d) {
     return fmaf(-1.0f / 12.0f, f(-2) + f(2), fmaf(4.0f / 3.0f, f(-1) + f(1), (-5.0f / 2.0f) * f(0))) * inv2d;
 }
 
-// CONV: CF_FEIEREISEN or CF_NONE (with FLUXRES: the shock-capturing flux
+// CONV: CF_FEIEREISEN, CF_KGP (H = (rhoE + p)/rho staged in place of E and
+// rho in place of rhoE) or CF_NONE (with FLUXRES: the shock-capturing flux
 // difference, kept in FP64).  VISC: 0, 1 constant, 2 Sutherland.
 template <int CF_TX, int CF_TY, int CF_TZ, int CONV, int VISC, bool FLUXRES, int TIME>
 __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const double* __restrict__ q, const double* __restrict__ F,
                                                               double* __restrict__ reg, double* __restrict__ qout,
                                                               const double* __restrict__ scal, Geom g, Consts c,
                                                               double A, double B, int xcd_remap) {
-    static_assert(CONV == CF_FEIEREISEN || CONV == CF_NONE, "mixed central kernel: Feiereisen or viscous-only");
+    static_assert(CONV == CF_FEIEREISEN || CONV == CF_KGP || CONV == CF_NONE, "mixed central kernel: Feiereisen, KGP or viscous-only");
+    constexpr bool KGP = CONV == CF_KGP;
     constexpr int NV = 5;
     constexpr int CF_BX = CF_TX + 4, CF_BY = CF_TY + 4, CF_BZ = CF_TZ + 4;
     constexpr int CF_BOX = CF_BX * CF_BY * CF_BZ;
@@ -44,7 +46,7 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
     __shared__ float s_mu[SUTH ? CF_BOX : 1];
     __shared__ float s_p[HAS_CONV ? CF_BOX : 1];
     __shared__ float s_E[HAS_CONV ? CF_BOX : 1];
-    __shared__ float s_r[HAS_CONV ? 4 : 1][HAS_CONV ? CF_BOX : 1];    // rho u0..2, rhoE
+    __shared__ float s_r[HAS_CONV ? 4 : 1][HAS_CONV ? CF_BOX : 1];    // rho u0..2, rhoE (KGP: rho)
 
     const int ntx = gridDim.x, nty = gridDim.y;
     const int ntiles = ntx * nty * gridDim.z;
@@ -68,10 +70,11 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
             ref_u[d] = ref_r[d] / r;
             ke -= 0.5 * (ref_u[d] * ref_u[d]) * r;
         }
-        ref_r[3] = q[4 * P + x];
-        ref_p = c.gamma_m1 * (ke + ref_r[3]);
+        const double rE = q[4 * P + x];
+        ref_p = c.gamma_m1 * (ke + rE);
         ref_T = (c.minf * c.minf) * c.gama * ref_p / r;
-        ref_E = ref_r[3] / r;
+        ref_E = KGP ? (rE + ref_p) / r : rE / r;
+        ref_r[3] = KGP ? r : rE;
         ref_mu = SUTH ? 1.4042 * ref_T * sqrt(ref_T) / (0.40417 + ref_T) : 1.0;
     }
 
@@ -104,10 +107,10 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
             }
             if (HAS_CONV) {
                 s_p[HAS_CONV ? b : 0] = float(p - ref_p);
-                s_E[HAS_CONV ? b : 0] = float(rhoE * inv_r - ref_E);
+                s_E[HAS_CONV ? b : 0] = float((KGP ? (rhoE + p) * inv_r : rhoE * inv_r) - ref_E);
 #pragma unroll
                 for (int d = 0; d < 3; ++d) s_r[HAS_CONV ? d : 0][HAS_CONV ? b : 0] = float(rv[d] - ref_r[d]);
-                s_r[HAS_CONV ? 3 : 0][HAS_CONV ? b : 0] = float(rhoE - ref_r[3]);
+                s_r[HAS_CONV ? 3 : 0][HAS_CONV ? b : 0] = float((KGP ? r : rhoE) - ref_r[3]);
             }
         }
     }
@@ -149,7 +152,57 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
 #pragma unroll
         for (int jd = 0; jd < 3; ++jd) G[m][jd] = d1f([&](int s) { return s_u[m][l + s * ls[jd]]; }, invd[jd]);
 
-    if (HAS_CONV) {
+    if (KGP) {
+        // KGP, enthalpy splitting, coefficients 1/4 (split_forms.py:178-234) on deviations:
+        // D(ab) = a_ref D(b') + b_ref D(a') + D(a'b'); s_E holds H', s_r[3] rho'
+        float Rrho = 0.f, eterm = 0.f;
+        float conv[3] = {0.f, 0.f, 0.f};
+        float rc[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) rc[v] = float(ref_r[v]) + s_r[HAS_CONV ? v : 0][HAS_CONV ? l : 0];
+        const float rhoc = rc[3], Hc = float(ref_E) + s_E[HAS_CONV ? l : 0];
+        const float rref = float(ref_r[3]), Href = float(ref_E);
+#pragma unroll
+        for (int jd = 0; jd < 3; ++jd) {
+            const int lj = ls[jd];
+            const float inv = invd[jd];
+            const float urj = float(ref_u[jd]), rurj = float(ref_r[jd]);
+            auto ud = [&](int s) { return s_u[jd][l + s * lj]; };
+            auto rd = [&](int v, int s) { return s_r[HAS_CONV ? v : 0][HAS_CONV ? l + s * lj : 0]; };
+            auto hd = [&](int s) { return s_E[HAS_CONV ? l + s * lj : 0]; };
+            const float ujc = uc[jd], rhoujc = rc[jd];
+            const float gjj = G[jd][jd];
+            const float d_rhouj = d1f([&](int s) { return rd(jd, s); }, inv);
+            const float d_rho = d1f([&](int s) { return rd(3, s); }, inv);
+            Rrho += 0.5f * d_rhouj + 0.5f * (ujc * d_rho + rhoc * gjj);
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                auto umd = [&](int s) { return s_u[m][l + s * lj]; };
+                const float urm = float(ref_u[m]);
+                const float umc = uc[m], rhoumc = rc[m];
+                const float d_um = G[m][jd];
+                const float d_rhouj_um = urm * d_rhouj + rurj * d_um + d1f([&](int s) { return umd(s) * rd(jd, s); }, inv);
+                const float d_rhoum = d1f([&](int s) { return rd(m, s); }, inv);
+                const float d_ujum = urj * d_um + urm * gjj + d1f([&](int s) { return ud(s) * umd(s); }, inv);
+                conv[m] += 0.25f * ((((((d_rhouj_um + umc * d_rhouj) + rhoujc * G[m][jd]) + ujc * d_rhoum) + rhoumc * gjj) +
+                                     rhoc * d_ujum) + (umc * ujc) * d_rho);
+            }
+            const float d_H = d1f(hd, inv);
+            const float d_Hrhouj = Href * d_rhouj + rurj * d_H + d1f([&](int s) { return hd(s) * rd(jd, s); }, inv);
+            const float d_Hrho = Href * d_rho + rref * d_H + d1f([&](int s) { return hd(s) * rd(3, s); }, inv);
+            const float d_Huj = Href * gjj + urj * d_H + d1f([&](int s) { return hd(s) * ud(s); }, inv);
+            eterm += 0.25f * ((((((d_Hrhouj + Hc * d_rhouj) + rhoujc * d_H) + ujc * d_Hrho) + (rhoc * Hc) * gjj) + rhoc * d_Huj) +
+                              (ujc * Hc) * d_rho);
+        }
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            const int lm = ls[m];
+            const float dp = d1f([&](int s) { return s_p[HAS_CONV ? l + s * lm : 0]; }, invd[m]);
+            Rv[1 + m] = -dp - conv[m];
+        }
+        Rv[0] = -Rrho;
+        Rv[4] = -eterm;
+    } else if (HAS_CONV) {
         const float* su = nullptr;
         (void)su;
         float div_rhou = 0.f, eterm = 0.f;

@@ -341,6 +341,7 @@ void fused_stage(osb_ctx* c, const double* q, double* qn, double A, double B) {
     if (c->mixed && c->d.conv_scheme != OSB_CONV_CENTRAL_SKEW) {
         // FP32 derivatives on deviations (the skew form stays FP64)
         if (c->shock) launch_mixed_v<CF_NONE, true, TIME>(c, q, qn, A, B);
+        else if (c->d.conv_scheme == OSB_CONV_CENTRAL_KGP) launch_mixed_v<CF_KGP, false, TIME>(c, q, qn, A, B);
         else launch_mixed_v<CF_FEIEREISEN, false, TIME>(c, q, qn, A, B);
         return;
     }
@@ -479,7 +480,7 @@ int validate(const osb_problem_desc* d, std::string& why) {
     const int cs = d->conv_scheme;
     const bool central_cs = cs == OSB_CONV_CENTRAL_FEIEREISEN || cs == OSB_CONV_CENTRAL_SKEW || cs == OSB_CONV_CENTRAL_KGP;
     if (cs != OSB_CONV_WENO_Z && cs != OSB_CONV_TENO && !central_cs) { why = "unknown conv_scheme"; return OSB_E_UNSUPPORTED; }
-    if (cs == OSB_CONV_CENTRAL_KGP && d->precision == OSB_PREC_MIXED) { why = "mixed precision is not built for the KGP split form"; return OSB_E_UNSUPPORTED; }
+    if (cs == OSB_CONV_CENTRAL_KGP && d->precision == OSB_PREC_MIXED && d->ndim != 3) { why = "mixed precision for the KGP split form is built for 3-D blocks"; return OSB_E_UNSUPPORTED; }
     if (cs == OSB_CONV_WENO_Z && d->order != 5) { why = "WENO-Z supports order 5"; return OSB_E_UNSUPPORTED; }
     if (cs == OSB_CONV_TENO && d->order != 5 && d->order != 6) { why = "TENO supports orders 5 and 6"; return OSB_E_UNSUPPORTED; }
     if (central_cs && d->order != 4) { why = "central scheme supports order 4"; return OSB_E_UNSUPPORTED; }

@@ -109,11 +109,13 @@ def test_mixed_precision_error_vs_fp64_oracle(hip, oracle):
     """FP32 flux / derivative evaluation over the FP64 state: relative L2
     against the FP64 oracle, per conservative variable: KH 128^2 x 100 steps,
     TENO5 + viscous TGV 24^3 x 20 (CFL), Sod 1024 x 1000, the central
-    viscous TGV 32^3 x 50 and KH 128^2 with GLF splitting (the stated bar is 1e-5)."""
+    viscous TGV 32^3 x 50 (Feiereisen and KGP) and KH 128^2 with GLF splitting
+    (the stated bar is 1e-5)."""
     cases = [(P.kelvin_helmholtz(n=128, steps=100, precision=nat.PREC_MIXED), 100),
              (P.tgv(n=24, steps=20, scheme="teno", minf=1.25, cfl=0.5, dt=0.0, precision=nat.PREC_MIXED), 20),
              (P.sod(n=1024, steps=1000, precision=nat.PREC_MIXED), 1000),
-             (P.tgv(n=32, steps=50, precision=nat.PREC_MIXED), 50)]
+             (P.tgv(n=32, steps=50, precision=nat.PREC_MIXED), 50),
+             (P.tgv(n=32, steps=50, scheme="kgp", precision=nat.PREC_MIXED), 50)]
     kg = P.kelvin_helmholtz(n=128, steps=100, precision=nat.PREC_MIXED)
     kg.desc.flux_type = nat.FLUX_GLF
     kg.name += "_glf"
