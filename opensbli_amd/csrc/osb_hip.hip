@@ -385,7 +385,9 @@ void filter_nd_t(osb_ctx* c, double* q) {
     if constexpr (HYB && NDIM >= 2) {
         double sd = 0.0;
         for (int d = 0; d < NDIM; ++d) sd += c->d.delta[d] * c->d.delta[d];
-        k_ducros<NDIM><<<nblocks(interior_points(g)), BLOCK, 0, c->stream>>>(q, c->kappa, g, fc, std::sqrt(sd));
+        // the face-flux array is free until the filter faces: it holds u for the sensor
+        k_velocity<NDIM><<<nblocks(box_points(g, 2)), BLOCK, 0, c->stream>>>(q, c->F, g);
+        k_ducros<NDIM><<<nblocks(interior_points(g)), BLOCK, 0, c->stream>>>(c->F, c->kappa, g, fc, std::sqrt(sd));
     }
     constexpr int D1 = NDIM > 1 ? 1 : 0, D2 = NDIM > 2 ? 2 : 0;
     const long P = (long)c->nv * g.Ptot;

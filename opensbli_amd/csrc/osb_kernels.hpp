@@ -88,12 +88,31 @@ __global__ void __launch_bounds__(BLOCK) k_filter_face(const double* __restrict_
     for (int v = 0; v < NV; ++v) F[(long)v * g.Ptot + x0] = out[v];
 }
 
+// u_m = rhou_m / rho over [-2, n+2) in the active directions, for the sensor
+template <int NDIM>
+__global__ void __launch_bounds__(BLOCK) k_velocity(const double* __restrict__ q, double* __restrict__ u, Geom g) {
+    int e[3], lo[3];
+    for (int d = 0; d < 3; ++d) {
+        const bool act = d < NDIM;
+        e[d] = act ? g.n[d] + 4 : 1;
+        lo[d] = act ? -2 : 0;
+    }
+    const long t = gtid();
+    if (t >= (long)e[0] * e[1] * e[2]) return;
+    int i, j, k;
+    unflatten(t, e, lo, i, j, k);
+    const long x = gidx(g, i, j, k);
+    const double r = q[x];
+#pragma unroll
+    for (int m = 0; m < NDIM; ++m) u[(long)m * g.Ptot + x] = q[(long)(1 + m) * g.Ptot + x] / r;
+}
+
 // Modified Ducros sensor (shock_sensors.py:16-59; generated
 // Central4-WENO5/opensbliblock00_kernels.h:572-617) over [0, n):
 // kappa = div^2 (1/2 - 1/2 tanh(2.5 + 500 div / sqrt(sum Delta^2))) / (div^2 + |curl u|^2 + 1e-40)
-// with central derivatives of u = rhou/rho; the halos of kappa stay zero.
+// with central derivatives of u (k_velocity); the halos of kappa stay zero.
 template <int NDIM>
-__global__ void __launch_bounds__(BLOCK) k_ducros(const double* __restrict__ q, double* __restrict__ kappa, Geom g, Consts c,
+__global__ void __launch_bounds__(BLOCK) k_ducros(const double* __restrict__ u, double* __restrict__ kappa, Geom g, Consts c,
                                                   double sd) {
     const int e[3] = {g.n[0], g.n[1], g.n[2]};
     const int lo[3] = {0, 0, 0};
@@ -110,8 +129,8 @@ __global__ void __launch_bounds__(BLOCK) k_ducros(const double* __restrict__ q, 
 #pragma unroll
         for (int d = 0; d < NDIM; ++d) {
             const long st = g.s[d];
-            G[m][d] = d1c([&](int s) { const long y = x + s * st; return q[(long)(1 + m) * P + y] / q[y]; }, c.invd[d], ii[d],
-                          g.n[d], c.carp[d]);
+            const double* um = u + (long)m * P;
+            G[m][d] = d1c([&](int s) { return um[x + s * st]; }, c.invd[d], ii[d], g.n[d], c.carp[d]);
         }
     double div = 0.0;
 #pragma unroll
