@@ -428,3 +428,12 @@ def test_out_of_memory_is_an_error_not_a_crash(hip):
         q = c.get_state()
     want, _ = run(hip, pr)
     assert np.array_equal(q, want)
+
+
+# --- long horizons: a one-ulp difference would have grown by now ----------------------
+@pytest.mark.parametrize("maker", [lambda: P.kelvin_helmholtz(n=128, steps=1000),   # WENO-Z, fused faces, walls
+                                   lambda: P.tgv(n=32, steps=300),                    # fused central + viscous
+                                   lambda: P.tgv_central4_weno5(n=24, steps=200)])    # fused KGP + hybrid filter
+def test_long_horizon_bitwise(hip, oracle, maker):
+    a, b = _compare(hip, oracle, maker())
+    assert np.array_equal(a, b)
