@@ -59,6 +59,9 @@ extern "C" {
 /* flux splitting of the shock-capturing schemes (LFWeno/LFTeno flux_type) */
 #define OSB_FLUX_LLF 0                /* local: lambda = max over {i, i+1}, shock_capturing.py:1000-1028 */
 #define OSB_FLUX_GLF 1                /* global: lambda = domain max per stage, shock_capturing.py:416-438, :838-857 */
+#define OSB_FLUX_HLLC 2               /* HLLCWeno(flux_type='HLLC'): WENO-Z of the characteristic state, HLLC
+                                         Riemann flux of the reconstructed states, shock_capturing.py:1031-1256 */
+#define OSB_FLUX_HLLC_LM 3            /* HLLCWeno(flux_type='HLLC-LM'): the low-Mach HLLC variant (same lines) */
 
 /* non-linear filter step after each full RK step (WENOFilter, filters/WENO_filter.py:406) */
 #define OSB_FILTER_NONE  0

@@ -207,7 +207,7 @@ def plan_problem(alg_or_block, parameters: Mapping, physics=None, precision: str
     shock, central, time = None, None, None
     for sch in block.discretisation_schemes.values():
         kinds = _kinds(sch)
-        if kinds & {"LFWeno", "LFTeno"}:
+        if kinds & {"LFWeno", "LFTeno", "HLLCWeno"}:
             if shock is not None:
                 raise NotImplementedError("one shock-capturing scheme per block")
             shock = sch
@@ -223,11 +223,16 @@ def plan_problem(alg_or_block, parameters: Mapping, physics=None, precision: str
     desc = dict(ndim=ndim, n=n, delta=delta)
     if shock is not None:
         kinds = _kinds(shock)
-        ft = getattr(shock, "flux_type", None) or "LLF"      # shock_capturing.py:780-781
-        if ft not in ("LLF", "GLF"):
+        ft = getattr(shock, "flux_type", None) or ("HLLC" if "HLLCWeno" in kinds else "LLF")   # shock_capturing.py:780-781
+        if "HLLCWeno" in kinds:
+            if ft not in ("HLLC", "HLLC-LM"):
+                raise NotImplementedError(f"HLLCWeno(flux_type={ft!r}): HLLC and HLLC-LM are implemented")
+            desc["flux_type"] = nat.FLUX_HLLC_LM if ft == "HLLC-LM" else nat.FLUX_HLLC
+        elif ft not in ("LLF", "GLF"):
             raise NotImplementedError(f"{type(shock).__name__}(flux_type={ft!r}): LLF and GLF are implemented")
-        desc["flux_type"] = nat.FLUX_GLF if ft == "GLF" else nat.FLUX_LLF
-        if "LFWeno" in kinds:
+        else:
+            desc["flux_type"] = nat.FLUX_GLF if ft == "GLF" else nat.FLUX_LLF
+        if kinds & {"LFWeno", "HLLCWeno"}:
             if shock.order != 5 or str(getattr(shock, "formulation", "JS")).upper() != "Z":
                 raise NotImplementedError(
                     f"LFWeno(order={shock.order}, formulation={shock.formulation!r}): only order 5, 'Z'")
@@ -241,6 +246,8 @@ def plan_problem(alg_or_block, parameters: Mapping, physics=None, precision: str
             if "eps" in p:
                 desc["teno_eps"] = float(p["eps"])
         desc["averaging"] = _averaging(shock)
+        if "HLLCWeno" in kinds and getattr(shock, "averaging", None) is None:
+            desc["averaging"] = nat.AVG_ROE        # HLLCCharacteristic's default (shock_capturing.py:1040-1041)
         desc["viscous"] = _viscous_model(physics)
         if desc["viscous"] != nat.VISC_NONE and (central is None or central.order != 4):
             raise NotImplementedError("viscous terms need a Central(4) scheme on the block")

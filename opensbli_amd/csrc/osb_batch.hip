@@ -53,7 +53,9 @@ struct Group {
     int total(int f) const { return offs[(size_t)f * (members.size() + 1) + members.size()]; }
 };
 
-bool batchable(const osb_ctx* c) { return c->shock && !c->central && !c->viscous && c->d.shock_filter == OSB_FILTER_NONE; }
+bool batchable(const osb_ctx* c) {
+    return c->shock && !c->central && !c->viscous && c->d.shock_filter == OSB_FILTER_NONE && c->d.flux_type <= OSB_FLUX_GLF;
+}
 
 inline unsigned vblocks(long work) { return (unsigned)((work + BLOCK - 1) / BLOCK); }
 

@@ -543,6 +543,149 @@ __device__ __forceinline__ void face_core(PT&& pt, const Consts& c, double (&out
     }
 }
 
+// sin(x) for x in [0, pi/2] by its Taylor series through x^21 (truncation
+// below 1.2e-18 there), Horner in x^2 with fused multiply-adds: the HLLC-LM
+// Mach switch evaluated identically on every platform.
+__device__ __forceinline__ double hllc_sin(double x) {
+    const double x2 = x * x;
+    double s = -1.0 / 51090942171709440000.0;                          // -1/21!
+    s = fma(s, x2, 1.0 / 121645100408832000.0);                         //  1/19!
+    s = fma(s, x2, -1.0 / 355687428096000.0);                           // -1/17!
+    s = fma(s, x2, 1.0 / 1307674368000.0);                              //  1/15!
+    s = fma(s, x2, -1.0 / 6227020800.0);                                // -1/13!
+    s = fma(s, x2, 1.0 / 39916800.0);                                   //  1/11!
+    s = fma(s, x2, -1.0 / 362880.0);                                    // -1/9!
+    s = fma(s, x2, 1.0 / 5040.0);                                       //  1/7!
+    s = fma(s, x2, -1.0 / 120.0);                                       // -1/5!
+    s = fma(s, x2, 1.0 / 6.0);                                          //  1/3!
+    return fma(-(x * x2), s, x);                                        // x - x^3 (1/3! - x^2/5! + ...)
+}
+
+// HLLC / HLLC-LM interface flux (shock_capturing.py:1031-1256) at x0 + 1/2
+// along DIR: WENO-Z of the characteristic state CS = L q at the six points
+// (the right reconstruction is the face's left state, the left one its right
+// state), back-projection, then the HLLC Riemann flux of the two states with
+// sL = min(ubar - abar, uL - aL), sR = max(ubar + abar, uR + aR), the contact
+// speed s*, the star states (sL in both star energies, as the reference
+// writes them) and, for LM, sL and sR scaled by
+// phi = sin(min(1, max(|uL/aL|, |uR/aR|)/0.1) pi/2).  FP64, the oracle's
+// operation order.
+template <int NDIM, int DIR, bool LM>
+__device__ __forceinline__ void face_hllc(const double* __restrict__ q, long Ptot, long st, long x0, const Consts& c,
+                                          double (&out)[NDIM + 2]) {
+    constexpr int NV = NDIM + 2;
+    double qs[6][NV];
+    double rho23[2], u23[2][NDIM], p23[2], E23[2], a23[2];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        const long xs = x0 + (long)(s - 2) * st;
+#pragma unroll
+        for (int m = 0; m < NV; ++m) qs[s][m] = q[(long)m * Ptot + xs];
+        if (s == 2 || s == 3) {
+            double f[NV], uu[NDIM], p, a;
+            point_prims<NDIM, DIR, false>(qs[s], c, uu, p, f, a);
+            const int w = s - 2;
+            rho23[w] = qs[s][0]; p23[w] = p; E23[w] = qs[s][NV - 1]; a23[w] = a;
+#pragma unroll
+            for (int d = 0; d < NDIM; ++d) u23[w][d] = uu[d];
+        }
+    }
+    double rho_av, u_av[NDIM], a_av;
+    if (c.avg_roe) {
+        const double sl = sqrt(rho23[0]), sr = sqrt(rho23[1]);
+        const double inv = 1.0 / (sl + sr);
+        double ke = 0.0;
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) { u_av[d] = (sl * u23[0][d] + sr * u23[1][d]) * inv; ke += u_av[d] * u_av[d]; }
+        const double Hr = ((p23[0] + E23[0]) / sl + (p23[1] + E23[1]) / sr) * inv;
+        rho_av = sqrt(rho23[0] * rho23[1]);
+        a_av = sqrt(c.gamma_m1 * (Hr - 0.5 * ke));
+    } else {
+        rho_av = 0.5 * (rho23[0] + rho23[1]);
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) u_av[d] = 0.5 * (u23[0][d] + u23[1][d]);
+        a_av = 0.5 * (a23[0] + a23[1]);
+    }
+    double L[NV][NV], R[NV][NV];
+    eigensystem<NDIM, DIR, double>(c.gamma_m1, rho_av, u_av, a_av, L, R);
+    double qL[NV], qR[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) { qL[v] = 0.0; qR[v] = 0.0; }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double cs[6];
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            double a = 0.0;
+#pragma unroll
+            for (int m = 0; m < NV; ++m) a = fma(L[k][m], qs[s][m], a);
+            cs[s] = a;
+        }
+        const double wr = wenoz_right<double>(cs, c.weno_eps), wl = wenoz_left<double>(cs, c.weno_eps);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { qL[v] = fma(R[v][k], wr, qL[v]); qR[v] = fma(R[v][k], wl, qR[v]); }
+    }
+    const double gama = c.gama;
+    const double irL = 1.0 / qL[0], irR = 1.0 / qR[0];
+    double uL[NDIM], uR[NDIM], keL = 0.0, keR = 0.0;
+#pragma unroll
+    for (int m = 0; m < NDIM; ++m) {
+        uL[m] = qL[1 + m] * irL;
+        uR[m] = qR[1 + m] * irR;
+        keL = fma(uL[m], uL[m], keL);
+        keR = fma(uR[m], uR[m], keR);
+    }
+    const double pL = (gama - 1.0) * (qL[NV - 1] - 0.5 * qL[0] * keL);
+    const double pR = (gama - 1.0) * (qR[NV - 1] - 0.5 * qR[0] * keR);
+    const double aL = sqrt(gama * pL * irL), aR = sqrt(gama * pR * irR);
+    const double ul = uL[DIR], ur = uR[DIR];
+    double sL = fmin(u_av[DIR] - a_av, ul - aL);
+    double sR = fmax(u_av[DIR] + a_av, ur + aR);
+    const double ss = ((pR - pL) + (qL[0] * ul * (sL - ul) - qR[0] * ur * (sR - ur))) / (qL[0] * (sL - ul) - qR[0] * (sR - ur));
+    if (LM) {
+        const double ml = fmax(fabs(ul / aL), fabs(ur / aR));
+        const double phi = hllc_sin(fmin(1.0, ml / 0.1) * (0.5 * 3.14159265358979323846));
+        sL = sL * phi;
+        sR = sR * phi;
+    }
+    double FL[NV], FR[NV], SL[NV], SR[NV];
+    FL[0] = qL[1 + DIR];
+    FR[0] = qR[1 + DIR];
+#pragma unroll
+    for (int m = 0; m < NDIM; ++m) {
+        FL[1 + m] = qL[1 + m] * ul + (m == DIR ? pL : 0.0);
+        FR[1 + m] = qR[1 + m] * ur + (m == DIR ? pR : 0.0);
+    }
+    FL[NV - 1] = (pL + qL[NV - 1]) * ul;
+    FR[NV - 1] = (pR + qR[NV - 1]) * ur;
+    const double fl = qL[0] * ((sL - ul) / (sL - ss)), fr = qR[0] * ((sR - ur) / (sR - ss));
+    SL[0] = fl;
+    SR[0] = fr;
+#pragma unroll
+    for (int m = 0; m < NDIM; ++m) {
+        SL[1 + m] = fl * (m == DIR ? ss : uL[m]);
+        SR[1 + m] = fr * (m == DIR ? ss : uR[m]);
+    }
+    SL[NV - 1] = fl * fma(ss - ul, ss + pL / (qL[0] * (sL - ul)), qL[NV - 1] * irL);
+    SR[NV - 1] = fr * fma(ss - ur, ss + pR / (qR[0] * (sL - ur)), qR[NV - 1] * irR);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        double f;
+        if (LM) {
+            if (sL >= 0.0) f = FL[v];
+            else if (sR <= 0.0) f = FR[v];
+            else f = 0.5 * (FL[v] + FR[v]) + 0.5 * ((sL * (SL[v] - qL[v]) + fabs(ss) * (SL[v] - SR[v])) + sR * (SR[v] - qR[v]));
+        } else {
+            if (sL >= 0.0) f = FL[v];
+            else if (sL <= 0.0 && 0.0 <= ss) f = fma(sL, SL[v] - qL[v], FL[v]);
+            else if (ss <= 0.0 && 0.0 <= sR) f = fma(sR, SR[v] - qR[v], FR[v]);
+            else if (sR <= 0.0) f = FR[v];
+            else f = 0.0;
+        }
+        out[v] = f;
+    }
+}
+
 // face flux at interface x0 + 1/2 along DIR reading the six points from the
 // state (q: FP64 state, `st` the stride along DIR)
 template <int NDIM, int DIR, int SCH, typename T, int FILT = 0>

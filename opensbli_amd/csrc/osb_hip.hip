@@ -117,6 +117,14 @@ void launch_flux_t(osb_ctx* c, const double* q) {
 
 template <int NDIM, int DIR>
 void launch_flux(osb_ctx* c, const double* q) {
+    if (c->d.flux_type == OSB_FLUX_HLLC || c->d.flux_type == OSB_FLUX_HLLC_LM) {     // FP64 only
+        double* F = c->F + (long)DIR * c->nv * c->g.Ptot;
+        Timed tm(c, PF_FACE_FLUX);
+        const long work = interior_points(c->g) / c->g.n[DIR] * (long)(c->g.n[DIR] + 1);
+        if (c->d.flux_type == OSB_FLUX_HLLC_LM) k_face_hllc<NDIM, DIR, true><<<nblocks(work), BLOCK, 0, c->stream>>>(q, F, c->g, c->c);
+        else k_face_hllc<NDIM, DIR, false><<<nblocks(work), BLOCK, 0, c->stream>>>(q, F, c->g, c->c);
+        return;
+    }
     if (c->mixed) launch_flux_t<NDIM, DIR, float>(c, q);
     else launch_flux_t<NDIM, DIR, double>(c, q);
 }
@@ -491,7 +499,9 @@ int validate(const osb_problem_desc* d, std::string& why) {
     if (d->viscous != OSB_VISC_NONE && d->ndim < 2) { why = "viscous terms need ndim >= 2"; return OSB_E_UNSUPPORTED; }
     if (d->time_scheme != OSB_TIME_RKLS3 && d->time_scheme != OSB_TIME_RK3) { why = "unknown time scheme"; return OSB_E_UNSUPPORTED; }
     if (d->precision != OSB_PREC_FP64 && d->precision != OSB_PREC_MIXED) { why = "unknown precision"; return OSB_E_UNSUPPORTED; }
-    if (d->flux_type != OSB_FLUX_LLF && d->flux_type != OSB_FLUX_GLF) { why = "unknown flux type"; return OSB_E_UNSUPPORTED; }
+    if (d->flux_type < OSB_FLUX_LLF || d->flux_type > OSB_FLUX_HLLC_LM) { why = "unknown flux type"; return OSB_E_UNSUPPORTED; }
+    if (d->flux_type >= OSB_FLUX_HLLC && cs != OSB_CONV_WENO_Z) { why = "HLLC is built for the WENO-Z scheme (HLLCWeno)"; return OSB_E_UNSUPPORTED; }
+    if (d->flux_type >= OSB_FLUX_HLLC && d->precision != OSB_PREC_FP64) { why = "HLLC runs in FP64 only"; return OSB_E_UNSUPPORTED; }
     if (d->shock_filter != OSB_FILTER_NONE && d->shock_filter != OSB_FILTER_WENOZ) { why = "unknown shock filter"; return OSB_E_UNSUPPORTED; }
     if (d->shock_filter == OSB_FILTER_WENOZ && d->precision != OSB_PREC_FP64) { why = "the WENO filter runs in FP64 only"; return OSB_E_UNSUPPORTED; }
     if (d->shock_filter == OSB_FILTER_WENOZ && d->flux_type != OSB_FLUX_LLF) { why = "the WENO filter runs with LLF splitting of the base scheme only"; return OSB_E_UNSUPPORTED; }
@@ -753,6 +763,10 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
         if (w == 16 || w == 32) c->fused_fw = w;
     }
     if (c->face_tiled) c->shock_fused = false;      // the LDS-tiled face variant keeps the separate update
+    if (desc->flux_type >= OSB_FLUX_HLLC) {           // HLLC faces run their own kernel and the separate update
+        c->face_tiled = false;
+        c->shock_fused = false;
+    }
     if (const char* e = std::getenv("OSB_MARCH_KZ")) c->march_kz = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("OSB_MARCH_TILE")) c->march_tile = std::atoi(e);
     *out = c;

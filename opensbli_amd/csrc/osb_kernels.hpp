@@ -51,6 +51,26 @@ __global__ void __launch_bounds__(BLOCK) k_face_flux(const double* __restrict__ 
     face_flux_item<NDIM, DIR, SCH, T>(gtid(), q, F, g, c);
 }
 
+// HLLC / HLLC-LM interface fluxes along DIR over faces [-1, n) (HLLCWeno,
+// shock_capturing.py:1031-1256)
+template <int NDIM, int DIR, bool LM>
+__global__ void __launch_bounds__(BLOCK) k_face_hllc(const double* __restrict__ q, double* __restrict__ F, Geom g, Consts c) {
+    constexpr int NV = NDIM + 2;
+    int e[3] = {g.n[0], g.n[1], g.n[2]};
+    e[DIR] += 1;
+    int lo[3] = {0, 0, 0};
+    lo[DIR] = -1;
+    const long t = gtid();
+    if (t >= (long)e[0] * e[1] * e[2]) return;
+    int i, j, k;
+    unflatten(t, e, lo, i, j, k);
+    const long x0 = gidx(g, i, j, k);
+    double out[NV];
+    face_hllc<NDIM, DIR, LM>(q, g.Ptot, g.s[DIR], x0, c, out);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) F[(long)v * g.Ptot + x0] = out[v];
+}
+
 // WENO filter fluxes along DIR (WENO_filter.py:406-470): faces [-1, n) as the
 // reconstruction kernels of the scheme; c carries RoeAverage and LLF.  ORDER 5
 // or 3; HYB: the Ducros hybrid selection (WENO_filter.py:561-584) -- a face is

@@ -54,6 +54,8 @@ BC_DIRICHLET = 7
 
 FLUX_LLF = 0
 FLUX_GLF = 1
+FLUX_HLLC = 2
+FLUX_HLLC_LM = 3
 
 FILTER_NONE = 0
 FILTER_WENOZ = 1

@@ -66,6 +66,28 @@ class LFWeno(Scheme):
         self.species = species
 
 
+class HLLCWeno(Scheme):
+    """HLLC Riemann fluxes of WENO-reconstructed characteristic states
+    (opensbli/schemes/spatial/weno.py:586, shock_capturing.py:1031);
+    flux_type 'HLLC' or 'HLLC-LM' (low-Mach variant).  Without an averaging
+    the characteristic decomposition uses RoeAverage (shock_capturing.py:1040-1041)."""
+
+    def __init__(self, order, physics=None, averaging=None, shock_filter=None, formulation="JS",
+                 conservative=True, flux_type="HLLC", flux_split=False):
+        if order % 2 == 0:
+            raise ValueError(f"Please set an odd-order for the WENO scheme, currently {order} is not supported")
+        if flux_type not in ("HLLC", "HLLC-LM"):
+            raise ValueError("Please select either HLLC or HLLC-LM for the flux-splitting.")
+        super().__init__("WenoDerivative", order)
+        self.physics = physics
+        self.averaging = averaging
+        self.shock_filter = shock_filter
+        self.formulation = formulation
+        self.conservative = conservative
+        self.flux_type = flux_type
+        self.flux_split = False
+
+
 class LFTeno(Scheme):
     """Local Lax-Friedrichs characteristic TENO (opensbli/schemes/spatial/teno.py:480)."""
 

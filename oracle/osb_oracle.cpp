@@ -453,6 +453,94 @@ double teno_side(const double* f, int side, const TenoParams& tp) {
 }
 
 // ---------------------------------------------------------------------------
+// sin(x) for x in [0, pi/2] by its Taylor series through x^21 (truncation
+// below 1.2e-18 there), Horner in x^2 with fused multiply-adds: the HLLC-LM
+// Mach switch evaluated identically on every platform.
+// ---------------------------------------------------------------------------
+inline double hllc_sin(double x) {
+    const double x2 = x * x;
+    double s = -1.0 / 51090942171709440000.0;                          // -1/21!
+    s = std::fma(s, x2, 1.0 / 121645100408832000.0);                    //  1/19!
+    s = std::fma(s, x2, -1.0 / 355687428096000.0);                      // -1/17!
+    s = std::fma(s, x2, 1.0 / 1307674368000.0);                         //  1/15!
+    s = std::fma(s, x2, -1.0 / 6227020800.0);                           // -1/13!
+    s = std::fma(s, x2, 1.0 / 39916800.0);                              //  1/11!
+    s = std::fma(s, x2, -1.0 / 362880.0);                               // -1/9!
+    s = std::fma(s, x2, 1.0 / 5040.0);                                  //  1/7!
+    s = std::fma(s, x2, -1.0 / 120.0);                                  // -1/5!
+    s = std::fma(s, x2, 1.0 / 6.0);                                     //  1/3! (sign below)
+    // sin x = x - x^3 (1/3! - x^2/5! + ...) = x - x^3 * s
+    return std::fma(-(x * x2), s, x);
+}
+
+// HLLC / HLLC-LM interface flux of reconstructed left and right conservative
+// states along dir (shock_capturing.py:1108-1256): wave speeds
+// sL = min(ubar - abar, uL - aL), sR = max(ubar + abar, uR + aR) with the face's
+// averaged state, the contact speed s*, the star states
+// rho_K (s_K - u_K)/(s_K - s*) [1, u (s* along dir), E_K/rho_K + (s* - u_K)(s* + p_K/(rho_K (sL - u_K)))]
+// (the reference writes sL in both star energies), then the four-region HLLC
+// flux or, for HLLC-LM, the low-Mach form with sL, sR scaled by
+// phi = sin(min(1, max(|uL/aL|, |uR/aR|)/0.1) pi/2).
+void hllc_flux(int nd, int dir, double gama, bool lm, const double* qL, const double* qR, double ubar, double abar,
+               double* out) {
+    const int nv = nd + 2;
+    const double irL = 1.0 / qL[0], irR = 1.0 / qR[0];
+    double uL[3] = {0, 0, 0}, uR[3] = {0, 0, 0}, keL = 0.0, keR = 0.0;
+    for (int m = 0; m < nd; ++m) {
+        uL[m] = qL[1 + m] * irL;
+        uR[m] = qR[1 + m] * irR;
+        keL = std::fma(uL[m], uL[m], keL);
+        keR = std::fma(uR[m], uR[m], keR);
+    }
+    const double pL = (gama - 1.0) * (qL[nv - 1] - 0.5 * qL[0] * keL);
+    const double pR = (gama - 1.0) * (qR[nv - 1] - 0.5 * qR[0] * keR);
+    const double aL = std::sqrt(gama * pL * irL), aR = std::sqrt(gama * pR * irR);
+    const double ul = uL[dir], ur = uR[dir];
+    double sL = std::min(ubar - abar, ul - aL);
+    double sR = std::max(ubar + abar, ur + aR);
+    const double ss = ((pR - pL) + (qL[0] * ul * (sL - ul) - qR[0] * ur * (sR - ur))) / (qL[0] * (sL - ul) - qR[0] * (sR - ur));
+    if (lm) {
+        const double ml = std::max(std::fabs(ul / aL), std::fabs(ur / aR));
+        const double phi = hllc_sin(std::min(1.0, ml / 0.1) * (0.5 * 3.14159265358979323846));
+        sL = sL * phi;
+        sR = sR * phi;
+    }
+    double FL[5], FR[5], SL[5], SR[5];
+    FL[0] = qL[1 + dir];
+    FR[0] = qR[1 + dir];
+    for (int m = 0; m < nd; ++m) {
+        FL[1 + m] = qL[1 + m] * ul + (m == dir ? pL : 0.0);
+        FR[1 + m] = qR[1 + m] * ur + (m == dir ? pR : 0.0);
+    }
+    FL[nv - 1] = (pL + qL[nv - 1]) * ul;
+    FR[nv - 1] = (pR + qR[nv - 1]) * ur;
+    const double fl = qL[0] * ((sL - ul) / (sL - ss)), fr = qR[0] * ((sR - ur) / (sR - ss));
+    SL[0] = fl;
+    SR[0] = fr;
+    for (int m = 0; m < nd; ++m) {
+        SL[1 + m] = fl * (m == dir ? ss : uL[m]);
+        SR[1 + m] = fr * (m == dir ? ss : uR[m]);
+    }
+    SL[nv - 1] = fl * std::fma(ss - ul, ss + pL / (qL[0] * (sL - ul)), qL[nv - 1] * irL);
+    SR[nv - 1] = fr * std::fma(ss - ur, ss + pR / (qR[0] * (sL - ur)), qR[nv - 1] * irR);
+    for (int v = 0; v < nv; ++v) {
+        double f;
+        if (lm) {
+            if (sL >= 0.0) f = FL[v];
+            else if (sR <= 0.0) f = FR[v];
+            else f = 0.5 * (FL[v] + FR[v]) + 0.5 * ((sL * (SL[v] - qL[v]) + std::fabs(ss) * (SL[v] - SR[v])) + sR * (SR[v] - qR[v]));
+        } else {
+            if (sL >= 0.0) f = FL[v];
+            else if (sL <= 0.0 && 0.0 <= ss) f = std::fma(sL, SL[v] - qL[v], FL[v]);
+            else if (ss <= 0.0 && 0.0 <= sR) f = std::fma(sR, SR[v] - qR[v], FR[v]);
+            else if (sR <= 0.0) f = FR[v];
+            else f = 0.0;
+        }
+        out[v] = f;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Characteristic flux reconstruction for one direction
 // (shock_capturing.py:786-891, 957-1028; weno.py:505-551; teno.py:510-559).
 // Writes the interface flux at i+1/2 into wk_dir[v] at index i over
@@ -470,6 +558,7 @@ void reconstruct_direction(osb_ctx* c, int dir, bool filter = false) {
     TenoParams tp{c->d.order, c->d.teno_eps, c->d.teno_ct};
     const double weps = c->d.weno_eps;
     const bool hybrid = filter && c->d.filter_sensor == OSB_SENSOR_DUCROS;
+    const bool hllc = !filter && (c->d.flux_type == OSB_FLUX_HLLC || c->d.flux_type == OSB_FLUX_HLLC_LM);
     for_range(r, [&](int i, int j, int k) {
         const size_t x0 = c->I(i, j, k);
         if (hybrid) {
@@ -519,6 +608,32 @@ void reconstruct_direction(osb_ctx* c, int dir, bool filter = false) {
         }
         Eig e;
         eigensystem(nd, dir, gamma_m1, rho_av, u_av, a_av, e);
+        if (hllc) {
+            // HLLCWeno: WENO-Z of the characteristic state CS = L q (no flux splitting); the
+            // right reconstruction (offsets -2..2) is the face's left state, the left one
+            // (-1..3) its right state (shock_capturing.py:1117-1135)
+            double wr[5], wl[5];
+            for (int kk = 0; kk < nv; ++kk) {
+                double cs[6];
+                for (int s = 0; s < 6; ++s) {
+                    double a = 0.0;
+                    for (int m = 0; m < nv; ++m) a = std::fma(e.L[kk][m], qv[s][m], a);
+                    cs[s] = a;
+                }
+                wr[kk] = wenoz_right(cs, weps);
+                wl[kk] = wenoz_left(cs, weps);
+            }
+            double qL[5], qR[5], fo[5];
+            for (int v = 0; v < nv; ++v) {
+                double sl = 0.0, sr = 0.0;
+                for (int kk = 0; kk < nv; ++kk) { sl = std::fma(e.Rm[v][kk], wr[kk], sl); sr = std::fma(e.Rm[v][kk], wl[kk], sr); }
+                qL[v] = sl;
+                qR[v] = sr;
+            }
+            hllc_flux(nd, dir, gama, c->d.flux_type == OSB_FLUX_HLLC_LM, qL, qR, u_av[dir], a_av, fo);
+            for (int v = 0; v < nv; ++v) wk[(size_t)v * c->Ptot + x0] = fo[v];
+            return;
+        }
         // local LF wave speeds: max over offsets {0, 1} of |eigenvalue| (shock_capturing.py:1000-1028)
         double lam[5];
         if (c->d.flux_type == OSB_FLUX_GLF) {
@@ -1123,7 +1238,9 @@ int validate(const osb_problem_desc* d, std::string& why) {
     if (d->viscous != OSB_VISC_NONE && d->ndim < 2) { why = "viscous terms need ndim >= 2"; return OSB_E_UNSUPPORTED; }
     if (d->time_scheme != OSB_TIME_RKLS3 && d->time_scheme != OSB_TIME_RK3) { why = "unknown time scheme"; return OSB_E_UNSUPPORTED; }
     if (d->precision != OSB_PREC_FP64 && d->precision != OSB_PREC_MIXED) { why = "unknown precision"; return OSB_E_UNSUPPORTED; }
-    if (d->flux_type != OSB_FLUX_LLF && d->flux_type != OSB_FLUX_GLF) { why = "unknown flux type"; return OSB_E_UNSUPPORTED; }
+    if (d->flux_type < OSB_FLUX_LLF || d->flux_type > OSB_FLUX_HLLC_LM) { why = "unknown flux type"; return OSB_E_UNSUPPORTED; }
+    if (d->flux_type >= OSB_FLUX_HLLC && cs != OSB_CONV_WENO_Z) { why = "HLLC is built for the WENO-Z scheme (HLLCWeno)"; return OSB_E_UNSUPPORTED; }
+    if (d->flux_type >= OSB_FLUX_HLLC && d->precision != OSB_PREC_FP64) { why = "HLLC runs in FP64 only"; return OSB_E_UNSUPPORTED; }
     if (d->shock_filter != OSB_FILTER_NONE && d->shock_filter != OSB_FILTER_WENOZ) { why = "unknown shock filter"; return OSB_E_UNSUPPORTED; }
     if (d->shock_filter == OSB_FILTER_WENOZ && d->precision != OSB_PREC_FP64) { why = "the WENO filter runs in FP64 only"; return OSB_E_UNSUPPORTED; }
     if (d->shock_filter == OSB_FILTER_WENOZ && d->flux_type != OSB_FLUX_LLF) { why = "the WENO filter runs with LLF splitting of the base scheme only"; return OSB_E_UNSUPPORTED; }
@@ -1518,6 +1635,9 @@ int osb_oracle_kappa(osb_ctx* c, double* out) {
             for (int i = 0; i < c->n[0]; ++i) out[o++] = c->kappa[c->I(i, j, k)];
     return OSB_OK;
 }
+
+// test hook: the HLLC-LM switch's sine
+double osb_oracle_hllc_sin(double x) { return hllc_sin(x); }
 
 // test hook: the Carpenter closure tables (4 x 6 first-derivative rows, then 2 x 5)
 int osb_oracle_carpenter(double* out) {

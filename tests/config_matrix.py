@@ -4,7 +4,8 @@ parity sweep (tests/test_gpu_matrix.py) and its CPU check.
 Each case is a small grid (8..20 points per direction) with a smooth random
 state (positive density and pressure, velocities below the sound speed), a
 scheme, boundary faces, time integrator, precision-free options and filter
-drawn from what both libraries accept."""
+drawn from what both libraries accept (including HLLC / HLLC-LM fluxes with
+WENO-Z)."""
 import math
 
 import numpy as np
@@ -90,6 +91,8 @@ def make_case(seed):
         desc.dt = 0.2 * min(h) / (1.0 + math.sqrt(gama))
     if not central and rng.random() < 0.3:
         desc.flux_type = nat.FLUX_GLF
+    elif conv == nat.CONV_WENO_Z and rng.random() < 0.35:
+        desc.flux_type = nat.FLUX_HLLC if rng.random() < 0.5 else nat.FLUX_HLLC_LM
     if desc.flux_type == nat.FLUX_LLF and rng.random() < 0.35:
         desc.shock_filter = nat.FILTER_WENOZ
         desc.filter_order = int(rng.choice([3, 5]))
@@ -98,4 +101,4 @@ def make_case(seed):
     return P.Problem(f"matrix_{seed}", desc, q.reshape(desc.shape), 3)
 
 
-CASES = list(range(20261201, 20261201 + 60))
+CASES = list(range(20261201, 20261201 + 72))

@@ -437,3 +437,18 @@ def test_out_of_memory_is_an_error_not_a_crash(hip):
 def test_long_horizon_bitwise(hip, oracle, maker):
     a, b = _compare(hip, oracle, maker())
     assert np.array_equal(a, b)
+
+
+# --- HLLC / HLLC-LM -----------------------------------------------------------------
+def _hllc(pr, flux):
+    d = nat.ProblemDesc(**{**pr.desc.__dict__, "flux_type": flux})
+    return P.Problem(pr.name + f"_hllc{flux}", d, pr.q0, pr.steps)
+
+
+@pytest.mark.parametrize("flux", [nat.FLUX_HLLC, nat.FLUX_HLLC_LM])
+@pytest.mark.parametrize("maker", [lambda: P.sod(n=400, steps=100), lambda: P.riemann_2d(n=48, steps=20),
+                                   lambda: P.kelvin_helmholtz(n=48, steps=20),
+                                   lambda: P.tgv(n=16, steps=5, scheme="weno", minf=1.25, cfl=0.5, dt=0.0)])
+def test_hllc_parity(hip, oracle, maker, flux):
+    a, b = _compare(hip, oracle, _hllc(maker(), flux))
+    assert np.array_equal(a, b)

@@ -885,3 +885,63 @@ def test_ducros_hybrid_thresholds(oracle):
     qh, _ = run(oracle, base)
     assert np.array_equal(qon, qp) and np.array_equal(qoff, qn)
     assert not np.array_equal(qh, qn) and not np.array_equal(qh, qp)      # the 0.05 selection is partial
+
+
+# --- HLLC / HLLC-LM (HLLCWeno, shock_capturing.py:1031-1256) --------------------------
+def test_hllc_sine_matches_libm(oracle):
+    """The HLLC-LM switch's own sine (Taylor series through x^21, the same
+    operations on CPU and GPU) against the C library on [0, pi/2]."""
+    L = oracle.lib
+    L.osb_oracle_hllc_sin.restype = C.c_double
+    L.osb_oracle_hllc_sin.argtypes = [C.c_double]
+    for x in np.linspace(0.0, math.pi / 2, 2001):
+        assert abs(L.osb_oracle_hllc_sin(float(x)) - math.sin(x)) <= 4.5e-16, x
+    assert L.osb_oracle_hllc_sin(0.0) == 0.0
+
+
+@pytest.mark.parametrize("flux", [nat.FLUX_HLLC, nat.FLUX_HLLC_LM])
+def test_hllc_sod_against_exact_solution(oracle, flux):
+    """apps/Sod_shock_tube/Sod_shock_tube.py's default method (WENO-Z with
+    HLLC-LM) and plain HLLC against the exact density at t = 0.2: less
+    dissipative than LLF on the same grid (the contact and shock are
+    sharper), converging with N; swapped states or a wrong star state make
+    it worse than LLF or unstable."""
+    ref = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "sod_exact_t0.2.txt"))
+    out = {}
+    for ft in (nat.FLUX_LLF, flux):
+        for n in (200, 1000):
+            pr = P.sod(n=n)
+            pr.desc.flux_type = ft
+            q, st = run(oracle, pr)
+            x = pr.desc.delta[0] * np.arange(n)
+            out[(ft, n)] = np.mean(np.abs(np.interp(ref[:, 0], x, q[0, 0, 0]) - ref[:, 1]))
+    assert out[(flux, 1000)] < 6e-4 and out[(flux, 200)] < 3e-3
+    assert out[(flux, 1000)] < out[(nat.FLUX_LLF, 1000)] and out[(flux, 200)] < out[(nat.FLUX_LLF, 200)]
+    assert out[(flux, 1000)] < 0.3 * out[(flux, 200)]
+
+
+@pytest.mark.parametrize("flux", [nat.FLUX_HLLC, nat.FLUX_HLLC_LM])
+def test_hllc_keeps_uniform_and_contact_states(oracle, flux):
+    """F(q, q) = F(q) (the star states reduce to q when s* = u), so a uniform
+    state stays put; a stationary contact (u = 0, uniform p) is an exact
+    HLLC solution (s* = 0, U*_K = U_K)."""
+    pr = P.entropy_wave(n=32, ndim=2, profile="sine", steps=4, filtered=False, scheme="weno")
+    pr.desc.flux_type = flux
+    q0 = pr.q0.copy()
+    q0[0] = 1.0
+    q0[1], q0[2] = 0.3, -0.2
+    q0[3] = 1.0 / 0.4 + 0.5 * (0.09 + 0.04)
+    q, _ = run(oracle, pr, state=q0)
+    assert np.abs(q - q0).max() <= 1e-14
+    st = P.entropy_wave(n=40, ndim=2, profile="step", steps=5, filtered=False, velocity=0.0, scheme="weno")
+    st.desc.flux_type = flux
+    q, _ = run(oracle, st)
+    assert np.abs(q - st.q0).max() <= 1e-14
+
+
+def test_hllc_validation(oracle):
+    pr = P.sod(n=32)
+    for kw in (dict(flux_type=nat.FLUX_HLLC, conv_scheme=nat.CONV_TENO), dict(flux_type=nat.FLUX_HLLC, precision=nat.PREC_MIXED),
+               dict(flux_type=nat.FLUX_HLLC_LM, shock_filter=nat.FILTER_WENOZ), dict(flux_type=9)):
+        with pytest.raises(nat.OSBError):
+            oracle.context(nat.ProblemDesc(**{**pr.desc.__dict__, **kw}))

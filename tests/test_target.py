@@ -441,3 +441,19 @@ def test_central4_weno5_plan():
     NS = NS_Split("KGP", 3, ["Re", "Pr", "gama", "Minf"], viscosity="dynamic", energy_formulation="enthalpy")
     desc = plan_problem(TraditionalAlgorithmRK(block), params, physics=NS)
     _same(desc, P.tgv_central4_weno5(n=16, dt=0.00084625 * 2.0).desc)
+
+
+def test_hllc_weno_plan(oracle):
+    """apps/Sod_shock_tube/Sod_shock_tube.py's scheme, HLLCWeno(order=5,
+    formulation='Z', averaging=SimpleAverage([0, 1]), flux_type='HLLC-LM'),
+    plans to FLUX_HLLC_LM with simple averaging; without an averaging HLLCWeno
+    uses RoeAverage."""
+    from opensbli_amd.surface import HLLCWeno
+    params = {"gama": "1.4", "dt": "0.0002", "niter": "5", "block0np0": "200", "Delta0block0": "1.0/(block0np0-1)"}
+    for avg, want_avg in ((SimpleAverage([0, 1]), nat.AVG_SIMPLE), (None, nat.AVG_ROE)):
+        alg = _block(1, [HLLCWeno(5, formulation="Z", averaging=avg, flux_type="HLLC-LM"), RungeKuttaLS(3)],
+                     [ZeroGradientOutletBC(0, 0), ZeroGradientOutletBC(0, 1)])
+        desc = plan_problem(alg, params)
+        assert desc.flux_type == nat.FLUX_HLLC_LM and desc.averaging == want_avg and desc.conv_scheme == nat.CONV_WENO_Z
+    with pytest.raises(ValueError):
+        HLLCWeno(5, flux_type="LLF")
