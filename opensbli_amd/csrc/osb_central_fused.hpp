@@ -58,29 +58,41 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_fused(const d
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
 
     // ---- phase 1: constituent relations on the halo box -------------------
-    for (int b = threadIdx.x; b < CF_BOX; b += CF_THREADS) {
+    // Unrolled over the box points of a thread; the loads are unconditional
+    // (clamped into the stored range) so the iterations can issue them
+    // together, and only the stores keep the skips.
+    constexpr int CF_ITERS = (CF_BOX + CF_THREADS - 1) / CF_THREADS;
+#pragma unroll
+    for (int it = 0; it < CF_ITERS; ++it) {
+        const int b = threadIdx.x + it * CF_THREADS;
+        if (CF_BOX % CF_THREADS != 0 && b >= CF_BOX) break;
         const int bx = b % CF_BX, by = (b / CF_BX) % CF_BY, bz = b / (CF_BX * CF_BY);
         const int i = i0 + bx - 2, j = j0 + by - 2, k = k0 + bz - 2;
         const int out = (bx < 2 || bx >= CF_TX + 2) + (by < 2 || by >= CF_TY + 2) + (bz < 2 || bz >= CF_TZ + 2);
-        if (out == 3) continue;                                            // box corners are never read
-        if (i >= n0 + 2 || j >= n1 + 2 || k >= n2 + 2) continue;           // beyond the stored halo
-        const long x = gidx(g, i, j, k);
+        // box corners are never read; points beyond the stored halo are not stored
+        const bool keep = out != 3 && i < n0 + 2 && j < n1 + 2 && k < n2 + 2;
+        const long x = gidx(g, min(i, n0 + 1), min(j, n1 + 1), min(k, n2 + 1));
         const double r = q[x];
+        const double rm[3] = {q[P + x], q[2 * P + x], q[3 * P + x]};
+        const double rhoE = q[4 * P + x];
         double ke = 0.0;
+        double uu[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            const double u = q[(long)(1 + d) * P + x] / r;
-            s_u[d][b] = u;
-            ke -= 0.5 * (u * u) * r;
+            uu[d] = rm[d] / r;
+            ke -= 0.5 * (uu[d] * uu[d]) * r;
         }
-        if (out <= 1) {
-            const double rhoE = q[4 * P + x];
-            const double p = c.gamma_m1 * (ke + rhoE);
-            const double TT = (c.minf * c.minf) * c.gama * p / r;
-            s_p[b] = p;
-            s_T[b] = TT;
-            s_E[b] = CONV == CF_KGP ? (rhoE + p) / r : rhoE / r;
-            if (VISC == 2) s_mu[VISC == 2 ? b : 0] = 1.4042 * TT * sqrt(TT) / (0.40417 + TT);
+        if (keep) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) s_u[d][b] = uu[d];
+            if (out <= 1) {
+                const double p = c.gamma_m1 * (ke + rhoE);
+                const double TT = (c.minf * c.minf) * c.gama * p / r;
+                s_p[b] = p;
+                s_T[b] = TT;
+                s_E[b] = CONV == CF_KGP ? (rhoE + p) / r : rhoE / r;
+                if (VISC == 2) s_mu[VISC == 2 ? b : 0] = 1.4042 * TT * sqrt(TT) / (0.40417 + TT);
+            }
         }
     }
     __syncthreads();
