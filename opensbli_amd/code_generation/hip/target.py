@@ -342,7 +342,8 @@ def plan_problem(alg_or_block, parameters: Mapping, physics=None, precision: str
     desc["bc"] = bc
     desc["bc_value"] = bc_value
 
-    for key, name in (("gama", "gama"), ("minf", "Minf"), ("re", "Re"), ("pr", "Pr")):
+    for key, name in (("gama", "gama"), ("minf", "Minf"), ("re", "Re"), ("pr", "Pr"),
+                      ("shock_filter_control", "shock_filter_control")):   # shock_capturing.py:1014-1016
         if name in p:
             desc[key] = float(p[name])
     if cfl is not None:

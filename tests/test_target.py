@@ -457,3 +457,13 @@ def test_hllc_weno_plan(oracle):
         assert desc.flux_type == nat.FLUX_HLLC_LM and desc.averaging == want_avg and desc.conv_scheme == nat.CONV_WENO_Z
     with pytest.raises(ValueError):
         HLLCWeno(5, flux_type="LLF")
+
+
+def test_shock_filter_control_parameter():
+    """The LLF wave-speed scale 'shock_filter_control' (declared with value 1.0,
+    shock_capturing.py:1014-1016) follows the script's parameter when given."""
+    alg = _block(1, [LFWeno(5, formulation="Z", averaging=SimpleAverage([0, 1])), RungeKuttaLS(3)],
+                 [ZeroGradientOutletBC(0, 0), ZeroGradientOutletBC(0, 1)])
+    params = {"gama": "1.4", "dt": "0.0002", "block0np0": "64", "Delta0block0": "1.0/(block0np0-1)"}
+    assert plan_problem(alg, params).shock_filter_control == 1.0
+    assert plan_problem(alg, {**params, "shock_filter_control": "1.5"}).shock_filter_control == 1.5
