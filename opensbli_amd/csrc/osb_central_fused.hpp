@@ -52,7 +52,16 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_fused(const d
     const int w = blockIdx.x + ntx * (blockIdx.y + nty * blockIdx.z);
     int tile = w;
     if (xcd_remap && (ntiles & 7) == 0) tile = (w & 7) * (ntiles >> 3) + (w >> 3);
-    const int tbx = tile % ntx, tby = (tile / ntx) % nty, tbz = tile / (ntx * nty);
+    int tbx = tile % ntx, tby = (tile / ntx) % nty, tbz = tile / (ntx * nty);
+    if (xcd_remap == 2 && (ntx & 7) == 0 && (nty & 3) == 0 && (gridDim.z & 1) == 0) {
+        // supertiles of 8 x 4 x 2 tiles (one XCD's 64 resident workgroups) walked in
+        // x, y, z order: a squarer resident region, so less of its halo leaves L2
+        const int in = tile & 63, sup = tile >> 6;
+        const int sx = ntx >> 3, sy = nty >> 2;
+        tbx = ((sup % sx) << 3) + (in & 7);
+        tby = (((sup / sx) % sy) << 2) + ((in >> 3) & 3);
+        tbz = ((sup / (sx * sy)) << 1) + (in >> 5);
+    }
     const int i0 = tbx * CF_TX, j0 = tby * CF_TY, k0 = tbz * CF_TZ;
     const long P = g.Ptot;
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];

@@ -753,7 +753,7 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     c->q_cur = c->qa;
     c->q_prev = c->qb;
     if (const char* e = std::getenv("OSB_FUSED_TILE")) c->fused_tile = std::max(0, std::min(CF_NTILES - 1, std::atoi(e)));
-    if (const char* e = std::getenv("OSB_XCD_REMAP")) c->xcd_remap = std::atoi(e) != 0;
+    if (const char* e = std::getenv("OSB_XCD_REMAP")) c->xcd_remap = std::max(0, std::min(2, std::atoi(e)));
     if (const char* e = std::getenv("OSB_CENTRAL_KERNEL")) c->march = std::strcmp(e, "march") == 0;
     if (const char* e = std::getenv("OSB_GRAPH")) c->use_graph = std::atoi(e) != 0;
     if (const char* e = std::getenv("OSB_FACE_KERNEL")) c->face_tiled = std::strcmp(e, "tile") == 0;

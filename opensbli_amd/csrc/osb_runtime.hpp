@@ -55,7 +55,8 @@ struct osb_ctx {
     int rk_stages = 3;              // RK-LS stages and coefficients (rk_LS.py:97-132)
     double rkA[5] = {0}, rkB[5] = {0};
     int fused_tile = 0;             // index into CF_TILES (autotuned)
-    int xcd_remap = 1;              // XCD-aware workgroup-to-tile order (OSB_XCD_REMAP=0 turns it off)
+    int xcd_remap = 2;              // XCD-aware workgroup-to-tile order: 2 = 8x4x2 supertiles per XCD range,
+                                    // 1 = x-fastest ranges, 0 = off (OSB_XCD_REMAP)
     bool march = false;             // 3-D central stage: 3-D tiles, or the z-march kernel (OSB_CENTRAL_KERNEL=march)
     int march_kz = 16;              // planes per z-march workgroup
     int march_tile = 0;             // 0: 16x16 columns, 1: 32x8
