@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(BLOCK) k_ducros(const double* __restrict__ u, 
                                                   double sd) {
     const int e[3] = {g.n[0], g.n[1], g.n[2]};
     const int lo[3] = {0, 0, 0};
-    const long t = gtid();
+    const long t = xcd_gtid();
     if (t >= (long)e[0] * e[1] * e[2]) return;
     int i, j, k;
     unflatten(t, e, lo, i, j, k);
@@ -185,7 +185,7 @@ __global__ void __launch_bounds__(BLOCK) k_filter_update(const double* __restric
     constexpr int NV = NDIM + 2;
     const int e[3] = {g.n[0], g.n[1], g.n[2]};
     const int lo[3] = {0, 0, 0};
-    const long t = gtid();
+    const long t = xcd_gtid();
     if (t >= (long)e[0] * e[1] * e[2]) return;
     int i, j, k;
     unflatten(t, e, lo, i, j, k);
