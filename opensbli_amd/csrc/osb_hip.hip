@@ -243,6 +243,10 @@ void rhs(osb_ctx* c, const double* q) {
 void apply_bcs(osb_ctx* c, double* q) {
     Timed tm(c, PF_BC);
     const Geom& g = c->g;
+    if (c->bc_periodic) {
+        k_bc_periodic<<<nblocks(bc_periodic_work(g)), BLOCK, 0, c->stream>>>(q, g, c->nv);
+        return;
+    }
     for (int dir = 0; dir < g.ndim; ++dir) {
         long work = 2L * (g.halo + 1);
         for (int d = 0; d < g.ndim; ++d)
@@ -767,6 +771,11 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
         c->face_tiled = false;
         c->shock_fused = false;
     }
+    c->bc_periodic = true;
+    for (int d = 0; d < c->g.ndim; ++d)
+        if (desc->bc[d][0] != OSB_BC_PERIODIC || desc->bc[d][1] != OSB_BC_PERIODIC || c->g.n[d] < c->g.halo) c->bc_periodic = false;
+    if (const char* e = std::getenv("OSB_BC_KERNEL"))
+        if (std::strcmp(e, "dir") == 0) c->bc_periodic = false;
     if (const char* e = std::getenv("OSB_MARCH_KZ")) c->march_kz = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("OSB_MARCH_TILE")) c->march_tile = std::atoi(e);
     *out = c;

@@ -953,6 +953,51 @@ __global__ void __launch_bounds__(BLOCK) k_bc(double* __restrict__ q, Geom g, in
     bc_item<DIR>(gtid(), q, g, nv, b);
 }
 
+// Halo points of the halo-extended box when every face is periodic, in one
+// pass: region d holds the points with directions > d interior, direction d
+// in its 2h halo layers and directions < d over the extended range, so the
+// regions partition the halo. Each point copies the interior point at its
+// index wrapped in every direction. With n[d] >= h in every direction that
+// is the value the direction-by-direction fill (k_bc<0>, k_bc<1>, k_bc<2>,
+// each over the extended range of the other directions) leaves at every halo
+// point, corners included; sources are interior points only, so no thread
+// reads what another writes.
+__host__ __device__ inline long bc_periodic_work(const Geom& g) {
+    long ext = 1, in = 1;
+    for (int d = 0; d < g.ndim; ++d) {
+        ext *= g.n[d] + 2 * g.halo;
+        in *= g.n[d];
+    }
+    return ext - in;
+}
+
+static __global__ void __launch_bounds__(BLOCK) k_bc_periodic(double* __restrict__ q, Geom g, int nv) {
+    long t = gtid();
+    const int h = g.halo;
+    for (int d = 0; d < g.ndim; ++d) {
+        int e[3], lo[3];
+        long cnt = 1;
+        for (int a = 0; a < 3; ++a) {
+            if (a >= g.ndim) { e[a] = 1; lo[a] = 0; }
+            else if (a < d) { e[a] = g.n[a] + 2 * h; lo[a] = -h; }
+            else if (a == d) { e[a] = 2 * h; lo[a] = 0; }
+            else { e[a] = g.n[a]; lo[a] = 0; }
+            cnt *= e[a];
+        }
+        if (t < cnt) {
+            int c[3];
+            unflatten(t, e, lo, c[0], c[1], c[2]);
+            c[d] = c[d] < h ? c[d] - h : g.n[d] + (c[d] - h);
+            int s[3] = {c[0], c[1], c[2]};
+            for (int a = 0; a < g.ndim; ++a) s[a] = s[a] < 0 ? s[a] + g.n[a] : (s[a] >= g.n[a] ? s[a] - g.n[a] : s[a]);
+            const long xd = gidx(g, c[0], c[1], c[2]), xs = gidx(g, s[0], s[1], s[2]);
+            for (int v = 0; v < nv; ++v) q[(long)v * g.Ptot + xd] = q[(long)v * g.Ptot + xs];
+            return;
+        }
+        t -= cnt;
+    }
+}
+
 // host interior layout (v, k, j, i) <-> padded device layout
 static __global__ void __launch_bounds__(BLOCK) k_scatter(const double* __restrict__ src, double* __restrict__ q, Geom g, int nv) {
     const long npts = (long)g.n[0] * g.n[1] * g.n[2];

@@ -64,6 +64,7 @@ struct osb_ctx {
     bool face_tiled = false;        // face fluxes one thread per face from L2/HBM; OSB_FACE_KERNEL=tile: LDS-tiled variant
     int fused_fw = 32;              // lines per fused tile: 32 or 16 (OSB_FUSED_FW)
     bool shock_fused = true;        // shock-only stages: last direction's faces fused with the update (OSB_SHOCK_FUSED=0: separate)
+    bool bc_periodic = false;       // every face periodic and n >= halo: one-pass halo fill (OSB_BC_KERNEL=dir: per direction)
 
     // captured time steps: graph p starts from buffer qa (p = 0) or qb (p = 1)
     bool use_graph = true;

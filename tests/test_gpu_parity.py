@@ -406,6 +406,38 @@ def test_weno3_ducros_filter_parity(hip, oracle, maker):
 
 
 # --- error paths that stay on the host side of every kernel ---------------------------
+@pytest.mark.parametrize("maker", [
+    lambda: P.tgv(n=12, steps=1),
+    lambda: P.tgv(n=8, scheme="teno", steps=1),
+    lambda: P.entropy_wave(n=16, ndim=1, steps=1),
+    lambda: P.entropy_wave(n=16, ndim=2, steps=1),
+], ids=["tgv3d_h2", "tgv3d_h4", "wave1d_h4", "wave2d_h4"])
+def test_periodic_halo_fill_one_pass(hip, maker, monkeypatch):
+    """Every face periodic: the one-pass halo fill leaves each halo point,
+    corners included, equal to the interior point at its index wrapped in
+    every direction (numpy's wrap padding), and bit-equal to the
+    direction-by-direction fill (OSB_BC_KERNEL=dir)."""
+    pr = maker()
+    nd = pr.desc.ndim
+    assert all(pr.desc.bc[d][0] == nat.BC_PERIODIC and pr.desc.bc[d][1] == nat.BC_PERIODIC for d in range(nd))
+    _announce(f"periodic halo fill {pr.name} n={list(pr.desc.n)}")
+    out = {}
+    for mode in ("one_pass", "dir"):
+        monkeypatch.setenv("OSB_BC_KERNEL", mode)
+        with hip.context(pr.desc) as c:
+            shp = c.padded_shape()
+            h = int(c.stats().halo)
+            qp = np.random.default_rng(7).standard_normal(shp)
+            c.set_state(qp, nat.LAYOUT_PADDED)
+            c.apply_bcs()
+            out[mode] = c.get_state(nat.LAYOUT_PADDED).reshape(shp)
+    pads = [(0, 0)] + [(h, h) if 3 - a - 1 < nd else (0, 0) for a in range(3)]
+    inner = tuple([slice(None)] + [slice(h, -h) if 3 - a - 1 < nd else slice(None) for a in range(3)])
+    want = np.pad(qp[inner], pads, mode="wrap")
+    assert np.array_equal(out["one_pass"], want)
+    assert np.array_equal(out["dir"], want)
+
+
 def test_out_of_memory_is_an_error_not_a_crash(hip):
     """A grid far beyond HBM fails context creation with OSB_E_NOMEM (every
     partial allocation released); the library keeps working afterwards, and
