@@ -32,7 +32,7 @@ template <int CF_TX, int CF_TY, int CF_TZ, int CONV, int VISC, bool FLUXRES, int
 __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_fused(const double* __restrict__ q, const double* __restrict__ F,
                                                               double* __restrict__ reg, double* __restrict__ qout,
                                                               const double* __restrict__ scal, Geom g, Consts c,
-                                                              double A, double B, int xcd_remap) {
+                                                              double A, double B, int xcd_remap, int wrap) {
     constexpr int NV = 5;
     constexpr int CF_BX = CF_TX + 4, CF_BY = CF_TY + 4, CF_BZ = CF_TZ + 4;
     constexpr int CF_BOX = CF_BX * CF_BY * CF_BZ;
@@ -284,13 +284,15 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_fused(const d
     }
 
     const double dt = scal[1];
+    double qv[NV];
     if (TIME == 1) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const long y = (long)v * P + x;
             const double d = fma(A, reg[y], dt * Rv[v]);
             reg[y] = d;
-            qout[y] = fma(B, d, q[y]);
+            qv[v] = fma(B, d, q[y]);
+            qout[y] = qv[v];
         }
     } else {
         // classic RK3: A = rknew, B = rkold; reg holds q_old
@@ -298,8 +300,28 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_fused(const d
         for (int v = 0; v < NV; ++v) {
             const long y = (long)v * P + x;
             const double o = reg[y];
-            qout[y] = fma(dt * A, Rv[v], o);
+            qv[v] = fma(dt * A, Rv[v], o);
+            qout[y] = qv[v];
             reg[y] = fma(dt * B, Rv[v], o);
+        }
+    }
+    // Every face periodic (n >= 2h in every direction): the point's images in
+    // the halo of qout, i.e. the halo points whose index wraps to this point
+    // (i + n where i < h, i - n where i >= n - h, per direction, and their
+    // combinations at edges and corners). Each halo point has exactly one
+    // such source, so this writes what the halo fill would copy there.
+    if (wrap) {
+        const int h = g.halo;
+        const int oi = i < h ? n0 : (i >= n0 - h ? -n0 : 0);
+        const int oj = j < h ? n1 : (j >= n1 - h ? -n1 : 0);
+        const int ok = k < h ? n2 : (k >= n2 - h ? -n2 : 0);
+        if (oi | oj | ok) {
+            for (int m = 1; m < 8; ++m) {
+                if (((m & 1) && !oi) || ((m & 2) && !oj) || ((m & 4) && !ok)) continue;
+                const long xh = gidx(g, i + ((m & 1) ? oi : 0), j + ((m & 2) ? oj : 0), k + ((m & 4) ? ok : 0));
+#pragma unroll
+                for (int v = 0; v < NV; ++v) qout[(long)v * P + xh] = qv[v];
+            }
         }
     }
 }

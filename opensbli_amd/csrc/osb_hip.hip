@@ -275,7 +275,7 @@ void launch_fused_tile(osb_ctx* c, const double* q, double* qn, double A, double
     const Geom& g = c->g;
     dim3 grid((g.n[0] + TX - 1) / TX, (g.n[1] + TY - 1) / TY, (g.n[2] + TZ - 1) / TZ);
     k_central_fused<TX, TY, TZ, CONV, VISC, FLUXRES, TIME><<<grid, TX * TY * TZ, 0, c->stream>>>(
-        q, c->F, c->du, qn, c->scal, g, c->c, A, B, c->xcd_remap);
+        q, c->F, c->du, qn, c->scal, g, c->c, A, B, c->xcd_remap, (int)(c->bc_periodic && c->bc_in_stage));
 }
 
 template <int CONV, int VISC, bool FLUXRES, int TIME>
@@ -341,8 +341,10 @@ void launch_mixed_v(osb_ctx* c, const double* q, double* qn, double A, double B)
 
 // One RK stage through the fused kernel: 3-D central schemes, and 3-D
 // shock-capturing schemes with viscous terms (face fluxes first).
+// Returns whether the stage kernel also wrote qn's halos (the tile kernel with
+// every face periodic), in which case the stage needs no halo fill.
 template <int TIME>
-void fused_stage(osb_ctx* c, const double* q, double* qn, double A, double B) {
+bool fused_stage(osb_ctx* c, const double* q, double* qn, double A, double B) {
     if (c->shock) {
         launch_glf(c, q);
         launch_flux<3, 0>(c, q);
@@ -355,21 +357,23 @@ void fused_stage(osb_ctx* c, const double* q, double* qn, double A, double B) {
         if (c->shock) launch_mixed_v<CF_NONE, true, TIME>(c, q, qn, A, B);
         else if (c->d.conv_scheme == OSB_CONV_CENTRAL_KGP) launch_mixed_v<CF_KGP, false, TIME>(c, q, qn, A, B);
         else launch_mixed_v<CF_FEIEREISEN, false, TIME>(c, q, qn, A, B);
-        return;
+        return false;
     }
+    const bool wrapped = c->bc_periodic && c->bc_in_stage;
     if (c->d.conv_scheme == OSB_CONV_CENTRAL_KGP) {           // tile kernel only
         launch_fused_v<CF_KGP, false, TIME>(c, q, qn, A, B);
-        return;
+        return wrapped;
     }
     if (c->march) {
         if (c->shock) launch_march_v<CF_NONE, true, TIME>(c, q, qn, A, B);
         else if (c->d.conv_scheme == OSB_CONV_CENTRAL_SKEW) launch_march_v<CF_SKEW, false, TIME>(c, q, qn, A, B);
         else launch_march_v<CF_FEIEREISEN, false, TIME>(c, q, qn, A, B);
-        return;
+        return false;
     }
     if (c->shock) launch_fused_v<CF_NONE, true, TIME>(c, q, qn, A, B);
     else if (c->d.conv_scheme == OSB_CONV_CENTRAL_SKEW) launch_fused_v<CF_SKEW, false, TIME>(c, q, qn, A, B);
     else launch_fused_v<CF_FEIEREISEN, false, TIME>(c, q, qn, A, B);
+    return wrapped;
 }
 
 // the fused 3-D stage kernel implements the Feiereisen, skew and KGP forms
@@ -444,9 +448,10 @@ void do_stage(osb_ctx* c, int s) {
         k_dt_finalize<<<1, 64, 0, c->stream>>>(c->scal, c->d.cfl);
     }
     double* qn = (c->q_cur == c->qa) ? c->qb : c->qa;
+    bool halos_done = false;
     if (c->d.time_scheme == OSB_TIME_RKLS3) {
         if (use_fused(c)) {
-            fused_stage<1>(c, c->q_cur, qn, c->rkA[s], c->rkB[s]);
+            halos_done = fused_stage<1>(c, c->q_cur, qn, c->rkA[s], c->rkB[s]);
         } else if (use_shock_stage(c)) {
             shock_stage<1>(c, c->q_cur, qn, c->rkA[s], c->rkB[s]);
         } else {
@@ -459,7 +464,7 @@ void do_stage(osb_ctx* c, int s) {
         const double rknew[3] = {2.0 / 3.0, 5.0 / 12.0, 3.0 / 5.0};
         if (s == 0) k_copy_interior<<<nblocks(nin), BLOCK, 0, c->stream>>>(c->q_cur, c->du, g, c->nv);
         if (use_fused(c)) {
-            fused_stage<2>(c, c->q_cur, qn, rknew[s], rkold[s]);
+            halos_done = fused_stage<2>(c, c->q_cur, qn, rknew[s], rkold[s]);
         } else if (use_shock_stage(c)) {
             shock_stage<2>(c, c->q_cur, qn, rknew[s], rkold[s]);
         } else {
@@ -468,7 +473,7 @@ void do_stage(osb_ctx* c, int s) {
             k_rk3_stage<<<nblocks(nin), BLOCK, 0, c->stream>>>(qn, c->du, c->R, c->scal, g, c->nv, rknew[s], rkold[s]);
         }
     }
-    apply_bcs(c, qn);
+    if (!halos_done) apply_bcs(c, qn);
     c->q_prev = c->q_cur;
     c->q_cur = qn;
     const int nst = c->d.time_scheme == OSB_TIME_RKLS3 ? c->rk_stages : 3;
@@ -774,8 +779,12 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     c->bc_periodic = true;
     for (int d = 0; d < c->g.ndim; ++d)
         if (desc->bc[d][0] != OSB_BC_PERIODIC || desc->bc[d][1] != OSB_BC_PERIODIC || c->g.n[d] < c->g.halo) c->bc_periodic = false;
-    if (const char* e = std::getenv("OSB_BC_KERNEL"))
+    for (int d = 0; d < c->g.ndim; ++d)
+        if (c->g.n[d] < 2 * c->g.halo) c->bc_in_stage = false;
+    if (const char* e = std::getenv("OSB_BC_KERNEL")) {
         if (std::strcmp(e, "dir") == 0) c->bc_periodic = false;
+        if (std::strcmp(e, "pass") == 0) c->bc_in_stage = false;
+    }
     if (const char* e = std::getenv("OSB_MARCH_KZ")) c->march_kz = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("OSB_MARCH_TILE")) c->march_tile = std::atoi(e);
     *out = c;

@@ -65,6 +65,8 @@ struct osb_ctx {
     int fused_fw = 32;              // lines per fused tile: 32 or 16 (OSB_FUSED_FW)
     bool shock_fused = true;        // shock-only stages: last direction's faces fused with the update (OSB_SHOCK_FUSED=0: separate)
     bool bc_periodic = false;       // every face periodic and n >= halo: one-pass halo fill (OSB_BC_KERNEL=dir: per direction)
+    bool bc_in_stage = true;        // with bc_periodic, the 3-D tile stage kernel writes the halos itself (n >= 2 halo;
+                                    // OSB_BC_KERNEL=pass: the separate one-pass fill)
 
     // captured time steps: graph p starts from buffer qa (p = 0) or qb (p = 1)
     bool use_graph = true;

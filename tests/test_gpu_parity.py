@@ -438,6 +438,35 @@ def test_periodic_halo_fill_one_pass(hip, maker, monkeypatch):
     assert np.array_equal(out["dir"], want)
 
 
+@pytest.mark.parametrize("maker", [
+    lambda: P.tgv(n=16, steps=2),
+    lambda: P.tgv(n=16, steps=2, scheme="kgp"),
+    lambda: P.tgv(n=16, steps=2, scheme="teno", minf=1.25),
+], ids=["feiereisen", "kgp", "teno_viscous"])
+def test_stage_kernel_halo_images(hip, maker, monkeypatch):
+    """Every face periodic: the 3-D tile stage kernel writes each updated
+    point's wrapped halo images itself, and the padded state after every
+    stage (halos included) equals the one with the separate one-pass fill
+    (OSB_BC_KERNEL=pass) and the per-direction fill (OSB_BC_KERNEL=dir)."""
+    pr = maker()
+    _announce(f"stage-kernel halo images {pr.name} n={list(pr.desc.n)}")
+    out = {}
+    for mode in ("in_stage", "pass", "dir"):
+        monkeypatch.setenv("OSB_BC_KERNEL", mode)
+        with hip.context(pr.desc) as c:
+            c.set_state(pr.q0)
+            snaps = []
+            for _ in range(pr.steps):
+                for s in range(3):
+                    c.stage(s)
+                    snaps.append(c.get_state(nat.LAYOUT_PADDED))
+            c.step(2)
+            snaps.append(c.get_state(nat.LAYOUT_PADDED))
+            out[mode] = snaps
+    for a, b, d in zip(out["in_stage"], out["pass"], out["dir"]):
+        assert np.array_equal(a, b) and np.array_equal(a, d)
+
+
 def test_out_of_memory_is_an_error_not_a_crash(hip):
     """A grid far beyond HBM fails context creation with OSB_E_NOMEM (every
     partial allocation released); the library keeps working afterwards, and
