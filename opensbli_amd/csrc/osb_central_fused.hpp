@@ -305,25 +305,8 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_fused(const d
             reg[y] = fma(dt * B, Rv[v], o);
         }
     }
-    // Every face periodic (n >= 2h in every direction): the point's images in
-    // the halo of qout, i.e. the halo points whose index wraps to this point
-    // (i + n where i < h, i - n where i >= n - h, per direction, and their
-    // combinations at edges and corners). Each halo point has exactly one
-    // such source, so this writes what the halo fill would copy there.
-    if (wrap) {
-        const int h = g.halo;
-        const int oi = i < h ? n0 : (i >= n0 - h ? -n0 : 0);
-        const int oj = j < h ? n1 : (j >= n1 - h ? -n1 : 0);
-        const int ok = k < h ? n2 : (k >= n2 - h ? -n2 : 0);
-        if (oi | oj | ok) {
-            for (int m = 1; m < 8; ++m) {
-                if (((m & 1) && !oi) || ((m & 2) && !oj) || ((m & 4) && !ok)) continue;
-                const long xh = gidx(g, i + ((m & 1) ? oi : 0), j + ((m & 2) ? oj : 0), k + ((m & 4) ? ok : 0));
-#pragma unroll
-                for (int v = 0; v < NV; ++v) qout[(long)v * P + xh] = qv[v];
-            }
-        }
-    }
+    // every face periodic: the point's wrapped images in the halo of qout
+    if (wrap) store_halo_images<NV>(qout, g, i, j, k, qv);
 }
 
 }  // namespace osb

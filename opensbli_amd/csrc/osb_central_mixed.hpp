@@ -32,7 +32,7 @@ template <int CF_TX, int CF_TY, int CF_TZ, int CONV, int VISC, bool FLUXRES, int
 __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const double* __restrict__ q, const double* __restrict__ F,
                                                               double* __restrict__ reg, double* __restrict__ qout,
                                                               const double* __restrict__ scal, Geom g, Consts c,
-                                                              double A, double B, int xcd_remap) {
+                                                              double A, double B, int xcd_remap, int wrap) {
     static_assert(CONV == CF_FEIEREISEN || CONV == CF_KGP || CONV == CF_NONE, "mixed central kernel: Feiereisen, KGP or viscous-only");
     constexpr bool KGP = CONV == CF_KGP;
     constexpr int NV = 5;
@@ -300,6 +300,7 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
     }
 
     const double dt = scal[1];
+    double qv[NV];
     if (TIME == 1) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -307,7 +308,8 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
             const double R = Rd[v] + double(Rv[v]);
             const double d = fma(A, reg[y], dt * R);
             reg[y] = d;
-            qout[y] = fma(B, d, q[y]);
+            qv[v] = fma(B, d, q[y]);
+            qout[y] = qv[v];
         }
     } else {
 #pragma unroll
@@ -315,10 +317,13 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
             const long y = (long)v * P + x;
             const double R = Rd[v] + double(Rv[v]);
             const double o = reg[y];
-            qout[y] = fma(dt * A, R, o);
+            qv[v] = fma(dt * A, R, o);
+            qout[y] = qv[v];
             reg[y] = fma(dt * B, R, o);
         }
     }
+    // every face periodic: the point's wrapped images in the halo of qout
+    if (wrap) store_halo_images<NV>(qout, g, i, j, k, qv);
 }
 
 }  // namespace osb

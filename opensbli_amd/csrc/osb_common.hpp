@@ -704,4 +704,26 @@ __device__ __forceinline__ void face_flux(const double* __restrict__ q, long Pto
     face_core<NDIM, DIR, SCH, T, decltype(pt)&, FILT>(pt, c, out);
 }
 
+// Every face periodic (n >= 2h in every direction): store the values qv of
+// interior point (i, j, k) at its images in the halo of q, i.e. the halo
+// points whose index wraps to it (i + n where i < h, i - n where i >= n - h,
+// per direction, and their combinations at edges and corners). Each halo
+// point has exactly one such source, so the stores leave what the halo fill
+// would copy there.
+template <int NV>
+__device__ __forceinline__ void store_halo_images(double* __restrict__ q, const Geom& g, int i, int j, int k,
+                                                  const double (&qv)[NV]) {
+    const int h = g.halo;
+    const int oi = i < h ? g.n[0] : (i >= g.n[0] - h ? -g.n[0] : 0);
+    const int oj = j < h ? g.n[1] : (j >= g.n[1] - h ? -g.n[1] : 0);
+    const int ok = k < h ? g.n[2] : (k >= g.n[2] - h ? -g.n[2] : 0);
+    if (!(oi | oj | ok)) return;
+    for (int m = 1; m < 8; ++m) {
+        if (((m & 1) && !oi) || ((m & 2) && !oj) || ((m & 4) && !ok)) continue;
+        const long xh = gidx(g, i + ((m & 1) ? oi : 0), j + ((m & 2) ? oj : 0), k + ((m & 4) ? ok : 0));
+#pragma unroll
+        for (int v = 0; v < NV; ++v) q[(long)v * g.Ptot + xh] = qv[v];
+    }
+}
+
 }  // namespace osb

@@ -327,7 +327,7 @@ void launch_mixed_t(osb_ctx* c, const double* q, double* qn, double A, double B)
     const Geom& g = c->g;
     dim3 grid((g.n[0] + 15) / 16, (g.n[1] + 3) / 4, (g.n[2] + 3) / 4);
     k_central_mixed<16, 4, 4, CONV, VISC, FLUXRES, TIME><<<grid, 256, 0, c->stream>>>(
-        q, c->F, c->du, qn, c->scal, g, c->c, A, B, c->xcd_remap);
+        q, c->F, c->du, qn, c->scal, g, c->c, A, B, c->xcd_remap, (int)(c->bc_periodic && c->bc_in_stage));
 }
 
 template <int CONV, bool FLUXRES, int TIME>
@@ -341,8 +341,9 @@ void launch_mixed_v(osb_ctx* c, const double* q, double* qn, double A, double B)
 
 // One RK stage through the fused kernel: 3-D central schemes, and 3-D
 // shock-capturing schemes with viscous terms (face fluxes first).
-// Returns whether the stage kernel also wrote qn's halos (the tile kernel with
-// every face periodic), in which case the stage needs no halo fill.
+// Returns whether the stage kernel also wrote qn's halos (the tile kernels,
+// FP64 and mixed, with every face periodic), in which case the stage needs no
+// halo fill.
 template <int TIME>
 bool fused_stage(osb_ctx* c, const double* q, double* qn, double A, double B) {
     if (c->shock) {
@@ -352,14 +353,14 @@ bool fused_stage(osb_ctx* c, const double* q, double* qn, double A, double B) {
         launch_flux<3, 2>(c, q);
     }
     Timed tm(c, PF_CENTRAL_FUSED);
+    const bool wrapped = c->bc_periodic && c->bc_in_stage;
     if (c->mixed && c->d.conv_scheme != OSB_CONV_CENTRAL_SKEW) {
         // FP32 derivatives on deviations (the skew form stays FP64)
         if (c->shock) launch_mixed_v<CF_NONE, true, TIME>(c, q, qn, A, B);
         else if (c->d.conv_scheme == OSB_CONV_CENTRAL_KGP) launch_mixed_v<CF_KGP, false, TIME>(c, q, qn, A, B);
         else launch_mixed_v<CF_FEIEREISEN, false, TIME>(c, q, qn, A, B);
-        return false;
+        return wrapped;
     }
-    const bool wrapped = c->bc_periodic && c->bc_in_stage;
     if (c->d.conv_scheme == OSB_CONV_CENTRAL_KGP) {           // tile kernel only
         launch_fused_v<CF_KGP, false, TIME>(c, q, qn, A, B);
         return wrapped;

@@ -442,10 +442,12 @@ def test_periodic_halo_fill_one_pass(hip, maker, monkeypatch):
     lambda: P.tgv(n=16, steps=2),
     lambda: P.tgv(n=16, steps=2, scheme="kgp"),
     lambda: P.tgv(n=16, steps=2, scheme="teno", minf=1.25),
-], ids=["feiereisen", "kgp", "teno_viscous"])
+    lambda: P.tgv(n=16, steps=2, precision=nat.PREC_MIXED),
+    lambda: P.tgv(n=16, steps=2, scheme="kgp", precision=nat.PREC_MIXED),
+], ids=["feiereisen", "kgp", "teno_viscous", "mixed_feiereisen", "mixed_kgp"])
 def test_stage_kernel_halo_images(hip, maker, monkeypatch):
-    """Every face periodic: the 3-D tile stage kernel writes each updated
-    point's wrapped halo images itself, and the padded state after every
+    """Every face periodic: the 3-D tile stage kernels (FP64 and mixed) write
+    each updated point's wrapped halo images themselves, and the padded state after every
     stage (halos included) equals the one with the separate one-pass fill
     (OSB_BC_KERNEL=pass) and the per-direction fill (OSB_BC_KERNEL=dir)."""
     pr = maker()
