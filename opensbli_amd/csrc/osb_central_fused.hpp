@@ -1,52 +1,118 @@
 // osb_central_fused.hpp -- one-launch RK stage for the central schemes on gfx950.
 //
-// One workgroup owns a 16 x 4 x 4 tile of grid points.  Phase 1 stages the
-// constituent relations of the tile plus a 2-point halo box in LDS:
-//   u0,u1,u2                at tile, faces and edges of the box (the edges feed
-//                           the mixed second derivatives D_b(D_a u))
-//   p, T, E=rhoE/rho, mu    at tile and faces (single-direction stencils);
-//                           the KGP form stores H=(rhoE+p)/rho in place of E
-// each evaluated once per box point.  Phase 2 gives every point its full
-// right-hand side -- split-form convective terms, viscous stresses and heat
-// conduction, or the interface-flux difference of a shock-capturing scheme --
-// followed directly by the RK update, so the stage reads q, du (and the face
-// fluxes) and writes du and q once.  The expression sequence matches the CPU
-// oracle (StoreSome.py:138-228, split_forms.py:111-144, 383-392).
+// One workgroup owns a TX x TY x TZ tile of grid points (16 x 4 x 4 by
+// default).  Phase 1 evaluates the constituent relations once per point of
+// the tile plus a 2-point halo box and stages them in LDS:
+//   u0, u1, u2, theta = p/rho, rho        (mu as well for Sutherland's law)
+// five FP64 fields = 51.2 KB at the default tile.  Every other pointwise
+// quantity the stencils difference is rebuilt at the stencil point from
+// these (rho u_m, p = rho theta, E = rhoE/rho = theta/(gama-1) + |u|^2/2,
+// rhoE = rho E, H = (rhoE+p)/rho = gama theta/(gama-1) + |u|^2/2,
+// T = gama Minf^2 theta), so phase 2 reads no stencil from HBM or L2.  Each
+// derivative is still the central difference of the pointwise field (the
+// same discrete operator as StoreSome.py:138-228 and split_forms.py:111-144,
+// 178-234, 383-392), evaluated with one reciprocal of rho per box point and
+// with stencil weights pre-scaled by 1/Delta.  The result rounds differently
+// from the CPU oracle's sequence of divisions; it agrees with it to the FP64
+// parity bar (relative L2 / L-inf <= 1e-11).
+//
+// Phase 2 gives every tile point its full right-hand side -- split-form
+// convective terms, viscous stresses and heat conduction, or the
+// interface-flux difference of a shock-capturing scheme -- followed directly
+// by the RK update, so the stage reads q, du (and the face fluxes) and writes
+// du and q once.
 #pragma once
 
 #include "osb_kernels.hpp"
 
 namespace osb {
 
-// tile shapes offered to the autotuner: {TX, TY, TZ}
+// tile shapes offered to the tile selection: {TX, TY, TZ}
 struct CfTile { int tx, ty, tz; };
-constexpr CfTile CF_TILES[] = {{16, 4, 4}, {8, 8, 4}, {32, 4, 2}, {16, 8, 4}, {32, 4, 4}};
-constexpr int CF_NTILES = 5;
+constexpr CfTile CF_TILES[] = {{16, 4, 4}, {8, 8, 4}, {32, 4, 2}};
+constexpr int CF_NTILES = 3;
 
 enum CfConv { CF_NONE = 0, CF_FEIEREISEN = 1, CF_SKEW = 2, CF_KGP = 3 };
+
+// Central differences with the pre-scaled weights of Consts (w1, w2, wm).
+// f(s) is the field at offset s along the direction.
+template <class A>
+__device__ __forceinline__ double fd1(A f, const double (&w)[2]) {
+    return fma(w[0], f(1) - f(-1), w[1] * (f(-2) - f(2)));
+}
+template <class A>
+__device__ __forceinline__ double fd2(A f, const double (&w)[3]) {
+    return fma(w[2], f(-2) + f(2), fma(w[1], f(-1) + f(1), w[0] * f(0)));
+}
+// D_b(D_a f) with f(r, s) the field at offset r along a and s along b: the
+// product of the two antisymmetric 4th-order first-derivative stencils,
+// grouped by |r|, |s| (wm = w_a(1) w_b(1), w_a(2) w_b(1), w_a(1) w_b(2), w_a(2) w_b(2))
+template <class A>
+__device__ __forceinline__ double fd11(A f, const double (&wm)[4]) {
+    const double x11 = (f(1, 1) - f(-1, 1)) - (f(1, -1) - f(-1, -1));
+    const double x21 = (f(2, 1) - f(-2, 1)) - (f(2, -1) - f(-2, -1));
+    const double x12 = (f(1, 2) - f(-1, 2)) - (f(1, -2) - f(-1, -2));
+    const double x22 = (f(2, 2) - f(-2, 2)) - (f(2, -2) - f(-2, -2));
+    return fma(wm[0], x11, fma(wm[1], x21, fma(wm[2], x12, wm[3] * x22)));
+}
+
+// Host side: the pre-scaled weights and constants the fused kernels read.
+// D1 f = (2/3)(f(1) - f(-1))/Delta + (1/12)(f(-2) - f(2))/Delta
+// D2 f = (-5/2 f(0) + 4/3 (f(-1) + f(1)) - 1/12 (f(-2) + f(2)))/Delta^2
+inline void fill_fused_consts(Consts& k) {
+    for (int d = 0; d < 3; ++d) {
+        k.w1[d][0] = (2.0 / 3.0) * k.invd[d];
+        k.w1[d][1] = (1.0 / 12.0) * k.invd[d];
+        k.w2[d][0] = (-5.0 / 2.0) * k.inv2d[d];
+        k.w2[d][1] = (4.0 / 3.0) * k.inv2d[d];
+        k.w2[d][2] = (-1.0 / 12.0) * k.inv2d[d];
+    }
+    const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+    for (int p = 0; p < 3; ++p) {
+        // stencil weights at offsets +1, +2: w(1) = w1[0], w(2) = -w1[1]
+        const double a1 = k.w1[pa[p]][0], a2 = -k.w1[pa[p]][1];
+        const double b1 = k.w1[pb[p]][0], b2 = -k.w1[pb[p]][1];
+        k.wm[p][0] = a1 * b1;
+        k.wm[p][1] = a2 * b1;
+        k.wm[p][2] = a1 * b2;
+        k.wm[p][3] = a2 * b2;
+    }
+    k.cE = 1.0 / k.gamma_m1;
+    k.cH = k.gama / k.gamma_m1;
+    k.cT = (k.minf * k.minf) * k.gama;
+    k.cheat_th = k.cheat * k.cT;
+}
+
+// waves per SIMD the fused kernel is compiled for (its VGPR budget: 2 -> 256)
+#ifndef OSB_CF_WAVES
+#define OSB_CF_WAVES 2
+#endif
+
+// index of the (a, b) pair in Consts::wm, a < b
+__host__ __device__ constexpr int cf_pair(int a, int b) { return a == 0 ? b - 1 : 2; }
 
 // CONV: CfConv; VISC: 0 none, 1 constant, 2 Sutherland; FLUXRES: start R from
 // the face-flux difference (shock-capturing schemes); TIME: OSB_TIME_RKLS3 (1)
 // or OSB_TIME_RK3 (2).
 template <int CF_TX, int CF_TY, int CF_TZ, int CONV, int VISC, bool FLUXRES, int TIME>
-__global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_fused(const double* __restrict__ q, const double* __restrict__ F,
-                                                              double* __restrict__ reg, double* __restrict__ qout,
-                                                              const double* __restrict__ scal, Geom g, Consts c,
-                                                              double A, double B, int xcd_remap, int wrap) {
+__global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ, OSB_CF_WAVES)
+k_central_fused(const double* __restrict__ q, const double* __restrict__ F, double* __restrict__ reg, double* __restrict__ qout,
+                const double* __restrict__ scal, Geom g, Consts c, double A, double B, int xcd_remap, int wrap) {
     constexpr int NV = 5;
     constexpr int CF_BX = CF_TX + 4, CF_BY = CF_TY + 4, CF_BZ = CF_TZ + 4;
     constexpr int CF_BOX = CF_BX * CF_BY * CF_BZ;
     constexpr int CF_THREADS = CF_TX * CF_TY * CF_TZ;
-    __shared__ double s_u[3][CF_BOX];
-    __shared__ double s_p[CF_BOX];
-    __shared__ double s_T[CF_BOX];
-    __shared__ double s_E[CF_BOX];
-    __shared__ double s_mu[VISC == 2 ? CF_BOX : 1];
+    constexpr bool HAS_CONV = CONV != CF_NONE;
+    constexpr bool SUTH = VISC == 2;
+    // LDS fields: u0, u1, u2, theta, [rho], [mu], one flat array
+    constexpr int F_TH = 3, F_RHO = 4, F_MU = HAS_CONV ? 5 : 4;
+    constexpr int NFIELD = 4 + (HAS_CONV ? 1 : 0) + (SUTH ? 1 : 0);
+    __shared__ double s_f[NFIELD * CF_BOX];
 
     // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs, so
     // workgroup w runs on XCD w % 8.  Giving XCD x the contiguous range of
-    // tiles [x*T/8, (x+1)*T/8) (x fastest, then y, then z) keeps neighbouring
-    // tiles -- which share halo points -- in the same L2.
+    // tiles [x*T/8, (x+1)*T/8) keeps neighbouring tiles -- which share halo
+    // points -- in the same L2.
     const int ntx = gridDim.x, nty = gridDim.y;
     const int ntiles = ntx * nty * gridDim.z;
     const int w = blockIdx.x + ntx * (blockIdx.y + nty * blockIdx.z);
@@ -54,7 +120,7 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_fused(const d
     if (xcd_remap && (ntiles & 7) == 0) tile = (w & 7) * (ntiles >> 3) + (w >> 3);
     int tbx = tile % ntx, tby = (tile / ntx) % nty, tbz = tile / (ntx * nty);
     if (xcd_remap == 2 && (ntx & 7) == 0 && (nty & 3) == 0 && (gridDim.z & 1) == 0) {
-        // supertiles of 8 x 4 x 2 tiles (one XCD's 64 resident workgroups) walked in
+        // supertiles of 8 x 4 x 2 tiles (one XCD's resident workgroups) walked in
         // x, y, z order: a squarer resident region, so less of its halo leaves L2
         const int in = tile & 63, sup = tile >> 6;
         const int sx = ntx >> 3, sy = nty >> 2;
@@ -67,52 +133,65 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_fused(const d
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
 
     // ---- phase 1: constituent relations on the halo box -------------------
-    // Unrolled over the box points of a thread; the loads are unconditional
-    // (clamped into the stored range) so the iterations can issue them
-    // together, and only the stores keep the skips.
+    // Every box point is evaluated and stored (the corners are never read);
+    // loads are clamped into the stored range, so past the high end of a
+    // partial tile they repeat the last halo layer, which only feeds points
+    // the tile does not update.
+    // Point offsets are 32-bit: a block's points per variable stay below 2^32
+    // (checked when the context is created), and each variable has its own
+    // base pointer.
     constexpr int CF_ITERS = (CF_BOX + CF_THREADS - 1) / CF_THREADS;
+    const double* __restrict__ q1 = q + P;
+    const double* __restrict__ q2 = q + 2 * P;
+    const double* __restrict__ q3 = q + 3 * P;
+    const double* __restrict__ q4 = q + 4 * P;
+    const unsigned p0 = (unsigned)g.P[0], p01 = (unsigned)g.P[0] * (unsigned)g.P[1];
+    // box point (0,0,0) is interior point (i0-2, j0-2, k0-2): padded index o - 2
+    const int ib = i0 + g.o[0] - 2, jb = j0 + g.o[1] - 2, kb = k0 + g.o[2] - 2;
+    const int imax = n0 + 1 + g.o[0], jmax = n1 + 1 + g.o[1], kmax = n2 + 1 + g.o[2];
 #pragma unroll
     for (int it = 0; it < CF_ITERS; ++it) {
         const int b = threadIdx.x + it * CF_THREADS;
         if (CF_BOX % CF_THREADS != 0 && b >= CF_BOX) break;
         const int bx = b % CF_BX, by = (b / CF_BX) % CF_BY, bz = b / (CF_BX * CF_BY);
-        const int i = i0 + bx - 2, j = j0 + by - 2, k = k0 + bz - 2;
-        const int out = (bx < 2 || bx >= CF_TX + 2) + (by < 2 || by >= CF_TY + 2) + (bz < 2 || bz >= CF_TZ + 2);
-        // box corners are never read; points beyond the stored halo are not stored
-        const bool keep = out != 3 && i < n0 + 2 && j < n1 + 2 && k < n2 + 2;
-        const long x = gidx(g, min(i, n0 + 1), min(j, n1 + 1), min(k, n2 + 1));
+        const unsigned x = (unsigned)min(ib + bx, imax) + p0 * (unsigned)min(jb + by, jmax) + p01 * (unsigned)min(kb + bz, kmax);
         const double r = q[x];
-        const double rm[3] = {q[P + x], q[2 * P + x], q[3 * P + x]};
-        const double rhoE = q[4 * P + x];
-        double ke = 0.0;
-        double uu[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            uu[d] = rm[d] / r;
-            ke -= 0.5 * (uu[d] * uu[d]) * r;
-        }
-        if (keep) {
-#pragma unroll
-            for (int d = 0; d < 3; ++d) s_u[d][b] = uu[d];
-            if (out <= 1) {
-                const double p = c.gamma_m1 * (ke + rhoE);
-                const double TT = (c.minf * c.minf) * c.gama * p / r;
-                s_p[b] = p;
-                s_T[b] = TT;
-                s_E[b] = CONV == CF_KGP ? (rhoE + p) / r : rhoE / r;
-                if (VISC == 2) s_mu[VISC == 2 ? b : 0] = 1.4042 * TT * sqrt(TT) / (0.40417 + TT);
-            }
+        const double rm0 = q1[x], rm1 = q2[x], rm2 = q3[x];
+        const double rhoE = q4[x];
+        const double ri = rcp_nr(r);
+        const double u0 = rm0 * ri, u1 = rm1 * ri, u2 = rm2 * ri;
+        const double p = c.gamma_m1 * fma(-0.5, fma(rm0, u0, fma(rm1, u1, rm2 * u2)), rhoE);
+        const double th = p * ri;
+        s_f[b] = u0;
+        s_f[CF_BOX + b] = u1;
+        s_f[2 * CF_BOX + b] = u2;
+        s_f[F_TH * CF_BOX + b] = th;
+        if (HAS_CONV) s_f[F_RHO * CF_BOX + b] = r;
+        if (SUTH) {
+            const double TT = c.cT * th;
+            s_f[F_MU * CF_BOX + b] = 1.4042 * TT * sqrt(TT) * rcp_nr(0.40417 + TT);
         }
     }
     __syncthreads();
 
     // ---- phase 2: right-hand side and RK update at one point --------------
+    // Every stencil value comes from LDS: rho u_m = rho u_m, rhoE = rho E and
+    // p = rho theta are rebuilt at the stencil point.  The reads share one base
+    // address (the box point at offset -2, -2, -2 from the thread's point), so
+    // each is that base plus a constant.
     const int tx = threadIdx.x % CF_TX, ty = (threadIdx.x / CF_TX) % CF_TY, tz = threadIdx.x / (CF_TX * CF_TY);
     const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
     if (i >= n0 || j >= n1 || k >= n2) return;
     const long x = gidx(g, i, j, k);
-    const int l = (tx + 2) + CF_BX * ((ty + 2) + CF_BY * (tz + 2));
-    const int ls[3] = {1, CF_BX, CF_BX * CF_BY};
+    constexpr int ls[3] = {1, CF_BX, CF_BX * CF_BY};
+    constexpr int ORG = 2 * (ls[0] + ls[1] + ls[2]);
+    const double* __restrict__ sb = s_f + (tx + CF_BX * (ty + CF_BY * tz));     // box index of the point, less ORG
+    auto S = [&](int f, int off) { return sb[f * CF_BOX + ORG + off]; };
+    // E or H at a stencil point from theta and the velocities
+    auto EH = [&](double coef, int off) {
+        const double a = S(0, off), b = S(1, off), d = S(2, off);
+        return fma(coef, S(F_TH, off), 0.5 * fma(a, a, fma(b, b, d * d)));
+    };
 
     double Rv[NV];
     if (FLUXRES) {
@@ -122,11 +201,14 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_fused(const d
             double acc = 0.0;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                const double* w = F + d * nvP + (long)v * P;
-                acc -= (w[x] - w[x - g.s[d]]) * c.invd[d];
+                const double* wf = F + d * nvP + (long)v * P;
+                acc -= (wf[x] - wf[x - g.s[d]]) * c.invd[d];
             }
             Rv[v] = acc;
         }
+    } else {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) Rv[v] = 0.0;
     }
 
     // stored-gradient values G[m][jd] = D_jd u_m at the point
@@ -135,89 +217,87 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_fused(const d
     for (int m = 0; m < 3; ++m)
 #pragma unroll
         for (int jd = 0; jd < 3; ++jd)
-            G[m][jd] = d1([&](int s) { return s_u[m][l + s * ls[jd]]; }, c.invd[jd]);
+            G[m][jd] = fd1([&](int s) { return S(m, s * ls[jd]); }, c.w1[jd]);
 
-    if (CONV != CF_NONE) {
-        const double rho = q[x], rhoE = q[4 * P + x];
+    if (HAS_CONV) {
+        constexpr int FR = HAS_CONV ? F_RHO : 0;
+        const double rho = S(FR, 0);
         double div_rhou = 0.0, Rrho = 0.0, eterm = 0.0;
         double conv[3] = {0.0, 0.0, 0.0};
 #pragma unroll
         for (int jd = 0; jd < 3; ++jd) {
-            const long st = g.s[jd];
             const int lj = ls[jd];
-            const double inv = c.invd[jd];
-            const double* rhou_j = q + (long)(1 + jd) * P;
-            auto qat = [&](const double* a, int s) { return a[x + s * st]; };
-            auto uj = [&](int s) { return s_u[jd][l + s * lj]; };
+            const double(&w)[2] = c.w1[jd];
+            auto rs = [&](int s) { return S(FR, s * lj); };
+            auto uj = [&](int s) { return S(jd, s * lj); };
+            auto ruj = [&](int s) { return rs(s) * uj(s); };
+            auto um_ = [&](int m, int s) { return S(m, s * lj); };
             if (CONV == CF_FEIEREISEN) {
-                const double d_rhouj = d1([&](int s) { return qat(rhou_j, s); }, inv);
+                auto Es = [&](int s) { return EH(c.cE, s * lj); };
+                const double d_rhouj = fd1(ruj, w);
                 div_rhou += d_rhouj;
 #pragma unroll
                 for (int m = 0; m < 3; ++m) {
-                    const double* rhou_m = q + (long)(1 + m) * P;
-                    const double d_prod = d1([&](int s) { return qat(rhou_m, s) * uj(s); }, inv);
-                    conv[m] += d_prod + qat(rhou_j, 0) * G[m][jd];
+                    const double d_prod = fd1([&](int s) { return ruj(s) * um_(m, s); }, w);
+                    conv[m] += d_prod + ruj(0) * G[m][jd];
                 }
-                const double d_Eu = d1([&](int s) { return qat(q + 4 * P, s) * uj(s); }, inv);
-                const double d_E = d1([&](int s) { return s_E[l + s * lj]; }, inv);
-                const double d_pu = d1([&](int s) { return s_p[l + s * lj] * uj(s); }, inv);
-                eterm += -0.5 * (d_Eu + qat(rhou_j, 0) * d_E) - d_pu;
+                const double d_Eu = fd1([&](int s) { return ruj(s) * Es(s); }, w);
+                const double d_E = fd1(Es, w);
+                const double d_pu = fd1([&](int s) { return ruj(s) * S(F_TH, s * lj); }, w);
+                eterm += -0.5 * (d_Eu + ruj(0) * d_E) - d_pu;
             } else if (CONV == CF_KGP) {
-                // KGP, enthalpy splitting, all coefficients 1/4 (split_forms.py:178-234): the term
-                // sequence of the CPU oracle, with u and H = (rhoE + p)/rho from LDS
-                const double ujc = uj(0), rhoujc = qat(rhou_j, 0);
+                // KGP, enthalpy splitting, all coefficients 1/4 (split_forms.py:178-234)
+                const double ujc = uj(0), rhoujc = ruj(0);
                 const double gjj = G[jd][jd];
-                auto Hs = [&](int s) { return s_E[l + s * lj]; };
-                const double d_rhouj = d1([&](int s) { return qat(rhou_j, s); }, inv);
-                const double d_rho = d1([&](int s) { return qat(q, s); }, inv);
+                auto Hs = [&](int s) { return EH(c.cH, s * lj); };
+                const double d_rhouj = fd1(ruj, w);
+                const double d_rho = fd1(rs, w);
                 Rrho += 0.5 * d_rhouj + 0.5 * (ujc * d_rho + rho * gjj);
 #pragma unroll
                 for (int m = 0; m < 3; ++m) {
-                    const double* rhou_m = q + (long)(1 + m) * P;
-                    auto um = [&](int s) { return s_u[m][l + s * lj]; };
-                    const double umc = um(0), rhoumc = qat(rhou_m, 0);
-                    const double d_rhouj_um = d1([&](int s) { return um(s) * qat(rhou_j, s); }, inv);
-                    const double d_rhoum = d1([&](int s) { return qat(rhou_m, s); }, inv);
-                    const double d_ujum = d1([&](int s) { return uj(s) * um(s); }, inv);
+                    const double umc = um_(m, 0), rhoumc = rho * umc;
+                    const double d_rhouj_um = fd1([&](int s) { return um_(m, s) * ruj(s); }, w);
+                    const double d_rhoum = fd1([&](int s) { return rs(s) * um_(m, s); }, w);
+                    const double d_ujum = fd1([&](int s) { return uj(s) * um_(m, s); }, w);
                     conv[m] += 0.25 * ((((((d_rhouj_um + umc * d_rhouj) + rhoujc * G[m][jd]) + ujc * d_rhoum) + rhoumc * gjj) +
                                         rho * d_ujum) + (umc * ujc) * d_rho);
                 }
                 const double Hc = Hs(0);
-                const double d_Hrhouj = d1([&](int s) { return Hs(s) * qat(rhou_j, s); }, inv);
-                const double d_H = d1(Hs, inv);
-                const double d_Hrho = d1([&](int s) { return Hs(s) * qat(q, s); }, inv);
-                const double d_Huj = d1([&](int s) { return Hs(s) * uj(s); }, inv);
+                const double d_Hrhouj = fd1([&](int s) { return Hs(s) * ruj(s); }, w);
+                const double d_H = fd1(Hs, w);
+                const double d_Hrho = fd1([&](int s) { return Hs(s) * rs(s); }, w);
+                const double d_Huj = fd1([&](int s) { return Hs(s) * uj(s); }, w);
                 eterm += 0.25 * ((((((d_Hrhouj + Hc * d_rhouj) + rhoujc * d_H) + ujc * d_Hrho) + (rho * Hc) * gjj) + rho * d_Huj) +
                                  (ujc * Hc) * d_rho);
             } else {
+                // skew-symmetric form (parsing.py:76-112)
+                auto rEs = [&](int s) { return rs(s) * EH(c.cE, s * lj); };
                 const double ujc = uj(0);
-                const double d_rhou = d1([&](int s) { return qat(q, s) * uj(s); }, inv);
-                const double d_rho = d1([&](int s) { return qat(q, s); }, inv);
-                const double d_u = d1([&](int s) { return uj(s); }, inv);
+                const double d_rhou = fd1(ruj, w);
+                const double d_rho = fd1(rs, w);
+                const double d_u = G[jd][jd];
                 Rrho -= 0.5 * (d_rhou + ujc * d_rho + rho * d_u);
 #pragma unroll
                 for (int m = 0; m < 3; ++m) {
-                    const double* rhou_m = q + (long)(1 + m) * P;
-                    const double d_prod = d1([&](int s) { return qat(rhou_m, s) * uj(s); }, inv);
-                    const double d_rm = d1([&](int s) { return qat(rhou_m, s); }, inv);
-                    conv[m] += 0.5 * (d_prod + ujc * d_rm + qat(rhou_m, 0) * d_u);
+                    const double d_prod = fd1([&](int s) { return ruj(s) * um_(m, s); }, w);
+                    const double d_rm = fd1([&](int s) { return rs(s) * um_(m, s); }, w);
+                    conv[m] += 0.5 * (d_prod + ujc * d_rm + (rho * um_(m, 0)) * d_u);
                 }
-                const double d_Eu = d1([&](int s) { return qat(q + 4 * P, s) * uj(s); }, inv);
-                const double d_rE = d1([&](int s) { return qat(q + 4 * P, s); }, inv);
-                const double d_pu = d1([&](int s) { return s_p[l + s * lj] * uj(s); }, inv);
-                eterm += -0.5 * (d_Eu + ujc * d_rE + rhoE * d_u) - d_pu;
+                const double d_Eu = fd1([&](int s) { return rEs(s) * uj(s); }, w);
+                const double d_rE = fd1(rEs, w);
+                const double d_pu = fd1([&](int s) { return ruj(s) * S(F_TH, s * lj); }, w);
+                eterm += -0.5 * (d_Eu + ujc * d_rE + rEs(0) * d_u) - d_pu;
             }
         }
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
-            const int lm = ls[m];
-            const double dp = d1([&](int s) { return s_p[l + s * lm]; }, c.invd[m]);
-            if (CONV == CF_FEIEREISEN) Rv[1 + m] = -dp - 0.5 * (conv[m] + s_u[m][l] * div_rhou);
+            const double dp = fd1([&](int s) { return S(FR, s * ls[m]) * S(F_TH, s * ls[m]); }, c.w1[m]);
+            if (CONV == CF_FEIEREISEN) Rv[1 + m] = -dp - 0.5 * (conv[m] + S(m, 0) * div_rhou);
             else Rv[1 + m] = -dp - conv[m];
         }
         if (CONV == CF_FEIEREISEN) {
             Rv[0] = -div_rhou;
-            Rv[4] = eterm - 0.5 * (rhoE / rho) * div_rhou;
+            Rv[4] = eterm - 0.5 * EH(c.cE, 0) * div_rhou;
         } else if (CONV == CF_KGP) {
             Rv[0] = -Rrho;
             Rv[4] = -eterm;
@@ -228,59 +308,53 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_fused(const d
     }
 
     if (VISC != 0) {
-        constexpr bool SUTH = VISC == 2;
-        double D2u[3][3], dmu[3], dT[3];
-        double lapT = 0.0;
-#pragma unroll
-        for (int jd = 0; jd < 3; ++jd) {
-            const int lj = ls[jd];
-            dmu[jd] = SUTH ? d1([&](int s) { return s_mu[SUTH ? l + s * lj : 0]; }, c.invd[jd]) : 0.0;
-            dT[jd] = d1([&](int s) { return s_T[l + s * lj]; }, c.invd[jd]);
-            lapT += d2([&](int s) { return s_T[l + s * lj]; }, c.inv2d[jd]);
-#pragma unroll
-            for (int m = 0; m < 3; ++m) D2u[m][jd] = d2([&](int s) { return s_u[m][l + s * lj]; }, c.inv2d[jd]);
-        }
+        // momentum: mu (lap u_m + 1/3 D_m div u) (+ S_mj D_j mu), energy:
+        // u_m mom_m + mu S_mj G_mj, heat: mu lap T (+ D_j T D_j mu); T = cT theta
+        const double muv = SUTH ? S(SUTH ? F_MU : 0, 0) : 1.0;
         double div = 0.0;
 #pragma unroll
         for (int m = 0; m < 3; ++m) div += G[m][m];
-        const double muv = SUTH ? s_mu[SUTH ? l : 0] : 1.0;
-        double Eacc = 0.0;
+        double heat = 0.0, Eacc = 0.0;
+        double dmu[3];
 #pragma unroll
-        for (int m = 0; m < 3; ++m) {
-            double Lm = (4.0 / 3.0) * D2u[m][m];
-#pragma unroll
-            for (int jd = 0; jd < 3; ++jd) {
-                if (jd == m) continue;
-                const int a = jd < m ? jd : m, b = jd < m ? m : jd;
-                // D_b(D_a u_jd): the inner derivative at the four points along b
-                const int la = ls[a], lb = ls[b];
-                auto ga = [&](int s) {
-                    const int lc = l + s * lb;
-                    return d1([&](int r) { return s_u[jd][lc + r * la]; }, c.invd[a]);
-                };
-                const double mixed = d1(ga, c.invd[b]);
-                Lm += D2u[m][jd] + (1.0 / 3.0) * mixed;
-            }
-            double S[3];
-#pragma unroll
-            for (int jd = 0; jd < 3; ++jd) S[jd] = G[m][jd] + G[jd][m] - (m == jd ? (2.0 / 3.0) * div : 0.0);
-            double mom = muv * Lm;
-            if (SUTH) {
-#pragma unroll
-                for (int jd = 0; jd < 3; ++jd) mom += S[jd] * dmu[jd];
-            }
-            Rv[1 + m] += c.invRe * mom;
-            double e = s_u[m][l] * mom;
-#pragma unroll
-            for (int jd = 0; jd < 3; ++jd) e += muv * S[jd] * G[m][jd];
-            Eacc += e;
+        for (int jd = 0; jd < 3; ++jd) {
+            const int lj = ls[jd];
+            heat += fd2([&](int s) { return S(F_TH, s * lj); }, c.w2[jd]);
+            dmu[jd] = SUTH ? fd1([&](int s) { return S(SUTH ? F_MU : 0, s * lj); }, c.w1[jd]) : 0.0;
         }
-        double heat = muv * lapT;
+        heat *= muv;
         if (SUTH) {
 #pragma unroll
-            for (int jd = 0; jd < 3; ++jd) heat += dT[jd] * dmu[jd];
+            for (int jd = 0; jd < 3; ++jd) heat += fd1([&](int s) { return S(F_TH, s * ls[jd]); }, c.w1[jd]) * dmu[jd];
         }
-        Rv[4] += c.invRe * Eacc + c.cheat * heat;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            double Lm = 0.0;
+#pragma unroll
+            for (int jd = 0; jd < 3; ++jd) {
+                const double d2 = fd2([&](int s) { return S(m, s * ls[jd]); }, c.w2[jd]);
+                if (jd == m) {
+                    Lm += (4.0 / 3.0) * d2;
+                } else {
+                    // D_b(D_a u_jd), a < b the pair {jd, m}
+                    const int a = jd < m ? jd : m, b = jd < m ? m : jd;
+                    const int la = ls[a], lb = ls[b];
+                    const double mixed = fd11([&](int r, int s) { return S(jd, r * la + s * lb); }, c.wm[cf_pair(a, b)]);
+                    Lm += d2 + (1.0 / 3.0) * mixed;
+                }
+            }
+            double mom = muv * Lm;
+            double e = 0.0;
+#pragma unroll
+            for (int jd = 0; jd < 3; ++jd) {
+                const double Smj = G[m][jd] + G[jd][m] - (m == jd ? (2.0 / 3.0) * div : 0.0);
+                if (SUTH) mom += Smj * dmu[jd];
+                e += Smj * G[m][jd];
+            }
+            Rv[1 + m] += c.invRe * mom;
+            Eacc += fma(S(m, 0), mom, muv * e);
+        }
+        Rv[4] += c.invRe * Eacc + c.cheat_th * heat;
     }
 
     const double dt = scal[1];

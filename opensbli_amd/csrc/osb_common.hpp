@@ -49,6 +49,13 @@ struct Consts {
     int carp[3];
     // WENO filter: Ducros thresholds of the hybrid selection
     double ducros_check, ducros_select;
+    // fused 3-D stage kernels (fill_fused_consts): central weights pre-scaled
+    // by 1/Delta per direction (w1: first, w2: second derivative), products of
+    // first-derivative weights for the mixed derivatives of the pairs (0,1),
+    // (0,2), (1,2), and the factors of theta = p/rho: E = cE theta + |u|^2/2,
+    // H = cH theta + |u|^2/2, T = cT theta, heat coefficient on theta
+    double w1[3][2], w2[3][3], wm[3][4];
+    double cE, cH, cT, cheat_th;
 };
 
 // ---------------------------------------------------------------------------

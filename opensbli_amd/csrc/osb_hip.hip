@@ -11,7 +11,6 @@
 // context's stream; nothing synchronises except the calls that return data.
 
 #include "osb_runtime.hpp"
-#include "osb_central_march.hpp"
 #include "osb_central_mixed.hpp"
 
 #include <algorithm>
@@ -283,8 +282,6 @@ void launch_fused_t(osb_ctx* c, const double* q, double* qn, double A, double B)
     switch (c->fused_tile) {
         case 1: launch_fused_tile<8, 8, 4, CONV, VISC, FLUXRES, TIME>(c, q, qn, A, B); break;
         case 2: launch_fused_tile<32, 4, 2, CONV, VISC, FLUXRES, TIME>(c, q, qn, A, B); break;
-        case 3: launch_fused_tile<16, 8, 4, CONV, VISC, FLUXRES, TIME>(c, q, qn, A, B); break;
-        case 4: launch_fused_tile<32, 4, 4, CONV, VISC, FLUXRES, TIME>(c, q, qn, A, B); break;
         default: launch_fused_tile<16, 4, 4, CONV, VISC, FLUXRES, TIME>(c, q, qn, A, B); break;
     }
 }
@@ -295,30 +292,6 @@ void launch_fused_v(osb_ctx* c, const double* q, double* qn, double A, double B)
         case OSB_VISC_NONE: if (!FLUXRES) launch_fused_t<CONV, 0, FLUXRES, TIME>(c, q, qn, A, B); break;
         case OSB_VISC_CONSTANT: launch_fused_t<CONV, 1, FLUXRES, TIME>(c, q, qn, A, B); break;
         default: launch_fused_t<CONV, 2, FLUXRES, TIME>(c, q, qn, A, B); break;
-    }
-}
-
-template <int TX, int TY, int CONV, int VISC, bool FLUXRES, int TIME>
-void launch_march_tile(osb_ctx* c, const double* q, double* qn, double A, double B) {
-    const Geom& g = c->g;
-    const int kz = c->march_kz;
-    dim3 grid((g.n[0] + TX - 1) / TX, (g.n[1] + TY - 1) / TY, (g.n[2] + kz - 1) / kz);
-    k_central_march<TX, TY, CONV, VISC, FLUXRES, TIME><<<grid, TX * TY, 0, c->stream>>>(
-        q, c->F, c->du, qn, c->scal, g, c->c, A, B, kz);
-}
-
-template <int CONV, int VISC, bool FLUXRES, int TIME>
-void launch_march_t(osb_ctx* c, const double* q, double* qn, double A, double B) {
-    if (c->march_tile == 1) launch_march_tile<32, 8, CONV, VISC, FLUXRES, TIME>(c, q, qn, A, B);
-    else launch_march_tile<16, 16, CONV, VISC, FLUXRES, TIME>(c, q, qn, A, B);
-}
-
-template <int CONV, bool FLUXRES, int TIME>
-void launch_march_v(osb_ctx* c, const double* q, double* qn, double A, double B) {
-    switch (c->d.viscous) {
-        case OSB_VISC_NONE: if (!FLUXRES) launch_march_t<CONV, 0, FLUXRES, TIME>(c, q, qn, A, B); break;
-        case OSB_VISC_CONSTANT: launch_march_t<CONV, 1, FLUXRES, TIME>(c, q, qn, A, B); break;
-        default: launch_march_t<CONV, 2, FLUXRES, TIME>(c, q, qn, A, B); break;
     }
 }
 
@@ -364,12 +337,6 @@ bool fused_stage(osb_ctx* c, const double* q, double* qn, double A, double B) {
     if (c->d.conv_scheme == OSB_CONV_CENTRAL_KGP) {           // tile kernel only
         launch_fused_v<CF_KGP, false, TIME>(c, q, qn, A, B);
         return wrapped;
-    }
-    if (c->march) {
-        if (c->shock) launch_march_v<CF_NONE, true, TIME>(c, q, qn, A, B);
-        else if (c->d.conv_scheme == OSB_CONV_CENTRAL_SKEW) launch_march_v<CF_SKEW, false, TIME>(c, q, qn, A, B);
-        else launch_march_v<CF_FEIEREISEN, false, TIME>(c, q, qn, A, B);
-        return false;
     }
     if (c->shock) launch_fused_v<CF_NONE, true, TIME>(c, q, qn, A, B);
     else if (c->d.conv_scheme == OSB_CONV_CENTRAL_SKEW) launch_fused_v<CF_SKEW, false, TIME>(c, q, qn, A, B);
@@ -496,6 +463,12 @@ int validate(const osb_problem_desc* d, std::string& why) {
     for (int i = 0; i < 3; ++i) {
         if (i < d->ndim && (d->n[i] < 8 || !(d->delta[i] > 0))) { why = "n >= 8 and delta > 0 required in active directions"; return OSB_E_INVALID; }
         if (i >= d->ndim && d->n[i] != 1) { why = "inactive directions must have n == 1"; return OSB_E_INVALID; }
+    }
+    {
+        // points per variable, halo included (at most 4 deep), must fit the kernels' 32-bit point offsets
+        double pts = 1.0;
+        for (int i = 0; i < d->ndim; ++i) pts *= (double)d->n[i] + 8.0;
+        if (pts >= 4294967296.0) { why = "more than 2^32 points per variable"; return OSB_E_UNSUPPORTED; }
     }
     const int cs = d->conv_scheme;
     const bool central_cs = cs == OSB_CONV_CENTRAL_FEIEREISEN || cs == OSB_CONV_CENTRAL_SKEW || cs == OSB_CONV_CENTRAL_KGP;
@@ -723,6 +696,7 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     k.glf = nullptr;
     k.ducros_check = desc->ducros_check;
     k.ducros_select = desc->ducros_select;
+    fill_fused_consts(k);
     for (int i = 0; i < 3; ++i) {
         k.carp[i] = 0;
         for (int sd = 0; sd < 2; ++sd)
@@ -764,7 +738,6 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     c->q_prev = c->qb;
     if (const char* e = std::getenv("OSB_FUSED_TILE")) c->fused_tile = std::max(0, std::min(CF_NTILES - 1, std::atoi(e)));
     if (const char* e = std::getenv("OSB_XCD_REMAP")) c->xcd_remap = std::max(0, std::min(2, std::atoi(e)));
-    if (const char* e = std::getenv("OSB_CENTRAL_KERNEL")) c->march = std::strcmp(e, "march") == 0;
     if (const char* e = std::getenv("OSB_GRAPH")) c->use_graph = std::atoi(e) != 0;
     if (const char* e = std::getenv("OSB_FACE_KERNEL")) c->face_tiled = std::strcmp(e, "tile") == 0;
     if (const char* e = std::getenv("OSB_SHOCK_FUSED")) c->shock_fused = std::strcmp(e, "0") != 0;
@@ -786,8 +759,6 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
         if (std::strcmp(e, "dir") == 0) c->bc_periodic = false;
         if (std::strcmp(e, "pass") == 0) c->bc_in_stage = false;
     }
-    if (const char* e = std::getenv("OSB_MARCH_KZ")) c->march_kz = std::max(1, std::atoi(e));
-    if (const char* e = std::getenv("OSB_MARCH_TILE")) c->march_tile = std::atoi(e);
     *out = c;
     return OSB_OK;
 }

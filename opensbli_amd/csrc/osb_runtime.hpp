@@ -57,9 +57,6 @@ struct osb_ctx {
     int fused_tile = 0;             // index into CF_TILES (autotuned)
     int xcd_remap = 2;              // XCD-aware workgroup-to-tile order: 2 = 8x4x2 supertiles per XCD range,
                                     // 1 = x-fastest ranges, 0 = off (OSB_XCD_REMAP)
-    bool march = false;             // 3-D central stage: 3-D tiles, or the z-march kernel (OSB_CENTRAL_KERNEL=march)
-    int march_kz = 16;              // planes per z-march workgroup
-    int march_tile = 0;             // 0: 16x16 columns, 1: 32x8
 
     bool face_tiled = false;        // face fluxes one thread per face from L2/HBM; OSB_FACE_KERNEL=tile: LDS-tiled variant
     int fused_fw = 32;              // lines per fused tile: 32 or 16 (OSB_FUSED_FW)

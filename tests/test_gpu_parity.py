@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from opensbli_amd import native as nat, problems as P
-from helpers import rel_errors, run
+from helpers import assert_exact_unless_fused, fp64_bar, fused_3d, rel_errors, run
 
 pytestmark = pytest.mark.gpu
 
@@ -26,9 +26,9 @@ def _compare(hip, oracle, pr, steps=None, tol=TOL):
     l2, li = rel_errors(a, b)
     print(f"   rel L2 {l2.max():.3e}  rel Linf {li.max():.3e}", flush=True)
     assert np.all(np.isfinite(a))
-    assert l2.max() <= tol and li.max() <= tol, (l2, li)
+    bound = fp64_bar(a, b, pr.desc, oracle, pr, steps, tol)
     assert sa.iter == sb.iter
-    assert math.isclose(sa.time, sb.time, rel_tol=1e-12, abs_tol=1e-15)
+    assert math.isclose(sa.time, sb.time, rel_tol=max(1e-12, bound), abs_tol=1e-15)
     return a, b
 
 
@@ -79,8 +79,9 @@ def test_residual_tgv_kgp(hip, oracle):
 
 def test_kgp_runs(hip, oracle):
     """KGP (enthalpy) with constant-viscosity terms: the separate central kernels in 3-D and 2-D."""
-    a, b = _compare(hip, oracle, P.tgv(n=24, steps=5, scheme="kgp"))
-    assert np.array_equal(a, b)
+    pr = P.tgv(n=24, steps=5, scheme="kgp")
+    a, b = _compare(hip, oracle, pr)
+    assert_exact_unless_fused(a, b, pr.desc)
     # a smooth periodic 2-D field with density and pressure variation
     n, gama = 40, 1.4
     h = 2 * math.pi / n
@@ -92,8 +93,9 @@ def test_kgp_runs(hip, oracle):
     desc = nat.ProblemDesc(ndim=2, n=[n, n], delta=[h, h], conv_scheme=nat.CONV_CENTRAL_KGP, order=4,
                            viscous=nat.VISC_CONSTANT, re=100.0, minf=0.3, bc=[[nat.BC_PERIODIC] * 2] * 2,
                            gama=gama, dt=2e-3)
-    a, b = _compare(hip, oracle, P.Problem("kgp_2d", desc, q, 20))
-    assert np.array_equal(a, b)
+    pr = P.Problem("kgp_2d", desc, q, 20)
+    a, b = _compare(hip, oracle, pr)
+    assert_exact_unless_fused(a, b, pr.desc)
 
 
 def test_residual_tg_sym(hip, oracle):
@@ -165,7 +167,8 @@ def test_glf_parity(hip, oracle, maker):
     pr = _glf(maker())
     _residual_compare(hip, oracle, pr)
     a, b = _compare(hip, oracle, pr)
-    assert np.array_equal(a, b)
+    if not fused_3d(pr.desc):
+        assert np.array_equal(a, b)
 
 
 def test_3d_shock_extrapolation_dirichlet(hip, oracle):
@@ -265,7 +268,8 @@ def test_hdf5_restart_on_the_gpu(hip, oracle, tmp_path):
                 io.write_state(paths[name], c)
         qa, ia, ta = io.read_state(paths["hip"], pr.desc)
         qb, ib, tb = io.read_state(paths["oracle"], pr.desc)
-        assert np.array_equal(qa, qb) and ia == ib == k and ta == tb
+        assert_exact_unless_fused(qa, qb, pr.desc)
+        assert ia == ib == k and ta == tb
         with hip.context(pr.desc) as c:
             io.restart(c, paths["hip"])
             c.step(pr.steps - k)
@@ -288,7 +292,8 @@ def test_stage_by_stage_equals_step(hip, oracle):
                     b.stage(s)
                     o.stage(s)
             qa, qb, qo = a.get_state(), b.get_state(), o.get_state()
-        assert np.array_equal(qa, qb) and np.array_equal(qb, qo)
+        assert np.array_equal(qa, qb)
+        assert_exact_unless_fused(qb, qo, pr.desc)
 
 
 def test_probes_on_the_gpu(hip):
@@ -319,8 +324,10 @@ def _filtered(pr, **kw):
     lambda: _filtered(P.kelvin_helmholtz(n=48, steps=10)),                           # 2-D WENO base, walls
 ])
 def test_weno_filter_parity(hip, oracle, maker):
-    a, b = _compare(hip, oracle, maker())
-    assert np.array_equal(a, b)
+    pr = maker()
+    a, b = _compare(hip, oracle, pr)
+    if not fused_3d(pr.desc):
+        assert np.array_equal(a, b)
 
 
 def test_weno_filter_stages_and_graphs(hip, oracle, monkeypatch):
@@ -375,8 +382,10 @@ def test_residual_closure_tgv(hip, oracle):
                    [[nat.BC_EXTRAPOLATION1] * 2, [nat.BC_PERIODIC] * 2], name="_ext"),                  # 2-D inviscid
 ])
 def test_closure_parity(hip, oracle, maker):
-    a, b = _compare(hip, oracle, maker())
-    assert np.array_equal(a, b)
+    pr = maker()
+    a, b = _compare(hip, oracle, pr)
+    if not fused_3d(pr.desc):
+        assert np.array_equal(a, b)
 
 
 # --- WENO3-Z filter with the Ducros hybrid selection (Central4-WENO5) -----------------
@@ -401,8 +410,10 @@ def _kgp2d_smooth(n=40, steps=12):
                       P.entropy_wave(n=48, steps=10).q0, 10),                           # order 3, every point
 ])
 def test_weno3_ducros_filter_parity(hip, oracle, maker):
-    a, b = _compare(hip, oracle, maker())
-    assert np.array_equal(a, b)
+    pr = maker()
+    a, b = _compare(hip, oracle, pr)
+    if not fused_3d(pr.desc):
+        assert np.array_equal(a, b)
 
 
 # --- error paths that stay on the host side of every kernel ---------------------------
@@ -473,13 +484,18 @@ def test_out_of_memory_is_an_error_not_a_crash(hip):
     """A grid far beyond HBM fails context creation with OSB_E_NOMEM (every
     partial allocation released); the library keeps working afterwards, and
     zero / negative step counts behave as documented."""
-    # descriptor only (no host state): 2048^3 points, ~3 x 345 GB of state buffers
-    big = nat.ProblemDesc(**{**P.tgv(n=16).desc.__dict__, "n": [2048] * 3, "delta": [2 * math.pi / 2048] * 3})
+    # descriptor only (no host state): 1600^3 points, ~3 x 166 GB of state buffers
+    big = nat.ProblemDesc(**{**P.tgv(n=16).desc.__dict__, "n": [1600] * 3, "delta": [2 * math.pi / 1600] * 3})
     _announce("oversized context")
-    assert hip.estimate_bytes(big) > 900 * 2 ** 30
+    assert hip.estimate_bytes(big) > 400 * 2 ** 30
     with pytest.raises(nat.OSBError) as e:
         hip.context(big)
     assert e.value.code == nat.E_NOMEM
+    # beyond 2^32 points per variable (the kernels' 32-bit point offsets): refused before any allocation
+    huge = nat.ProblemDesc(**{**big.__dict__, "n": [2048] * 3, "delta": [2 * math.pi / 2048] * 3})
+    with pytest.raises(nat.OSBError) as e:
+        hip.context(huge)
+    assert e.value.code == nat.E_UNSUPPORTED
     pr = P.kelvin_helmholtz(n=32, steps=2)
     with hip.context(pr.desc) as c:
         c.set_state(pr.q0)
@@ -493,13 +509,16 @@ def test_out_of_memory_is_an_error_not_a_crash(hip):
     assert np.array_equal(q, want)
 
 
-# --- long horizons: a one-ulp difference would have grown by now ----------------------
+# --- long horizons: a one-ulp difference would have grown by now (bitwise paths), and
+# the fused 3-D path stays within the FP64 bar over hundreds of steps --------------
 @pytest.mark.parametrize("maker", [lambda: P.kelvin_helmholtz(n=128, steps=1000),   # WENO-Z, fused faces, walls
                                    lambda: P.tgv(n=32, steps=300),                    # fused central + viscous
                                    lambda: P.tgv_central4_weno5(n=24, steps=200)])    # fused KGP + hybrid filter
-def test_long_horizon_bitwise(hip, oracle, maker):
-    a, b = _compare(hip, oracle, maker())
-    assert np.array_equal(a, b)
+def test_long_horizon(hip, oracle, maker):
+    pr = maker()
+    a, b = _compare(hip, oracle, pr)
+    if not fused_3d(pr.desc):
+        assert np.array_equal(a, b)
 
 
 # --- HLLC / HLLC-LM -----------------------------------------------------------------
@@ -513,5 +532,7 @@ def _hllc(pr, flux):
                                    lambda: P.kelvin_helmholtz(n=48, steps=20),
                                    lambda: P.tgv(n=16, steps=5, scheme="weno", minf=1.25, cfl=0.5, dt=0.0)])
 def test_hllc_parity(hip, oracle, maker, flux):
-    a, b = _compare(hip, oracle, _hllc(maker(), flux))
-    assert np.array_equal(a, b)
+    pr = _hllc(maker(), flux)
+    a, b = _compare(hip, oracle, pr)
+    if not fused_3d(pr.desc):
+        assert np.array_equal(a, b)
