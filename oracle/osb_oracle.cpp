@@ -139,6 +139,23 @@ void constituent_relations(osb_ctx* c, const Range& r, bool need_T_mu) {
 // reference does (create_LEV_inverses / create_REV_inverses,
 // shock_capturing.py:475-530) and of gamma-1 (invgamma_m1, TENO6/opensbli.cpp:140).
 // ---------------------------------------------------------------------------
+// Roe average of the states at offsets 0 and 1 (averaging.py:65-124):
+// rho = sqrt(rho_a rho_b), u and H = (p + rhoE)/rho weighted by sqrt(rho),
+// a = sqrt((gama-1)(H - |u|^2/2)).
+inline void roe_average(int nd, double gamma_m1, double rl, double rr, const double* ua, const double* ub, double pa, double pb,
+                        double rEa, double rEb, double& rho_av, double* u_av, double& a_av) {
+    const double srl = std::sqrt(rl), srr = std::sqrt(rr);
+    rho_av = std::sqrt(rl * rr);
+    const double inv = 1.0 / (srl + srr);
+    double ke = 0.0;
+    for (int m = 0; m < nd; ++m) {
+        u_av[m] = (srl * ua[m] + srr * ub[m]) * inv;
+        ke += u_av[m] * u_av[m];
+    }
+    const double Hroe = ((pa + rEa) / srl + (pb + rEb) / srr) * inv;
+    a_av = std::sqrt(gamma_m1 * (Hroe - 0.5 * ke));
+}
+
 struct Eig {
     double L[5][5];
     double Rm[5][5];
@@ -589,18 +606,10 @@ void reconstruct_direction(osb_ctx* c, int dir, bool filter = false) {
         const size_t xa = x0, xb = x0 + st;
         double rho_av, u_av[3] = {0, 0, 0}, a_av;
         if (filter || c->d.averaging == OSB_AVG_ROE) {   // the filter's scheme averages by Roe (WENO_filter.py:438)
-            const double rl = c->q[xa], rr = c->q[xb];
-            const double srl = std::sqrt(rl), srr = std::sqrt(rr);
-            rho_av = std::sqrt(rl * rr);
-            const double inv = 1.0 / (srl + srr);
-            double ke = 0.0;
-            for (int m = 0; m < nd; ++m) {
-                u_av[m] = (srl * c->u[(size_t)m * c->Ptot + xa] + srr * c->u[(size_t)m * c->Ptot + xb]) * inv;
-                ke += u_av[m] * u_av[m];
-            }
-            const double Hroe = ((c->p[xa] + c->q[(size_t)(nd + 1) * c->Ptot + xa]) / srl +
-                                 (c->p[xb] + c->q[(size_t)(nd + 1) * c->Ptot + xb]) / srr) * inv;
-            a_av = std::sqrt(gamma_m1 * (Hroe - 0.5 * ke));
+            double ua[3], ub[3];
+            for (int m = 0; m < nd; ++m) { ua[m] = c->u[(size_t)m * c->Ptot + xa]; ub[m] = c->u[(size_t)m * c->Ptot + xb]; }
+            roe_average(nd, gamma_m1, c->q[xa], c->q[xb], ua, ub, c->p[xa], c->p[xb], c->q[(size_t)(nd + 1) * c->Ptot + xa],
+                        c->q[(size_t)(nd + 1) * c->Ptot + xb], rho_av, u_av, a_av);
         } else {
             rho_av = 0.5 * (c->q[xa] + c->q[xb]);
             for (int m = 0; m < nd; ++m) u_av[m] = 0.5 * (c->u[(size_t)m * c->Ptot + xa] + c->u[(size_t)m * c->Ptot + xb]);
@@ -1591,6 +1600,23 @@ int osb_oracle_eigensystem(int ndim, int dir, double gama, double rho, const dou
 
 // scalar RK step of dq/dt = lam*q with the solver's stage updates
 // stages = 0: the default coefficients of `scheme`; else RK-LS with A/B given
+// Roe average of two conservative states (the face routine's averaging):
+// out = rho, u0, u1, u2, a
+int osb_oracle_roe_average(int ndim, double gama, const double* qa, const double* qb, double* out) {
+    double ua[3] = {0, 0, 0}, ub[3] = {0, 0, 0}, kea = 0.0, keb = 0.0;
+    for (int m = 0; m < ndim; ++m) {
+        ua[m] = qa[1 + m] / qa[0];
+        ub[m] = qb[1 + m] / qb[0];
+        kea -= 0.5 * (ua[m] * ua[m]) * qa[0];
+        keb -= 0.5 * (ub[m] * ub[m]) * qb[0];
+    }
+    const double pa = (gama - 1.0) * (kea + qa[ndim + 1]), pb = (gama - 1.0) * (keb + qb[ndim + 1]);
+    double u_av[3] = {0, 0, 0};
+    roe_average(ndim, gama - 1.0, qa[0], qb[0], ua, ub, pa, pb, qa[ndim + 1], qb[ndim + 1], out[0], u_av, out[4]);
+    for (int m = 0; m < 3; ++m) out[1 + m] = u_av[m];
+    return OSB_OK;
+}
+
 double osb_oracle_rk_scalar(int scheme, int stages, const double* A, const double* B, double lam, double dt, double q0) {
     double q = q0, reg = 0.0;
     if (scheme == OSB_TIME_RKLS3 && stages > 0) {

@@ -449,13 +449,28 @@ def test_periodic_halo_fill_one_pass(hip, maker, monkeypatch):
     assert np.array_equal(out["dir"], want)
 
 
+def _tgv_box(n, scheme="central", precision=nat.PREC_FP64, steps=2):
+    """The TGV on a periodic box of n = [n0, n1, n2] points (2 pi per direction),
+    anisotropic and not a multiple of the stage kernel's 16 x 4 x 4 tile."""
+    pr = P.tgv(n=16, steps=steps, scheme=scheme, precision=precision)
+    delta = [2 * math.pi / m for m in n]
+    q = P._tgv_fields(tuple(n), tuple(delta), pr.desc.gama, pr.desc.minf)
+    d = nat.ProblemDesc(**{**pr.desc.__dict__, "n": list(n), "delta": delta})
+    return P.Problem(f"tgv_{scheme}_{n[0]}x{n[1]}x{n[2]}", d, q, steps)
+
+
 @pytest.mark.parametrize("maker", [
     lambda: P.tgv(n=16, steps=2),
     lambda: P.tgv(n=16, steps=2, scheme="kgp"),
     lambda: P.tgv(n=16, steps=2, scheme="teno", minf=1.25),
     lambda: P.tgv(n=16, steps=2, precision=nat.PREC_MIXED),
     lambda: P.tgv(n=16, steps=2, scheme="kgp", precision=nat.PREC_MIXED),
-], ids=["feiereisen", "kgp", "teno_viscous", "mixed_feiereisen", "mixed_kgp"])
+    lambda: _tgv_box([20, 12, 10]),
+    lambda: _tgv_box([20, 12, 10], scheme="kgp"),
+    lambda: _tgv_box([20, 12, 10], precision=nat.PREC_MIXED),
+    lambda: _tgv_box([20, 12, 10], scheme="kgp", precision=nat.PREC_MIXED),
+], ids=["feiereisen", "kgp", "teno_viscous", "mixed_feiereisen", "mixed_kgp",
+        "feiereisen_20x12x10", "kgp_20x12x10", "mixed_feiereisen_20x12x10", "mixed_kgp_20x12x10"])
 def test_stage_kernel_halo_images(hip, maker, monkeypatch):
     """Every face periodic: the 3-D tile stage kernels (FP64 and mixed) write
     each updated point's wrapped halo images themselves, and the padded state after every
@@ -478,6 +493,22 @@ def test_stage_kernel_halo_images(hip, maker, monkeypatch):
             out[mode] = snaps
     for a, b, d in zip(out["in_stage"], out["pass"], out["dir"]):
         assert np.array_equal(a, b) and np.array_equal(a, d)
+
+
+@pytest.mark.parametrize("scheme", ["central", "kgp"])
+def test_tgv_anisotropic_ragged_tiles(hip, oracle, scheme):
+    """A 20 x 12 x 10 box: partly filled stage-kernel tiles in every direction
+    and distinct extents (an n0/n1/n2 mix-up would show), FP64 against the
+    oracle and mixed precision against the FP64 oracle."""
+    pr = _tgv_box([20, 12, 10], scheme=scheme, steps=6)
+    _compare(hip, oracle, pr)
+    mx = _tgv_box([20, 12, 10], scheme=scheme, precision=nat.PREC_MIXED, steps=6)
+    _announce(f"mixed {mx.name}")
+    a, _ = run(hip, mx)
+    b, _ = run(oracle, pr)
+    l2, _ = rel_errors(a, b)
+    print(f"   mixed vs FP64 oracle rel L2 {np.array2string(l2, precision=2)}", flush=True)
+    assert l2.max() <= 1e-5, l2
 
 
 def test_out_of_memory_is_an_error_not_a_crash(hip):

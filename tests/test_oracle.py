@@ -945,3 +945,168 @@ def test_hllc_validation(oracle):
                dict(flux_type=nat.FLUX_HLLC_LM, shock_filter=nat.FILTER_WENOZ), dict(flux_type=9)):
         with pytest.raises(nat.OSBError):
             oracle.context(nat.ProblemDesc(**{**pr.desc.__dict__, **kw}))
+
+
+# --------------------------------------------------------------------------------
+# a3: RoeAverage (averaging.py:65-124) -- the Roe property of the averaged state
+# --------------------------------------------------------------------------------
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_roe_average_satisfies_the_roe_property(hooks, ndim):
+    """At the Roe state the eigensystem reproduces the flux jump exactly:
+    F(q_R) - F(q_L) = R Lambda L (q_R - q_L) along every direction.  An
+    arithmetic mean, an unweighted enthalpy or a sound speed from the mean
+    pressure breaks it at first order in the jump."""
+    L_ = hooks
+    dp = C.POINTER(C.c_double)
+    L_.osb_oracle_roe_average.argtypes = [C.c_int, C.c_double, dp, dp, dp]
+    rng = np.random.default_rng(11 + ndim)
+    gama, nv = 1.4, ndim + 2
+
+    def state():
+        rho = rng.uniform(0.2, 2.5)
+        u = rng.uniform(-1.5, 1.5, ndim)
+        p = rng.uniform(0.1, 4.0)
+        return np.concatenate([[rho], rho * u, [p / (gama - 1) + 0.5 * rho * np.dot(u, u)]])
+
+    for _ in range(6):
+        qa, qb = state(), state()
+        out = np.zeros(5)
+        a_, ap = _arr(qa)
+        b_, bp = _arr(qb)
+        L_.osb_oracle_roe_average(ndim, gama, ap, bp, out.ctypes.data_as(dp))
+        rho, u, a = out[0], out[1:4].copy(), out[4]
+        for d in range(ndim):
+            uu, up = _arr(u)
+            Lm, Lp = _arr(np.zeros(nv * nv))
+            Rm, Rp = _arr(np.zeros(nv * nv))
+            L_.osb_oracle_eigensystem(ndim, d, gama, rho, up, a, Lp, Rp)
+            Lm, Rm = Lm.reshape(nv, nv), Rm.reshape(nv, nv)
+            ev = [u[d] - a, u[d], u[d] + a] if ndim == 1 else [u[d]] * ndim + [u[d] + a, u[d] - a]
+            jump = _euler_flux(qb, ndim, d, gama) - _euler_flux(qa, ndim, d, gama)
+            got = Rm @ np.diag(ev) @ Lm @ (qb - qa)
+            # the eigenvectors carry 15-digit literals (sqrt(2) printed to 15 s.f.), so the
+            # identity holds to ~1e-14 relative to the flux scale
+            assert np.allclose(got, jump, rtol=0, atol=1e-12 * np.abs(jump).max()), (d, got, jump)
+    # the arithmetic mean state does not have the property (the test can fail)
+    qa, qb = state(), state()
+    mean = 0.5 * (qa + qb)
+    rho = mean[0]
+    u = np.zeros(3)
+    u[:ndim] = mean[1:1 + ndim] / rho
+    p = (gama - 1) * (mean[-1] - 0.5 * rho * np.dot(u, u))
+    a = math.sqrt(gama * p / rho)
+    uu, up = _arr(u)
+    Lm, Lp = _arr(np.zeros(nv * nv))
+    Rm, Rp = _arr(np.zeros(nv * nv))
+    L_.osb_oracle_eigensystem(ndim, 0, gama, rho, up, a, Lp, Rp)
+    ev = [u[0] - a, u[0], u[0] + a] if ndim == 1 else [u[0]] * ndim + [u[0] + a, u[0] - a]
+    jump = _euler_flux(qb, ndim, 0, gama) - _euler_flux(qa, ndim, 0, gama)
+    got = Rm.reshape(nv, nv) @ np.diag(ev) @ Lm.reshape(nv, nv) @ (qb - qa)
+    assert not np.allclose(got, jump, rtol=0, atol=1e-6 * np.abs(jump).max())
+
+
+# --------------------------------------------------------------------------------
+# a10: the CFL time step, dt = CFL / max over interior points of sum_d (|u_d| + a)/Delta_d
+# --------------------------------------------------------------------------------
+def _wave_sum(q, ndim, gama, delta):
+    u = [q[1 + d] / q[0] for d in range(ndim)]
+    p = (gama - 1.0) * (q[ndim + 1] - 0.5 * q[0] * sum(x * x for x in u))
+    a = np.sqrt(gama * p / q[0])
+    return sum((np.abs(u[d]) + a) / delta[d] for d in range(ndim))
+
+
+def test_cfl_dt_is_the_interior_maximum_wave_speed(oracle, hooks):
+    """dt of each step is CFL over the maximum, over the interior points of the
+    state at the start of the step, of sum_d (|u_d| + a)/Delta_d; the time
+    accumulates those dts.  Anisotropic spacing tells a sum over directions
+    from a maximum or a missing 1/Delta; linearly extrapolated halos holding
+    faster states tell an interior maximum from one over the padded block."""
+    rng = np.random.default_rng(5)
+    n, gama, cfl = [24, 18], 1.4, 0.4
+    delta = [1.0 / 24, 1.0 / 60]
+    X = (np.arange(n[0])[None, :] + 0.5) / n[0]
+    row = lambda: rng.random((n[1], 1))                                 # random per row, smooth along x0
+    rho = (1.0 + 0.3 * row()) * (1.0 + 0.1 * X)
+    u = 3.0 * (1.0 - X) + 0.5 * row()                                   # fastest towards the x0 = 0 face
+    v = 0.4 * (row() - 0.5) * np.cos(2 * np.pi * X)
+    p = (1.0 + 0.5 * row()) * np.ones_like(X)
+    q0 = np.stack([rho, rho * u, rho * v, p / (gama - 1) + 0.5 * rho * (u * u + v * v)])[:, None]
+    desc = nat.ProblemDesc(ndim=2, n=n, delta=delta, conv_scheme=nat.CONV_WENO_Z, averaging=nat.AVG_ROE,
+                           bc=[[nat.BC_EXTRAPOLATION1, nat.BC_EXTRAPOLATION1], [nat.BC_PERIODIC] * 2],
+                           gama=gama, cfl=cfl, dt=0.0)
+    with oracle.context(desc) as c:
+        c.set_state(q0)
+        c.apply_bcs()
+        Q, H = _padded(hooks, c)
+        s_int = _wave_sum(Q[:, 0, H:H + n[1], H:H + n[0]], 2, gama, delta)
+        s_halo = _wave_sum(Q[:, 0, H:H + n[1], H - 3:H], 2, gama, delta)     # the layers the WENO stencil reads
+        assert s_halo.max() > 1.05 * s_int.max()                # the halos would change dt
+        c.step(1)
+        st1 = c.stats()
+        dt1 = cfl / _wave_sum(q0[:, 0], 2, gama, delta).max()
+        assert math.isclose(st1.last_dt, dt1, rel_tol=1e-14)
+        assert st1.time == st1.last_dt
+        q1 = c.get_state()
+        c.step(1)
+        st2 = c.stats()
+    dt2 = cfl / _wave_sum(q1[:, 0], 2, gama, delta).max()       # from the state at the second step's start
+    assert math.isclose(st2.last_dt, dt2, rel_tol=1e-14) and dt2 != dt1
+    assert math.isclose(st2.time, dt1 + dt2, rel_tol=1e-14)
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_cfl_dt_uniform_state_closed_form(oracle, ndim):
+    """A uniform state: dt = CFL / sum_d (|u_d| + a)/Delta_d with a = sqrt(gama p/rho)."""
+    gama, cfl, rho, p = 1.4, 0.5, 1.3, 0.9
+    u = [0.7, -0.4, 0.25][:ndim]
+    n = [12, 10, 8][:ndim]
+    delta = [0.1, 0.03, 0.07][:ndim]
+    shape = (1 if ndim < 3 else n[2], 1 if ndim < 2 else n[1], n[0])
+    q = np.stack([np.full(shape, rho)] + [np.full(shape, rho * x) for x in u] +
+                 [np.full(shape, p / (gama - 1) + 0.5 * rho * sum(x * x for x in u))])
+    desc = nat.ProblemDesc(ndim=ndim, n=n + [1] * (3 - ndim), delta=delta + [1.0] * (3 - ndim), conv_scheme=nat.CONV_WENO_Z,
+                           bc=[[nat.BC_PERIODIC] * 2] * ndim, gama=gama, cfl=cfl, dt=0.0)
+    a = math.sqrt(gama * p / rho)
+    want = cfl / sum((abs(u[d]) + a) / delta[d] for d in range(ndim))
+    with oracle.context(desc) as c:
+        c.set_state(q)
+        c.step(1)
+        st = c.stats()
+    assert math.isclose(st.last_dt, want, rel_tol=1e-14), (st.last_dt, want)
+
+
+# --------------------------------------------------------------------------------
+# a6: constant-viscosity terms (split_forms.py:383-392) on a shear wave
+# --------------------------------------------------------------------------------
+def test_constant_viscosity_shear_wave():
+    """rho = 1, p uniform, u0 = U sin(x1), u1 = u2 = 0 (Feiereisen + constant mu):
+    the convective terms vanish, the momentum residual of rho u0 is
+    -(U/Re) sin(x1) and the energy residual (U^2/Re) cos(2 x1) (u0 d2u0 + (du0)^2;
+    T uniform, no heat flux), all other residuals zero.  The 4th-order stencils
+    reach these at 4th order."""
+    lib = nat.Library(os.path.join(ROOT, "oracle", "libosb_oracle.so"))
+    U, re, gama, minf = 0.3, 250.0, 1.4, 0.2
+    errs = []
+    for n in (16, 32, 64):
+        h = 2 * math.pi / n
+        y = h * np.arange(n)
+        shape = (8, n, 8)
+        uu = np.broadcast_to((U * np.sin(y))[None, :, None], shape)
+        p = 1.0 / (gama * minf * minf)
+        q = np.stack([np.ones(shape), uu, np.zeros(shape), np.zeros(shape), np.full(shape, p / (gama - 1)) + 0.5 * uu * uu])
+        desc = nat.ProblemDesc(ndim=3, n=[8, n, 8], delta=[0.3, h, 0.5], conv_scheme=nat.CONV_CENTRAL_FEIEREISEN, order=4,
+                               viscous=nat.VISC_CONSTANT, re=re, pr=0.71, minf=minf, gama=gama,
+                               bc=[[nat.BC_PERIODIC] * 2] * 3, dt=1e-3)
+        with lib.context(desc) as c:
+            c.set_state(q)
+            R = c.residual()
+        want_m = -(U / re) * np.sin(y)
+        want_e = (U * U / re) * np.cos(2 * y)
+        e_m = np.abs(R[1] - want_m[None, :, None]).max()
+        e_e = np.abs(R[4] - want_e[None, :, None]).max()
+        assert np.abs(R[0]).max() < 1e-13 and np.abs(R[2]).max() < 1e-13 and np.abs(R[3]).max() < 1e-13
+        errs.append((e_m, e_e))
+    for (m0, e0), (m1, e1) in zip(errs, errs[1:]):
+        assert m0 / m1 > 14.0 and e0 / e1 > 14.0, errs      # 4th order: 16x per halving
+    # D2 of sin at h = 2 pi/64: relative error ~ h^4/90 = 1e-6
+    assert errs[-1][0] < 2e-6 * U / re and errs[-1][1] < 1e-5 * U * U / re
