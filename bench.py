@@ -48,6 +48,9 @@ def pmc_traffic(family, workload, precision):
     return None
 
 
+WORKLOADS = ("tgv_central_128", "tgv_teno5_256", "sod_1024", "tgv_teno6_128", "tgv_central4weno5_128")
+
+
 def make_problem(name, precision):
     from opensbli_amd import native as nat, problems as P
     prec = nat.PREC_MIXED if precision == "mixed" else nat.PREC_FP64
@@ -79,19 +82,25 @@ def ensure_built():
 
 
 def time_gpu(problem, steps, warmup):
-    """Wall time of `steps` steps (captured time steps replayed as hipGraphs),
-    then the same number of steps again, launched eagerly with HIP events
-    around every kernel, for the per-kernel durations."""
+    """Wall time of `steps` steps (captured time steps replayed as hipGraphs) on
+    a stream the context adopts, with HIP events recorded on that stream around
+    them; then the same number of steps again, launched eagerly with HIP events
+    around every kernel, for each kernel family's share of the GPU time."""
     import torch
     from opensbli_amd import native as nat
     hip = nat.load_hip()
+    stream = torch.cuda.Stream()
     with hip.context(problem.desc) as c:
+        c.set_stream(stream.cuda_stream)
         c.set_state(problem.q0)
         c.step(warmup)
         c.sync()
         torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         t0 = time.perf_counter()
+        ev0.record(stream)
         c.step(steps)
+        ev1.record(stream)
         c.sync()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
@@ -101,26 +110,67 @@ def time_gpu(problem, steps, warmup):
         c.sync()
         prof = c.profile_read()
         c.profile_enable(False)
-    return t1 - t0, prof, state
+    return t1 - t0, prof, state, ev0.elapsed_time(ev1) / 1e3
 
 
-def cpu_baseline(oracle_path, problem, budget_s=15.0):
+def cpu_info():
+    """Host CPU model and the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return model, avail
+
+
+def _cpu_leg(oracle_path, workload, threads, budget_s):
+    """Time the oracle on a bounded sample of the workload in a child process
+    with `threads` OpenMP threads bound to cores; returns its JSON record."""
+    import subprocess
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-leg", workload, "--cpu-budget", str(budget_s)],
+                         env=env, capture_output=True, text=True, check=True)
+    return json.loads(out.stdout.strip().splitlines()[-1])
+
+
+def cpu_leg_main(workload, budget_s):
+    """Child side of _cpu_leg: run the oracle (no GPU use in this process)."""
     from opensbli_amd import native as nat
-    lib = nat.Library(oracle_path)
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    problem = make_problem(workload, "fp64")
+    lib = nat.Library(os.path.join(ROOT, "oracle", "libosb_oracle.so"))
     with lib.context(problem.desc) as c:
         c.set_state(problem.q0)
         t0 = time.perf_counter()
         c.step(1)
-        t1 = time.perf_counter()
-        per = max(t1 - t0, 1e-6)
+        per = max(time.perf_counter() - t0, 1e-6)
         n = max(1, min(problem.steps, int(budget_s / per)))
         t0 = time.perf_counter()
         c.step(n)
         t1 = time.perf_counter()
-    value = problem.desc.npoints * 3.0 * n / (t1 - t0) / 1e9
-    return {"value": value, "unit": "GPts·stage/s", "cores": cores, "kind": "port",
-            "sample": f"{problem.name}: {n} of the workload's steps after 1 warm-up step, oracle OpenMP build, {cores} threads"}
+    print(json.dumps({"steps": n, "seconds": t1 - t0, "gpts_stage_per_s": problem.desc.npoints * 3.0 * n / (t1 - t0) / 1e9}))
+
+
+def cpu_baseline(oracle_path, workload, problem, budget_s=15.0):
+    """The oracle on the host cores: all the cores this process may use (at
+    most OMP_NUM_THREADS when set), OpenMP threads bound one per core, and one
+    thread, each on a bounded sample of the workload's steps."""
+    model, avail = cpu_info()
+    threads = min(avail, int(os.environ.get("OMP_NUM_THREADS", avail)))
+    many = _cpu_leg(oracle_path, workload, threads, budget_s)
+    one = _cpu_leg(oracle_path, workload, 1, budget_s)
+    return {"value": many["gpts_stage_per_s"], "unit": "GPts·stage/s", "cores": threads, "kind": "port",
+            "sample": f"{problem.name}: {many['steps']} of the workload's steps after 1 warm-up step, oracle OpenMP "
+                      f"build, {threads} threads (OMP_PROC_BIND=close, OMP_PLACES=cores)",
+            "cpu_model": model, "cpus_available": avail,
+            "single_thread": {"value": one["gpts_stage_per_s"], "cores": 1,
+                              "sample": f"{problem.name}: {one['steps']} steps after 1 warm-up step, 1 thread"}}
 
 
 def _rel_l2(a, b):
@@ -214,9 +264,76 @@ def reference_benchmarks(steps=20, warmup=2):
     out = {}
     for name in ("tgv_teno6_128", "tgv_central4weno5_128"):
         pr = make_problem(name, "fp64")
-        dt, prof, _ = time_gpu(pr, steps, warmup)
+        dt, prof, _, _ = time_gpu(pr, steps, warmup)
         out[name] = {"value": pr.desc.npoints * 3.0 * steps / dt / 1e9, "unit": "GPts*stage/s", "steps_timed": steps,
                      "kernel_ms_per_step": {k: v[0] / steps for k, v in prof.items() if v[1]}}
+    return out
+
+
+def kernel_roofline(problem, workload, precision, prof, t_events, dt_wall, steps):
+    """Roofline entry of the dominant kernel family: algorithmic bytes of one RK
+    stage over the grid (read q, read du, write du, write q) per launch, over
+    that family's average launch duration in the timed region -- the region's
+    HIP-event time on the context's stream times the family's share of the
+    GPU time in the per-kernel profile pass, over its launches."""
+    fam, (ms, nlaunch) = max(prof.items(), key=lambda kv: kv[1][0])
+    total_ms = sum(v[0] for v in prof.values())
+    share = ms / total_ms if total_ms > 0 else 1.0
+    avg_s = t_events * share / max(nlaunch, 1)
+    npts = problem.desc.npoints
+    bytes_alg = BYTES_PER_PT_STAGE[problem.desc.ndim] * npts
+    achieved = bytes_alg / avg_s / 1e9
+    stage_s = dt_wall / (3.0 * steps)
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic(fam, workload, precision),
+            "kernel": fam, "kernel_avg_us": avg_s * 1e6, "kernel_launches": nlaunch,
+            "kernel_share_of_gpu_time": share, "kernel_avg_us_profile_pass": ms / max(nlaunch, 1) * 1e3,
+            "timed_region_events_s": t_events, "bytes_alg_per_launch": bytes_alg,
+            "stage_achieved_GBs": bytes_alg / stage_s / 1e9}
+
+
+def profile_value(name, key):
+    """A `key: value` line of a committed profile summary (None if absent)."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(key + ":"):
+            return float(line.split(":")[1])
+    return None
+
+
+FP64_VALU_PEAK_TFLOPS = 78.6
+
+
+def teno256_line(steps=10, warmup=2):
+    """The supersonic TGV at 256^3 (TENO5, C_T = 1e-5, Roe, LLF, Sutherland
+    viscous terms, CFL 0.5 from the device reduction), FP64 and mixed: GPts*stage/s,
+    the HBM roofline of the stage, the FP64 VALU fraction of the FP64 stage (FP64
+    flops per stage from the rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 summary), and the
+    mixed state's error against the FP64 run after the same steps."""
+    out = {"workload": "tgv_teno5_256", "grid": [256, 256, 256], "steps_timed": steps}
+    states = {}
+    for prec in ("fp64", "mixed"):
+        pr = make_problem("tgv_teno5_256", prec)
+        dt, prof, st, t_ev = time_gpu(pr, steps, warmup)
+        npts = pr.desc.npoints
+        stage_s = t_ev / (3.0 * steps)
+        rec = {"value": npts * 3.0 * steps / dt / 1e9, "ms_per_step": dt / steps * 1e3,
+               "kernel_ms_per_step": {k: v[0] / steps for k, v in prof.items() if v[1]},
+               "hbm": {"bytes_alg_per_stage": BYTES_PER_PT_STAGE[3] * npts,
+                       "achieved_GBs": BYTES_PER_PT_STAGE[3] * npts / stage_s / 1e9, "peak_GBs": HBM_PEAK_GBS}}
+        rec["hbm"]["frac"] = rec["hbm"]["achieved_GBs"] / HBM_PEAK_GBS
+        if prec == "fp64":
+            flops = profile_value("tgv_teno5_256_pmc.md", "fp64_flops_per_stage")
+            if flops:
+                rec["valu_fp64"] = {"flops_per_stage": flops, "achieved_TFLOPs": flops / stage_s / 1e12,
+                                    "peak_TFLOPs": FP64_VALU_PEAK_TFLOPS,
+                                    "frac": flops / stage_s / 1e12 / FP64_VALU_PEAK_TFLOPS}
+        out[prec] = rec
+        states[prec] = st
+    st_err, pv = _rel_l2(states["mixed"], states["fp64"])
+    out["mixed"]["rel_l2_vs_fp64"] = {"state": st_err, "per_variable": pv}
     return out
 
 
@@ -224,7 +341,7 @@ def mixed_headline(problem_name, steps, warmup):
     """The headline workload in mixed precision, with its error against FP64."""
     from opensbli_amd import native as nat
     pm = make_problem(problem_name, "mixed")
-    dt_m, _, qm = time_gpu(pm, steps, warmup)
+    dt_m, _, qm, _ = time_gpu(pm, steps, warmup)
     pf = make_problem(problem_name, "fp64")
     hip = nat.load_hip()
     with hip.context(pf.desc) as c:
@@ -248,9 +365,13 @@ def main():
                     help="skip the mixed-precision, KH-ensemble and serving lines added to the default run")
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--serving-requests", type=int, default=64)
+    ap.add_argument("--cpu-leg", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed run, copy a 256 MiB FP64 buffer 5 times (PMC byte-counter calibration)")
     args = ap.parse_args()
+    if args.cpu_leg:
+        cpu_leg_main(args.cpu_leg, args.cpu_budget)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
@@ -269,7 +390,7 @@ def main():
               flush=True)
         return
     problem = make_problem(args.workload, args.precision)
-    dt_wall, prof, state = time_gpu(problem, args.steps, args.warmup)
+    dt_wall, prof, state, t_events = time_gpu(problem, args.steps, args.warmup)
     import numpy as np
     if not np.all(np.isfinite(state)):
         raise SystemExit("non-finite state after the timed steps")
@@ -278,17 +399,7 @@ def main():
 
     # roofline of the dominant kernel family: algorithmic bytes of one RK stage over
     # the grid (read q, read du, write du, write q) per launch of a per-stage kernel
-    fam, (ms, nlaunch) = max(prof.items(), key=lambda kv: kv[1][0])
-    avg_s = ms / 1e3 / max(nlaunch, 1)
-    ndim = problem.desc.ndim
-    bytes_alg = BYTES_PER_PT_STAGE[ndim] * npts
-    achieved = bytes_alg / avg_s / 1e9
-    stage_s = dt_wall / (3.0 * args.steps)
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic(fam, args.workload, args.precision),
-                "kernel": fam, "kernel_avg_us": avg_s * 1e6, "kernel_launches": nlaunch,
-                "bytes_alg_per_launch": bytes_alg,
-                "stage_achieved_GBs": bytes_alg / stage_s / 1e9}
+    roofline = kernel_roofline(problem, args.workload, args.precision, prof, t_events, dt_wall, args.steps)
     out = {
         "metric": METRIC,
         "value": value,
@@ -308,12 +419,13 @@ def main():
         "kernel_ms": {k: v[0] for k, v in prof.items() if v[1]},
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(oracle_path, problem, args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(oracle_path, args.workload, problem, args.cpu_budget)
     if not args.no_extras and args.workload == "tgv_central_128" and args.precision == "fp64":
         out["mixed_precision"] = mixed_headline(args.workload, args.steps, args.warmup)
         out["ensemble"] = ensemble_bench(args.ensemble_steps)
         out["serving"] = serving_bench(args.serving_requests)
         out["reference_benchmarks"] = reference_benchmarks()
+        out["tgv_teno5_256"] = teno256_line()
     if args.copy_probe:
         import torch
         a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
