@@ -218,6 +218,22 @@ int  osb_set_state_device(osb_ctx* ctx, const double* dev, int layout);
 int  osb_get_state_device(osb_ctx* ctx, double* dev, int layout);
 int  osb_set_stream(osb_ctx* ctx, void* hip_stream);
 
+/* Allocation-free stepping: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): advances a caller-owned
+ * block by nsteps on `hip_stream`, allocating nothing.  q is the padded state
+ * (nv x padded points, variable-major, x fastest, halos filled -- the
+ * OSB_LAYOUT_PADDED layout of osb_get_state), updated in place; du is the RK
+ * register in the same layout (zero before a run's first step).  The other
+ * buffers a context would hold come from a fixed 8 MiB array per GPU, held
+ * until the process ends; a block needing more (osb_estimate_bytes less the
+ * state, register and staging) is refused with OSB_E_UNSUPPORTED and
+ * osb_last_error(NULL) says why.  Calls that overlap on one GPU share that
+ * array and race: the caller serialises them.  The steps launch eagerly
+ * (identical kernels to osb_step); with CFL > 0 each step's dt stays on the
+ * device (time is the caller's to keep).                                     */
+int  osb_step_noalloc(const osb_problem_desc* desc, double* q, double* du, int nsteps, void* hip_stream);
+
 /* Batched ensembles: many instances of possibly different sizes advanced by
  * one launch per RK stage (persistent kernels, hipGraph replay).  No
  * counterpart in the reference, which runs one block per executable.        */

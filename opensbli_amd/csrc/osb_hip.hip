@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -625,33 +626,10 @@ void free_ctx(osb_ctx* c) {
     delete c;
 }
 
-}  // namespace
-
-extern "C" {
-
-int osb_abi_version(void) { return OSB_ABI_VERSION; }
-
-int osb_estimate_bytes(const osb_problem_desc* desc, int64_t* out) {
-    if (!out) return set_err(nullptr, OSB_E_INVALID, "null output pointer");
-    std::string why;
-    int rc = validate(desc, why);
-    if (rc) return set_err(nullptr, rc, why);
-    *out = (int64_t)plan_alloc(desc).bytes();
-    return OSB_OK;
-}
-
-const char* osb_last_error(const osb_ctx* ctx) { return ctx ? ctx->err.c_str() : g_osb_create_error.c_str(); }
-
-int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
-    if (!out) return set_err(nullptr, OSB_E_INVALID, "null output pointer");
-    *out = nullptr;
-    std::string why;
-    int rc = validate(desc, why);
-    if (rc) return set_err(nullptr, rc, why);
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(nullptr, OSB_E_DEVICE, "no HIP device available");
-
-    osb_ctx* c = new osb_ctx();
+// Every field of a context that needs no device memory: the descriptor,
+// geometry, constants, RK coefficients and the kernel choices (with their
+// environment overrides).  osb_ctx_create and osb_step_noalloc share it.
+void init_ctx(osb_ctx* c, const osb_problem_desc* desc) {
     c->d = *desc;
     (void)hipGetDevice(&c->device);
     const int nd = desc->ndim;
@@ -703,6 +681,104 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
             if (i < desc->ndim && desc->bc[i][sd] >= OSB_BC_EXTRAPOLATION0) k.carp[i] |= 1 << sd;
     }
 
+    if (const char* e = std::getenv("OSB_FUSED_TILE")) c->fused_tile = std::max(0, std::min(CF_NTILES - 1, std::atoi(e)));
+    if (const char* e = std::getenv("OSB_XCD_REMAP")) c->xcd_remap = std::max(0, std::min(2, std::atoi(e)));
+    if (const char* e = std::getenv("OSB_GRAPH")) c->use_graph = std::atoi(e) != 0;
+    if (const char* e = std::getenv("OSB_FACE_KERNEL")) c->face_tiled = std::strcmp(e, "tile") == 0;
+    if (const char* e = std::getenv("OSB_SHOCK_FUSED")) c->shock_fused = std::strcmp(e, "0") != 0;
+    if (const char* e = std::getenv("OSB_FUSED_FW")) {
+        const int w = std::atoi(e);
+        if (w == 16 || w == 32) c->fused_fw = w;
+    }
+    if (c->face_tiled) c->shock_fused = false;      // the LDS-tiled face variant keeps the separate update
+    if (desc->flux_type >= OSB_FLUX_HLLC) {           // HLLC faces run their own kernel and the separate update
+        c->face_tiled = false;
+        c->shock_fused = false;
+    }
+    c->bc_periodic = true;
+    for (int d = 0; d < c->g.ndim; ++d)
+        if (desc->bc[d][0] != OSB_BC_PERIODIC || desc->bc[d][1] != OSB_BC_PERIODIC || c->g.n[d] < c->g.halo) c->bc_periodic = false;
+    for (int d = 0; d < c->g.ndim; ++d)
+        if (c->g.n[d] < 2 * c->g.halo) c->bc_in_stage = false;
+    if (const char* e = std::getenv("OSB_BC_KERNEL")) {
+        if (std::strcmp(e, "dir") == 0) c->bc_periodic = false;
+        if (std::strcmp(e, "pass") == 0) c->bc_in_stage = false;
+    }
+}
+
+
+// Allocation-free stepping (osb_step_noalloc): every device buffer of a
+// context beyond the caller's state and RK register comes from this array,
+// one copy per GPU, held until the process ends.  Host code uses its
+// per-device address from hipGetSymbolAddress, never the array's name.
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = size_t(1) << 20;       // 8 MiB
+constexpr int NOALLOC_MAX_DEVICES = 64;
+__device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+// the current device's copy of `scratch` (looked up once per device)
+int noalloc_scratch(double** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) { (void)hipGetLastError(); return set_err(nullptr, OSB_E_DEVICE, std::string("hipGetDevice: ") + hipGetErrorString(e)); }
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return set_err(nullptr, OSB_E_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            return set_err(nullptr, OSB_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        }
+        g_scratch_ptr[dev] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return OSB_OK;
+}
+
+// clear n doubles of the scratch, then set the scalars' dt (scal[1]): fill
+// kernels rather than memsets, so the call can be captured into a hipGraph
+__global__ void k_noalloc_clear(double* p, long n) {
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) p[t] = 0.0;
+}
+__global__ void k_set_dt(double* scal, double dt) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) scal[1] = dt;
+}
+// the padded state back into the caller's buffer
+__global__ void k_copy_all(double* __restrict__ dst, const double* __restrict__ src, long n) {
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) dst[t] = src[t];
+}
+
+}  // namespace
+
+extern "C" {
+
+int osb_abi_version(void) { return OSB_ABI_VERSION; }
+
+int osb_estimate_bytes(const osb_problem_desc* desc, int64_t* out) {
+    if (!out) return set_err(nullptr, OSB_E_INVALID, "null output pointer");
+    std::string why;
+    int rc = validate(desc, why);
+    if (rc) return set_err(nullptr, rc, why);
+    *out = (int64_t)plan_alloc(desc).bytes();
+    return OSB_OK;
+}
+
+const char* osb_last_error(const osb_ctx* ctx) { return ctx ? ctx->err.c_str() : g_osb_create_error.c_str(); }
+
+int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
+    if (!out) return set_err(nullptr, OSB_E_INVALID, "null output pointer");
+    *out = nullptr;
+    std::string why;
+    int rc = validate(desc, why);
+    if (rc) return set_err(nullptr, rc, why);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(nullptr, OSB_E_DEVICE, "no HIP device available");
+
+    osb_ctx* c = new osb_ctx();
+    init_ctx(c, desc);
+    Consts& k = c->c;
     const AllocPlan ap = plan_alloc(desc);
     int r = OSB_OK;
     if (!r) r = dalloc(c, &c->qa, ap.q);
@@ -736,29 +812,6 @@ int osb_ctx_create(const osb_problem_desc* desc, osb_ctx** out) {
     if (r) { free_ctx(c); return r; }
     c->q_cur = c->qa;
     c->q_prev = c->qb;
-    if (const char* e = std::getenv("OSB_FUSED_TILE")) c->fused_tile = std::max(0, std::min(CF_NTILES - 1, std::atoi(e)));
-    if (const char* e = std::getenv("OSB_XCD_REMAP")) c->xcd_remap = std::max(0, std::min(2, std::atoi(e)));
-    if (const char* e = std::getenv("OSB_GRAPH")) c->use_graph = std::atoi(e) != 0;
-    if (const char* e = std::getenv("OSB_FACE_KERNEL")) c->face_tiled = std::strcmp(e, "tile") == 0;
-    if (const char* e = std::getenv("OSB_SHOCK_FUSED")) c->shock_fused = std::strcmp(e, "0") != 0;
-    if (const char* e = std::getenv("OSB_FUSED_FW")) {
-        const int w = std::atoi(e);
-        if (w == 16 || w == 32) c->fused_fw = w;
-    }
-    if (c->face_tiled) c->shock_fused = false;      // the LDS-tiled face variant keeps the separate update
-    if (desc->flux_type >= OSB_FLUX_HLLC) {           // HLLC faces run their own kernel and the separate update
-        c->face_tiled = false;
-        c->shock_fused = false;
-    }
-    c->bc_periodic = true;
-    for (int d = 0; d < c->g.ndim; ++d)
-        if (desc->bc[d][0] != OSB_BC_PERIODIC || desc->bc[d][1] != OSB_BC_PERIODIC || c->g.n[d] < c->g.halo) c->bc_periodic = false;
-    for (int d = 0; d < c->g.ndim; ++d)
-        if (c->g.n[d] < 2 * c->g.halo) c->bc_in_stage = false;
-    if (const char* e = std::getenv("OSB_BC_KERNEL")) {
-        if (std::strcmp(e, "dir") == 0) c->bc_periodic = false;
-        if (std::strcmp(e, "pass") == 0) c->bc_in_stage = false;
-    }
     *out = c;
     return OSB_OK;
 }
@@ -1035,6 +1088,57 @@ int osb_get_state_device(osb_ctx* c, double* dev, int layout) {
 int osb_state_device_ptr(osb_ctx* c, int var, void** out) {
     if (!c || !out || var < 0 || var >= c->nv) return OSB_E_INVALID;
     *out = (void*)(c->q_cur + (long)var * c->g.Ptot);
+    return OSB_OK;
+}
+
+
+// Allocation-free stepping of a caller-owned block (no counterpart in the
+// reference).  q: padded state (nv x padded points, halos filled), advanced in
+// place; du: the RK-LS register (classic RK3: the saved state), same layout,
+// zero before a run's first step.  The context's other buffers are carved from
+// the library's scratch array; the steps are launched eagerly on `stream`.
+int osb_step_noalloc(const osb_problem_desc* desc, double* q, double* du, int nsteps, void* stream) {
+    std::string why;
+    int rc = validate(desc, why);
+    if (rc) return set_err(nullptr, rc, why);
+    if (!q || !du || nsteps < 0) return set_err(nullptr, OSB_E_INVALID, "osb_step_noalloc: null buffer or negative step count");
+    double* base = nullptr;
+    rc = noalloc_scratch(&base);
+    if (rc) return rc;
+    const AllocPlan ap = plan_alloc(desc);
+    const size_t need = ap.q + ap.R + ap.F + ap.kappa + ap.cr + ap.gr + OSB_SCAL_SIZE + (size_t)PART_BLOCKS * 4;
+    if (need > NOALLOC_SCRATCH_DOUBLES)
+        return set_err(nullptr, OSB_E_UNSUPPORTED,
+                       "osb_step_noalloc: this block needs " + std::to_string(need) + " doubles of scratch; the library holds " +
+                           std::to_string(NOALLOC_SCRATCH_DOUBLES) + " (a smaller block, or a context from osb_ctx_create)");
+    osb_ctx c;
+    init_ctx(&c, desc);
+    c.use_graph = false;
+    c.stream = (hipStream_t)stream;
+    c.qa = q;
+    c.du = du;
+    double* p = base;
+    auto carve = [&](double** dst, size_t n) { *dst = n ? p : nullptr; p += n; };
+    carve(&c.qb, ap.q);
+    carve(&c.R, ap.R);
+    carve(&c.F, ap.F);
+    carve(&c.kappa, ap.kappa);
+    carve(&c.cr, ap.cr);
+    carve(&c.gr, ap.gr);
+    carve(&c.scal, OSB_SCAL_SIZE);
+    c.part_blocks = PART_BLOCKS;
+    carve(&c.part, (size_t)PART_BLOCKS * 4);
+    if (c.shock && desc->flux_type == OSB_FLUX_GLF) c.c.glf = c.scal + OSB_SCAL_GLF;
+    // every buffer of a new context starts zeroed; the scratch keeps the last call's contents
+    k_noalloc_clear<<<std::min<long>(nblocks((long)need), 2048), BLOCK, 0, c.stream>>>(base, (long)need);
+    k_set_dt<<<1, 64, 0, c.stream>>>(c.scal, desc->dt);
+    c.q_cur = c.qa;
+    c.q_prev = c.qb;
+    for (int i = 0; i < nsteps; ++i) do_step(&c);
+    const long n = (long)c.nv * c.g.Ptot;
+    if (c.q_cur != q) k_copy_all<<<std::min<long>(nblocks(n), 4096), BLOCK, 0, c.stream>>>(q, c.q_cur, n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_err(nullptr, OSB_E_DEVICE, std::string("osb_step_noalloc launch: ") + hipGetErrorString(e));
     return OSB_OK;
 }
 

@@ -285,6 +285,9 @@ class Library:
             L.osb_batch_get_stats.argtypes = [vp, C.c_int, C.POINTER(StatsC)]
             L.osb_batch_autotune.argtypes = [vp]
             L.osb_batch_describe.argtypes = [vp, C.c_char_p, C.c_int]
+        self.has_noalloc = hasattr(L, "osb_step_noalloc")
+        if self.has_noalloc:
+            L.osb_step_noalloc.argtypes = [C.POINTER(ProblemDescC), vp, vp, C.c_int, vp]
         L.osb_estimate_bytes.argtypes = [C.POINTER(ProblemDescC), C.POINTER(C.c_int64)]
         L.osb_estimate_bytes.restype = C.c_int
         self.has_pool = hasattr(L, "osb_pool_create")
@@ -310,6 +313,16 @@ class Library:
 
     def pool(self, hbm_budget_bytes: int, max_solves: int = 64, nstreams: int = 4) -> "Pool":
         return Pool(self, hbm_budget_bytes, max_solves, nstreams)
+
+    def step_noalloc(self, desc: ProblemDesc, q_ptr: int, du_ptr: int, nsteps: int, stream_ptr: int = 0):
+        """osb_step_noalloc: advance a caller-owned padded state (device
+        pointer q_ptr) and RK register (du_ptr) by nsteps on the given HIP
+        stream without any device allocation (see include/osb.h)."""
+        cd = desc.to_c()
+        rc = self.lib.osb_step_noalloc(C.byref(cd), C.c_void_p(q_ptr), C.c_void_p(du_ptr), int(nsteps), C.c_void_p(stream_ptr))
+        if rc != OK:
+            msg = self.lib.osb_last_error(None)
+            raise OSBError(rc, msg.decode() if msg else "")
 
     def estimate_bytes(self, desc: ProblemDesc) -> int:
         cd = desc.to_c()

@@ -1559,6 +1559,7 @@ int osb_nan_check(osb_ctx* c) {
 int osb_state_device_ptr(osb_ctx* c, int, void**) { return fail(c, OSB_E_UNSUPPORTED, "oracle has no device buffers"); }
 int osb_set_stream(osb_ctx* c, void*) { return fail(c, OSB_E_UNSUPPORTED, "oracle has no streams"); }
 int osb_set_state_device(osb_ctx* c, const double*, int) { return fail(c, OSB_E_UNSUPPORTED, "oracle has no device buffers"); }
+int osb_step_noalloc(const osb_problem_desc*, double*, double*, int, void*) { return fail(nullptr, OSB_E_UNSUPPORTED, "oracle has no device buffers"); }
 int osb_get_state_device(osb_ctx* c, double*, int) { return fail(c, OSB_E_UNSUPPORTED, "oracle has no device buffers"); }
 
 }  // extern "C"
