@@ -38,6 +38,7 @@ class Request:
     slot: Optional[int] = None
     result: Optional[np.ndarray] = None
     iters: int = 0
+    finite: bool = True             # False: the solve's state went non-finite (failed alone)
 
     @property
     def time_to_solution(self) -> float:
@@ -92,6 +93,12 @@ class ServeReport:
     peak_bytes: int
     extra: dict = field(default_factory=dict)
 
+    @property
+    def failed(self) -> List[Request]:
+        """Requests whose state went non-finite (each fails alone: instances
+        share launches but not data, so the others are unaffected)."""
+        return [r for r in self.requests if not r.finite]
+
 
 def serve(lib: nat.Library, requests: List[Request], hbm_budget_bytes: int, max_solves: int = 32,
           nstreams: int = 4, chunk: int = 25, keep_results: bool = False, realtime: bool = True,
@@ -144,6 +151,10 @@ def serve(lib: nat.Library, requests: List[Request], hbm_budget_bytes: int, max_
                     q, st = pool.fetch(slot)
                     r.finished = time.perf_counter() - t0
                     r.iters = st.iter
+                    # the analogue of the generated loop's ops_NaNcheck on rho
+                    # (helperfunctions.py:201-214), over the whole state and
+                    # failing this request only
+                    r.finite = bool(np.isfinite(q).all())
                     if keep_results:
                         r.result = q
                     pool.release(slot)
