@@ -45,7 +45,6 @@ struct Group {
     int cap[BF_COUNT];                     // persistent grid size; 0 = one block per virtual block
     float tuned_us[BF_COUNT];
     int parity = 0;
-    bool tiled = false;                    // tiled face-flux kernel (the members' choice)
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     Group() { for (int f = 0; f < BF_COUNT; ++f) { cap[f] = 0; tuned_us[f] = 0.f; } }
     int n() const { return (int)members.size(); }
@@ -89,23 +88,14 @@ int berr(osb_batch* b, int code, const std::string& m) {
         }                                                                                     \
     } while (0)
 
-template <int DIR>
-long face_family_tiles(const Geom& g) {
-    switch (g.ndim) {
-        case 1: return face_tiles<1, DIR>(g);
-        case 2: return face_tiles<2, DIR>(g);
-        default: return face_tiles<3, DIR>(g);
-    }
-}
-
 // work of family f for one instance, in threads (BLOCK threads = one virtual
-// block; with tiled face fluxes a face-flux virtual block is one tile)
-long family_work(const osb_ctx* c, int f, bool tiled) {
+// block)
+long family_work(const osb_ctx* c, int f) {
     const Geom& g = c->g;
     switch (f) {
-        case BF_FACE0: return tiled ? face_family_tiles<0>(g) * BLOCK : face_flux_work<3, 0>(g);
-        case BF_FACE1: return g.ndim > 1 ? (tiled ? face_family_tiles<1>(g) * BLOCK : face_flux_work<3, 1>(g)) : 0;
-        case BF_FACE2: return g.ndim > 2 ? (tiled ? face_family_tiles<2>(g) * BLOCK : face_flux_work<3, 2>(g)) : 0;
+        case BF_FACE0: return face_flux_work<3, 0>(g);
+        case BF_FACE1: return g.ndim > 1 ? face_flux_work<3, 1>(g) : 0;
+        case BF_FACE2: return g.ndim > 2 ? face_flux_work<3, 2>(g) : 0;
         case BF_BC0: return bc_work<0>(g);
         case BF_BC1: return g.ndim > 1 ? bc_work<1>(g) : 0;
         case BF_BC2: return g.ndim > 2 ? bc_work<2>(g) : 0;
@@ -146,7 +136,6 @@ int build_groups(osb_batch* b) {
         gp->members.push_back(i);
         gp->any_cfl = gp->any_cfl || c->d.cfl > 0;
         gp->any_glf = gp->any_glf || c->c.glf != nullptr;
-        gp->tiled = c->face_tiled;
     }
     for (auto& gr : b->groups) {
         const int n = gr.n();
@@ -155,7 +144,7 @@ int build_groups(osb_batch* b) {
             int acc = 0;
             for (int m = 0; m < n; ++m) {
                 gr.offs[(size_t)f * (n + 1) + m] = acc;
-                const long vb = vblocks(family_work(b->ctx[gr.members[m]], f, gr.tiled));
+                const long vb = vblocks(family_work(b->ctx[gr.members[m]], f));
                 if (acc + vb > 0x7fffffffL) return berr(b, OSB_E_INVALID, "batch too large for 32-bit block offsets");
                 acc += (int)vb;
             }
@@ -210,14 +199,6 @@ unsigned grid_of(const Group& gr, int f) {
 template <int NDIM, int DIR, typename T>
 void launch_face(hipStream_t st, const Group& gr, int parity, unsigned grid) {
     const int f = BF_FACE0 + DIR;
-    if (gr.tiled) {
-        switch (gr.sch) {
-            case SCH_WENOZ: kb_face_flux_tile<NDIM, DIR, SCH_WENOZ, T><<<grid, BLOCK, 0, st>>>(gr.d_items, gr.dof(f), gr.n(), parity); break;
-            case SCH_TENO5: kb_face_flux_tile<NDIM, DIR, SCH_TENO5, T><<<grid, BLOCK, 0, st>>>(gr.d_items, gr.dof(f), gr.n(), parity); break;
-            default: kb_face_flux_tile<NDIM, DIR, SCH_TENO6, T><<<grid, BLOCK, 0, st>>>(gr.d_items, gr.dof(f), gr.n(), parity); break;
-        }
-        return;
-    }
     switch (gr.sch) {
         case SCH_WENOZ: kb_face_flux<NDIM, DIR, SCH_WENOZ, T><<<grid, BLOCK, 0, st>>>(gr.d_items, gr.dof(f), gr.n(), parity); break;
         case SCH_TENO5: kb_face_flux<NDIM, DIR, SCH_TENO5, T><<<grid, BLOCK, 0, st>>>(gr.d_items, gr.dof(f), gr.n(), parity); break;
@@ -614,7 +595,6 @@ int signature_group(osb_pool* p, const osb_ctx* c) {
     p->groups.emplace_back();
     PoolGroup& pg = p->groups.back();
     pg.g.ndim = c->g.ndim; pg.g.sch = c->sch; pg.g.mixed = c->mixed; pg.g.time = c->d.time_scheme; pg.g.stages = stages;
-    pg.g.tiled = c->face_tiled;
     pg.g.fused = c->shock_fused;
     pg.stream = p->streams[p->next_stream++ % p->streams.size()];
     if (hipEventCreateWithFlags(&pg.ev, hipEventDisableTiming) != hipSuccess) pg.ev = nullptr;
@@ -662,7 +642,7 @@ int launch_group_chunk(osb_pool* p, PoolGroup& pg, const std::vector<int>& membe
         long acc = 0;
         for (int m = 0; m < n; ++m) {
             g.offs[(size_t)f * (n + 1) + m] = (int)acc;
-            acc += vblocks(family_work(p->slots[members[m]].c, f, g.tiled));
+            acc += vblocks(family_work(p->slots[members[m]].c, f));
             if (acc > 0x7fffffffL) return perr(p, OSB_E_INVALID, "pool group too large for 32-bit block offsets");
         }
         g.offs[(size_t)f * (n + 1) + n] = (int)acc;

@@ -50,17 +50,6 @@ __global__ void __launch_bounds__(BLOCK) kb_face_flux(const BItem* __restrict__ 
     }
 }
 
-// tiled face fluxes: one virtual block is one tile of the item
-template <int NDIM, int DIR, int SCH, typename T>
-__global__ void __launch_bounds__(BLOCK) kb_face_flux_tile(const BItem* __restrict__ items, const int* __restrict__ offs, int n,
-                                                           int parity) {
-    OSB_BATCH_LOOP {
-        const int i = find_item(offs, n, vb);
-        const BItem& it = items[i];
-        face_flux_tile<NDIM, DIR, SCH, T>(vb - offs[i], it.q[parity], it.F + (long)DIR * (NDIM + 2) * it.g.Ptot, it.g, it.c);
-    }
-}
-
 // last direction's faces fused with the update: one virtual block is one tile
 template <int NDIM, int SCH, typename T, int TIME>
 __global__ void __launch_bounds__(BLOCK) kb_face_resid(const BItem* __restrict__ items, const int* __restrict__ offs, int n,

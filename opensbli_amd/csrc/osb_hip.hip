@@ -98,15 +98,6 @@ template <int NDIM, int DIR, typename T>
 void launch_flux_t(osb_ctx* c, const double* q) {
     double* F = c->F + (long)DIR * c->nv * c->g.Ptot;
     Timed tm(c, PF_FACE_FLUX);
-    if (c->face_tiled) {
-        const unsigned nt = (unsigned)face_tiles<NDIM, DIR>(c->g);
-        switch (c->sch) {
-            case SCH_WENOZ: k_face_flux_tile<NDIM, DIR, SCH_WENOZ, T><<<nt, BLOCK, 0, c->stream>>>(q, F, c->g, c->c); break;
-            case SCH_TENO5: k_face_flux_tile<NDIM, DIR, SCH_TENO5, T><<<nt, BLOCK, 0, c->stream>>>(q, F, c->g, c->c); break;
-            default: k_face_flux_tile<NDIM, DIR, SCH_TENO6, T><<<nt, BLOCK, 0, c->stream>>>(q, F, c->g, c->c); break;
-        }
-        return;
-    }
     long work = interior_points(c->g) / c->g.n[DIR] * (long)(c->g.n[DIR] + 1);
     switch (c->sch) {
         case SCH_WENOZ: k_face_flux<NDIM, DIR, SCH_WENOZ, T><<<nblocks(work), BLOCK, 0, c->stream>>>(q, F, c->g, c->c); break;
@@ -201,8 +192,7 @@ void launch_face_resid_w(osb_ctx* c, const double* q, double* qn, double A, doub
 template <int NDIM, int TIME, typename T>
 void launch_face_resid(osb_ctx* c, const double* q, double* qn, double A, double B) {
     Timed tm(c, PF_FACE_RESID);
-    if (NDIM == 1 || c->fused_fw != 16) launch_face_resid_w<NDIM, TIME, T, FUSED_FW>(c, q, qn, A, B);
-    else launch_face_resid_w<NDIM, TIME, T, 16>(c, q, qn, A, B);
+    launch_face_resid_w<NDIM, TIME, T, FUSED_FW>(c, q, qn, A, B);
 }
 
 template <int NDIM, int TIME>
@@ -682,19 +672,8 @@ void init_ctx(osb_ctx* c, const osb_problem_desc* desc) {
     }
 
     if (const char* e = std::getenv("OSB_FUSED_TILE")) c->fused_tile = std::max(0, std::min(CF_NTILES - 1, std::atoi(e)));
-    if (const char* e = std::getenv("OSB_XCD_REMAP")) c->xcd_remap = std::max(0, std::min(2, std::atoi(e)));
     if (const char* e = std::getenv("OSB_GRAPH")) c->use_graph = std::atoi(e) != 0;
-    if (const char* e = std::getenv("OSB_FACE_KERNEL")) c->face_tiled = std::strcmp(e, "tile") == 0;
-    if (const char* e = std::getenv("OSB_SHOCK_FUSED")) c->shock_fused = std::strcmp(e, "0") != 0;
-    if (const char* e = std::getenv("OSB_FUSED_FW")) {
-        const int w = std::atoi(e);
-        if (w == 16 || w == 32) c->fused_fw = w;
-    }
-    if (c->face_tiled) c->shock_fused = false;      // the LDS-tiled face variant keeps the separate update
-    if (desc->flux_type >= OSB_FLUX_HLLC) {           // HLLC faces run their own kernel and the separate update
-        c->face_tiled = false;
-        c->shock_fused = false;
-    }
+    if (desc->flux_type >= OSB_FLUX_HLLC) c->shock_fused = false;   // HLLC faces run their own kernel and the separate update
     c->bc_periodic = true;
     for (int d = 0; d < c->g.ndim; ++d)
         if (desc->bc[d][0] != OSB_BC_PERIODIC || desc->bc[d][1] != OSB_BC_PERIODIC || c->g.n[d] < c->g.halo) c->bc_periodic = false;

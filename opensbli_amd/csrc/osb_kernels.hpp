@@ -214,124 +214,6 @@ __global__ void __launch_bounds__(BLOCK) k_filter_update(const double* __restric
     }
 }
 
-// ---------------------------------------------------------------------------
-// Interface fluxes along DIR, tiled: a workgroup owns FL consecutive faces on
-// each of FW lines.  Phase 1 evaluates the primitives and physical flux of
-// every point the tile's stencils touch once and stages them in LDS; phase 2
-// evaluates one face per thread from LDS.  DIR = 0: FL = 64 faces x FW = 4
-// lines (1-D: 256 x 1), consecutive threads along x.  DIR > 0: FW = 16 x
-// columns x FL = 16 faces, consecutive threads along x.  Same arithmetic as
-// face_flux, so the fluxes are identical.
-// ---------------------------------------------------------------------------
-template <int NDIM, int DIR>
-struct FaceTile {
-    static constexpr int FL = DIR == 0 ? (NDIM == 1 ? 256 : 64) : 16;   // faces per line
-    static constexpr int FW = DIR == 0 ? (NDIM == 1 ? 1 : 4) : 16;      // lines
-    static constexpr int NPL = FL + 5;                                  // points per line
-    static constexpr int NPT = NPL * FW;
-    static_assert(FL * FW == BLOCK, "one face per thread");
-};
-
-template <int NDIM, int DIR>
-__host__ __device__ inline long face_tiles(const Geom& g) {
-    using FT = FaceTile<NDIM, DIR>;
-    const long nseg = (g.n[DIR] + 1 + FT::FL - 1) / FT::FL;
-    if (DIR == 0) return nseg * ((g.n[1] + FT::FW - 1) / FT::FW) * g.n[2];
-    const long ncol = (g.n[0] + FT::FW - 1) / FT::FW;
-    return ncol * nseg * (DIR == 1 ? g.n[2] : g.n[1]);
-}
-
-template <int NDIM, int DIR, int SCH, typename T>
-__device__ __forceinline__ void face_flux_tile(long tile, const double* __restrict__ q, double* __restrict__ F, const Geom& g,
-                                               const Consts& c) {
-    constexpr int NV = NDIM + 2;
-    using FT = FaceTile<NDIM, DIR>;
-    constexpr int FL = FT::FL, FW = FT::FW, NPL = FT::NPL, NPT = FT::NPT;
-    constexpr bool FAST = !std::is_same<T, double>::value;
-    __shared__ double s_v[NV][NPT];
-    __shared__ double s_f[NV][NPT];
-    __shared__ double s_uu[NDIM][NPT];
-    __shared__ double s_p[NPT];
-    __shared__ double s_a[NPT];
-    const int nd = g.n[DIR];
-    // tile -> first face fs along DIR, and the lines
-    const long nseg = (nd + 1 + FL - 1) / FL;
-    int fs, a0, a1;                 // a0: first line / column, a1: the remaining index
-    if (DIR == 0) {
-        const long nlb = (g.n[1] + FW - 1) / FW;
-        fs = (int)(tile % nseg) * FL - 1;
-        const long r = tile / nseg;
-        a0 = (int)(r % nlb) * FW;   // first j
-        a1 = (int)(r / nlb);        // k
-    } else {
-        const long ncol = (g.n[0] + FW - 1) / FW;
-        a0 = (int)(tile % ncol) * FW;   // first i
-        const long r = tile / ncol;
-        fs = (int)(r % nseg) * FL - 1;
-        a1 = (int)(r / nseg);       // k (DIR 1) or j (DIR 2)
-    }
-    // point (line w, position l along DIR) -> LDS slot and grid coordinates
-    auto slot = [&](int w, int l) { return DIR == 0 ? l + NPL * w : w + FW * l; };
-    auto coords = [&](int w, int l, int& i, int& j, int& k) {
-        const int pd = fs - 2 + l;
-        if (DIR == 0) { i = pd; j = a0 + w; k = a1; }
-        else if (DIR == 1) { i = a0 + w; j = pd; k = a1; }
-        else { i = a0 + w; j = a1; k = pd; }
-    };
-    __syncthreads();                // the previous tile of a persistent loop is done with LDS
-    for (int e = threadIdx.x; e < NPT; e += BLOCK) {
-        const int w = DIR == 0 ? e / NPL : e % FW;
-        const int l = DIR == 0 ? e % NPL : e / FW;
-        int i, j, k;
-        coords(w, l, i, j, k);
-        const int pd = fs - 2 + l;
-        const bool line_ok = DIR == 0 ? (j < g.n[1]) : (i < g.n[0]);
-        if (!line_ok || pd >= nd + g.halo) continue;     // beyond the stored halo: never read by a valid face
-        const long x = gidx(g, i, j, k);
-        double v[NV], f[NV], uu[NDIM], p, a;
-#pragma unroll
-        for (int m = 0; m < NV; ++m) v[m] = q[(long)m * g.Ptot + x];
-        point_prims<NDIM, DIR, FAST>(v, c, uu, p, f, a);
-        const int sl = slot(w, l);
-#pragma unroll
-        for (int m = 0; m < NV; ++m) { s_v[m][sl] = v[m]; s_f[m][sl] = f[m]; }
-#pragma unroll
-        for (int d = 0; d < NDIM; ++d) s_uu[d][sl] = uu[d];
-        s_p[sl] = p;
-        s_a[sl] = a;
-    }
-    __syncthreads();
-    const int lf = DIR == 0 ? (int)threadIdx.x % FL : (int)threadIdx.x / FW;
-    const int w = DIR == 0 ? (int)threadIdx.x / FL : (int)threadIdx.x % FW;
-    const int f = fs + lf;
-    int i, j, k;
-    coords(w, lf + 2, i, j, k);     // point f (face f + 1/2 is stored at point f)
-    const bool line_ok = DIR == 0 ? (j < g.n[1]) : (i < g.n[0]);
-    // no early return: in the persistent batched loop this thread has further tiles
-    if (line_ok && f < nd) {
-        double out[NV];
-        face_core<NDIM, DIR, SCH, T>(
-            [&](int s, double (&v)[NV], double (&ff)[NV], double (&uu)[NDIM], double& p, double& a) {
-                const int sl = slot(w, lf + s);
-#pragma unroll
-                for (int m = 0; m < NV; ++m) { v[m] = s_v[m][sl]; ff[m] = s_f[m][sl]; }
-#pragma unroll
-                for (int d = 0; d < NDIM; ++d) uu[d] = s_uu[d][sl];
-                p = s_p[sl];
-                a = s_a[sl];
-            },
-            c, out);
-        const long x = gidx(g, i, j, k);
-#pragma unroll
-        for (int v = 0; v < NV; ++v) F[(long)v * g.Ptot + x] = out[v];
-    }
-}
-
-template <int NDIM, int DIR, int SCH, typename T>
-__global__ void __launch_bounds__(BLOCK) k_face_flux_tile(const double* __restrict__ q, double* __restrict__ F, Geom g, Consts c) {
-    face_flux_tile<NDIM, DIR, SCH, T>(blockIdx.x, q, F, g, c);
-}
-
 // flux difference at interior point x: -sum_d (F_d(x) - F_d(x - s_d)) / Delta_d
 template <int NDIM>
 __device__ __forceinline__ double flux_difference(const double* __restrict__ F, long nvP, long P, int v, long x, const Geom& g,

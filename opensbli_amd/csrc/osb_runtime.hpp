@@ -55,12 +55,9 @@ struct osb_ctx {
     int rk_stages = 3;              // RK-LS stages and coefficients (rk_LS.py:97-132)
     double rkA[5] = {0}, rkB[5] = {0};
     int fused_tile = 0;             // index into CF_TILES (autotuned)
-    int xcd_remap = 2;              // XCD-aware workgroup-to-tile order: 2 = 8x4x2 supertiles per XCD range,
-                                    // 1 = x-fastest ranges, 0 = off (OSB_XCD_REMAP)
+    int xcd_remap = 2;              // XCD-aware workgroup-to-tile order: 8x4x2 supertiles per XCD range
 
-    bool face_tiled = false;        // face fluxes one thread per face from L2/HBM; OSB_FACE_KERNEL=tile: LDS-tiled variant
-    int fused_fw = 32;              // lines per fused tile: 32 or 16 (OSB_FUSED_FW)
-    bool shock_fused = true;        // shock-only stages: last direction's faces fused with the update (OSB_SHOCK_FUSED=0: separate)
+    bool shock_fused = true;        // shock-only stages: last direction's faces fused with the update (HLLC: separate)
     bool bc_periodic = false;       // every face periodic and n >= halo: one-pass halo fill (OSB_BC_KERNEL=dir: per direction)
     bool bc_in_stage = true;        // with bc_periodic, the 3-D tile stage kernel writes the halos itself (n >= 2 halo;
                                     // OSB_BC_KERNEL=pass: the separate one-pass fill)

@@ -52,9 +52,9 @@ class LFWeno(Scheme):
     def __init__(self, order, physics=None, averaging=None, shock_filter=None, formulation="JS",
                  conservative=True, flux_type="LLF", flux_split=True, species=None):
         if flux_type not in ("LLF", "GLF"):
-            raise ValueError("Please select either LLF or GLF for the flux-splitting.")
+            raise ValueError(f"flux_type must be 'LLF' or 'GLF' for Lax-Friedrichs splitting, not {flux_type!r}")
         if order % 2 == 0:
-            raise ValueError(f"Please set an odd-order for the WENO scheme, currently {order} is not supported")
+            raise ValueError(f"WENO schemes have odd orders; order {order} is not available")
         super().__init__("WenoDerivative", order)
         self.physics = physics
         self.averaging = averaging
@@ -75,9 +75,9 @@ class HLLCWeno(Scheme):
     def __init__(self, order, physics=None, averaging=None, shock_filter=None, formulation="JS",
                  conservative=True, flux_type="HLLC", flux_split=False):
         if order % 2 == 0:
-            raise ValueError(f"Please set an odd-order for the WENO scheme, currently {order} is not supported")
+            raise ValueError(f"WENO schemes have odd orders; order {order} is not available")
         if flux_type not in ("HLLC", "HLLC-LM"):
-            raise ValueError("Please select either HLLC or HLLC-LM for the flux-splitting.")
+            raise ValueError(f"flux_type must be 'HLLC' or 'HLLC-LM' for the HLLC Riemann flux, not {flux_type!r}")
         super().__init__("WenoDerivative", order)
         self.physics = physics
         self.averaging = averaging
@@ -94,7 +94,7 @@ class LFTeno(Scheme):
     def __init__(self, order, formulation=None, physics=None, averaging=None, sensor=None, store_sensor=False,
                  conservative=True, flux_type="LLF", flux_split=True, passive_scalar=False):
         if flux_type not in ("LLF", "GLF"):
-            raise ValueError("Please select either LLF or GLF for the flux-splitting.")
+            raise ValueError(f"flux_type must be 'LLF' or 'GLF' for Lax-Friedrichs splitting, not {flux_type!r}")
         super().__init__("TenoDerivative", order)
         self.formulation = formulation
         self.physics = physics
@@ -153,7 +153,7 @@ class RungeKuttaLS(Scheme):
                          0.384471116121269]),
                 }
                 if n not in tables:
-                    raise ValueError("The 3rd order RK-SSP is defined for 3, 4, or 5 stages.")
+                    raise ValueError(f"third-order SSP low-storage Runge-Kutta has 3, 4 or 5 stages, not {n}")
                 self.rk_a, self.rk_b = tables[n]
             else:
                 n = 3
@@ -164,7 +164,7 @@ class RungeKuttaLS(Scheme):
             self.rk_a = [0.0, -0.4178904745, -1.192151694643, -1.697784692471, -1.514183444257]
             self.rk_b = [0.1496590219993, 0.3792103129999, 0.8229550293869, 0.6994504559488, 0.1530572479681]
         else:
-            raise NotImplementedError("Only 3rd and 4th order RK schemes are currently implemented.")
+            raise NotImplementedError(f"low-storage Runge-Kutta of order {order}: orders 3 and 4 are available")
         self.stages = n
         self.n_stages = n
 
@@ -179,7 +179,7 @@ class RungeKutta(Scheme):
 
     def __init__(self, order, constant_dt=None):
         if order != 3:
-            raise NotImplementedError("RungeKutta is implemented for order 3")
+            raise NotImplementedError(f"classic Runge-Kutta of order {order}: order 3 is available")
         super().__init__("RungeKutta", order)
         self.constant_dt = constant_dt
 
@@ -240,7 +240,7 @@ class ExtrapolationBC(BoundaryConditionBase):
     def __init__(self, direction, side, order, scheme=None, plane=True):
         super().__init__(direction, side, plane)
         if order not in (0, 1):
-            raise ValueError("Only zeroth order and linear extrapolation currently implemented.")
+            raise ValueError(f"extrapolation order {order}: orders 0 (constant) and 1 (linear) are available")
         self.order = order
         self.modification_scheme = scheme if scheme else Carpenter()
 
@@ -274,8 +274,7 @@ class NS_Split:
                  energy_formulation="none", debug=False):
         known = ("Divergence", "Blaisdell", "Feiereisen", "Jameson", "Kok", "KGP", "KEEP")
         if split_type not in known:
-            raise NotImplementedError(
-                "Only Divergence, Blasidell, Feiereisen, Jameson, Kok, KGP, and KEEP splitting methods are implemented.")
+            raise NotImplementedError(f"split form {split_type!r} is not one of {', '.join(known)}")
         self.split_type = split_type
         self.ndim = ndim
         self.viscosity = viscosity
@@ -454,12 +453,12 @@ class SimulationMonitor:
                  NaN_check=None, output_file="output.log"):
         arrays, probe_locations = list(arrays), list(probe_locations)
         if len(arrays) != len(probe_locations):
-            raise ValueError("The number of arrays must equal the number of probe locations.")
+            raise ValueError(f"one probed array per probe location: {len(arrays)} arrays for {len(probe_locations)} locations")
         self.blocks = _flatten([blocks])
         nd = self.blocks[0].ndim
         for loc in probe_locations:
             if isinstance(loc, tuple) and len(loc) != nd:
-                raise ValueError("The number of probe indices (i,j,k) must match the dimensions of the problem (block.ndim).")
+                raise ValueError(f"probe location {loc} has {len(loc)} indices for a {nd}-D block")
         self.arrays = arrays
         self.probe_locations = probe_locations
         self.frequency = int(print_frequency)
