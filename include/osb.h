@@ -209,6 +209,12 @@ int  osb_nan_check(osb_ctx* ctx);
 int  osb_profile_enable(osb_ctx* ctx, int enable);
 int  osb_profile_read(osb_ctx* ctx, int idx, char* name, int name_len, double* total_ms, int64_t* launches);
 
+/* The context's kernel path and launch shapes as JSON in buf (HIP library:
+ * the fused 3-D stage kernel's tile shape, timed at the first step unless
+ * OSB_FUSED_TILE forces one, with each candidate's microseconds per launch).
+ * No counterpart in the reference, whose generated code fixes its kernels. */
+int  osb_describe(osb_ctx* ctx, char* buf, int len);
+
 /* Device buffers (HIP library only): pointer to the FP64 conservative
  * variable `var` (padded layout) for zero-copy interop.                     */
 int  osb_state_device_ptr(osb_ctx* ctx, int var, void** out);

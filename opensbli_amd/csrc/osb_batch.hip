@@ -8,8 +8,12 @@
 //   [CFL: zero, wave speed, dt]  [RK3: save q]
 //   per stage: face fluxes per direction, flux difference + RK update, BCs per direction
 // That sequence is captured once per parity into a hipGraph and replayed.
-// Other instances (3-D central / viscous, which use the fused stage kernel)
-// are stepped one after another on the same stream.
+// 3-D central instances (FP64, RK-LS, every face periodic) form groups by
+// (split form, viscosity, stages): per stage one launch of the fused stage
+// kernel over the tiles of every member, which also writes the wrapped halo
+// images.  Other instances (3-D shock capturing with viscous terms, mixed
+// precision or boundary faces on the central path, the WENO filter) are
+// stepped one after another on the same stream.
 //
 // Launch shapes: the batched kernels are persistent over virtual blocks; the
 // grid of each face-flux family is picked by timing a few candidates on the
@@ -33,7 +37,11 @@ enum BFam { BF_FACE0, BF_FACE1, BF_FACE2, BF_RESID, BF_BC0, BF_BC1, BF_BC2, BF_W
 const char* const kBFamName[BF_COUNT] = {"face_flux_x", "face_flux_y", "face_flux_z", "resid_rk", "bc_x", "bc_y", "bc_z",
                                          "wavespeed", "save_state"};
 
+enum GroupKind { GK_SHOCK = 0, GK_CENTRAL = 1 };
+
 struct Group {
+    int kind = GK_SHOCK;                   // shock-capturing stages, or the fused 3-D central stage
+    int conv = 0, visc = 0;                // GK_CENTRAL: split form (CfConv) and viscosity model
     int ndim = 0, sch = 0, mixed = 0, time = 0, stages = 0;
     bool any_cfl = false;
     bool any_glf = false;
@@ -52,8 +60,36 @@ struct Group {
     int total(int f) const { return offs[(size_t)f * (members.size() + 1) + members.size()]; }
 };
 
-bool batchable(const osb_ctx* c) {
-    return c->shock && !c->central && !c->viscous && c->d.shock_filter == OSB_FILTER_NONE && c->d.flux_type <= OSB_FLUX_GLF;
+// GK_SHOCK, GK_CENTRAL, or -1: stepped alone
+int group_kind(const osb_ctx* c) {
+    if (c->shock && !c->central && !c->viscous && c->d.shock_filter == OSB_FILTER_NONE && c->d.flux_type <= OSB_FLUX_GLF)
+        return GK_SHOCK;
+    bool closure = false;
+    for (int d = 0; d < 3; ++d) closure = closure || c->c.carp[d] != 0;
+    if (c->g.ndim == 3 && c->central && !c->shock && !c->mixed && !closure && c->bc_periodic && c->bc_in_stage &&
+        c->d.shock_filter == OSB_FILTER_NONE && c->d.time_scheme == OSB_TIME_RKLS3)
+        return GK_CENTRAL;
+    return -1;
+}
+
+int cf_conv(const osb_ctx* c) {
+    return c->d.conv_scheme == OSB_CONV_CENTRAL_SKEW ? CF_SKEW : (c->d.conv_scheme == OSB_CONV_CENTRAL_KGP ? CF_KGP : CF_FEIEREISEN);
+}
+
+// same group: kind and the signature of that kind
+bool same_signature(const Group& gr, const osb_ctx* c, int kind, int stages) {
+    if (gr.kind != kind || gr.ndim != c->g.ndim || gr.time != c->d.time_scheme || gr.stages != stages || gr.mixed != (int)c->mixed)
+        return false;
+    if (kind == GK_CENTRAL) return gr.conv == cf_conv(c) && gr.visc == c->d.viscous;
+    return gr.sch == c->sch && gr.fused == c->shock_fused;
+}
+
+void set_signature(Group& gr, const osb_ctx* c, int kind, int stages) {
+    gr.kind = kind;
+    gr.ndim = c->g.ndim; gr.sch = c->sch; gr.mixed = c->mixed; gr.time = c->d.time_scheme; gr.stages = stages;
+    gr.fused = c->shock_fused;
+    gr.conv = cf_conv(c);
+    gr.visc = c->d.viscous;
 }
 
 inline unsigned vblocks(long work) { return (unsigned)((work + BLOCK - 1) / BLOCK); }
@@ -92,6 +128,11 @@ int berr(osb_batch* b, int code, const std::string& m) {
 // block)
 long family_work(const osb_ctx* c, int f) {
     const Geom& g = c->g;
+    if (group_kind(c) == GK_CENTRAL) {
+        if (f == BF_RESID) return kb_central_tiles(g) * BLOCK;           // one virtual block per tile
+        if (f == BF_WAVE) return (long)g.n[0] * g.n[1] * g.n[2];
+        return 0;
+    }
     switch (f) {
         case BF_FACE0: return face_flux_work<3, 0>(g);
         case BF_FACE1: return g.ndim > 1 ? face_flux_work<3, 1>(g) : 0;
@@ -121,17 +162,16 @@ void free_group(Group& gr) {
 int build_groups(osb_batch* b) {
     for (int i = 0; i < (int)b->ctx.size(); ++i) {
         const osb_ctx* c = b->ctx[i];
-        if (!batchable(c)) { b->solo.push_back(i); continue; }
+        const int kind = group_kind(c);
+        if (kind < 0) { b->solo.push_back(i); continue; }
         const int stages = c->d.time_scheme == OSB_TIME_RKLS3 ? c->rk_stages : 3;
         Group* gp = nullptr;
         for (auto& gr : b->groups)
-            if (gr.ndim == c->g.ndim && gr.sch == c->sch && gr.mixed == (int)c->mixed && gr.time == c->d.time_scheme &&
-                gr.stages == stages && gr.fused == c->shock_fused) { gp = &gr; break; }
+            if (same_signature(gr, c, kind, stages)) { gp = &gr; break; }
         if (!gp) {
             b->groups.emplace_back();
             gp = &b->groups.back();
-            gp->ndim = c->g.ndim; gp->sch = c->sch; gp->mixed = c->mixed; gp->time = c->d.time_scheme; gp->stages = stages;
-            gp->fused = c->shock_fused;
+            set_signature(*gp, c, kind, stages);
         }
         gp->members.push_back(i);
         gp->any_cfl = gp->any_cfl || c->d.cfl > 0;
@@ -248,6 +288,33 @@ void launch_bcs(hipStream_t st, const Group& gr, int parity) {
     if (NDIM > 2) kb_bc<2><<<grid_of(gr, BF_BC2), BLOCK, 0, st>>>(gr.d_items, gr.dof(BF_BC2), gr.n(), parity);
 }
 
+template <int CONV>
+void launch_central_v(hipStream_t st, const Group& gr, int parity, int stage) {
+    const unsigned grid = grid_of(gr, BF_RESID);
+    switch (gr.visc) {
+        case OSB_VISC_NONE: kb_central_fused<CONV, 0><<<grid, KB_CF_TX * KB_CF_TY * KB_CF_TZ, 0, st>>>(gr.d_items, gr.dof(BF_RESID), gr.n(), parity, stage); break;
+        case OSB_VISC_CONSTANT: kb_central_fused<CONV, 1><<<grid, KB_CF_TX * KB_CF_TY * KB_CF_TZ, 0, st>>>(gr.d_items, gr.dof(BF_RESID), gr.n(), parity, stage); break;
+        default: kb_central_fused<CONV, 2><<<grid, KB_CF_TX * KB_CF_TY * KB_CF_TZ, 0, st>>>(gr.d_items, gr.dof(BF_RESID), gr.n(), parity, stage); break;
+    }
+}
+
+// one time step of a 3-D central group: [CFL] then per stage the fused stage
+// kernel over every member's tiles (halo images included: no BC launches)
+void enqueue_central_step(hipStream_t st, const Group& gr, int parity) {
+    const int n = gr.n();
+    if (gr.any_cfl) {
+        kb_zero_wavespeed<<<(n + 63) / 64, 64, 0, st>>>(gr.d_items, n);
+        kb_wavespeed<3><<<grid_of(gr, BF_WAVE), BLOCK, 0, st>>>(gr.d_items, gr.dof(BF_WAVE), n, parity);
+        kb_dt_finalize<<<(n + 63) / 64, 64, 0, st>>>(gr.d_items, n);
+    }
+    for (int s = 0; s < gr.stages; ++s) {
+        if (gr.conv == CF_SKEW) launch_central_v<CF_SKEW>(st, gr, parity, s);
+        else if (gr.conv == CF_KGP) launch_central_v<CF_KGP>(st, gr, parity, s);
+        else launch_central_v<CF_FEIEREISEN>(st, gr, parity, s);
+        parity ^= 1;
+    }
+}
+
 // one time step of a group, starting at `parity`
 template <int NDIM>
 void enqueue_group_step_nd(hipStream_t st, const Group& gr, int parity) {
@@ -281,6 +348,10 @@ void enqueue_group_step_nd(hipStream_t st, const Group& gr, int parity) {
 }
 
 void enqueue_group_step(hipStream_t st, const Group& gr, int parity) {
+    if (gr.kind == GK_CENTRAL) {
+        enqueue_central_step(st, gr, parity);
+        return;
+    }
     switch (gr.ndim) {
         case 1: enqueue_group_step_nd<1>(st, gr, parity); break;
         case 2: enqueue_group_step_nd<2>(st, gr, parity); break;
@@ -310,6 +381,7 @@ int autotune(osb_batch* b) {
     B_TRY(b, hipEventCreate(&e0));
     B_TRY(b, hipEventCreate(&e1));
     for (auto& gr : b->groups) {
+        if (gr.kind != GK_SHOCK) continue;                            // the fused stage kernels change du: not timed
         for (int d = 0; d < gr.ndim - (gr.fused ? 1 : 0); ++d) {     // the fused family changes du: not timed
             const int f = BF_FACE0 + d;
             const int tot = gr.total(f);
@@ -499,14 +571,16 @@ int osb_batch_describe(osb_batch* b, char* buf, int len) {
     for (size_t k = 0; k < b->groups.size(); ++k) {
         const Group& gr = b->groups[k];
         char tmp[256];
-        std::snprintf(tmp, sizeof(tmp), "%s{\"ndim\": %d, \"scheme\": %d, \"mixed\": %d, \"time\": %d, \"stages\": %d, \"members\": %d, \"tuning\": {",
-                      k ? ", " : "", gr.ndim, gr.sch, gr.mixed, gr.time, gr.stages, gr.n());
+        std::snprintf(tmp, sizeof(tmp), "%s{\"kind\": \"%s\", \"ndim\": %d, \"scheme\": %d, \"mixed\": %d, \"time\": %d, \"stages\": %d, \"members\": %d, \"tuning\": {",
+                      k ? ", " : "", gr.kind == GK_CENTRAL ? "central" : "shock", gr.ndim, gr.kind == GK_CENTRAL ? gr.conv : gr.sch,
+                      gr.mixed, gr.time, gr.stages, gr.n());
         s += tmp;
         bool first = true;
         for (int f = 0; f < BF_COUNT; ++f) {
             if (gr.total(f) == 0 || (f >= BF_FACE0 + gr.ndim && f <= BF_FACE2)) continue;
             if (gr.fused && f == BF_FACE0 + gr.ndim - 1) continue;          // evaluated inside face_resid_rk
-            const char* name = (gr.fused && f == BF_RESID) ? "face_resid_rk" : kBFamName[f];
+            const char* name = gr.kind == GK_CENTRAL && f == BF_RESID ? "central_fused"
+                               : ((gr.fused && f == BF_RESID) ? "face_resid_rk" : kBFamName[f]);
             std::snprintf(tmp, sizeof(tmp), "%s\"%s\": {\"virtual_blocks\": %d, \"grid\": %u, \"us\": %.2f}", first ? "" : ", ",
                           name, gr.total(f), grid_of(gr, f), (double)gr.tuned_us[f]);
             s += tmp;
@@ -586,16 +660,12 @@ int perr(osb_pool* p, int code, const std::string& m) {
 
 int signature_group(osb_pool* p, const osb_ctx* c) {
     const int stages = c->d.time_scheme == OSB_TIME_RKLS3 ? c->rk_stages : 3;
-    for (int i = 0; i < (int)p->groups.size(); ++i) {
-        const Group& g = p->groups[i].g;
-        if (g.ndim == c->g.ndim && g.sch == c->sch && g.mixed == (int)c->mixed && g.time == c->d.time_scheme && g.stages == stages &&
-            g.fused == c->shock_fused)
-            return i;
-    }
+    const int kind = group_kind(c);
+    for (int i = 0; i < (int)p->groups.size(); ++i)
+        if (same_signature(p->groups[i].g, c, kind, stages)) return i;
     p->groups.emplace_back();
     PoolGroup& pg = p->groups.back();
-    pg.g.ndim = c->g.ndim; pg.g.sch = c->sch; pg.g.mixed = c->mixed; pg.g.time = c->d.time_scheme; pg.g.stages = stages;
-    pg.g.fused = c->shock_fused;
+    set_signature(pg.g, c, kind, stages);
     pg.stream = p->streams[p->next_stream++ % p->streams.size()];
     if (hipEventCreateWithFlags(&pg.ev, hipEventDisableTiming) != hipSuccess) pg.ev = nullptr;
     return (int)p->groups.size() - 1;
@@ -775,7 +845,7 @@ int osb_pool_admit(osb_pool* p, const osb_problem_desc* desc, const double* host
     PoolSolve& s = p->slots[sl];
     s = PoolSolve{};
     s.c = c;
-    s.group = batchable(c) ? signature_group(p, c) : -1;
+    s.group = group_kind(c) >= 0 ? signature_group(p, c) : -1;
     s.stream = s.group >= 0 ? p->groups[s.group].stream : p->streams[p->next_stream++ % p->streams.size()];
     r = osb_set_stream(c, s.stream);
     if (!r) r = osb_set_state(c, host_q0, OSB_LAYOUT_INTERIOR);

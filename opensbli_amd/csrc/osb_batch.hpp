@@ -8,6 +8,7 @@
 #pragma once
 
 #include "osb_kernels.hpp"
+#include "osb_central_fused.hpp"
 
 namespace osb {
 
@@ -158,6 +159,31 @@ __global__ void __launch_bounds__(BLOCK) kb_glf(const BItem* __restrict__ items,
         }
         if (on) glf_block_to<NDIM>(m, const_cast<double*>(it.c.glf));
     }
+}
+
+// 3-D central RK-LS stage of every item, all faces periodic: one virtual
+// block is one 16 x 4 x 4 tile of an item (x, then y, then z), evaluated by
+// the single-block kernel's tile code, which also writes the wrapped halo
+// images; the LDS box is reused by the workgroup's next tile after a barrier
+constexpr int KB_CF_TX = 16, KB_CF_TY = 4, KB_CF_TZ = 4;
+template <int CONV, int VISC>
+__global__ void __launch_bounds__(KB_CF_TX * KB_CF_TY * KB_CF_TZ, OSB_CF_WAVES)
+kb_central_fused(const BItem* __restrict__ items, const int* __restrict__ offs, int n, int parity, int stage) {
+    const int total_ = offs[n];
+    for (int vb = blockIdx.x; vb < total_; vb += gridDim.x) {
+        if (vb != (int)blockIdx.x) __syncthreads();
+        const int i = find_item(offs, n, vb);
+        const BItem& it = items[i];
+        const int tile = vb - offs[i];
+        const int ntx = (it.g.n[0] + KB_CF_TX - 1) / KB_CF_TX, nty = (it.g.n[1] + KB_CF_TY - 1) / KB_CF_TY;
+        const int tbx = tile % ntx, tby = (tile / ntx) % nty, tbz = tile / (ntx * nty);
+        cf_tile_stage<KB_CF_TX, KB_CF_TY, KB_CF_TZ, CONV, VISC, false, OSB_TIME_RKLS3>(
+            tbx * KB_CF_TX, tby * KB_CF_TY, tbz * KB_CF_TZ, it.q[parity], nullptr, it.du, it.q[parity ^ 1], it.scal, it.g, it.c,
+            it.rkA[stage], it.rkB[stage], 1);
+    }
+}
+__host__ inline long kb_central_tiles(const Geom& g) {
+    return (long)((g.n[0] + KB_CF_TX - 1) / KB_CF_TX) * ((g.n[1] + KB_CF_TY - 1) / KB_CF_TY) * ((g.n[2] + KB_CF_TZ - 1) / KB_CF_TZ);
 }
 
 static __global__ void kb_dt_finalize(const BItem* __restrict__ items, int n) {

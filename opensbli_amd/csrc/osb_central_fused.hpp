@@ -94,10 +94,14 @@ __host__ __device__ constexpr int cf_pair(int a, int b) { return a == 0 ? b - 1 
 // CONV: CfConv; VISC: 0 none, 1 constant, 2 Sutherland; FLUXRES: start R from
 // the face-flux difference (shock-capturing schemes); TIME: OSB_TIME_RKLS3 (1)
 // or OSB_TIME_RK3 (2).
+// One RK stage over the tile whose first point is interior point (i0, j0, k0):
+// the body of k_central_fused (one tile per workgroup) and of the batched
+// kb_central_fused (tiles of many blocks).
 template <int CF_TX, int CF_TY, int CF_TZ, int CONV, int VISC, bool FLUXRES, int TIME>
-__global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ, OSB_CF_WAVES)
-k_central_fused(const double* __restrict__ q, const double* __restrict__ F, double* __restrict__ reg, double* __restrict__ qout,
-                const double* __restrict__ scal, Geom g, Consts c, double A, double B, int xcd_remap, int wrap) {
+__device__ __forceinline__ void cf_tile_stage(const int i0, const int j0, const int k0, const double* __restrict__ q,
+                                              const double* __restrict__ F, double* __restrict__ reg,
+                                              double* __restrict__ qout, const double* __restrict__ scal, const Geom& g,
+                                              const Consts& c, const double A, const double B, const int wrap) {
     constexpr int NV = 5;
     constexpr int CF_BX = CF_TX + 4, CF_BY = CF_TY + 4, CF_BZ = CF_TZ + 4;
     constexpr int CF_BOX = CF_BX * CF_BY * CF_BZ;
@@ -109,26 +113,6 @@ k_central_fused(const double* __restrict__ q, const double* __restrict__ F, doub
     constexpr int NFIELD = 4 + (HAS_CONV ? 1 : 0) + (SUTH ? 1 : 0);
     __shared__ double s_f[NFIELD * CF_BOX];
 
-    // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs, so
-    // workgroup w runs on XCD w % 8.  Giving XCD x the contiguous range of
-    // tiles [x*T/8, (x+1)*T/8) keeps neighbouring tiles -- which share halo
-    // points -- in the same L2.
-    const int ntx = gridDim.x, nty = gridDim.y;
-    const int ntiles = ntx * nty * gridDim.z;
-    const int w = blockIdx.x + ntx * (blockIdx.y + nty * blockIdx.z);
-    int tile = w;
-    if (xcd_remap && (ntiles & 7) == 0) tile = (w & 7) * (ntiles >> 3) + (w >> 3);
-    int tbx = tile % ntx, tby = (tile / ntx) % nty, tbz = tile / (ntx * nty);
-    if (xcd_remap == 2 && (ntx & 7) == 0 && (nty & 3) == 0 && (gridDim.z & 1) == 0) {
-        // supertiles of 8 x 4 x 2 tiles (one XCD's resident workgroups) walked in
-        // x, y, z order: a squarer resident region, so less of its halo leaves L2
-        const int in = tile & 63, sup = tile >> 6;
-        const int sx = ntx >> 3, sy = nty >> 2;
-        tbx = ((sup % sx) << 3) + (in & 7);
-        tby = (((sup / sx) % sy) << 2) + ((in >> 3) & 3);
-        tbz = ((sup / (sx * sy)) << 1) + (in >> 5);
-    }
-    const int i0 = tbx * CF_TX, j0 = tby * CF_TY, k0 = tbz * CF_TZ;
     const long P = g.Ptot;
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
 
@@ -381,6 +365,33 @@ k_central_fused(const double* __restrict__ q, const double* __restrict__ F, doub
     }
     // every face periodic: the point's wrapped images in the halo of qout
     if (wrap) store_halo_images<NV>(qout, g, i, j, k, qv);
+}
+
+template <int CF_TX, int CF_TY, int CF_TZ, int CONV, int VISC, bool FLUXRES, int TIME>
+__global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ, OSB_CF_WAVES)
+k_central_fused(const double* __restrict__ q, const double* __restrict__ F, double* __restrict__ reg, double* __restrict__ qout,
+                const double* __restrict__ scal, Geom g, Consts c, double A, double B, int xcd_remap, int wrap) {
+    // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs, so
+    // workgroup w runs on XCD w % 8.  Giving XCD x the contiguous range of
+    // tiles [x*T/8, (x+1)*T/8) keeps neighbouring tiles -- which share halo
+    // points -- in the same L2.
+    const int ntx = gridDim.x, nty = gridDim.y;
+    const int ntiles = ntx * nty * gridDim.z;
+    const int w = blockIdx.x + ntx * (blockIdx.y + nty * blockIdx.z);
+    int tile = w;
+    if (xcd_remap && (ntiles & 7) == 0) tile = (w & 7) * (ntiles >> 3) + (w >> 3);
+    int tbx = tile % ntx, tby = (tile / ntx) % nty, tbz = tile / (ntx * nty);
+    if (xcd_remap == 2 && (ntx & 7) == 0 && (nty & 3) == 0 && (gridDim.z & 1) == 0) {
+        // supertiles of 8 x 4 x 2 tiles (one XCD's resident workgroups) walked in
+        // x, y, z order: a squarer resident region, so less of its halo leaves L2
+        const int in = tile & 63, sup = tile >> 6;
+        const int sx = ntx >> 3, sy = nty >> 2;
+        tbx = ((sup % sx) << 3) + (in & 7);
+        tby = (((sup / sx) % sy) << 2) + ((in >> 3) & 3);
+        tbz = ((sup / (sx * sy)) << 1) + (in >> 5);
+    }
+    cf_tile_stage<CF_TX, CF_TY, CF_TZ, CONV, VISC, FLUXRES, TIME>(tbx * CF_TX, tby * CF_TY, tbz * CF_TZ, q, F, reg, qout, scal, g,
+                                                                  c, A, B, wrap);
 }
 
 }  // namespace osb

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -348,6 +349,86 @@ int check_launch(osb_ctx* c) {
     return OSB_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Tile autotuning of the FP64 fused 3-D stage kernel.  At a context's first
+// step every tile shape of CF_TILES runs the stage kernel once untimed and
+// three times timed (HIP events on the context's stream) on the context's own
+// state.  The trial launches write only the first stage's output buffer and
+// the RK register: the first real stage overwrites the former, and the
+// register is cleared afterwards.  The fastest shape is kept and remembered
+// for the process per kernel variant and grid, so contexts of one shape (a
+// serving pool's) tune once.  OSB_FUSED_TILE=<index> forces a shape.
+// ---------------------------------------------------------------------------
+struct TileKey {
+    int conv, visc, shock, time, n0, n1, n2;
+    bool operator<(const TileKey& o) const {
+        const int a[7] = {conv, visc, shock, time, n0, n1, n2}, b[7] = {o.conv, o.visc, o.shock, o.time, o.n0, o.n1, o.n2};
+        return std::lexicographical_compare(a, a + 7, b, b + 7);
+    }
+};
+std::map<TileKey, std::pair<int, std::vector<float>>> g_tile_cache;
+std::mutex g_tile_mu;
+
+template <int TIME>
+void fused_kernel_only(osb_ctx* c, const double* q, double* qn, double A, double B) {
+    if (c->d.conv_scheme == OSB_CONV_CENTRAL_KGP) launch_fused_v<CF_KGP, false, TIME>(c, q, qn, A, B);
+    else if (c->shock) launch_fused_v<CF_NONE, true, TIME>(c, q, qn, A, B);
+    else if (c->d.conv_scheme == OSB_CONV_CENTRAL_SKEW) launch_fused_v<CF_SKEW, false, TIME>(c, q, qn, A, B);
+    else launch_fused_v<CF_FEIEREISEN, false, TIME>(c, q, qn, A, B);
+}
+
+int tune_fused_tile(osb_ctx* c) {
+    c->tile_tuned = true;
+    if (!use_fused(c) || c->mixed || c->tile_forced) return OSB_OK;
+    const TileKey key{c->d.conv_scheme, c->d.viscous, (int)c->shock, c->d.time_scheme, c->g.n[0], c->g.n[1], c->g.n[2]};
+    {
+        std::lock_guard<std::mutex> lock(g_tile_mu);
+        auto it = g_tile_cache.find(key);
+        if (it != g_tile_cache.end()) {
+            c->fused_tile = it->second.first;
+            c->tile_us = it->second.second;
+            return OSB_OK;
+        }
+    }
+    double* qn = (c->q_cur == c->qa) ? c->qb : c->qa;
+    const bool rk3 = c->d.time_scheme == OSB_TIME_RK3;
+    const double A = rk3 ? 2.0 / 3.0 : c->rkA[0], B = rk3 ? 1.0 / 4.0 : c->rkB[0];
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(c, hipEventCreate(&e0));
+    hipError_t ec = hipEventCreate(&e1);
+    if (ec != hipSuccess) { (void)hipEventDestroy(e0); HIP_TRY(c, ec); }
+    auto launch = [&]() {
+        if (rk3) fused_kernel_only<2>(c, c->q_cur, qn, A, B);
+        else fused_kernel_only<1>(c, c->q_cur, qn, A, B);
+    };
+    float best = 1e30f;
+    int best_t = 0;
+    c->tile_us.assign(CF_NTILES, 0.f);
+    hipError_t e = hipSuccess;
+    for (int t = 0; t < CF_NTILES && e == hipSuccess; ++t) {
+        c->fused_tile = t;
+        launch();
+        e = hipEventRecord(e0, c->stream);
+        for (int r = 0; r < 3; ++r) launch();
+        if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        c->tile_us[t] = ms * 1000.f / 3.f;
+        if (c->tile_us[t] < best) { best = c->tile_us[t]; best_t = t; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIP_TRY(c, e);
+    c->fused_tile = best_t;
+    HIP_TRY(c, hipMemsetAsync(c->du, 0, sizeof(double) * c->nv * c->g.Ptot, c->stream));
+    int r = check_launch(c);
+    if (r) return r;
+    std::lock_guard<std::mutex> lock(g_tile_mu);
+    g_tile_cache[key] = {best_t, c->tile_us};
+    return OSB_OK;
+}
+
 // WENO filter after the full step, in place on q (WENO_filter.py:406-536):
 // the Ducros sensor (HYB), the filter fluxes per direction, the update
 template <int NDIM, int ORDER, bool HYB>
@@ -671,7 +752,10 @@ void init_ctx(osb_ctx* c, const osb_problem_desc* desc) {
             if (i < desc->ndim && desc->bc[i][sd] >= OSB_BC_EXTRAPOLATION0) k.carp[i] |= 1 << sd;
     }
 
-    if (const char* e = std::getenv("OSB_FUSED_TILE")) c->fused_tile = std::max(0, std::min(CF_NTILES - 1, std::atoi(e)));
+    if (const char* e = std::getenv("OSB_FUSED_TILE")) {
+        c->fused_tile = std::max(0, std::min(CF_NTILES - 1, std::atoi(e)));
+        c->tile_forced = true;
+    }
     if (const char* e = std::getenv("OSB_GRAPH")) c->use_graph = std::atoi(e) != 0;
     if (desc->flux_type >= OSB_FLUX_HLLC) c->shock_fused = false;   // HLLC faces run their own kernel and the separate update
     c->bc_periodic = true;
@@ -918,6 +1002,10 @@ int graph_step(osb_ctx* c) {
 int osb_step(osb_ctx* c, int nsteps) {
     if (!c || nsteps < 0) return OSB_E_INVALID;
     if (c->next_stage != 0) return set_err(c, OSB_E_INVALID, "a step is partly advanced by osb_stage; finish its stages first");
+    if (nsteps > 0 && !c->tile_tuned) {
+        int r = tune_fused_tile(c);
+        if (r) return r;
+    }
     const bool graphs = c->use_graph && !c->prof_on;
     for (int i = 0; i < nsteps; ++i) {
         int r = graphs ? graph_step(c) : do_step(c);
@@ -933,6 +1021,10 @@ int osb_stage(osb_ctx* c, int s) {
     if (s < 0 || s >= stage_count(c)) return set_err(c, OSB_E_INVALID, "stage index out of range");
     if (s != c->next_stage)
         return set_err(c, OSB_E_INVALID, "stages run in order: expected stage " + std::to_string(c->next_stage));
+    if (s == 0 && !c->tile_tuned) {
+        int r = tune_fused_tile(c);
+        if (r) return r;
+    }
     do_stage(c, s);                       // eager: single stages are not captured
     c->next_stage = (s + 1) % stage_count(c);
     if (c->next_stage == 0) {
@@ -1070,6 +1162,32 @@ int osb_state_device_ptr(osb_ctx* c, int var, void** out) {
     return OSB_OK;
 }
 
+
+// The context's kernel path and launch shapes as JSON (no counterpart in the
+// reference; the analogue of the generated code's choice of kernels).
+int osb_describe(osb_ctx* c, char* buf, int len) {
+    if (!c || !buf || len < 1) return OSB_E_INVALID;
+    const char* path = use_fused(c) ? (c->mixed ? "central_mixed" : "central_fused")
+                                    : (use_shock_stage(c) ? (c->shock_fused ? "face_flux+face_resid" : "face_flux+resid_rk")
+                                                          : "unfused");
+    std::string s = std::string("{\"path\": \"") + path + "\"";
+    if (use_fused(c) && !c->mixed) {
+        const CfTile t = CF_TILES[c->fused_tile];
+        char tmp[160];
+        std::snprintf(tmp, sizeof(tmp), ", \"tile\": [%d, %d, %d], \"tile_choice\": \"%s\", \"tile_us\": [", t.tx, t.ty, t.tz,
+                      c->tile_forced ? "forced" : (c->tile_tuned ? "autotuned" : "default"));
+        s += tmp;
+        for (size_t i = 0; i < c->tile_us.size(); ++i) {
+            std::snprintf(tmp, sizeof(tmp), "%s%.2f", i ? ", " : "", (double)c->tile_us[i]);
+            s += tmp;
+        }
+        s += "]";
+    }
+    s += ", \"graphs\": " + std::string(c->use_graph ? "true" : "false") + "}";
+    std::strncpy(buf, s.c_str(), (size_t)len - 1);
+    buf[len - 1] = '\0';
+    return (int)s.size() < len ? OSB_OK : OSB_E_INVALID;
+}
 
 // Allocation-free stepping of a caller-owned block (no counterpart in the
 // reference).  q: padded state (nv x padded points, halos filled), advanced in

@@ -54,7 +54,10 @@ struct osb_ctx {
     int next_stage = 0;             // osb_stage: the stage the current step continues with
     int rk_stages = 3;              // RK-LS stages and coefficients (rk_LS.py:97-132)
     double rkA[5] = {0}, rkB[5] = {0};
-    int fused_tile = 0;             // index into CF_TILES (autotuned)
+    int fused_tile = 0;             // index into CF_TILES: timed at the first step (tune_fused_tile)
+    bool tile_tuned = false;        // the tile choice has been made
+    bool tile_forced = false;       // OSB_FUSED_TILE chose it
+    std::vector<float> tile_us;     // timed microseconds per launch of each CF_TILES shape
     int xcd_remap = 2;              // XCD-aware workgroup-to-tile order: 8x4x2 supertiles per XCD range
 
     bool shock_fused = true;        // shock-only stages: last direction's faces fused with the update (HLLC: separate)

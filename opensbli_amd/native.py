@@ -285,6 +285,8 @@ class Library:
             L.osb_batch_get_stats.argtypes = [vp, C.c_int, C.POINTER(StatsC)]
             L.osb_batch_autotune.argtypes = [vp]
             L.osb_batch_describe.argtypes = [vp, C.c_char_p, C.c_int]
+        if hasattr(L, "osb_describe"):
+            L.osb_describe.argtypes = [vp, C.c_char_p, C.c_int]
         self.has_noalloc = hasattr(L, "osb_step_noalloc")
         if self.has_noalloc:
             L.osb_step_noalloc.argtypes = [C.POINTER(ProblemDescC), vp, vp, C.c_int, vp]
@@ -498,6 +500,13 @@ class Context:
             out[name.value.decode()] = (ms.value, n.value)
             i += 1
         return out
+
+    def describe(self) -> dict:
+        """The context's kernel path and launch shapes (osb_describe)."""
+        import json
+        buf = C.create_string_buffer(4096)
+        self._check(self.lib.lib.osb_describe(self.h, buf, len(buf)))
+        return json.loads(buf.value.decode())
 
     def set_stream(self, stream_handle: int):
         self._check(self.lib.lib.osb_set_stream(self.h, C.c_void_p(stream_handle)))

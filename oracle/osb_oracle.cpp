@@ -1560,6 +1560,11 @@ int osb_state_device_ptr(osb_ctx* c, int, void**) { return fail(c, OSB_E_UNSUPPO
 int osb_set_stream(osb_ctx* c, void*) { return fail(c, OSB_E_UNSUPPORTED, "oracle has no streams"); }
 int osb_set_state_device(osb_ctx* c, const double*, int) { return fail(c, OSB_E_UNSUPPORTED, "oracle has no device buffers"); }
 int osb_step_noalloc(const osb_problem_desc*, double*, double*, int, void*) { return fail(nullptr, OSB_E_UNSUPPORTED, "oracle has no device buffers"); }
+int osb_describe(osb_ctx* c, char* buf, int len) {
+    if (!c || !buf || len < 1) return OSB_E_INVALID;
+    std::snprintf(buf, (size_t)len, "{\"path\": \"oracle\"}");
+    return OSB_OK;
+}
 int osb_get_state_device(osb_ctx* c, double*, int) { return fail(c, OSB_E_UNSUPPORTED, "oracle has no device buffers"); }
 
 }  // extern "C"

@@ -20,6 +20,15 @@ def _announce(name):
     print(f"[gpu-case] {name}", flush=True)
 
 
+def _tgv_box(n, steps=5):
+    import math
+    pr = P.tgv(n=16, steps=steps)
+    delta = [2 * math.pi / m for m in n]
+    q = P._tgv_fields(tuple(n), tuple(delta), pr.desc.gama, pr.desc.minf)
+    d = nat.ProblemDesc(**{**pr.desc.__dict__, "n": list(n), "delta": delta})
+    return P.Problem(f"tgv_central_{n[0]}x{n[1]}x{n[2]}", d, q, steps)
+
+
 def _ensemble():
     """Instances of several groups and sizes, and two that run unbatched."""
     probs = []
@@ -56,7 +65,28 @@ def _ensemble():
     w3.desc.viscous = nat.VISC_NONE                   # 3-D shock-only group
     w3.name += "_inviscid"
     probs.append(w3)
-    probs.append(P.tgv(n=16, steps=5))                # fused central path (stepped alone)
+    # 3-D central instances (FP64, RK-LS, every face periodic): batched central
+    # groups by split form and viscosity, any sizes, CFL next to constant dt
+    probs.append(P.tgv(n=16, steps=5))
+    probs.append(P.tgv(n=24, steps=5))
+    probs.append(_tgv_box([20, 12, 10], steps=5))
+    probs.append(P.tgv(n=16, steps=5, scheme="kgp"))
+    ks = P.tgv(n=20, steps=5, scheme="kgp", minf=1.25)
+    ks.desc.viscous = nat.VISC_SUTHERLAND
+    ks.name += "_sutherland"
+    probs.append(ks)
+    sk = P.tgv(n=16, steps=5)
+    sk.desc.conv_scheme = nat.CONV_CENTRAL_SKEW
+    sk.name += "_skew"
+    probs.append(sk)
+    cf = P.tgv(n=16, steps=5, minf=1.25, cfl=0.5, dt=0.0)
+    cf.name += "_cfl"
+    probs.append(cf)
+    probs.append(P.tgv(n=16, steps=5, precision=nat.PREC_MIXED))     # mixed central: stepped alone
+    cl = P.tgv(n=16, steps=5)
+    cl.desc.bc = [[nat.BC_EXTRAPOLATION0, nat.BC_EXTRAPOLATION1], [nat.BC_PERIODIC] * 2, [nat.BC_PERIODIC] * 2]
+    cl.name += "_closure"                                              # closure faces: stepped alone
+    probs.append(cl)
     probs.append(P.tgv(n=16, steps=5, scheme="teno", minf=1.25, cfl=0.5, dt=0.0))
     sf = P.sod(n=112, steps=5)
     sf.desc.shock_filter = nat.FILTER_WENOZ           # filtered contexts are stepped alone
@@ -80,7 +110,9 @@ def test_batch_bitwise_equals_single(hip, graphs, monkeypatch):
         print("  ", info, flush=True)
         got = [b.get_state(i) for i in range(len(probs))]
         stats = [b.stats(i) for i in range(len(probs))]
-    assert info["solo"] == 3 and len(info["groups"]) >= 5
+    assert info["solo"] == 4 and len(info["groups"]) >= 8
+    central = [g for g in info["groups"] if g["kind"] == "central"]
+    assert len(central) == 4 and sum(g["members"] for g in central) == 7
     for p, q, st in zip(probs, got, stats):
         _announce(f"  compare {p.name}")
         want, wst = run(hip, p, steps=5)

@@ -3,6 +3,7 @@ L2 and L-inf <= 1e-11 per conservative variable, as stated for the FP64 path).
 
 Each case prints its name before touching the GPU so a failing run names it."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -509,6 +510,48 @@ def test_tgv_anisotropic_ragged_tiles(hip, oracle, scheme):
     l2, _ = rel_errors(a, b)
     print(f"   mixed vs FP64 oracle rel L2 {np.array2string(l2, precision=2)}", flush=True)
     assert l2.max() <= 1e-5, l2
+
+
+@pytest.mark.parametrize("tile", ["0", "1", "2"])
+def test_fused_stage_tiles_against_oracle(hip, oracle, tile, monkeypatch):
+    """Every tile shape the fused 3-D stage kernel offers (forced with
+    OSB_FUSED_TILE) meets the oracle on a ragged anisotropic box (Feiereisen,
+    KGP + Sutherland) and on the TENO5 + viscous path."""
+    monkeypatch.setenv("OSB_FUSED_TILE", tile)
+    ks = _tgv_box([20, 12, 10], scheme="kgp", steps=4)
+    ks.desc.viscous = nat.VISC_SUTHERLAND
+    ks.desc.minf = 1.25
+    ks.q0 = P._tgv_fields((20, 12, 10), tuple(ks.desc.delta), ks.desc.gama, 1.25)
+    for pr in (_tgv_box([20, 12, 10], steps=4), ks, P.tgv(n=16, steps=3, scheme="teno", minf=1.25)):
+        with hip.context(pr.desc) as c:
+            info = c.describe()
+        assert info["tile_choice"] == "forced", info
+        _compare(hip, oracle, pr)
+
+
+def test_fused_stage_tile_autotuned(hip):
+    """At the first step the tile shapes are timed on the context's own state and
+    the fastest kept; the choice does not change the results."""
+    pr = P.tgv(n=48, steps=2)
+    _announce("tile autotune tgv48")
+    with hip.context(pr.desc) as c:
+        c.set_state(pr.q0)
+        assert c.describe()["tile_choice"] == "default"
+        c.step(2)
+        q = c.get_state()
+        info = c.describe()
+    print("  ", info, flush=True)
+    us = info["tile_us"]
+    assert info["tile_choice"] == "autotuned" and len(us) == 3 and min(us) > 0
+    tiles = [[16, 4, 4], [8, 8, 4], [32, 4, 2]]
+    assert info["tile"] == tiles[int(np.argmin(us))]
+    for t in range(3):
+        os.environ["OSB_FUSED_TILE"] = str(t)
+        try:
+            forced, _ = run(hip, pr)
+        finally:
+            del os.environ["OSB_FUSED_TILE"]
+        assert np.array_equal(forced, q), t
 
 
 def test_out_of_memory_is_an_error_not_a_crash(hip):
