@@ -105,12 +105,13 @@ def time_gpu(problem, steps, warmup):
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         state = c.get_state()
+        path = c.describe()
         c.profile_enable(True)
         c.step(steps)
         c.sync()
         prof = c.profile_read()
         c.profile_enable(False)
-    return t1 - t0, prof, state, ev0.elapsed_time(ev1) / 1e3
+    return t1 - t0, prof, state, ev0.elapsed_time(ev1) / 1e3, path
 
 
 def cpu_info():
@@ -239,6 +240,48 @@ def ensemble_bench(steps, count=64, seed=20261203):
     }
 
 
+def tgv_ensemble_bench(steps=200, count=48, seed=20261204):
+    """3-D TGV ensemble (FP64 central): each instance stepped alone, one after
+    another, against the batched form (one fused stage launch per stage for every
+    instance); the batched results are bitwise those of the instances alone."""
+    import numpy as np
+    import torch
+    from opensbli_amd import native as nat, problems as P
+    hip = nat.load_hip()
+    probs = P.tgv_ensemble(count=count, seed=seed, steps=steps)
+    work = sum(p.desc.npoints * 3.0 * steps for p in probs)
+    t_plain, alone = 0.0, []
+    for p in probs:
+        with hip.context(p.desc) as c:
+            c.set_state(p.q0)
+            c.step(1)
+            c.sync()
+            t0 = time.perf_counter()
+            c.step(steps)
+            c.sync()
+            t_plain += time.perf_counter() - t0
+            alone.append(c.get_state())
+    with hip.batch([p.desc for p in probs]) as b:
+        for i, p in enumerate(probs):
+            b.set_state(i, p.q0)
+        b.step(1)
+        b.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.sync()
+        t_batch = time.perf_counter() - t0
+        info = b.describe()
+        same = all(np.array_equal(b.get_state(i), q) for i, q in enumerate(alone))
+    return {
+        "workload": f"tgv_ensemble_{count}", "instances": count, "seed": seed,
+        "sizes": sorted({p.desc.n[0] for p in probs}), "steps_timed": steps, "work_gpts_stage": work / 1e9,
+        "plain_fp64": work / t_plain / 1e9, "batched_fp64": work / t_batch / 1e9,
+        "speedup_batched_vs_plain": t_plain / t_batch, "batched_bitwise_equal_alone": same,
+        "groups": [{k: g[k] for k in ("kind", "members")} for g in info["groups"]], "solo": info["solo"],
+    }
+
+
 def serving_bench(count=64, rate=50.0, seed=20261203, budget_gib=8.0, max_solves=32, chunk=25):
     from opensbli_amd import native as nat, serving as S
     hip = nat.load_hip()
@@ -264,7 +307,7 @@ def reference_benchmarks(steps=20, warmup=2):
     out = {}
     for name in ("tgv_teno6_128", "tgv_central4weno5_128"):
         pr = make_problem(name, "fp64")
-        dt, prof, _, _ = time_gpu(pr, steps, warmup)
+        dt, prof, _, _, _ = time_gpu(pr, steps, warmup)
         out[name] = {"value": pr.desc.npoints * 3.0 * steps / dt / 1e9, "unit": "GPts*stage/s", "steps_timed": steps,
                      "kernel_ms_per_step": {k: v[0] / steps for k, v in prof.items() if v[1]}}
     return out
@@ -316,7 +359,7 @@ def teno256_line(steps=10, warmup=2):
     states = {}
     for prec in ("fp64", "mixed"):
         pr = make_problem("tgv_teno5_256", prec)
-        dt, prof, st, t_ev = time_gpu(pr, steps, warmup)
+        dt, prof, st, t_ev, _ = time_gpu(pr, steps, warmup)
         npts = pr.desc.npoints
         stage_s = t_ev / (3.0 * steps)
         rec = {"value": npts * 3.0 * steps / dt / 1e9, "ms_per_step": dt / steps * 1e3,
@@ -341,7 +384,7 @@ def mixed_headline(problem_name, steps, warmup):
     """The headline workload in mixed precision, with its error against FP64."""
     from opensbli_amd import native as nat
     pm = make_problem(problem_name, "mixed")
-    dt_m, _, qm, _ = time_gpu(pm, steps, warmup)
+    dt_m, _, qm, _, _ = time_gpu(pm, steps, warmup)
     pf = make_problem(problem_name, "fp64")
     hip = nat.load_hip()
     with hip.context(pf.desc) as c:
@@ -365,6 +408,7 @@ def main():
                     help="skip the mixed-precision, KH-ensemble and serving lines added to the default run")
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--serving-requests", type=int, default=64)
+    ap.add_argument("--tgv-ensemble-steps", type=int, default=200)
     ap.add_argument("--cpu-leg", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed run, copy a 256 MiB FP64 buffer 5 times (PMC byte-counter calibration)")
@@ -379,18 +423,20 @@ def main():
         sys.exit(2)
 
     oracle_path = ensure_built()
-    if args.workload in ("kh_ensemble", "serving"):
-        extra = ensemble_bench(args.ensemble_steps) if args.workload == "kh_ensemble" else \
-            serving_bench(args.serving_requests)
-        value = extra["batched_mixed"] if args.workload == "kh_ensemble" else extra["aggregate_gpts_stage_per_s"]
+    if args.workload in ("kh_ensemble", "tgv_ensemble", "serving"):
+        extra = {"kh_ensemble": lambda: ensemble_bench(args.ensemble_steps), "tgv_ensemble": lambda: tgv_ensemble_bench(args.tgv_ensemble_steps),
+                 "serving": lambda: serving_bench(args.serving_requests)}[args.workload]()
+        value = {"kh_ensemble": "batched_mixed", "tgv_ensemble": "batched_fp64",
+                 "serving": "aggregate_gpts_stage_per_s"}[args.workload]
+        value = extra[value]
         print(json.dumps({"metric": METRIC, "value": value, "unit": "GPts·stage/s", "n_gpus": 1,
-                          "steps": args.ensemble_steps, "warmup": 1, "higher_is_better": True, "scaling": "weak",
+                          "steps": extra.get("steps_timed", args.ensemble_steps), "warmup": 1, "higher_is_better": True, "scaling": "weak",
                           "vs_baseline": None, "dtype": "f64-state/f32-flux" if args.workload == "kh_ensemble" else "f64",
                           "data": "synthetic (seeded)", "config": {"workload": extra["workload"]}, "detail": extra}),
               flush=True)
         return
     problem = make_problem(args.workload, args.precision)
-    dt_wall, prof, state, t_events = time_gpu(problem, args.steps, args.warmup)
+    dt_wall, prof, state, t_events, path = time_gpu(problem, args.steps, args.warmup)
     import numpy as np
     if not np.all(np.isfinite(state)):
         raise SystemExit("non-finite state after the timed steps")
@@ -417,12 +463,14 @@ def main():
                    "scheme": args.workload.split("_")[1], "steps_timed": args.steps},
         "roofline": roofline,
         "kernel_ms": {k: v[0] for k, v in prof.items() if v[1]},
+        "kernel_path": path,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(oracle_path, args.workload, problem, args.cpu_budget)
     if not args.no_extras and args.workload == "tgv_central_128" and args.precision == "fp64":
         out["mixed_precision"] = mixed_headline(args.workload, args.steps, args.warmup)
         out["ensemble"] = ensemble_bench(args.ensemble_steps)
+        out["tgv_ensemble"] = tgv_ensemble_bench(args.tgv_ensemble_steps)
         out["serving"] = serving_bench(args.serving_requests)
         out["reference_benchmarks"] = reference_benchmarks()
         out["tgv_teno5_256"] = teno256_line()

@@ -290,7 +290,7 @@ void launch_bcs(hipStream_t st, const Group& gr, int parity) {
 
 template <int CONV>
 void launch_central_v(hipStream_t st, const Group& gr, int parity, int stage) {
-    const unsigned grid = grid_of(gr, BF_RESID);
+    const unsigned grid = (unsigned)std::max(1, gr.total(BF_RESID));     // one workgroup per tile
     switch (gr.visc) {
         case OSB_VISC_NONE: kb_central_fused<CONV, 0><<<grid, KB_CF_TX * KB_CF_TY * KB_CF_TZ, 0, st>>>(gr.d_items, gr.dof(BF_RESID), gr.n(), parity, stage); break;
         case OSB_VISC_CONSTANT: kb_central_fused<CONV, 1><<<grid, KB_CF_TX * KB_CF_TY * KB_CF_TZ, 0, st>>>(gr.d_items, gr.dof(BF_RESID), gr.n(), parity, stage); break;

@@ -24,6 +24,20 @@ struct BItem {
     double rkA[5], rkB[5]; // stage coefficients (classic RK3: rknew, rkold)
 };
 
+// Items are read through the constant address space: a pointer loaded from
+// constant memory is taken to be a global one, so the kernels access the state
+// with global loads and stores.  Read as plain memory, the pointers' address
+// space is unknown and every access is a FLAT instruction, which also counts
+// against the LDS wait counter (a wait for an LDS read then waits for every HBM
+// load in flight).  The table is never written while a launch runs.  The host
+// pass only type-checks kernel bodies, so the qualifier is device-side only.
+#ifdef __HIP_DEVICE_COMPILE__
+typedef const __attribute__((address_space(4))) BItem CBItem;
+#else
+typedef const BItem CBItem;
+#endif
+__device__ __forceinline__ CBItem& item_const(const BItem* items, int i) { return ((CBItem*)items)[i]; }
+
 // index of the item owning virtual block vb: largest i with offs[i] <= vb
 __device__ __forceinline__ int find_item(const int* __restrict__ offs, int n, int vb) {
     int lo = 0, hi = n - 1;
@@ -44,7 +58,7 @@ __global__ void __launch_bounds__(BLOCK) kb_face_flux(const BItem* __restrict__ 
                                                       int parity) {
     OSB_BATCH_LOOP {
         const int i = find_item(offs, n, vb);
-        const BItem& it = items[i];
+        CBItem& it = item_const(items, i);
         const long t = (long)(vb - offs[i]) * BLOCK + threadIdx.x;
         // fluxes of direction DIR occupy the DIR-th nv x Ptot block of F
         face_flux_item<NDIM, DIR, SCH, T>(t, it.q[parity], it.F + (long)DIR * (NDIM + 2) * it.g.Ptot, it.g, it.c);
@@ -57,7 +71,7 @@ __global__ void __launch_bounds__(BLOCK) kb_face_resid(const BItem* __restrict__
                                                        int parity, int stage) {
     OSB_BATCH_LOOP {
         const int i = find_item(offs, n, vb);
-        const BItem& it = items[i];
+        CBItem& it = item_const(items, i);
         face_resid_tile<NDIM, SCH, T, TIME, FUSED_FW>(vb - offs[i], it.q[parity], it.F, it.q[parity ^ 1], it.du, it.scal[1], it.g, it.c,
                                             it.rkA[stage], it.rkB[stage]);
     }
@@ -68,7 +82,7 @@ __global__ void __launch_bounds__(BLOCK) kb_resid_rk(const BItem* __restrict__ i
                                                      int parity, int stage) {
     OSB_BATCH_LOOP {
         const int i = find_item(offs, n, vb);
-        const BItem& it = items[i];
+        CBItem& it = item_const(items, i);
         const long t = (long)(vb - offs[i]) * BLOCK + threadIdx.x;
         resid_rk_item<NDIM, TIME>(t, it.F, it.q[parity], it.q[parity ^ 1], it.du, it.scal[1], it.g, it.c,
                                   it.rkA[stage], it.rkB[stage]);
@@ -80,7 +94,7 @@ __global__ void __launch_bounds__(BLOCK) kb_bc(const BItem* __restrict__ items, 
                                                int parity) {
     OSB_BATCH_LOOP {
         const int i = find_item(offs, n, vb);
-        const BItem& it = items[i];
+        CBItem& it = item_const(items, i);
         const long t = (long)(vb - offs[i]) * BLOCK + threadIdx.x;
         bc_item<DIR>(t, it.q[parity], it.g, it.g.ndim + 2, it.bc[DIR]);
     }
@@ -91,7 +105,7 @@ static __global__ void __launch_bounds__(BLOCK) kb_save_state(const BItem* __res
                                                        int parity) {
     OSB_BATCH_LOOP {
         const int i = find_item(offs, n, vb);
-        const BItem& it = items[i];
+        CBItem& it = item_const(items, i);
         const Geom& g = it.g;
         const long t = (long)(vb - offs[i]) * BLOCK + threadIdx.x;
         const int e[3] = {g.n[0], g.n[1], g.n[2]};
@@ -107,7 +121,7 @@ static __global__ void __launch_bounds__(BLOCK) kb_save_state(const BItem* __res
 // CFL: reset, per-instance max wave speed, dt = cfl / max and time += dt
 static __global__ void kb_zero_wavespeed(const BItem* __restrict__ items, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && items[i].cfl > 0) reinterpret_cast<unsigned long long*>(items[i].scal)[0] = 0ull;
+    if (i < n && item_const(items, i).cfl > 0) reinterpret_cast<unsigned long long*>(item_const(items, i).scal)[0] = 0ull;
 }
 
 template <int NDIM>
@@ -115,7 +129,7 @@ __global__ void __launch_bounds__(BLOCK) kb_wavespeed(const BItem* __restrict__ 
                                                       int parity) {
     OSB_BATCH_LOOP {
         const int i = find_item(offs, n, vb);
-        const BItem& it = items[i];
+        CBItem& it = item_const(items, i);
         const Geom& g = it.g;
         const long t = (long)(vb - offs[i]) * BLOCK + threadIdx.x;
         const int e[3] = {g.n[0], g.n[1], g.n[2]};
@@ -133,8 +147,8 @@ __global__ void __launch_bounds__(BLOCK) kb_wavespeed(const BItem* __restrict__ 
 // GLF wave speeds of every GLF item (it.c.glf set), as k_glf_zero / k_glf
 static __global__ void kb_glf_zero(const BItem* __restrict__ items, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !items[i].c.glf) return;
-    unsigned long long* z = reinterpret_cast<unsigned long long*>(const_cast<double*>(items[i].c.glf));
+    if (i >= n || !item_const(items, i).c.glf) return;
+    unsigned long long* z = reinterpret_cast<unsigned long long*>(const_cast<double*>(item_const(items, i).c.glf));
     for (int r = 0; r < 9; ++r) z[r] = 0ull;
 }
 
@@ -143,7 +157,7 @@ __global__ void __launch_bounds__(BLOCK) kb_glf(const BItem* __restrict__ items,
                                                 int parity) {
     OSB_BATCH_LOOP {
         const int i = find_item(offs, n, vb);
-        const BItem& it = items[i];
+        CBItem& it = item_const(items, i);
         const Geom& g = it.g;
         const long t = (long)(vb - offs[i]) * BLOCK + threadIdx.x;
         const int e[3] = {g.n[0], g.n[1], g.n[2]};
@@ -166,21 +180,27 @@ __global__ void __launch_bounds__(BLOCK) kb_glf(const BItem* __restrict__ items,
 // the single-block kernel's tile code, which also writes the wrapped halo
 // images; the LDS box is reused by the workgroup's next tile after a barrier
 constexpr int KB_CF_TX = 16, KB_CF_TY = 4, KB_CF_TZ = 4;
+// Workgroup v of a launch runs on XCD v % 8.  This permutation of [0, total)
+// gives XCD x the contiguous range of virtual blocks [x*(q+1), (x+1)*(q+1)) for
+// x < r and the ranges of q after them (total = 8q + r), so neighbouring tiles
+// of an instance, which share halo points, stay in one L2.
+__device__ __forceinline__ int xcd_contiguous(int v, int total) {
+    const int q = total >> 3, r = total & 7, x = v & 7, k = v >> 3;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
 template <int CONV, int VISC>
 __global__ void __launch_bounds__(KB_CF_TX * KB_CF_TY * KB_CF_TZ, OSB_CF_WAVES)
 kb_central_fused(const BItem* __restrict__ items, const int* __restrict__ offs, int n, int parity, int stage) {
-    const int total_ = offs[n];
-    for (int vb = blockIdx.x; vb < total_; vb += gridDim.x) {
-        if (vb != (int)blockIdx.x) __syncthreads();
-        const int i = find_item(offs, n, vb);
-        const BItem& it = items[i];
-        const int tile = vb - offs[i];
-        const int ntx = (it.g.n[0] + KB_CF_TX - 1) / KB_CF_TX, nty = (it.g.n[1] + KB_CF_TY - 1) / KB_CF_TY;
-        const int tbx = tile % ntx, tby = (tile / ntx) % nty, tbz = tile / (ntx * nty);
-        cf_tile_stage<KB_CF_TX, KB_CF_TY, KB_CF_TZ, CONV, VISC, false, OSB_TIME_RKLS3>(
-            tbx * KB_CF_TX, tby * KB_CF_TY, tbz * KB_CF_TZ, it.q[parity], nullptr, it.du, it.q[parity ^ 1], it.scal, it.g, it.c,
-            it.rkA[stage], it.rkB[stage], 1);
-    }
+    // one workgroup per tile of every member (the launch covers offs[n] tiles)
+    const int vb = xcd_contiguous(blockIdx.x, offs[n]);
+    const int i = find_item(offs, n, vb);
+    CBItem& it = item_const(items, i);
+    const int tile = vb - offs[i];
+    const int ntx = (it.g.n[0] + KB_CF_TX - 1) / KB_CF_TX, nty = (it.g.n[1] + KB_CF_TY - 1) / KB_CF_TY;
+    const int tbx = tile % ntx, tby = (tile / ntx) % nty, tbz = tile / (ntx * nty);
+    cf_tile_stage<KB_CF_TX, KB_CF_TY, KB_CF_TZ, CONV, VISC, false, OSB_TIME_RKLS3>(
+        tbx * KB_CF_TX, tby * KB_CF_TY, tbz * KB_CF_TZ, it.q[parity], nullptr, it.du, it.q[parity ^ 1], it.scal, it.g, it.c,
+        it.rkA[stage], it.rkB[stage], 1);
 }
 __host__ inline long kb_central_tiles(const Geom& g) {
     return (long)((g.n[0] + KB_CF_TX - 1) / KB_CF_TX) * ((g.n[1] + KB_CF_TY - 1) / KB_CF_TY) * ((g.n[2] + KB_CF_TZ - 1) / KB_CF_TZ);
@@ -188,10 +208,10 @@ __host__ inline long kb_central_tiles(const Geom& g) {
 
 static __global__ void kb_dt_finalize(const BItem* __restrict__ items, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !(items[i].cfl > 0)) return;
-    double* scal = items[i].scal;
+    if (i >= n || !(item_const(items, i).cfl > 0)) return;
+    double* scal = item_const(items, i).scal;
     const double m = __longlong_as_double((long long)reinterpret_cast<unsigned long long*>(scal)[0]);
-    const double dt = items[i].cfl / m;
+    const double dt = item_const(items, i).cfl / m;
     scal[1] = dt;
     scal[2] += dt;
 }

@@ -401,22 +401,32 @@ int tune_fused_tile(osb_ctx* c) {
         if (rk3) fused_kernel_only<2>(c, c->q_cur, qn, A, B);
         else fused_kernel_only<1>(c, c->q_cur, qn, A, B);
     };
+    // pass 0: one warm-up and three timed launches per shape (an estimate);
+    // pass 1: each shape timed over enough launches for ~0.5 ms of work, so
+    // that launch gaps and timer resolution do not decide between small grids
     float best = 1e30f;
     int best_t = 0;
     c->tile_us.assign(CF_NTILES, 0.f);
     hipError_t e = hipSuccess;
-    for (int t = 0; t < CF_NTILES && e == hipSuccess; ++t) {
+    auto time_shape = [&](int t, int reps) -> float {
         c->fused_tile = t;
         launch();
         e = hipEventRecord(e0, c->stream);
-        for (int r = 0; r < 3; ++r) launch();
+        for (int r = 0; r < reps; ++r) launch();
         if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0.f;
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-        c->tile_us[t] = ms * 1000.f / 3.f;
-        if (c->tile_us[t] < best) { best = c->tile_us[t]; best_t = t; }
+        return ms * 1000.f / (float)reps;
+    };
+    for (int pass = 0; pass < 2 && e == hipSuccess; ++pass) {
+        for (int t = 0; t < CF_NTILES && e == hipSuccess; ++t) {
+            const int reps = pass == 0 ? 3 : std::min(200, std::max(3, (int)(500.f / std::max(c->tile_us[t], 1.f))));
+            c->tile_us[t] = time_shape(t, reps);
+        }
     }
+    for (int t = 0; t < CF_NTILES; ++t)
+        if (c->tile_us[t] < best) { best = c->tile_us[t]; best_t = t; }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     HIP_TRY(c, e);

@@ -291,6 +291,22 @@ def kh_ensemble(count: int = 64, seed: int = 20261203, sizes=(128, 256, 512, 102
     return probs
 
 
+def tgv_ensemble(count: int = 48, seed: int = 20261204, sizes=(32, 48, 64, 96), steps: int = 200):
+    """Seeded ensemble of 3-D Taylor-Green vortices of uneven sizes (FP64,
+    Feiereisen split + constant viscosity, every face periodic): a Mach number in
+    [0.1, 0.3] and a Reynolds number in [400, 1600] per instance."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    probs = []
+    for i in range(count):
+        n = int(rng.choice(np.array(sizes)))
+        minf = float(rng.uniform(0.1, 0.3))
+        re = float(rng.uniform(400.0, 1600.0))
+        pr = tgv(n=n, steps=steps, minf=minf, re=re)
+        pr.name = f"tgv{i}_{n}"
+        probs.append(pr)
+    return probs
+
+
 # ---------------------------------------------------------------------------
 # 2-D four-quadrant Riemann problem (serving stream)
 # ---------------------------------------------------------------------------
