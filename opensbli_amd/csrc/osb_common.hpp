@@ -280,10 +280,15 @@ __device__ __forceinline__ T frecip(T x) {
     else return __builtin_amdgcn_rcpf(x);
 }
 
-template <int NDIM, int DIR, typename T>
+// HALF: L is returned times 1/2, folded into the factors only L uses (1/a^2,
+// 1/rho, beta, 1/D and the literal 1).  Scaling by a power of two commutes with
+// rounding, so every entry is exactly half the unscaled one (no subnormals
+// arise here), and so are the characteristic values L q built from it.
+template <int NDIM, int DIR, typename T, bool HALF = false>
 __device__ __forceinline__ void eigensystem(T gm1, T rho, const T* u, T a, T (&L)[NDIM + 2][NDIM + 2],
                                             T (&R)[NDIM + 2][NDIM + 2]) {
     constexpr int NV = NDIM + 2;
+    constexpr T HL = HALF ? T(0.5) : T(1);
 #pragma unroll
     for (int i = 0; i < NV; ++i)
 #pragma unroll
@@ -293,7 +298,7 @@ __device__ __forceinline__ void eigensystem(T gm1, T rho, const T* u, T a, T (&L
         const T inva = frecip<T>(a);
         const T H = (a * a) * frecip<T>(gm1) + (u0 * u0) * T(0.5);
         const T D = T(2) * H - u0 * u0;
-        const T invD = frecip<T>(D);
+        const T invD = frecip<T>(D) * HL;
         R[0][0] = 1; R[0][1] = 1; R[0][2] = 1;
         R[1][0] = u0 - a; R[1][1] = u0; R[1][2] = u0 + a;
         R[2][0] = H - u0 * a; R[2][1] = (u0 * u0) * T(0.5); R[2][2] = H + u0 * a;
@@ -313,10 +318,11 @@ __device__ __forceinline__ void eigensystem(T gm1, T rho, const T* u, T a, T (&L
     for (int d = 0; d < NDIM; ++d) q2 += u[d] * u[d];
     const T phi_sq = gm1 * (q2 * T(0.5));
     const T theta = u[DIR];
-    const T inva = frecip<T>(a), invrho = frecip<T>(rho);
-    const T inva2 = inva * inva;
+    const T inva = frecip<T>(a), invrho0 = frecip<T>(rho);
+    const T inva2 = inva * inva * HL;
     const T alpha = rho * T(INV_SQRT2) * inva;
-    const T bta = T(INV_SQRT2) * invrho * inva;
+    const T bta = T(INV_SQRT2) * invrho0 * inva * HL;
+    const T invrho = invrho0 * HL;
     const T igm1 = frecip<T>(gm1);
     const T a2 = a * a;
     if constexpr (NDIM == 2) {
@@ -327,7 +333,7 @@ __device__ __forceinline__ void eigensystem(T gm1, T rho, const T* u, T a, T (&L
         R[3][0] = phi_sq * igm1; R[3][1] = rho * (k1 * u0 - k0 * u1);
         R[3][2] = alpha * ((phi_sq + a2) * igm1 + a * theta);
         R[3][3] = alpha * ((phi_sq + a2) * igm1 - a * theta);
-        L[0][0] = T(1) - phi_sq * inva2; L[0][1] = gm1 * u0 * inva2; L[0][2] = gm1 * u1 * inva2; L[0][3] = -gm1 * inva2;
+        L[0][0] = HL - phi_sq * inva2; L[0][1] = gm1 * u0 * inva2; L[0][2] = gm1 * u1 * inva2; L[0][3] = -gm1 * inva2;
         L[1][0] = -(k1 * u0 - k0 * u1) * invrho; L[1][1] = k1 * invrho; L[1][2] = -k0 * invrho;
         L[2][0] = bta * (phi_sq - a * theta); L[2][1] = bta * (k0 * a - gm1 * u0);
         L[2][2] = bta * (k1 * a - gm1 * u1); L[2][3] = bta * gm1;
@@ -347,7 +353,7 @@ __device__ __forceinline__ void eigensystem(T gm1, T rho, const T* u, T a, T (&L
     R[4][2] = k2 * phi_sq * igm1 + rho * (k1 * u0 - k0 * u1);
     R[4][3] = alpha * ((phi_sq + a2) * igm1 + theta * a);
     R[4][4] = alpha * ((phi_sq + a2) * igm1 - theta * a);
-    const T c1 = T(1) - phi_sq * inva2;
+    const T c1 = HL - phi_sq * inva2;
     L[0][0] = k0 * c1 - (k2 * u1 - k1 * u2) * invrho;
     L[0][1] = k0 * gm1 * u0 * inva2;
     L[0][2] = k0 * gm1 * u1 * inva2 + k2 * invrho;
@@ -400,7 +406,8 @@ __device__ __forceinline__ void point_prims(const double (&v)[NDIM + 2], const C
 #pragma unroll
     for (int d = 0; d < NDIM; ++d) f[1 + d] = v[1 + d] * u_dir + (d == DIR ? p : 0.0);
     f[NV - 1] = (p + v[NV - 1]) * u_dir;
-    a = FAST ? sqrt(c.gama * p * inv_r) : sqrt(c.gama * p / r);
+    // mixed precision: the sound speed only enters the FP32 wave speeds
+    a = FAST ? (double)__builtin_amdgcn_sqrtf((float)(c.gama * p * inv_r)) : sqrt(c.gama * p / r);
 }
 
 // ---------------------------------------------------------------------------
@@ -453,7 +460,25 @@ __device__ __forceinline__ void face_core(PT&& pt, const Consts& c, double (&out
     if (FILT != 3) point(5);   // the order-3 filter reads offsets -1..2 only
     // averaged state (averaging.py:31-124)
     T rho_av, u_av[NDIM], a_av;
-    if (c.avg_roe) {
+    if (DEV && c.avg_roe) {
+        // mixed precision: the averaged state only builds the FP32 eigenvectors,
+        // so it is formed in FP32 (hardware square roots and reciprocals, 1 ulp
+        // on the positive normal densities and enthalpies here; sqrt(rho_L rho_R)
+        // as sqrt(rho_L) sqrt(rho_R))
+        const float sl = __builtin_amdgcn_sqrtf((float)rho23[0]), sr = __builtin_amdgcn_sqrtf((float)rho23[1]);
+        const float inv = __builtin_amdgcn_rcpf(sl + sr);
+        float ke = 0.f;
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) {
+            const float ua = (sl * (float)u23[0][d] + sr * (float)u23[1][d]) * inv;
+            ke += ua * ua;
+            u_av[d] = T(ua);
+        }
+        const float Hr = ((float)(p23[0] + E23[0]) * __builtin_amdgcn_rcpf(sl) +
+                          (float)(p23[1] + E23[1]) * __builtin_amdgcn_rcpf(sr)) * inv;
+        rho_av = T(sl * sr);
+        a_av = T(__builtin_amdgcn_sqrtf((float)c.gamma_m1 * (Hr - 0.5f * ke)));
+    } else if (c.avg_roe) {
         const double sl = sqrt(rho23[0]), sr = sqrt(rho23[1]);
         const double inv = wrecip<T>(sl + sr);
         double ke = 0.0;
@@ -471,8 +496,11 @@ __device__ __forceinline__ void face_core(PT&& pt, const Consts& c, double (&out
         for (int d = 0; d < NDIM; ++d) u_av[d] = T(0.5 * (u23[0][d] + u23[1][d]));
         a_av = T(0.5 * (a23[0] + a23[1]));
     }
+    // L times 1/2 unless the filter needs the unscaled characteristic fluxes:
+    // f+- = (L F +- lambda L q) / 2 then need no multiplication
+    constexpr bool HALF = FILT == 0;
     T L[NV][NV], R[NV][NV];
-    eigensystem<NDIM, DIR, T>(T(c.gamma_m1), rho_av, u_av, a_av, L, R);
+    eigensystem<NDIM, DIR, T, HALF>(T(c.gamma_m1), rho_av, u_av, a_av, L, R);
     // wave speeds: global (GLF, the stage's grid maxima in the eigenvalue order
     // of the direction, shock_capturing.py:838-857) or local Lax-Friedrichs over
     // offsets {0,1} (shock_capturing.py:1000-1028)
@@ -515,8 +543,13 @@ __device__ __forceinline__ void face_core(PT&& pt, const Consts& c, double (&out
             T cs = T(0), cf = T(0);
 #pragma unroll
             for (int m = 0; m < NV; ++m) { cs = fma(L[k][m], qs[s][m], cs); cf = fma(L[k][m], F[s][m], cf); }
-            fp[s] = T(0.5) * fma(lam[k], cs, cf);
-            fm[s] = T(0.5) * fma(-lam[k], cs, cf);
+            if constexpr (HALF) {
+                fp[s] = fma(lam[k], cs, cf);
+                fm[s] = fma(-lam[k], cs, cf);
+            } else {
+                fp[s] = T(0.5) * fma(lam[k], cs, cf);
+                fm[s] = T(0.5) * fma(-lam[k], cs, cf);
+            }
             cfv[s] = cf;
         }
         T rec;
