@@ -73,26 +73,51 @@ __device__ __forceinline__ double js_first(double a, double b, double c) { retur
 __device__ __forceinline__ double js_centre(double a, double b, double c) { return js_sum(a - 2.0 * b + c, a - c); }
 __device__ __forceinline__ double js_last(double a, double b, double c) { return js_sum(a - 2.0 * b + c, a - 4.0 * b + 3.0 * c); }
 
+// Mixed precision: the three indicators over five consecutive values from
+// their FP64 first and second differences (the FP32 inputs make the first
+// differences exact): the same quantities as js_last(f0, f1, f2),
+// js_centre(f1, f2, f3) and js_first(f2, f3, f4), with h the half of each
+// second combination (h^2 = s2^2 / 4), in 20 operations instead of 30.
+__device__ __forceinline__ void js_diff3(double f0, double f1, double f2, double f3, double f4, double& b_last,
+                                         double& b_centre, double& b_first) {
+    const double d1 = f1 - f0, d2 = f2 - f1, d3 = f3 - f2, d4 = f4 - f3;
+    const double sl = d2 - d1, sc = d3 - d2, sf = d4 - d3;
+    const double hl = fma(0.5, sl, d2);           // (f0 - 4 f1 + 3 f2) / 2
+    const double hc = 0.5 * (d2 + d3);            // (f3 - f1) / 2
+    const double hf = fma(0.5, sf, -d3);          // (3 f2 - 4 f3 + f4) / 2
+    b_last = fma((13.0 / 12.0) * sl, sl, hl * hl);
+    b_centre = fma((13.0 / 12.0) * sc, sc, hc * hc);
+    b_first = fma((13.0 / 12.0) * sf, sf, hf * hf);
+}
+
 template <typename T>
 __device__ __forceinline__ T eno3(T c0, T c1, T c2, T x0, T x1, T x2) { return fma(c2, x2, fma(c1, x1, c0 * x0)); }
 
-// Reciprocals of the non-linear weights.  FP64 path: IEEE division, as the
-// oracle.  Mixed path: the hardware reciprocal refined by two Newton steps
-// (an FP64-accurate reciprocal without the scaled division sequence).
+// Reciprocals.  FP64 path: IEEE division, as the oracle.  rcp_nr: the hardware
+// reciprocal refined by two Newton steps (the IEEE quotient on every input
+// measured, without the scaled division sequence), used by the fused central
+// stage kernel.
 __device__ __forceinline__ double rcp_nr(double x) {
     double r = __builtin_amdgcn_rcp(x);
     r = fma(fma(-x, r, 1.0), r, r);
     return fma(fma(-x, r, 1.0), r, r);
 }
+// One Newton step: the hardware reciprocal is good to 2^-24 relative on
+// gfx950, one step to about 1e-15 (two reproduce the IEEE quotient).  For the
+// mixed-precision path, whose FP64 results are rounded to FP32 downstream.
+__device__ __forceinline__ double rcp_nr1(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(fma(-x, r, 1.0), r, r);
+}
 template <typename T>
 __device__ __forceinline__ double wrecip(double x) {
     if constexpr (std::is_same<T, double>::value) return 1.0 / x;
-    else return rcp_nr(x);
+    else return rcp_nr1(x);
 }
 template <typename T>
 __device__ __forceinline__ double wdiv(double a, double b) {
     if constexpr (std::is_same<T, double>::value) return a / b;
-    else return a * rcp_nr(b);
+    else return a * rcp_nr1(b);
 }
 
 template <typename T>
@@ -122,7 +147,7 @@ __device__ __forceinline__ void wenoz_weights(double d0, double d1, double d2, d
         const double n0 = d0 * (e0 + t2) * (e1 * e2);
         const double n1 = d1 * (e1 + t2) * (e0 * e2);
         const double n2 = d2 * (e2 + t2) * (e0 * e1);
-        const double inv = rcp_nr(n0 + n1 + n2);
+        const double inv = rcp_nr1(n0 + n1 + n2);
         w0 = n0 * inv; w1 = n1 * inv; w2 = n2 * inv;
     }
 }
@@ -138,9 +163,14 @@ __device__ __forceinline__ double weno_sensor(double d0, double d1, double d2, d
 // sensor (FP64 only): the filter's shock sensor of these weights.
 template <typename T>
 __device__ __forceinline__ T wenoz_right(const T* f, double eps, double* sensor = nullptr) {
-    const double b0 = js_first(f[2], f[3], f[4]);     // stencil (i, i+1, i+2), d = 3/10
-    const double b1 = js_centre(f[1], f[2], f[3]);    // stencil (i-1, i, i+1), d = 6/10
-    const double b2 = js_last(f[0], f[1], f[2]);      // stencil (i-2, i-1, i), d = 1/10
+    double b0, b1, b2;
+    if constexpr (std::is_same<T, double>::value) {
+        b0 = js_first(f[2], f[3], f[4]);     // stencil (i, i+1, i+2), d = 3/10
+        b1 = js_centre(f[1], f[2], f[3]);    // stencil (i-1, i, i+1), d = 6/10
+        b2 = js_last(f[0], f[1], f[2]);      // stencil (i-2, i-1, i), d = 1/10
+    } else {
+        js_diff3(f[0], f[1], f[2], f[3], f[4], b2, b1, b0);
+    }
     double w0, w1, w2;
     wenoz_weights<T>(3.0 / 10.0, 6.0 / 10.0, 1.0 / 10.0, b0, b1, b2, eps, w0, w1, w2);
     if (sensor) *sensor = weno_sensor(3.0 / 10.0, 6.0 / 10.0, 1.0 / 10.0, w0, w1, w2);
@@ -151,9 +181,14 @@ __device__ __forceinline__ T wenoz_right(const T* f, double eps, double* sensor 
 }
 template <typename T>
 __device__ __forceinline__ T wenoz_left(const T* f, double eps, double* sensor = nullptr) {
-    const double b0 = js_first(f[3], f[4], f[5]);     // (i+1, i+2, i+3), d = 1/10
-    const double b1 = js_centre(f[2], f[3], f[4]);    // (i, i+1, i+2),   d = 6/10
-    const double b2 = js_last(f[1], f[2], f[3]);      // (i-1, i, i+1),   d = 3/10
+    double b0, b1, b2;
+    if constexpr (std::is_same<T, double>::value) {
+        b0 = js_first(f[3], f[4], f[5]);     // (i+1, i+2, i+3), d = 1/10
+        b1 = js_centre(f[2], f[3], f[4]);    // (i, i+1, i+2),   d = 6/10
+        b2 = js_last(f[1], f[2], f[3]);      // (i-1, i, i+1),   d = 3/10
+    } else {
+        js_diff3(f[1], f[2], f[3], f[4], f[5], b2, b1, b0);
+    }
     double w0, w1, w2;
     wenoz_weights<T>(1.0 / 10.0, 6.0 / 10.0, 3.0 / 10.0, b0, b1, b2, eps, w0, w1, w2);
     if (sensor) *sensor = weno_sensor(1.0 / 10.0, 6.0 / 10.0, 3.0 / 10.0, w0, w1, w2);
@@ -383,6 +418,23 @@ __device__ __forceinline__ void eigensystem(T gm1, T rho, const T* u, T a, T (&L
     }
 }
 
+// Entries of the left eigenvectors that are zero for every state: never set,
+// or a product with a zero direction selector k_d.  2-D: the shear row's last
+// entry and its entry 1 + DIR.  3-D: in rows r = 0..2, entry 4 unless DIR = r,
+// and entry 1 + j unless DIR = r or (r != j and DIR = 3 - r - j).
+template <int NDIM, int DIR>
+__device__ constexpr bool l_zero(int r, int m) {
+    if (NDIM == 2) return r == 1 && (m == 3 || m == 1 + DIR);
+    if (NDIM == 3 && r <= 2 && DIR != r) {
+        if (m == 4) return true;
+        if (m >= 1) {
+            const int j = m - 1;
+            return r == j || DIR != 3 - r - j;
+        }
+    }
+    return false;
+}
+
 // ---------------------------------------------------------------------------
 // Primitive variables and the physical flux along DIR of one point
 // (constituent relations + flux vector, shock_capturing.py:786-836).  FAST
@@ -394,7 +446,7 @@ __device__ __forceinline__ void point_prims(const double (&v)[NDIM + 2], const C
     constexpr int NV = NDIM + 2;
     const double r = v[0];
     double ke = 0.0;
-    const double inv_r = FAST ? rcp_nr(r) : 0.0;
+    const double inv_r = FAST ? rcp_nr1(r) : 0.0;
 #pragma unroll
     for (int d = 0; d < NDIM; ++d) {
         uu[d] = FAST ? v[1 + d] * inv_r : v[1 + d] / r;
@@ -542,7 +594,11 @@ __device__ __forceinline__ void face_core(PT&& pt, const Consts& c, double (&out
         for (int s = S0; s < S1; ++s) {
             T cs = T(0), cf = T(0);
 #pragma unroll
-            for (int m = 0; m < NV; ++m) { cs = fma(L[k][m], qs[s][m], cs); cf = fma(L[k][m], F[s][m], cf); }
+            for (int m = 0; m < NV; ++m) {
+                if (DEV && l_zero<NDIM, DIR>(k, m)) continue;     // mixed precision: structural zeros skipped
+                cs = fma(L[k][m], qs[s][m], cs);
+                cf = fma(L[k][m], F[s][m], cf);
+            }
             if constexpr (HALF) {
                 fp[s] = fma(lam[k], cs, cf);
                 fm[s] = fma(-lam[k], cs, cf);
