@@ -189,7 +189,7 @@ __device__ __forceinline__ int xcd_contiguous(int v, int total) {
     return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 template <int CONV, int VISC>
-__global__ void __launch_bounds__(KB_CF_TX * KB_CF_TY * KB_CF_TZ, OSB_CF_WAVES)
+__global__ void __launch_bounds__(KB_CF_TX * KB_CF_TY * KB_CF_TZ, cf_waves(KB_CF_TX, KB_CF_TY, KB_CF_TZ, cf_nfield(CONV, VISC)))
 kb_central_fused(const BItem* __restrict__ items, const int* __restrict__ offs, int n, int parity, int stage) {
     // one workgroup per tile of every member (the launch covers offs[n] tiles)
     const int vb = xcd_contiguous(blockIdx.x, offs[n]);

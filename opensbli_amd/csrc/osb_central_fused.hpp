@@ -29,7 +29,7 @@ namespace osb {
 
 // tile shapes offered to the tile selection: {TX, TY, TZ}
 struct CfTile { int tx, ty, tz; };
-constexpr CfTile CF_TILES[] = {{16, 4, 4}, {8, 8, 4}, {32, 4, 2}};
+constexpr CfTile CF_TILES[] = {{16, 4, 4}, {16, 8, 4}, {8, 8, 8}};
 constexpr int CF_NTILES = 3;
 
 enum CfConv { CF_NONE = 0, CF_FEIEREISEN = 1, CF_SKEW = 2, CF_KGP = 3 };
@@ -83,10 +83,25 @@ inline void fill_fused_consts(Consts& k) {
     k.cheat_th = k.cheat * k.cT;
 }
 
-// waves per SIMD the fused kernel is compiled for (its VGPR budget: 2 -> 256)
-#ifndef OSB_CF_WAVES
-#define OSB_CF_WAVES 2
-#endif
+// Waves per SIMD a fused-kernel variant is compiled for: as many workgroups
+// as its LDS box allows on a CU (160 KB), at most 4 waves per SIMD (a 128-VGPR
+// budget).  A 256-thread tile with five FP64 fields (51.2 KB) gets 3, with
+// six (Sutherland's mu or KGP) 2; a 512-thread 16x8x4 tile with five fields
+// (76.8 KB) gets 4.
+__host__ __device__ constexpr int cf_nfield(int conv, int visc) { return 4 + (conv != 0 ? 1 : 0) + (visc == 2 ? 1 : 0); }
+__host__ __device__ constexpr int cf_waves(int tx, int ty, int tz, int nfield) {
+    const int blocks_per_cu = 163840 / (nfield * (tx + 4) * (ty + 4) * (tz + 4) * 8);
+    const int waves = blocks_per_cu * (tx * ty * tz / 64) / 4;
+    return waves > 4 ? 4 : (waves < 1 ? 1 : waves);
+}
+
+// Scheduling fence between the sections of phase 2 (the velocity gradients,
+// each direction's convective terms, each momentum component's viscous
+// terms).  Without it the scheduler hoists the LDS reads of later sections
+// ahead to hide their latency, which takes the kernel to 248 VGPRs and 2
+// waves per SIMD; with it each section's values are loaded, used and dead
+// before the next, about 110 VGPRs, and occupancy hides the latency instead.
+__device__ __forceinline__ void cf_section() { __builtin_amdgcn_sched_barrier(0); }
 
 // index of the (a, b) pair in Consts::wm, a < b
 __host__ __device__ constexpr int cf_pair(int a, int b) { return a == 0 ? b - 1 : 2; }
@@ -110,7 +125,7 @@ __device__ __forceinline__ void cf_tile_stage(const int i0, const int j0, const 
     constexpr bool SUTH = VISC == 2;
     // LDS fields: u0, u1, u2, theta, [rho], [mu], one flat array
     constexpr int F_TH = 3, F_RHO = 4, F_MU = HAS_CONV ? 5 : 4;
-    constexpr int NFIELD = 4 + (HAS_CONV ? 1 : 0) + (SUTH ? 1 : 0);
+    constexpr int NFIELD = cf_nfield(CONV, VISC);
     __shared__ double s_f[NFIELD * CF_BOX];
 
     const long P = g.Ptot;
@@ -202,6 +217,7 @@ __device__ __forceinline__ void cf_tile_stage(const int i0, const int j0, const 
 #pragma unroll
         for (int jd = 0; jd < 3; ++jd)
             G[m][jd] = fd1([&](int s) { return S(m, s * ls[jd]); }, c.w1[jd]);
+    cf_section();
 
     if (HAS_CONV) {
         constexpr int FR = HAS_CONV ? F_RHO : 0;
@@ -245,6 +261,7 @@ __device__ __forceinline__ void cf_tile_stage(const int i0, const int j0, const 
                     const double d_ujum = fd1([&](int s) { return uj(s) * um_(m, s); }, w);
                     conv[m] += 0.25 * ((((((d_rhouj_um + umc * d_rhouj) + rhoujc * G[m][jd]) + ujc * d_rhoum) + rhoumc * gjj) +
                                         rho * d_ujum) + (umc * ujc) * d_rho);
+                    cf_section();
                 }
                 const double Hc = Hs(0);
                 const double d_Hrhouj = fd1([&](int s) { return Hs(s) * ruj(s); }, w);
@@ -272,10 +289,12 @@ __device__ __forceinline__ void cf_tile_stage(const int i0, const int j0, const 
                 const double d_pu = fd1([&](int s) { return ruj(s) * S(F_TH, s * lj); }, w);
                 eterm += -0.5 * (d_Eu + ujc * d_rE + rEs(0) * d_u) - d_pu;
             }
+            cf_section();
         }
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
             const double dp = fd1([&](int s) { return S(FR, s * ls[m]) * S(F_TH, s * ls[m]); }, c.w1[m]);
+            cf_section();
             if (CONV == CF_FEIEREISEN) Rv[1 + m] = -dp - 0.5 * (conv[m] + S(m, 0) * div_rhou);
             else Rv[1 + m] = -dp - conv[m];
         }
@@ -307,6 +326,7 @@ __device__ __forceinline__ void cf_tile_stage(const int i0, const int j0, const 
             dmu[jd] = SUTH ? fd1([&](int s) { return S(SUTH ? F_MU : 0, s * lj); }, c.w1[jd]) : 0.0;
         }
         heat *= muv;
+        cf_section();
         if (SUTH) {
 #pragma unroll
             for (int jd = 0; jd < 3; ++jd) heat += fd1([&](int s) { return S(F_TH, s * ls[jd]); }, c.w1[jd]) * dmu[jd];
@@ -337,6 +357,7 @@ __device__ __forceinline__ void cf_tile_stage(const int i0, const int j0, const 
             }
             Rv[1 + m] += c.invRe * mom;
             Eacc += fma(S(m, 0), mom, muv * e);
+            cf_section();
         }
         Rv[4] += c.invRe * Eacc + c.cheat_th * heat;
     }
@@ -368,7 +389,7 @@ __device__ __forceinline__ void cf_tile_stage(const int i0, const int j0, const 
 }
 
 template <int CF_TX, int CF_TY, int CF_TZ, int CONV, int VISC, bool FLUXRES, int TIME>
-__global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ, OSB_CF_WAVES)
+__global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ, cf_waves(CF_TX, CF_TY, CF_TZ, cf_nfield(CONV, VISC)))
 k_central_fused(const double* __restrict__ q, const double* __restrict__ F, double* __restrict__ reg, double* __restrict__ qout,
                 const double* __restrict__ scal, Geom g, Consts c, double A, double B, int xcd_remap, int wrap) {
     // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs, so

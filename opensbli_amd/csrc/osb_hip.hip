@@ -272,8 +272,8 @@ void launch_fused_tile(osb_ctx* c, const double* q, double* qn, double A, double
 template <int CONV, int VISC, bool FLUXRES, int TIME>
 void launch_fused_t(osb_ctx* c, const double* q, double* qn, double A, double B) {
     switch (c->fused_tile) {
-        case 1: launch_fused_tile<8, 8, 4, CONV, VISC, FLUXRES, TIME>(c, q, qn, A, B); break;
-        case 2: launch_fused_tile<32, 4, 2, CONV, VISC, FLUXRES, TIME>(c, q, qn, A, B); break;
+        case 1: launch_fused_tile<16, 8, 4, CONV, VISC, FLUXRES, TIME>(c, q, qn, A, B); break;
+        case 2: launch_fused_tile<8, 8, 8, CONV, VISC, FLUXRES, TIME>(c, q, qn, A, B); break;
         default: launch_fused_tile<16, 4, 4, CONV, VISC, FLUXRES, TIME>(c, q, qn, A, B); break;
     }
 }
