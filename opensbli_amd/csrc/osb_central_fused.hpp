@@ -210,28 +210,43 @@ __device__ __forceinline__ void cf_tile_stage(const int i0, const int j0, const 
         for (int v = 0; v < NV; ++v) Rv[v] = 0.0;
     }
 
-    // stored-gradient values G[m][jd] = D_jd u_m at the point
-    double G[3][3];
+    // Phase 2 runs in sections: one per direction jd, which reads only the
+    // fields along the jd line through the point (the gradient column
+    // G[.][jd], the jd part of the convective terms, of the pressure gradient,
+    // of the Laplacians and of the heat flux), then one per momentum component
+    // for the mixed second derivatives of the viscous stress.
+    constexpr int FR = HAS_CONV ? F_RHO : 0;
+    constexpr int FM = SUTH ? F_MU : 0;
+    double G[3][3];          // G[m][jd] = D_jd u_m
+    double Lm[3] = {0.0, 0.0, 0.0};      // sum_jd c_jd D_jd^2 u_m, c = 4/3 along m, else 1
+    double heat = 0.0, dth[3], dmu[3];
+    double div_rhou = 0.0, Rrho = 0.0, eterm = 0.0, dp[3];
+    double conv[3] = {0.0, 0.0, 0.0};
+    const double rho = HAS_CONV ? S(FR, 0) : 0.0;
 #pragma unroll
-    for (int m = 0; m < 3; ++m)
+    for (int jd = 0; jd < 3; ++jd) {
+        const int lj = ls[jd];
+        const double(&w)[2] = c.w1[jd];
 #pragma unroll
-        for (int jd = 0; jd < 3; ++jd)
-            G[m][jd] = fd1([&](int s) { return S(m, s * ls[jd]); }, c.w1[jd]);
-    cf_section();
-
-    if (HAS_CONV) {
-        constexpr int FR = HAS_CONV ? F_RHO : 0;
-        const double rho = S(FR, 0);
-        double div_rhou = 0.0, Rrho = 0.0, eterm = 0.0;
-        double conv[3] = {0.0, 0.0, 0.0};
+        for (int m = 0; m < 3; ++m) G[m][jd] = fd1([&](int s) { return S(m, s * lj); }, w);
+        if (VISC != 0) {
 #pragma unroll
-        for (int jd = 0; jd < 3; ++jd) {
-            const int lj = ls[jd];
-            const double(&w)[2] = c.w1[jd];
+            for (int m = 0; m < 3; ++m) {
+                const double d2 = fd2([&](int s) { return S(m, s * lj); }, c.w2[jd]);
+                Lm[m] += jd == m ? (4.0 / 3.0) * d2 : d2;
+            }
+            heat += fd2([&](int s) { return S(F_TH, s * lj); }, c.w2[jd]);
+            if (SUTH) {
+                dmu[jd] = fd1([&](int s) { return S(FM, s * lj); }, w);
+                dth[jd] = fd1([&](int s) { return S(F_TH, s * lj); }, w);
+            }
+        }
+        if (HAS_CONV) {
             auto rs = [&](int s) { return S(FR, s * lj); };
             auto uj = [&](int s) { return S(jd, s * lj); };
             auto ruj = [&](int s) { return rs(s) * uj(s); };
             auto um_ = [&](int m, int s) { return S(m, s * lj); };
+            dp[jd] = fd1([&](int s) { return rs(s) * S(F_TH, s * lj); }, w);
             if (CONV == CF_FEIEREISEN) {
                 auto Es = [&](int s) { return EH(c.cE, s * lj); };
                 const double d_rhouj = fd1(ruj, w);
@@ -261,7 +276,6 @@ __device__ __forceinline__ void cf_tile_stage(const int i0, const int j0, const 
                     const double d_ujum = fd1([&](int s) { return uj(s) * um_(m, s); }, w);
                     conv[m] += 0.25 * ((((((d_rhouj_um + umc * d_rhouj) + rhoujc * G[m][jd]) + ujc * d_rhoum) + rhoumc * gjj) +
                                         rho * d_ujum) + (umc * ujc) * d_rho);
-                    cf_section();
                 }
                 const double Hc = Hs(0);
                 const double d_Hrhouj = fd1([&](int s) { return Hs(s) * ruj(s); }, w);
@@ -289,65 +303,51 @@ __device__ __forceinline__ void cf_tile_stage(const int i0, const int j0, const 
                 const double d_pu = fd1([&](int s) { return ruj(s) * S(F_TH, s * lj); }, w);
                 eterm += -0.5 * (d_Eu + ujc * d_rE + rEs(0) * d_u) - d_pu;
             }
-            cf_section();
         }
+        cf_section();
+    }
+
+    if (HAS_CONV) {
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
-            const double dp = fd1([&](int s) { return S(FR, s * ls[m]) * S(F_TH, s * ls[m]); }, c.w1[m]);
-            cf_section();
-            if (CONV == CF_FEIEREISEN) Rv[1 + m] = -dp - 0.5 * (conv[m] + S(m, 0) * div_rhou);
-            else Rv[1 + m] = -dp - conv[m];
+            if (CONV == CF_FEIEREISEN) Rv[1 + m] = Rv[1 + m] - dp[m] - 0.5 * (conv[m] + S(m, 0) * div_rhou);
+            else Rv[1 + m] = Rv[1 + m] - dp[m] - conv[m];
         }
         if (CONV == CF_FEIEREISEN) {
-            Rv[0] = -div_rhou;
-            Rv[4] = eterm - 0.5 * EH(c.cE, 0) * div_rhou;
+            Rv[0] -= div_rhou;
+            Rv[4] += eterm - 0.5 * EH(c.cE, 0) * div_rhou;
         } else if (CONV == CF_KGP) {
-            Rv[0] = -Rrho;
-            Rv[4] = -eterm;
+            Rv[0] -= Rrho;
+            Rv[4] -= eterm;
         } else {
-            Rv[0] = Rrho;
-            Rv[4] = eterm;
+            Rv[0] += Rrho;
+            Rv[4] += eterm;
         }
     }
 
     if (VISC != 0) {
         // momentum: mu (lap u_m + 1/3 D_m div u) (+ S_mj D_j mu), energy:
         // u_m mom_m + mu S_mj G_mj, heat: mu lap T (+ D_j T D_j mu); T = cT theta
-        const double muv = SUTH ? S(SUTH ? F_MU : 0, 0) : 1.0;
-        double div = 0.0;
-#pragma unroll
-        for (int m = 0; m < 3; ++m) div += G[m][m];
-        double heat = 0.0, Eacc = 0.0;
-        double dmu[3];
-#pragma unroll
-        for (int jd = 0; jd < 3; ++jd) {
-            const int lj = ls[jd];
-            heat += fd2([&](int s) { return S(F_TH, s * lj); }, c.w2[jd]);
-            dmu[jd] = SUTH ? fd1([&](int s) { return S(SUTH ? F_MU : 0, s * lj); }, c.w1[jd]) : 0.0;
-        }
+        const double muv = SUTH ? S(FM, 0) : 1.0;
+        const double div = G[0][0] + G[1][1] + G[2][2];
         heat *= muv;
-        cf_section();
         if (SUTH) {
 #pragma unroll
-            for (int jd = 0; jd < 3; ++jd) heat += fd1([&](int s) { return S(F_TH, s * ls[jd]); }, c.w1[jd]) * dmu[jd];
+            for (int jd = 0; jd < 3; ++jd) heat += dth[jd] * dmu[jd];
         }
+        double Eacc = 0.0;
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
-            double Lm = 0.0;
+            // the mixed derivatives D_m D_jd u_jd, jd != m, in the (a, b) = sorted (jd, m) plane
+            double mixed = 0.0;
 #pragma unroll
             for (int jd = 0; jd < 3; ++jd) {
-                const double d2 = fd2([&](int s) { return S(m, s * ls[jd]); }, c.w2[jd]);
-                if (jd == m) {
-                    Lm += (4.0 / 3.0) * d2;
-                } else {
-                    // D_b(D_a u_jd), a < b the pair {jd, m}
-                    const int a = jd < m ? jd : m, b = jd < m ? m : jd;
-                    const int la = ls[a], lb = ls[b];
-                    const double mixed = fd11([&](int r, int s) { return S(jd, r * la + s * lb); }, c.wm[cf_pair(a, b)]);
-                    Lm += d2 + (1.0 / 3.0) * mixed;
-                }
+                if (jd == m) continue;
+                const int a = jd < m ? jd : m, b = jd < m ? m : jd;
+                const int la = ls[a], lb = ls[b];
+                mixed += fd11([&](int r, int s) { return S(jd, r * la + s * lb); }, c.wm[cf_pair(a, b)]);
             }
-            double mom = muv * Lm;
+            double mom = muv * fma(1.0 / 3.0, mixed, Lm[m]);
             double e = 0.0;
 #pragma unroll
             for (int jd = 0; jd < 3; ++jd) {
