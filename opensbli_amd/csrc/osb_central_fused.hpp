@@ -131,6 +131,22 @@ __device__ __forceinline__ void cf_tile_stage(const int i0, const int j0, const 
     const long P = g.Ptot;
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
 
+    // The RK operands of the thread's own point (du and q, or q_old) are
+    // loaded first, so that they arrive while phases 1 and 2 run.
+    const int tx = threadIdx.x % CF_TX, ty = (threadIdx.x / CF_TX) % CF_TY, tz = threadIdx.x / (CF_TX * CF_TY);
+    const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
+    const bool active = i < n0 && j < n1 && k < n2;
+    const long x = active ? gidx(g, i, j, k) : 0;
+    double r_pt[NV], q_pt[NV];
+    if (active) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            r_pt[v] = reg[(long)v * P + x];
+            if (TIME == 1) q_pt[v] = q[(long)v * P + x];
+        }
+    }
+    cf_section();
+
     // ---- phase 1: constituent relations on the halo box -------------------
     // Every box point is evaluated and stored (the corners are never read);
     // loads are clamped into the stored range, so past the high end of a
@@ -178,10 +194,7 @@ __device__ __forceinline__ void cf_tile_stage(const int i0, const int j0, const 
     // p = rho theta are rebuilt at the stencil point.  The reads share one base
     // address (the box point at offset -2, -2, -2 from the thread's point), so
     // each is that base plus a constant.
-    const int tx = threadIdx.x % CF_TX, ty = (threadIdx.x / CF_TX) % CF_TY, tz = threadIdx.x / (CF_TX * CF_TY);
-    const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
-    if (i >= n0 || j >= n1 || k >= n2) return;
-    const long x = gidx(g, i, j, k);
+    if (!active) return;
     constexpr int ls[3] = {1, CF_BX, CF_BX * CF_BY};
     constexpr int ORG = 2 * (ls[0] + ls[1] + ls[2]);
     const double* __restrict__ sb = s_f + (tx + CF_BX * (ty + CF_BY * tz));     // box index of the point, less ORG
@@ -368,9 +381,9 @@ __device__ __forceinline__ void cf_tile_stage(const int i0, const int j0, const 
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const long y = (long)v * P + x;
-            const double d = fma(A, reg[y], dt * Rv[v]);
+            const double d = fma(A, r_pt[v], dt * Rv[v]);
             reg[y] = d;
-            qv[v] = fma(B, d, q[y]);
+            qv[v] = fma(B, d, q_pt[v]);
             qout[y] = qv[v];
         }
     } else {
@@ -378,7 +391,7 @@ __device__ __forceinline__ void cf_tile_stage(const int i0, const int j0, const 
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const long y = (long)v * P + x;
-            const double o = reg[y];
+            const double o = r_pt[v];
             qv[v] = fma(dt * A, Rv[v], o);
             qout[y] = qv[v];
             reg[y] = fma(dt * B, Rv[v], o);
