@@ -411,7 +411,8 @@ def main():
     ap.add_argument("--tgv-ensemble-steps", type=int, default=200)
     ap.add_argument("--cpu-leg", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--copy-probe", action="store_true",
-                    help="after the timed run, copy a 256 MiB FP64 buffer 5 times (PMC byte-counter calibration)")
+                    help="after the timed run, copy a 2 GiB FP64 buffer of random values 5 times (PMC byte-counter "
+                         "calibration; larger than the 256 MB last-level cache)")
     args = ap.parse_args()
     if args.cpu_leg:
         cpu_leg_main(args.cpu_leg, args.cpu_budget)
@@ -476,7 +477,7 @@ def main():
         out["tgv_teno5_256"] = teno256_line()
     if args.copy_probe:
         import torch
-        a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        a = torch.rand(256 * 1024 * 1024, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)
         for _ in range(5):
             b.copy_(a)

@@ -53,7 +53,15 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
     const int w = blockIdx.x + ntx * (blockIdx.y + nty * blockIdx.z);
     int tile = w;
     if (xcd_remap && (ntiles & 7) == 0) tile = (w & 7) * (ntiles >> 3) + (w >> 3);
-    const int tbx = tile % ntx, tby = (tile / ntx) % nty, tbz = tile / (ntx * nty);
+    int tbx = tile % ntx, tby = (tile / ntx) % nty, tbz = tile / (ntx * nty);
+    if (xcd_remap == 2 && (ntx & 7) == 0 && (nty & 3) == 0 && (gridDim.z & 1) == 0) {
+        // supertiles of 8 x 4 x 2 tiles walked in x, y, z order within each XCD's range
+        const int in = tile & 63, sup = tile >> 6;
+        const int sx = ntx >> 3, sy = nty >> 2;
+        tbx = ((sup % sx) << 3) + (in & 7);
+        tby = (((sup / sx) % sy) << 2) + ((in >> 3) & 3);
+        tbz = ((sup / (sx * sy)) << 1) + (in >> 5);
+    }
     const int i0 = tbx * CF_TX, j0 = tby * CF_TY, k0 = tbz * CF_TZ;
     const long P = g.Ptot;
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
@@ -77,6 +85,22 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
         ref_r[3] = KGP ? r : rE;
         ref_mu = SUTH ? 1.4042 * ref_T * sqrt(ref_T) / (0.40417 + ref_T) : 1.0;
     }
+
+    // RK operands of the thread's point, loaded before phase 1 so that they
+    // arrive while phases 1 and 2 run
+    const int tx = threadIdx.x % CF_TX, ty = (threadIdx.x / CF_TX) % CF_TY, tz = threadIdx.x / (CF_TX * CF_TY);
+    const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
+    const bool active = i < n0 && j < n1 && k < n2;
+    const long x = active ? gidx(g, i, j, k) : 0;
+    double r_pt[NV], q_pt[NV];
+    if (active) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            r_pt[v] = reg[(long)v * P + x];
+            if (TIME == 1) q_pt[v] = q[(long)v * P + x];
+        }
+    }
+    cf_section();
 
     // ---- phase 1: constituent relations (FP64) -> FP32 deviations in LDS ----
     for (int b = threadIdx.x; b < CF_BOX; b += CF_THREADS) {
@@ -117,10 +141,7 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
     __syncthreads();
 
     // ---- phase 2: FP32 right-hand side, FP64 RK update --------------------
-    const int tx = threadIdx.x % CF_TX, ty = (threadIdx.x / CF_TX) % CF_TY, tz = threadIdx.x / (CF_TX * CF_TY);
-    const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
-    if (i >= n0 || j >= n1 || k >= n2) return;
-    const long x = gidx(g, i, j, k);
+    if (!active) return;
     const int l = (tx + 2) + CF_BX * ((ty + 2) + CF_BY * (tz + 2));
     const int ls[3] = {1, CF_BX, CF_BX * CF_BY};
     const float invd[3] = {float(c.invd[0]), float(c.invd[1]), float(c.invd[2])};
@@ -151,6 +172,7 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
     for (int m = 0; m < 3; ++m)
 #pragma unroll
         for (int jd = 0; jd < 3; ++jd) G[m][jd] = d1f([&](int s) { return s_u[m][l + s * ls[jd]]; }, invd[jd]);
+    cf_section();
 
     if (KGP) {
         // KGP, enthalpy splitting, coefficients 1/4 (split_forms.py:178-234) on deviations:
@@ -193,6 +215,7 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
             const float d_Huj = Href * gjj + urj * d_H + d1f([&](int s) { return hd(s) * ud(s); }, inv);
             eterm += 0.25f * ((((((d_Hrhouj + Hc * d_rhouj) + rhoujc * d_H) + ujc * d_Hrho) + (rhoc * Hc) * gjj) + rhoc * d_Huj) +
                               (ujc * Hc) * d_rho);
+            cf_section();
         }
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
@@ -234,6 +257,7 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
             const float d_pp = d1f([&](int s) { return s_p[HAS_CONV ? l + s * lj : 0]; }, inv);
             const float d_pu = float(ref_p) * d_uj + urj * d_pp + d1f([&](int s) { return s_p[HAS_CONV ? l + s * lj : 0] * ud(s); }, inv);
             eterm += -0.5f * (d_Eu + rc[jd] * d_E) - d_pu;
+            cf_section();
         }
         const float Ec = float(ref_E) + s_E[HAS_CONV ? l : 0];
 #pragma unroll
@@ -257,6 +281,7 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
             lapT += d2f([&](int s) { return s_T[VISC ? l + s * lj : 0]; }, inv2d[jd]);
 #pragma unroll
             for (int m = 0; m < 3; ++m) D2u[m][jd] = d2f([&](int s) { return s_u[m][l + s * lj]; }, inv2d[jd]);
+            cf_section();
         }
         float div = 0.f;
 #pragma unroll
@@ -290,6 +315,7 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
 #pragma unroll
             for (int jd = 0; jd < 3; ++jd) e += muv * S[jd] * G[m][jd];
             Eacc += e;
+            cf_section();
         }
         float heat = muv * lapT;
         if (SUTH) {
@@ -306,9 +332,9 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
         for (int v = 0; v < NV; ++v) {
             const long y = (long)v * P + x;
             const double R = Rd[v] + double(Rv[v]);
-            const double d = fma(A, reg[y], dt * R);
+            const double d = fma(A, r_pt[v], dt * R);
             reg[y] = d;
-            qv[v] = fma(B, d, q[y]);
+            qv[v] = fma(B, d, q_pt[v]);
             qout[y] = qv[v];
         }
     } else {
@@ -316,7 +342,7 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
         for (int v = 0; v < NV; ++v) {
             const long y = (long)v * P + x;
             const double R = Rd[v] + double(Rv[v]);
-            const double o = reg[y];
+            const double o = r_pt[v];
             qv[v] = fma(dt * A, R, o);
             qout[y] = qv[v];
             reg[y] = fma(dt * B, R, o);
