@@ -416,13 +416,15 @@ k_central_fused(const double* __restrict__ q, const double* __restrict__ F, doub
     if (xcd_remap && (ntiles & 7) == 0) tile = (w & 7) * (ntiles >> 3) + (w >> 3);
     int tbx = tile % ntx, tby = (tile / ntx) % nty, tbz = tile / (ntx * nty);
     if (xcd_remap == 2 && (ntx & 7) == 0 && (nty & 3) == 0 && (gridDim.z & 1) == 0) {
-        // supertiles of 8 x 4 x 2 tiles (one XCD's resident workgroups) walked in
-        // x, y, z order: a squarer resident region, so less of its halo leaves L2
+        // supertiles of 8 x 4 x 2 tiles (about one XCD's resident workgroups)
+        // walked z first: each XCD's range is then a slab of whole supertile
+        // columns, and consecutive supertiles share their z-halo planes while
+        // they are in L2 (a squarer resident region than single tiles)
         const int in = tile & 63, sup = tile >> 6;
-        const int sx = ntx >> 3, sy = nty >> 2;
-        tbx = ((sup % sx) << 3) + (in & 7);
-        tby = (((sup / sx) % sy) << 2) + ((in >> 3) & 3);
-        tbz = ((sup / (sx * sy)) << 1) + (in >> 5);
+        const int sz = gridDim.z >> 1, sy = nty >> 2;
+        tbz = ((sup % sz) << 1) + (in >> 5);
+        tby = (((sup / sz) % sy) << 2) + ((in >> 3) & 3);
+        tbx = ((sup / (sz * sy)) << 3) + (in & 7);
     }
     cf_tile_stage<CF_TX, CF_TY, CF_TZ, CONV, VISC, FLUXRES, TIME>(tbx * CF_TX, tby * CF_TY, tbz * CF_TZ, q, F, reg, qout, scal, g,
                                                                   c, A, B, wrap);

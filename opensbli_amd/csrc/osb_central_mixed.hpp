@@ -55,12 +55,12 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
     if (xcd_remap && (ntiles & 7) == 0) tile = (w & 7) * (ntiles >> 3) + (w >> 3);
     int tbx = tile % ntx, tby = (tile / ntx) % nty, tbz = tile / (ntx * nty);
     if (xcd_remap == 2 && (ntx & 7) == 0 && (nty & 3) == 0 && (gridDim.z & 1) == 0) {
-        // supertiles of 8 x 4 x 2 tiles walked in x, y, z order within each XCD's range
+        // supertiles of 8 x 4 x 2 tiles walked z first within each XCD's range (as k_central_fused)
         const int in = tile & 63, sup = tile >> 6;
-        const int sx = ntx >> 3, sy = nty >> 2;
-        tbx = ((sup % sx) << 3) + (in & 7);
-        tby = (((sup / sx) % sy) << 2) + ((in >> 3) & 3);
-        tbz = ((sup / (sx * sy)) << 1) + (in >> 5);
+        const int sz = gridDim.z >> 1, sy = nty >> 2;
+        tbz = ((sup % sz) << 1) + (in >> 5);
+        tby = (((sup / sz) % sy) << 2) + ((in >> 3) & 3);
+        tbx = ((sup / (sz * sy)) << 3) + (in & 7);
     }
     const int i0 = tbx * CF_TX, j0 = tby * CF_TY, k0 = tbz * CF_TZ;
     const long P = g.Ptot;
