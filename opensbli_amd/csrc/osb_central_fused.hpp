@@ -1,8 +1,10 @@
 // osb_central_fused.hpp -- one-launch RK stage for the central schemes on gfx950.
 //
-// One workgroup owns a TX x TY x TZ tile of grid points (16 x 4 x 4 by
-// default).  Phase 1 evaluates the constituent relations once per point of
-// the tile plus a 2-point halo box and stages them in LDS:
+// One workgroup owns a TX x TY x TZ tile of grid points (16 x 4 x 4,
+// 16 x 8 x 4 or 8 x 8 x 8, timed at a context's first step).  Each thread
+// first issues the loads of its own point's RK operands (du and q), which are
+// consumed only at the end.  Phase 1 evaluates the constituent relations
+// once per point of the tile plus a 2-point halo box and stages them in LDS:
 //   u0, u1, u2, theta = p/rho, rho        (mu as well for Sutherland's law)
 // five FP64 fields = 51.2 KB at the default tile.  Every other pointwise
 // quantity the stencils difference is rebuilt at the stencil point from
@@ -20,7 +22,9 @@
 // convective terms, viscous stresses and heat conduction, or the
 // interface-flux difference of a shock-capturing scheme -- followed directly
 // by the RK update, so the stage reads q, du (and the face fluxes) and writes
-// du and q once.
+// du and q once.  It runs in sections separated by scheduling fences (one per
+// direction, one per momentum component of the viscous stress), which keeps
+// the kernel near 130 VGPRs.
 #pragma once
 
 #include "osb_kernels.hpp"
