@@ -17,6 +17,15 @@
 // points per side like the reference's ops_dat declarations
 // (opensbli/schemes/spatial/scheme.py:48-61).
 //
+// Each function here is pinned by a check independent of this file: the
+// reference's known answers (the TENO6 128^3 README line, tg_sym, the Sod
+// exact solution), published tables (WENO5/TENO5 weights, RK-LS
+// coefficients, Carpenter's closure rows) and mathematical identities
+// (polynomial exactness and convergence order, L R = I and A R = R Lambda,
+// the Roe property, conservation, shift invariance, the CFL definition
+// recomputed from the state, a constant-viscosity shear wave); see
+// tests/test_oracle.py.  No function is left parity-unpinned.
+//
 // Build: see oracle/Makefile (g++ -O3 -fopenmp -ffp-contract=off).
 
 #include "../include/osb.h"
