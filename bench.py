@@ -278,7 +278,9 @@ def tgv_ensemble_bench(steps=200, count=48, seed=20261204):
         "sizes": sorted({p.desc.n[0] for p in probs}), "steps_timed": steps, "work_gpts_stage": work / 1e9,
         "plain_fp64": work / t_plain / 1e9, "batched_fp64": work / t_batch / 1e9,
         "speedup_batched_vs_plain": t_plain / t_batch, "batched_bitwise_equal_alone": same,
-        "groups": [{k: g[k] for k in ("kind", "members")} for g in info["groups"]], "solo": info["solo"],
+        "groups": [dict({k: g[k] for k in ("kind", "members")},
+                        **{k: g["tuning"][k] for k in ("tile", "tile_us") if k in g.get("tuning", {})})
+                   for g in info["groups"]], "solo": info["solo"],
     }
 
 

@@ -53,8 +53,13 @@ struct Group {
     int cap[BF_COUNT];                     // persistent grid size; 0 = one block per virtual block
     float tuned_us[BF_COUNT];
     int parity = 0;
+    int cf_tile = 0;                       // GK_CENTRAL: tile shape, an index into CF_TILES (timed by autotune)
+    float tile_us[CF_NTILES];
     hipGraphExec_t exec[2] = {nullptr, nullptr};
-    Group() { for (int f = 0; f < BF_COUNT; ++f) { cap[f] = 0; tuned_us[f] = 0.f; } }
+    Group() {
+        for (int f = 0; f < BF_COUNT; ++f) { cap[f] = 0; tuned_us[f] = 0.f; }
+        for (int t = 0; t < CF_NTILES; ++t) tile_us[t] = 0.f;
+    }
     int n() const { return (int)members.size(); }
     const int* dof(int f) const { return d_offs + (size_t)f * (members.size() + 1); }
     int total(int f) const { return offs[(size_t)f * (members.size() + 1) + members.size()]; }
@@ -125,11 +130,11 @@ int berr(osb_batch* b, int code, const std::string& m) {
     } while (0)
 
 // work of family f for one instance, in threads (BLOCK threads = one virtual
-// block)
-long family_work(const osb_ctx* c, int f) {
+// block); cf_tile: the central group's tile shape
+long family_work(const osb_ctx* c, int f, int cf_tile = 0) {
     const Geom& g = c->g;
     if (group_kind(c) == GK_CENTRAL) {
-        if (f == BF_RESID) return kb_central_tiles(g) * BLOCK;           // one virtual block per tile
+        if (f == BF_RESID) return kb_central_tiles(g, CF_TILES[cf_tile]) * BLOCK;   // one virtual block per tile
         if (f == BF_WAVE) return (long)g.n[0] * g.n[1] * g.n[2];
         return 0;
     }
@@ -184,7 +189,7 @@ int build_groups(osb_batch* b) {
             int acc = 0;
             for (int m = 0; m < n; ++m) {
                 gr.offs[(size_t)f * (n + 1) + m] = acc;
-                const long vb = vblocks(family_work(b->ctx[gr.members[m]], f));
+                const long vb = vblocks(family_work(b->ctx[gr.members[m]], f, gr.cf_tile));
                 if (acc + vb > 0x7fffffffL) return berr(b, OSB_E_INVALID, "batch too large for 32-bit block offsets");
                 acc += (int)vb;
             }
@@ -288,13 +293,24 @@ void launch_bcs(hipStream_t st, const Group& gr, int parity) {
     if (NDIM > 2) kb_bc<2><<<grid_of(gr, BF_BC2), BLOCK, 0, st>>>(gr.d_items, gr.dof(BF_BC2), gr.n(), parity);
 }
 
+template <int TX, int TY, int TZ, int CONV>
+void launch_central_t(hipStream_t st, const Group& gr, int parity, int stage) {
+    const unsigned grid = (unsigned)std::max(1, gr.total(BF_RESID));     // one workgroup per tile
+    const dim3 block(TX * TY * TZ);
+    switch (gr.visc) {
+        case OSB_VISC_NONE: kb_central_fused<TX, TY, TZ, CONV, 0><<<grid, block, 0, st>>>(gr.d_items, gr.dof(BF_RESID), gr.n(), parity, stage); break;
+        case OSB_VISC_CONSTANT: kb_central_fused<TX, TY, TZ, CONV, 1><<<grid, block, 0, st>>>(gr.d_items, gr.dof(BF_RESID), gr.n(), parity, stage); break;
+        default: kb_central_fused<TX, TY, TZ, CONV, 2><<<grid, block, 0, st>>>(gr.d_items, gr.dof(BF_RESID), gr.n(), parity, stage); break;
+    }
+}
+// the tile shapes of CF_TILES, in its order
 template <int CONV>
 void launch_central_v(hipStream_t st, const Group& gr, int parity, int stage) {
-    const unsigned grid = (unsigned)std::max(1, gr.total(BF_RESID));     // one workgroup per tile
-    switch (gr.visc) {
-        case OSB_VISC_NONE: kb_central_fused<CONV, 0><<<grid, KB_CF_TX * KB_CF_TY * KB_CF_TZ, 0, st>>>(gr.d_items, gr.dof(BF_RESID), gr.n(), parity, stage); break;
-        case OSB_VISC_CONSTANT: kb_central_fused<CONV, 1><<<grid, KB_CF_TX * KB_CF_TY * KB_CF_TZ, 0, st>>>(gr.d_items, gr.dof(BF_RESID), gr.n(), parity, stage); break;
-        default: kb_central_fused<CONV, 2><<<grid, KB_CF_TX * KB_CF_TY * KB_CF_TZ, 0, st>>>(gr.d_items, gr.dof(BF_RESID), gr.n(), parity, stage); break;
+    static_assert(CF_NTILES == 3, "one case per tile shape");
+    switch (gr.cf_tile) {
+        case 1: launch_central_t<16, 8, 4, CONV>(st, gr, parity, stage); break;
+        case 2: launch_central_t<8, 8, 8, CONV>(st, gr, parity, stage); break;
+        default: launch_central_t<16, 4, 4, CONV>(st, gr, parity, stage); break;
     }
 }
 
@@ -374,12 +390,64 @@ int capture_graphs(osb_batch* b, Group& gr) {
     return OSB_OK;
 }
 
-// Pick the persistent grid of each face-flux family: time each candidate on
-// the batch's own state (face fluxes only read q and overwrite F).
+// The central group's tile shape t: its stage launch covers every member's
+// tiles of that shape, so the stage family's virtual-block offsets follow it.
+int set_central_tile(osb_batch* b, Group& gr, int t) {
+    gr.cf_tile = t;
+    const int n = gr.n();
+    int acc = 0;
+    for (int m = 0; m < n; ++m) {
+        gr.offs[(size_t)BF_RESID * (n + 1) + m] = acc;
+        acc += (int)vblocks(family_work(b->ctx[gr.members[m]], BF_RESID, t));
+    }
+    gr.offs[(size_t)BF_RESID * (n + 1) + n] = acc;
+    B_TRY(b, hipMemcpyAsync(gr.d_offs + (size_t)BF_RESID * (n + 1), gr.offs.data() + (size_t)BF_RESID * (n + 1),
+                            sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream));
+    return OSB_OK;
+}
+
+// Launch shapes, timed on the batch's own state before its first step:
+// * shock groups: the persistent grid of each face-flux family (face fluxes
+//   only read q and overwrite F);
+// * 3-D central groups: the tile shape of the fused stage kernel (CF_TILES).
+//   The trial launches write the members' next buffer, which the first real
+//   stage overwrites, and their RK register, which is cleared afterwards
+//   (every RK-LS scheme's first stage multiplies it by zero, so only its
+//   finiteness matters).
 int autotune(osb_batch* b) {
     hipEvent_t e0, e1;
     B_TRY(b, hipEventCreate(&e0));
     B_TRY(b, hipEventCreate(&e1));
+    for (auto& gr : b->groups) {
+        if (gr.kind != GK_CENTRAL) continue;
+        auto run = [&]() {
+            if (gr.conv == CF_SKEW) launch_central_v<CF_SKEW>(b->stream, gr, gr.parity, 0);
+            else if (gr.conv == CF_KGP) launch_central_v<CF_KGP>(b->stream, gr, gr.parity, 0);
+            else launch_central_v<CF_FEIEREISEN>(b->stream, gr, gr.parity, 0);
+        };
+        float best = 1e30f;
+        int best_t = 0;
+        for (int t = 0; t < CF_NTILES; ++t) {
+            int r = set_central_tile(b, gr, t);
+            if (r) return r;
+            run();
+            B_TRY(b, hipEventRecord(e0, b->stream));
+            for (int k = 0; k < 3; ++k) run();
+            B_TRY(b, hipEventRecord(e1, b->stream));
+            B_TRY(b, hipEventSynchronize(e1));
+            float ms = 0.f;
+            B_TRY(b, hipEventElapsedTime(&ms, e0, e1));
+            gr.tile_us[t] = ms * 1000.f / 3.f;
+            if (gr.tile_us[t] < best) { best = gr.tile_us[t]; best_t = t; }
+        }
+        int r = set_central_tile(b, gr, best_t);
+        if (r) return r;
+        gr.tuned_us[BF_RESID] = best;
+        for (int m : gr.members) {
+            const osb_ctx* c = b->ctx[m];
+            B_TRY(b, hipMemsetAsync(c->du, 0, sizeof(double) * c->nv * c->g.Ptot, b->stream));
+        }
+    }
     for (auto& gr : b->groups) {
         if (gr.kind != GK_SHOCK) continue;                            // the fused stage kernels change du: not timed
         for (int d = 0; d < gr.ndim - (gr.fused ? 1 : 0); ++d) {     // the fused family changes du: not timed
@@ -576,6 +644,13 @@ int osb_batch_describe(osb_batch* b, char* buf, int len) {
                       gr.mixed, gr.time, gr.stages, gr.n());
         s += tmp;
         bool first = true;
+        if (gr.kind == GK_CENTRAL) {
+            const CfTile t = CF_TILES[gr.cf_tile];
+            std::snprintf(tmp, sizeof(tmp), "\"tile\": [%d, %d, %d], \"tile_us\": [%.2f, %.2f, %.2f]", t.tx, t.ty, t.tz,
+                          (double)gr.tile_us[0], (double)gr.tile_us[1], (double)gr.tile_us[2]);
+            s += tmp;
+            first = false;
+        }
         for (int f = 0; f < BF_COUNT; ++f) {
             if (gr.total(f) == 0 || (f >= BF_FACE0 + gr.ndim && f <= BF_FACE2)) continue;
             if (gr.fused && f == BF_FACE0 + gr.ndim - 1) continue;          // evaluated inside face_resid_rk
@@ -712,7 +787,7 @@ int launch_group_chunk(osb_pool* p, PoolGroup& pg, const std::vector<int>& membe
         long acc = 0;
         for (int m = 0; m < n; ++m) {
             g.offs[(size_t)f * (n + 1) + m] = (int)acc;
-            acc += vblocks(family_work(p->slots[members[m]].c, f));
+            acc += vblocks(family_work(p->slots[members[m]].c, f, g.cf_tile));
             if (acc > 0x7fffffffL) return perr(p, OSB_E_INVALID, "pool group too large for 32-bit block offsets");
         }
         g.offs[(size_t)f * (n + 1) + n] = (int)acc;

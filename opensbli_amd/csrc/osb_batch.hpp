@@ -176,10 +176,9 @@ __global__ void __launch_bounds__(BLOCK) kb_glf(const BItem* __restrict__ items,
 }
 
 // 3-D central RK-LS stage of every item, all faces periodic: one virtual
-// block is one 16 x 4 x 4 tile of an item (x, then y, then z), evaluated by
-// the single-block kernel's tile code, which also writes the wrapped halo
-// images; the LDS box is reused by the workgroup's next tile after a barrier
-constexpr int KB_CF_TX = 16, KB_CF_TY = 4, KB_CF_TZ = 4;
+// block is one TX x TY x TZ tile of an item (x, then y, then z; a tile shape
+// of CF_TILES, timed by the batch's autotuner), evaluated by the single-block
+// kernel's tile code, which also writes the wrapped halo images.
 // Workgroup v of a launch runs on XCD v % 8.  This permutation of [0, total)
 // gives XCD x the contiguous range of virtual blocks [x*(q+1), (x+1)*(q+1)) for
 // x < r and the ranges of q after them (total = 8q + r), so neighbouring tiles
@@ -188,22 +187,22 @@ __device__ __forceinline__ int xcd_contiguous(int v, int total) {
     const int q = total >> 3, r = total & 7, x = v & 7, k = v >> 3;
     return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
-template <int CONV, int VISC>
-__global__ void __launch_bounds__(KB_CF_TX * KB_CF_TY * KB_CF_TZ, cf_waves(KB_CF_TX, KB_CF_TY, KB_CF_TZ, cf_nfield(CONV, VISC)))
+template <int TX, int TY, int TZ, int CONV, int VISC>
+__global__ void __launch_bounds__(TX * TY * TZ, cf_waves(TX, TY, TZ, cf_nfield(CONV, VISC)))
 kb_central_fused(const BItem* __restrict__ items, const int* __restrict__ offs, int n, int parity, int stage) {
     // one workgroup per tile of every member (the launch covers offs[n] tiles)
     const int vb = xcd_contiguous(blockIdx.x, offs[n]);
     const int i = find_item(offs, n, vb);
     CBItem& it = item_const(items, i);
     const int tile = vb - offs[i];
-    const int ntx = (it.g.n[0] + KB_CF_TX - 1) / KB_CF_TX, nty = (it.g.n[1] + KB_CF_TY - 1) / KB_CF_TY;
+    const int ntx = (it.g.n[0] + TX - 1) / TX, nty = (it.g.n[1] + TY - 1) / TY;
     const int tbx = tile % ntx, tby = (tile / ntx) % nty, tbz = tile / (ntx * nty);
-    cf_tile_stage<KB_CF_TX, KB_CF_TY, KB_CF_TZ, CONV, VISC, false, OSB_TIME_RKLS3>(
-        tbx * KB_CF_TX, tby * KB_CF_TY, tbz * KB_CF_TZ, it.q[parity], nullptr, it.du, it.q[parity ^ 1], it.scal, it.g, it.c,
-        it.rkA[stage], it.rkB[stage], 1);
+    cf_tile_stage<TX, TY, TZ, CONV, VISC, false, OSB_TIME_RKLS3>(tbx * TX, tby * TY, tbz * TZ, it.q[parity], nullptr, it.du,
+                                                                 it.q[parity ^ 1], it.scal, it.g, it.c, it.rkA[stage],
+                                                                 it.rkB[stage], 1);
 }
-__host__ inline long kb_central_tiles(const Geom& g) {
-    return (long)((g.n[0] + KB_CF_TX - 1) / KB_CF_TX) * ((g.n[1] + KB_CF_TY - 1) / KB_CF_TY) * ((g.n[2] + KB_CF_TZ - 1) / KB_CF_TZ);
+__host__ inline long kb_central_tiles(const Geom& g, const CfTile& t) {
+    return (long)((g.n[0] + t.tx - 1) / t.tx) * ((g.n[1] + t.ty - 1) / t.ty) * ((g.n[2] + t.tz - 1) / t.tz);
 }
 
 static __global__ void kb_dt_finalize(const BItem* __restrict__ items, int n) {

@@ -113,6 +113,11 @@ def test_batch_bitwise_equals_single(hip, graphs, monkeypatch):
     assert info["solo"] == 4 and len(info["groups"]) >= 8
     central = [g for g in info["groups"] if g["kind"] == "central"]
     assert len(central) == 4 and sum(g["members"] for g in central) == 7
+    for g in central:
+        # each central group's tile shape was timed on its own members before the first step
+        tu = g["tuning"]
+        assert tu["tile"] in ([16, 4, 4], [16, 8, 4], [8, 8, 8]) and min(tu["tile_us"]) > 0, tu
+        assert tu["tile"] == [[16, 4, 4], [16, 8, 4], [8, 8, 8]][int(np.argmin(tu["tile_us"]))], tu
     for p, q, st in zip(probs, got, stats):
         _announce(f"  compare {p.name}")
         want, wst = run(hip, p, steps=5)
