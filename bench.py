@@ -237,6 +237,7 @@ def ensemble_bench(steps, count=64, seed=20261203):
         "speedup_batched_mixed_vs_plain_fp64": res["batched_mixed"] * t_plain / work * 1e9,
         "mixed_rel_l2_vs_fp64_worst_instance": {"state": worst_state, "per_variable_max": worst_var},
         "autotuned_face_flux_grid": {k: v["grid"] for k, v in tuning.items() if k.startswith("face_flux")},
+        "autotuned_face_resid_width": {k: tuning[k] for k in ("resid_w", "resid_w_us") if k in tuning},
     }
 
 

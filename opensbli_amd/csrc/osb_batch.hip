@@ -55,10 +55,13 @@ struct Group {
     int parity = 0;
     int cf_tile = 0;                       // GK_CENTRAL: tile shape, an index into CF_TILES (timed by autotune)
     float tile_us[CF_NTILES];
+    int resid_w = FUSED_FW;                // GK_SHOCK, fused: lines per face-residual tile (RESID_WS, timed by autotune)
+    float resid_w_us[RESID_NW];
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     Group() {
         for (int f = 0; f < BF_COUNT; ++f) { cap[f] = 0; tuned_us[f] = 0.f; }
         for (int t = 0; t < CF_NTILES; ++t) tile_us[t] = 0.f;
+        for (int t = 0; t < RESID_NW; ++t) resid_w_us[t] = 0.f;
     }
     int n() const { return (int)members.size(); }
     const int* dof(int f) const { return d_offs + (size_t)f * (members.size() + 1); }
@@ -130,8 +133,9 @@ int berr(osb_batch* b, int code, const std::string& m) {
     } while (0)
 
 // work of family f for one instance, in threads (BLOCK threads = one virtual
-// block); cf_tile: the central group's tile shape
-long family_work(const osb_ctx* c, int f, int cf_tile = 0) {
+// block); cf_tile: the central group's tile shape; resid_w: the fused
+// face-residual tiles' width
+long family_work(const osb_ctx* c, int f, int cf_tile = 0, int resid_w = FUSED_FW) {
     const Geom& g = c->g;
     if (group_kind(c) == GK_CENTRAL) {
         if (f == BF_RESID) return kb_central_tiles(g, CF_TILES[cf_tile]) * BLOCK;   // one virtual block per tile
@@ -147,7 +151,9 @@ long family_work(const osb_ctx* c, int f, int cf_tile = 0) {
         case BF_BC2: return g.ndim > 2 ? bc_work<2>(g) : 0;
         case BF_RESID:
             if (c->shock_fused) {                            // fused last-direction tiles
-                const long t = g.ndim == 1 ? fused_tiles<1>(g) : (g.ndim == 2 ? fused_tiles<2>(g) : fused_tiles<3>(g));
+                const long t = g.ndim == 1 ? fused_tiles<1>(g)
+                                           : (resid_w == 16 ? (g.ndim == 2 ? fused_tiles<2, 16>(g) : fused_tiles<3, 16>(g))
+                                                            : (g.ndim == 2 ? fused_tiles<2>(g) : fused_tiles<3>(g)));
                 return t * BLOCK;
             }
             return (long)g.n[0] * g.n[1] * g.n[2];
@@ -189,7 +195,7 @@ int build_groups(osb_batch* b) {
             int acc = 0;
             for (int m = 0; m < n; ++m) {
                 gr.offs[(size_t)f * (n + 1) + m] = acc;
-                const long vb = vblocks(family_work(b->ctx[gr.members[m]], f, gr.cf_tile));
+                const long vb = vblocks(family_work(b->ctx[gr.members[m]], f, gr.cf_tile, gr.resid_w));
                 if (acc + vb > 0x7fffffffL) return berr(b, OSB_E_INVALID, "batch too large for 32-bit block offsets");
                 acc += (int)vb;
             }
@@ -264,20 +270,26 @@ void launch_face_dir(hipStream_t st, const Group& gr, int dir, int parity, unsig
     else launch_face_p<NDIM, (NDIM > 2 ? 2 : 0)>(st, gr, parity, grid);
 }
 
-template <int NDIM, int TIME, typename T>
-void launch_face_resid(hipStream_t st, const Group& gr, int parity, int stage) {
+template <int NDIM, int TIME, typename T, int W>
+void launch_face_resid_w(hipStream_t st, const Group& gr, int parity, int stage) {
     const unsigned grid = grid_of(gr, BF_RESID);
     switch (gr.sch) {
         case SCH_WENOZ:
-            kb_face_resid<NDIM, SCH_WENOZ, T, TIME><<<grid, BLOCK, 0, st>>>(gr.d_items, gr.dof(BF_RESID), gr.n(), parity, stage);
+            kb_face_resid<NDIM, SCH_WENOZ, T, TIME, W><<<grid, BLOCK, 0, st>>>(gr.d_items, gr.dof(BF_RESID), gr.n(), parity, stage);
             break;
         case SCH_TENO5:
-            kb_face_resid<NDIM, SCH_TENO5, T, TIME><<<grid, BLOCK, 0, st>>>(gr.d_items, gr.dof(BF_RESID), gr.n(), parity, stage);
+            kb_face_resid<NDIM, SCH_TENO5, T, TIME, W><<<grid, BLOCK, 0, st>>>(gr.d_items, gr.dof(BF_RESID), gr.n(), parity, stage);
             break;
         default:
-            kb_face_resid<NDIM, SCH_TENO6, T, TIME><<<grid, BLOCK, 0, st>>>(gr.d_items, gr.dof(BF_RESID), gr.n(), parity, stage);
+            kb_face_resid<NDIM, SCH_TENO6, T, TIME, W><<<grid, BLOCK, 0, st>>>(gr.d_items, gr.dof(BF_RESID), gr.n(), parity, stage);
             break;
     }
+}
+template <int NDIM, int TIME, typename T>
+void launch_face_resid(hipStream_t st, const Group& gr, int parity, int stage) {
+    static_assert(RESID_NW == 2 && RESID_WS[0] == 32 && RESID_WS[1] == 16, "one case per width");
+    if (NDIM > 1 && gr.resid_w == 16) launch_face_resid_w<NDIM, TIME, T, 16>(st, gr, parity, stage);
+    else launch_face_resid_w<NDIM, TIME, T, FUSED_FW>(st, gr, parity, stage);
 }
 
 template <int NDIM, int TIME>
@@ -406,14 +418,32 @@ int set_central_tile(osb_batch* b, Group& gr, int t) {
     return OSB_OK;
 }
 
+// The fused shock group's face-residual width w: its launch covers every
+// member's tiles of that width, so that family's offsets follow it.
+int set_resid_w(osb_batch* b, Group& gr, int w) {
+    gr.resid_w = w;
+    const int n = gr.n();
+    int acc = 0;
+    for (int m = 0; m < n; ++m) {
+        gr.offs[(size_t)BF_RESID * (n + 1) + m] = acc;
+        acc += (int)vblocks(family_work(b->ctx[gr.members[m]], BF_RESID, gr.cf_tile, w));
+    }
+    gr.offs[(size_t)BF_RESID * (n + 1) + n] = acc;
+    B_TRY(b, hipMemcpyAsync(gr.d_offs + (size_t)BF_RESID * (n + 1), gr.offs.data() + (size_t)BF_RESID * (n + 1),
+                            sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream));
+    return OSB_OK;
+}
+
 // Launch shapes, timed on the batch's own state before its first step:
 // * shock groups: the persistent grid of each face-flux family (face fluxes
 //   only read q and overwrite F);
-// * 3-D central groups: the tile shape of the fused stage kernel (CF_TILES).
-//   The trial launches write the members' next buffer, which the first real
-//   stage overwrites, and their RK register, which is cleared afterwards
-//   (every RK-LS scheme's first stage multiplies it by zero, so only its
-//   finiteness matters).
+// * 3-D central groups: the tile shape of the fused stage kernel (CF_TILES);
+// * 2-D/3-D shock groups with the fused last direction: the width of the
+//   face-residual tiles (RESID_WS).
+// The trial launches of the last two write the members' next buffer, which
+// the first real stage overwrites, and their RK register, which is cleared
+// afterwards (every RK-LS scheme's first stage multiplies it by zero, so
+// only its finiteness matters; classic RK3 saves the state into it first).
 int autotune(osb_batch* b) {
     hipEvent_t e0, e1;
     B_TRY(b, hipEventCreate(&e0));
@@ -441,6 +471,40 @@ int autotune(osb_batch* b) {
             if (gr.tile_us[t] < best) { best = gr.tile_us[t]; best_t = t; }
         }
         int r = set_central_tile(b, gr, best_t);
+        if (r) return r;
+        gr.tuned_us[BF_RESID] = best;
+        for (int m : gr.members) {
+            const osb_ctx* c = b->ctx[m];
+            B_TRY(b, hipMemsetAsync(c->du, 0, sizeof(double) * c->nv * c->g.Ptot, b->stream));
+        }
+    }
+    for (auto& gr : b->groups) {
+        if (gr.kind != GK_SHOCK || !gr.fused || gr.ndim < 2) continue;
+        auto run = [&]() {
+            if (gr.ndim == 2) {
+                if (gr.time == OSB_TIME_RK3) launch_face_resid_p<2, 2>(b->stream, gr, gr.parity, 0);
+                else launch_face_resid_p<2, 1>(b->stream, gr, gr.parity, 0);
+            } else {
+                if (gr.time == OSB_TIME_RK3) launch_face_resid_p<3, 2>(b->stream, gr, gr.parity, 0);
+                else launch_face_resid_p<3, 1>(b->stream, gr, gr.parity, 0);
+            }
+        };
+        float best = 1e30f;
+        int best_w = FUSED_FW;
+        for (int t = 0; t < RESID_NW; ++t) {
+            int r = set_resid_w(b, gr, RESID_WS[t]);
+            if (r) return r;
+            run();
+            B_TRY(b, hipEventRecord(e0, b->stream));
+            for (int k = 0; k < 3; ++k) run();
+            B_TRY(b, hipEventRecord(e1, b->stream));
+            B_TRY(b, hipEventSynchronize(e1));
+            float ms = 0.f;
+            B_TRY(b, hipEventElapsedTime(&ms, e0, e1));
+            gr.resid_w_us[t] = ms * 1000.f / 3.f;
+            if (gr.resid_w_us[t] < best) { best = gr.resid_w_us[t]; best_w = RESID_WS[t]; }
+        }
+        int r = set_resid_w(b, gr, best_w);
         if (r) return r;
         gr.tuned_us[BF_RESID] = best;
         for (int m : gr.members) {
@@ -644,6 +708,12 @@ int osb_batch_describe(osb_batch* b, char* buf, int len) {
                       gr.mixed, gr.time, gr.stages, gr.n());
         s += tmp;
         bool first = true;
+        if (gr.kind == GK_SHOCK && gr.fused && gr.ndim >= 2) {
+            std::snprintf(tmp, sizeof(tmp), "\"resid_w\": %d, \"resid_w_us\": [%.2f, %.2f]", gr.resid_w, (double)gr.resid_w_us[0],
+                          (double)gr.resid_w_us[1]);
+            s += tmp;
+            first = false;
+        }
         if (gr.kind == GK_CENTRAL) {
             const CfTile t = CF_TILES[gr.cf_tile];
             std::snprintf(tmp, sizeof(tmp), "\"tile\": [%d, %d, %d], \"tile_us\": [%.2f, %.2f, %.2f]", t.tx, t.ty, t.tz,
@@ -787,7 +857,7 @@ int launch_group_chunk(osb_pool* p, PoolGroup& pg, const std::vector<int>& membe
         long acc = 0;
         for (int m = 0; m < n; ++m) {
             g.offs[(size_t)f * (n + 1) + m] = (int)acc;
-            acc += vblocks(family_work(p->slots[members[m]].c, f, g.cf_tile));
+            acc += vblocks(family_work(p->slots[members[m]].c, f, g.cf_tile, g.resid_w));
             if (acc > 0x7fffffffL) return perr(p, OSB_E_INVALID, "pool group too large for 32-bit block offsets");
         }
         g.offs[(size_t)f * (n + 1) + n] = (int)acc;

@@ -66,14 +66,15 @@ __global__ void __launch_bounds__(BLOCK) kb_face_flux(const BItem* __restrict__ 
 }
 
 // last direction's faces fused with the update: one virtual block is one tile
-template <int NDIM, int SCH, typename T, int TIME>
+// of W lines (a width of RESID_WS, timed by the batch's autotuner)
+template <int NDIM, int SCH, typename T, int TIME, int W>
 __global__ void __launch_bounds__(BLOCK) kb_face_resid(const BItem* __restrict__ items, const int* __restrict__ offs, int n,
                                                        int parity, int stage) {
     OSB_BATCH_LOOP {
         const int i = find_item(offs, n, vb);
         CBItem& it = item_const(items, i);
-        face_resid_tile<NDIM, SCH, T, TIME, FUSED_FW>(vb - offs[i], it.q[parity], it.F, it.q[parity ^ 1], it.du, it.scal[1], it.g, it.c,
-                                            it.rkA[stage], it.rkB[stage]);
+        face_resid_tile<NDIM, SCH, T, TIME, W>(vb - offs[i], it.q[parity], it.F, it.q[parity ^ 1], it.du, it.scal[1], it.g, it.c,
+                                               it.rkA[stage], it.rkB[stage]);
     }
 }
 

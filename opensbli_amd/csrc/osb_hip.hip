@@ -193,7 +193,20 @@ void launch_face_resid_w(osb_ctx* c, const double* q, double* qn, double A, doub
 template <int NDIM, int TIME, typename T>
 void launch_face_resid(osb_ctx* c, const double* q, double* qn, double A, double B) {
     Timed tm(c, PF_FACE_RESID);
-    launch_face_resid_w<NDIM, TIME, T, FUSED_FW>(c, q, qn, A, B);
+    static_assert(RESID_NW == 2 && RESID_WS[0] == 32 && RESID_WS[1] == 16, "one case per width");
+    if (NDIM > 1 && c->resid_w == 16) launch_face_resid_w<NDIM, TIME, T, 16>(c, q, qn, A, B);
+    else launch_face_resid_w<NDIM, TIME, T, FUSED_FW>(c, q, qn, A, B);
+}
+// the fused face-residual launch alone (launch-shape timing)
+void launch_face_resid_any(osb_ctx* c, const double* q, double* qn, double A, double B) {
+    const bool rk3 = c->d.time_scheme == OSB_TIME_RK3;
+    if (c->g.ndim == 2) {
+        if (c->mixed) { if (rk3) launch_face_resid<2, 2, float>(c, q, qn, A, B); else launch_face_resid<2, 1, float>(c, q, qn, A, B); }
+        else { if (rk3) launch_face_resid<2, 2, double>(c, q, qn, A, B); else launch_face_resid<2, 1, double>(c, q, qn, A, B); }
+    } else {
+        if (c->mixed) { if (rk3) launch_face_resid<3, 2, float>(c, q, qn, A, B); else launch_face_resid<3, 1, float>(c, q, qn, A, B); }
+        else { if (rk3) launch_face_resid<3, 2, double>(c, q, qn, A, B); else launch_face_resid<3, 1, double>(c, q, qn, A, B); }
+    }
 }
 
 template <int NDIM, int TIME>
@@ -377,8 +390,63 @@ void fused_kernel_only(osb_ctx* c, const double* q, double* qn, double A, double
     else launch_fused_v<CF_FEIEREISEN, false, TIME>(c, q, qn, A, B);
 }
 
+// Width of the fused face-residual tiles (2-D/3-D shock capturing without
+// viscous terms), timed like the tile shapes: each RESID_WS width runs the
+// launch once untimed and three times timed on the context's state (writing
+// the next buffer and the RK register, which is cleared afterwards), the
+// fastest kept and remembered per kernel variant and grid.  OSB_RESID_W=<16|32>
+// forces a width.
+std::map<TileKey, std::pair<int, std::vector<float>>> g_resid_cache;
+
+int tune_resid_w(osb_ctx* c) {
+    const TileKey key{c->sch, (int)c->mixed, c->g.ndim, c->d.time_scheme, c->g.n[0], c->g.n[1], c->g.n[2]};
+    {
+        std::lock_guard<std::mutex> lock(g_tile_mu);
+        auto it = g_resid_cache.find(key);
+        if (it != g_resid_cache.end()) {
+            c->resid_w = it->second.first;
+            c->resid_w_us = it->second.second;
+            return OSB_OK;
+        }
+    }
+    double* qn = (c->q_cur == c->qa) ? c->qb : c->qa;
+    const bool rk3 = c->d.time_scheme == OSB_TIME_RK3;
+    const double A = rk3 ? 2.0 / 3.0 : c->rkA[0], B = rk3 ? 1.0 / 4.0 : c->rkB[0];
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(c, hipEventCreate(&e0));
+    hipError_t ec = hipEventCreate(&e1);
+    if (ec != hipSuccess) { (void)hipEventDestroy(e0); HIP_TRY(c, ec); }
+    c->resid_w_us.assign(RESID_NW, 0.f);
+    hipError_t e = hipSuccess;
+    int best_w = FUSED_FW;
+    float best = 1e30f;
+    for (int t = 0; t < RESID_NW && e == hipSuccess; ++t) {
+        c->resid_w = RESID_WS[t];
+        launch_face_resid_any(c, c->q_cur, qn, A, B);
+        e = hipEventRecord(e0, c->stream);
+        for (int r = 0; r < 3 && e == hipSuccess; ++r) launch_face_resid_any(c, c->q_cur, qn, A, B);
+        if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        c->resid_w_us[t] = ms * 1000.f / 3.f;
+        if (c->resid_w_us[t] < best) { best = c->resid_w_us[t]; best_w = RESID_WS[t]; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIP_TRY(c, e);
+    c->resid_w = best_w;
+    HIP_TRY(c, hipMemsetAsync(c->du, 0, sizeof(double) * c->nv * c->g.Ptot, c->stream));
+    int r = check_launch(c);
+    if (r) return r;
+    std::lock_guard<std::mutex> lock(g_tile_mu);
+    g_resid_cache[key] = {best_w, c->resid_w_us};
+    return OSB_OK;
+}
+
 int tune_fused_tile(osb_ctx* c) {
     c->tile_tuned = true;
+    if (use_shock_stage(c) && c->shock_fused && c->g.ndim >= 2) return c->resid_forced ? OSB_OK : tune_resid_w(c);
     if (!use_fused(c) || c->mixed || c->tile_forced) return OSB_OK;
     const TileKey key{c->d.conv_scheme, c->d.viscous, (int)c->shock, c->d.time_scheme, c->g.n[0], c->g.n[1], c->g.n[2]};
     {
@@ -765,6 +833,11 @@ void init_ctx(osb_ctx* c, const osb_problem_desc* desc) {
     if (const char* e = std::getenv("OSB_FUSED_TILE")) {
         c->fused_tile = std::max(0, std::min(CF_NTILES - 1, std::atoi(e)));
         c->tile_forced = true;
+    }
+    if (const char* e = std::getenv("OSB_RESID_W")) {
+        const int w = std::atoi(e);
+        for (int t = 0; t < RESID_NW; ++t)
+            if (RESID_WS[t] == w) { c->resid_w = w; c->resid_forced = true; }
     }
     if (const char* e = std::getenv("OSB_GRAPH")) c->use_graph = std::atoi(e) != 0;
     if (desc->flux_type >= OSB_FLUX_HLLC) c->shock_fused = false;   // HLLC faces run their own kernel and the separate update
@@ -1181,6 +1254,17 @@ int osb_describe(osb_ctx* c, char* buf, int len) {
                                     : (use_shock_stage(c) ? (c->shock_fused ? "face_flux+face_resid" : "face_flux+resid_rk")
                                                           : "unfused");
     std::string s = std::string("{\"path\": \"") + path + "\"";
+    if (use_shock_stage(c) && c->shock_fused && c->g.ndim >= 2) {
+        char tmp[160];
+        std::snprintf(tmp, sizeof(tmp), ", \"resid_w\": %d, \"resid_w_choice\": \"%s\", \"resid_w_us\": [", c->resid_w,
+                      c->resid_forced ? "forced" : (c->tile_tuned ? "autotuned" : "default"));
+        s += tmp;
+        for (size_t i = 0; i < c->resid_w_us.size(); ++i) {
+            std::snprintf(tmp, sizeof(tmp), "%s%.2f", i ? ", " : "", (double)c->resid_w_us[i]);
+            s += tmp;
+        }
+        s += "]";
+    }
     if (use_fused(c) && !c->mixed) {
         const CfTile t = CF_TILES[c->fused_tile];
         char tmp[160];

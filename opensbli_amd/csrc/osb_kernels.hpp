@@ -295,6 +295,9 @@ __global__ void __launch_bounds__(BLOCK) k_resid_rk(const double* __restrict__ F
 // and 8 x 32 on KH 256^2 and 1024^2 (coalescing along x outweighs the
 // recomputed face, one in eight)
 constexpr int FUSED_FW = 32;
+// widths offered to the launch-shape timing of contexts and batches (2-D/3-D)
+constexpr int RESID_WS[] = {32, 16};
+constexpr int RESID_NW = 2;
 
 template <int NDIM, int W = FUSED_FW>
 struct FusedTile {

@@ -58,6 +58,9 @@ struct osb_ctx {
     bool tile_tuned = false;        // the tile choice has been made
     bool tile_forced = false;       // OSB_FUSED_TILE chose it
     std::vector<float> tile_us;     // timed microseconds per launch of each CF_TILES shape
+    int resid_w = osb::FUSED_FW;    // lines per tile of the fused face-residual kernel (RESID_WS), timed at the first step
+    std::vector<float> resid_w_us;  // timed microseconds per launch of each RESID_WS width
+    bool resid_forced = false;      // OSB_RESID_W chose it
     int xcd_remap = 2;              // XCD-aware workgroup-to-tile order: 8x4x2 supertiles per XCD range
 
     bool shock_fused = true;        // shock-only stages: last direction's faces fused with the update (HLLC: separate)

@@ -113,6 +113,12 @@ def test_batch_bitwise_equals_single(hip, graphs, monkeypatch):
     assert info["solo"] == 4 and len(info["groups"]) >= 8
     central = [g for g in info["groups"] if g["kind"] == "central"]
     assert len(central) == 4 and sum(g["members"] for g in central) == 7
+    fused = [g for g in info["groups"] if g["kind"] == "shock" and g["ndim"] >= 2 and "resid_w" in g["tuning"]]
+    assert fused
+    for g in fused:
+        # the fused shock groups' face-residual width was timed likewise
+        tu = g["tuning"]
+        assert tu["resid_w"] == [32, 16][int(np.argmin(tu["resid_w_us"]))] and min(tu["resid_w_us"]) > 0, tu
     for g in central:
         # each central group's tile shape was timed on its own members before the first step
         tu = g["tuning"]

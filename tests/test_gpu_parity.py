@@ -529,6 +529,53 @@ def test_fused_stage_tiles_against_oracle(hip, oracle, tile, monkeypatch):
         _compare(hip, oracle, pr)
 
 
+@pytest.mark.parametrize("width", ["16", "32"])
+def test_face_resid_width_bitwise(hip, oracle, width, monkeypatch):
+    """Both widths of the fused face-residual tiles (forced with OSB_RESID_W)
+    round exactly like the oracle: 2-D on a box that fills neither width's
+    tiles (WENO-Z, KH; TENO5 on a Riemann problem) and 3-D inviscid WENO-Z."""
+    monkeypatch.setenv("OSB_RESID_W", width)
+    kh = P.kelvin_helmholtz(n=40, steps=6)
+    rm = P.riemann_2d(n=36, steps=6)
+    rm.desc.conv_scheme = nat.CONV_TENO
+    rm.name += "_teno5"
+    w3 = P.tgv(n=20, steps=4, scheme="weno", minf=1.25, cfl=0.5, dt=0.0)
+    w3.desc.viscous = nat.VISC_NONE
+    w3.name += "_inviscid"
+    for pr in (kh, rm, w3):
+        with hip.context(pr.desc) as c:
+            c.set_state(pr.q0)
+            c.step(1)
+            info = c.describe()
+        assert info["resid_w"] == int(width) and info["resid_w_choice"] == "forced", info
+        a, _ = run(hip, pr)
+        b, _ = run(oracle, pr)
+        assert np.array_equal(a, b), pr.name
+
+
+def test_face_resid_width_autotuned(hip):
+    """At the first step the face-residual widths are timed on the context's
+    state and the fastest kept; the choice does not change the results."""
+    pr = P.kelvin_helmholtz(n=96, steps=3)
+    _announce("face-resid width autotune kh96")
+    with hip.context(pr.desc) as c:
+        c.set_state(pr.q0)
+        c.step(3)
+        q = c.get_state()
+        info = c.describe()
+    print("  ", info, flush=True)
+    us = info["resid_w_us"]
+    assert info["resid_w_choice"] == "autotuned" and len(us) == 2 and min(us) > 0
+    assert info["resid_w"] == [32, 16][int(np.argmin(us))]
+    for w in ("16", "32"):
+        os.environ["OSB_RESID_W"] = w
+        try:
+            forced, _ = run(hip, pr)
+        finally:
+            del os.environ["OSB_RESID_W"]
+        assert np.array_equal(forced, q), w
+
+
 def test_fused_stage_tile_autotuned(hip):
     """At the first step the tile shapes are timed on the context's own state and
     the fastest kept; the choice does not change the results."""
