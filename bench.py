@@ -463,6 +463,14 @@ def main():
     # roofline of the dominant kernel family: algorithmic bytes of one RK stage over
     # the grid (read q, read du, write du, write q) per launch of a per-stage kernel
     roofline = kernel_roofline(problem, args.workload, args.precision, prof, t_events, dt_wall, args.steps)
+    # the same kernel against the FP64 vector peak: FP64 flops per launch from
+    # the rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 summary of this workload
+    flops = profile_value("central_fused_tgv128_pmc.md", "fp64_flops_per_launch") \
+        if (args.workload, args.precision, roofline.get("kernel")) == ("tgv_central_128", "fp64", "central_fused") else None
+    if flops:
+        tf = flops / (roofline["kernel_avg_us"] * 1e-6) / 1e12
+        roofline["valu_fp64"] = {"flops_per_launch": flops, "achieved_TFLOPs": tf, "peak_TFLOPs": FP64_VALU_PEAK_TFLOPS,
+                                 "frac": tf / FP64_VALU_PEAK_TFLOPS}
     out = {
         "metric": METRIC,
         "value": value,
