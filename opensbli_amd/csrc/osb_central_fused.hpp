@@ -99,12 +99,12 @@ __host__ __device__ constexpr int cf_waves(int tx, int ty, int tz, int nfield) {
     return waves > 4 ? 4 : (waves < 1 ? 1 : waves);
 }
 
-// Scheduling fence between the sections of phase 2 (the velocity gradients,
-// each direction's convective terms, each momentum component's viscous
-// terms).  Without it the scheduler hoists the LDS reads of later sections
-// ahead to hide their latency, which takes the kernel to 248 VGPRs and 2
-// waves per SIMD; with it each section's values are loaded, used and dead
-// before the next, about 110 VGPRs, and occupancy hides the latency instead.
+// Scheduling fence between the sections of phase 2 (one per direction, one
+// per momentum component's mixed derivatives).  Without it the scheduler
+// hoists the LDS reads of later sections ahead to hide their latency, which
+// takes the kernel to 248 VGPRs and 2 waves per SIMD; with it each section's
+// values are loaded, used and dead before the next (about 130 VGPRs with the
+// RK operands held from the start), and occupancy hides the latency instead.
 __device__ __forceinline__ void cf_section() { __builtin_amdgcn_sched_barrier(0); }
 
 // index of the (a, b) pair in Consts::wm, a < b
@@ -194,8 +194,8 @@ __device__ __forceinline__ void cf_tile_stage(const int i0, const int j0, const 
     __syncthreads();
 
     // ---- phase 2: right-hand side and RK update at one point --------------
-    // Every stencil value comes from LDS: rho u_m = rho u_m, rhoE = rho E and
-    // p = rho theta are rebuilt at the stencil point.  The reads share one base
+    // Every stencil value comes from LDS: the momenta rho * u_m, rhoE = rho E
+    // and p = rho theta are rebuilt at the stencil point.  The reads share one base
     // address (the box point at offset -2, -2, -2 from the thread's point), so
     // each is that base plus a constant.
     if (!active) return;
