@@ -28,6 +28,7 @@
 #pragma once
 
 #include "osb_kernels.hpp"
+#include "osb_order.hpp"
 
 namespace osb {
 
@@ -408,28 +409,10 @@ __device__ __forceinline__ void cf_tile_stage(const int i0, const int j0, const 
 template <int CF_TX, int CF_TY, int CF_TZ, int CONV, int VISC, bool FLUXRES, int TIME>
 __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ, cf_waves(CF_TX, CF_TY, CF_TZ, cf_nfield(CONV, VISC)))
 k_central_fused(const double* __restrict__ q, const double* __restrict__ F, double* __restrict__ reg, double* __restrict__ qout,
-                const double* __restrict__ scal, Geom g, Consts c, double A, double B, int xcd_remap, int wrap) {
-    // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs, so
-    // workgroup w runs on XCD w % 8.  Giving XCD x the contiguous range of
-    // tiles [x*T/8, (x+1)*T/8) keeps neighbouring tiles -- which share halo
-    // points -- in the same L2.
-    const int ntx = gridDim.x, nty = gridDim.y;
-    const int ntiles = ntx * nty * gridDim.z;
-    const int w = blockIdx.x + ntx * (blockIdx.y + nty * blockIdx.z);
-    int tile = w;
-    if (xcd_remap && (ntiles & 7) == 0) tile = (w & 7) * (ntiles >> 3) + (w >> 3);
-    int tbx = tile % ntx, tby = (tile / ntx) % nty, tbz = tile / (ntx * nty);
-    if (xcd_remap == 2 && (ntx & 7) == 0 && (nty & 3) == 0 && (gridDim.z & 1) == 0) {
-        // supertiles of 8 x 4 x 2 tiles (about one XCD's resident workgroups)
-        // walked z first: each XCD's range is then a slab of whole supertile
-        // columns, and consecutive supertiles share their z-halo planes while
-        // they are in L2 (a squarer resident region than single tiles)
-        const int in = tile & 63, sup = tile >> 6;
-        const int sz = gridDim.z >> 1, sy = nty >> 2;
-        tbz = ((sup % sz) << 1) + (in >> 5);
-        tby = (((sup / sz) % sy) << 2) + ((in >> 3) & 3);
-        tbx = ((sup / (sz * sy)) << 3) + (in & 7);
-    }
+                const double* __restrict__ scal, Geom g, Consts c, double A, double B, CfOrder o, int wrap) {
+    // one tile per workgroup, in the XCD-aware order o (a 1-D grid)
+    int tbx, tby, tbz;
+    cf_tile_of(o, blockIdx.x, tbx, tby, tbz);
     cf_tile_stage<CF_TX, CF_TY, CF_TZ, CONV, VISC, FLUXRES, TIME>(tbx * CF_TX, tby * CF_TY, tbz * CF_TZ, q, F, reg, qout, scal, g,
                                                                   c, A, B, wrap);
 }

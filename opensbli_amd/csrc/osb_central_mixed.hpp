@@ -32,7 +32,7 @@ template <int CF_TX, int CF_TY, int CF_TZ, int CONV, int VISC, bool FLUXRES, int
 __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const double* __restrict__ q, const double* __restrict__ F,
                                                               double* __restrict__ reg, double* __restrict__ qout,
                                                               const double* __restrict__ scal, Geom g, Consts c,
-                                                              double A, double B, int xcd_remap, int wrap) {
+                                                              double A, double B, CfOrder o, int wrap) {
     static_assert(CONV == CF_FEIEREISEN || CONV == CF_KGP || CONV == CF_NONE, "mixed central kernel: Feiereisen, KGP or viscous-only");
     constexpr bool KGP = CONV == CF_KGP;
     constexpr int NV = 5;
@@ -48,20 +48,9 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
     __shared__ float s_E[HAS_CONV ? CF_BOX : 1];
     __shared__ float s_r[HAS_CONV ? 4 : 1][HAS_CONV ? CF_BOX : 1];    // rho u0..2, rhoE (KGP: rho)
 
-    const int ntx = gridDim.x, nty = gridDim.y;
-    const int ntiles = ntx * nty * gridDim.z;
-    const int w = blockIdx.x + ntx * (blockIdx.y + nty * blockIdx.z);
-    int tile = w;
-    if (xcd_remap && (ntiles & 7) == 0) tile = (w & 7) * (ntiles >> 3) + (w >> 3);
-    int tbx = tile % ntx, tby = (tile / ntx) % nty, tbz = tile / (ntx * nty);
-    if (xcd_remap == 2 && (ntx & 7) == 0 && (nty & 3) == 0 && (gridDim.z & 1) == 0) {
-        // supertiles of 8 x 4 x 2 tiles walked z first within each XCD's range (as k_central_fused)
-        const int in = tile & 63, sup = tile >> 6;
-        const int sz = gridDim.z >> 1, sy = nty >> 2;
-        tbz = ((sup % sz) << 1) + (in >> 5);
-        tby = (((sup / sz) % sy) << 2) + ((in >> 3) & 3);
-        tbx = ((sup / (sz * sy)) << 3) + (in & 7);
-    }
+    // one tile per workgroup, in the XCD-aware order o (as k_central_fused)
+    int tbx, tby, tbz;
+    cf_tile_of(o, blockIdx.x, tbx, tby, tbz);
     const int i0 = tbx * CF_TX, j0 = tby * CF_TY, k0 = tbz * CF_TZ;
     const long P = g.Ptot;
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];

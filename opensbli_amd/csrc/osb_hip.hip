@@ -277,9 +277,10 @@ void launch_wavespeed(osb_ctx* c, const double* q) {
 template <int TX, int TY, int TZ, int CONV, int VISC, bool FLUXRES, int TIME>
 void launch_fused_tile(osb_ctx* c, const double* q, double* qn, double A, double B) {
     const Geom& g = c->g;
-    dim3 grid((g.n[0] + TX - 1) / TX, (g.n[1] + TY - 1) / TY, (g.n[2] + TZ - 1) / TZ);
-    k_central_fused<TX, TY, TZ, CONV, VISC, FLUXRES, TIME><<<grid, TX * TY * TZ, 0, c->stream>>>(
-        q, c->F, c->du, qn, c->scal, g, c->c, A, B, c->xcd_remap, (int)(c->bc_periodic && c->bc_in_stage));
+    const unsigned ntx = (g.n[0] + TX - 1) / TX, nty = (g.n[1] + TY - 1) / TY, ntz = (g.n[2] + TZ - 1) / TZ;
+    k_central_fused<TX, TY, TZ, CONV, VISC, FLUXRES, TIME><<<ntx * nty * ntz, TX * TY * TZ, 0, c->stream>>>(
+        q, c->F, c->du, qn, c->scal, g, c->c, A, B, cf_order(ntx, nty, ntz, c->xcd_remap),
+        (int)(c->bc_periodic && c->bc_in_stage));
 }
 
 template <int CONV, int VISC, bool FLUXRES, int TIME>
@@ -303,9 +304,10 @@ void launch_fused_v(osb_ctx* c, const double* q, double* qn, double A, double B)
 template <int CONV, int VISC, bool FLUXRES, int TIME>
 void launch_mixed_t(osb_ctx* c, const double* q, double* qn, double A, double B) {
     const Geom& g = c->g;
-    dim3 grid((g.n[0] + 15) / 16, (g.n[1] + 3) / 4, (g.n[2] + 3) / 4);
-    k_central_mixed<16, 4, 4, CONV, VISC, FLUXRES, TIME><<<grid, 256, 0, c->stream>>>(
-        q, c->F, c->du, qn, c->scal, g, c->c, A, B, c->xcd_remap, (int)(c->bc_periodic && c->bc_in_stage));
+    const unsigned ntx = (g.n[0] + 15) / 16, nty = (g.n[1] + 3) / 4, ntz = (g.n[2] + 3) / 4;
+    k_central_mixed<16, 4, 4, CONV, VISC, FLUXRES, TIME><<<ntx * nty * ntz, 256, 0, c->stream>>>(
+        q, c->F, c->du, qn, c->scal, g, c->c, A, B, cf_order(ntx, nty, ntz, c->xcd_remap),
+        (int)(c->bc_periodic && c->bc_in_stage));
 }
 
 template <int CONV, bool FLUXRES, int TIME>

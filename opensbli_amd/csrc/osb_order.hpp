@@ -1,0 +1,77 @@
+// osb_order.hpp -- tile order of the 3-D stage kernels: the tile that
+// workgroup w of a one-dimensional launch evaluates.  Integer code only, shared
+// by the kernels and by the host code that launches them.
+//
+// Workgroups are dealt round-robin to the 8 XCDs, so workgroup w runs on XCD
+// w % 8.
+//   mode 0: tile w, x fastest.
+//   mode 1 (tile count a multiple of 8): XCD x takes the contiguous range of
+//     tiles [x*T/8, (x+1)*T/8), so neighbouring tiles -- which share halo
+//     points -- stay in the same L2.
+//   mode 2 (tiles along x, y, z multiples of 8, 4, 2): the same ranges over
+//     supertiles of 8 x 4 x 2 tiles (about one XCD's resident workgroups)
+//     walked z first.  Each XCD's range is then a slab of whole supertile
+//     columns, and consecutive supertiles share their z-halo planes while they
+//     are in L2.
+// The index n (a tile in modes 0 and 1, a supertile in mode 2) splits into
+// digits of radix c0, then c1.  Its two quotients n / c0 and n / (c0 c1) come
+// from the host's multiplicative inverses, so a workgroup's prologue is a few
+// scalar multiplies instead of six integer-division sequences.
+#pragma once
+
+namespace osb {
+
+struct CfOrder {
+    int mode;
+    unsigned tq;                  // tiles per XCD range (modes 1 and 2)
+    unsigned c0, c1;              // modes 0/1: tiles along x, y; mode 2: supertiles along z, y
+    unsigned mp0, l0, mp1, l1;    // inverses of c0 and of c0 * c1 (cf_divisor)
+};
+
+// n / d = (n + umulhi(n, mp)) >> l for every 32-bit n, with l = ceil(log2 d)
+// and mp = ceil(2^(32+l) / d) - 2^32 (Granlund and Montgomery, "Division by
+// invariant integers using multiplication", PLDI 1994, sec. 4).  1 <= d <= 2^31.
+__host__ inline void cf_divisor(unsigned d, unsigned& mp, unsigned& l) {
+    l = 0;
+    while ((1ull << l) < d) ++l;
+    const unsigned long long m = ((1ull << (32 + l)) + d - 1) / d;
+    mp = (unsigned)(m - (1ull << 32));
+}
+__host__ __device__ __forceinline__ unsigned cf_div(unsigned n, unsigned mp, unsigned l) {
+    const unsigned t = (unsigned)(((unsigned long long)n * mp) >> 32);
+    return (unsigned)(((unsigned long long)n + t) >> l);
+}
+
+// ntx, nty, ntz: tiles along x, y, z; xcd_remap: 0 mode 0, 1 mode 1 where it
+// applies, 2 mode 2 where it applies (else mode 1, else mode 0)
+__host__ inline CfOrder cf_order(unsigned ntx, unsigned nty, unsigned ntz, int xcd_remap) {
+    CfOrder o{};
+    const unsigned ntiles = ntx * nty * ntz;
+    o.mode = xcd_remap == 2 && ntx % 8 == 0 && nty % 4 == 0 && ntz % 2 == 0 ? 2 : (xcd_remap && ntiles % 8 == 0 ? 1 : 0);
+    o.tq = ntiles / 8;
+    o.c0 = o.mode == 2 ? ntz / 2 : ntx;
+    o.c1 = o.mode == 2 ? nty / 4 : nty;
+    cf_divisor(o.c0, o.mp0, o.l0);
+    cf_divisor(o.c0 * o.c1, o.mp1, o.l1);
+    return o;
+}
+
+// tile coordinates (in tiles) of workgroup w
+__host__ __device__ __forceinline__ void cf_tile_of(const CfOrder& o, unsigned w, int& tbx, int& tby, int& tbz) {
+    const unsigned tile = o.mode ? (w & 7) * o.tq + (w >> 3) : w;
+    const unsigned n = o.mode == 2 ? tile >> 6 : tile;
+    const unsigned a = cf_div(n, o.mp0, o.l0), b = cf_div(n, o.mp1, o.l1);
+    const int r0 = (int)(n - a * o.c0), r1 = (int)(a - b * o.c1), r2 = (int)b;
+    if (o.mode == 2) {
+        const int in = (int)(tile & 63);
+        tbz = (r0 << 1) + (in >> 5);
+        tby = (r1 << 2) + ((in >> 3) & 3);
+        tbx = (r2 << 3) + (in & 7);
+    } else {
+        tbx = r0;
+        tby = r1;
+        tbz = r2;
+    }
+}
+
+}  // namespace osb

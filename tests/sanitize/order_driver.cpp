@@ -1,0 +1,139 @@
+// Checks the tile order of the 3-D stage kernels (opensbli_amd/csrc/osb_order.hpp)
+// on the host, built with AddressSanitizer and UndefinedBehaviorSanitizer by
+// tests/test_sanitize.py:
+//  - the multiplicative-inverse quotient cf_div against integer division, for
+//    every divisor below 70000 on sampled and extreme numerators, and for
+//    divisors up to 2^31 on random ones;
+//  - for every tile grid up to 24 x 24 x 24 and the shapes of the 128^3-1024^3
+//    grids, in each mode: every workgroup gets a tile in range, each tile
+//    exactly once, the same tile as the plain division form of the order, and
+//    the mode's locality property (mode 1: XCD x gets the linear tile range
+//    [x T/8, (x+1) T/8); mode 2: XCD x gets the same range of the
+//    supertile-major order, so with a supertile count divisible by 8 each
+//    supertile lies on one XCD and each XCD's supertiles are contiguous in
+//    z-first order).
+// Exits non-zero on the first failed check.
+#define __host__
+#define __device__
+#define __forceinline__ inline
+#include "osb_order.hpp"
+
+#include <cstdio>
+#include <random>
+#include <vector>
+
+using osb::CfOrder;
+
+namespace {
+
+long g_bad = 0;
+
+void fail(const char* what, unsigned a, unsigned b, unsigned c, int m) {
+    if (g_bad++ < 8) std::printf("FAIL %s (%u, %u, %u) mode arg %d\n", what, a, b, c, m);
+}
+
+void check_div() {
+    std::mt19937_64 rng(20261204);
+    for (unsigned d = 1; d < 70000; ++d) {
+        unsigned mp, l;
+        osb::cf_divisor(d, mp, l);
+        const unsigned edge[] = {0u, 1u, d - 1, d, d + 1, 2 * d - 1, 0x7fffffffu, 0x80000000u, 0xfffffffeu, 0xffffffffu};
+        for (unsigned n : edge)
+            if (osb::cf_div(n, mp, l) != n / d) fail("cf_div edge", d, n, 0, 0);
+        for (int k = 0; k < 64; ++k) {
+            const unsigned n = k < 32 ? (unsigned)(k * 104729u) % (1u << 24) : (unsigned)rng();
+            if (osb::cf_div(n, mp, l) != n / d) fail("cf_div", d, n, 0, 0);
+        }
+    }
+    const unsigned big[] = {65537u, 999983u, 1u << 20, (1u << 20) + 1, 0x7fffffffu, 0x80000000u};
+    for (unsigned d : big) {
+        unsigned mp, l;
+        osb::cf_divisor(d, mp, l);
+        for (int k = 0; k < 200000; ++k) {
+            const unsigned n = (unsigned)rng();
+            if (osb::cf_div(n, mp, l) != n / d) fail("cf_div big", d, n, 0, 0);
+        }
+    }
+    std::printf("ok cf_div\n");
+}
+
+// the order written with plain integer division
+void plain_tile(unsigned ntx, unsigned nty, unsigned ntz, int remap, unsigned w, int& x, int& y, int& z) {
+    const unsigned T = ntx * nty * ntz;
+    unsigned t = w;
+    if (remap && T % 8 == 0) t = (w % 8) * (T / 8) + w / 8;
+    x = t % ntx;
+    y = (t / ntx) % nty;
+    z = t / (ntx * nty);
+    if (remap == 2 && ntx % 8 == 0 && nty % 4 == 0 && ntz % 2 == 0) {
+        const unsigned in = t % 64, sup = t / 64, sz = ntz / 2, sy = nty / 4;
+        z = 2 * (sup % sz) + in / 32;
+        y = 4 * ((sup / sz) % sy) + (in / 8) % 4;
+        x = 8 * (sup / (sz * sy)) + in % 8;
+    }
+}
+
+void check_walk(unsigned ntx, unsigned nty, unsigned ntz, int remap) {
+    const CfOrder o = osb::cf_order(ntx, nty, ntz, remap);
+    const unsigned T = ntx * nty * ntz;
+    const bool m2 = remap == 2 && ntx % 8 == 0 && nty % 4 == 0 && ntz % 2 == 0;
+    const int want = m2 ? 2 : (remap && T % 8 == 0 ? 1 : 0);
+    if (o.mode != want) return fail("mode", ntx, nty, ntz, remap);
+    std::vector<unsigned char> seen(T, 0);
+    std::vector<int> sup_xcd(T / 64 + 1, -1);
+    std::vector<unsigned> sup_lo(8, ~0u), sup_hi(8, 0), sup_n(8, 0);
+    for (unsigned w = 0; w < T; ++w) {
+        int x, y, z, px, py, pz;
+        osb::cf_tile_of(o, w, x, y, z);
+        if (x < 0 || y < 0 || z < 0 || (unsigned)x >= ntx || (unsigned)y >= nty || (unsigned)z >= ntz)
+            return fail("tile out of range", ntx, nty, ntz, remap);
+        plain_tile(ntx, nty, ntz, remap, w, px, py, pz);
+        if (x != px || y != py || z != pz) return fail("differs from the division form", ntx, nty, ntz, remap);
+        const unsigned id = x + ntx * (y + nty * (unsigned)z);
+        if (seen[id]++) return fail("tile visited twice", ntx, nty, ntz, remap);
+        const unsigned xcd = w % 8;
+        if (o.mode == 1 && (id < xcd * (T / 8) || id >= (xcd + 1) * (T / 8))) return fail("mode 1 range", ntx, nty, ntz, remap);
+        if (o.mode == 2) {
+            // position in the supertile-major order: z-first supertile index, then
+            // the tile within its supertile (x fastest, then y, then z)
+            const unsigned s = z / 2 + (ntz / 2) * (y / 4 + (nty / 4) * (x / 8));
+            const unsigned pos = 64 * s + x % 8 + 8 * (y % 4) + 32 * (z % 2);
+            if (pos < xcd * (T / 8) || pos >= (xcd + 1) * (T / 8)) return fail("mode 2 range", ntx, nty, ntz, remap);
+            if ((T / 64) % 8 == 0) {
+                if (sup_xcd[s] < 0) {
+                    sup_xcd[s] = (int)xcd;
+                    sup_lo[xcd] = s < sup_lo[xcd] ? s : sup_lo[xcd];
+                    sup_hi[xcd] = s > sup_hi[xcd] ? s : sup_hi[xcd];
+                    ++sup_n[xcd];
+                } else if (sup_xcd[s] != (int)xcd) {
+                    return fail("supertile split over XCDs", ntx, nty, ntz, remap);
+                }
+            }
+        }
+    }
+    if (o.mode == 2 && (T / 64) % 8 == 0)
+        for (int x = 0; x < 8; ++x)
+            if (sup_n[x] != sup_hi[x] - sup_lo[x] + 1) return fail("XCD supertiles not contiguous", ntx, nty, ntz, remap);
+}
+
+}  // namespace
+
+int main() {
+    check_div();
+    long walks = 0;
+    for (unsigned x = 1; x <= 24; ++x)
+        for (unsigned y = 1; y <= 24; ++y)
+            for (unsigned z = 1; z <= 24; ++z)
+                for (int r = 0; r < 3; ++r, ++walks) check_walk(x, y, z, r);
+    // 128^3 .. 1024^3 at the 16x4x4, 16x8x4 and 8x8x8 tiles, and ragged grids
+    const unsigned shapes[][3] = {{8, 32, 32}, {8, 16, 32}, {16, 16, 16}, {16, 64, 64}, {16, 32, 64}, {32, 32, 32},
+                                  {64, 256, 256}, {64, 128, 256}, {128, 128, 128}, {7, 33, 65}, {9, 12, 30}};
+    for (const auto& s : shapes)
+        for (int r = 0; r < 3; ++r, ++walks) check_walk(s[0], s[1], s[2], r);
+    if (g_bad) {
+        std::printf("%ld failures\n", g_bad);
+        return 1;
+    }
+    std::printf("ok tile walks %ld\nall ok\n", walks);
+    return 0;
+}

@@ -1,6 +1,7 @@
 """Host code under AddressSanitizer and UndefinedBehaviorSanitizer: the CPU
 oracle and the HDF5 I/O library, each compiled with -fsanitize=address,undefined
-into a driver that runs it through its C ABI (tests/sanitize/).  A sanitizer
+into a driver that runs it through its C ABI, and the stage kernels' tile-order
+arithmetic (osb_order.hpp) in a driver of its own (tests/sanitize/).  A sanitizer
 report, a failed run or a non-finite result fails the test."""
 import os
 import subprocess
@@ -39,3 +40,11 @@ def test_io_library_under_sanitizers(tmp_path):
         pytest.skip("no HDF5 C library")
     _build(tmp_path, "io_driver")
     _run([str(tmp_path / "io_driver"), str(tmp_path)], tmp_path)
+
+
+def test_tile_order_under_sanitizers(tmp_path):
+    """Fast quotients against integer division; every tile once, in range, the
+    same as the plain division form, with each mode's XCD locality."""
+    _build(tmp_path, "order_driver")
+    out = _run([str(tmp_path / "order_driver")], tmp_path)
+    assert "ok cf_div" in out and "ok tile walks" in out
