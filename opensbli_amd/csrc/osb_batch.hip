@@ -223,6 +223,7 @@ int upload_items(osb_batch* b) {
             x.scal = c->scal;
             x.g = c->g;
             x.c = c->c;
+            kb_central_orders(c->g, x.ord);
             for (int d = 0; d < 3; ++d) x.bc[d] = bc_sides(c->d, d);
             x.cfl = c->d.cfl;
             if (c->d.time_scheme == OSB_TIME_RKLS3) {
@@ -885,6 +886,7 @@ int launch_group_chunk(osb_pool* p, PoolGroup& pg, const std::vector<int>& membe
         x.q[0] = c->q_cur;
         x.q[1] = c->q_cur == c->qa ? c->qb : c->qa;
         x.du = c->du; x.F = c->F; x.scal = c->scal; x.g = c->g; x.c = c->c;
+        kb_central_orders(c->g, x.ord);
         for (int d = 0; d < 3; ++d) x.bc[d] = bc_sides(c->d, d);
         x.cfl = c->d.cfl;
         g.any_cfl = g.any_cfl || c->d.cfl > 0;

@@ -22,6 +22,7 @@ struct BItem {
     BcSides bc[3];         // boundary types and Dirichlet values per direction
     double cfl;            // > 0: dt from the wave-speed reduction
     double rkA[5], rkB[5]; // stage coefficients (classic RK3: rknew, rkold)
+    CfOrder ord[CF_NTILES];    // 3-D central groups: the walk of the instance's tiles at each tile shape
 };
 
 // Items are read through the constant address space: a pointer loaded from
@@ -177,9 +178,11 @@ __global__ void __launch_bounds__(BLOCK) kb_glf(const BItem* __restrict__ items,
 }
 
 // 3-D central RK-LS stage of every item, all faces periodic: one virtual
-// block is one TX x TY x TZ tile of an item (x, then y, then z; a tile shape
-// of CF_TILES, timed by the batch's autotuner), evaluated by the single-block
-// kernel's tile code, which also writes the wrapped halo images.
+// block is one TX x TY x TZ tile of an item (a tile shape of CF_TILES, timed
+// by the batch's autotuner), evaluated by the single-block kernel's tile code,
+// which also writes the wrapped halo images.  An item's tiles are walked in
+// its order it.ord (8 x 4 x 2 supertiles z first where the tile counts allow,
+// else x, then y, then z), with host-computed quotients.
 // Workgroup v of a launch runs on XCD v % 8.  This permutation of [0, total)
 // gives XCD x the contiguous range of virtual blocks [x*(q+1), (x+1)*(q+1)) for
 // x < r and the ranges of q after them (total = 8q + r), so neighbouring tiles
@@ -195,15 +198,23 @@ kb_central_fused(const BItem* __restrict__ items, const int* __restrict__ offs, 
     const int vb = xcd_contiguous(blockIdx.x, offs[n]);
     const int i = find_item(offs, n, vb);
     CBItem& it = item_const(items, i);
-    const int tile = vb - offs[i];
-    const int ntx = (it.g.n[0] + TX - 1) / TX, nty = (it.g.n[1] + TY - 1) / TY;
-    const int tbx = tile % ntx, tby = (tile / ntx) % nty, tbz = tile / (ntx * nty);
+    constexpr int TI = cf_tile_index(TX, TY, TZ);
+    static_assert(TI >= 0, "a tile shape of CF_TILES");
+    int tbx, tby, tbz;
+    cf_tile_at(it.ord[TI], (unsigned)(vb - offs[i]), tbx, tby, tbz);
     cf_tile_stage<TX, TY, TZ, CONV, VISC, false, OSB_TIME_RKLS3>(tbx * TX, tby * TY, tbz * TZ, it.q[parity], nullptr, it.du,
                                                                  it.q[parity ^ 1], it.scal, it.g, it.c, it.rkA[stage],
                                                                  it.rkB[stage], 1);
 }
 __host__ inline long kb_central_tiles(const Geom& g, const CfTile& t) {
     return (long)((g.n[0] + t.tx - 1) / t.tx) * ((g.n[1] + t.ty - 1) / t.ty) * ((g.n[2] + t.tz - 1) / t.tz);
+}
+// an item's tile walk at each tile shape (BItem::ord)
+__host__ inline void kb_central_orders(const Geom& g, CfOrder (&ord)[CF_NTILES]) {
+    for (int t = 0; t < CF_NTILES; ++t) {
+        const CfTile& s = CF_TILES[t];
+        ord[t] = cf_order((g.n[0] + s.tx - 1) / s.tx, (g.n[1] + s.ty - 1) / s.ty, (g.n[2] + s.tz - 1) / s.tz, 2);
+    }
 }
 
 static __global__ void kb_dt_finalize(const BItem* __restrict__ items, int n) {

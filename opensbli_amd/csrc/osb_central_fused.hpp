@@ -36,6 +36,12 @@ namespace osb {
 struct CfTile { int tx, ty, tz; };
 constexpr CfTile CF_TILES[] = {{16, 4, 4}, {16, 8, 4}, {8, 8, 8}};
 constexpr int CF_NTILES = 3;
+// index of a tile shape in CF_TILES, -1 if it is not one
+__host__ __device__ constexpr int cf_tile_index(int tx, int ty, int tz) {
+    for (int t = 0; t < CF_NTILES; ++t)
+        if (CF_TILES[t].tx == tx && CF_TILES[t].ty == ty && CF_TILES[t].tz == tz) return t;
+    return -1;
+}
 
 enum CfConv { CF_NONE = 0, CF_FEIEREISEN = 1, CF_SKEW = 2, CF_KGP = 3 };
 

@@ -56,14 +56,16 @@ __host__ inline CfOrder cf_order(unsigned ntx, unsigned nty, unsigned ntz, int x
     return o;
 }
 
-// tile coordinates (in tiles) of workgroup w
-__host__ __device__ __forceinline__ void cf_tile_of(const CfOrder& o, unsigned w, int& tbx, int& tby, int& tbz) {
-    const unsigned tile = o.mode ? (w & 7) * o.tq + (w >> 3) : w;
-    const unsigned n = o.mode == 2 ? tile >> 6 : tile;
+// tile coordinates (in tiles) of position p of the order before the XCD split:
+// linear in modes 0 and 1, supertile-major in mode 2 (the batched stage kernel
+// applies its own XCD split over all instances and walks each instance's tiles
+// in this order)
+__host__ __device__ __forceinline__ void cf_tile_at(const CfOrder& o, unsigned p, int& tbx, int& tby, int& tbz) {
+    const unsigned n = o.mode == 2 ? p >> 6 : p;
     const unsigned a = cf_div(n, o.mp0, o.l0), b = cf_div(n, o.mp1, o.l1);
     const int r0 = (int)(n - a * o.c0), r1 = (int)(a - b * o.c1), r2 = (int)b;
     if (o.mode == 2) {
-        const int in = (int)(tile & 63);
+        const int in = (int)(p & 63);
         tbz = (r0 << 1) + (in >> 5);
         tby = (r1 << 2) + ((in >> 3) & 3);
         tbx = (r2 << 3) + (in & 7);
@@ -72,6 +74,11 @@ __host__ __device__ __forceinline__ void cf_tile_of(const CfOrder& o, unsigned w
         tby = r1;
         tbz = r2;
     }
+}
+
+// tile coordinates of workgroup w of a single-instance launch
+__host__ __device__ __forceinline__ void cf_tile_of(const CfOrder& o, unsigned w, int& tbx, int& tby, int& tbz) {
+    cf_tile_at(o, o.mode ? (w & 7) * o.tq + (w >> 3) : w, tbx, tby, tbz);
 }
 
 }  // namespace osb

@@ -11,7 +11,9 @@
 //    [x T/8, (x+1) T/8); mode 2: XCD x gets the same range of the
 //    supertile-major order, so with a supertile count divisible by 8 each
 //    supertile lies on one XCD and each XCD's supertiles are contiguous in
-//    z-first order).
+//    z-first order), and the order before the XCD split (the batched
+//    kernel's walk of one instance) visits every tile once, supertile by
+//    supertile in mode 2.
 // Exits non-zero on the first failed check.
 #define __host__
 #define __device__
@@ -114,6 +116,22 @@ void check_walk(unsigned ntx, unsigned nty, unsigned ntz, int remap) {
     if (o.mode == 2 && (T / 64) % 8 == 0)
         for (int x = 0; x < 8; ++x)
             if (sup_n[x] != sup_hi[x] - sup_lo[x] + 1) return fail("XCD supertiles not contiguous", ntx, nty, ntz, remap);
+    // the order before the XCD split (the batched kernel's walk of one instance):
+    // each tile once; in mode 2 the 64 tiles of a supertile consecutive
+    std::vector<unsigned char> seen2(T, 0);
+    for (unsigned p = 0; p < T; ++p) {
+        int x, y, z;
+        osb::cf_tile_at(o, p, x, y, z);
+        if (x < 0 || y < 0 || z < 0 || (unsigned)x >= ntx || (unsigned)y >= nty || (unsigned)z >= ntz)
+            return fail("cf_tile_at out of range", ntx, nty, ntz, remap);
+        if (seen2[x + ntx * (y + nty * (unsigned)z)]++) return fail("cf_tile_at visits a tile twice", ntx, nty, ntz, remap);
+        if (o.mode == 2) {
+            const unsigned s = z / 2 + (ntz / 2) * (y / 4 + (nty / 4) * (x / 8));
+            if (s != p / 64) return fail("cf_tile_at supertile", ntx, nty, ntz, remap);
+        } else if (x + ntx * (y + nty * (unsigned)z) != p) {
+            return fail("cf_tile_at not linear", ntx, nty, ntz, remap);
+        }
+    }
 }
 
 }  // namespace
