@@ -114,7 +114,7 @@ static __global__ void __launch_bounds__(BLOCK) kb_save_state(const BItem* __res
         const int lo[3] = {0, 0, 0};
         if (t >= (long)e[0] * e[1] * e[2]) continue;
         int a, b, c;
-        unflatten(t, e, lo, a, b, c);
+        unflatten_dv(t, e, g.dv[0], lo, a, b, c);
         const long x = gidx(g, a, b, c);
         for (int v = 0; v < g.ndim + 2; ++v) it.du[(long)v * g.Ptot + x] = it.q[parity][(long)v * g.Ptot + x];
     }
@@ -139,7 +139,7 @@ __global__ void __launch_bounds__(BLOCK) kb_wavespeed(const BItem* __restrict__ 
         double m = 0.0;
         if (it.cfl > 0 && t < (long)e[0] * e[1] * e[2]) {
             int a, b, c;
-            unflatten(t, e, lo, a, b, c);
+            unflatten_dv(t, e, g.dv[0], lo, a, b, c);
             m = wavespeed_point<NDIM>(it.q[parity], gidx(g, a, b, c), g, it.c);
         }
         if (it.cfl > 0) block_max_to(m, it.scal);     // uniform over the block
@@ -170,7 +170,7 @@ __global__ void __launch_bounds__(BLOCK) kb_glf(const BItem* __restrict__ items,
         const bool on = it.c.glf != nullptr;          // uniform over the block: one item per virtual block
         if (on && t < (long)e[0] * e[1] * e[2]) {
             int a, b, c;
-            unflatten(t, e, lo, a, b, c);
+            unflatten_dv(t, e, g.dv[0], lo, a, b, c);
             glf_point<NDIM>(it.q[parity], gidx(g, a, b, c), g.Ptot, it.c, m);
         }
         if (on) glf_block_to<NDIM>(m, const_cast<double*>(it.c.glf));

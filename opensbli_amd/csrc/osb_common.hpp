@@ -8,6 +8,7 @@
 #include <type_traits>
 
 #include "../../include/osb.h"
+#include "osb_order.hpp"
 
 namespace osb {
 
@@ -25,6 +26,12 @@ struct Geom {
     int ndim;
     long s[3];         // strides (1, P0, P0*P1)
     long Ptot;         // points per variable
+    // multiplicative inverses (cf_divisor: mp, l of an extent e0, then of
+    // e0 e1) that split flat indices without integer division: dv[0] the
+    // interior points (n0, n0 n1), dv[1 + d] the faces along d (n_d + 1 along
+    // d), dv[4 + w] the fused face-residual tiles at width RESID_WS[w]
+    // (columns, columns x segments).  Set by geom_divisors.
+    unsigned dv[6][4];
 };
 
 __host__ __device__ inline long gidx(const Geom& g, int i, int j, int k) {

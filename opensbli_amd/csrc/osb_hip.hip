@@ -705,6 +705,7 @@ Geom make_geom(const osb_problem_desc* desc) {
     g.s[1] = g.P[0];
     g.s[2] = (long)g.P[0] * g.P[1];
     g.Ptot = (long)g.P[0] * g.P[1] * g.P[2];
+    geom_divisors(g);
     return g;
 }
 

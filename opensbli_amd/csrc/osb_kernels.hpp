@@ -27,6 +27,16 @@ __device__ __forceinline__ void unflatten(long t, const int (&e)[3], const int (
     j = (int)(r % e1) + lo[1];
     k = (int)(r / e1) + lo[2];
 }
+// the same split from the host's inverses dv of e0 and e0 e1 (Geom::dv), for
+// t below the extents' product (< 2^32 by the context's size check)
+__device__ __forceinline__ void unflatten_dv(long t, const int (&e)[3], const unsigned (&dv)[4], const int (&lo)[3], int& i,
+                                             int& j, int& k) {
+    const unsigned u = (unsigned)t;
+    const unsigned a = cf_div(u, dv[0], dv[1]), b = cf_div(u, dv[2], dv[3]);
+    i = (int)(u - a * (unsigned)e[0]) + lo[0];
+    j = (int)(a - b * (unsigned)e[1]) + lo[1];
+    k = (int)b + lo[2];
+}
 
 // ---------------------------------------------------------------------------
 // Interface fluxes along DIR for faces i+1/2, i in [-1, n) along DIR and the
@@ -42,7 +52,7 @@ __device__ __forceinline__ void face_flux_item(long t, const double* __restrict_
     lo[DIR] = -1;
     if (t >= (long)e[0] * e[1] * e[2]) return;
     int i, j, k;
-    unflatten(t, e, lo, i, j, k);
+    unflatten_dv(t, e, g.dv[1 + DIR], lo, i, j, k);
     const long x0 = gidx(g, i, j, k);
     double out[NV];
     face_flux<NDIM, DIR, SCH, T>(q, g.Ptot, g.s[DIR], x0, c, out);
@@ -73,7 +83,7 @@ __global__ void __launch_bounds__(BLOCK) k_face_hllc(const double* __restrict__ 
     const long t = gtid();
     if (t >= (long)e[0] * e[1] * e[2]) return;
     int i, j, k;
-    unflatten(t, e, lo, i, j, k);
+    unflatten_dv(t, e, g.dv[1 + DIR], lo, i, j, k);
     const long x0 = gidx(g, i, j, k);
     double out[NV];
     face_hllc<NDIM, DIR, LM>(q, g.Ptot, g.s[DIR], x0, c, out);
@@ -149,7 +159,7 @@ __global__ void __launch_bounds__(BLOCK) k_ducros(const double* __restrict__ u, 
     const long t = xcd_gtid();
     if (t >= (long)e[0] * e[1] * e[2]) return;
     int i, j, k;
-    unflatten(t, e, lo, i, j, k);
+    unflatten_dv(t, e, g.dv[0], lo, i, j, k);
     const int ii[3] = {i, j, k};
     const long x = gidx(g, i, j, k);
     const long P = g.Ptot;
@@ -188,7 +198,7 @@ __global__ void __launch_bounds__(BLOCK) k_filter_update(const double* __restric
     const long t = xcd_gtid();
     if (t >= (long)e[0] * e[1] * e[2]) return;
     int i, j, k;
-    unflatten(t, e, lo, i, j, k);
+    unflatten_dv(t, e, g.dv[0], lo, i, j, k);
     const long x = gidx(g, i, j, k);
     if (HYB) {
         double km = kappa[x];
@@ -236,7 +246,7 @@ __global__ void __launch_bounds__(BLOCK) k_flux_residual(const double* __restric
     const long t = gtid();
     if (t >= (long)e[0] * e[1] * e[2]) return;
     int i, j, k;
-    unflatten(t, e, lo, i, j, k);
+    unflatten_dv(t, e, g.dv[0], lo, i, j, k);
     const long x = gidx(g, i, j, k);
     const long nvP = (long)NV * g.Ptot;
 #pragma unroll
@@ -255,7 +265,7 @@ __device__ __forceinline__ void resid_rk_item(long t, const double* __restrict__
     const int lo[3] = {0, 0, 0};
     if (t >= (long)e[0] * e[1] * e[2]) return;
     int i, j, k;
-    unflatten(t, e, lo, i, j, k);
+    unflatten_dv(t, e, g.dv[0], lo, i, j, k);
     const long x = gidx(g, i, j, k);
     const long nvP = (long)NV * g.Ptot;
 #pragma unroll
@@ -318,6 +328,21 @@ __host__ __device__ inline long fused_tiles(const Geom& g) {
     return ncol * nseg * (NDIM == 3 ? g.n[1] : 1);
 }
 
+// Geom::dv: the inverses the kernels split flat indices with
+__host__ inline void geom_divisors(Geom& g) {
+    auto set = [](unsigned(&d)[4], long e0, long e1) {
+        cf_divisor((unsigned)e0, d[0], d[1]);
+        cf_divisor((unsigned)(e0 * e1), d[2], d[3]);
+    };
+    set(g.dv[0], g.n[0], g.n[1]);
+    for (int d = 0; d < 3; ++d) set(g.dv[1 + d], g.n[0] + (d == 0 ? 1 : 0), g.n[1] + (d == 1 ? 1 : 0));
+    const int D = g.ndim - 1;
+    for (int w = 0; w < RESID_NW; ++w) {
+        const int fw = g.ndim == 1 ? 1 : RESID_WS[w], pl = BLOCK / fw - 1;
+        set(g.dv[4 + w], (g.n[0] + fw - 1) / fw, (g.n[D] + pl - 1) / pl);
+    }
+}
+
 template <int NDIM, int SCH, typename T, int TIME, int W = FUSED_FW>
 __device__ __forceinline__ void face_resid_tile(long tile, const double* __restrict__ q, const double* __restrict__ F,
                                                 double* __restrict__ qout, double* __restrict__ du, double dt, const Geom& g,
@@ -331,11 +356,15 @@ __device__ __forceinline__ void face_resid_tile(long tile, const double* __restr
     if (NDIM == 1) {
         j0 = (int)tile * PL;
     } else {
-        const long ncol = (g.n[0] + FW - 1) / FW;
-        col0 = (int)(tile % ncol) * FW;
-        const long r = tile / ncol;
-        j0 = (int)(r % nseg) * PL;
-        other = (int)(r / nseg);          // k for NDIM 2 (always 0), j for NDIM 3
+        // column, segment and (3-D) j of the tile, from the host's inverses of
+        // the column count and of columns x segments at this width
+        static_assert(W == RESID_WS[0] || W == RESID_WS[1], "a width of RESID_WS");
+        const unsigned(&dv)[4] = g.dv[4 + (W == RESID_WS[0] ? 0 : 1)];
+        const unsigned ncol = (unsigned)((g.n[0] + FW - 1) / FW), u = (unsigned)tile;
+        const unsigned a = cf_div(u, dv[0], dv[1]), b = cf_div(u, dv[2], dv[3]);
+        col0 = (int)(u - a * ncol) * FW;
+        j0 = (int)(a - b * (unsigned)nseg) * PL;
+        other = (int)b;                   // k for NDIM 2 (always 0), j for NDIM 3
     }
     const int tid = (int)threadIdx.x;
     const int w = tid % FW, lf = tid / FW;
@@ -519,7 +548,7 @@ __global__ void __launch_bounds__(BLOCK) k_central_convective(const double* __re
     const long t = gtid();
     if (t >= (long)e[0] * e[1] * e[2]) return;
     int i, j, k;
-    unflatten(t, e, lo, i, j, k);
+    unflatten_dv(t, e, g.dv[0], lo, i, j, k);
     const int ii[3] = {i, j, k};
     const long x = gidx(g, i, j, k);
     const long P = g.Ptot;
@@ -631,7 +660,7 @@ __global__ void __launch_bounds__(BLOCK) k_viscous(const double* __restrict__ cr
     const long t = gtid();
     if (t >= (long)e[0] * e[1] * e[2]) return;
     int i, j, k;
-    unflatten(t, e, lo, i, j, k);
+    unflatten_dv(t, e, g.dv[0], lo, i, j, k);
     const int ii[3] = {i, j, k};
     const long x = gidx(g, i, j, k);
     const long P = g.Ptot;
@@ -704,7 +733,7 @@ static __global__ void __launch_bounds__(BLOCK) k_rkls(const double* __restrict_
     const long t = gtid();
     if (t >= (long)e[0] * e[1] * e[2]) return;
     int i, j, k;
-    unflatten(t, e, lo, i, j, k);
+    unflatten_dv(t, e, g.dv[0], lo, i, j, k);
     const long x = gidx(g, i, j, k);
     const double dt = scal[1];
     for (int v = 0; v < nv; ++v) {
@@ -724,7 +753,7 @@ static __global__ void __launch_bounds__(BLOCK) k_rk3_stage(double* __restrict__
     const long t = gtid();
     if (t >= (long)e[0] * e[1] * e[2]) return;
     int i, j, k;
-    unflatten(t, e, lo, i, j, k);
+    unflatten_dv(t, e, g.dv[0], lo, i, j, k);
     const long x = gidx(g, i, j, k);
     const double dt = scal[1];
     for (int v = 0; v < nv; ++v) {
@@ -741,7 +770,7 @@ static __global__ void __launch_bounds__(BLOCK) k_copy_interior(const double* __
     const long t = gtid();
     if (t >= (long)e[0] * e[1] * e[2]) return;
     int i, j, k;
-    unflatten(t, e, lo, i, j, k);
+    unflatten_dv(t, e, g.dv[0], lo, i, j, k);
     const long x = gidx(g, i, j, k);
     for (int v = 0; v < nv; ++v) dst[(long)v * g.Ptot + x] = src[(long)v * g.Ptot + x];
 }
@@ -968,7 +997,7 @@ __global__ void __launch_bounds__(BLOCK) k_wavespeed(const double* __restrict__ 
     double m = 0.0;
     for (long t = gtid(); t < tot; t += (long)gridDim.x * blockDim.x) {
         int i, j, k;
-        unflatten(t, e, lo, i, j, k);
+        unflatten_dv(t, e, g.dv[0], lo, i, j, k);
         m = fmax(m, wavespeed_point<NDIM>(q, gidx(g, i, j, k), g, c));
     }
     block_max_to(m, scal);
@@ -1025,7 +1054,7 @@ __global__ void __launch_bounds__(BLOCK) k_glf(const double* __restrict__ q, dou
     for (int r = 0; r < 3 * NDIM; ++r) m[r] = 0.0;
     for (long t = gtid(); t < tot; t += (long)gridDim.x * blockDim.x) {
         int i, j, k;
-        unflatten(t, e, lo, i, j, k);
+        unflatten_dv(t, e, g.dv[0], lo, i, j, k);
         glf_point<NDIM>(q, gidx(g, i, j, k), g.Ptot, c, m);
     }
     glf_block_to<NDIM>(m, glf);
@@ -1052,7 +1081,7 @@ static __global__ void __launch_bounds__(BLOCK) k_tgv_partials(const double* __r
     const long P = g.Ptot;
     for (long t = gtid(); t < tot; t += (long)gridDim.x * blockDim.x) {
         int i, j, k;
-        unflatten(t, e, lo, i, j, k);
+        unflatten_dv(t, e, g.dv[0], lo, i, j, k);
         const long x = gidx(g, i, j, k);
         double G[3][3];
         for (int m = 0; m < 3; ++m)
@@ -1096,7 +1125,7 @@ static __global__ void __launch_bounds__(BLOCK) k_nan_count(const double* __rest
     const long t = gtid();
     if (t >= (long)e[0] * e[1] * e[2]) return;
     int i, j, k;
-    unflatten(t, e, lo, i, j, k);
+    unflatten_dv(t, e, g.dv[0], lo, i, j, k);
     const double r = q[gidx(g, i, j, k)];
     if (!isfinite(r)) atomicAdd(cnt, 1u);
 }

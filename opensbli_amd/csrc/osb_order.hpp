@@ -30,11 +30,12 @@ struct CfOrder {
 
 // n / d = (n + umulhi(n, mp)) >> l for every 32-bit n, with l = ceil(log2 d)
 // and mp = ceil(2^(32+l) / d) - 2^32 (Granlund and Montgomery, "Division by
-// invariant integers using multiplication", PLDI 1994, sec. 4).  1 <= d <= 2^31.
+// invariant integers using multiplication", PLDI 1994, sec. 4).  d >= 1; for
+// d > 2^31 (l = 32, never a power of two) ceil(2^64 / d) = (2^64 - 1) / d + 1.
 __host__ inline void cf_divisor(unsigned d, unsigned& mp, unsigned& l) {
     l = 0;
     while ((1ull << l) < d) ++l;
-    const unsigned long long m = ((1ull << (32 + l)) + d - 1) / d;
+    const unsigned long long m = l < 32 ? ((1ull << (32 + l)) + d - 1) / d : ~0ull / d + 1;
     mp = (unsigned)(m - (1ull << 32));
 }
 __host__ __device__ __forceinline__ unsigned cf_div(unsigned n, unsigned mp, unsigned l) {

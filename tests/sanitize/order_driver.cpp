@@ -3,7 +3,7 @@
 // tests/test_sanitize.py:
 //  - the multiplicative-inverse quotient cf_div against integer division, for
 //    every divisor below 70000 on sampled and extreme numerators, and for
-//    divisors up to 2^31 on random ones;
+//    larger divisors up to 2^32 - 1 on random and extreme ones;
 //  - for every tile grid up to 24 x 24 x 24 and the shapes of the 128^3-1024^3
 //    grids, in each mode: every workgroup gets a tile in range, each tile
 //    exactly once, the same tile as the plain division form of the order, and
@@ -47,7 +47,7 @@ void check_div() {
             if (osb::cf_div(n, mp, l) != n / d) fail("cf_div", d, n, 0, 0);
         }
     }
-    const unsigned big[] = {65537u, 999983u, 1u << 20, (1u << 20) + 1, 0x7fffffffu, 0x80000000u};
+    const unsigned big[] = {65537u, 999983u, 1u << 20, (1u << 20) + 1, 0x7fffffffu, 0x80000000u, 0x80000001u, 0xc0000000u, 0xfffffffeu, 0xffffffffu};
     for (unsigned d : big) {
         unsigned mp, l;
         osb::cf_divisor(d, mp, l);
@@ -55,6 +55,9 @@ void check_div() {
             const unsigned n = (unsigned)rng();
             if (osb::cf_div(n, mp, l) != n / d) fail("cf_div big", d, n, 0, 0);
         }
+        const unsigned edge[] = {0u, d - 1, d, 0xfffffffeu, 0xffffffffu};
+        for (unsigned n : edge)
+            if (osb::cf_div(n, mp, l) != n / d) fail("cf_div big edge", d, n, 0, 0);
     }
     std::printf("ok cf_div\n");
 }
