@@ -30,8 +30,9 @@ struct Geom {
     // e0 e1) that split flat indices without integer division: dv[0] the
     // interior points (n0, n0 n1), dv[1 + d] the faces along d (n_d + 1 along
     // d), dv[4 + w] the fused face-residual tiles at width RESID_WS[w]
-    // (columns, columns x segments).  Set by geom_divisors.
-    unsigned dv[6][4];
+    // (columns, columns x segments), dv[6 + d] the boundary layers of
+    // direction d (bc_work).  Set by geom_divisors.
+    unsigned dv[9][4];
 };
 
 __host__ __device__ inline long gidx(const Geom& g, int i, int j, int k) {

@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(BLOCK) k_filter_face(const double* __restrict_
     const long t = xcd_gtid();
     if (t >= (long)e[0] * e[1] * e[2]) return;
     int i, j, k;
-    unflatten(t, e, lo, i, j, k);
+    unflatten_dv(t, e, g.dv[1 + DIR], lo, i, j, k);
     const long x0 = gidx(g, i, j, k);
     double out[NV];
     bool active = true;
@@ -340,6 +340,12 @@ __host__ inline void geom_divisors(Geom& g) {
     for (int w = 0; w < RESID_NW; ++w) {
         const int fw = g.ndim == 1 ? 1 : RESID_WS[w], pl = BLOCK / fw - 1;
         set(g.dv[4 + w], (g.n[0] + fw - 1) / fw, (g.n[D] + pl - 1) / pl);
+    }
+    for (int d = 0; d < 3; ++d) {               // bc_item's extents
+        long e[3];
+        for (int a = 0; a < 3; ++a) e[a] = a < g.ndim ? g.n[a] + 2 * g.halo : 1;
+        e[d] = 2 * (g.halo + 1);
+        set(g.dv[6 + d], e[0], e[1]);
     }
 }
 
@@ -806,10 +812,10 @@ __device__ __forceinline__ void bc_item(long t, double* __restrict__ q, const Ge
     lo[DIR] = 0;
     if (t >= (long)e[0] * e[1] * e[2]) return;
     int ii[3];
-    unflatten(t, e, lo, ii[0], ii[1], ii[2]);
-    const int layer = ii[DIR];
-    const int side = layer / (h + 1);
-    const int tt = layer % (h + 1);
+    unflatten_dv(t, e, g.dv[6 + DIR], lo, ii[0], ii[1], ii[2]);
+    const int layer = ii[DIR];                      // in [0, 2h + 2)
+    const int side = layer > h ? 1 : 0;
+    const int tt = layer - side * (h + 1);
     const int type = b.type[side];
     const int n = g.n[DIR];
     const int bnd = side == 0 ? 0 : n - 1;          // boundary point
