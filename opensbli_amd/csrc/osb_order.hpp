@@ -1,6 +1,8 @@
-// osb_order.hpp -- tile order of the 3-D stage kernels: the tile that
-// workgroup w of a one-dimensional launch evaluates.  Integer code only, shared
-// by the kernels and by the host code that launches them.
+// osb_order.hpp -- division by invariant integers, and the tile order of the
+// 3-D stage kernels (the tile that workgroup w of a one-dimensional launch
+// evaluates).  Integer code only, shared by the kernels and by the host code
+// that launches them; the kernels also split their flat thread indices with
+// these inverses (Geom::dv).
 //
 // Workgroups are dealt round-robin to the 8 XCDs, so workgroup w runs on XCD
 // w % 8.
@@ -10,9 +12,9 @@
 //     points -- stay in the same L2.
 //   mode 2 (tiles along x, y, z multiples of 8, 4, 2): the same ranges over
 //     supertiles of 8 x 4 x 2 tiles (about one XCD's resident workgroups)
-//     walked z first.  Each XCD's range is then a slab of whole supertile
-//     columns, and consecutive supertiles share their z-halo planes while they
-//     are in L2.
+//     walked z first.  With a supertile count divisible by 8, each XCD's range
+//     is a slab of whole supertile columns, and consecutive supertiles share
+//     their z-halo planes while they are in L2.
 // The index n (a tile in modes 0 and 1, a supertile in mode 2) splits into
 // digits of radix c0, then c1.  Its two quotients n / c0 and n / (c0 c1) come
 // from the host's multiplicative inverses, so a workgroup's prologue is a few
