@@ -451,6 +451,13 @@ int autotune(osb_batch* b) {
     B_TRY(b, hipEventCreate(&e1));
     for (auto& gr : b->groups) {
         if (gr.kind != GK_CENTRAL) continue;
+        // OSB_FUSED_TILE forces the tile shape of every context, so of the group too
+        const osb_ctx* c0 = b->ctx[gr.members[0]];
+        if (c0->tile_forced) {
+            int r = set_central_tile(b, gr, c0->fused_tile);
+            if (r) return r;
+            continue;
+        }
         auto run = [&]() {
             if (gr.conv == CF_SKEW) launch_central_v<CF_SKEW>(b->stream, gr, gr.parity, 0);
             else if (gr.conv == CF_KGP) launch_central_v<CF_KGP>(b->stream, gr, gr.parity, 0);
@@ -481,6 +488,12 @@ int autotune(osb_batch* b) {
     }
     for (auto& gr : b->groups) {
         if (gr.kind != GK_SHOCK || !gr.fused || gr.ndim < 2) continue;
+        const osb_ctx* c0 = b->ctx[gr.members[0]];
+        if (c0->resid_forced) {                                       // OSB_RESID_W, as for the contexts
+            int r = set_resid_w(b, gr, c0->resid_w);
+            if (r) return r;
+            continue;
+        }
         auto run = [&]() {
             if (gr.ndim == 2) {
                 if (gr.time == OSB_TIME_RK3) launch_face_resid_p<2, 2>(b->stream, gr, gr.parity, 0);

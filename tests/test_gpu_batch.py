@@ -20,9 +20,11 @@ def _announce(name):
     print(f"[gpu-case] {name}", flush=True)
 
 
-def _tgv_box(n, steps=5):
+def _tgv_box(n, steps=5, dt_n=16):
+    """TGV fields on an n[0] x n[1] x n[2] box, with the time step of the
+    dt_n^3 cube"""
     import math
-    pr = P.tgv(n=16, steps=steps)
+    pr = P.tgv(n=dt_n, steps=steps)
     delta = [2 * math.pi / m for m in n]
     q = P._tgv_fields(tuple(n), tuple(delta), pr.desc.gama, pr.desc.minf)
     d = nat.ProblemDesc(**{**pr.desc.__dict__, "n": list(n), "delta": delta})
@@ -130,6 +132,33 @@ def test_batch_bitwise_equals_single(hip, graphs, monkeypatch):
         assert np.array_equal(q, want), p.name
         assert st.iter == wst.iter == 5
         assert st.time == wst.time, (p.name, st.time, wst.time)
+
+
+@pytest.mark.parametrize("tile", [0, 1, 2])
+def test_batch_central_tile_orders_bitwise(hip, tile, monkeypatch):
+    """Batched 3-D central stages at each tile shape (forced by OSB_FUSED_TILE),
+    with instances whose tile counts take the supertile walk (128 x 32 x 16 at
+    every shape, 128 x 16 x 8 at 16 x 4 x 4) next to ones that walk tiles
+    linearly: each instance bitwise equal to itself stepped alone."""
+    monkeypatch.setenv("OSB_FUSED_TILE", str(tile))
+    probs = [_tgv_box([128, 32, 16], steps=3, dt_n=128), _tgv_box([128, 16, 8], steps=3, dt_n=128),
+             P.tgv(n=16, steps=3), _tgv_box([48, 20, 12], steps=3, dt_n=48)]
+    _announce(f"central batch, tile {tile}")
+    with hip.batch([p.desc for p in probs]) as b:
+        for i, p in enumerate(probs):
+            b.set_state(i, p.q0)
+        b.step(3)
+        b.sync()
+        info = b.describe()
+        got = [b.get_state(i) for i in range(len(probs))]
+    assert all(np.all(np.isfinite(q)) for q in got)
+    central = [g for g in info["groups"] if g["kind"] == "central"]
+    assert len(central) == 1 and central[0]["members"] == len(probs), info
+    assert central[0]["tuning"]["tile"] == [[16, 4, 4], [16, 8, 4], [8, 8, 8]][tile], info
+    for p, q in zip(probs, got):
+        _announce(f"  compare {p.name}")
+        want, _ = run(hip, p, steps=3)
+        assert np.array_equal(q, want), p.name
 
 
 def test_batch_fp64_matches_oracle(hip, oracle):
