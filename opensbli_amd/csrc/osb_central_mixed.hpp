@@ -92,39 +92,47 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
     cf_section();
 
     // ---- phase 1: constituent relations (FP64) -> FP32 deviations in LDS ----
-    for (int b = threadIdx.x; b < CF_BOX; b += CF_THREADS) {
+    // Every box point is evaluated and stored (the corners, and the scalar
+    // fields on the edges, are never read), with the loads clamped into the
+    // stored range as in k_central_fused, so the loop unrolls and all of a
+    // thread's box loads are in flight together.
+    constexpr int CF_ITERS = (CF_BOX + CF_THREADS - 1) / CF_THREADS;
+    const double* __restrict__ q1 = q + P;
+    const double* __restrict__ q2 = q + 2 * P;
+    const double* __restrict__ q3 = q + 3 * P;
+    const double* __restrict__ q4 = q + 4 * P;
+    const unsigned p0 = (unsigned)g.P[0], p01 = (unsigned)g.P[0] * (unsigned)g.P[1];
+    const int ib = i0 + g.o[0] - 2, jb = j0 + g.o[1] - 2, kb = k0 + g.o[2] - 2;
+    const int imax = n0 + 1 + g.o[0], jmax = n1 + 1 + g.o[1], kmax = n2 + 1 + g.o[2];
+#pragma unroll
+    for (int it = 0; it < CF_ITERS; ++it) {
+        const int b = threadIdx.x + it * CF_THREADS;
+        if (CF_BOX % CF_THREADS != 0 && b >= CF_BOX) break;
         const int bx = b % CF_BX, by = (b / CF_BX) % CF_BY, bz = b / (CF_BX * CF_BY);
-        const int i = i0 + bx - 2, j = j0 + by - 2, k = k0 + bz - 2;
-        const int out = (bx < 2 || bx >= CF_TX + 2) + (by < 2 || by >= CF_TY + 2) + (bz < 2 || bz >= CF_TZ + 2);
-        if (out == 3) continue;
-        if (i >= n0 + 2 || j >= n1 + 2 || k >= n2 + 2) continue;
-        const long x = gidx(g, i, j, k);
+        const unsigned x = (unsigned)min(ib + bx, imax) + p0 * (unsigned)min(jb + by, jmax) + p01 * (unsigned)min(kb + bz, kmax);
         const double r = q[x];
         const double inv_r = rcp_nr(r);
         double ke = 0.0;
-        double rv[3];
+        const double rv[3] = {q1[x], q2[x], q3[x]};
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            rv[d] = q[(long)(1 + d) * P + x];
             const double u = rv[d] * inv_r;
             s_u[d][b] = float(u - ref_u[d]);
             ke -= 0.5 * (u * u) * r;
         }
-        if (out <= 1) {
-            const double rhoE = q[4 * P + x];
-            const double p = c.gamma_m1 * (ke + rhoE);
-            if (VISC) {
-                const double TT = (c.minf * c.minf) * c.gama * p * inv_r;
-                s_T[VISC ? b : 0] = float(TT - ref_T);
-                if (SUTH) s_mu[SUTH ? b : 0] = float(1.4042 * TT * sqrt(TT) / (0.40417 + TT) - ref_mu);
-            }
-            if (HAS_CONV) {
-                s_p[HAS_CONV ? b : 0] = float(p - ref_p);
-                s_E[HAS_CONV ? b : 0] = float((KGP ? (rhoE + p) * inv_r : rhoE * inv_r) - ref_E);
+        const double rhoE = q4[x];
+        const double p = c.gamma_m1 * (ke + rhoE);
+        if (VISC) {
+            const double TT = (c.minf * c.minf) * c.gama * p * inv_r;
+            s_T[VISC ? b : 0] = float(TT - ref_T);
+            if (SUTH) s_mu[SUTH ? b : 0] = float(1.4042 * TT * sqrt(TT) / (0.40417 + TT) - ref_mu);
+        }
+        if (HAS_CONV) {
+            s_p[HAS_CONV ? b : 0] = float(p - ref_p);
+            s_E[HAS_CONV ? b : 0] = float((KGP ? (rhoE + p) * inv_r : rhoE * inv_r) - ref_E);
 #pragma unroll
-                for (int d = 0; d < 3; ++d) s_r[HAS_CONV ? d : 0][HAS_CONV ? b : 0] = float(rv[d] - ref_r[d]);
-                s_r[HAS_CONV ? 3 : 0][HAS_CONV ? b : 0] = float((KGP ? r : rhoE) - ref_r[3]);
-            }
+            for (int d = 0; d < 3; ++d) s_r[HAS_CONV ? d : 0][HAS_CONV ? b : 0] = float(rv[d] - ref_r[d]);
+            s_r[HAS_CONV ? 3 : 0][HAS_CONV ? b : 0] = float((KGP ? r : rhoE) - ref_r[3]);
         }
     }
     __syncthreads();
