@@ -55,28 +55,8 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
     const long P = g.Ptot;
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
 
-    // reference state: the tile's first point (in the interior, identical in every thread)
-    double ref_u[3], ref_p, ref_T, ref_E, ref_mu, ref_r[4];
-    {
-        const long x = gidx(g, i0, j0, k0);
-        const double r = q[x];
-        double ke = 0.0;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            ref_r[d] = q[(long)(1 + d) * P + x];
-            ref_u[d] = ref_r[d] / r;
-            ke -= 0.5 * (ref_u[d] * ref_u[d]) * r;
-        }
-        const double rE = q[4 * P + x];
-        ref_p = c.gamma_m1 * (ke + rE);
-        ref_T = (c.minf * c.minf) * c.gama * ref_p / r;
-        ref_E = KGP ? (rE + ref_p) / r : rE / r;
-        ref_r[3] = KGP ? r : rE;
-        ref_mu = SUTH ? 1.4042 * ref_T * sqrt(ref_T) / (0.40417 + ref_T) : 1.0;
-    }
-
-    // RK operands of the thread's point, loaded before phase 1 so that they
-    // arrive while phases 1 and 2 run
+    // RK operands of the thread's point, loaded first so that they arrive
+    // while phases 1 and 2 run
     const int tx = threadIdx.x % CF_TX, ty = (threadIdx.x / CF_TX) % CF_TY, tz = threadIdx.x / (CF_TX * CF_TY);
     const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
     const bool active = i < n0 && j < n1 && k < n2;
@@ -92,35 +72,66 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
     cf_section();
 
     // ---- phase 1: constituent relations (FP64) -> FP32 deviations in LDS ----
-    // Every box point is evaluated and stored (the corners, and the scalar
-    // fields on the edges, are never read), with the loads clamped into the
-    // stored range as in k_central_fused, so the loop unrolls and all of a
-    // thread's box loads are in flight together.
+    // The loads come first: the reference point's, then every box point's
+    // (all in flight together), then the reference state and the box points'
+    // relations.  Every box point is evaluated and stored (the corners, and
+    // the scalar fields on the edges, are never read), with the loads clamped
+    // into the stored range as in k_central_fused.
     constexpr int CF_ITERS = (CF_BOX + CF_THREADS - 1) / CF_THREADS;
-    const double* __restrict__ q1 = q + P;
-    const double* __restrict__ q2 = q + 2 * P;
-    const double* __restrict__ q3 = q + 3 * P;
-    const double* __restrict__ q4 = q + 4 * P;
+    const double* __restrict__ qvar[NV] = {q, q + P, q + 2 * P, q + 3 * P, q + 4 * P};
     const unsigned p0 = (unsigned)g.P[0], p01 = (unsigned)g.P[0] * (unsigned)g.P[1];
     const int ib = i0 + g.o[0] - 2, jb = j0 + g.o[1] - 2, kb = k0 + g.o[2] - 2;
     const int imax = n0 + 1 + g.o[0], jmax = n1 + 1 + g.o[1], kmax = n2 + 1 + g.o[2];
+    // reference state: the tile's first point (in the interior, identical in every thread)
+    double refq[NV];
+    {
+        const unsigned xr = (unsigned)(ib + 2) + p0 * (unsigned)(jb + 2) + p01 * (unsigned)(kb + 2);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) refq[v] = qvar[v][xr];
+    }
+    double bq[CF_ITERS][NV];
+#pragma unroll
+    for (int it = 0; it < CF_ITERS; ++it) {
+        const int b = threadIdx.x + it * CF_THREADS;
+        if (CF_BOX % CF_THREADS == 0 || b < CF_BOX) {
+            const int bx = b % CF_BX, by = (b / CF_BX) % CF_BY, bz = b / (CF_BX * CF_BY);
+            const unsigned xb = (unsigned)min(ib + bx, imax) + p0 * (unsigned)min(jb + by, jmax) + p01 * (unsigned)min(kb + bz, kmax);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) bq[it][v] = qvar[v][xb];
+        }
+    }
+    double ref_u[3], ref_p, ref_T, ref_E, ref_mu, ref_r[4];
+    {
+        const double r = refq[0];
+        double ke = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            ref_r[d] = refq[1 + d];
+            ref_u[d] = ref_r[d] / r;
+            ke -= 0.5 * (ref_u[d] * ref_u[d]) * r;
+        }
+        const double rE = refq[4];
+        ref_p = c.gamma_m1 * (ke + rE);
+        ref_T = (c.minf * c.minf) * c.gama * ref_p / r;
+        ref_E = KGP ? (rE + ref_p) / r : rE / r;
+        ref_r[3] = KGP ? r : rE;
+        ref_mu = SUTH ? 1.4042 * ref_T * sqrt(ref_T) / (0.40417 + ref_T) : 1.0;
+    }
 #pragma unroll
     for (int it = 0; it < CF_ITERS; ++it) {
         const int b = threadIdx.x + it * CF_THREADS;
         if (CF_BOX % CF_THREADS != 0 && b >= CF_BOX) break;
-        const int bx = b % CF_BX, by = (b / CF_BX) % CF_BY, bz = b / (CF_BX * CF_BY);
-        const unsigned x = (unsigned)min(ib + bx, imax) + p0 * (unsigned)min(jb + by, jmax) + p01 * (unsigned)min(kb + bz, kmax);
-        const double r = q[x];
+        const double r = bq[it][0];
         const double inv_r = rcp_nr(r);
         double ke = 0.0;
-        const double rv[3] = {q1[x], q2[x], q3[x]};
+        const double rv[3] = {bq[it][1], bq[it][2], bq[it][3]};
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const double u = rv[d] * inv_r;
             s_u[d][b] = float(u - ref_u[d]);
             ke -= 0.5 * (u * u) * r;
         }
-        const double rhoE = q4[x];
+        const double rhoE = bq[it][4];
         const double p = c.gamma_m1 * (ke + rhoE);
         if (VISC) {
             const double TT = (c.minf * c.minf) * c.gama * p * inv_r;
