@@ -100,22 +100,26 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
             for (int v = 0; v < NV; ++v) bq[it][v] = qvar[v][xb];
         }
     }
+    // The deviations' algebra is exact for any reference values (a derivative
+    // of a' = a - a_ref is that of a, and the product expansion holds for any
+    // a_ref, b_ref), so the reference state only has to be close to the
+    // tile's values: it is formed with the refined hardware reciprocal.
     double ref_u[3], ref_p, ref_T, ref_E, ref_mu, ref_r[4];
     {
-        const double r = refq[0];
+        const double r = refq[0], inv_r = rcp_nr1(r);
         double ke = 0.0;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             ref_r[d] = refq[1 + d];
-            ref_u[d] = ref_r[d] / r;
+            ref_u[d] = ref_r[d] * inv_r;
             ke -= 0.5 * (ref_u[d] * ref_u[d]) * r;
         }
         const double rE = refq[4];
         ref_p = c.gamma_m1 * (ke + rE);
-        ref_T = (c.minf * c.minf) * c.gama * ref_p / r;
-        ref_E = KGP ? (rE + ref_p) / r : rE / r;
+        ref_T = (c.minf * c.minf) * c.gama * ref_p * inv_r;
+        ref_E = KGP ? (rE + ref_p) * inv_r : rE * inv_r;
         ref_r[3] = KGP ? r : rE;
-        ref_mu = SUTH ? 1.4042 * ref_T * sqrt(ref_T) / (0.40417 + ref_T) : 1.0;
+        ref_mu = SUTH ? 1.4042 * ref_T * sqrt(ref_T) * rcp_nr1(0.40417 + ref_T) : 1.0;
     }
 #pragma unroll
     for (int it = 0; it < CF_ITERS; ++it) {
