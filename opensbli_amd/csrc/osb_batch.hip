@@ -440,15 +440,39 @@ int set_resid_w(osb_batch* b, Group& gr, int w) {
 //   only read q and overwrite F);
 // * 3-D central groups: the tile shape of the fused stage kernel (CF_TILES);
 // * 2-D/3-D shock groups with the fused last direction: the width of the
-//   face-residual tiles (RESID_WS).
-// The trial launches of the last two write the members' next buffer, which
-// the first real stage overwrites, and their RK register, which is cleared
-// afterwards (every RK-LS scheme's first stage multiplies it by zero, so
-// only its finiteness matters; classic RK3 saves the state into it first).
+//   face-residual tiles (RESID_WS), timed over a whole stage (the face-flux
+//   launches at their chosen grids, then the face-residual launch), so that
+//   the residual kernel reads the fluxes as a real stage leaves them.
+// Each candidate is timed in TUNE_ROUNDS rounds that alternate between the
+// candidates, and keeps its best round: a single pass of 3 launches per
+// candidate drifted by more than the widths' difference from box to box.
+// The trial launches that update write the members' next buffer, which the
+// first real stage overwrites, and their RK register, which is cleared
+// afterwards (every RK-LS scheme's first stage multiplies it by zero, so only
+// its finiteness matters; classic RK3 saves the state into it first).
 int autotune(osb_batch* b) {
     hipEvent_t e0, e1;
     B_TRY(b, hipEventCreate(&e0));
     B_TRY(b, hipEventCreate(&e1));
+    // average microseconds of 3 launches of run() after one untimed launch
+    auto time_us = [&](auto&& run, float& us) -> int {
+        run();
+        B_TRY(b, hipEventRecord(e0, b->stream));
+        for (int k = 0; k < 3; ++k) run();
+        B_TRY(b, hipEventRecord(e1, b->stream));
+        B_TRY(b, hipEventSynchronize(e1));
+        float ms = 0.f;
+        B_TRY(b, hipEventElapsedTime(&ms, e0, e1));
+        us = ms * 1000.f / 3.f;
+        return OSB_OK;
+    };
+    auto clear_du = [&](const Group& gr) -> int {
+        for (int m : gr.members) {
+            const osb_ctx* c = b->ctx[m];
+            B_TRY(b, hipMemsetAsync(c->du, 0, sizeof(double) * c->nv * c->g.Ptot, b->stream));
+        }
+        return OSB_OK;
+    };
     for (auto& gr : b->groups) {
         if (gr.kind != GK_CENTRAL) continue;
         // OSB_FUSED_TILE forces the tile shape of every context, so of the group too
@@ -463,27 +487,55 @@ int autotune(osb_batch* b) {
             else if (gr.conv == CF_KGP) launch_central_v<CF_KGP>(b->stream, gr, gr.parity, 0);
             else launch_central_v<CF_FEIEREISEN>(b->stream, gr, gr.parity, 0);
         };
-        float best = 1e30f;
+        for (int t = 0; t < CF_NTILES; ++t) gr.tile_us[t] = 1e30f;
+        for (int round = 0; round < TUNE_ROUNDS; ++round)
+            for (int t = 0; t < CF_NTILES; ++t) {
+                int r = set_central_tile(b, gr, t);
+                if (r) return r;
+                float us = 0.f;
+                r = time_us(run, us);
+                if (r) return r;
+                gr.tile_us[t] = std::min(gr.tile_us[t], us);
+            }
         int best_t = 0;
-        for (int t = 0; t < CF_NTILES; ++t) {
-            int r = set_central_tile(b, gr, t);
-            if (r) return r;
-            run();
-            B_TRY(b, hipEventRecord(e0, b->stream));
-            for (int k = 0; k < 3; ++k) run();
-            B_TRY(b, hipEventRecord(e1, b->stream));
-            B_TRY(b, hipEventSynchronize(e1));
-            float ms = 0.f;
-            B_TRY(b, hipEventElapsedTime(&ms, e0, e1));
-            gr.tile_us[t] = ms * 1000.f / 3.f;
-            if (gr.tile_us[t] < best) { best = gr.tile_us[t]; best_t = t; }
-        }
+        for (int t = 1; t < CF_NTILES; ++t)
+            if (gr.tile_us[t] < gr.tile_us[best_t]) best_t = t;
         int r = set_central_tile(b, gr, best_t);
         if (r) return r;
-        gr.tuned_us[BF_RESID] = best;
-        for (int m : gr.members) {
-            const osb_ctx* c = b->ctx[m];
-            B_TRY(b, hipMemsetAsync(c->du, 0, sizeof(double) * c->nv * c->g.Ptot, b->stream));
+        gr.tuned_us[BF_RESID] = gr.tile_us[best_t];
+        r = clear_du(gr);
+        if (r) return r;
+    }
+    for (auto& gr : b->groups) {
+        if (gr.kind != GK_SHOCK) continue;                            // the fused stage kernels change du: not timed
+        for (int d = 0; d < gr.ndim - (gr.fused ? 1 : 0); ++d) {     // the fused family changes du: not timed
+            const int f = BF_FACE0 + d;
+            const int tot = gr.total(f);
+            const int cands[5] = {0, 8192, 4096, 2048, 1024};
+            float best_us[5];
+            for (int ci = 0; ci < 5; ++ci) best_us[ci] = 1e30f;
+            for (int round = 0; round < TUNE_ROUNDS; ++round)
+                for (int ci = 0; ci < 5; ++ci) {
+                    const int cap = cands[ci];
+                    if (cap > 0 && cap >= tot) continue;
+                    const unsigned grid = (unsigned)(cap > 0 ? cap : std::max(tot, 1));
+                    auto run = [&]() {
+                        switch (gr.ndim) {
+                            case 1: launch_face_dir<1>(b->stream, gr, d, gr.parity, grid); break;
+                            case 2: launch_face_dir<2>(b->stream, gr, d, gr.parity, grid); break;
+                            default: launch_face_dir<3>(b->stream, gr, d, gr.parity, grid); break;
+                        }
+                    };
+                    float us = 0.f;
+                    int r = time_us(run, us);
+                    if (r) return r;
+                    best_us[ci] = std::min(best_us[ci], us);
+                }
+            int best = 0;
+            for (int ci = 1; ci < 5; ++ci)
+                if (best_us[ci] < best_us[best]) best = ci;
+            gr.cap[f] = cands[best];
+            gr.tuned_us[f] = best_us[best];
         }
     }
     for (auto& gr : b->groups) {
@@ -494,7 +546,13 @@ int autotune(osb_batch* b) {
             if (r) return r;
             continue;
         }
+        // one stage's face fluxes (at their tuned grids) and its face-residual launch
         auto run = [&]() {
+            for (int d = 0; d < gr.ndim - 1; ++d) {
+                const unsigned grid = grid_of(gr, BF_FACE0 + d);
+                if (gr.ndim == 2) launch_face_dir<2>(b->stream, gr, d, gr.parity, grid);
+                else launch_face_dir<3>(b->stream, gr, d, gr.parity, grid);
+            }
             if (gr.ndim == 2) {
                 if (gr.time == OSB_TIME_RK3) launch_face_resid_p<2, 2>(b->stream, gr, gr.parity, 0);
                 else launch_face_resid_p<2, 1>(b->stream, gr, gr.parity, 0);
@@ -503,61 +561,24 @@ int autotune(osb_batch* b) {
                 else launch_face_resid_p<3, 1>(b->stream, gr, gr.parity, 0);
             }
         };
-        float best = 1e30f;
-        int best_w = FUSED_FW;
-        for (int t = 0; t < RESID_NW; ++t) {
-            int r = set_resid_w(b, gr, RESID_WS[t]);
-            if (r) return r;
-            run();
-            B_TRY(b, hipEventRecord(e0, b->stream));
-            for (int k = 0; k < 3; ++k) run();
-            B_TRY(b, hipEventRecord(e1, b->stream));
-            B_TRY(b, hipEventSynchronize(e1));
-            float ms = 0.f;
-            B_TRY(b, hipEventElapsedTime(&ms, e0, e1));
-            gr.resid_w_us[t] = ms * 1000.f / 3.f;
-            if (gr.resid_w_us[t] < best) { best = gr.resid_w_us[t]; best_w = RESID_WS[t]; }
-        }
-        int r = set_resid_w(b, gr, best_w);
-        if (r) return r;
-        gr.tuned_us[BF_RESID] = best;
-        for (int m : gr.members) {
-            const osb_ctx* c = b->ctx[m];
-            B_TRY(b, hipMemsetAsync(c->du, 0, sizeof(double) * c->nv * c->g.Ptot, b->stream));
-        }
-    }
-    for (auto& gr : b->groups) {
-        if (gr.kind != GK_SHOCK) continue;                            // the fused stage kernels change du: not timed
-        for (int d = 0; d < gr.ndim - (gr.fused ? 1 : 0); ++d) {     // the fused family changes du: not timed
-            const int f = BF_FACE0 + d;
-            const int tot = gr.total(f);
-            const int cands[5] = {0, 8192, 4096, 2048, 1024};
-            float best = 1e30f;
-            int best_cap = 0;
-            for (int ci = 0; ci < 5; ++ci) {
-                const int cap = cands[ci];
-                if (cap > 0 && cap >= tot) continue;
-                const unsigned grid = (unsigned)(cap > 0 ? cap : std::max(tot, 1));
-                auto run = [&]() {
-                    switch (gr.ndim) {
-                        case 1: launch_face_dir<1>(b->stream, gr, d, gr.parity, grid); break;
-                        case 2: launch_face_dir<2>(b->stream, gr, d, gr.parity, grid); break;
-                        default: launch_face_dir<3>(b->stream, gr, d, gr.parity, grid); break;
-                    }
-                };
-                run();
-                B_TRY(b, hipEventRecord(e0, b->stream));
-                for (int r = 0; r < 3; ++r) run();
-                B_TRY(b, hipEventRecord(e1, b->stream));
-                B_TRY(b, hipEventSynchronize(e1));
-                float ms = 0.f;
-                B_TRY(b, hipEventElapsedTime(&ms, e0, e1));
-                const float us = ms * 1000.f / 3.f;
-                if (us < best) { best = us; best_cap = cap; }
+        for (int t = 0; t < RESID_NW; ++t) gr.resid_w_us[t] = 1e30f;
+        for (int round = 0; round < TUNE_ROUNDS; ++round)
+            for (int t = 0; t < RESID_NW; ++t) {
+                int r = set_resid_w(b, gr, RESID_WS[t]);
+                if (r) return r;
+                float us = 0.f;
+                r = time_us(run, us);
+                if (r) return r;
+                gr.resid_w_us[t] = std::min(gr.resid_w_us[t], us);
             }
-            gr.cap[f] = best_cap;
-            gr.tuned_us[f] = best;
-        }
+        int best_t = 0;
+        for (int t = 1; t < RESID_NW; ++t)
+            if (gr.resid_w_us[t] < gr.resid_w_us[best_t]) best_t = t;
+        int r = set_resid_w(b, gr, RESID_WS[best_t]);
+        if (r) return r;
+        gr.tuned_us[BF_RESID] = gr.resid_w_us[best_t];
+        r = clear_du(gr);
+        if (r) return r;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -308,6 +308,9 @@ constexpr int FUSED_FW = 32;
 // widths offered to the launch-shape timing of contexts and batches (2-D/3-D)
 constexpr int RESID_WS[] = {32, 16};
 constexpr int RESID_NW = 2;
+// rounds in which the autotuners time each candidate (alternating between the
+// candidates, keeping each one's best round)
+constexpr int TUNE_ROUNDS = 3;
 
 template <int NDIM, int W = FUSED_FW>
 struct FusedTile {
