@@ -161,6 +161,31 @@ def test_batch_central_tile_orders_bitwise(hip, tile, monkeypatch):
         assert np.array_equal(q, want), p.name
 
 
+@pytest.mark.parametrize("width", [16, 32])
+def test_batch_face_resid_width_bitwise(hip, width, monkeypatch):
+    """Batched shock groups at each face-residual width (forced by
+    OSB_RESID_W), FP64 and mixed, on boxes that fill neither width's tiles:
+    each instance bitwise equal to itself stepped alone at that width."""
+    monkeypatch.setenv("OSB_RESID_W", str(width))
+    probs = [P.kelvin_helmholtz(n=40, steps=4), P.kelvin_helmholtz(n=72, steps=4), P.riemann_2d(n=36, steps=4),
+             P.kelvin_helmholtz(n=44, steps=4, precision=nat.PREC_MIXED),
+             P.kelvin_helmholtz(n=68, steps=4, precision=nat.PREC_MIXED)]
+    _announce(f"shock batch, width {width}")
+    with hip.batch([p.desc for p in probs]) as b:
+        for i, p in enumerate(probs):
+            b.set_state(i, p.q0)
+        b.step(4)
+        b.sync()
+        info = b.describe()
+        got = [b.get_state(i) for i in range(len(probs))]
+    fused = [g for g in info["groups"] if g["kind"] == "shock" and "resid_w" in g["tuning"]]
+    assert len(fused) >= 2 and all(g["tuning"]["resid_w"] == width for g in fused), info
+    for p, q in zip(probs, got):
+        _announce(f"  compare {p.name}")
+        want, _ = run(hip, p, steps=4)
+        assert np.array_equal(q, want), p.name
+
+
 def test_batch_fp64_matches_oracle(hip, oracle):
     probs = [P.kelvin_helmholtz(n=48, steps=20), P.riemann_2d(n=48, steps=20), P.sod(n=200, steps=20),
              P.riemann_2d(n=40, steps=20, boundaries="mixed"), P.sod(n=160, steps=20, boundaries="dirichlet")]
