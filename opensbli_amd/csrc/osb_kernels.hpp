@@ -27,16 +27,7 @@ __device__ __forceinline__ void unflatten(long t, const int (&e)[3], const int (
     j = (int)(r % e1) + lo[1];
     k = (int)(r / e1) + lo[2];
 }
-// the same split from the host's inverses dv of e0 and e0 e1 (Geom::dv), for
-// t below the extents' product (< 2^32 by the context's size check)
-__device__ __forceinline__ void unflatten_dv(long t, const int (&e)[3], const unsigned (&dv)[4], const int (&lo)[3], int& i,
-                                             int& j, int& k) {
-    const unsigned u = (unsigned)t;
-    const unsigned a = cf_div(u, dv[0], dv[1]), b = cf_div(u, dv[2], dv[3]);
-    i = (int)(u - a * (unsigned)e[0]) + lo[0];
-    j = (int)(a - b * (unsigned)e[1]) + lo[1];
-    k = (int)b + lo[2];
-}
+// (unflatten_dv, osb_order.hpp: the same split from the host's inverses)
 
 // ---------------------------------------------------------------------------
 // Interface fluxes along DIR for faces i+1/2, i in [-1, n) along DIR and the
@@ -333,23 +324,10 @@ __host__ __device__ inline long fused_tiles(const Geom& g) {
 
 // Geom::dv: the inverses the kernels split flat indices with
 __host__ inline void geom_divisors(Geom& g) {
-    auto set = [](unsigned(&d)[4], long e0, long e1) {
-        cf_divisor((unsigned)e0, d[0], d[1]);
-        cf_divisor((unsigned)(e0 * e1), d[2], d[3]);
-    };
-    set(g.dv[0], g.n[0], g.n[1]);
-    for (int d = 0; d < 3; ++d) set(g.dv[1 + d], g.n[0] + (d == 0 ? 1 : 0), g.n[1] + (d == 1 ? 1 : 0));
-    const int D = g.ndim - 1;
-    for (int w = 0; w < RESID_NW; ++w) {
-        const int fw = g.ndim == 1 ? 1 : RESID_WS[w], pl = BLOCK / fw - 1;
-        set(g.dv[4 + w], (g.n[0] + fw - 1) / fw, (g.n[D] + pl - 1) / pl);
-    }
-    for (int d = 0; d < 3; ++d) {               // bc_item's extents
-        long e[3];
-        for (int a = 0; a < 3; ++a) e[a] = a < g.ndim ? g.n[a] + 2 * g.halo : 1;
-        e[d] = 2 * (g.halo + 1);
-        set(g.dv[6 + d], e[0], e[1]);
-    }
+    static_assert(RESID_NW == 2, "index_divisors takes two widths");
+    const int n[3] = {g.n[0], g.n[1], g.n[2]};
+    const int ws[2] = {RESID_WS[0], RESID_WS[1]};
+    index_divisors(g.ndim, n, g.halo, ws, BLOCK, g.dv);
 }
 
 template <int NDIM, int SCH, typename T, int TIME, int W = FUSED_FW>

@@ -59,6 +59,45 @@ __host__ inline CfOrder cf_order(unsigned ntx, unsigned nty, unsigned ntz, int x
     return o;
 }
 
+// t split over extents e (x fastest) with lower bounds lo, from the inverses
+// dv of e[0] and e[0] e[1] (cf_divisor); t below the extents' product, < 2^32
+__host__ __device__ __forceinline__ void unflatten_dv(long t, const int (&e)[3], const unsigned (&dv)[4], const int (&lo)[3],
+                                                      int& i, int& j, int& k) {
+    const unsigned u = (unsigned)t;
+    const unsigned a = cf_div(u, dv[0], dv[1]), b = cf_div(u, dv[2], dv[3]);
+    i = (int)(u - a * (unsigned)e[0]) + lo[0];
+    j = (int)(a - b * (unsigned)e[1]) + lo[1];
+    k = (int)b + lo[2];
+}
+
+// The inverses the kernels split flat indices with (Geom::dv), for a block of
+// ndim dimensions, interior extents n and halo depth halo: dv[0] the interior
+// points (n0, n0 n1); dv[1 + d] the faces along d (n_d + 1 along d); dv[4 + w]
+// the fused face-residual tiles of width resid_ws[w] (block threads per tile:
+// columns of the width, segments of block / width - 1 points along the last
+// direction); dv[6 + d] the boundary layers of direction d (2 (halo + 1)
+// along d, n + 2 halo across in the active directions).
+__host__ inline void index_divisors(int ndim, const int (&n)[3], int halo, const int (&resid_ws)[2], int block,
+                                    unsigned (&dv)[9][4]) {
+    auto set = [](unsigned(&d)[4], long e0, long e1) {
+        cf_divisor((unsigned)e0, d[0], d[1]);
+        cf_divisor((unsigned)(e0 * e1), d[2], d[3]);
+    };
+    set(dv[0], n[0], n[1]);
+    for (int d = 0; d < 3; ++d) set(dv[1 + d], n[0] + (d == 0 ? 1 : 0), n[1] + (d == 1 ? 1 : 0));
+    const int D = ndim - 1;
+    for (int w = 0; w < 2; ++w) {
+        const int fw = ndim == 1 ? 1 : resid_ws[w], pl = block / fw - 1;
+        set(dv[4 + w], (n[0] + fw - 1) / fw, (n[D] + pl - 1) / pl);
+    }
+    for (int d = 0; d < 3; ++d) {
+        long e[3];
+        for (int a = 0; a < 3; ++a) e[a] = a < ndim ? n[a] + 2 * halo : 1;
+        e[d] = 2 * (halo + 1);
+        set(dv[6 + d], e[0], e[1]);
+    }
+}
+
 // tile coordinates (in tiles) of position p of the order before the XCD split:
 // linear in modes 0 and 1, supertile-major in mode 2 (the batched stage kernel
 // applies its own XCD split over all instances and walks each instance's tiles

@@ -14,6 +14,10 @@
 //    z-first order), and the order before the XCD split (the batched
 //    kernel's walk of one instance) visits every tile once, supertile by
 //    supertile in mode 2.
+//  - the kernels' flat index splits (unflatten_dv over index_divisors'
+//    inverses, as Geom::dv holds them) against integer division for random
+//    1-D/2-D/3-D blocks: interior points, faces along each direction, the
+//    fused face-residual tiles at both widths and the boundary layers.
 // Exits non-zero on the first failed check.
 #define __host__
 #define __device__
@@ -137,10 +141,80 @@ void check_walk(unsigned ntx, unsigned nty, unsigned ntz, int remap) {
     }
 }
 
+// the kernels' flat index splits (unflatten_dv over index_divisors' inverses)
+// against integer division, over every extent family the kernels split: the
+// interior, the faces along each direction, the fused face-residual tiles at
+// both widths and the boundary layers of each direction
+void split_ref(long t, const int (&e)[3], const int (&lo)[3], int& i, int& j, int& k) {
+    i = (int)(t % e[0]) + lo[0];
+    j = (int)((t / e[0]) % e[1]) + lo[1];
+    k = (int)(t / ((long)e[0] * e[1])) + lo[2];
+}
+
+void check_family(const int (&e)[3], const int (&lo)[3], const unsigned (&dv)[4], std::mt19937_64& rng, int ndim, int n0) {
+    const long tot = (long)e[0] * e[1] * e[2];
+    auto one = [&](long t) {
+        int i, j, k, ri, rj, rk;
+        osb::unflatten_dv(t, e, dv, lo, i, j, k);
+        split_ref(t, e, lo, ri, rj, rk);
+        if (i != ri || j != rj || k != rk) fail("unflatten_dv", (unsigned)ndim, (unsigned)n0, (unsigned)t, e[0]);
+    };
+    if (tot <= 200000) {
+        for (long t = 0; t < tot; ++t) one(t);
+    } else {
+        for (int s = 0; s < 200000; ++s) one((long)(rng() % (unsigned long long)tot));
+        one(tot - 1);
+    }
+}
+
+void check_index_splits() {
+    std::mt19937_64 rng(7);
+    const int ws[2] = {32, 16};
+    const int block = 256;
+    long cases = 0;
+    for (int ndim = 1; ndim <= 3; ++ndim)
+        for (int c = 0; c < 60; ++c) {
+            int n[3] = {1, 1, 1};
+            for (int d = 0; d < ndim; ++d) n[d] = 8 + (int)(rng() % (ndim == 1 ? 5000 : (ndim == 2 ? 700 : 120)));
+            if (c == 0)
+                for (int d = 0; d < ndim; ++d) n[d] = ndim == 3 ? 256 : 1024;
+            const int halo = (c & 1) ? 4 : 2;
+            unsigned dv[9][4];
+            osb::index_divisors(ndim, n, halo, ws, block, dv);
+            const int zero[3] = {0, 0, 0};
+            check_family(n, zero, dv[0], rng, ndim, n[0]);                       // interior
+            for (int d = 0; d < ndim; ++d) {                                    // faces along d
+                int e[3] = {n[0], n[1], n[2]}, lo[3] = {0, 0, 0};
+                e[d] += 1;
+                lo[d] = -1;
+                check_family(e, lo, dv[1 + d], rng, ndim, n[0]);
+            }
+            if (ndim >= 2)
+                for (int w = 0; w < 2; ++w) {                                   // face-residual tiles
+                    const int fw = ws[w], pl = block / fw - 1, D = ndim - 1;
+                    const int e[3] = {(n[0] + fw - 1) / fw, (n[D] + pl - 1) / pl, ndim == 3 ? n[1] : 1};
+                    check_family(e, zero, dv[4 + w], rng, ndim, n[0]);
+                }
+            for (int d = 0; d < ndim; ++d) {                                    // boundary layers of d
+                int e[3], lo[3];
+                for (int a = 0; a < 3; ++a) {
+                    e[a] = a < ndim ? n[a] + 2 * halo : 1;
+                    lo[a] = a < ndim ? -halo : 0;
+                }
+                e[d] = 2 * (halo + 1);
+                lo[d] = 0;
+                check_family(e, lo, dv[6 + d], rng, ndim, n[0]);
+            }
+            ++cases;
+        }
+    std::printf("ok index splits %ld blocks\n", cases);
+}
+
 }  // namespace
 
 int main() {
     check_div();
+    check_index_splits();
     long walks = 0;
     for (unsigned x = 1; x <= 24; ++x)
         for (unsigned y = 1; y <= 24; ++y)

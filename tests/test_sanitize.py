@@ -44,7 +44,8 @@ def test_io_library_under_sanitizers(tmp_path):
 
 def test_tile_order_under_sanitizers(tmp_path):
     """Fast quotients against integer division; every tile once, in range, the
-    same as the plain division form, with each mode's XCD locality."""
+    same as the plain division form, with each mode's XCD locality; the
+    kernels' flat index splits against integer division."""
     _build(tmp_path, "order_driver")
     out = _run([str(tmp_path / "order_driver")], tmp_path)
-    assert "ok cf_div" in out and "ok tile walks" in out
+    assert "ok cf_div" in out and "ok tile walks" in out and "ok index splits" in out
