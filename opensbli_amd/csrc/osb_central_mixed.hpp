@@ -153,6 +153,14 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
     __syncthreads();
 
     // ---- phase 2: FP32 right-hand side, FP64 RK update --------------------
+    // Fences between the phase-2 sections only where they keep the occupancy:
+    // with the convective terms the kernel's occupancy is set by its LDS (3
+    // workgroups per CU, or 2 with Sutherland's law) and the scheduler may use
+    // the VGPRs that leaves (105 -> 154; the Feiereisen TGV stage 108.7 ->
+    // 104 us); the viscous-only kernel (4 workgroups per CU) keeps them.
+    auto phase2_fence = [] {
+        if (!HAS_CONV) cf_section();
+    };
     if (!active) return;
     const int l = (tx + 2) + CF_BX * ((ty + 2) + CF_BY * (tz + 2));
     const int ls[3] = {1, CF_BX, CF_BX * CF_BY};
@@ -184,7 +192,7 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
     for (int m = 0; m < 3; ++m)
 #pragma unroll
         for (int jd = 0; jd < 3; ++jd) G[m][jd] = d1f([&](int s) { return s_u[m][l + s * ls[jd]]; }, invd[jd]);
-    cf_section();
+    phase2_fence();
 
     if (KGP) {
         // KGP, enthalpy splitting, coefficients 1/4 (split_forms.py:178-234) on deviations:
@@ -269,7 +277,7 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
             const float d_pp = d1f([&](int s) { return s_p[HAS_CONV ? l + s * lj : 0]; }, inv);
             const float d_pu = float(ref_p) * d_uj + urj * d_pp + d1f([&](int s) { return s_p[HAS_CONV ? l + s * lj : 0] * ud(s); }, inv);
             eterm += -0.5f * (d_Eu + rc[jd] * d_E) - d_pu;
-            cf_section();
+            phase2_fence();
         }
         const float Ec = float(ref_E) + s_E[HAS_CONV ? l : 0];
 #pragma unroll
@@ -293,7 +301,7 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
             lapT += d2f([&](int s) { return s_T[VISC ? l + s * lj : 0]; }, inv2d[jd]);
 #pragma unroll
             for (int m = 0; m < 3; ++m) D2u[m][jd] = d2f([&](int s) { return s_u[m][l + s * lj]; }, inv2d[jd]);
-            cf_section();
+            phase2_fence();
         }
         float div = 0.f;
 #pragma unroll
@@ -327,7 +335,7 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
 #pragma unroll
             for (int jd = 0; jd < 3; ++jd) e += muv * S[jd] * G[m][jd];
             Eacc += e;
-            cf_section();
+            phase2_fence();
         }
         float heat = muv * lapT;
         if (SUTH) {
