@@ -79,3 +79,12 @@ def assert_exact_unless_fused(a, b, desc, tol=1e-11):
         assert l2.max() <= tol and li.max() <= tol, (l2, li)
     else:
         assert np.array_equal(a, b), float(np.abs(a - b).max())
+
+
+def kept_fastest(choice, options, times_us, resolution=0.01):
+    """An autotuner's choice among `options` is one whose reported time (printed
+    to `resolution` microseconds) is the smallest, up to that rounding: two
+    candidates whose times round alike may be kept either way."""
+    assert choice in options, (choice, options)
+    t = times_us[options.index(choice)]
+    return t <= min(times_us) + resolution + 1e-9

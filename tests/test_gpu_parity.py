@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from opensbli_amd import native as nat, problems as P
-from helpers import assert_exact_unless_fused, fp64_bar, fused_3d, rel_errors, run
+from helpers import assert_exact_unless_fused, fp64_bar, fused_3d, kept_fastest, rel_errors, run
 
 pytestmark = pytest.mark.gpu
 
@@ -566,7 +566,7 @@ def test_face_resid_width_autotuned(hip):
     print("  ", info, flush=True)
     us = info["resid_w_us"]
     assert info["resid_w_choice"] == "autotuned" and len(us) == 2 and min(us) > 0
-    assert info["resid_w"] == [32, 16][int(np.argmin(us))]
+    assert kept_fastest(info["resid_w"], [32, 16], us)
     for w in ("16", "32"):
         os.environ["OSB_RESID_W"] = w
         try:
@@ -591,7 +591,7 @@ def test_fused_stage_tile_autotuned(hip):
     us = info["tile_us"]
     assert info["tile_choice"] == "autotuned" and len(us) == 3 and min(us) > 0
     tiles = [[16, 4, 4], [16, 8, 4], [8, 8, 8]]
-    assert info["tile"] == tiles[int(np.argmin(us))]
+    assert kept_fastest(info["tile"], tiles, us)
     for t in range(3):
         os.environ["OSB_FUSED_TILE"] = str(t)
         try:
