@@ -235,7 +235,7 @@ __global__ void __launch_bounds__(CF_TX * CF_TY * CF_TZ) k_central_mixed(const d
             const float d_Huj = Href * gjj + urj * d_H + d1f([&](int s) { return hd(s) * ud(s); }, inv);
             eterm += 0.25f * ((((((d_Hrhouj + Hc * d_rhouj) + rhoujc * d_H) + ujc * d_Hrho) + (rhoc * Hc) * gjj) + rhoc * d_Huj) +
                               (ujc * Hc) * d_rho);
-            cf_section();
+            phase2_fence();
         }
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
