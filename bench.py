@@ -363,6 +363,22 @@ def profile_value(name, key):
 
 
 FP64_VALU_PEAK_TFLOPS = 78.6
+FP32_VALU_PEAK_TFLOPS = 157.3      # published, with packed FP32 (off in this build)
+
+
+def mixed_valu(key, stage_s):
+    """FP32 and FP64 VALU rates of a mixed-precision stage from the rocprofv3
+    summary: FP32 against its published peak, FP64 against its own, and their
+    sum against the FP64 rate, which bounds it without packed FP32."""
+    f32 = profile_value("mixed_precision_valu_pmc.md", key + "_mixed_fp32_flops_per_stage")
+    f64 = profile_value("mixed_precision_valu_pmc.md", key + "_mixed_fp64_flops_per_stage")
+    if not f32 or not f64:
+        return None
+    t32, t64 = f32 / stage_s / 1e12, f64 / stage_s / 1e12
+    return {"fp32_flops_per_stage": f32, "fp64_flops_per_stage": f64,
+            "fp32_TFLOPs": t32, "fp32_frac_of_157.3": t32 / FP32_VALU_PEAK_TFLOPS,
+            "fp64_TFLOPs": t64, "fp64_frac_of_78.6": t64 / FP64_VALU_PEAK_TFLOPS,
+            "sum_frac_of_78.6": (t32 + t64) / FP64_VALU_PEAK_TFLOPS}
 
 
 def teno256_line(steps=10, warmup=2):
@@ -389,6 +405,10 @@ def teno256_line(steps=10, warmup=2):
                 rec["valu_fp64"] = {"flops_per_stage": flops, "achieved_TFLOPs": flops / stage_s / 1e12,
                                     "peak_TFLOPs": FP64_VALU_PEAK_TFLOPS,
                                     "frac": flops / stage_s / 1e12 / FP64_VALU_PEAK_TFLOPS}
+        else:
+            valu = mixed_valu("teno256", stage_s)
+            if valu:
+                rec["valu"] = valu
         out[prec] = rec
         states[prec] = st
     st_err, pv = _rel_l2(states["mixed"], states["fp64"])
@@ -400,7 +420,7 @@ def mixed_headline(problem_name, steps, warmup):
     """The headline workload in mixed precision, with its error against FP64."""
     from opensbli_amd import native as nat
     pm = make_problem(problem_name, "mixed")
-    dt_m, _, qm, _, _ = time_gpu(pm, steps, warmup)
+    dt_m, _, qm, t_ev, _ = time_gpu(pm, steps, warmup)
     pf = make_problem(problem_name, "fp64")
     hip = nat.load_hip()
     with hip.context(pf.desc) as c:
@@ -408,7 +428,11 @@ def mixed_headline(problem_name, steps, warmup):
         c.step(warmup + steps)
         qf = c.get_state()
     st, pv = _rel_l2(qm, qf)
-    return {"value": pm.desc.npoints * 3.0 * steps / dt_m / 1e9, "rel_l2_vs_fp64": {"state": st, "per_variable": pv}}
+    out = {"value": pm.desc.npoints * 3.0 * steps / dt_m / 1e9, "rel_l2_vs_fp64": {"state": st, "per_variable": pv}}
+    valu = mixed_valu({"tgv_central_128": "tgv128", "kh_1024": "kh1024"}.get(problem_name, ""), t_ev / (3.0 * steps))
+    if valu:
+        out["valu"] = valu
+    return out
 
 
 def main():
