@@ -97,3 +97,28 @@ def test_tgv_teno5_256_cfl_mixed(hip, teno256):
     print(f"   mixed vs FP64 oracle: state rel L2 {whole:.3e}; per variable rel L2 {np.array2string(l2, precision=2)}, "
           f"rel Linf max {li.max():.3e}", flush=True)
     assert l2.max() <= MIXED_BAR, l2
+
+
+
+@pytest.mark.parametrize("n,order", [(96, "XCD ranges"), (80, "XCD ranges"), (72, "linear"), (192, "XCD ranges")])
+def test_tgv_central_tile_orders_at_size(hip, oracle, n, order, monkeypatch):
+    """The stage kernels' tile orders other than the 128^3 supertile walk, on
+    TGV cubes whose 16 x 4 x 4 tile counts (forced by OSB_FUSED_TILE) do not
+    take supertiles: XCD ranges of linear tiles (96^3, 80^3, 192^3; 80^3 with
+    ragged x tiles) and the plain linear order (72^3: 1620 tiles, not a
+    multiple of 8).  After 4 steps, FP64 against the oracle at the FP64 bar and
+    mixed precision within its bar of the FP64 oracle."""
+    monkeypatch.setenv("OSB_FUSED_TILE", "0")
+    pr = P.tgv(n=n, steps=4)
+    _announce(f"tile order '{order}': oracle {pr.name} x {pr.steps} steps")
+    b, sb = run(oracle, pr)
+    _announce(f"tile order '{order}': hip {pr.name} FP64")
+    a, sa = run(hip, pr)
+    assert np.all(np.isfinite(a))
+    fp64_bar(a, b, pr.desc, oracle, pr)
+    assert sa.iter == sb.iter == 4
+    mx = _mixed(pr)
+    _announce(f"tile order '{order}': hip {mx.name}")
+    m, _ = run(hip, mx)
+    l2, _ = rel_errors(m, b)
+    assert l2.max() <= MIXED_BAR, l2
