@@ -24,7 +24,8 @@
 // by the RK update, so the stage reads q, du (and the face fluxes) and writes
 // du and q once.  It runs in sections separated by scheduling fences (one per
 // direction, one per momentum component of the viscous stress), which keeps
-// the kernel near 130 VGPRs.
+// the kernel within the 168 VGPRs of 3 waves/SIMD (160 for Feiereisen with
+// constant viscosity at 16 x 4 x 4).
 #pragma once
 
 #include "osb_kernels.hpp"
@@ -110,8 +111,9 @@ __host__ __device__ constexpr int cf_waves(int tx, int ty, int tz, int nfield) {
 // per momentum component's mixed derivatives).  Without it the scheduler
 // hoists the LDS reads of later sections ahead to hide their latency, which
 // takes the kernel to 248 VGPRs and 2 waves per SIMD; with it each section's
-// values are loaded, used and dead before the next (about 130 VGPRs with the
-// RK operands held from the start), and occupancy hides the latency instead.
+// values are loaded, used and dead before the next (160 VGPRs with phase 1's
+// loads issued together and the RK operands held from the start), and
+// occupancy hides the latency instead.
 __device__ __forceinline__ void cf_section() { __builtin_amdgcn_sched_barrier(0); }
 
 // index of the (a, b) pair in Consts::wm, a < b

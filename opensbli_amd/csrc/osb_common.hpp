@@ -8,6 +8,7 @@
 #include <type_traits>
 
 #include "../../include/osb.h"
+#include "osb_eigzero.hpp"
 #include "osb_order.hpp"
 
 namespace osb {
@@ -426,23 +427,6 @@ __device__ __forceinline__ void eigensystem(T gm1, T rho, const T* u, T a, T (&L
     }
 }
 
-// Entries of the left eigenvectors that are zero for every state: never set,
-// or a product with a zero direction selector k_d.  2-D: the shear row's last
-// entry and its entry 1 + DIR.  3-D: in rows r = 0..2, entry 4 unless DIR = r,
-// and entry 1 + j unless DIR = r or (r != j and DIR = 3 - r - j).
-template <int NDIM, int DIR>
-__device__ constexpr bool l_zero(int r, int m) {
-    if (NDIM == 2) return r == 1 && (m == 3 || m == 1 + DIR);
-    if (NDIM == 3 && r <= 2 && DIR != r) {
-        if (m == 4) return true;
-        if (m >= 1) {
-            const int j = m - 1;
-            return r == j || DIR != 3 - r - j;
-        }
-    }
-    return false;
-}
-
 // ---------------------------------------------------------------------------
 // Primitive variables and the physical flux along DIR of one point
 // (constituent relations + flux vector, shock_capturing.py:786-836).  FAST
@@ -603,7 +587,12 @@ __device__ __forceinline__ void face_core(PT&& pt, const Consts& c, double (&out
             T cs = T(0), cf = T(0);
 #pragma unroll
             for (int m = 0; m < NV; ++m) {
-                if (DEV && l_zero<NDIM, DIR>(k, m)) continue;     // mixed precision: structural zeros skipped
+                // structural zeros skipped (mixed precision, and FP64 in 3-D): their
+                // products are zeros (of either sign) for every finite state, so
+                // only the sign of a zero sum can differ.  The 2-D FP64 faces keep
+                // them: there the skip raised the register count past 3 waves/SIMD
+                // and slowed the fused face-residual kernel 7 % (KH 1024^2).
+                if ((DEV || NDIM == 3) && l_zero<NDIM, DIR>(k, m)) continue;
                 cs = fma(L[k][m], qs[s][m], cs);
                 cf = fma(L[k][m], F[s][m], cf);
             }
@@ -636,7 +625,8 @@ __device__ __forceinline__ void face_core(PT&& pt, const Consts& c, double (&out
             rec = recon_pair<SCH, T>(fp, fm, c);
         }
 #pragma unroll
-        for (int m = 0; m < NV; ++m) acc[m] = fma(R[m][k], rec, acc[m]);
+        for (int m = 0; m < NV; ++m)
+            if (NDIM != 3 || !r_zero<NDIM, DIR>(m, k)) acc[m] = fma(R[m][k], rec, acc[m]);
     }
     if (DEV) {
 #pragma unroll

@@ -1,7 +1,9 @@
 """Host code under AddressSanitizer and UndefinedBehaviorSanitizer: the CPU
 oracle and the HDF5 I/O library, each compiled with -fsanitize=address,undefined
 into a driver that runs it through its C ABI, and the stage kernels' tile-order
-arithmetic (osb_order.hpp) in a driver of its own (tests/sanitize/).  A sanitizer
+arithmetic (osb_order.hpp) and the eigenvector structural zeros
+(osb_eigzero.hpp, against the oracle's eigensystem) in drivers of their own
+(tests/sanitize/).  A sanitizer
 report, a failed run or a non-finite result fails the test."""
 import os
 import subprocess
@@ -49,3 +51,13 @@ def test_tile_order_under_sanitizers(tmp_path):
     _build(tmp_path, "order_driver")
     out = _run([str(tmp_path / "order_driver")], tmp_path)
     assert "ok cf_div" in out and "ok tile walks" in out and "ok index splits" in out
+
+
+def test_eigenvector_structural_zeros_under_sanitizers(tmp_path):
+    """The L and R entries that the face kernels skip are exactly the zero
+    entries of the oracle's eigensystem, in 1-D, 2-D and 3-D along every
+    direction, on random states."""
+    _build(tmp_path, "eigzero_driver")
+    out = _run([str(tmp_path / "eigzero_driver")], tmp_path)
+    assert "ok structural zeros" in out
+    assert "ndim 3 dir 2: 6 L and 6 R zero entries per state" in out, out
